@@ -1,0 +1,199 @@
+/*
+ * opalsc.h -- C ABI of the MI355X (gfx950) space-charge PIC step.
+ *
+ * A drop-in for the OPAL-T space-charge path: cloud-in-cell (CIC) charge
+ * scatter, Hockney open-boundary FFT Poisson solve with the integrated
+ * Green's function (IGF) on the doubled grid, CIC field gather and the Boris
+ * push of ParallelTTracker.  Reference paths below are relative to the OPAL
+ * 2.0 source tree.
+ *
+ * Conventions (shared by every entry point):
+ *  - Every function returns 0 on success or a negative OPALSC_E* code; the
+ *    message is available from opalsc_last_error(ctx).  No C++ exception
+ *    crosses this boundary.
+ *  - Particle vectors are AoS double[N][3] (x,y,z), scalars double[N], as
+ *    OPAL's ParticleAttrib<Vektor<double,3>> stores them.  Grids are
+ *    x-fastest double[n2][n1][n0] over the interior cells (no guard layer).
+ *  - Pointers are host pointers unless the name ends in _dev.  The caller
+ *    owns every array it passes; the context owns all device memory.
+ *  - One context per host thread.  Work is ordered on the context's HIP
+ *    stream; every host-pointer entry point returns after its results are in
+ *    host memory.
+ *  - All physics is in OPAL's units: R in m, P = beta*gamma, Q in C, dt in s,
+ *    species charge in units of e, species mass as rest energy in eV.
+ */
+#ifndef OPALSC_H
+#define OPALSC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPALSC_ABI_VERSION 1
+
+enum {
+    OPALSC_OK = 0,
+    OPALSC_EINVAL = -1,  /* bad argument */
+    OPALSC_EHIP = -2,    /* HIP runtime error */
+    OPALSC_ENOMEM = -3,  /* device allocation failed */
+    OPALSC_EMESH = -4,   /* "h<0, can not build a mesh" (PartBunchBase.hpp:870-872) */
+    OPALSC_ESTATE = -5,  /* call out of order (e.g. track before bunch) */
+    OPALSC_ECAP = -6     /* more particles than the context was created for */
+};
+
+enum { OPALSC_FP64 = 0, OPALSC_MIXED = 1 };
+enum { OPALSC_GREENS_INTEGRATED = 0, OPALSC_GREENS_STANDARD = 1 };
+enum { OPALSC_GRID_RHO = 0, OPALSC_GRID_PHI = 1, OPALSC_GRID_EFIELD = 2 };
+
+typedef struct opalsc_ctx opalsc_ctx;
+
+/* FIELDSOLVER attributes (src/Structure/FieldSolver.cpp:123-200) plus sizing. */
+typedef struct {
+    int n[3];              /* MX, MY, MT: cells per dimension (each >= 4) */
+    int greens;            /* GREENSF: OPALSC_GREENS_INTEGRATED or _STANDARD */
+    int precision;         /* OPALSC_FP64, or OPALSC_MIXED (FP32 FFT/convolution/gather) */
+    double bboxincr;       /* BBOXINCR in percent (OPAL default 2) */
+    int64_t max_particles; /* capacity of the device particle buffers */
+    int device;            /* HIP device ordinal */
+    int sort_interval;     /* tracker: re-sort the bunch by cell every k steps (0: never) */
+} opalsc_config;
+
+/* DataSink statistics (src/Structure/DataSink.cpp:360-391) as computed by
+ * PartBunchBase::calcBeamParameters (PartBunchBase.hpp:1529-1613). */
+typedef struct {
+    double rmean[3], pmean[3], rrms[3], prms[3], eps_norm[3], eps[3], rprms[3];
+    double gamma_mean, ekin_MeV;
+    double t, pathLength; /* tracker time and path length after the step */
+    int64_t n;
+    int64_t dropped; /* CIC deposits that fell outside the grid during the step */
+} opalsc_stats;
+
+/* Tracker set-up: ParallelTTracker state (src/Algorithms/ParallelTTracker.cpp:214-426)
+ * and the analytic stand-ins for element fields used by the benchmark
+ * configurations: B = (0,0,Bz), E = (0,0,Ez0*sin(2*pi*rf_freq*t + rf_phase)),
+ * both expressed in the reference frame. */
+typedef struct {
+    double dT;
+    double charge;   /* species charge / e */
+    double mass_eV;  /* species rest energy */
+    double coupling; /* 1/(4 pi eps0) (src/Track/TrackRun.cpp:542-543) */
+    double Bz, Ez0, rf_freq, rf_phase;
+    double t, pathLength;
+    double RefPartR[3], RefPartP[3];
+    int space_charge; /* 0 behaves as FSTYPE=NONE */
+} opalsc_track_params;
+
+/* Synthetic bunch generated on the device from a counter-based RNG:
+ * x,y,z ~ N(0, sigma_r), p - (0,0,p0) ~ N(0, sigma_p) (absolute beta*gamma),
+ * a fraction halo_frac replaced by points uniform in the ball of radius
+ * halo_radius*sigma (positions only), uniform macro-charge q_total/N. */
+typedef struct {
+    int64_t n;
+    uint64_t seed;
+    double sigma_r[3], sigma_p[3];
+    double p0;
+    double q_total;
+    double halo_frac, halo_radius;
+} opalsc_gauss_params;
+
+/* --- context ----------------------------------------------------------- */
+int opalsc_abi_version(void);
+int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg);
+int opalsc_destroy(opalsc_ctx *ctx);
+/* Last error message of ctx (or of the most recent failed create when ctx is NULL). */
+const char *opalsc_last_error(const opalsc_ctx *ctx);
+int opalsc_synchronize(opalsc_ctx *ctx);
+
+/* --- Poisson solver ------------------------------------------------------
+ * FFTPoissonSolver::computePotential(Field_t& rho, Vector_t hr),
+ * src/Solvers/FFTPoissonSolver.cpp:337-382: in place, rho in, phi out, exact
+ * circular convolution of the zero-padded rho with the (mirrored) Green's
+ * function on the doubled grid, without the h^3 and 1/(4 pi eps0) factors. */
+int opalsc_compute_potential(opalsc_ctx *ctx, double *rho_inout, const double hr[3]);
+
+/* --- mesh and cloud-in-cell ----------------------------------------------
+ * PartBunchBase::boundp (src/Classic/Algorithms/PartBunchBase.hpp:802-883):
+ * bounds of R, BBOXINCR padding, hr = (max-min)/(n-1), origin = min - hr/2. */
+int opalsc_boundp(opalsc_ctx *ctx, int64_t N, const double *R, double hr[3], double origin[3]);
+
+/* IntCICImpl<3> cell lookup (ippl/src/Particle/IntCIC.h:816-851;
+ * ippl/src/Meshes/UniformCartesian.hpp:1025-1039,1084-1097): per particle the
+ * lower CIC grid point lgp[3] and normalised offsets dpos[3]. */
+int opalsc_cic_index(opalsc_ctx *ctx, int64_t N, const double *R, const double origin[3],
+                     const double hr[3], int32_t *lgp, double *dpos);
+
+/* Stable sort of particles by the x-fastest linear index of their lgp cell:
+ * perm[i] = original index of the i-th particle in cell order.  No
+ * counterpart in the reference (its scatter is a serial loop); it is the
+ * order the device scatter and gather run in. */
+int opalsc_sort_by_cell(opalsc_ctx *ctx, int64_t N, const double *R, const double origin[3],
+                        const double hr[3], uint32_t *perm);
+
+/* --- self fields ---------------------------------------------------------
+ * PartBunch::computeSelfFields(), src/Classic/Algorithms/PartBunch.cpp:482-648:
+ * scatter Q*dt with CIC, rho /= dT, Lorentz-scaled mesh, Poisson solve,
+ * E = -Grad(phi) scaled by (gz, gz, 1/gz), CIC gather into Ef (overwritten),
+ * Bf += (-betaC*Ey, betaC*Ex, 0) (accumulated).  Q is updated in place to
+ * (Q*dt)/dt exactly as the reference leaves it.  origin/hr are the mesh
+ * set by boundp. */
+int opalsc_compute_self_fields(opalsc_ctx *ctx, int64_t N, const double *R, const double *P,
+                               double *Q, const double *dt, double dT, const double origin[3],
+                               const double hr[3], double coupling, double *Ef, double *Bf);
+
+/* Grids of the most recent compute_self_fields / track step: OPALSC_GRID_RHO
+ * (charge density entering the solver, n0*n1*n2), OPALSC_GRID_PHI (potential
+ * in V, n0*n1*n2), OPALSC_GRID_EFIELD (AoS [n0*n1*n2][3] after the Lorentz
+ * scaling).  Mixed precision grids are widened to double. */
+int opalsc_get_grid(opalsc_ctx *ctx, int which, double *out);
+
+/* --- pusher --------------------------------------------------------------
+ * ParallelTTracker::pushParticles (src/Algorithms/ParallelTTracker.h:448-455)
+ * = switchToUnitlessPositions + BorisPusher::push + switchOffUnitlessPositions. */
+int opalsc_push_half(opalsc_ctx *ctx, int64_t N, double *R, const double *P, const double *dt);
+
+/* ParallelTTracker::kickParticles (src/Algorithms/ParallelTTracker.h:442-446)
+ * = BorisPusher::kick (src/Steppers/BorisPusher.h:56-113) per particle. */
+int opalsc_kick(opalsc_ctx *ctx, int64_t N, double *P, const double *Ef, const double *Bf,
+                const double *dt, double charge, double mass_eV);
+
+/* PartBunchBase::calcBeamParameters (src/Classic/Algorithms/PartBunchBase.hpp:1529-1613). */
+int opalsc_beam_parameters(opalsc_ctx *ctx, int64_t N, const double *R, const double *P,
+                           double mass_eV, opalsc_stats *out);
+
+/* --- device-resident tracker --------------------------------------------
+ * The ParallelTTracker::execute step loop (ParallelTTracker.cpp:343-388) with
+ * the bunch kept in HBM: push, beam-frame rotation, boundp, self fields,
+ * external fields, kick, push, reference-frame update, statistics. */
+int opalsc_bunch_set(opalsc_ctx *ctx, int64_t N, const double *R, const double *P, const double *Q);
+int opalsc_bunch_generate(opalsc_ctx *ctx, const opalsc_gauss_params *g);
+/* Download R, P, Q in the order the bunch was set (any may be NULL). */
+int opalsc_bunch_get(opalsc_ctx *ctx, double *R, double *P, double *Q);
+int opalsc_track_init(opalsc_ctx *ctx, const opalsc_track_params *p);
+/* Run nsteps steps; per_step (may be NULL) receives nsteps statistics. */
+int opalsc_track(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);
+int opalsc_track_get_params(opalsc_ctx *ctx, opalsc_track_params *out);
+/* Mesh (hr, origin) of the most recent track step or boundp call. */
+int opalsc_get_mesh(opalsc_ctx *ctx, double hr[3], double origin[3]);
+
+/* --- correctly rounded FP64 atan/log used by the IGF --------------------
+ * The host evaluation runs the same double-double code as the device. */
+int opalsc_host_atan_cr(int64_t n, const double *x, double *y);
+int opalsc_host_log_cr(int64_t n, const double *x, double *y);
+int opalsc_device_atan_cr(opalsc_ctx *ctx, int64_t n, const double *x, double *y);
+int opalsc_device_log_cr(opalsc_ctx *ctx, int64_t n, const double *x, double *y);
+
+/* --- timing --------------------------------------------------------------
+ * Per-kernel HIP-event timing of the tracker, recorded on the context's
+ * stream when enabled: names[i] and total milliseconds ms[i] over the steps
+ * since the last reset, launches[i] the launch count.  Returns the number of
+ * kernel classes (<= cap). */
+int opalsc_profile_enable(opalsc_ctx *ctx, int enable);
+int opalsc_profile_read(opalsc_ctx *ctx, int cap, const char **names, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPALSC_H */

@@ -1,0 +1,966 @@
+/*
+ * opal_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A single-threaded FP64 CPU restatement of OPAL 2.0's space-charge PIC step
+ * (cloud-in-cell scatter, Hockney open-boundary FFT Poisson solve with the
+ * integrated Green's function, CIC gather, Boris kick/push, bunch moments,
+ * beam-frame rotation and reference-frame update).  It is the checker that the
+ * HIP library is compared against; nothing in the product links or calls it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+ *
+ * Every function follows the operation order of the reference function it
+ * cites (file:line relative to the OPAL source tree), because bit-exact
+ * agreement of cell indices and interpolation weights depends on that order.
+ * Build with -ffp-contract=off (see oracle/Makefile): no FMA contraction.
+ *
+ * Layouts (same as the reference): particle vectors are AoS double[N][3];
+ * fields are x-fastest, interior cells only (no guard layer); the vector
+ * field E is AoS double[n0*n1*n2][3].
+ *
+ * Reference-side differences that are deliberate (and documented at the
+ * function): the antiparallel branch of getQuaternion draws a random axis in
+ * the reference; here it is the fixed axis (1,0,0).  Particles whose cell
+ * lookup is non-finite or out of int range are dropped from the deposit and
+ * gather zero field (the reference would abort or index out of bounds).
+ */
+#include <complex.h>
+#include <math.h>
+#include <quadmath.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_C 299792458.0              /* src/Classic/Physics/Physics.h:52 */
+#define ORC_EPSILON_0 8.854187817e-12  /* src/Classic/Physics/Physics.h:58 */
+#define ORC_PI 3.14159265358979323846  /* src/Classic/Physics/Physics.h:31 */
+
+typedef double complex cplx;
+
+int orc_version(void) { return 1; }
+
+/* ------------------------------------------------------------------------ */
+/* Bounds and mesh                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* PartBunchBase<T,Dim>::getLocalBounds, src/Classic/Algorithms/PartBunchBase.hpp:1203-1218.
+ * Note the `else if`: a value smaller than rmin is never compared with rmax. */
+void orc_get_bounds(int64_t N, const double *R, double rmin[3], double rmax[3]) {
+    if (N == 0) {
+        for (int d = 0; d < 3; ++d) rmin[d] = rmax[d] = 0.0;
+        return;
+    }
+    for (int d = 0; d < 3; ++d) rmin[d] = rmax[d] = R[d];
+    for (int64_t i = 1; i < N; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            double v = R[3 * i + d];
+            if (rmin[d] > v) rmin[d] = v;
+            else if (rmax[d] < v) rmax[d] = v;
+        }
+    }
+}
+
+/* PartBunchBase<T,Dim>::boundp, src/Classic/Algorithms/PartBunchBase.hpp:802-883
+ * (non-DC beam, not emitting).  dh = BBOXINCR/100 (src/Structure/FieldSolver.cpp:487).
+ * Returns 0, or -1 when h0*h1*h2 <= 0 ("h<0, can not build a mesh"). */
+int orc_boundp(int64_t N, const double *R, const int n[3], double dh,
+               double hr[3], double origin[3]) {
+    double rmin[3], rmax[3];
+    orc_get_bounds(N, R, rmin, rmax);
+    for (int i = 0; i < 3; ++i) {
+        double length = fabs(rmax[i] - rmin[i]);
+        if (length < 1e-10) {
+            rmax[i] += 1e-10;
+            rmin[i] -= 1e-10;
+        } else {
+            rmax[i] += dh * length;
+            rmin[i] -= dh * length;
+        }
+        hr[i] = (rmax[i] - rmin[i]) / (n[i] - 1);
+    }
+    if (hr[0] * hr[1] * hr[2] <= 0) return -1;
+    for (int i = 0; i < 3; ++i) origin[i] = rmin[i] - hr[i] / 2.0;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cloud-in-cell                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* One dimension of IntCICImpl<3>::scatter/gather's cell lookup:
+ *   FindNGP -> UniformCartesian::getVertexBelow (ippl/src/Meshes/UniformCartesian.hpp:1025-1039)
+ *   FindPos -> getCellPosition (UniformCartesian.hpp:1084-1097)
+ *   FindDelta -> getDeltaCell (UniformCartesian.hpp:1151-1158) = h*1
+ *   lgp offset and dpos (ippl/src/Particle/IntCIC.h:837-851).
+ * Returns 0 when the lookup is out of int range or non-finite. */
+static int cic_dim(double x, double o, double h, int *lgp, double *dpos) {
+    double t = (x - o) / h;
+    if (!(t > -1073741824.0 && t < 1073741824.0)) return 0;
+    int i = (int)t;
+    int ngp = (x >= o) ? i : i - 1;
+    double g = ((double)ngp + 0.5) * h + o;
+    double delta = h * 1;
+    int off = 0;
+    if (g > x) {
+        off = -1;
+        g = g - delta;
+    }
+    *lgp = ngp + off;
+    *dpos = (x - g) / delta;
+    return 1;
+}
+
+/* Per-particle lower grid point (lgp, the cell the 8-point stencil starts at)
+ * and normalised offset dpos; valid[i] = 0 for dropped particles. */
+void orc_cic_index(int64_t N, const double *R, const double origin[3], const double hr[3],
+                   int32_t *lgp, double *dpos, uint8_t *valid) {
+    for (int64_t p = 0; p < N; ++p) {
+        int ok = 1;
+        for (int d = 0; d < 3; ++d) {
+            int l = 0;
+            double dp = 0.0;
+            ok &= cic_dim(R[3 * p + d], origin[d], hr[d], &l, &dp);
+            lgp[3 * p + d] = l;
+            dpos[3 * p + d] = dp;
+        }
+        if (valid) valid[p] = (uint8_t)ok;
+    }
+}
+
+static inline int in_grid(int i, int j, int k, const int n[3]) {
+    return i >= 0 && j >= 0 && k >= 0 && i < n[0] && j < n[1] && k < n[2];
+}
+
+/* IntCICImpl<3>::scatter, ippl/src/Particle/IntCIC.h:816-860, applied
+ * sequentially over particles as ParticleAttrib::scatter does
+ * (ippl/src/Particle/ParticleAttrib.h:201-225).  rho is accumulated, not
+ * cleared.  Deposits that land outside the interior are dropped: on one rank
+ * accumGuardCells has no neighbour to add guard cells into
+ * (ippl/src/Field/BareField.hpp:764+).  Returns the number of dropped
+ * (particle, corner) deposits. */
+int64_t orc_scatter(int64_t N, const double *R, const double *pdata, const double origin[3],
+                    const double hr[3], const int n[3], double *rho) {
+    int64_t dropped = 0;
+    for (int64_t p = 0; p < N; ++p) {
+        int l[3];
+        double dp[3];
+        int ok = 1;
+        for (int d = 0; d < 3; ++d) ok &= cic_dim(R[3 * p + d], origin[d], hr[d], &l[d], &dp[d]);
+        if (!ok) {
+            dropped += 8;
+            continue;
+        }
+        const double q = pdata[p];
+        const double a0 = 1 - dp[0], a1 = 1 - dp[1], a2 = 1 - dp[2];
+        const double b0 = dp[0], b1 = dp[1], b2 = dp[2];
+        double w[8];
+        w[0] = a0 * a1 * a2 * q;
+        w[1] = b0 * a1 * a2 * q;
+        w[2] = a0 * b1 * a2 * q;
+        w[3] = b0 * b1 * a2 * q;
+        w[4] = a0 * a1 * b2 * q;
+        w[5] = b0 * a1 * b2 * q;
+        w[6] = a0 * b1 * b2 * q;
+        w[7] = b0 * b1 * b2 * q;
+        for (int c = 0; c < 8; ++c) {
+            int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
+            if (in_grid(i, j, k, n)) rho[((int64_t)k * n[1] + j) * n[0] + i] += w[c];
+            else ++dropped;
+        }
+    }
+    return dropped;
+}
+
+/* IntCICImpl<3>::gather, ippl/src/Particle/IntCIC.h:933-977 for a Vektor field:
+ * Ef = sum over the 8 corners, in order 000,100,...,111, of ((a*b)*c)*E.
+ * Corners outside the interior read the ZeroFace guard value 0. */
+void orc_gather(int64_t N, const double *R, const double *E, const double origin[3],
+                const double hr[3], const int n[3], double *Ef) {
+    for (int64_t p = 0; p < N; ++p) {
+        int l[3];
+        double dp[3];
+        int ok = 1;
+        for (int d = 0; d < 3; ++d) ok &= cic_dim(R[3 * p + d], origin[d], hr[d], &l[d], &dp[d]);
+        if (!ok) {
+            Ef[3 * p] = Ef[3 * p + 1] = Ef[3 * p + 2] = 0.0;
+            continue;
+        }
+        const double a0 = 1 - dp[0], a1 = 1 - dp[1], a2 = 1 - dp[2];
+        const double b0 = dp[0], b1 = dp[1], b2 = dp[2];
+        double w[8];
+        w[0] = a0 * a1 * a2;
+        w[1] = b0 * a1 * a2;
+        w[2] = a0 * b1 * a2;
+        w[3] = b0 * b1 * a2;
+        w[4] = a0 * a1 * b2;
+        w[5] = b0 * a1 * b2;
+        w[6] = a0 * b1 * b2;
+        w[7] = b0 * b1 * b2;
+        for (int d = 0; d < 3; ++d) {
+            double acc = 0.0;
+            for (int c = 0; c < 8; ++c) {
+                int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
+                double e = in_grid(i, j, k, n) ? E[3 * (((int64_t)k * n[1] + j) * n[0] + i) + d] : 0.0;
+                double term = w[c] * e;
+                acc = (c == 0) ? term : acc + term;
+            }
+            Ef[3 * p + d] = acc;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT (test-only mixed-radix complex FFT; any sign convention gives the    */
+/* same circular convolution, which is all the reference's RC/CR pair       */
+/* computes: ippl/src/FFT/FFT.hpp:768-775,1214,1308,1571,1673)              */
+/* ------------------------------------------------------------------------ */
+
+static int smallest_factor(int n) {
+    if (n % 4 == 0) return 4;
+    for (int p = 2; p * p <= n; ++p)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* Decimation-in-time recursion: X[0..n) = DFT of x[0], x[xs], ... with
+ * tw[k*tws] = exp(sign*2*pi*i*k/n). */
+static void fft_rec(const cplx *x, size_t xs, cplx *X, int n, const cplx *tw, size_t tws) {
+    if (n == 1) {
+        X[0] = x[0];
+        return;
+    }
+    int p = smallest_factor(n);
+    int m = n / p;
+    for (int r = 0; r < p; ++r) fft_rec(x + r * xs, xs * p, X + (size_t)r * m, m, tw, tws * p);
+    cplx t[64];
+    cplx *tt = (p <= 64) ? t : (cplx *)malloc(sizeof(cplx) * p);
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r) tt[r] = X[(size_t)r * m + k] * tw[((size_t)r * k % n) * tws];
+        for (int q = 0; q < p; ++q) {
+            cplx s = 0;
+            for (int r = 0; r < p; ++r) s += tt[r] * tw[((size_t)r * q * m % n) * tws];
+            X[k + (size_t)q * m] = s;
+        }
+    }
+    if (tt != t) free(tt);
+}
+
+/* In-place 1D complex DFT of a contiguous line; sign -1 forward, +1 backward, unnormalised. */
+void orc_fft1d(cplx *line, int n, int sign) {
+    cplx *tw = (cplx *)malloc(sizeof(cplx) * n);
+    cplx *in = (cplx *)malloc(sizeof(cplx) * n);
+    for (int k = 0; k < n; ++k) {
+        double a = 2.0 * ORC_PI * (double)k / (double)n;
+        tw[k] = cos(a) + sign * sin(a) * I;
+    }
+    memcpy(in, line, sizeof(cplx) * n);
+    fft_rec(in, 1, line, n, tw, 1);
+    free(in);
+    free(tw);
+}
+
+/* In-place 3D complex DFT, x-fastest dims m[0..2]. */
+void orc_fft3d(cplx *a, const int m[3], int sign) {
+    int maxm = m[0] > m[1] ? m[0] : m[1];
+    if (m[2] > maxm) maxm = m[2];
+    cplx *line = (cplx *)malloc(sizeof(cplx) * maxm);
+    cplx *in = (cplx *)malloc(sizeof(cplx) * maxm);
+    cplx *tw = (cplx *)malloc(sizeof(cplx) * maxm);
+    const size_t stride[3] = {1, (size_t)m[0], (size_t)m[0] * m[1]};
+    for (int ax = 0; ax < 3; ++ax) {
+        int L = m[ax];
+        for (int k = 0; k < L; ++k) {
+            double ang = 2.0 * ORC_PI * (double)k / (double)L;
+            tw[k] = cos(ang) + sign * sin(ang) * I;
+        }
+        int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+        for (int u = 0; u < m[o2]; ++u)
+            for (int v = 0; v < m[o1]; ++v) {
+                size_t base = (size_t)u * stride[o2] + (size_t)v * stride[o1];
+                for (int k = 0; k < L; ++k) in[k] = a[base + (size_t)k * stride[ax]];
+                fft_rec(in, 1, line, L, tw, 1);
+                for (int k = 0; k < L; ++k) a[base + (size_t)k * stride[ax]] = line[k];
+            }
+    }
+    free(line);
+    free(in);
+    free(tw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Green's functions                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* atan and log inside the IGF corner term.  The 8-corner difference cancels
+ * ~(r/h)^3 of the corner magnitude, so one-ulp libm differences reach ~1e-9 of
+ * far-field G and ~1e-11 of E at n=128: two implementations only agree to
+ * the 1e-12 parity bar if their atan/log agree bit for bit.  Mode 1 (default)
+ * evaluates them in binary128 and rounds once to double, i.e. correctly
+ * rounded values, which no libm choice can dispute; mode 0 uses the host libm
+ * as the reference binary does (glibc 2.35 here: not correctly rounded for
+ * ~0.2% of atan and ~0.01% of log arguments). */
+static int g_igf_libm_mode = 1;
+void orc_set_igf_libm(int mode) { g_igf_libm_mode = mode; }
+static double igf_atan(double x) {
+    return g_igf_libm_mode ? (double)atanq((__float128)x) : atan(x);
+}
+static double igf_log(double x) {
+    return g_igf_libm_mode ? (double)logq((__float128)x) : log(x);
+}
+double orc_atan_cr(double x) { return (double)atanq((__float128)x); }
+double orc_log_cr(double x) { return (double)logq((__float128)x); }
+
+/* Antiderivative term of FFTPoissonSolver::integratedGreensFunction,
+ * src/Solvers/FFTPoissonSolver.cpp:472-489, at corner (i,j,k). */
+static double igf_corner(int i, int j, int k, const double hr[3], double cellVolume) {
+    double v0 = i * hr[0] - hr[0] / 2;
+    double v1 = j * hr[1] - hr[1] / 2;
+    double v2 = k * hr[2] - hr[2] / 2;
+    double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    double t = -v2 * v2 * igf_atan(v0 * v1 / (v2 * r)) / 2;
+    t += -v1 * v1 * igf_atan(v0 * v2 / (v1 * r)) / 2;
+    t += -v0 * v0 * igf_atan(v1 * v2 / (v0 * r)) / 2;
+    t += v1 * v2 * igf_log(v0 + r);
+    t += v0 * v2 * igf_log(v1 + r);
+    t += v0 * v1 * igf_log(v2 + r);
+    return t / cellVolume;
+}
+
+/* Integrated Green's function octant G[i,j,k], i in [0,n0], j in [0,n1],
+ * k in [0,n2] (x-fastest, (n0+1)(n1+1)(n2+1) values):
+ *   corners T (src/Solvers/FFTPoissonSolver.cpp:469-495),
+ *   8-corner difference in the reference's order (FFTPoissonSolver.cpp:510-517),
+ *   and, if apply_quirk, G[0,0,0] := G[0,0,1] (FFTPoissonSolver.cpp:657).
+ * Index n needs T at n+1, which the reference reads from outside its
+ * tmpgreen field; any value there is harmless because the physical potential
+ * never reads G at index n (x-y spans (-n,n) on the 2n-periodic grid).  Here T
+ * at n+1 is the same formula. */
+void orc_igf_octant(const int n[3], const double hr[3], double *G, int apply_quirk) {
+    const int t0 = n[0] + 2, t1 = n[1] + 2, t2 = n[2] + 2;
+    double cellVolume = hr[0] * hr[1] * hr[2];
+    double *T = (double *)malloc(sizeof(double) * (size_t)t0 * t1 * t2);
+    for (int k = 0; k < t2; ++k)
+        for (int j = 0; j < t1; ++j)
+            for (int i = 0; i < t0; ++i)
+                T[((size_t)k * t1 + j) * t0 + i] = igf_corner(i, j, k, hr, cellVolume);
+#define TT(a, b, c) T[((size_t)(c) * t1 + (b)) * t0 + (a)]
+    const int g0 = n[0] + 1, g1 = n[1] + 1, g2 = n[2] + 1;
+    for (int k = 0; k < g2; ++k)
+        for (int j = 0; j < g1; ++j)
+            for (int i = 0; i < g0; ++i) {
+                double g = TT(i + 1, j + 1, k + 1);
+                g += TT(i + 1, j, k);
+                g += TT(i, j + 1, k);
+                g += TT(i, j, k + 1);
+                g -= TT(i + 1, j + 1, k);
+                g -= TT(i + 1, j, k + 1);
+                g -= TT(i, j + 1, k + 1);
+                g -= TT(i, j, k);
+                G[((size_t)k * g1 + j) * g0 + i] = g;
+            }
+#undef TT
+    free(T);
+    if (apply_quirk) G[0] = G[(size_t)1 * g1 * g0];
+}
+
+/* STANDARD Green's function octant, SpecializedGreensFunction<3>::calculate
+ * (src/Solvers/FFTPoissonSolver.cpp:32-43) with grnIField (FFTPoissonSolver.cpp:228-234):
+ * G = 1/sqrt(i^2 h0^2 + j^2 h1^2 + k^2 h2^2), G[0,0,0] := G[0,0,1]. */
+void orc_green_standard_octant(const int n[3], const double hr[3], double *G) {
+    const int g0 = n[0] + 1, g1 = n[1] + 1, g2 = n[2] + 1;
+    double hsq0 = hr[0] * hr[0], hsq1 = hr[1] * hr[1], hsq2 = hr[2] * hr[2];
+    for (int k = 0; k < g2; ++k)
+        for (int j = 0; j < g1; ++j)
+            for (int i = 0; i < g0; ++i) {
+                double g = (double)(i * i) * hsq0 + (double)(j * j) * hsq1 + (double)(k * k) * hsq2;
+                G[((size_t)k * g1 + j) * g0 + i] = 1.0 / sqrt(g);
+            }
+    G[0] = G[(size_t)1 * g1 * g0];
+}
+
+/* Even mirror of the octant onto the doubled grid (2n0,2n1,2n2):
+ * FFTPoissonSolver::mirrorRhoField, src/Solvers/FFTPoissonSolver.cpp:641-664. */
+void orc_mirror_green(const int n[3], const double *Goct, double *G2) {
+    const int m0 = 2 * n[0], m1 = 2 * n[1], m2 = 2 * n[2];
+    const int g0 = n[0] + 1, g1 = n[1] + 1;
+    for (int z = 0; z < m2; ++z) {
+        int fz = z <= n[2] ? z : m2 - z;
+        for (int y = 0; y < m1; ++y) {
+            int fy = y <= n[1] ? y : m1 - y;
+            for (int x = 0; x < m0; ++x) {
+                int fx = x <= n[0] ? x : m0 - x;
+                G2[((size_t)z * m1 + y) * m0 + x] = Goct[((size_t)fz * g1 + fy) * g0 + fx];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Poisson solve                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Circular convolution on the doubled grid of the zero-padded rho with the
+ * doubled Green's function; rho_inout holds rho on entry and phi on exit
+ * (x-fastest n0*n1*n2).  FFTPoissonSolver::computePotential,
+ * src/Solvers/FFTPoissonSolver.cpp:337-382 (forward transform of rho2 and of
+ * G unnormalised, pointwise product, backward transform scaled by 1/M). */
+void orc_convolve_doubled(const int n[3], double *rho_inout, const double *G2) {
+    const int m[3] = {2 * n[0], 2 * n[1], 2 * n[2]};
+    const size_t M = (size_t)m[0] * m[1] * m[2];
+    cplx *a = (cplx *)calloc(M, sizeof(cplx));
+    cplx *g = (cplx *)malloc(M * sizeof(cplx));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                a[((size_t)k * m[1] + j) * m[0] + i] = rho_inout[((size_t)k * n[1] + j) * n[0] + i];
+    for (size_t q = 0; q < M; ++q) g[q] = G2[q];
+    orc_fft3d(a, m, -1);
+    orc_fft3d(g, m, -1);
+    for (size_t q = 0; q < M; ++q) a[q] *= g[q];
+    orc_fft3d(a, m, +1);
+    double inv = 1.0 / (double)M;
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                rho_inout[((size_t)k * n[1] + j) * n[0] + i] = creal(a[((size_t)k * m[1] + j) * m[0] + i]) * inv;
+    free(a);
+    free(g);
+}
+
+/* FFTPoissonSolver::computePotential(rho, hr), FFTPoissonSolver.cpp:337-382,
+ * with the Green's function chosen by `greens` (0 INTEGRATED, 1 STANDARD).
+ * In place: rho in, phi out; no h^3 or 1/(4 pi eps0) factors. */
+void orc_compute_potential(const int n[3], double *rho_inout, const double hr[3], int greens) {
+    const size_t go = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    const size_t M = (size_t)8 * n[0] * n[1] * n[2];
+    double *Goct = (double *)malloc(go * sizeof(double));
+    double *G2 = (double *)malloc(M * sizeof(double));
+    if (greens == 1) orc_green_standard_octant(n, hr, Goct);
+    else orc_igf_octant(n, hr, Goct, 1);
+    orc_mirror_green(n, Goct, G2);
+    orc_convolve_doubled(n, rho_inout, G2);
+    free(Goct);
+    free(G2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gradient                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Grad Cell->Cell, ippl/src/Meshes/UniformCartesian.hpp:2127-2200: central
+ * differences (guard value 0 from ZeroFace, ippl/src/Field/BCond.h:594-605),
+ * then the border faces of component d replaced by one-sided second-order
+ * differences.  Output r = +Grad (the caller negates). */
+void orc_grad(const int n[3], const double *x, const double hr[3], double *r) {
+    const int n0 = n[0], n1 = n[1], n2 = n[2];
+    const double id0 = 0.5 / hr[0], id1 = 0.5 / hr[1], id2 = 0.5 / hr[2];
+#define XV(i, j, k) (((i) < 0 || (j) < 0 || (k) < 0 || (i) >= n0 || (j) >= n1 || (k) >= n2) ? 0.0 : x[((size_t)(k) * n1 + (j)) * n0 + (i)])
+#define RR(i, j, k, d) r[3 * (((size_t)(k) * n1 + (j)) * n0 + (i)) + (d)]
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                double sx = XV(i + 1, j, k) - XV(i - 1, j, k);
+                double sy = XV(i, j + 1, k) - XV(i, j - 1, k);
+                double sz = XV(i, j, k + 1) - XV(i, j, k - 1);
+                RR(i, j, k, 0) = (id0 * sx + 0.0 * sy) + 0.0 * sz;
+                RR(i, j, k, 1) = (0.0 * sx + id1 * sy) + 0.0 * sz;
+                RR(i, j, k, 2) = (0.0 * sx + 0.0 * sy) + id2 * sz;
+            }
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j) {
+            double s = (-1.0 * XV(2, j, k) + 4.0 * XV(1, j, k)) - 3.0 * XV(0, j, k);
+            RR(0, j, k, 0) = id0 * s;
+            s = (1.0 * XV(n0 - 3, j, k) - 4.0 * XV(n0 - 2, j, k)) + 3.0 * XV(n0 - 1, j, k);
+            RR(n0 - 1, j, k, 0) = id0 * s;
+        }
+    for (int k = 0; k < n2; ++k)
+        for (int i = 0; i < n0; ++i) {
+            double s = (-1.0 * XV(i, 2, k) + 4.0 * XV(i, 1, k)) - 3.0 * XV(i, 0, k);
+            RR(i, 0, k, 1) = id1 * s;
+            s = (1.0 * XV(i, n1 - 3, k) - 4.0 * XV(i, n1 - 2, k)) + 3.0 * XV(i, n1 - 1, k);
+            RR(i, n1 - 1, k, 1) = id1 * s;
+        }
+    for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n0; ++i) {
+            double s = (-1.0 * XV(i, j, 2) + 4.0 * XV(i, j, 1)) - 3.0 * XV(i, j, 0);
+            RR(i, j, 0, 2) = id2 * s;
+            s = (1.0 * XV(i, j, n2 - 3) - 4.0 * XV(i, j, n2 - 2)) + 3.0 * XV(i, j, n2 - 1);
+            RR(i, j, n2 - 1, 2) = id2 * s;
+        }
+#undef XV
+#undef RR
+}
+
+/* ------------------------------------------------------------------------ */
+/* Self fields                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* PartBunch::computeSelfFields(), src/Classic/Algorithms/PartBunch.cpp:482-648.
+ * Q is modified in place exactly as the reference does (Q *= dt; scatter;
+ * Q /= dt).  Ef is overwritten, Bf accumulated.  Optional outputs rho_out
+ * (scaled charge density that enters the solver), phi_out (potential in V)
+ * and E_out (AoS grid field after the Lorentz scaling) may be NULL.
+ * Returns the number of dropped deposits. */
+int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, double *Q,
+                                const double *dt, double dT, const double origin[3],
+                                const double hr[3], const int n[3], double coupling, int greens,
+                                double *Ef, double *Bf, double *rho_out, double *phi_out,
+                                double *E_out) {
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    double *rho = (double *)calloc(ncell, sizeof(double));
+    double *E = (double *)malloc(3 * ncell * sizeof(double));
+    for (int64_t i = 0; i < N; ++i) Q[i] *= dt[i];
+    int64_t dropped = orc_scatter(N, R, Q, origin, hr, n, rho);
+    for (int64_t i = 0; i < N; ++i) Q[i] /= dt[i];
+    for (size_t c = 0; c < ncell; ++c) rho[c] /= dT;
+
+    double gammaz = 0.0;
+    for (int64_t i = 0; i < N; ++i) gammaz += P[3 * i + 2];
+    gammaz /= (double)N;
+    gammaz *= gammaz;
+    gammaz = sqrt(gammaz + 1.0);
+    double hs[3] = {hr[0] * 1.0, hr[1] * 1.0, hr[2] * 1.0};
+    hs[2] *= gammaz;
+    double tmp2 = 1 / hs[0] * 1 / hs[1] * 1 / hs[2];
+    for (size_t c = 0; c < ncell; ++c) rho[c] *= tmp2;
+    if (rho_out) memcpy(rho_out, rho, ncell * sizeof(double));
+
+    orc_compute_potential(n, rho, hs, greens);
+
+    double vol = hs[0] * hs[1] * hs[2];
+    for (size_t c = 0; c < ncell; ++c) rho[c] *= vol;
+    for (size_t c = 0; c < ncell; ++c) rho[c] *= coupling;
+    if (phi_out) memcpy(phi_out, rho, ncell * sizeof(double));
+
+    orc_grad(n, rho, hr, E);
+    const double sc[3] = {gammaz / 1.0, gammaz / 1.0, 1.0 / (1.0 * gammaz)};
+    for (size_t c = 0; c < ncell; ++c)
+        for (int d = 0; d < 3; ++d) E[3 * c + d] = -E[3 * c + d] * sc[d];
+    if (E_out) memcpy(E_out, E, 3 * ncell * sizeof(double));
+
+    orc_gather(N, R, E, origin, hr, n, Ef);
+
+    double betaC = sqrt(gammaz * gammaz - 1.0) / gammaz / ORC_C;
+    for (int64_t i = 0; i < N; ++i) {
+        Bf[3 * i + 0] = Bf[3 * i + 0] - betaC * Ef[3 * i + 1];
+        Bf[3 * i + 1] = Bf[3 * i + 1] + betaC * Ef[3 * i + 0];
+    }
+    free(rho);
+    free(E);
+    return dropped;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boris pusher                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* ParallelTTracker::pushParticles (src/Algorithms/ParallelTTracker.h:448-455):
+ * switchToUnitlessPositions(true) (PartBunchBase.hpp:468-488),
+ * BorisPusher::push (src/Steppers/BorisPusher.h:116-124),
+ * switchOffUnitlessPositions(true) (PartBunchBase.hpp:493-513). */
+void orc_push(int64_t N, double *R, const double *P, const double *dt) {
+    for (int64_t i = 0; i < N; ++i) {
+        double s = ORC_C * dt[i];
+        for (int d = 0; d < 3; ++d) R[3 * i + d] /= s;
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        const double *p = P + 3 * i;
+        double g = sqrt(1.0 + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+        for (int d = 0; d < 3; ++d) R[3 * i + d] += 0.5 * p[d] / g;
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        double s = ORC_C * dt[i];
+        for (int d = 0; d < 3; ++d) R[3 * i + d] *= s;
+    }
+}
+
+static void boris_kick1(double P[3], const double Ef[3], const double Bf[3], double dt,
+                        double mass, double charge) {
+    /* BorisPusher::kick, src/Steppers/BorisPusher.h:56-113 */
+    double e = 0.5 * dt * charge * ORC_C / mass;
+    for (int d = 0; d < 3; ++d) P[d] += e * Ef[d];
+    double gamma = sqrt(1.0 + (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]));
+    double tf = 0.5 * dt * charge * ORC_C * ORC_C / (gamma * mass);
+    double t[3] = {tf * Bf[0], tf * Bf[1], tf * Bf[2]};
+    double w[3] = {P[0] + (P[1] * t[2] - P[2] * t[1]), P[1] + (P[2] * t[0] - P[0] * t[2]),
+                   P[2] + (P[0] * t[1] - P[1] * t[0])};
+    double sf = 2.0 / (1.0 + (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+    double s[3] = {sf * t[0], sf * t[1], sf * t[2]};
+    double c[3] = {w[1] * s[2] - w[2] * s[1], w[2] * s[0] - w[0] * s[2], w[0] * s[1] - w[1] * s[0]};
+    for (int d = 0; d < 3; ++d) P[d] += c[d];
+    double e2 = 0.5 * dt * charge * ORC_C / mass;
+    for (int d = 0; d < 3; ++d) P[d] += e2 * Ef[d];
+}
+
+/* ParallelTTracker::kickParticles, src/Algorithms/ParallelTTracker.h:442-446. */
+void orc_kick(int64_t N, double *P, const double *Ef, const double *Bf, const double *dt,
+              double charge, double mass) {
+    for (int64_t i = 0; i < N; ++i) boris_kick1(P + 3 * i, Ef + 3 * i, Bf + 3 * i, dt[i], mass, charge);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Moments and beam parameters                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double rmean[3], pmean[3], rrms[3], prms[3], eps_norm[3], eps[3], rprms[3];
+    double gamma_mean, ekin_MeV;
+    double moments[27];
+    int64_t n;
+} orc_stats;
+
+/* PartBunchBase::calcMoments, src/Classic/Algorithms/PartBunchBase.hpp:2185-2258
+ * (sequential accumulation; out[0..5] first moments, out[6..26] lower triangle
+ * row by row). */
+void orc_calc_moments(int64_t N, const double *R, const double *P, double out[27]) {
+    for (int q = 0; q < 27; ++q) out[q] = 0.0;
+    for (int64_t k = 0; k < N; ++k) {
+        double part[6] = {R[3 * k], P[3 * k], R[3 * k + 1], P[3 * k + 1], R[3 * k + 2], P[3 * k + 2]};
+        int l = 6;
+        for (int i = 0; i < 6; ++i) {
+            out[i] += part[i];
+            for (int j = 0; j <= i; ++j) out[l++] += part[i] * part[j];
+        }
+    }
+}
+
+static double mom(const double m[27], int i, int j) {
+    if (j > i) {
+        int t = i;
+        i = j;
+        j = t;
+    }
+    return m[6 + i * (i + 1) / 2 + j];
+}
+
+/* PartBunchBase::calcBeamParameters, PartBunchBase.hpp:1529-1613, with
+ * calcEMean (PartBunchBase.hpp:2099-2116).  mass_eV is the species rest energy. */
+void orc_beam_parameters_from_moments(int64_t N, const double m[27], double gamma_sum,
+                                      double mass_eV, orc_stats *s) {
+    memset(s, 0, sizeof(*s));
+    s->n = N;
+    memcpy(s->moments, m, sizeof(double) * 27);
+    if (N == 0) return;
+    const double Nd = (double)N;
+    double rsq[3], psq[3], rp[3], eps2[3];
+    for (int i = 0; i < 3; ++i) {
+        s->rmean[i] = m[2 * i] / Nd;
+        s->pmean[i] = m[2 * i + 1] / Nd;
+        rsq[i] = mom(m, 2 * i, 2 * i) - Nd * s->rmean[i] * s->rmean[i];
+        psq[i] = mom(m, 2 * i + 1, 2 * i + 1) - Nd * s->pmean[i] * s->pmean[i];
+        if (psq[i] < 0) psq[i] = 0;
+        rp[i] = mom(m, 2 * i, 2 * i + 1) - Nd * s->rmean[i] * s->pmean[i];
+    }
+    for (int i = 0; i < 3; ++i) eps2[i] = (rsq[i] * psq[i] - rp[i] * rp[i]) / (Nd * Nd);
+    for (int i = 0; i < 3; ++i) rp[i] /= Nd;
+    for (int i = 0; i < 3; ++i) {
+        s->rrms[i] = sqrt(rsq[i] / Nd);
+        s->prms[i] = sqrt(psq[i] / Nd);
+        s->eps_norm[i] = sqrt(eps2[i] > 0.0 ? eps2[i] : 0.0);
+        double tmp = s->rrms[i] * s->prms[i];
+        double fac = (tmp == 0) ? 0.0 : 1.0 / tmp;
+        s->rprms[i] = rp[i] * fac;
+    }
+    double gamma = gamma_sum / Nd;
+    s->gamma_mean = gamma;
+    for (int i = 0; i < 3; ++i) s->eps[i] = s->eps_norm[i] / (gamma * sqrt(1.0 - 1.0 / (gamma * gamma)));
+    (void)mass_eV;
+}
+
+void orc_calc_beam_parameters(int64_t N, const double *R, const double *P, double mass_eV,
+                              orc_stats *s) {
+    double m[27];
+    orc_calc_moments(N, R, P, m);
+    double gsum = 0.0, esum = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        const double *p = P + 3 * i;
+        double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        gsum += sqrt(1.0 + pp);
+        esum += sqrt(pp + 1.0);
+    }
+    orc_beam_parameters_from_moments(N, m, gsum, mass_eV, s);
+    if (N > 0) {
+        esum -= (double)N;
+        esum *= mass_eV * 1.0e-6;
+        s->ekin_MeV = esum / (double)N;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Quaternions and coordinate-system transformations                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double origin[3];
+    double q[4];
+    double M[9]; /* row-major */
+} orc_trafo;
+
+static void quat_mul(const double a[4], const double b[4], double out[4]) {
+    /* Quaternion::operator*=, src/Classic/Algorithms/Quaternion.cpp:84-93 */
+    double d = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+    double c0 = a[2] * b[3] - a[3] * b[2];
+    double c1 = a[3] * b[1] - a[1] * b[3];
+    double c2 = a[1] * b[2] - a[2] * b[1];
+    double r0 = a[0] * b[0] - d;
+    double r1 = a[0] * b[1] + b[0] * a[1] + c0;
+    double r2 = a[0] * b[2] + b[0] * a[2] + c1;
+    double r3 = a[0] * b[3] + b[0] * a[3] + c2;
+    out[0] = r0;
+    out[1] = r1;
+    out[2] = r2;
+    out[3] = r3;
+}
+
+static void quat_rotate(const double q[4], const double v[3], double out[3]) {
+    /* Quaternion::rotate, Quaternion.cpp:122-133: (q * (v * conj(q))).imag() */
+    double qv[4] = {0.0, v[0], v[1], v[2]};
+    double qc[4] = {q[0], -q[1], -q[2], -q[3]};
+    double t[4], r[4];
+    quat_mul(qv, qc, t);
+    quat_mul(q, t, r);
+    out[0] = r[1];
+    out[1] = r[2];
+    out[2] = r[3];
+}
+
+static void quat_rotation_matrix(const double qin[4], double M[9]) {
+    /* Quaternion::getRotationMatrix, Quaternion.cpp:135-150 (normalises first) */
+    double len = sqrt(qin[0] * qin[0] + qin[1] * qin[1] + qin[2] * qin[2] + qin[3] * qin[3]);
+    double r[4] = {qin[0] / len, qin[1] / len, qin[2] / len, qin[3] / len};
+    M[0] = 1 - 2 * (r[2] * r[2] + r[3] * r[3]);
+    M[1] = 2 * (-r[0] * r[3] + r[1] * r[2]);
+    M[2] = 2 * (r[0] * r[2] + r[1] * r[3]);
+    M[3] = 2 * (r[0] * r[3] + r[1] * r[2]);
+    M[4] = 1 - 2 * (r[1] * r[1] + r[3] * r[3]);
+    M[5] = 2 * (-r[0] * r[1] + r[2] * r[3]);
+    M[6] = 2 * (-r[0] * r[2] + r[1] * r[3]);
+    M[7] = 2 * (r[0] * r[1] + r[2] * r[3]);
+    M[8] = 1 - 2 * (r[1] * r[1] + r[2] * r[2]);
+}
+
+/* getQuaternion(u, ref=(0,0,1)), Quaternion.cpp:34-69.  The antiparallel
+ * branch draws a random axis in the reference; here it is (1,0,0). */
+void orc_get_quaternion_to_z(const double uin[3], double q[4]) {
+    double len = sqrt(uin[0] * uin[0] + uin[1] * uin[1] + uin[2] * uin[2]);
+    if (len < 1e-12) { /* the reference throws ("length of vector less than 1e-12") */
+        q[0] = 1.0;
+        q[1] = q[2] = q[3] = 0.0;
+        return;
+    }
+    double u[3] = {uin[0] / len, uin[1] / len, uin[2] / len};
+    const double ref[3] = {0.0 / 1.0, 0.0 / 1.0, 1.0 / 1.0};
+    double axis[3] = {u[1] * ref[2] - u[2] * ref[1], u[2] * ref[0] - u[0] * ref[2],
+                      u[0] * ref[1] - u[1] * ref[0]};
+    double normAxis = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    double dur = u[0] * ref[0] + u[1] * ref[1] + u[2] * ref[2];
+    if (normAxis < 1e-12) {
+        if (fabs(dur - 1.0) < 1e-12) {
+            q[0] = 1.0;
+            q[1] = q[2] = q[3] = 0.0;
+            return;
+        }
+        q[0] = 0.0;
+        q[1] = 1.0;
+        q[2] = 0.0;
+        q[3] = 0.0;
+        return;
+    }
+    for (int d = 0; d < 3; ++d) axis[d] /= normAxis;
+    double cosAngle = sqrt(0.5 * (1 + dur));
+    double sinAngle = sqrt(1 - cosAngle * cosAngle);
+    q[0] = cosAngle;
+    q[1] = sinAngle * axis[0];
+    q[2] = sinAngle * axis[1];
+    q[3] = sinAngle * axis[2];
+}
+
+/* CoordinateSystemTrafo(origin, orientation), CoordinateSystemTrafo.cpp:19-23 */
+void orc_trafo_make(const double origin[3], const double q[4], orc_trafo *t) {
+    memcpy(t->origin, origin, sizeof(double) * 3);
+    memcpy(t->q, q, sizeof(double) * 4);
+    quat_rotation_matrix(q, t->M);
+}
+
+/* CoordinateSystemTrafo::invert, CoordinateSystemTrafo.cpp:25-29 */
+void orc_trafo_inverted(const orc_trafo *a, orc_trafo *t) {
+    double o[3];
+    quat_rotate(a->q, a->origin, o);
+    for (int d = 0; d < 3; ++d) t->origin[d] = -o[d];
+    t->q[0] = a->q[0];
+    for (int d = 1; d < 4; ++d) t->q[d] = -a->q[d];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) t->M[3 * r + c] = a->M[3 * c + r];
+}
+
+/* transformTo / rotateTo, CoordinateSystemTrafo.h:77-94 (dot(Tenzor,Vektor),
+ * ippl/src/AppTypes/TSVMetaDot.h:255-263). */
+static void mat_vec(const double M[9], const double v[3], double out[3]) {
+    double a = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+    double b = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+    double c = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+}
+
+void orc_transform_to(const orc_trafo *t, int64_t N, double *R) {
+    for (int64_t i = 0; i < N; ++i) {
+        double v[3] = {R[3 * i] - t->origin[0], R[3 * i + 1] - t->origin[1], R[3 * i + 2] - t->origin[2]};
+        mat_vec(t->M, v, R + 3 * i);
+    }
+}
+
+void orc_rotate_to(const orc_trafo *t, int64_t N, double *V) {
+    for (int64_t i = 0; i < N; ++i) {
+        double v[3] = {V[3 * i], V[3 * i + 1], V[3 * i + 2]};
+        mat_vec(t->M, v, V + 3 * i);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tracker                                                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int n[3];
+    double dh;        /* BBOXINCR/100 */
+    int greens;       /* 0 INTEGRATED, 1 STANDARD */
+    int space_charge; /* 0: FSTYPE=NONE */
+    double dT;
+    double charge;  /* species charge in units of e */
+    double mass_eV; /* species rest energy */
+    double coupling;
+    /* analytic stand-ins for element fields, in the reference frame:
+     * B = (0,0,Bz), E = (0,0,Ez0*sin(2*pi*rf_freq*t + rf_phase)) */
+    double Bz, Ez0, rf_freq, rf_phase;
+    /* tracker state */
+    double t, pathLength;
+    double RefPartR[3], RefPartP[3];
+} orc_tracker;
+
+static void ext_fields(const orc_tracker *tr, double t, double E[3], double B[3]) {
+    E[0] = 0.0;
+    E[1] = 0.0;
+    E[2] = tr->Ez0 == 0.0 ? 0.0 : tr->Ez0 * sin(2.0 * ORC_PI * tr->rf_freq * t + tr->rf_phase);
+    B[0] = 0.0;
+    B[1] = 0.0;
+    B[2] = tr->Bz;
+}
+
+/* ParallelTTracker::updateReferenceParticle (src/Algorithms/ParallelTTracker.cpp:1123-1187)
+ * and updateRefToLabCSTrafo (ParallelTTracker.cpp:1199-1213).  Returns the
+ * update trafo applied to the bunch by transformBunch. */
+static void update_ref(orc_tracker *tr, orc_trafo *upd) {
+    double dt = tr->t < tr->dT ? tr->t : tr->dT;
+    double scale = ORC_C * dt;
+    double *R = tr->RefPartR, *P = tr->RefPartP;
+    for (int d = 0; d < 3; ++d) R[d] /= scale;
+    {
+        double g = sqrt(1.0 + (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]));
+        for (int d = 0; d < 3; ++d) R[d] += 0.5 * P[d] / g;
+    }
+    for (int d = 0; d < 3; ++d) R[d] *= scale;
+    double E[3], B[3];
+    ext_fields(tr, tr->t - 0.5 * dt, E, B);
+    double Ef[3] = {0.0 + E[0], 0.0 + E[1], 0.0 + E[2]};
+    double Bf[3] = {0.0 + B[0], 0.0 + B[1], 0.0 + B[2]};
+    boris_kick1(P, Ef, Bf, dt, tr->mass_eV, tr->charge);
+    for (int d = 0; d < 3; ++d) R[d] /= scale;
+    {
+        double g = sqrt(1.0 + (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]));
+        for (int d = 0; d < 3; ++d) R[d] += 0.5 * P[d] / g;
+    }
+    for (int d = 0; d < 3; ++d) R[d] *= scale;
+
+    tr->pathLength += sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    double q[4];
+    orc_get_quaternion_to_z(P, q);
+    orc_trafo_make(R, q, upd);
+    double nr[3], np[3];
+    double v[3] = {R[0] - upd->origin[0], R[1] - upd->origin[1], R[2] - upd->origin[2]};
+    mat_vec(upd->M, v, nr);
+    mat_vec(upd->M, P, np);
+    memcpy(R, nr, sizeof(nr));
+    memcpy(P, np, sizeof(np));
+}
+
+/* The space-charge part of one step, ParallelTTracker::computeSpaceChargeFields
+ * (src/Algorithms/ParallelTTracker.cpp:546-592) for a bunch without energy bins:
+ * moments -> quaternion -> rotate R into the beam frame -> boundp ->
+ * computeSelfFields -> rotate R, Ef, Bf back.  mesh_out (hr, origin) optional. */
+int64_t orc_space_charge(orc_tracker *tr, int64_t N, double *R, const double *P, double *Q,
+                         const double *dt, double *Ef, double *Bf, double mesh_out[6],
+                         double *rho_out, double *phi_out, double *E_out) {
+    double m[27];
+    orc_calc_moments(N, R, P, m);
+    double pmean[3] = {m[1] / (double)N, m[3] / (double)N, m[5] / (double)N};
+    double qa[4], qb[4];
+    orc_get_quaternion_to_z(pmean, qa);
+    qb[0] = qa[0];
+    qb[1] = -qa[1];
+    qb[2] = -qa[2];
+    qb[3] = -qa[3];
+    double o[3] = {0.0, 0.0, tr->pathLength};
+    orc_trafo b2r, r2b;
+    orc_trafo_make(o, qb, &b2r);
+    orc_trafo_inverted(&b2r, &r2b);
+    orc_transform_to(&r2b, N, R);
+    double hr[3], origin[3];
+    if (orc_boundp(N, R, tr->n, tr->dh, hr, origin) != 0) return -1;
+    if (mesh_out) {
+        memcpy(mesh_out, hr, sizeof(hr));
+        memcpy(mesh_out + 3, origin, sizeof(origin));
+    }
+    int64_t dropped = orc_compute_self_fields(N, R, P, Q, dt, tr->dT, origin, hr, tr->n, tr->coupling,
+                                              tr->greens, Ef, Bf, rho_out, phi_out, E_out);
+    orc_transform_to(&b2r, N, R);
+    orc_rotate_to(&b2r, N, Ef);
+    orc_rotate_to(&b2r, N, Bf);
+    return dropped;
+}
+
+/* ParallelTTracker::execute loop body, src/Algorithms/ParallelTTracker.cpp:343-388,
+ * for nsteps steps with per-step statistics (DataSink stat quantities after
+ * each step).  Returns 0 or a negative error. */
+int orc_track(orc_tracker *tr, int64_t N, double *R, double *P, double *Q, double *dt, int nsteps,
+              orc_stats *stats) {
+    double *Ef = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+    double *Bf = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+    for (int s = 0; s < nsteps; ++s) {
+        orc_push(N, R, P, dt);
+        memset(Ef, 0, sizeof(double) * 3 * N);
+        memset(Bf, 0, sizeof(double) * 3 * N);
+        if (tr->space_charge && N > 0) {
+            if (orc_space_charge(tr, N, R, P, Q, dt, Ef, Bf, NULL, NULL, NULL, NULL) < 0) {
+                free(Ef);
+                free(Bf);
+                return -1;
+            }
+        }
+        for (int64_t i = 0; i < N; ++i) {
+            double E[3], B[3];
+            ext_fields(tr, tr->t + 0.5 * dt[i], E, B);
+            for (int d = 0; d < 3; ++d) {
+                Ef[3 * i + d] += E[d];
+                Bf[3 * i + d] += B[d];
+            }
+        }
+        orc_kick(N, P, Ef, Bf, dt, tr->charge, tr->mass_eV);
+        orc_push(N, R, P, dt);
+        for (int64_t i = 0; i < N; ++i) dt[i] = tr->dT;
+        tr->t += tr->dT;
+        if (tr->t > 0.0) {
+            orc_trafo upd;
+            update_ref(tr, &upd);
+            orc_transform_to(&upd, N, R);
+            orc_rotate_to(&upd, N, P);
+        }
+        if (stats) orc_calc_beam_parameters(N, R, P, tr->mass_eV, &stats[s]);
+    }
+    free(Ef);
+    free(Bf);
+    return 0;
+}
+
+size_t orc_sizeof_stats(void) { return sizeof(orc_stats); }
+size_t orc_sizeof_tracker(void) { return sizeof(orc_tracker); }

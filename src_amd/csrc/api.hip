@@ -1,0 +1,743 @@
+// C ABI of the space-charge step (include/opalsc.h).  Host code only: owns
+// the context's device memory and stream and sequences the kernels.  No C++
+// exception crosses the boundary; every entry point returns a status code.
+#include "../../include/opalsc.h"
+#include "common.hpp"
+#include "crmath.hpp"
+#include "frame.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace opalsc;
+
+static_assert(sizeof(opalsc_stats) == sizeof(StatsOut), "stats layout");
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t b) {
+        if (b <= bytes) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (b == 0) return true;
+        if (hipMalloc(&p, b) != hipSuccess) return false;
+        bytes = b;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+std::string g_create_error;
+
+}  // namespace
+
+struct opalsc_ctx {
+    opalsc_config cfg;
+    std::string err;
+    hipStream_t stream = nullptr;
+    PoissonGrid grid;
+    Bunch bunch;
+    Scalars *d_sc = nullptr;
+    TrackState *d_ts = nullptr;
+    double *partials = nullptr;
+    unsigned long long *d_counters = nullptr;  // [0] qd max bits, [1] dropped
+    DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
+    double qmax = 0.0;        // max |Q| of the resident bunch
+    bool have_bunch = false, have_track = false;
+    Profiler prof;
+    double dh = 0.02;
+    int steps_done = 0;
+};
+
+#define FAIL(ctx, code, msg)   \
+    do {                       \
+        (ctx)->err = (msg);    \
+        return (code);         \
+    } while (0)
+
+#define HIPCHK(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) FAIL(ctx, OPALSC_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_LAUNCH(ctx) HIPCHK(ctx, hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// profiler
+// ---------------------------------------------------------------------------
+namespace opalsc {
+const char *prof_name(int id) {
+    static const char *names[PROF_COUNT] = {"push_rotate_bounds", "scalar", "sort", "zero_rho",
+                                            "deposit", "green", "green_dct", "fft_x", "fft_y",
+                                            "fft_z_conv", "grad", "gather_kick_push"};
+    return (id >= 0 && id < PROF_COUNT) ? names[id] : "?";
+}
+void Profiler::start() {
+    if (!enabled) return;
+    mark(-1);
+}
+void Profiler::mark(int id) {
+    if (!enabled) return;
+    if (used == (int)pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        pool.push_back(e);
+        ids.push_back(0);
+    }
+    ids[used] = id;
+    (void)hipEventRecord(pool[used], stream);
+    ++used;
+}
+void Profiler::flush() {
+    if (!enabled || used == 0) return;
+    (void)hipEventSynchronize(pool[used - 1]);
+    for (int i = 1; i < used; ++i) {
+        if (ids[i] < 0) continue;
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, pool[i - 1], pool[i]) == hipSuccess) {
+            ms[ids[i]] += t;
+            launches[ids[i]] += 1;
+        }
+    }
+    used = 0;
+}
+void Profiler::destroy() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    ids.clear();
+    used = 0;
+}
+}  // namespace opalsc
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_abi_version(void) { return OPALSC_ABI_VERSION; }
+
+extern "C" const char *opalsc_last_error(const opalsc_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+static int alloc_bunch(opalsc_ctx *c, int64_t cap) {
+    Bunch &b = c->bunch;
+    b.cap = cap;
+    double **arrs[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz, &b.q, &b.tmp, &b.qd};
+    for (double **a : arrs)
+        if (hipMalloc((void **)a, (size_t)std::max<int64_t>(cap, 1) * 8) != hipSuccess)
+            FAIL(c, OPALSC_ENOMEM, "hipMalloc of particle arrays failed");
+    uint32_t **u[] = {&b.id, &b.keys, &b.vals, &b.keys2, &b.vals2};
+    for (uint32_t **a : u)
+        if (hipMalloc((void **)a, (size_t)std::max<int64_t>(cap, 1) * 4) != hipSuccess)
+            FAIL(c, OPALSC_ENOMEM, "hipMalloc of particle index arrays failed");
+    b.sort_tmp_bytes = sort_temp_bytes(std::max<int64_t>(cap, 1));
+    if (hipMalloc(&b.sort_tmp, std::max<size_t>(b.sort_tmp_bytes, 16)) != hipSuccess)
+        FAIL(c, OPALSC_ENOMEM, "hipMalloc of sort scratch failed");
+    return 0;
+}
+
+static void free_bunch(Bunch &b) {
+    void *ps[] = {b.x, b.y, b.z, b.px, b.py, b.pz, b.q, b.tmp, b.qd, b.id, b.keys, b.vals, b.keys2, b.vals2, b.sort_tmp};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    b = Bunch();
+}
+
+extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
+    if (!out || !cfg) {
+        g_create_error = "null argument";
+        return OPALSC_EINVAL;
+    }
+    *out = nullptr;
+    if (cfg->precision != OPALSC_FP64 && cfg->precision != OPALSC_MIXED) {
+        g_create_error = "precision must be OPALSC_FP64 or OPALSC_MIXED";
+        return OPALSC_EINVAL;
+    }
+    if (cfg->greens != OPALSC_GREENS_INTEGRATED && cfg->greens != OPALSC_GREENS_STANDARD) {
+        g_create_error = "greens must be INTEGRATED or STANDARD";
+        return OPALSC_EINVAL;
+    }
+    if (cfg->max_particles < 0 || cfg->max_particles > (int64_t)0xFFFFFFF0) {
+        g_create_error = "max_particles out of range (0 .. 2^32-16)";
+        return OPALSC_EINVAL;
+    }
+    opalsc_ctx *c = new (std::nothrow) opalsc_ctx();
+    if (!c) {
+        g_create_error = "out of host memory";
+        return OPALSC_ENOMEM;
+    }
+    c->cfg = *cfg;
+    c->dh = cfg->bboxincr / 100.0;
+    auto bail = [&](int code) {
+        g_create_error = c->err;
+        opalsc_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(cfg->device) != hipSuccess) {
+        c->err = "hipSetDevice failed";
+        return bail(OPALSC_EHIP);
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(OPALSC_EHIP);
+    }
+    c->prof.stream = c->stream;
+    if (c->grid.init(cfg->n, cfg->precision == OPALSC_FP64, cfg->greens, c->stream, c->err) != 0)
+        return bail(OPALSC_EINVAL);
+    c->grid.prof = &c->prof;
+    if (alloc_bunch(c, cfg->max_particles) != 0) return bail(OPALSC_ENOMEM);
+    if (hipMalloc((void **)&c->d_sc, sizeof(Scalars)) != hipSuccess ||
+        hipMalloc((void **)&c->d_ts, sizeof(TrackState)) != hipSuccess ||
+        hipMalloc((void **)&c->partials, sizeof(double) * kRedBlocksMax * 64) != hipSuccess ||
+        hipMalloc((void **)&c->d_counters, sizeof(unsigned long long) * 4) != hipSuccess) {
+        c->err = "hipMalloc of scalar buffers failed";
+        return bail(OPALSC_ENOMEM);
+    }
+    Scalars z;
+    memset(&z, 0, sizeof(z));
+    for (int d = 0; d < 3; ++d) z.n[d] = cfg->n[d];
+    if (hipMemcpy(c->d_sc, &z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 4) != hipSuccess) {
+        c->err = "initialisation copy failed";
+        return bail(OPALSC_EHIP);
+    }
+    *out = c;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_destroy(opalsc_ctx *c) {
+    if (!c) return OPALSC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->grid.destroy();
+    free_bunch(c->bunch);
+    DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb, &c->statsb, &c->tmpd};
+    for (DevBuf *b : bs) b->release();
+    void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    c->prof.destroy();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_synchronize(opalsc_ctx *c) {
+    if (!c) return OPALSC_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+static int check_n(opalsc_ctx *c, int64_t N) {
+    if (N < 0) FAIL(c, OPALSC_EINVAL, "negative particle count");
+    if (N > c->bunch.cap) FAIL(c, OPALSC_ECAP, "more particles than max_particles");
+    return 0;
+}
+
+static int upload(opalsc_ctx *c, DevBuf &b, const void *src, size_t bytes) {
+    if (!b.ensure(bytes)) FAIL(c, OPALSC_ENOMEM, "hipMalloc of staging buffer failed");
+    if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+static void upload_scalars(opalsc_ctx *c, const Scalars &s) {
+    (void)hipMemcpyAsync(c->d_sc, &s, sizeof(s), hipMemcpyHostToDevice, c->stream);
+}
+
+static int download_scalars(opalsc_ctx *c, Scalars &s) {
+    HIPCHK(c, hipMemcpyAsync(&s, c->d_sc, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson solver
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double hr[3]) {
+    if (!c || !rho || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    Scalars s;
+    memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) {
+        s.n[d] = c->cfg.n[d];
+        s.hr[d] = hr[d];
+        s.hs[d] = hr[d];
+    }
+    s.dT = 1.0;
+    s.tmp2 = 1.0;
+    s.q2e = 1.0;
+    upload_scalars(c, s);
+    const size_t nc = c->grid.ncell;
+    HIPCHK(c, hipMemcpyAsync(c->grid.rho_f64, rho, nc * 8, hipMemcpyHostToDevice, c->stream));
+    c->grid.green(c->d_sc, c->grid.fp64);
+    c->grid.solve(c->d_sc, 1, 0);
+    CHECK_LAUNCH(c);
+    c->grid.widen(1, (double *)c->grid.rho_f64);
+    HIPCHK(c, hipMemcpyAsync(rho, c->grid.rho_f64, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mesh, CIC, sort
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_boundp(opalsc_ctx *c, int64_t N, const double *R, double hr[3], double origin[3]) {
+    if (!c || (!R && N > 0) || !hr || !origin) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    launch_bounds(c->stream, N, c->aosR.as<double>(), c->partials);
+    launch_boundp_scalars(c->stream, c->partials, N > 0 ? red_blocks(N) : 0, c->cfg.n, c->dh, c->d_sc);
+    CHECK_LAUNCH(c);
+    Scalars s;
+    if (int e = download_scalars(c, s)) return e;
+    if (!s.mesh_ok) FAIL(c, OPALSC_EMESH, "boundp: h<0, can not build a mesh");
+    for (int d = 0; d < 3; ++d) {
+        hr[d] = s.hr[d];
+        origin[d] = s.origin[d];
+    }
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_cic_index(opalsc_ctx *c, int64_t N, const double *R, const double origin[3],
+                                const double hr[3], int32_t *lgp, double *dpos) {
+    if (!c || (N > 0 && (!R || !lgp || !dpos)) || !origin || !hr) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) return OPALSC_OK;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (!c->aosP.ensure((size_t)N * 24) || !c->aosEf.ensure((size_t)N * 24))
+        FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    launch_cic_index(c->stream, N, c->aosR.as<double>(), origin, hr, c->aosP.as<int32_t>(), c->aosEf.as<double>());
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(lgp, c->aosP.p, (size_t)N * 12, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dpos, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+static int key_bits(const int n[3]) {
+    uint64_t m = (uint64_t)n[0] * n[1] * n[2] + 1;
+    int b = 1;
+    while (b < 32 && (1ull << b) < m) ++b;
+    return b;
+}
+
+extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, const double origin[3],
+                                   const double hr[3], uint32_t *perm) {
+    if (!c || (N > 0 && (!R || !perm)) || !origin || !hr) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) return OPALSC_OK;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    Bunch &b = c->bunch;
+    launch_soa_from_aos(c->stream, N, c->aosR.as<double>(), b.x, b.y, b.z);
+    Scalars s;
+    memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) {
+        s.n[d] = c->cfg.n[d];
+        s.hr[d] = hr[d];
+        s.origin[d] = origin[d];
+    }
+    upload_scalars(c, s);
+    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, b.keys, b.vals);
+    if (sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, key_bits(c->cfg.n)))
+        FAIL(c, OPALSC_EHIP, "radix sort failed");
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(perm, b.vals2, (size_t)N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_bunch = false;  // SoA buffers were reused
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// self fields
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double *R, const double *P,
+                                          double *Q, const double *dt, double dT,
+                                          const double origin[3], const double hr[3],
+                                          double coupling, double *Ef, double *Bf) {
+    if (!c || !origin || !hr) return OPALSC_EINVAL;
+    if (N > 0 && (!R || !P || !Q || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) FAIL(c, OPALSC_EINVAL, "computeSelfFields needs particles (the Lorentz factor divides by N)");
+    Bunch &b = c->bunch;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosBf, Bf, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->dtb, dt, (size_t)N * 8)) return e;
+    if (!c->aosEf.ensure((size_t)N * 24)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    HIPCHK(c, hipMemcpyAsync(b.q, Q, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 2, c->stream));
+    launch_soa_from_aos(c->stream, N, c->aosR.as<double>(), b.x, b.y, b.z);
+    launch_selffield_prep(c->stream, N, c->aosP.as<double>(), b.q, c->dtb.as<double>(), b.qd, c->partials,
+                          &c->d_counters[0]);
+    launch_selffield_scalars(c->stream, c->partials, red_blocks(N), N, dT, origin, hr, c->cfg.n, coupling,
+                             &c->d_counters[0], c->d_sc);
+    launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+    launch_deposit(c->stream, 0, N, b.x, b.y, b.z, nullptr, b.qd, (long long *)c->grid.rho_i64, c->d_sc,
+                   &c->d_counters[1]);
+    c->grid.green(c->d_sc, c->grid.fp64);
+    c->grid.solve(c->d_sc, 0, 1);
+    c->grid.grad(c->d_sc);
+    launch_gather_selffield(c->stream, c->grid.fp64, N, b.x, b.y, b.z, c->grid.Eg, c->d_sc,
+                            c->aosEf.as<double>(), c->aosBf.as<double>());
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(Ef, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Q, b.q, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_bunch = false;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_get_grid(opalsc_ctx *c, int which, double *out) {
+    if (!c || !out) return OPALSC_EINVAL;
+    const size_t nc = c->grid.ncell;
+    if (which == OPALSC_GRID_RHO) {
+        c->grid.rho_to_double(c->d_sc, (double *)c->grid.rho_f64);
+        HIPCHK(c, hipMemcpyAsync(out, c->grid.rho_f64, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    } else if (which == OPALSC_GRID_PHI) {
+        c->grid.widen(1, (double *)c->grid.rho_f64);
+        HIPCHK(c, hipMemcpyAsync(out, c->grid.rho_f64, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    } else if (which == OPALSC_GRID_EFIELD) {
+        if (!c->tmpd.ensure(nc * 24)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+        c->grid.widen(2, c->tmpd.as<double>());
+        HIPCHK(c, hipMemcpyAsync(out, c->tmpd.p, nc * 24, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        FAIL(c, OPALSC_EINVAL, "unknown grid");
+    }
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pusher, moments
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_push_half(opalsc_ctx *c, int64_t N, double *R, const double *P, const double *dt) {
+    if (!c || (N > 0 && (!R || !P || !dt))) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) return OPALSC_OK;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->dtb, dt, (size_t)N * 8)) return e;
+    launch_push(c->stream, N, c->aosR.as<double>(), c->aosP.as<double>(), c->dtb.as<double>());
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(R, c->aosR.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_kick(opalsc_ctx *c, int64_t N, double *P, const double *Ef, const double *Bf,
+                           const double *dt, double charge, double mass_eV) {
+    if (!c || (N > 0 && (!P || !Ef || !Bf || !dt))) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) return OPALSC_OK;
+    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosEf, Ef, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosBf, Bf, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->dtb, dt, (size_t)N * 8)) return e;
+    launch_kick(c->stream, N, c->aosP.as<double>(), c->aosEf.as<double>(), c->aosBf.as<double>(),
+                c->dtb.as<double>(), charge, mass_eV);
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(P, c->aosP.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_beam_parameters(opalsc_ctx *c, int64_t N, const double *R, const double *P,
+                                      double mass_eV, opalsc_stats *out) {
+    if (!c || !out || (N > 0 && (!R || !P))) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) {
+        memset(out, 0, sizeof(*out));
+        return OPALSC_OK;
+    }
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+    if (!c->statsb.ensure(sizeof(opalsc_stats))) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    launch_moments(c->stream, N, c->aosR.as<double>(), c->aosP.as<double>(), c->partials);
+    launch_stats_from_partials(c->stream, c->partials, red_blocks(N), N, mass_eV, c->statsb.p);
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(out, c->statsb.p, sizeof(opalsc_stats), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident tracker
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const double *P, const double *Q) {
+    if (!c || (N > 0 && (!R || !P || !Q))) return OPALSC_EINVAL;
+    if (int e = check_n(c, N)) return e;
+    Bunch &b = c->bunch;
+    b.N = N;
+    if (N > 0) {
+        if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+        launch_soa_from_aos(c->stream, N, c->aosR.as<double>(), b.x, b.y, b.z);
+        if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+        launch_soa_from_aos(c->stream, N, c->aosP.as<double>(), b.px, b.py, b.pz);
+        HIPCHK(c, hipMemcpyAsync(b.q, Q, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+        launch_iota(c->stream, N, b.id);
+        CHECK_LAUNCH(c);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // the staging copies are not needed by the tracker
+        c->aosR.release();
+        c->aosP.release();
+    }
+    double qm = 0.0;
+    for (int64_t i = 0; i < N; ++i) qm = std::max(qm, std::fabs(Q[i]));
+    c->qmax = qm;
+    c->have_bunch = true;
+    c->have_track = false;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g) {
+    if (!c || !g) return OPALSC_EINVAL;
+    if (int e = check_n(c, g->n)) return e;
+    Bunch &b = c->bunch;
+    b.N = g->n;
+    const double qeach = g->n > 0 ? g->q_total / (double)g->n : 0.0;
+    launch_generate(c->stream, g->n, g->seed, g->sigma_r, g->sigma_p, g->p0, qeach, g->halo_frac,
+                    g->halo_radius, b);
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->qmax = std::fabs(qeach);
+    c->have_bunch = true;
+    c->have_track = false;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_bunch_get(opalsc_ctx *c, double *R, double *P, double *Q) {
+    if (!c) return OPALSC_EINVAL;
+    if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "no resident bunch");
+    Bunch &b = c->bunch;
+    const int64_t N = b.N;
+    if (N == 0) return OPALSC_OK;
+    std::vector<uint32_t> id(N);
+    std::vector<double> tmp(N);
+    HIPCHK(c, hipMemcpyAsync(id.data(), b.id, (size_t)N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto fetch = [&](const double *src, double *dst, int stride, int off) -> int {
+        HIPCHK(c, hipMemcpy(tmp.data(), src, (size_t)N * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; ++i) dst[(size_t)id[i] * stride + off] = tmp[i];
+        return 0;
+    };
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (R) {
+        if (int e = fetch(b.x, R, 3, 0)) return e;
+        if (int e = fetch(b.y, R, 3, 1)) return e;
+        if (int e = fetch(b.z, R, 3, 2)) return e;
+    }
+    if (P) {
+        if (int e = fetch(b.px, P, 3, 0)) return e;
+        if (int e = fetch(b.py, P, 3, 1)) return e;
+        if (int e = fetch(b.pz, P, 3, 2)) return e;
+    }
+    if (Q)
+        if (int e = fetch(b.q, Q, 1, 0)) return e;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
+    if (!c || !p) return OPALSC_EINVAL;
+    if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "track_init before bunch_set/bunch_generate");
+    if (!(p->dT > 0.0)) FAIL(c, OPALSC_EINVAL, "dT must be positive");
+    TrackState ts;
+    memset(&ts, 0, sizeof(ts));
+    ts.dT = p->dT;
+    ts.charge = p->charge;
+    ts.mass_eV = p->mass_eV;
+    ts.coupling = p->coupling;
+    ts.Bz = p->Bz;
+    ts.Ez0 = p->Ez0;
+    ts.rf_freq = p->rf_freq;
+    ts.rf_phase = p->rf_phase;
+    ts.t = p->t;
+    ts.pathLength = p->pathLength;
+    for (int d = 0; d < 3; ++d) {
+        ts.RefPartR[d] = p->RefPartR[d];
+        ts.RefPartP[d] = p->RefPartP[d];
+    }
+    ts.space_charge = p->space_charge;
+    ts.step = 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_ts, &ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    Scalars s;
+    memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) s.n[d] = c->cfg.n[d];
+    s.N = c->bunch.N;
+    s.qd_absmax = c->qmax * p->dT * (1.0 + 0x1p-50);
+    s.dT = p->dT;
+    s.coupling = p->coupling;
+    upload_scalars(c, s);
+    launch_moments_soa(c->stream, c->bunch, c->partials);
+    launch_track_init_moments(c->stream, c->partials, red_blocks(c->bunch.N), c->d_ts, c->d_sc);
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_track = true;
+    c->steps_done = 0;
+    return OPALSC_OK;
+}
+
+static void sort_bunch(opalsc_ctx *c) {
+    Bunch &b = c->bunch;
+    const int64_t N = b.N;
+    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, b.keys, b.vals);
+    sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, key_bits(c->cfg.n));
+    double **arrs[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz, &b.q};
+    for (double **a : arrs) {
+        launch_permute(c->stream, N, b.vals2, *a, b.tmp);
+        std::swap(*a, b.tmp);
+    }
+    launch_permute_u32(c->stream, N, b.vals2, b.id, b.keys);
+    std::swap(b.id, b.keys);
+}
+
+extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
+    if (!c || nsteps < 0) return OPALSC_EINVAL;
+    if (!c->have_track) FAIL(c, OPALSC_ESTATE, "track before track_init");
+    if (nsteps == 0) return OPALSC_OK;
+    if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps)) FAIL(c, OPALSC_ENOMEM, "stats buffer");
+    Bunch &b = c->bunch;
+    const int nb = red_blocks(b.N);
+    const bool cr = c->grid.fp64;
+    c->prof.start();
+    for (int s = 0; s < nsteps; ++s) {
+        launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N);
+        c->prof.mark(PROF_SCALAR);
+        launch_track_alpha(c->stream, b, c->d_sc, c->d_ts, c->partials);
+        c->prof.mark(PROF_PUSH_BOUNDS);
+        launch_track_mesh(c->stream, c->partials, nb, c->cfg.n, c->dh, c->d_ts, c->d_sc);
+        c->prof.mark(PROF_SCALAR);
+        const int interval = c->cfg.sort_interval;
+        if (interval > 0 && (c->steps_done % interval) == 0) {
+            sort_bunch(c);
+            c->prof.mark(PROF_SORT);
+        }
+        launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+        HIPCHK(c, hipMemsetAsync(&c->d_counters[1], 0, sizeof(unsigned long long), c->stream));
+        c->prof.mark(PROF_ZERO);
+        launch_deposit(c->stream, 1, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
+                       c->d_sc, &c->d_counters[1]);
+        c->prof.mark(PROF_DEPOSIT);
+        c->grid.green(c->d_sc, cr);
+        c->grid.solve(c->d_sc, 0, 1);
+        c->grid.grad(c->d_sc);
+        launch_track_gamma(c->stream, c->grid.fp64, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials);
+        c->prof.mark(PROF_GATHER_PUSH);
+        launch_track_post(c->stream, c->partials, nb, c->d_ts, c->d_sc,
+                          (char *)c->statsb.p + sizeof(opalsc_stats) * s, &c->d_counters[1]);
+        c->prof.mark(PROF_SCALAR);
+        ++c->steps_done;
+        CHECK_LAUNCH(c);
+    }
+    if (per_step)
+        HIPCHK(c, hipMemcpyAsync(per_step, c->statsb.p, sizeof(opalsc_stats) * (size_t)nsteps,
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prof.flush();
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_track_get_params(opalsc_ctx *c, opalsc_track_params *out) {
+    if (!c || !out) return OPALSC_EINVAL;
+    if (!c->have_track) FAIL(c, OPALSC_ESTATE, "no tracker state");
+    TrackState ts;
+    HIPCHK(c, hipMemcpyAsync(&ts, c->d_ts, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->dT = ts.dT;
+    out->charge = ts.charge;
+    out->mass_eV = ts.mass_eV;
+    out->coupling = ts.coupling;
+    out->Bz = ts.Bz;
+    out->Ez0 = ts.Ez0;
+    out->rf_freq = ts.rf_freq;
+    out->rf_phase = ts.rf_phase;
+    out->t = ts.t;
+    out->pathLength = ts.pathLength;
+    for (int d = 0; d < 3; ++d) {
+        out->RefPartR[d] = ts.RefPartR[d];
+        out->RefPartP[d] = ts.RefPartP[d];
+    }
+    out->space_charge = ts.space_charge;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_get_mesh(opalsc_ctx *c, double hr[3], double origin[3]) {
+    if (!c || !hr || !origin) return OPALSC_EINVAL;
+    Scalars s;
+    if (int e = download_scalars(c, s)) return e;
+    for (int d = 0; d < 3; ++d) {
+        hr[d] = s.hr[d];
+        origin[d] = s.origin[d];
+    }
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// correctly rounded math hooks
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_host_atan_cr(int64_t n, const double *x, double *y) {
+    if (n < 0 || (n > 0 && (!x || !y))) return OPALSC_EINVAL;
+    for (int64_t i = 0; i < n; ++i) y[i] = crm::atan_cr(x[i]);
+    return OPALSC_OK;
+}
+extern "C" int opalsc_host_log_cr(int64_t n, const double *x, double *y) {
+    if (n < 0 || (n > 0 && (!x || !y))) return OPALSC_EINVAL;
+    for (int64_t i = 0; i < n; ++i) y[i] = crm::log_cr(x[i]);
+    return OPALSC_OK;
+}
+static int device_cr(opalsc_ctx *c, int which, int64_t n, const double *x, double *y) {
+    if (!c || n < 0 || (n > 0 && (!x || !y))) return OPALSC_EINVAL;
+    if (n == 0) return OPALSC_OK;
+    if (int e = upload(c, c->aosR, x, (size_t)n * 8)) return e;
+    if (!c->aosP.ensure((size_t)n * 8)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    launch_cr_test(c->stream, which, n, c->aosR.as<double>(), c->aosP.as<double>());
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(y, c->aosP.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+extern "C" int opalsc_device_atan_cr(opalsc_ctx *c, int64_t n, const double *x, double *y) {
+    return device_cr(c, 0, n, x, y);
+}
+extern "C" int opalsc_device_log_cr(opalsc_ctx *c, int64_t n, const double *x, double *y) {
+    return device_cr(c, 1, n, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// profiling
+// ---------------------------------------------------------------------------
+extern "C" int opalsc_profile_enable(opalsc_ctx *c, int enable) {
+    if (!c) return OPALSC_EINVAL;
+    c->prof.flush();
+    c->prof.enabled = enable != 0;
+    for (int i = 0; i < PROF_COUNT; ++i) {
+        c->prof.ms[i] = 0.0;
+        c->prof.launches[i] = 0;
+    }
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_profile_read(opalsc_ctx *c, int cap, const char **names, double *ms, int64_t *launches) {
+    if (!c) return OPALSC_EINVAL;
+    c->prof.flush();
+    int n = std::min<int>(cap, PROF_COUNT);
+    for (int i = 0; i < n; ++i) {
+        if (names) names[i] = prof_name(i);
+        if (ms) ms[i] = c->prof.ms[i];
+        if (launches) launches[i] = c->prof.launches[i];
+    }
+    return PROF_COUNT;
+}

@@ -1,0 +1,174 @@
+// Shared definitions for the gfx950 space-charge kernels.
+//
+// All translation units are compiled with -ffp-contract=off: every multiply
+// and add below is rounded separately, in the operation order of the OPAL
+// function each helper cites, so that cell indices and CIC weights come out
+// bit-identical to the reference's scalar code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace opalsc {
+
+constexpr double kC = 299792458.0;               // src/Classic/Physics/Physics.h:52
+constexpr double kPi = 3.14159265358979323846;   // src/Classic/Physics/Physics.h:31
+constexpr int kThreads = 256;                    // threads per block for particle kernels
+constexpr int kRedBlocksMax = 2048;              // reduction partial slots
+constexpr int kMomentSlots = 32;                 // 27 moments + gamma sum + spare
+
+template <typename T> struct vec4_of;
+template <> struct vec4_of<float> { using type = float4; };
+template <> struct vec4_of<double> { using type = double4; };
+template <typename T> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+
+// Mesh, step scalars and trafos, produced on the device by the scalar kernels
+// and read by every grid and particle kernel of a step.
+struct Scalars {
+    int n[3];
+    double hr[3], origin[3];   // mesh of the beam frame (boundp)
+    double rmin[3], rmax[3];   // particle bounds
+    // deposit fixed point: value = int * q2e; e chosen from N*max|Q*dt|
+    double q2e, e2q;
+    double qd_absmax;
+    // computeSelfFields scalars (src/Classic/Algorithms/PartBunch.cpp:496-644)
+    double dT, gammaz, betaC, hs[3], tmp2, vol, coupling;
+    double gscale[3];          // (gz, gz, 1/gz)
+    double psum[3];            // sum of P over the bunch
+    // coordinate transformations for the step (row-major 3x3)
+    double M_rb[9], o_rb[3];   // reference -> beam
+    double M_br[9], o_br[3];   // beam -> reference
+    double M_u[9], o_u[3];     // reference-frame update (transformBunch)
+    double Eext[3], Bext[3];   // external field for the bunch at t + dT/2
+    int mesh_ok;
+    int64_t N;
+};
+
+// Tracker state carried across steps on the device.
+struct TrackState {
+    double dT, charge, mass_eV, coupling;
+    double Bz, Ez0, rf_freq, rf_phase;
+    double t, pathLength;
+    double RefPartR[3], RefPartP[3];
+    int space_charge;
+    int step;
+};
+
+// ---- cloud-in-cell lookup -------------------------------------------------
+// One dimension of IntCICImpl<3>::scatter/gather (ippl/src/Particle/IntCIC.h:826-851):
+// getVertexBelow (ippl/src/Meshes/UniformCartesian.hpp:1025-1039), getCellPosition
+// (UniformCartesian.hpp:1084-1097), getDeltaCell = h*1 (UniformCartesian.hpp:1151-1158),
+// lgp offset -1 when the cell centre lies above the particle.  Returns false
+// (particle dropped) when (x-o)/h is non-finite or beyond +-2^30.
+__host__ __device__ inline bool cic_dim(double x, double o, double h, int &lgp, double &dpos) {
+    double t = (x - o) / h;
+    if (!(t > -1073741824.0 && t < 1073741824.0)) return false;
+    int i = (int)t;
+    int ngp = (x >= o) ? i : i - 1;
+    double g = ((double)ngp + 0.5) * h + o;
+    double delta = h * 1;
+    int off = 0;
+    if (g > x) {
+        off = -1;
+        g = g - delta;
+    }
+    lgp = ngp + off;
+    dpos = (x - g) / delta;
+    return true;
+}
+
+// CIC corner weights in the reference order 000,100,010,110,001,101,011,111,
+// each ((a*b)*c) (ippl/src/Particle/IntCIC.h:853-860).
+__host__ __device__ inline void cic_weights(const double d[3], double w[8]) {
+    const double a0 = 1 - d[0], a1 = 1 - d[1], a2 = 1 - d[2];
+    const double b0 = d[0], b1 = d[1], b2 = d[2];
+    w[0] = a0 * a1 * a2;
+    w[1] = b0 * a1 * a2;
+    w[2] = a0 * b1 * a2;
+    w[3] = b0 * b1 * a2;
+    w[4] = a0 * a1 * b2;
+    w[5] = b0 * a1 * b2;
+    w[6] = a0 * b1 * b2;
+    w[7] = b0 * b1 * b2;
+}
+
+// ---- Boris pusher ----------------------------------------------------------
+// Half drift in unitless positions: switchToUnitlessPositions, BorisPusher::push,
+// switchOffUnitlessPositions (src/Classic/Algorithms/PartBunchBase.hpp:468-513,
+// src/Steppers/BorisPusher.h:116-124).
+__host__ __device__ inline void push_half(double &x, double &y, double &z, double px, double py,
+                                          double pz, double dt) {
+    const double s = kC * dt;
+    x /= s;
+    y /= s;
+    z /= s;
+    const double g = sqrt(1.0 + (px * px + py * py + pz * pz));
+    x += 0.5 * px / g;
+    y += 0.5 * py / g;
+    z += 0.5 * pz / g;
+    x *= s;
+    y *= s;
+    z *= s;
+}
+
+// BorisPusher::kick (src/Steppers/BorisPusher.h:56-113).
+__host__ __device__ inline void boris_kick(double P[3], const double E[3], const double B[3],
+                                           double dt, double mass, double charge) {
+    const double e = 0.5 * dt * charge * kC / mass;
+    P[0] += e * E[0];
+    P[1] += e * E[1];
+    P[2] += e * E[2];
+    const double gamma = sqrt(1.0 + (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]));
+    const double tf = 0.5 * dt * charge * kC * kC / (gamma * mass);
+    const double t0 = tf * B[0], t1 = tf * B[1], t2 = tf * B[2];
+    const double w0 = P[0] + (P[1] * t2 - P[2] * t1);
+    const double w1 = P[1] + (P[2] * t0 - P[0] * t2);
+    const double w2 = P[2] + (P[0] * t1 - P[1] * t0);
+    const double sf = 2.0 / (1.0 + (t0 * t0 + t1 * t1 + t2 * t2));
+    const double s0 = sf * t0, s1 = sf * t1, s2 = sf * t2;
+    const double c0 = w1 * s2 - w2 * s1;
+    const double c1 = w2 * s0 - w0 * s2;
+    const double c2 = w0 * s1 - w1 * s0;
+    P[0] += c0;
+    P[1] += c1;
+    P[2] += c2;
+    const double e2 = 0.5 * dt * charge * kC / mass;
+    P[0] += e2 * E[0];
+    P[1] += e2 * E[1];
+    P[2] += e2 * E[2];
+}
+
+// dot(Tenzor<double,3>, Vektor<double,3>) (ippl/src/AppTypes/TSVMetaDot.h:255-263).
+__host__ __device__ inline void mat_vec(const double *M, const double v[3], double out[3]) {
+    const double a = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+    const double b = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+    const double c = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+}
+
+// CoordinateSystemTrafo::transformTo (src/Classic/Algorithms/CoordinateSystemTrafo.h:77-80).
+__host__ __device__ inline void transform_to(const double *M, const double *o, double r[3]) {
+    const double v[3] = {r[0] - o[0], r[1] - o[1], r[2] - o[2]};
+    mat_vec(M, v, r);
+}
+
+// ---- wave/block reductions ------------------------------------------------
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+}  // namespace opalsc

@@ -1,0 +1,159 @@
+// LDS-resident batched 1D FFT for gfx950 (no vendor FFT library).
+//
+// A workgroup of kThreads threads transforms W lines of length L held in LDS
+// with an interleaved layout: element (line, idx) lives at idx*P + line,
+// P = W + 1.  Consecutive threads work on consecutive lines, so every LDS
+// access of a stage touches consecutive addresses and the odd pitch keeps
+// the transposing global loads of the x pass conflict-free.
+//
+// Each stage is a Stockham autosort radix-R step: thread-owned butterflies
+// are read into registers, twiddled, transformed, and written back after a
+// barrier.  E = W*L/kThreads elements per thread is a compile-time constant
+// divisible by every radix used, so each thread owns exactly E/R butterflies
+// per stage.  Radices 2, 3, 4 and 8 are supported: L = 2^a or 3*2^a.
+#pragma once
+#include "common.hpp"
+
+namespace opalsc {
+
+struct FftPlan {
+    int L;           // transform length
+    int W;           // lines per workgroup
+    int P;           // LDS pitch (W + 1)
+    int E;           // elements per thread (template parameter of the kernels)
+    int nst;         // number of stages
+    int radix[12];   // radix of each stage
+    const void *tw;  // exp(-2 pi i m / L), m in [0, L), complex T
+};
+
+template <typename C>
+__device__ inline C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C>
+__device__ inline C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C>
+__device__ inline C cmul(C a, C b) { return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename C>
+__device__ inline C cmul_mi(C a, bool inv) { return inv ? C{-a.y, a.x} : C{a.y, -a.x}; }
+
+template <typename C>
+__device__ inline void dft2(C *v, bool) {
+    C a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+}
+
+template <typename C>
+__device__ inline void dft3(C *v, bool inv) {
+    using T = decltype(v[0].x);
+    const T h = T(0.866025403784438646763723170752936183);  // sqrt(3)/2
+    C t1 = cadd(v[1], v[2]);
+    C t2 = C{v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y};
+    C d = csub(v[1], v[2]);
+    C s = C{h * d.x, h * d.y};
+    C r = cmul_mi(s, inv);  // forward: -i*s
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, r);
+    v[2] = csub(t2, r);
+}
+
+template <typename C>
+__device__ inline void dft4(C *v, bool inv) {
+    C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    C a2 = cadd(v[1], v[3]), a3 = cmul_mi(csub(v[1], v[3]), inv);
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[3] = csub(a1, a3);
+}
+
+template <typename C>
+__device__ inline void dft8(C *v, bool inv) {
+    using T = decltype(v[0].x);
+    const T r = T(0.707106781186547524400844362104849039);
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    dft4(e, inv);
+    dft4(o, inv);
+    // twiddles W8^k, forward W8 = (1-i)/sqrt2
+    C o1 = inv ? C{r * (o[1].x - o[1].y), r * (o[1].x + o[1].y)}
+               : C{r * (o[1].x + o[1].y), r * (o[1].y - o[1].x)};
+    C o2 = cmul_mi(o[2], inv);
+    C o3 = inv ? C{-r * (o[3].x + o[3].y), r * (o[3].x - o[3].y)}
+               : C{r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+template <typename C, int R>
+__device__ inline void dftR(C *v, bool inv) {
+    if constexpr (R == 2) dft2(v, inv);
+    else if constexpr (R == 3) dft3(v, inv);
+    else if constexpr (R == 4) dft4(v, inv);
+    else dft8(v, inv);
+}
+
+// One Stockham stage on LDS data; Ns = product of the radices already applied.
+template <typename C, int E, int R>
+__device__ inline void fft_stage(C *buf, const C *tw, int L, int W, int P, int Ns, bool inv) {
+    constexpr int NB = E / R;
+    const int LR = L / R;
+    const int twstep = L / (Ns * R);
+    C v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int g = threadIdx.x + b * kThreads;
+        const int line = g % W, j = g / W;
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * LR) * P + line];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            C w = tw[r * k * twstep];
+            if (inv) w.y = -w.y;
+            v[b][r] = cmul(v[b][r], w);
+        }
+        dftR<C, R>(v[b], inv);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int g = threadIdx.x + b * kThreads;
+        const int line = g % W, j = g / W;
+        const int k = j % Ns;
+        const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[(idxD + r * Ns) * P + line] = v[b][r];
+    }
+    __syncthreads();
+}
+
+// Transform the W lines in buf (caller has synchronised after filling it).
+template <typename C, int E>
+__device__ inline void fft_lds(C *buf, const FftPlan &pl, bool inv) {
+    const C *tw = (const C *)pl.tw;
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.radix[s];
+        if constexpr (E % 8 == 0) {
+            if (R == 8) { fft_stage<C, E, 8>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 8; continue; }
+        }
+        if constexpr (E % 4 == 0) {
+            if (R == 4) { fft_stage<C, E, 4>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 4; continue; }
+        }
+        if constexpr (E % 3 == 0) {
+            if (R == 3) { fft_stage<C, E, 3>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 3; continue; }
+        }
+        if constexpr (E % 2 == 0) {
+            if (R == 2) { fft_stage<C, E, 2>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 2; continue; }
+        }
+    }
+}
+
+}  // namespace opalsc

@@ -1,0 +1,145 @@
+// Host-side launch interfaces between the C ABI and the kernels.
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+
+#include <string>
+#include <vector>
+
+namespace opalsc {
+
+enum ProfId {
+    PROF_PUSH_BOUNDS = 0,  // push + beam-frame rotation + bounds
+    PROF_SCALAR,           // single-thread scalar kernels
+    PROF_SORT,             // cell keys, radix sort, permutation
+    PROF_ZERO,             // clear charge grid
+    PROF_DEPOSIT,          // CIC scatter (int64 fixed point)
+    PROF_GREEN,            // IGF / standard Green's function octant
+    PROF_DCT,              // DCT-I of the octant -> Ghat
+    PROF_FFT_X,            // r2c / c2r along x
+    PROF_FFT_Y,            // along y
+    PROF_FFT_Z,            // z: forward * Ghat * inverse
+    PROF_GRAD,             // E = -Grad(phi)
+    PROF_GATHER_PUSH,      // gather + kick + push + frame update + moments
+    PROF_COUNT
+};
+
+const char *prof_name(int id);
+
+// Accumulates HIP-event intervals between consecutive marks on one stream.
+struct Profiler {
+    bool enabled = false;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> ids;
+    int used = 0;
+    double ms[PROF_COUNT] = {0};
+    long long launches[PROF_COUNT] = {0};
+    void start();
+    void mark(int id);
+    void flush();  // synchronises and folds the recorded intervals into ms[]
+    void destroy();
+};
+
+struct PoissonGrid {
+    int n[3] = {0, 0, 0}, L[3] = {0, 0, 0}, K0 = 0;
+    bool fp64 = true;
+    int greens = 0;
+    size_t ncell = 0, noct = 0;
+    FftPlan plan[3];
+    void *twd[3] = {nullptr, nullptr, nullptr};
+    void *rho_i64 = nullptr;  // long long[ncell]
+    void *rho_f64 = nullptr;  // double[ncell] (compute_potential input)
+    void *A = nullptr, *B = nullptr;
+    void *Goct = nullptr;     // double[noct]
+    void *Ghat = nullptr;     // T[noct]
+    void *phi = nullptr;      // T[ncell]
+    void *Eg = nullptr;       // vec4<T>[ncell]
+    hipStream_t stream = nullptr;
+    Profiler *prof = nullptr;
+
+    int init(const int nn[3], bool fp64, int greens, hipStream_t s, std::string &err);
+    void destroy();
+    void green(const Scalars *d_sc, bool correctly_rounded);
+    // src_kind 0: int64 fixed-point rho_i64 (self-field scaling); 1: rho_f64 as is
+    // outscale 1: phi *= vol*coupling (self fields); 0: potential only
+    void solve(const Scalars *d_sc, int src_kind, int outscale);
+    void grad(const Scalars *d_sc);
+    void rho_to_double(const Scalars *d_sc, double *out);
+    void widen(int which, double *out);
+    void profile_mark(int id) {
+        if (prof) prof->mark(id);
+    }
+};
+
+// Particle buffers in structure-of-arrays form.
+struct Bunch {
+    int64_t cap = 0, N = 0;
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    double *q = nullptr;
+    uint32_t *id = nullptr;
+    double *tmp = nullptr;       // permutation scratch
+    double *qd = nullptr;        // Q*dt (self-field entry point)
+    uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+};
+
+// Launchers (particles.hip)
+int red_blocks(int64_t N);
+void launch_soa_from_aos(hipStream_t s, int64_t N, const double *R, double *x, double *y, double *z);
+void launch_aos_from_soa(hipStream_t s, int64_t N, const double *x, const double *y, const double *z, double *R);
+void launch_selffield_prep(hipStream_t s, int64_t N, const double *P, double *Q, const double *dt,
+                           double *qd, double *partials, unsigned long long *qdmax_bits);
+void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
+                              const double origin[3], const double hr[3], const int n[3],
+                              double coupling, const unsigned long long *qdmax_bits, Scalars *sc);
+void launch_deposit(hipStream_t s, int mode, int64_t N, const double *x, const double *y,
+                    const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
+                    unsigned long long *dropped);
+void launch_gather_selffield(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y,
+                             const double *z, const void *Eg, const Scalars *sc, double *Ef,
+                             double *Bf);
+void launch_cic_index(hipStream_t s, int64_t N, const double *R, const double origin[3],
+                      const double hr[3], int32_t *lgp, double *dpos);
+void launch_bounds(hipStream_t s, int64_t N, const double *R, double *partials);
+void launch_boundp_scalars(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
+                           Scalars *sc);
+void launch_push(hipStream_t s, int64_t N, double *R, const double *P, const double *dt);
+void launch_kick(hipStream_t s, int64_t N, double *P, const double *Ef, const double *Bf,
+                 const double *dt, double charge, double mass);
+void launch_moments(hipStream_t s, int64_t N, const double *R, const double *P, double *partials);
+void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, int64_t N,
+                                double mass_eV, void *stats_out);
+void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,
+                      const Scalars *sc, uint32_t *keys, uint32_t *vals);
+void launch_permute(hipStream_t s, int64_t N, const uint32_t *perm, const double *src, double *dst);
+void launch_permute_u32(hipStream_t s, int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst);
+void launch_generate(hipStream_t s, int64_t N, uint64_t seed, const double sr[3], const double sp[3],
+                     double p0, double qeach, double halo_frac, double halo_radius, Bunch &b);
+void launch_absmax(hipStream_t s, int64_t N, const double *v, unsigned long long *out_bits);
+void launch_iota(hipStream_t s, int64_t N, uint32_t *v);
+void launch_fill_i64(hipStream_t s, size_t n, long long *p);
+void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y);
+
+// Tracker kernels
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N);
+void launch_track_alpha(hipStream_t s, Bunch &b, const Scalars *sc, const TrackState *ts,
+                        double *partials);
+void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
+                       const TrackState *ts, Scalars *sc);
+void launch_track_gamma(hipStream_t s, bool fp64, Bunch &b, const void *Eg, const Scalars *sc,
+                        const TrackState *ts, double *partials);
+void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
+                       void *stats_out, const unsigned long long *dropped);
+void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,
+                               Scalars *sc);
+void launch_moments_soa(hipStream_t s, const Bunch &b, double *partials);
+
+// Sorting (sort.hip)
+size_t sort_temp_bytes(int64_t N);
+int sort_pairs(hipStream_t s, void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout,
+               const uint32_t *vin, uint32_t *vout, int64_t N, int end_bit);
+
+}  // namespace opalsc

@@ -1,0 +1,876 @@
+// Particle kernels of the space-charge step, gfx950.
+//
+// Device layout of the bunch is structure-of-arrays (x,y,z,px,py,pz,q,id):
+// every particle kernel streams 8-byte words with unit stride across a wave.
+// Reductions are deterministic: each thread accumulates its grid-stride
+// particles in order, threads and blocks are combined in a fixed butterfly /
+// block order, and sums are carried as double-double so that the moment sums
+// are the correctly rounded sums of the per-particle terms.
+#include "common.hpp"
+#include "crmath.hpp"
+#include "frame.hpp"
+#include "kernels.hpp"
+
+namespace opalsc {
+
+using crm::dd;
+using crm::dd_add;
+
+constexpr int kSlots = 64;  // doubles per block in the partials buffer
+
+int red_blocks(int64_t N) {
+    int64_t b = (N + kThreads - 1) / kThreads;
+    if (b > kRedBlocksMax) b = kRedBlocksMax;
+    return (int)b;
+}
+
+static int stream_blocks(int64_t N) {
+    int64_t b = (N + kThreads - 1) / kThreads;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---- deterministic block reductions ---------------------------------------
+template <int K>
+__device__ inline void block_sum_dd(double (&h)[K], double (&l)[K], double *lds) {
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double oh = __shfl_xor(h[k], off, 64), ol = __shfl_xor(l[k], off, 64);
+            dd r = dd_add(dd{h[k], l[k]}, dd{oh, ol});
+            h[k] = r.hi;
+            l[k] = r.lo;
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) {
+            lds[(wave * K + k) * 2] = h[k];
+            lds[(wave * K + k) * 2 + 1] = l[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; ++k) {
+            dd a = {lds[2 * k], lds[2 * k + 1]};
+            for (int w = 1; w < nw; ++w) a = dd_add(a, dd{lds[(w * K + k) * 2], lds[(w * K + k) * 2 + 1]});
+            h[k] = a.hi;
+            l[k] = a.lo;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline void block_minmax(double (&mn)[3], double (&mx)[3], double *lds) {
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = wave_min(mn[d]);
+        mx[d] = wave_max(mx[d]);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) {
+            lds[wave * 6 + d] = mn[d];
+            lds[wave * 6 + 3 + d] = mx[d];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < nw; ++w)
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = fmin(mn[d], lds[w * 6 + d]);
+                mx[d] = fmax(mx[d], lds[w * 6 + 3 + d]);
+            }
+    __syncthreads();
+}
+
+// Sum K dd values over nb partial blocks (slots 2k, 2k+1) with one block of
+// 256 threads; result valid in thread 0.
+template <int K>
+__device__ inline void reduce_partials_dd(const double *partials, int nb, double (&out)[K], double *lds) {
+    double h[K], l[K];
+    for (int k = 0; k < K; ++k) h[k] = l[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int k = 0; k < K; ++k) {
+            dd r = dd_add(dd{h[k], l[k]}, dd{partials[b * kSlots + 2 * k], partials[b * kSlots + 2 * k + 1]});
+            h[k] = r.hi;
+            l[k] = r.lo;
+        }
+    block_sum_dd<K>(h, l, lds);
+    for (int k = 0; k < K; ++k) out[k] = h[k] + l[k];
+}
+
+__device__ inline void reduce_partials_minmax(const double *partials, int nb, double (&mn)[3],
+                                              double (&mx)[3], double *lds) {
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = __builtin_inf();
+        mx[d] = -__builtin_inf();
+    }
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], partials[b * kSlots + d]);
+            mx[d] = fmax(mx[d], partials[b * kSlots + 3 + d]);
+        }
+    block_minmax(mn, mx, lds);
+}
+
+// ---- layout conversion ------------------------------------------------------
+__global__ void k_soa_from_aos(int64_t N, const double *R, double *x, double *y, double *z) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] = R[3 * i];
+        y[i] = R[3 * i + 1];
+        z[i] = R[3 * i + 2];
+    }
+}
+__global__ void k_aos_from_soa(int64_t N, const double *x, const double *y, const double *z, double *R) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        R[3 * i] = x[i];
+        R[3 * i + 1] = y[i];
+        R[3 * i + 2] = z[i];
+    }
+}
+void launch_soa_from_aos(hipStream_t s, int64_t N, const double *R, double *x, double *y, double *z) {
+    if (N > 0) k_soa_from_aos<<<stream_blocks(N), kThreads, 0, s>>>(N, R, x, y, z);
+}
+void launch_aos_from_soa(hipStream_t s, int64_t N, const double *x, const double *y, const double *z, double *R) {
+    if (N > 0) k_aos_from_soa<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, R);
+}
+
+// ---- computeSelfFields entry point -----------------------------------------
+// Q *= dt; scatter; Q /= dt (PartBunch.cpp:493-495) and sum(P)[2] (PartBunch.cpp:499).
+__global__ __launch_bounds__(256) void k_selffield_prep(int64_t N, const double *P, double *Q,
+                                                        const double *dt, double *qd,
+                                                        double *partials,
+                                                        unsigned long long *qdmax_bits) {
+    __shared__ double lds[4 * 2 * 2];
+    double h[1] = {0.0}, l[1] = {0.0};
+    double amax = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = Q[i] * dt[i];
+        qd[i] = v;
+        Q[i] = v / dt[i];
+        amax = fmax(amax, fabs(v));
+        dd r = crm::two_sum(h[0], P[3 * i + 2]);
+        h[0] = r.hi;
+        l[0] += r.lo;
+    }
+    block_sum_dd<1>(h, l, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x * kSlots] = h[0];
+        partials[blockIdx.x * kSlots + 1] = l[0];
+    }
+    for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(qdmax_bits, (unsigned long long)__double_as_longlong(amax));
+}
+
+void launch_selffield_prep(hipStream_t s, int64_t N, const double *P, double *Q, const double *dt,
+                           double *qd, double *partials, unsigned long long *qdmax_bits) {
+    if (N <= 0) return;
+    k_selffield_prep<<<red_blocks(N), kThreads, 0, s>>>(N, P, Q, dt, qd, partials, qdmax_bits);
+}
+
+// Fixed-point exponent: every cell sum is bounded by N*max|Q*dt|; scaling by
+// 2^e with N*max|qd|*2^e < 2^62 leaves headroom below 2^63 for rounding.
+__device__ inline void set_fixed_point(Scalars *sc, int64_t N, double qdmax) {
+    double B = (double)N * qdmax * (1.0 + 0x1p-40);
+    int e = 0;
+    if (B > 0.0) {
+        int E;
+        (void)frexp(B, &E);
+        e = 62 - E;
+        if (e > 1000) e = 1000;
+    }
+    sc->e2q = ldexp(1.0, e);
+    sc->q2e = ldexp(1.0, -e);
+    sc->qd_absmax = qdmax;
+}
+
+// The computeSelfFields scalars (PartBunch.cpp:497-511,542-554,573-575,641).
+__device__ inline void selffield_scalars(Scalars *sc, double psum_z, int64_t N, double dT,
+                                         double coupling) {
+    double gammaz = psum_z / (double)N;
+    gammaz *= gammaz;
+    gammaz = sqrt(gammaz + 1.0);
+    sc->gammaz = gammaz;
+    sc->dT = dT;
+    sc->coupling = coupling;
+    double hs[3] = {sc->hr[0] * 1.0, sc->hr[1] * 1.0, sc->hr[2] * 1.0};
+    hs[2] *= gammaz;
+    for (int d = 0; d < 3; ++d) sc->hs[d] = hs[d];
+    sc->tmp2 = 1 / hs[0] * 1 / hs[1] * 1 / hs[2];
+    sc->vol = hs[0] * hs[1] * hs[2];
+    sc->gscale[0] = gammaz / 1.0;
+    sc->gscale[1] = gammaz / 1.0;
+    sc->gscale[2] = 1.0 / (1.0 * gammaz);
+    sc->betaC = sqrt(gammaz * gammaz - 1.0) / gammaz / kC;
+}
+
+__global__ __launch_bounds__(256) void k_selffield_scalars(const double *partials, int nb, int64_t N,
+                                                           double dT, double o0, double o1, double o2,
+                                                           double h0, double h1, double h2, int n0,
+                                                           int n1, int n2, double coupling,
+                                                           const unsigned long long *qdmax_bits,
+                                                           Scalars *sc) {
+    __shared__ double lds[4 * 2 * 2];
+    double out[1];
+    reduce_partials_dd<1>(partials, nb, out, lds);
+    if (threadIdx.x != 0) return;
+    sc->n[0] = n0;
+    sc->n[1] = n1;
+    sc->n[2] = n2;
+    sc->origin[0] = o0;
+    sc->origin[1] = o1;
+    sc->origin[2] = o2;
+    sc->hr[0] = h0;
+    sc->hr[1] = h1;
+    sc->hr[2] = h2;
+    sc->N = N;
+    sc->mesh_ok = 1;
+    sc->psum[2] = out[0];
+    selffield_scalars(sc, out[0], N, dT, coupling);
+    set_fixed_point(sc, N, __longlong_as_double((long long)*qdmax_bits));
+}
+
+void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
+                              const double origin[3], const double hr[3], const int n[3],
+                              double coupling, const unsigned long long *qdmax_bits, Scalars *sc) {
+    k_selffield_scalars<<<1, kThreads, 0, s>>>(partials, nb, N, dT, origin[0], origin[1], origin[2],
+                                               hr[0], hr[1], hr[2], n[0], n[1], n[2], coupling,
+                                               qdmax_bits, sc);
+}
+
+// ---- CIC scatter into int64 fixed point -------------------------------------
+// IntCICImpl<3>::scatter (ippl/src/Particle/IntCIC.h:816-860): the 8 weights
+// ((a*b)*c)*qd are rounded to the fixed-point grid and added with 64-bit
+// integer atomics, so the charge grid does not depend on the order in which
+// particles are processed.  MODE 0: qd given; MODE 1: qd = Q*dT with Q
+// updated to (Q*dT)/dT as the reference leaves it (PartBunch.cpp:493-495).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_deposit(int64_t N, const double *x, const double *y,
+                                                 const double *z, double *q, const double *qdv,
+                                                 long long *rho, const Scalars *sc,
+                                                 unsigned long long *dropped) {
+    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    const double e2q = sc->e2q, dT = sc->dT;
+    unsigned long long nd = 0;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        double qd;
+        if constexpr (MODE == 0) {
+            qd = qdv[p];
+        } else {
+            const double qv = q[p];
+            qd = qv * dT;
+            const double qn = qd / dT;
+            if (qn != qv) q[p] = qn;
+        }
+        int l[3];
+        double d[3];
+        bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
+        ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
+        ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
+        if (!ok) {
+            nd += 8;
+            continue;
+        }
+        double w[8];
+        cic_weights(d, w);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
+            if (i >= 0 && j >= 0 && k >= 0 && i < n0 && j < n1 && k < n2) {
+                const double v = w[c] * qd;
+                const long long iv = (long long)rint(v * e2q);
+                if (iv != 0)
+                    atomicAdd((unsigned long long *)&rho[((size_t)k * n1 + j) * n0 + i], (unsigned long long)iv);
+            } else {
+                ++nd;
+            }
+        }
+    }
+    if (nd) atomicAdd(dropped, nd);
+}
+
+void launch_deposit(hipStream_t s, int mode, int64_t N, const double *x, const double *y,
+                    const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
+                    unsigned long long *dropped) {
+    if (N <= 0) return;
+    int blocks = (int)std::min<int64_t>((N + kThreads - 1) / kThreads, 65536);
+    if (mode == 0) k_deposit<0><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+    else k_deposit<1><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+}
+
+// ---- CIC gather ---------------------------------------------------------------
+// IntCICImpl<3>::gather (ippl/src/Particle/IntCIC.h:933-977): sum over the 8
+// corners in reference order of ((a*b)*c)*E; corners outside the grid read 0.
+template <typename T>
+__device__ inline void gather_E(const typename vec4_of<T>::type *Eg, const int l[3], const double d[3],
+                                int n0, int n1, int n2, double Ef[3]) {
+    double w[8];
+    cic_weights(d, w);
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
+        typename vec4_of<T>::type e;
+        if (i >= 0 && j >= 0 && k >= 0 && i < n0 && j < n1 && k < n2) e = Eg[((size_t)k * n1 + j) * n0 + i];
+        else e.x = e.y = e.z = e.w = T(0);
+        const T wc = (T)w[c];
+        if (c == 0) {
+            a0 = wc * e.x;
+            a1 = wc * e.y;
+            a2 = wc * e.z;
+        } else {
+            a0 = a0 + wc * e.x;
+            a1 = a1 + wc * e.y;
+            a2 = a2 + wc * e.z;
+        }
+    }
+    Ef[0] = (double)a0;
+    Ef[1] = (double)a1;
+    Ef[2] = (double)a2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_selffield(int64_t N, const double *x, const double *y,
+                                                          const double *z,
+                                                          const typename vec4_of<T>::type *Eg,
+                                                          const Scalars *sc, double *Ef, double *Bf) {
+    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    const double betaC = sc->betaC;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        int l[3];
+        double d[3], e[3] = {0.0, 0.0, 0.0};
+        bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
+        ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
+        ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
+        if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, e);
+        Ef[3 * p] = e[0];
+        Ef[3 * p + 1] = e[1];
+        Ef[3 * p + 2] = e[2];
+        Bf[3 * p] = Bf[3 * p] - betaC * e[1];
+        Bf[3 * p + 1] = Bf[3 * p + 1] + betaC * e[0];
+    }
+}
+
+void launch_gather_selffield(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y,
+                             const double *z, const void *Eg, const Scalars *sc, double *Ef,
+                             double *Bf) {
+    if (N <= 0) return;
+    if (fp64) k_gather_selffield<double><<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, (const double4 *)Eg, sc, Ef, Bf);
+    else k_gather_selffield<float><<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, (const float4 *)Eg, sc, Ef, Bf);
+}
+
+__global__ void k_cic_index(int64_t N, const double *R, double o0, double o1, double o2, double h0,
+                            double h1, double h2, int32_t *lgp, double *dpos) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        const double o[3] = {o0, o1, o2}, h[3] = {h0, h1, h2};
+        for (int d = 0; d < 3; ++d) {
+            int l = 0;
+            double dp = 0.0;
+            cic_dim(R[3 * p + d], o[d], h[d], l, dp);
+            lgp[3 * p + d] = l;
+            dpos[3 * p + d] = dp;
+        }
+    }
+}
+
+void launch_cic_index(hipStream_t s, int64_t N, const double *R, const double origin[3],
+                      const double hr[3], int32_t *lgp, double *dpos) {
+    if (N <= 0) return;
+    k_cic_index<<<stream_blocks(N), kThreads, 0, s>>>(N, R, origin[0], origin[1], origin[2], hr[0],
+                                                      hr[1], hr[2], lgp, dpos);
+}
+
+// ---- bounds, push, kick, moments (AoS entry points) --------------------------
+__global__ __launch_bounds__(256) void k_bounds(int64_t N, const double *R, double *partials) {
+    __shared__ double lds[4 * 6];
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], R[3 * i + d]);
+            mx[d] = fmax(mx[d], R[3 * i + d]);
+        }
+    block_minmax(mn, mx, lds);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            partials[blockIdx.x * kSlots + d] = mn[d];
+            partials[blockIdx.x * kSlots + 3 + d] = mx[d];
+        }
+}
+
+void launch_bounds(hipStream_t s, int64_t N, const double *R, double *partials) {
+    if (N > 0) k_bounds<<<red_blocks(N), kThreads, 0, s>>>(N, R, partials);
+}
+
+__global__ __launch_bounds__(256) void k_boundp_scalars(const double *partials, int nb, int n0, int n1,
+                                                        int n2, double dh, Scalars *sc) {
+    __shared__ double lds[4 * 6];
+    double mn[3], mx[3];
+    reduce_partials_minmax(partials, nb, mn, mx, lds);
+    if (threadIdx.x != 0) return;
+    if (nb == 0)
+        for (int d = 0; d < 3; ++d) mn[d] = mx[d] = 0.0;
+    const int n[3] = {n0, n1, n2};
+    for (int d = 0; d < 3; ++d) {
+        sc->n[d] = n[d];
+        sc->rmin[d] = mn[d];
+        sc->rmax[d] = mx[d];
+    }
+    sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin) ? 1 : 0;
+}
+
+void launch_boundp_scalars(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
+                           Scalars *sc) {
+    k_boundp_scalars<<<1, kThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, sc);
+}
+
+__global__ void k_push(int64_t N, double *R, const double *P, const double *dt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double x = R[3 * i], y = R[3 * i + 1], z = R[3 * i + 2];
+        push_half(x, y, z, P[3 * i], P[3 * i + 1], P[3 * i + 2], dt[i]);
+        R[3 * i] = x;
+        R[3 * i + 1] = y;
+        R[3 * i + 2] = z;
+    }
+}
+void launch_push(hipStream_t s, int64_t N, double *R, const double *P, const double *dt) {
+    if (N > 0) k_push<<<stream_blocks(N), kThreads, 0, s>>>(N, R, P, dt);
+}
+
+__global__ void k_kick(int64_t N, double *P, const double *Ef, const double *Bf, const double *dt,
+                       double charge, double mass) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double p[3] = {P[3 * i], P[3 * i + 1], P[3 * i + 2]};
+        const double e[3] = {Ef[3 * i], Ef[3 * i + 1], Ef[3 * i + 2]};
+        const double b[3] = {Bf[3 * i], Bf[3 * i + 1], Bf[3 * i + 2]};
+        boris_kick(p, e, b, dt[i], mass, charge);
+        P[3 * i] = p[0];
+        P[3 * i + 1] = p[1];
+        P[3 * i + 2] = p[2];
+    }
+}
+void launch_kick(hipStream_t s, int64_t N, double *P, const double *Ef, const double *Bf,
+                 const double *dt, double charge, double mass) {
+    if (N > 0) k_kick<<<stream_blocks(N), kThreads, 0, s>>>(N, P, Ef, Bf, dt, charge, mass);
+}
+
+// part = (x, px, y, py, z, pz): 6 sums, 21 products (calcMoments order), gamma sum.
+__device__ inline void accumulate_moments(double (&m)[28], double x, double y, double z, double px,
+                                          double py, double pz) {
+    const double part[6] = {x, px, y, py, z, pz};
+    int l = 6;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        m[i] += part[i];
+#pragma unroll
+        for (int j = 0; j <= i; ++j) m[l++] += part[i] * part[j];
+    }
+    m[27] += sqrt(1.0 + (px * px + py * py + pz * pz));
+}
+
+__device__ inline void store_moment_partials(double (&m)[28], double *partials, double *lds) {
+    double l[28];
+    for (int k = 0; k < 28; ++k) l[k] = 0.0;
+    block_sum_dd<28>(m, l, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 28; ++k) {
+            partials[blockIdx.x * kSlots + 2 * k] = m[k];
+            partials[blockIdx.x * kSlots + 2 * k + 1] = l[k];
+        }
+}
+
+__global__ __launch_bounds__(256) void k_moments(int64_t N, const double *R, const double *P, double *partials) {
+    __shared__ double lds[4 * 28 * 2];
+    double m[28];
+    for (int k = 0; k < 28; ++k) m[k] = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        accumulate_moments(m, R[3 * i], R[3 * i + 1], R[3 * i + 2], P[3 * i], P[3 * i + 1], P[3 * i + 2]);
+    store_moment_partials(m, partials, lds);
+}
+void launch_moments(hipStream_t s, int64_t N, const double *R, const double *P, double *partials) {
+    if (N > 0) k_moments<<<red_blocks(N), kThreads, 0, s>>>(N, R, P, partials);
+}
+
+__global__ __launch_bounds__(256) void k_moments_soa(int64_t N, const double *x, const double *y,
+                                                     const double *z, const double *px,
+                                                     const double *py, const double *pz, double *partials) {
+    __shared__ double lds[4 * 28 * 2];
+    double m[28];
+    for (int k = 0; k < 28; ++k) m[k] = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        accumulate_moments(m, x[i], y[i], z[i], px[i], py[i], pz[i]);
+    store_moment_partials(m, partials, lds);
+}
+void launch_moments_soa(hipStream_t s, const Bunch &b, double *partials) {
+    if (b.N > 0) k_moments_soa<<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, partials);
+}
+
+__global__ __launch_bounds__(256) void k_stats_from_partials(const double *partials, int nb, int64_t N,
+                                                             double mass_eV, StatsOut *out) {
+    __shared__ double lds[4 * 28 * 2];
+    double m[28];
+    reduce_partials_dd<28>(partials, nb, m, lds);
+    if (threadIdx.x != 0) return;
+    StatsOut s;
+    beam_parameters(N, m, m[27], mass_eV, s);
+    s.t = 0.0;
+    s.pathLength = 0.0;
+    s.dropped = 0;
+    *out = s;
+}
+void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, int64_t N,
+                                double mass_eV, void *stats_out) {
+    k_stats_from_partials<<<1, kThreads, 0, s>>>(partials, nb, N, mass_eV, (StatsOut *)stats_out);
+}
+
+// ---- sorting support ------------------------------------------------------------
+__global__ void k_cell_keys(int64_t N, const double *x, const double *y, const double *z,
+                            const Scalars *sc, uint32_t *keys, uint32_t *vals) {
+    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    const uint32_t out = (uint32_t)n0 * n1 * n2;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        int l[3];
+        double d[3];
+        bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
+        ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
+        ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
+        uint32_t key = out;
+        if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
+            key = ((uint32_t)l[2] * n1 + l[1]) * n0 + l[0];
+        keys[p] = key;
+        vals[p] = (uint32_t)p;
+    }
+}
+void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,
+                      const Scalars *sc, uint32_t *keys, uint32_t *vals) {
+    if (N > 0) k_cell_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, sc, keys, vals);
+}
+
+__global__ void k_permute(int64_t N, const uint32_t *perm, const double *src, double *dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[perm[i]];
+}
+__global__ void k_permute_u32(int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[perm[i]];
+}
+void launch_permute(hipStream_t s, int64_t N, const uint32_t *perm, const double *src, double *dst) {
+    if (N > 0) k_permute<<<stream_blocks(N), kThreads, 0, s>>>(N, perm, src, dst);
+}
+void launch_permute_u32(hipStream_t s, int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst) {
+    if (N > 0) k_permute_u32<<<stream_blocks(N), kThreads, 0, s>>>(N, perm, src, dst);
+}
+__global__ void k_iota(int64_t N, uint32_t *v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = (uint32_t)i;
+}
+void launch_iota(hipStream_t s, int64_t N, uint32_t *v) {
+    if (N > 0) k_iota<<<stream_blocks(N), kThreads, 0, s>>>(N, v);
+}
+
+__global__ void k_fill_i64(size_t n, long long *p) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+void launch_fill_i64(hipStream_t s, size_t n, long long *p) {
+    int blocks = (int)std::min<size_t>((n + 1023) / 1024, 16384);
+    k_fill_i64<<<blocks, kThreads, 0, s>>>(n, p);
+}
+
+__global__ void k_absmax(int64_t N, const double *v, unsigned long long *out_bits) {
+    double a = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        a = fmax(a, fabs(v[i]));
+    for (int off = 32; off > 0; off >>= 1) a = fmax(a, __shfl_xor(a, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, (unsigned long long)__double_as_longlong(a));
+}
+void launch_absmax(hipStream_t s, int64_t N, const double *v, unsigned long long *out_bits) {
+    if (N > 0) k_absmax<<<red_blocks(N), kThreads, 0, s>>>(N, v, out_bits);
+}
+
+// ---- synthetic bunch ---------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ inline double uniform01(uint64_t seed, uint64_t i, int s) {
+    uint64_t h = splitmix64(seed ^ splitmix64(i * 16 + (uint64_t)s));
+    return ((double)(h >> 11) + 0.5) * 0x1p-53;
+}
+
+__global__ void k_generate(int64_t N, uint64_t seed, double sx, double sy, double sz, double spx,
+                           double spy, double spz, double p0, double qeach, double halo_frac,
+                           double halo_radius, double *x, double *y, double *z, double *px,
+                           double *py, double *pz, double *q, uint32_t *id) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double nrm[6];
+        for (int k = 0; k < 3; ++k) {
+            double u1 = uniform01(seed, i, 2 * k), u2 = uniform01(seed, i, 2 * k + 1);
+            double r = sqrt(-2.0 * log(u1)), th = 2.0 * kPi * u2;
+            nrm[2 * k] = r * cos(th);
+            nrm[2 * k + 1] = r * sin(th);
+        }
+        double X = sx * nrm[0], Y = sy * nrm[1], Z = sz * nrm[2];
+        if (halo_frac > 0.0 && uniform01(seed, i, 6) < halo_frac) {
+            double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            double rad = halo_radius * cbrt(uniform01(seed, i, 7)) / len;
+            X = sx * nrm[0] * rad;
+            Y = sy * nrm[1] * rad;
+            Z = sz * nrm[2] * rad;
+        }
+        x[i] = X;
+        y[i] = Y;
+        z[i] = Z;
+        px[i] = spx * nrm[3];
+        py[i] = spy * nrm[4];
+        pz[i] = p0 + spz * nrm[5];
+        q[i] = qeach;
+        id[i] = (uint32_t)i;
+    }
+}
+void launch_generate(hipStream_t s, int64_t N, uint64_t seed, const double sr[3], const double sp[3],
+                     double p0, double qeach, double halo_frac, double halo_radius, Bunch &b) {
+    if (N > 0)
+        k_generate<<<stream_blocks(N), kThreads, 0, s>>>(N, seed, sr[0], sr[1], sr[2], sp[0], sp[1], sp[2], p0,
+                                                         qeach, halo_frac, halo_radius, b.x, b.y, b.z, b.px,
+                                                         b.py, b.pz, b.q, b.id);
+}
+
+__global__ void k_cr_test(int which, int64_t n, const double *x, double *y) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = which == 0 ? crm::atan_cr(x[i]) : crm::log_cr(x[i]);
+}
+void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y) {
+    if (n > 0) k_cr_test<<<stream_blocks(n), kThreads, 0, s>>>(which, n, x, y);
+}
+
+// ---- tracker ----------------------------------------------------------------------
+// Step-start scalars: beam-frame trafos from the bunch's mean momentum
+// (ParallelTTracker::computeSpaceChargeFields, ParallelTTracker.cpp:549-559),
+// external fields at t + dT/2 (ParallelTTracker.cpp:630), then t += dT and the
+// reference-particle update whose trafo transformBunch applies at step end
+// (ParallelTTracker.cpp:362-367,1199-1213).
+__global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    TrackState s = *ts;
+    sc->dT = s.dT;
+    sc->coupling = s.coupling;
+    sc->N = N;
+    if (s.space_charge && N > 0) {
+        const double pmean[3] = {sc->psum[0] / (double)N, sc->psum[1] / (double)N, sc->psum[2] / (double)N};
+        double qa[4];
+        quat_to_z(pmean, qa);
+        const double qb[4] = {qa[0], -qa[1], -qa[2], -qa[3]};
+        const double o[3] = {0.0, 0.0, s.pathLength};
+        Trafo b2r = trafo_make(o, qb);
+        Trafo r2b = trafo_inverted(b2r);
+        for (int k = 0; k < 9; ++k) {
+            sc->M_br[k] = b2r.M[k];
+            sc->M_rb[k] = r2b.M[k];
+        }
+        for (int d = 0; d < 3; ++d) {
+            sc->o_br[d] = b2r.origin[d];
+            sc->o_rb[d] = r2b.origin[d];
+        }
+    }
+    ext_fields(s, s.t + 0.5 * s.dT, sc->Eext, sc->Bext);
+    s.t += s.dT;
+    Trafo upd;
+    if (s.t > 0.0) {
+        upd = update_reference(s);
+    } else {
+        const double o[3] = {0.0, 0.0, 0.0}, q[4] = {1.0, 0.0, 0.0, 0.0};
+        upd = trafo_make(o, q);
+    }
+    for (int k = 0; k < 9; ++k) sc->M_u[k] = upd.M[k];
+    for (int d = 0; d < 3; ++d) sc->o_u[d] = upd.origin[d];
+    s.step += 1;
+    *ts = s;
+}
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N) {
+    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N);
+}
+
+// push (timeIntegration1) + rotation into the beam frame + bounds (boundp's get_bounds).
+__global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *x, double *y, double *z,
+                                                     const double *px, const double *py,
+                                                     const double *pz, const Scalars *sc,
+                                                     const TrackState *ts, double *partials) {
+    __shared__ double lds[4 * 6];
+    const double dT = ts->dT;
+    const bool sc_on = ts->space_charge != 0;
+    double M[9], o[3];
+    for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
+    for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double r[3] = {x[i], y[i], z[i]};
+        push_half(r[0], r[1], r[2], px[i], py[i], pz[i], dT);
+        if (sc_on) transform_to(M, o, r);
+        x[i] = r[0];
+        y[i] = r[1];
+        z[i] = r[2];
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], r[d]);
+            mx[d] = fmax(mx[d], r[d]);
+        }
+    }
+    block_minmax(mn, mx, lds);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            partials[blockIdx.x * kSlots + d] = mn[d];
+            partials[blockIdx.x * kSlots + 3 + d] = mx[d];
+        }
+}
+void launch_track_alpha(hipStream_t s, Bunch &b, const Scalars *sc, const TrackState *ts,
+                        double *partials) {
+    if (b.N > 0)
+        k_track_alpha<<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts, partials);
+}
+
+__global__ __launch_bounds__(256) void k_track_mesh(const double *partials, int nb, int n0, int n1,
+                                                    int n2, double dh, const TrackState *ts, Scalars *sc) {
+    __shared__ double lds[4 * 6];
+    double mn[3], mx[3];
+    reduce_partials_minmax(partials, nb, mn, mx, lds);
+    if (threadIdx.x != 0) return;
+    const int n[3] = {n0, n1, n2};
+    for (int d = 0; d < 3; ++d) {
+        sc->n[d] = n[d];
+        sc->rmin[d] = mn[d];
+        sc->rmax[d] = mx[d];
+    }
+    sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin) ? 1 : 0;
+    const int64_t N = sc->N;
+    selffield_scalars(sc, sc->psum[2], N, ts->dT, ts->coupling);
+    // every particle deposits |Q*dT| <= max|Q|*dT*(1+2^-52)
+    set_fixed_point(sc, N, sc->qd_absmax);
+}
+void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
+                       const TrackState *ts, Scalars *sc) {
+    k_track_mesh<<<1, kThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, sc);
+}
+
+// Gather + B from E (PartBunch.cpp:632-644), rotation back to the reference
+// frame (ParallelTTracker.cpp:586-591), external fields, kick + push
+// (timeIntegration2), transformBunch with the reference-frame update, and the
+// moment sums of the step's statistics.
+template <typename T>
+__global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, double *y, double *z,
+                                                     double *px, double *py, double *pz,
+                                                     const typename vec4_of<T>::type *Eg,
+                                                     const Scalars *sc, const TrackState *ts,
+                                                     double *partials) {
+    __shared__ double lds[4 * 28 * 2];
+    const bool sc_on = ts->space_charge != 0;
+    const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
+    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    const double betaC = sc->betaC;
+    double Mb[9], ob[3], Mu[9], ou[3], Ee[3], Be[3];
+    for (int k = 0; k < 9; ++k) {
+        Mb[k] = sc->M_br[k];
+        Mu[k] = sc->M_u[k];
+    }
+    for (int d = 0; d < 3; ++d) {
+        ob[d] = sc->o_br[d];
+        ou[d] = sc->o_u[d];
+        Ee[d] = sc->Eext[d];
+        Be[d] = sc->Bext[d];
+    }
+    double m[28];
+    for (int k = 0; k < 28; ++k) m[k] = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double r[3] = {x[i], y[i], z[i]};
+        double p[3] = {px[i], py[i], pz[i]};
+        double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
+        if (sc_on) {
+            int l[3];
+            double d[3];
+            bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
+            ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
+            ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
+            if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, Ef);
+            Bf[0] = Bf[0] - betaC * Ef[1];
+            Bf[1] = Bf[1] + betaC * Ef[0];
+            transform_to(Mb, ob, r);
+            double t3[3];
+            mat_vec(Mb, Ef, t3);
+            Ef[0] = t3[0];
+            Ef[1] = t3[1];
+            Ef[2] = t3[2];
+            mat_vec(Mb, Bf, t3);
+            Bf[0] = t3[0];
+            Bf[1] = t3[1];
+            Bf[2] = t3[2];
+        }
+        for (int k = 0; k < 3; ++k) {
+            Ef[k] += Ee[k];
+            Bf[k] += Be[k];
+        }
+        boris_kick(p, Ef, Bf, dT, mass, charge);
+        push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
+        transform_to(Mu, ou, r);
+        double pn[3];
+        mat_vec(Mu, p, pn);
+        x[i] = r[0];
+        y[i] = r[1];
+        z[i] = r[2];
+        px[i] = pn[0];
+        py[i] = pn[1];
+        pz[i] = pn[2];
+        accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+    }
+    store_moment_partials(m, partials, lds);
+}
+void launch_track_gamma(hipStream_t s, bool fp64, Bunch &b, const void *Eg, const Scalars *sc,
+                        const TrackState *ts, double *partials) {
+    if (b.N <= 0) return;
+    if (fp64)
+        k_track_gamma<double><<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,
+                                                                   (const double4 *)Eg, sc, ts, partials);
+    else
+        k_track_gamma<float><<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,
+                                                                  (const float4 *)Eg, sc, ts, partials);
+}
+
+// Step statistics (DataSink / calcBeamParameters) and the momentum sums the
+// next step's rotation and Lorentz factor need.
+__global__ __launch_bounds__(256) void k_track_post(const double *partials, int nb, TrackState *ts,
+                                                    Scalars *sc, StatsOut *out,
+                                                    const unsigned long long *dropped) {
+    __shared__ double lds[4 * 28 * 2];
+    double m[28];
+    reduce_partials_dd<28>(partials, nb, m, lds);
+    if (threadIdx.x != 0) return;
+    sc->psum[0] = m[1];
+    sc->psum[1] = m[3];
+    sc->psum[2] = m[5];
+    if (out) {
+        StatsOut s;
+        beam_parameters(sc->N, m, m[27], ts->mass_eV, s);
+        s.t = ts->t;
+        s.pathLength = ts->pathLength;
+        s.dropped = dropped ? (int64_t)*dropped : 0;
+        *out = s;
+    }
+}
+void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
+                       void *stats_out, const unsigned long long *dropped) {
+    k_track_post<<<1, kThreads, 0, s>>>(partials, nb, ts, sc, (StatsOut *)stats_out, dropped);
+}
+void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,
+                               Scalars *sc) {
+    k_track_post<<<1, kThreads, 0, s>>>(partials, nb, ts, sc, nullptr, nullptr);
+}
+
+}  // namespace opalsc

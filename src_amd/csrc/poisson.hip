@@ -1,0 +1,598 @@
+// Hockney open-boundary Poisson solve on the doubled grid, gfx950.
+//
+// Reference: FFTPoissonSolver::computePotential and integratedGreensFunction
+// (src/Solvers/FFTPoissonSolver.cpp:337-382,459-522,641-664).  The reference
+// zero-pads rho to (2n)^3, r2c-transforms rho and the mirrored Green's
+// function, multiplies, and c2r-transforms back: an exact circular
+// convolution.  Here the same convolution is computed with pruned passes that
+// never touch the zero half of the padded input or the unused half of the
+// output (all complex buffers are K0 = n0+1 wide in x):
+//
+//   X-fwd  rho (n0*n1*n2)          -> A[z<n2][y<n1][kx<K0]     r2c along x
+//   Y-fwd  A                       -> B[z<n2][ky<2n1][kx]      along y, input y<n1
+//   Z-conv B (in place)            fwd along z (input z<n2), * Ghat, inverse, keep z<n2
+//   Y-inv  B                       -> A                        keep y<n1
+//   X-inv  A                       -> phi (n0*n1*n2)           c2r, keep x<n0
+//
+// The Green's function is real and even in every axis on the doubled grid, so
+// its transform Ghat is real and even as well: it is the 3D DCT-I of the
+// (n+1)^3 octant, computed here with three DCT passes instead of a (2n)^3 r2c.
+#include "common.hpp"
+#include "crmath.hpp"
+#include "fft.hpp"
+#include "kernels.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace opalsc {
+
+// ---------------------------------------------------------------------------
+// Green's function octant (FP64 in both precisions)
+// ---------------------------------------------------------------------------
+
+template <bool CR>
+__device__ inline double igf_corner(int i, int j, int k, double h0, double h1, double h2,
+                                    double cellVolume) {
+    // FFTPoissonSolver::integratedGreensFunction, FFTPoissonSolver.cpp:472-491
+    double v0 = i * h0 - h0 / 2;
+    double v1 = j * h1 - h1 / 2;
+    double v2 = k * h2 - h2 / 2;
+    double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    double a0, a1, a2, l0, l1, l2;
+    if constexpr (CR) {
+        a0 = crm::atan_cr(v0 * v1 / (v2 * r));
+        a1 = crm::atan_cr(v0 * v2 / (v1 * r));
+        a2 = crm::atan_cr(v1 * v2 / (v0 * r));
+        l0 = crm::log_cr(v0 + r);
+        l1 = crm::log_cr(v1 + r);
+        l2 = crm::log_cr(v2 + r);
+    } else {
+        a0 = atan(v0 * v1 / (v2 * r));
+        a1 = atan(v0 * v2 / (v1 * r));
+        a2 = atan(v1 * v2 / (v0 * r));
+        l0 = log(v0 + r);
+        l1 = log(v1 + r);
+        l2 = log(v2 + r);
+    }
+    double t = -v2 * v2 * a0 / 2;
+    t += -v1 * v1 * a1 / 2;
+    t += -v0 * v0 * a2 / 2;
+    t += v1 * v2 * l0;
+    t += v0 * v2 * l1;
+    t += v0 * v1 * l2;
+    return t / cellVolume;
+}
+
+// G octant [0,n0]x[0,n1]x[0,n2] from corner antiderivatives T on an LDS tile
+// (8x8x4 G values from 9x9x5 corners), 8-corner difference in the reference's
+// order (FFTPoissonSolver.cpp:510-517) and G[0,0,0] := G[0,0,1]
+// (FFTPoissonSolver.cpp:657).  Index n uses T at n+1 from the same formula;
+// the physical potential never reads G at index n.
+template <bool CR>
+__global__ __launch_bounds__(256) void k_igf_octant(double *G, int n0, int n1, int n2,
+                                                    const Scalars *sc) {
+    __shared__ double Tc[5][9][9];
+    const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2];
+    const double vol = h0 * h1 * h2;
+    const int gx0 = blockIdx.x * 8, gy0 = blockIdx.y * 8, gz0 = blockIdx.z * 4;
+    for (int c = threadIdx.x; c < 405; c += 256) {
+        int cx = c % 9, cy = (c / 9) % 9, cz = c / 81;
+        Tc[cz][cy][cx] = igf_corner<CR>(gx0 + cx, gy0 + cy, gz0 + cz, h0, h1, h2, vol);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 7, ty = (threadIdx.x >> 3) & 7, tz = threadIdx.x >> 6;
+    const int i = gx0 + tx, j = gy0 + ty, k = gz0 + tz;
+    if (i > n0 || j > n1 || k > n2) return;
+    int ax = tx, ay = ty, az = tz;
+    if (i == 0 && j == 0 && k == 0) az = 1;  // quirk: G[0,0,0] = G[0,0,1]
+    double g = Tc[az + 1][ay + 1][ax + 1];
+    g += Tc[az][ay][ax + 1];
+    g += Tc[az][ay + 1][ax];
+    g += Tc[az + 1][ay][ax];
+    g -= Tc[az][ay + 1][ax + 1];
+    g -= Tc[az + 1][ay][ax + 1];
+    g -= Tc[az + 1][ay + 1][ax];
+    g -= Tc[az][ay][ax];
+    G[((size_t)k * (n1 + 1) + j) * (n0 + 1) + i] = g;
+}
+
+// STANDARD Green's function, SpecializedGreensFunction<3>::calculate
+// (FFTPoissonSolver.cpp:32-43): G = 1/sqrt(i^2 h0^2 + j^2 h1^2 + k^2 h2^2), G[0] := G[0,0,1].
+__global__ __launch_bounds__(256) void k_green_standard(double *G, int n0, int n1, int n2,
+                                                        const Scalars *sc) {
+    const size_t total = (size_t)(n0 + 1) * (n1 + 1) * (n2 + 1);
+    const double hsq0 = sc->hs[0] * sc->hs[0], hsq1 = sc->hs[1] * sc->hs[1],
+                 hsq2 = sc->hs[2] * sc->hs[2];
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total;
+         q += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(q % (n0 + 1));
+        int j = (int)((q / (n0 + 1)) % (n1 + 1));
+        int k = (int)(q / ((size_t)(n0 + 1) * (n1 + 1)));
+        if (i == 0 && j == 0 && k == 0) k = 1;
+        double g = (double)(i * i) * hsq0 + (double)(j * j) * hsq1 + (double)(k * k) * hsq2;
+        G[q] = 1.0 / sqrt(g);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DCT-I passes (even extension of each octant line, length-2n FFT, real part)
+// ---------------------------------------------------------------------------
+
+template <typename T, int E>
+__global__ __launch_bounds__(256) void k_dct_x(const double *G, T *Gh, FftPlan pl, int n0,
+                                               int nlines) {
+    using C = typename cplx_of<T>::type;
+    extern __shared__ char smem[];
+    C *buf = (C *)smem;
+    const int W = pl.W, P = pl.P, L = pl.L, m = n0 + 1;
+    const int line0 = blockIdx.x * W;
+    for (int e = threadIdx.x; e < W * m; e += kThreads) {
+        int ln = e / m, i = e - ln * m;
+        int line = line0 + ln;
+        T v = line < nlines ? (T)G[(size_t)line * m + i] : T(0);
+        buf[i * P + ln] = C{v, T(0)};
+        if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{v, T(0)};
+    }
+    __syncthreads();
+    fft_lds<C, E>(buf, pl, false);
+    for (int e = threadIdx.x; e < W * m; e += kThreads) {
+        int ln = e / m, k = e - ln * m;
+        int line = line0 + ln;
+        if (line < nlines) Gh[(size_t)line * m + k] = buf[k * P + ln].x;
+    }
+}
+
+// DCT-I along a strided axis of Gh in place: element i of line (x, o) at
+// o*ostride + x + i*S, x in [0, nx), i in [0, npts).
+template <typename T, int E>
+__global__ __launch_bounds__(256) void k_dct_strided(T *Gh, FftPlan pl, int nx, int npts,
+                                                     size_t S, size_t ostride) {
+    using C = typename cplx_of<T>::type;
+    extern __shared__ char smem[];
+    C *buf = (C *)smem;
+    const int W = pl.W, P = pl.P, L = pl.L;
+    const int x0 = blockIdx.x * W;
+    const size_t base = (size_t)blockIdx.y * ostride + x0;
+    const int nn = npts - 1;
+    for (int e = threadIdx.x; e < W * npts; e += kThreads) {
+        int ln = e % W, i = e / W;
+        T v = (x0 + ln < nx) ? Gh[base + ln + i * S] : T(0);
+        buf[i * P + ln] = C{v, T(0)};
+        if (i >= 1 && i <= nn - 1) buf[(L - i) * P + ln] = C{v, T(0)};
+    }
+    __syncthreads();
+    fft_lds<C, E>(buf, pl, false);
+    for (int e = threadIdx.x; e < W * npts; e += kThreads) {
+        int ln = e % W, k = e / W;
+        if (x0 + ln < nx) Gh[base + ln + k * S] = buf[k * P + ln].x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Convolution passes
+// ---------------------------------------------------------------------------
+
+// SRC 0: int64 fixed-point charge (value = i*q2e, then /dT, *tmp2:
+//        PartBunch.cpp:494-510); SRC 1: double rho as given.
+template <typename T, int E, int SRC>
+__global__ __launch_bounds__(256) void k_xfwd(const void *src, typename cplx_of<T>::type *A,
+                                              FftPlan pl, int n0, int nlines, const Scalars *sc) {
+    using C = typename cplx_of<T>::type;
+    extern __shared__ char smem[];
+    C *buf = (C *)smem;
+    const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1;
+    const int line0 = blockIdx.x * W;
+    double q2e = 1.0, dT = 1.0, tmp2 = 1.0;
+    if constexpr (SRC == 0) {
+        q2e = sc->q2e;
+        dT = sc->dT;
+        tmp2 = sc->tmp2;
+    }
+    for (int e = threadIdx.x; e < W * n0; e += kThreads) {
+        int ln = e / n0, x = e - ln * n0;
+        int line = line0 + ln;
+        T v = T(0);
+        if (line < nlines) {
+            double d;
+            if constexpr (SRC == 0) {
+                d = (double)((const long long *)src)[(size_t)line * n0 + x] * q2e;
+                d = d / dT;
+                d = d * tmp2;
+            } else {
+                d = ((const double *)src)[(size_t)line * n0 + x];
+            }
+            v = (T)d;
+        }
+        buf[x * P + ln] = C{v, T(0)};
+    }
+    for (int e = threadIdx.x; e < W * (L - n0); e += kThreads) {
+        int ln = e % W, x = n0 + e / W;
+        buf[x * P + ln] = C{T(0), T(0)};
+    }
+    __syncthreads();
+    fft_lds<C, E>(buf, pl, false);
+    for (int e = threadIdx.x; e < W * K0; e += kThreads) {
+        int ln = e / K0, k = e - ln * K0;
+        int line = line0 + ln;
+        if (line < nlines) A[(size_t)line * K0 + k] = buf[k * P + ln];
+    }
+}
+
+// Strided pass over one tile of W consecutive kx at fixed outer index:
+// element i of line kx at in[obase + i*inS + kx], i < nin (zero beyond), the
+// transformed line written to out[obase_out + i*outS + kx] for i < nout.
+// MODE 0: forward; MODE 1: inverse; MODE 2: forward, *Ghat, inverse (z-conv).
+template <typename T, int E, int MODE>
+__global__ __launch_bounds__(256) void k_strided(const typename cplx_of<T>::type *in,
+                                                 typename cplx_of<T>::type *out, FftPlan pl,
+                                                 int K0, int nin, int nout, size_t inS,
+                                                 size_t inO, size_t outS, size_t outO,
+                                                 const T *Gh, int n0, int n1, int n2, int L1) {
+    using C = typename cplx_of<T>::type;
+    extern __shared__ char smem[];
+    C *buf = (C *)smem;
+    const int W = pl.W, P = pl.P, L = pl.L;
+    const int kx0 = blockIdx.x * W;
+    const int o = blockIdx.y;
+    const size_t ib = (size_t)o * inO + kx0, ob = (size_t)o * outO + kx0;
+    for (int e = threadIdx.x; e < W * nin; e += kThreads) {
+        int ln = e % W, i = e / W;
+        C v = C{T(0), T(0)};
+        if (kx0 + ln < K0) v = in[ib + i * inS + ln];
+        buf[i * P + ln] = v;
+    }
+    for (int e = threadIdx.x; e < W * (L - nin); e += kThreads) {
+        int ln = e % W, i = nin + e / W;
+        buf[i * P + ln] = C{T(0), T(0)};
+    }
+    __syncthreads();
+    fft_lds<C, E>(buf, pl, MODE == 1);
+    if constexpr (MODE == 2) {
+        // o is ky; Ghat folded into the octant (Ghat[2n-k] = Ghat[k])
+        const int fy = o <= n1 ? o : L1 - o;
+        for (int e = threadIdx.x; e < W * L; e += kThreads) {
+            int ln = e % W, kz = e / W;
+            int fz = kz <= n2 ? kz : L - kz;
+            int kx = kx0 + ln;
+            if (kx < K0) {
+                T g = Gh[((size_t)fz * (n1 + 1) + fy) * (n0 + 1) + kx];
+                C v = buf[kz * P + ln];
+                buf[kz * P + ln] = C{v.x * g, v.y * g};
+            }
+        }
+        __syncthreads();
+        fft_lds<C, E>(buf, pl, true);
+    }
+    for (int e = threadIdx.x; e < W * nout; e += kThreads) {
+        int ln = e % W, i = e / W;
+        if (kx0 + ln < K0) out[ob + i * outS + ln] = buf[i * P + ln];
+    }
+}
+
+// c2r along x, keep x < n0, phi = ((raw*invM)*s1)*s2.  OUTSCALE 1: s1 = vol,
+// s2 = coupling (PartBunch.cpp:542-554); OUTSCALE 0: s1 = s2 = 1.
+template <typename T, int E, int OUTSCALE>
+__global__ __launch_bounds__(256) void k_xinv(const typename cplx_of<T>::type *A, T *phi,
+                                              FftPlan pl, int n0, int nlines, double invM,
+                                              const Scalars *sc) {
+    using C = typename cplx_of<T>::type;
+    extern __shared__ char smem[];
+    C *buf = (C *)smem;
+    const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1;
+    const int line0 = blockIdx.x * W;
+    for (int e = threadIdx.x; e < W * K0; e += kThreads) {
+        int ln = e / K0, k = e - ln * K0;
+        int line = line0 + ln;
+        C v = C{T(0), T(0)};
+        if (line < nlines) v = A[(size_t)line * K0 + k];
+        buf[k * P + ln] = v;
+        if (k >= 1 && k <= n0 - 1) buf[(L - k) * P + ln] = C{v.x, -v.y};
+    }
+    __syncthreads();
+    fft_lds<C, E>(buf, pl, true);
+    double s1 = 1.0, s2 = 1.0;
+    if constexpr (OUTSCALE == 1) {
+        s1 = sc->vol;
+        s2 = sc->coupling;
+    }
+    for (int e = threadIdx.x; e < W * n0; e += kThreads) {
+        int ln = e / n0, x = e - ln * n0;
+        int line = line0 + ln;
+        if (line < nlines) {
+            if constexpr (sizeof(T) == 8) {
+                double d = buf[x * P + ln].x * invM;
+                d = d * s1;
+                d = d * s2;
+                phi[(size_t)line * n0 + x] = d;
+            } else {
+                float d = buf[x * P + ln].x * (float)invM;
+                d = d * (float)s1;
+                d = d * (float)s2;
+                phi[(size_t)line * n0 + x] = d;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E = -Grad(phi) * (gz, gz, 1/gz)   (PartBunch.cpp:573-575; Grad Cell->Cell,
+// ippl/src/Meshes/UniformCartesian.hpp:2127-2200; ZeroFace guards)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>::type *Eg, int n0,
+                                              int n1, int n2, const Scalars *sc) {
+    const size_t total = (size_t)n0 * n1 * n2;
+    const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
+    const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
+    const size_t sy = n0, sz = (size_t)n0 * n1;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total;
+         c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % n0);
+        const int j = (int)((c / n0) % n1);
+        const int k = (int)(c / sz);
+        const T *p = phi + c;
+        T r0, r1, r2;
+        if (i == 0) r0 = id0 * ((T(-1.0) * p[2] + T(4.0) * p[1]) - T(3.0) * p[0]);
+        else if (i == n0 - 1) r0 = id0 * ((T(1.0) * p[-2] - T(4.0) * p[-1]) + T(3.0) * p[0]);
+        else r0 = id0 * (p[1] - p[-1]);
+        if (j == 0) r1 = id1 * ((T(-1.0) * p[2 * sy] + T(4.0) * p[sy]) - T(3.0) * p[0]);
+        else if (j == n1 - 1) r1 = id1 * ((T(1.0) * p[-2 * (ptrdiff_t)sy] - T(4.0) * p[-(ptrdiff_t)sy]) + T(3.0) * p[0]);
+        else r1 = id1 * (p[sy] - p[-(ptrdiff_t)sy]);
+        if (k == 0) r2 = id2 * ((T(-1.0) * p[2 * sz] + T(4.0) * p[sz]) - T(3.0) * p[0]);
+        else if (k == n2 - 1) r2 = id2 * ((T(1.0) * p[-2 * (ptrdiff_t)sz] - T(4.0) * p[-(ptrdiff_t)sz]) + T(3.0) * p[0]);
+        else r2 = id2 * (p[sz] - p[-(ptrdiff_t)sz]);
+        typename vec4_of<T>::type e;
+        e.x = -r0 * g0;
+        e.y = -r1 * g1;
+        e.z = -r2 * g2;
+        e.w = T(0);
+        Eg[c] = e;
+    }
+}
+
+// rho entering the solver, for diagnostics
+__global__ void k_rho_to_double(const long long *rho, double *out, size_t n, const Scalars *sc) {
+    const double q2e = sc->q2e, dT = sc->dT, tmp2 = sc->tmp2;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x) {
+        double d = (double)rho[c] * q2e;
+        d = d / dT;
+        d = d * tmp2;
+        out[c] = d;
+    }
+}
+
+template <typename T>
+__global__ void k_widen(const T *in, double *out, size_t n, int comps, int stride) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x)
+        for (int d = 0; d < comps; ++d) out[c * comps + d] = (double)in[c * stride + d];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static bool make_plan(int L, bool fp64, FftPlan &pl) {
+    int a = 0, b = 0, m = L;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    while (m % 3 == 0) { m /= 3; ++b; }
+    if (m != 1 || b > 1 || a < 2) return false;
+    pl.L = L;
+    if (b == 0) pl.E = fp64 ? 8 : 16;
+    else pl.E = fp64 ? 12 : 24;
+    if ((pl.E * kThreads) % L != 0) return false;
+    pl.W = pl.E * kThreads / L;
+    if (pl.W < 1) return false;
+    pl.P = pl.W + 1;
+    pl.nst = 0;
+    const bool use8 = (pl.E % 8) == 0;
+    int rem = a;
+    if (b) pl.radix[pl.nst++] = 3;
+    while (rem >= 3 && use8) { pl.radix[pl.nst++] = 8; rem -= 3; }
+    while (rem >= 2) { pl.radix[pl.nst++] = 4; rem -= 2; }
+    if (rem == 1) pl.radix[pl.nst++] = 2;
+    return true;
+}
+
+template <typename T>
+static void make_twiddles(int L, std::vector<char> &out) {
+    using C = typename cplx_of<T>::type;
+    out.resize(sizeof(C) * L);
+    C *p = (C *)out.data();
+    for (int m = 0; m < L; ++m) {
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)L;
+        p[m].x = (T)cosl(ang);
+        p[m].y = (T)sinl(ang);
+    }
+}
+
+static size_t lds_bytes(const FftPlan &pl, size_t csize) { return (size_t)pl.L * pl.P * csize; }
+
+#define OPALSC_DISPATCH_E(E_, ...)                             \
+    switch (E_) {                                              \
+    case 8: { constexpr int EV = 8; __VA_ARGS__; } break;      \
+    case 12: { constexpr int EV = 12; __VA_ARGS__; } break;    \
+    case 16: { constexpr int EV = 16; __VA_ARGS__; } break;    \
+    case 24: { constexpr int EV = 24; __VA_ARGS__; } break;    \
+    default: break;                                            \
+    }
+
+int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, std::string &err) {
+    stream = s;
+    fp64 = fp64_;
+    greens = greens_;
+    for (int d = 0; d < 3; ++d) {
+        n[d] = nn[d];
+        L[d] = 2 * nn[d];
+        if (nn[d] < 4) { err = "n must be >= 4 in every dimension"; return -1; }
+        if (!make_plan(L[d], fp64, plan[d])) {
+            err = "unsupported grid size n=" + std::to_string(nn[d]) + " (need n = 2^a or 3*2^a, a>=1)";
+            return -1;
+        }
+    }
+    K0 = n[0] + 1;
+    const size_t cs = fp64 ? 16 : 8, rs = fp64 ? 8 : 4;
+    ncell = (size_t)n[0] * n[1] * n[2];
+    noct = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    const size_t szA = (size_t)K0 * n[1] * n[2] * cs;
+    const size_t szB = (size_t)K0 * L[1] * n[2] * cs;
+    auto alloc = [&](void **p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes) != hipSuccess) { err = "hipMalloc failed (" + std::to_string(bytes) + " bytes)"; return false; }
+        return true;
+    };
+    if (!alloc(&rho_i64, ncell * 8) || !alloc(&rho_f64, ncell * 8) || !alloc(&A, szA) ||
+        !alloc(&B, szB) || !alloc(&Goct, noct * 8) || !alloc(&Ghat, noct * rs) ||
+        !alloc(&phi, ncell * rs) || !alloc(&Eg, ncell * 4 * rs))
+        return -1;
+    for (int d = 0; d < 3; ++d) {
+        std::vector<char> tw;
+        if (fp64) make_twiddles<double>(L[d], tw);
+        else make_twiddles<float>(L[d], tw);
+        if (!alloc(&twd[d], tw.size())) return -1;
+        if (hipMemcpy(twd[d], tw.data(), tw.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "twiddle upload failed";
+            return -1;
+        }
+        plan[d].tw = twd[d];
+    }
+    return 0;
+}
+
+void PoissonGrid::destroy() {
+    void *ps[] = {rho_i64, rho_f64, A, B, Goct, Ghat, phi, Eg, twd[0], twd[1], twd[2]};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    rho_i64 = rho_f64 = A = B = Goct = Ghat = phi = Eg = nullptr;
+    twd[0] = twd[1] = twd[2] = nullptr;
+}
+
+template <typename T>
+static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    if (g.greens == 1) {
+        k_green_standard<<<1024, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
+    } else {
+        dim3 grid((n0 + 1 + 7) / 8, (n1 + 1 + 7) / 8, (n2 + 1 + 3) / 4);
+        if (cr) k_igf_octant<true><<<grid, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
+        else k_igf_octant<false><<<grid, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
+    }
+    g.profile_mark(PROF_GREEN);
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C);
+    {
+        const FftPlan &pl = g.plan[0];
+        int nlines = (n1 + 1) * (n2 + 1);
+        dim3 grid((nlines + pl.W - 1) / pl.W);
+        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                    (const double *)g.Goct, (T *)g.Ghat, pl, n0, nlines));
+    }
+    {
+        const FftPlan &pl = g.plan[1];
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1);
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                    (T *)g.Ghat, pl, n0 + 1, n1 + 1, (size_t)(n0 + 1),
+                                    (size_t)(n0 + 1) * (n1 + 1)));
+    }
+    {
+        const FftPlan &pl = g.plan[2];
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                    (T *)g.Ghat, pl, n0 + 1, n2 + 1,
+                                    (size_t)(n0 + 1) * (n1 + 1), (size_t)(n0 + 1)));
+    }
+    g.profile_mark(PROF_DCT);
+}
+
+void PoissonGrid::green(const Scalars *d_sc, bool cr) {
+    if (fp64) green_impl<double>(*this, d_sc, cr);
+    else green_impl<float>(*this, d_sc, cr);
+}
+
+template <typename T>
+static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int outscale) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C);
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], K0 = g.K0;
+    const int L1 = g.L[1], L2 = g.L[2];
+    C *A = (C *)g.A, *B = (C *)g.B;
+    const int nlines = n1 * n2;
+    {
+        const FftPlan &pl = g.plan[0];
+        dim3 grid((nlines + pl.W - 1) / pl.W);
+        if (src_kind == 0)
+            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 0><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                        g.rho_i64, A, pl, n0, nlines, d_sc))
+        else
+            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 1><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                        g.rho_f64, A, pl, n0, nlines, d_sc))
+    }
+    g.profile_mark(PROF_FFT_X);
+    {
+        const FftPlan &pl = g.plan[1];
+        dim3 grid((K0 + pl.W - 1) / pl.W, n2);
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 0><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                    A, B, pl, K0, n1, L1, (size_t)K0, (size_t)K0 * n1, (size_t)K0,
+                                    (size_t)K0 * L1, nullptr, n0, n1, n2, L1));
+    }
+    g.profile_mark(PROF_FFT_Y);
+    {
+        const FftPlan &pl = g.plan[2];
+        dim3 grid((K0 + pl.W - 1) / pl.W, L1);
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                    B, B, pl, K0, n2, n2, (size_t)K0 * L1, (size_t)K0,
+                                    (size_t)K0 * L1, (size_t)K0, (const T *)g.Ghat, n0, n1, n2, L1));
+    }
+    g.profile_mark(PROF_FFT_Z);
+    {
+        const FftPlan &pl = g.plan[1];
+        dim3 grid((K0 + pl.W - 1) / pl.W, n2);
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 1><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                    B, A, pl, K0, L1, n1, (size_t)K0, (size_t)K0 * L1, (size_t)K0,
+                                    (size_t)K0 * n1, nullptr, n0, n1, n2, L1));
+    }
+    g.profile_mark(PROF_FFT_Y);
+    {
+        const FftPlan &pl = g.plan[0];
+        dim3 grid((nlines + pl.W - 1) / pl.W);
+        const double invM = 1.0 / ((double)(2 * n0) * (double)(2 * n1) * (double)(2 * n2));
+        if (outscale)
+            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                        A, (T *)g.phi, pl, n0, nlines, invM, d_sc))
+        else
+            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 0><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
+                                        A, (T *)g.phi, pl, n0, nlines, invM, d_sc))
+    }
+    g.profile_mark(PROF_FFT_X);
+    (void)L2;
+}
+
+void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale) {
+    if (fp64) solve_impl<double>(*this, d_sc, src_kind, outscale);
+    else solve_impl<float>(*this, d_sc, src_kind, outscale);
+}
+
+void PoissonGrid::grad(const Scalars *d_sc) {
+    const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
+    if (fp64) k_grad<double><<<blocks, 256, 0, stream>>>((const double *)phi, (double4 *)Eg, n[0], n[1], n[2], d_sc);
+    else k_grad<float><<<blocks, 256, 0, stream>>>((const float *)phi, (float4 *)Eg, n[0], n[1], n[2], d_sc);
+    profile_mark(PROF_GRAD);
+}
+
+void PoissonGrid::rho_to_double(const Scalars *d_sc, double *out) {
+    k_rho_to_double<<<1024, 256, 0, stream>>>((const long long *)rho_i64, out, ncell, d_sc);
+}
+
+void PoissonGrid::widen(int which, double *out) {
+    if (which == 1) {
+        if (fp64) (void)hipMemcpyAsync(out, phi, ncell * 8, hipMemcpyDeviceToDevice, stream);
+        else k_widen<float><<<1024, 256, 0, stream>>>((const float *)phi, out, ncell, 1, 1);
+    } else {
+        if (fp64) k_widen<double><<<1024, 256, 0, stream>>>((const double *)Eg, out, ncell, 3, 4);
+        else k_widen<float><<<1024, 256, 0, stream>>>((const float *)Eg, out, ncell, 3, 4);
+    }
+}
+
+}  // namespace opalsc

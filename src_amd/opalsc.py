@@ -1,0 +1,348 @@
+"""Python host mirror of the space-charge step's C ABI (include/opalsc.h).
+
+Names follow the reference interfaces they stand in for:
+  FFTPoissonSolver.computePotential    src/Solvers/FFTPoissonSolver.cpp:337-382
+  SpaceChargeContext.boundp            src/Classic/Algorithms/PartBunchBase.hpp:802-883
+  SpaceChargeContext.computeSelfFields src/Classic/Algorithms/PartBunch.cpp:482-648
+  SpaceChargeContext.pushParticles     src/Algorithms/ParallelTTracker.h:448-455
+  SpaceChargeContext.kickParticles     src/Algorithms/ParallelTTracker.h:442-446
+  SpaceChargeContext.calcBeamParameters src/Classic/Algorithms/PartBunchBase.hpp:1529-1613
+  Tracker.track                         src/Algorithms/ParallelTTracker.cpp:343-388
+
+All work runs in the HIP library libopalsc.so; importing this module raises
+if the library is missing, and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("OPALSC_LIB", os.path.join(_HERE, "libopalsc.so"))
+
+FP64, MIXED = 0, 1
+GREENS_INTEGRATED, GREENS_STANDARD = 0, 1
+GRID_RHO, GRID_PHI, GRID_EFIELD = 0, 1, 2
+
+CLIGHT = 299792458.0                       # src/Classic/Physics/Physics.h:52
+EPSILON_0 = 8.854187817e-12                # src/Classic/Physics/Physics.h:58
+PI = 3.14159265358979323846
+COUPLING = 1.0 / (4 * PI * EPSILON_0)      # src/Track/TrackRun.cpp:542
+M_P_EV = 0.93827204e+00 * 1e9              # src/Classic/Physics/Physics.h:88 (GeV -> eV)
+
+EXPORTS = [
+    "opalsc_abi_version", "opalsc_create", "opalsc_destroy", "opalsc_last_error",
+    "opalsc_synchronize", "opalsc_compute_potential", "opalsc_boundp", "opalsc_cic_index",
+    "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_get_grid", "opalsc_push_half",
+    "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
+    "opalsc_bunch_get", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
+    "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
+    "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
+]
+
+
+class Config(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("greens", C.c_int), ("precision", C.c_int),
+                ("bboxincr", C.c_double), ("max_particles", C.c_int64), ("device", C.c_int),
+                ("sort_interval", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(name, C.c_double * 3) for name in
+                ("rmean", "pmean", "rrms", "prms", "eps_norm", "eps", "rprms")] + [
+        ("gamma_mean", C.c_double), ("ekin_MeV", C.c_double), ("t", C.c_double),
+        ("pathLength", C.c_double), ("n", C.c_int64), ("dropped", C.c_int64)]
+
+
+class TrackParams(C.Structure):
+    _fields_ = [("dT", C.c_double), ("charge", C.c_double), ("mass_eV", C.c_double),
+                ("coupling", C.c_double), ("Bz", C.c_double), ("Ez0", C.c_double),
+                ("rf_freq", C.c_double), ("rf_phase", C.c_double), ("t", C.c_double),
+                ("pathLength", C.c_double), ("RefPartR", C.c_double * 3),
+                ("RefPartP", C.c_double * 3), ("space_charge", C.c_int)]
+
+
+class GaussParams(C.Structure):
+    _fields_ = [("n", C.c_int64), ("seed", C.c_uint64), ("sigma_r", C.c_double * 3),
+                ("sigma_p", C.c_double * 3), ("p0", C.c_double), ("q_total", C.c_double),
+                ("halo_frac", C.c_double), ("halo_radius", C.c_double)]
+
+
+class OpalscError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libopalsc.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libopalsc.so not found at %s: build it with "
+                      "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i64, dp = C.c_void_p, C.c_int64, C.c_void_p
+    sig = {
+        "opalsc_abi_version": ([], C.c_int),
+        "opalsc_create": ([C.POINTER(vp), C.POINTER(Config)], C.c_int),
+        "opalsc_destroy": ([vp], C.c_int),
+        "opalsc_last_error": ([vp], C.c_char_p),
+        "opalsc_synchronize": ([vp], C.c_int),
+        "opalsc_compute_potential": ([vp, dp, dp], C.c_int),
+        "opalsc_boundp": ([vp, i64, dp, dp, dp], C.c_int),
+        "opalsc_cic_index": ([vp, i64, dp, dp, dp, dp, dp], C.c_int),
+        "opalsc_sort_by_cell": ([vp, i64, dp, dp, dp, dp], C.c_int),
+        "opalsc_compute_self_fields": ([vp, i64, dp, dp, dp, dp, C.c_double, dp, dp, C.c_double,
+                                        dp, dp], C.c_int),
+        "opalsc_get_grid": ([vp, C.c_int, dp], C.c_int),
+        "opalsc_push_half": ([vp, i64, dp, dp, dp], C.c_int),
+        "opalsc_kick": ([vp, i64, dp, dp, dp, dp, C.c_double, C.c_double], C.c_int),
+        "opalsc_beam_parameters": ([vp, i64, dp, dp, C.c_double, C.POINTER(Stats)], C.c_int),
+        "opalsc_bunch_set": ([vp, i64, dp, dp, dp], C.c_int),
+        "opalsc_bunch_generate": ([vp, C.POINTER(GaussParams)], C.c_int),
+        "opalsc_bunch_get": ([vp, dp, dp, dp], C.c_int),
+        "opalsc_track_init": ([vp, C.POINTER(TrackParams)], C.c_int),
+        "opalsc_track": ([vp, C.c_int, dp], C.c_int),
+        "opalsc_track_get_params": ([vp, C.POINTER(TrackParams)], C.c_int),
+        "opalsc_get_mesh": ([vp, dp, dp], C.c_int),
+        "opalsc_host_atan_cr": ([i64, dp, dp], C.c_int),
+        "opalsc_host_log_cr": ([i64, dp, dp], C.c_int),
+        "opalsc_device_atan_cr": ([vp, i64, dp, dp], C.c_int),
+        "opalsc_device_log_cr": ([vp, i64, dp, dp], C.c_int),
+        "opalsc_profile_enable": ([vp, C.c_int], C.c_int),
+        "opalsc_profile_read": ([vp, C.c_int, C.POINTER(C.c_char_p), dp, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _d(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def stats_to_dict(s):
+    d = {k: np.array(getattr(s, k)[:]) for k in
+         ("rmean", "pmean", "rrms", "prms", "eps_norm", "eps", "rprms")}
+    d.update(gamma_mean=s.gamma_mean, ekin_MeV=s.ekin_MeV, t=s.t, pathLength=s.pathLength,
+             n=s.n, dropped=s.dropped)
+    return d
+
+
+def host_atan_cr(x):
+    x = _d(x).ravel()
+    y = np.empty_like(x)
+    _check(None, lib().opalsc_host_atan_cr(len(x), _ptr(x), _ptr(y)))
+    return y
+
+
+def host_log_cr(x):
+    x = _d(x).ravel()
+    y = np.empty_like(x)
+    _check(None, lib().opalsc_host_log_cr(len(x), _ptr(x), _ptr(y)))
+    return y
+
+
+def _check(ctx, rc):
+    if rc != 0:
+        msg = lib().opalsc_last_error(ctx)
+        raise OpalscError("opalsc error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+class SpaceChargeContext:
+    """One opalsc context: a FIELDSOLVER of n cells per dimension on one GPU."""
+
+    def __init__(self, n, precision=FP64, greens=GREENS_INTEGRATED, bboxincr=2.0,
+                 max_particles=1 << 20, device=0, sort_interval=0):
+        cfg = Config()
+        cfg.n[:] = [int(v) for v in n]
+        cfg.greens, cfg.precision, cfg.bboxincr = greens, precision, bboxincr
+        cfg.max_particles, cfg.device, cfg.sort_interval = int(max_particles), device, sort_interval
+        self.n = tuple(int(v) for v in n)
+        self.precision = precision
+        h = C.c_void_p()
+        rc = lib().opalsc_create(C.byref(h), C.byref(cfg))
+        if rc != 0:
+            _check(None, rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().opalsc_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, rc):
+        _check(self._h, rc)
+
+    @property
+    def ncell(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+    # FFTPoissonSolver::computePotential
+    def computePotential(self, rho, hr):
+        phi = _d(rho).ravel().copy()
+        assert phi.size == self.ncell
+        self._c(lib().opalsc_compute_potential(self._h, _ptr(phi), _ptr(_d(hr))))
+        return phi
+
+    def boundp(self, R):
+        R = _d(R, (-1, 3))
+        hr, origin = np.zeros(3), np.zeros(3)
+        self._c(lib().opalsc_boundp(self._h, len(R), _ptr(R), _ptr(hr), _ptr(origin)))
+        return hr, origin
+
+    def cic_index(self, R, origin, hr):
+        R = _d(R, (-1, 3))
+        lgp = np.zeros((len(R), 3), np.int32)
+        dpos = np.zeros((len(R), 3))
+        self._c(lib().opalsc_cic_index(self._h, len(R), _ptr(R), _ptr(_d(origin)), _ptr(_d(hr)),
+                                       _ptr(lgp), _ptr(dpos)))
+        return lgp, dpos
+
+    def sort_by_cell(self, R, origin, hr):
+        R = _d(R, (-1, 3))
+        perm = np.zeros(len(R), np.uint32)
+        self._c(lib().opalsc_sort_by_cell(self._h, len(R), _ptr(R), _ptr(_d(origin)),
+                                          _ptr(_d(hr)), _ptr(perm)))
+        return perm
+
+    # PartBunch::computeSelfFields
+    def computeSelfFields(self, R, P, Q, dt, dT, origin, hr, coupling=COUPLING, Bf=None):
+        R, P = _d(R, (-1, 3)), _d(P, (-1, 3))
+        N = len(R)
+        Q = _d(Q).copy()
+        dt = _d(dt)
+        Ef = np.zeros((N, 3))
+        Bf = np.zeros((N, 3)) if Bf is None else _d(Bf, (-1, 3)).copy()
+        self._c(lib().opalsc_compute_self_fields(self._h, N, _ptr(R), _ptr(P), _ptr(Q), _ptr(dt),
+                                                 dT, _ptr(_d(origin)), _ptr(_d(hr)), coupling,
+                                                 _ptr(Ef), _ptr(Bf)))
+        return Ef, Bf, Q
+
+    def get_grid(self, which):
+        out = np.zeros(self.ncell * (3 if which == GRID_EFIELD else 1))
+        self._c(lib().opalsc_get_grid(self._h, which, _ptr(out)))
+        return out.reshape(-1, 3) if which == GRID_EFIELD else out
+
+    def pushParticles(self, R, P, dt):
+        R = _d(R, (-1, 3)).copy()
+        self._c(lib().opalsc_push_half(self._h, len(R), _ptr(R), _ptr(_d(P, (-1, 3))), _ptr(_d(dt))))
+        return R
+
+    def kickParticles(self, P, Ef, Bf, dt, charge=1.0, mass_eV=M_P_EV):
+        P = _d(P, (-1, 3)).copy()
+        self._c(lib().opalsc_kick(self._h, len(P), _ptr(P), _ptr(_d(Ef, (-1, 3))),
+                                  _ptr(_d(Bf, (-1, 3))), _ptr(_d(dt)), charge, mass_eV))
+        return P
+
+    def calcBeamParameters(self, R, P, mass_eV=M_P_EV):
+        s = Stats()
+        R, P = _d(R, (-1, 3)), _d(P, (-1, 3))
+        self._c(lib().opalsc_beam_parameters(self._h, len(R), _ptr(R), _ptr(P), mass_eV,
+                                             C.byref(s)))
+        return stats_to_dict(s)
+
+    def device_atan_cr(self, x):
+        x = _d(x).ravel()
+        y = np.empty_like(x)
+        self._c(lib().opalsc_device_atan_cr(self._h, len(x), _ptr(x), _ptr(y)))
+        return y
+
+    def device_log_cr(self, x):
+        x = _d(x).ravel()
+        y = np.empty_like(x)
+        self._c(lib().opalsc_device_log_cr(self._h, len(x), _ptr(x), _ptr(y)))
+        return y
+
+    # ---- device-resident tracker (ParallelTTracker::execute) ----
+    def set_bunch(self, R, P, Q):
+        R, P, Q = _d(R, (-1, 3)), _d(P, (-1, 3)), _d(Q)
+        self._c(lib().opalsc_bunch_set(self._h, len(R), _ptr(R), _ptr(P), _ptr(Q)))
+        self.N = len(R)
+
+    def generate_bunch(self, n, seed, sigma_r, sigma_p, p0, q_total, halo_frac=0.0,
+                       halo_radius=5.0):
+        g = GaussParams()
+        g.n, g.seed = int(n), int(seed)
+        g.sigma_r[:] = list(sigma_r)
+        g.sigma_p[:] = list(sigma_p)
+        g.p0, g.q_total, g.halo_frac, g.halo_radius = p0, q_total, halo_frac, halo_radius
+        self._c(lib().opalsc_bunch_generate(self._h, C.byref(g)))
+        self.N = int(n)
+
+    def get_bunch(self):
+        R = np.zeros((self.N, 3))
+        P = np.zeros((self.N, 3))
+        Q = np.zeros(self.N)
+        self._c(lib().opalsc_bunch_get(self._h, _ptr(R), _ptr(P), _ptr(Q)))
+        return R, P, Q
+
+    def track_init(self, dT, charge=1.0, mass_eV=M_P_EV, coupling=COUPLING, Bz=0.0, Ez0=0.0,
+                   rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), space_charge=True):
+        p = TrackParams()
+        p.dT, p.charge, p.mass_eV, p.coupling = dT, charge, mass_eV, coupling
+        p.Bz, p.Ez0, p.rf_freq, p.rf_phase = Bz, Ez0, rf_freq, rf_phase
+        p.t, p.pathLength = 0.0, 0.0
+        p.RefPartR[:] = [0.0, 0.0, 0.0]
+        p.RefPartP[:] = list(ref_p)
+        p.space_charge = int(space_charge)
+        self._c(lib().opalsc_track_init(self._h, C.byref(p)))
+
+    def track(self, nsteps, want_stats=True):
+        stats = (Stats * max(nsteps, 1))() if want_stats else None
+        self._c(lib().opalsc_track(self._h, int(nsteps),
+                                   C.cast(stats, C.c_void_p) if want_stats else None))
+        return [stats_to_dict(stats[i]) for i in range(nsteps)] if want_stats else None
+
+    def track_params(self):
+        p = TrackParams()
+        self._c(lib().opalsc_track_get_params(self._h, C.byref(p)))
+        return p
+
+    def mesh(self):
+        hr, origin = np.zeros(3), np.zeros(3)
+        self._c(lib().opalsc_get_mesh(self._h, _ptr(hr), _ptr(origin)))
+        return hr, origin
+
+    def profile_enable(self, on=True):
+        self._c(lib().opalsc_profile_enable(self._h, int(on)))
+
+    def profile_read(self):
+        names = (C.c_char_p * 32)()
+        ms = np.zeros(32)
+        launches = np.zeros(32, np.int64)
+        n = lib().opalsc_profile_read(self._h, 32, names, _ptr(ms), _ptr(launches))
+        return {names[i].decode(): (float(ms[i]), int(launches[i])) for i in range(n)}
+
+
+class FFTPoissonSolver:
+    """PoissonSolver interface (src/Solvers/PoissonSolver.h:16-70) over the HIP solver."""
+
+    def __init__(self, n, greens="INTEGRATED", precision=FP64, device=0):
+        g = GREENS_INTEGRATED if greens.upper() == "INTEGRATED" else GREENS_STANDARD
+        self._ctx = SpaceChargeContext(n, precision=precision, greens=g, max_particles=0,
+                                       device=device)
+
+    def computePotential(self, rho, hr):
+        return self._ctx.computePotential(rho, hr)

@@ -1,0 +1,265 @@
+"""ctypes bindings to the CPU oracle (oracle/libopal_oracle.so).
+
+Test infrastructure only: the oracle is the FP64 CPU restatement of OPAL's
+space-charge step that the HIP library is checked against.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg use this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_ROOT, "oracle", "libopal_oracle.so")
+
+PD = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+PI32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+PU8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+PC = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+
+CLIGHT = 299792458.0
+EPSILON_0 = 8.854187817e-12
+PI = 3.14159265358979323846
+COUPLING = 1.0 / (4 * PI * EPSILON_0)
+M_P_EV = 0.93827204e+00 * 1e9
+
+
+class Stats(C.Structure):
+    _fields_ = [(name, C.c_double * 3) for name in
+                ("rmean", "pmean", "rrms", "prms", "eps_norm", "eps", "rprms")] + [
+        ("gamma_mean", C.c_double), ("ekin_MeV", C.c_double),
+        ("moments", C.c_double * 27), ("n", C.c_int64)]
+
+
+class Tracker(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("dh", C.c_double), ("greens", C.c_int),
+                ("space_charge", C.c_int), ("dT", C.c_double), ("charge", C.c_double),
+                ("mass_eV", C.c_double), ("coupling", C.c_double), ("Bz", C.c_double),
+                ("Ez0", C.c_double), ("rf_freq", C.c_double), ("rf_phase", C.c_double),
+                ("t", C.c_double), ("pathLength", C.c_double), ("RefPartR", C.c_double * 3),
+                ("RefPartP", C.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(_ROOT, "oracle")])
+    L = C.CDLL(_LIB_PATH)
+    i64, i32p = C.c_int64, C.POINTER(C.c_int)
+    d3 = C.c_double * 3
+    L.orc_get_bounds.argtypes = [i64, PD, PD, PD]
+    L.orc_boundp.argtypes = [i64, PD, PI32, C.c_double, PD, PD]
+    L.orc_boundp.restype = C.c_int
+    L.orc_cic_index.argtypes = [i64, PD, PD, PD, PI32, PD, PU8]
+    L.orc_scatter.argtypes = [i64, PD, PD, PD, PD, PI32, PD]
+    L.orc_scatter.restype = i64
+    L.orc_gather.argtypes = [i64, PD, PD, PD, PD, PI32, PD]
+    L.orc_fft3d.argtypes = [PC, PI32, C.c_int]
+    L.orc_fft1d.argtypes = [PC, C.c_int, C.c_int]
+    L.orc_igf_octant.argtypes = [PI32, PD, PD, C.c_int]
+    L.orc_green_standard_octant.argtypes = [PI32, PD, PD]
+    L.orc_mirror_green.argtypes = [PI32, PD, PD]
+    L.orc_convolve_doubled.argtypes = [PI32, PD, PD]
+    L.orc_compute_potential.argtypes = [PI32, PD, PD, C.c_int]
+    L.orc_grad.argtypes = [PI32, PD, PD, PD]
+    L.orc_compute_self_fields.argtypes = [i64, PD, PD, PD, PD, C.c_double, PD, PD, PI32,
+                                          C.c_double, C.c_int, PD, PD, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+    L.orc_compute_self_fields.restype = i64
+    L.orc_push.argtypes = [i64, PD, PD, PD]
+    L.orc_kick.argtypes = [i64, PD, PD, PD, PD, C.c_double, C.c_double]
+    L.orc_calc_moments.argtypes = [i64, PD, PD, PD]
+    L.orc_calc_beam_parameters.argtypes = [i64, PD, PD, C.c_double, C.POINTER(Stats)]
+    L.orc_get_quaternion_to_z.argtypes = [PD, PD]
+    L.orc_track.argtypes = [C.POINTER(Tracker), i64, PD, PD, PD, PD, C.c_int, C.POINTER(Stats)]
+    L.orc_track.restype = C.c_int
+    L.orc_space_charge.argtypes = [C.POINTER(Tracker), i64, PD, PD, PD, PD, PD, PD, PD,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_space_charge.restype = i64
+    assert L.orc_sizeof_stats() == C.sizeof(Stats)
+    assert L.orc_sizeof_tracker() == C.sizeof(Tracker)
+    _lib = L
+    return L
+
+
+def _n(n):
+    return np.ascontiguousarray(n, dtype=np.int32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def get_bounds(R):
+    R = _d(R)
+    lo, hi = np.zeros(3), np.zeros(3)
+    lib().orc_get_bounds(len(R), R, lo, hi)
+    return lo, hi
+
+
+def boundp(R, n, dh=0.02):
+    R = _d(R)
+    hr, origin = np.zeros(3), np.zeros(3)
+    rc = lib().orc_boundp(len(R), R, _n(n), dh, hr, origin)
+    if rc != 0:
+        raise ValueError("h<0, can not build a mesh")
+    return hr, origin
+
+
+def cic_index(R, origin, hr):
+    R = _d(R)
+    N = len(R)
+    lgp = np.zeros((N, 3), np.int32)
+    dpos = np.zeros((N, 3))
+    valid = np.zeros(N, np.uint8)
+    lib().orc_cic_index(N, R, _d(origin), _d(hr), lgp, dpos, valid)
+    return lgp, dpos, valid
+
+
+def scatter(R, q, origin, hr, n, rho=None):
+    R = _d(R)
+    n = _n(n)
+    if rho is None:
+        rho = np.zeros(int(n[0]) * int(n[1]) * int(n[2]))
+    dropped = lib().orc_scatter(len(R), R, _d(q), _d(origin), _d(hr), n, rho)
+    return rho, dropped
+
+
+def gather(R, E, origin, hr, n):
+    R = _d(R)
+    Ef = np.zeros((len(R), 3))
+    lib().orc_gather(len(R), R, _d(E), _d(origin), _d(hr), _n(n), Ef)
+    return Ef
+
+
+def fft3d(a, sign):
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    m = _n(a.shape[::-1])
+    lib().orc_fft3d(a, m, sign)
+    return a
+
+
+def igf_octant(n, hr, quirk=True):
+    n = _n(n)
+    G = np.zeros(int(np.prod(n + 1)))
+    lib().orc_igf_octant(n, _d(hr), G, 1 if quirk else 0)
+    return G.reshape(int(n[2]) + 1, int(n[1]) + 1, int(n[0]) + 1)
+
+
+def green_standard_octant(n, hr):
+    n = _n(n)
+    G = np.zeros(int(np.prod(n + 1)))
+    lib().orc_green_standard_octant(n, _d(hr), G)
+    return G.reshape(int(n[2]) + 1, int(n[1]) + 1, int(n[0]) + 1)
+
+
+def mirror_green(n, Goct):
+    n = _n(n)
+    G2 = np.zeros(int(np.prod(2 * n)))
+    lib().orc_mirror_green(n, _d(Goct).ravel(), G2)
+    return G2.reshape(2 * int(n[2]), 2 * int(n[1]), 2 * int(n[0]))
+
+
+def convolve_doubled(n, rho, G2):
+    phi = _d(rho).ravel().copy()
+    lib().orc_convolve_doubled(_n(n), phi, _d(G2).ravel())
+    return phi
+
+
+def compute_potential(n, rho, hr, greens=0):
+    phi = _d(rho).ravel().copy()
+    lib().orc_compute_potential(_n(n), phi, _d(hr), greens)
+    return phi
+
+
+def grad(n, phi, hr):
+    n = _n(n)
+    r = np.zeros(int(np.prod(n)) * 3)
+    lib().orc_grad(n, _d(phi).ravel(), _d(hr), r)
+    return r.reshape(-1, 3)
+
+
+def compute_self_fields(R, P, Q, dt, dT, origin, hr, n, coupling=COUPLING, greens=0, Bf=None,
+                        want_grids=False):
+    R, P = _d(R), _d(P)
+    Q = _d(Q).copy()
+    N = len(R)
+    n = _n(n)
+    ncell = int(np.prod(n))
+    Ef = np.zeros((N, 3))
+    Bf = np.zeros((N, 3)) if Bf is None else _d(Bf).copy()
+    rho = np.zeros(ncell) if want_grids else None
+    phi = np.zeros(ncell) if want_grids else None
+    E = np.zeros(3 * ncell) if want_grids else None
+    ptr = (lambda a: a.ctypes.data if a is not None else None)
+    dropped = lib().orc_compute_self_fields(N, R, P, Q, _d(dt), dT, _d(origin), _d(hr), n,
+                                            coupling, greens, Ef, Bf, ptr(rho), ptr(phi), ptr(E))
+    out = dict(Ef=Ef, Bf=Bf, Q=Q, dropped=dropped)
+    if want_grids:
+        out.update(rho=rho, phi=phi, E=E.reshape(-1, 3))
+    return out
+
+
+def push(R, P, dt):
+    R = _d(R).copy()
+    lib().orc_push(len(R), R, _d(P), _d(dt))
+    return R
+
+
+def kick(P, Ef, Bf, dt, charge=1.0, mass=M_P_EV):
+    P = _d(P).copy()
+    lib().orc_kick(len(P), P, _d(Ef), _d(Bf), _d(dt), charge, mass)
+    return P
+
+
+def calc_moments(R, P):
+    m = np.zeros(27)
+    lib().orc_calc_moments(len(R), _d(R), _d(P), m)
+    return m
+
+
+def beam_parameters(R, P, mass=M_P_EV):
+    s = Stats()
+    lib().orc_calc_beam_parameters(len(R), _d(R), _d(P), mass, C.byref(s))
+    return stats_dict(s)
+
+
+def stats_dict(s):
+    d = {k: np.array(getattr(s, k)[:]) for k in
+         ("rmean", "pmean", "rrms", "prms", "eps_norm", "eps", "rprms")}
+    d.update(gamma_mean=s.gamma_mean, ekin_MeV=s.ekin_MeV, moments=np.array(s.moments[:]), n=s.n)
+    return d
+
+
+def quaternion_to_z(u):
+    q = np.zeros(4)
+    lib().orc_get_quaternion_to_z(_d(u), q)
+    return q
+
+
+def make_tracker(n, dT, charge=1.0, mass=M_P_EV, dh=0.02, greens=0, space_charge=True, Bz=0.0,
+                 Ez0=0.0, rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), coupling=COUPLING):
+    t = Tracker()
+    t.n[:] = list(n)
+    t.dh, t.greens, t.space_charge, t.dT = dh, greens, int(space_charge), dT
+    t.charge, t.mass_eV, t.coupling = charge, mass, coupling
+    t.Bz, t.Ez0, t.rf_freq, t.rf_phase = Bz, Ez0, rf_freq, rf_phase
+    t.t, t.pathLength = 0.0, 0.0
+    t.RefPartR[:] = [0.0, 0.0, 0.0]
+    t.RefPartP[:] = list(ref_p)
+    return t
+
+
+def track(tracker, R, P, Q, dt, nsteps):
+    R, P, Q, dt = (_d(a).copy() for a in (R, P, Q, dt))
+    stats = (Stats * max(nsteps, 1))()
+    rc = lib().orc_track(C.byref(tracker), len(R), R, P, Q, dt, nsteps, stats)
+    if rc != 0:
+        raise RuntimeError("orc_track failed: %d" % rc)
+    return R, P, Q, dt, [stats_dict(stats[i]) for i in range(nsteps)]

@@ -1,0 +1,145 @@
+"""Parity of the HIP library (through its C ABI) with the CPU oracle.
+
+Bars (from the FP64 drop-in requirement): cell indices, CIC offsets, the
+cell sort, push and kick bit-exact; rho/phi/E/Ef/Bf within 1e-12 relative L2;
+statistics within 1e-10 relative where the statistic is well conditioned.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import gaussian_bunch, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+sc = pytest.importorskip("src_amd")
+
+
+def ctx(n, precision=sc.FP64, greens=sc.GREENS_INTEGRATED, N=1 << 17, sort_interval=0):
+    return sc.SpaceChargeContext(n, precision=precision, greens=greens, max_particles=N,
+                                 sort_interval=sort_interval)
+
+
+def test_cr_math_device_equals_host_and_quad():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-4, 4, 50000), 10 ** rng.uniform(-20, 20, 50000)])
+    with ctx((8, 8, 8), N=200000) as c:
+        da, dl = c.device_atan_cr(x), c.device_log_cr(np.abs(x) + 1e-300)
+    ha, hl = sc.host_atan_cr(x), sc.host_log_cr(np.abs(x) + 1e-300)
+    assert np.array_equal(da, ha)
+    assert np.array_equal(dl, hl)
+    L = O.lib()
+    L.orc_atan_cr.restype = L.orc_log_cr.restype = O.C.c_double
+    L.orc_atan_cr.argtypes = L.orc_log_cr.argtypes = [O.C.c_double]
+    idx = rng.integers(0, len(x), 3000)
+    assert all(L.orc_atan_cr(float(x[i])) == ha[i] for i in idx)
+    assert all(L.orc_log_cr(float(abs(x[i]) + 1e-300)) == hl[i] for i in idx)
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (16, 24, 40)])
+def test_boundp_and_cic_index_bit_exact(n):
+    R, P, Q = gaussian_bunch(100000, seed=1)
+    R[:, 2] *= 3.0
+    hr_o, org_o = O.boundp(R, n)
+    lgp_o, dpos_o, valid = O.cic_index(R, org_o, hr_o)
+    with ctx(n) as c:
+        hr, org = c.boundp(R)
+        lgp, dpos = c.cic_index(R, org, hr)
+    assert np.array_equal(hr, hr_o) and np.array_equal(org, org_o)
+    assert valid.all()
+    assert np.array_equal(lgp, lgp_o)
+    assert np.array_equal(dpos, dpos_o)
+
+
+def test_sort_by_cell_is_stable_and_bit_exact():
+    n = (32, 32, 32)
+    R, P, Q = gaussian_bunch(100000, seed=2)
+    hr, org = O.boundp(R, n)
+    lgp, _, _ = O.cic_index(R, org, hr)
+    key = (lgp[:, 2].astype(np.int64) * n[1] + lgp[:, 1]) * n[0] + lgp[:, 0]
+    with ctx(n) as c:
+        perm = c.sort_by_cell(R, org, hr)
+    assert np.array_equal(perm.astype(np.int64), np.argsort(key, kind="stable"))
+
+
+@pytest.mark.parametrize("greens", [0, 1])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12)])
+def test_compute_potential_fp64(n, greens):
+    rng = np.random.default_rng(4)
+    rho = rng.normal(size=n[0] * n[1] * n[2])
+    hr = np.array([1.1e-4, 0.9e-4, 2.3e-4])
+    ref = O.compute_potential(n, rho, hr, greens)
+    with ctx(n, greens=greens) as c:
+        phi = c.computePotential(rho, hr)
+    assert rel_l2(phi, ref) < 1e-12
+
+
+def _self_fields_case(n, N, seed, precision=sc.FP64):
+    R, P, Q = gaussian_bunch(N, seed=seed, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    hr, org = O.boundp(R, n)
+    ref = O.compute_self_fields(R, P, Q, dt, 1e-11, org, hr, n, want_grids=True)
+    with ctx(n, precision=precision, N=N) as c:
+        Ef, Bf, Q2 = c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr)
+        grids = dict(rho=c.get_grid(sc.GRID_RHO), phi=c.get_grid(sc.GRID_PHI),
+                     E=c.get_grid(sc.GRID_EFIELD))
+    return ref, Ef, Bf, Q2, grids
+
+
+@pytest.mark.parametrize("n,N", [((32, 32, 32), 100000), ((16, 32, 24), 20000)])
+def test_compute_self_fields_fp64(n, N):
+    ref, Ef, Bf, Q2, g = _self_fields_case(n, N, seed=5)
+    assert ref["dropped"] == 0
+    assert np.array_equal(Q2, ref["Q"])
+    assert rel_l2(g["rho"], ref["rho"]) < 1e-12
+    assert rel_l2(g["phi"], ref["phi"]) < 1e-12
+    assert rel_l2(g["E"], ref["E"]) < 1e-12
+    assert rel_l2(Ef, ref["Ef"]) < 1e-12
+    assert rel_l2(Bf, ref["Bf"]) < 1e-12
+
+
+def test_compute_self_fields_mixed_within_1e5():
+    ref, Ef, Bf, Q2, g = _self_fields_case((32, 32, 32), 100000, seed=6, precision=sc.MIXED)
+    assert rel_l2(g["E"], ref["E"]) < 1e-5
+    assert rel_l2(Ef, ref["Ef"]) < 1e-5
+
+
+def test_push_and_kick_bit_exact():
+    R, P, Q = gaussian_bunch(50000, seed=7)
+    rng = np.random.default_rng(8)
+    dt = rng.uniform(0.5e-11, 2e-11, len(R))
+    Ef = rng.normal(size=R.shape) * 1e6
+    Bf = rng.normal(size=R.shape) * 0.5
+    with ctx((8, 8, 8), N=len(R)) as c:
+        R2 = c.pushParticles(R, P, dt)
+        P2 = c.kickParticles(P, Ef, Bf, dt, 1.0, O.M_P_EV)
+    assert np.array_equal(R2, O.push(R, P, dt))
+    assert np.array_equal(P2, O.kick(P, Ef, Bf, dt, 1.0, O.M_P_EV))
+
+
+def test_beam_parameters():
+    R, P, Q = gaussian_bunch(200000, seed=9)
+    ref = O.beam_parameters(R, P)
+    with ctx((8, 8, 8), N=len(R)) as c:
+        s = c.calcBeamParameters(R, P)
+    for k in ("rmean", "pmean", "rrms", "prms", "eps_norm"):
+        assert np.allclose(s[k], ref[k], rtol=1e-10, atol=0), k
+    assert abs(s["gamma_mean"] - ref["gamma_mean"]) <= 1e-15 * ref["gamma_mean"]
+
+
+def test_tracker_matches_oracle_c1_shape():
+    n, N, steps = (32, 32, 32), 100000, 5
+    R, P, Q = gaussian_bunch(N, seed=1, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181))
+    Ro, Po, Qo, _, so = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, N=N) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+        sg = c.track(steps)
+        Rg, Pg, Qg = c.get_bunch()
+    assert rel_l2(Rg, Ro) < 1e-12
+    assert rel_l2(Pg - P, Po - P) < 1e-9
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm", "prms"):
+            assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])

@@ -1,0 +1,368 @@
+"""Pins of the CPU oracle against mathematics that does not depend on it.
+
+The reference ships no golden vectors for the space-charge path (its tests
+never run the FFT solver), so every oracle function is checked here against
+an independent fact: closed forms, brute force, numerical quadrature,
+numpy's pocketfft, mpmath, or a conservation law.  Each check is chosen so a
+plausible slip (a dropped IGF term, a wrong sign, a transposed index, an
+off-by-one cell) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import gaussian_bunch, rel_l2
+
+C = O.CLIGHT
+
+
+# ---------------------------------------------------------------- bounds
+def test_bounds_hand_case_and_padding():
+    R = np.array([[0.0, 1.0, -2.0], [3.0, -1.0, 5.0], [1.0, 0.5, 0.0]])
+    lo, hi = O.get_bounds(R)
+    assert lo.tolist() == [0.0, -1.0, -2.0] and hi.tolist() == [3.0, 1.0, 5.0]
+    n = (8, 16, 32)
+    hr, origin = O.boundp(R, n, dh=0.02)
+    for d in range(3):
+        L = hi[d] - lo[d]
+        mn, mx = lo[d] - 0.02 * L, hi[d] + 0.02 * L
+        assert hr[d] == (mx - mn) / (n[d] - 1)
+        assert origin[d] == mn - hr[d] / 2.0
+
+
+def test_boundp_degenerate_extent_branch():
+    R = np.zeros((5, 3))
+    R[:, 0] = np.linspace(-1, 1, 5)
+    hr, origin = O.boundp(R, (8, 8, 8))
+    assert hr[1] == (1e-10 - (-1e-10)) / 7
+    assert origin[2] == -1e-10 - hr[2] / 2
+
+
+# ---------------------------------------------------------------- CIC
+def test_cic_particle_on_cell_centre_deposits_to_one_cell():
+    n, hr, origin = (8, 8, 8), np.array([0.1, 0.2, 0.3]), np.array([-0.4, -0.8, -1.2])
+    i, j, k = 3, 4, 5
+    x = np.array([[(i + 0.5) * hr[0] + origin[0], (j + 0.5) * hr[1] + origin[1],
+                   (k + 0.5) * hr[2] + origin[2]]])
+    lgp, dpos, valid = O.cic_index(x, origin, hr)
+    assert lgp.tolist() == [[i, j, k]] and dpos.tolist() == [[0.0, 0.0, 0.0]]
+    rho, dropped = O.scatter(x, [2.5], origin, hr, n)
+    assert dropped == 0
+    assert rho[(k * n[1] + j) * n[0] + i] == 2.5 and np.count_nonzero(rho) == 1
+
+
+def test_cic_hand_two_particles():
+    n, hr, origin = (4, 4, 4), np.ones(3), np.zeros(3)
+    # cell centres at 0.5,1.5,...: x=1.25 -> lgp 0, d=0.75 ; x=2.0 -> lgp 1, d=0.5
+    R = np.array([[1.25, 0.5, 0.5], [2.0, 2.0, 0.5]])
+    lgp, dpos, _ = O.cic_index(R, origin, hr)
+    assert lgp.tolist() == [[0, 0, 0], [1, 1, 0]]
+    assert dpos.tolist() == [[0.75, 0.0, 0.0], [0.5, 0.5, 0.0]]
+    rho, _ = O.scatter(R, [1.0, 2.0], origin, hr, n)
+    r = rho.reshape(4, 4, 4)
+    assert r[0, 0, 0] == 0.25 and r[0, 0, 1] == 0.75
+    assert r[0, 1, 1] == r[0, 1, 2] == r[0, 2, 1] == r[0, 2, 2] == 0.5
+
+
+def test_cic_conservation_and_partition_of_unity():
+    R, P, Q = gaussian_bunch(20000, seed=3)
+    n = (16, 16, 16)
+    hr, origin = O.boundp(R, n)
+    rng = np.random.default_rng(1)
+    q = rng.uniform(0.5, 1.5, len(R))
+    rho, dropped = O.scatter(R, q, origin, hr, n)
+    assert dropped == 0
+    assert abs(rho.sum() - math.fsum(q)) < 1e-12 * q.sum()
+    lgp, dpos, valid = O.cic_index(R, origin, hr)
+    assert valid.all() and (dpos >= 0).all() and (dpos <= 1).all()
+    assert (lgp >= 0).all() and (lgp <= np.array(n) - 2).all()
+
+
+def test_cic_negative_side_uses_vertex_below():
+    # getVertexBelow: (int)((x-o)/h), minus one when x < origin
+    lgp, dpos, _ = O.cic_index(np.array([[-0.3, -1.0, 0.2]]), np.zeros(3), np.ones(3))
+    assert lgp.tolist() == [[-1, -2, -1]]
+    rho, dropped = O.scatter(np.array([[-0.3, 0.5, 0.5]]), [1.0], np.zeros(3), np.ones(3), (4, 4, 4))
+    assert dropped == 4
+
+
+def test_gather_linear_field_exact():
+    n, hr, origin = (10, 12, 14), np.array([0.1, 0.2, 0.15]), np.array([-0.5, -1.2, -1.05])
+    a = np.array([1.5, -0.7, 2.25])
+    ii, jj, kk = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    pos = [(g + 0.5) * hr[d] + origin[d] for d, g in enumerate((ii, jj, kk))]
+    f = a[0] * pos[0] + a[1] * pos[1] + a[2] * pos[2] + 0.3
+    E = np.zeros((n[2], n[1], n[0], 3))
+    for d in range(3):
+        E[..., d] = (f * (d + 1)).transpose(2, 1, 0)
+    rng = np.random.default_rng(2)
+    lo = origin + hr * 0.6
+    hi = origin + hr * (np.array(n) - 0.6)
+    R = lo + rng.random((500, 3)) * (hi - lo)
+    Ef = O.gather(R, E.reshape(-1, 3), origin, hr, n)
+    expect = (R @ a + 0.3)[:, None] * np.array([1, 2, 3])
+    assert np.max(np.abs(Ef - expect)) < 1e-12 * np.max(np.abs(expect))
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(6, 8, 12), (4, 4, 4), (3, 10, 16), (24, 6, 8)])
+def test_fft3d_matches_numpy(shape):
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    assert np.max(np.abs(O.fft3d(a, -1) - np.fft.fftn(a))) < 1e-12 * np.abs(a).sum()
+    assert np.max(np.abs(O.fft3d(a, +1) - np.fft.ifftn(a) * a.size)) < 1e-12 * np.abs(a).sum()
+
+
+# ---------------------------------------------------------------- Green's functions
+def _gl_cube_integral(c, h, m=24):
+    """Gauss-Legendre integral of 1/r over the cell centred at c (not containing 0)."""
+    x, w = np.polynomial.legendre.leggauss(m)
+    pts = [c[d] + 0.5 * h[d] * x for d in range(3)]
+    X, Y, Z = np.meshgrid(*pts, indexing="ij")
+    W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    return (W / np.sqrt(X * X + Y * Y + Z * Z)).sum() * (0.5 ** 3) * h[0] * h[1] * h[2]
+
+
+def test_igf_self_cell_closed_form():
+    # average of 1/r over the unit cube centred at 0 = 3 ln(2+sqrt3) - pi/2 (before the quirk)
+    G = O.igf_octant((4, 4, 4), np.ones(3), quirk=False)
+    exact = 3 * math.log(2 + math.sqrt(3)) - math.pi / 2
+    assert abs(G[0, 0, 0] - exact) < 1e-14
+    # pyramid reduction: int_[0,1]^3 1/r = 3/2 int_[0,1]^2 (1+u^2+v^2)^-1/2 du dv
+    from scipy import integrate
+    I2, _ = integrate.dblquad(lambda v, u: 1 / math.sqrt(1 + u * u + v * v), 0, 1, 0, 1,
+                              epsabs=1e-14, epsrel=1e-14)
+    assert abs(2 * 1.5 * I2 - exact) < 1e-12
+
+
+@pytest.mark.parametrize("cell", [(1, 0, 0), (1, 1, 1), (2, 1, 0), (0, 3, 2)])
+def test_igf_matches_quadrature(cell):
+    hr = np.array([1.0e-4, 0.7e-4, 1.9e-4])
+    n = (6, 6, 6)
+    G = O.igf_octant(n, hr, quirk=False)
+    c = np.array(cell) * hr
+    ref = _gl_cube_integral(c, hr, m=40) / (hr[0] * hr[1] * hr[2])
+    assert abs(G[cell[2], cell[1], cell[0]] - ref) < 1e-10 * ref
+
+
+def test_igf_far_field_is_one_over_r_and_quirk():
+    n, hr = (24, 24, 24), np.array([1.0, 1.0, 1.0])
+    G = O.igf_octant(n, hr)
+    assert G[0, 0, 0] == G[1, 0, 0]  # G[x=0,y=0,z=0] := G[x=0,y=0,z=1]
+    for (i, j, k) in [(10, 4, 7), (20, 0, 3), (5, 15, 12)]:
+        r = math.sqrt(i * i + j * j + k * k)
+        # cube average of the harmonic 1/r differs from 1/r at O(h^4/r^5)
+        assert abs(G[k, j, i] * r - 1) < 2.0 / r ** 4
+    # homogeneity: scaling h by s scales the cell average of 1/r by 1/s
+    G2 = O.igf_octant((6, 6, 6), np.array([2.0, 2.0, 2.0]), quirk=False)
+    G1 = O.igf_octant((6, 6, 6), np.array([1.0, 1.0, 1.0]), quirk=False)
+    assert np.allclose(G2, G1 / 2, rtol=1e-9, atol=0)
+
+
+def test_igf_quad_vs_glibc_far_field_sensitivity():
+    """glibc's atan/log are not correctly rounded for a few IGF arguments; the
+    8-corner difference turns those ulps into ~1e-9 of far-field G.  The
+    oracle uses correctly rounded values; the two agree to that level."""
+    L = O.lib()
+    L.orc_set_igf_libm.argtypes = [O.C.c_int]
+    n, hr = (48, 48, 48), np.array([1.0e-4, 1.2e-4, 2.5e-4])
+    Gcr = O.igf_octant(n, hr)
+    L.orc_set_igf_libm(0)
+    try:
+        Gg = O.igf_octant(n, hr)
+    finally:
+        L.orc_set_igf_libm(1)
+    d = np.abs(Gcr - Gg) / np.abs(Gcr)
+    assert d.max() < 1e-8
+    assert (d > 0).mean() < 0.2
+
+
+def test_standard_green_function():
+    n, hr = (5, 6, 7), np.array([0.3, 0.5, 0.7])
+    G = O.green_standard_octant(n, hr)
+    assert abs(G[2, 3, 4] * math.sqrt(16 * 0.09 + 9 * 0.25 + 4 * 0.49) - 1) < 1e-15
+    assert G[0, 0, 0] == G[1, 0, 0]
+    assert abs(G[1, 0, 0] * 0.7 - 1) < 1e-15
+
+
+def test_mirror_is_even_extension():
+    n = (3, 4, 5)
+    Goct = np.arange(4 * 5 * 6, dtype=float).reshape(6, 5, 4)
+    G2 = O.mirror_green(n, Goct)
+    for (z, y, x) in [(7, 1, 5), (9, 7, 3), (0, 4, 4), (5, 5, 3)]:
+        f = lambda i, m: i if i <= m else 2 * m - i  # noqa: E731
+        assert G2[z, y, x] == Goct[f(z, 5), f(y, 4), f(x, 3)]
+
+
+# ---------------------------------------------------------------- Poisson solve
+def test_convolution_equals_brute_force():
+    rng = np.random.default_rng(5)
+    n = (5, 4, 6)
+    rho = rng.normal(size=np.prod(n))
+    G2 = O.mirror_green(n, O.igf_octant(n, np.array([1.0, 0.8, 1.3])))
+    phi = O.convolve_doubled(n, rho, G2)
+    r3 = rho.reshape(n[2], n[1], n[0])
+    M = G2.shape
+    bf = np.zeros_like(r3)
+    zz, yy, xx = np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")
+    for z in range(n[2]):
+        for y in range(n[1]):
+            for x in range(n[0]):
+                bf[z, y, x] = (G2[(z - zz) % M[0], (y - yy) % M[1], (x - xx) % M[2]] * r3).sum()
+    assert rel_l2(phi, bf) < 1e-14
+
+
+def test_point_charge_potential_is_green_function():
+    n, hr = (8, 8, 8), np.array([1.0, 1.0, 1.0])
+    rho = np.zeros(512)
+    c = (2, 5, 3)
+    rho[(c[2] * 8 + c[1]) * 8 + c[0]] = 1.0
+    phi = O.compute_potential(n, rho, hr).reshape(8, 8, 8)
+    G = O.igf_octant(n, hr)
+    for (x, y, z) in [(2, 5, 3), (7, 0, 0), (0, 2, 6)]:
+        g = G[abs(z - c[2]), abs(y - c[1]), abs(x - c[0])]
+        assert abs(phi[z, y, x] - g) < 1e-14 * G.max()
+
+
+def test_compute_potential_matches_numpy_fft():
+    rng = np.random.default_rng(6)
+    n, hr = (12, 8, 10), np.array([1e-4, 2e-4, 1.5e-4])
+    rho = rng.normal(size=np.prod(n))
+    phi = O.compute_potential(n, rho, hr)
+    G2 = O.mirror_green(n, O.igf_octant(n, hr))
+    a = np.zeros(G2.shape)
+    a[:n[2], :n[1], :n[0]] = rho.reshape(n[2], n[1], n[0])
+    ref = np.real(np.fft.ifftn(np.fft.fftn(a) * np.fft.fftn(G2)))[:n[2], :n[1], :n[0]]
+    assert rel_l2(phi, ref) < 1e-13
+
+
+# ---------------------------------------------------------------- gradient
+def test_grad_exact_on_quadratics_including_borders():
+    n, hr = (7, 6, 9), np.array([0.5, 0.3, 0.2])
+    z, y, x = np.meshgrid(*[np.arange(n[d]) * hr[d] for d in (2, 1, 0)], indexing="ij")
+    phi = 1.3 * x * x - 0.4 * x * y + 2.0 * z * z + 0.7 * y - 1.1 * z * x + 5.0
+    g = O.grad(n, phi.ravel(), hr).reshape(n[2], n[1], n[0], 3)
+    ex = 2.6 * x - 0.4 * y - 1.1 * z
+    ey = -0.4 * x + 0.7 + 0 * y
+    ez = 4.0 * z - 1.1 * x
+    for d, e in enumerate((ex, ey, ez)):
+        assert np.max(np.abs(g[..., d] - e)) < 1e-11
+
+
+# ---------------------------------------------------------------- self fields
+def test_self_field_of_gaussian_bunch_matches_erf_formula():
+    """Rest-frame-like test (p0 tiny): the field of a spherical Gaussian of
+    charge Q is Q/(4 pi eps0 r^2) [erf(r/(sqrt2 s)) - sqrt(2/pi) r/s exp(-r^2/2s^2)].
+    CIC smoothing biases the grid field at O(h^2/s^2), so the check is a
+    convergence test: the error must be a few percent and shrink with n (a
+    wrong scale factor or a missing h^3 would be off by O(1))."""
+    from scipy.special import erf
+    N, s, Qt = 400000, 1e-3, 1e-9
+    R, P, Q = gaussian_bunch(N, seed=12, q_total=Qt, p0=1e-9, sigma_p=(0, 0, 0))
+    r = np.linalg.norm(R, axis=1)
+    sel = (r > 0.8 * s) & (r < 2.5 * s)
+    rr = r[sel]
+    x = rr / (math.sqrt(2) * s)
+    Er = O.COUPLING * Qt / rr ** 2 * (erf(x) - np.sqrt(2 / np.pi) * rr / s * np.exp(-x * x))
+    errs = []
+    for n in ((32, 32, 32), (64, 64, 64)):
+        hr, origin = O.boundp(R, n)
+        out = O.compute_self_fields(R, P, Q, np.full(N, 1e-11), 1e-11, origin, hr, n)
+        Erad = (out["Ef"][sel] * R[sel]).sum(axis=1) / rr
+        errs.append(np.median(np.abs(Erad / Er - 1)))
+    assert errs[1] < 0.03 and errs[1] < 0.6 * errs[0], errs
+
+
+# ---------------------------------------------------------------- pusher
+def test_push_is_half_drift():
+    rng = np.random.default_rng(7)
+    R = rng.normal(size=(100, 3))
+    P = rng.normal(size=(100, 3))
+    dt = rng.uniform(1e-12, 1e-11, 100)
+    g = np.sqrt(1 + (P * P).sum(1))
+    R2 = O.push(R, P, dt)
+    assert np.allclose(R2, R + 0.5 * C * dt[:, None] * P / g[:, None], rtol=1e-14, atol=1e-20)
+
+
+def test_kick_pure_E_and_pure_B():
+    m, q, dt = O.M_P_EV, 1.0, 1e-11
+    P = np.array([[0.0, 0.01, 0.5]])
+    E = np.array([[1e6, -2e5, 3e5]])
+    P2 = O.kick(P, E, np.zeros((1, 3)), [dt], q, m)
+    assert np.allclose(P2, P + dt * q * C / m * E, rtol=1e-15, atol=1e-18)
+    B = np.array([[0.0, 0.0, 1.5]])
+    P3 = O.kick(P, np.zeros((1, 3)), B, [dt], q, m)
+    assert abs(np.linalg.norm(P3) - np.linalg.norm(P)) < 1e-15
+    gamma = math.sqrt(1 + (P * P).sum())
+    t = 0.5 * dt * q * C * C / (gamma * m) * 1.5
+    ang = math.atan2(P3[0, 0], P3[0, 1]) - math.atan2(P[0, 0], P[0, 1])
+    assert abs(abs(ang) - 2 * math.atan(t)) < 1e-12
+    assert P3[0, 2] == P[0, 2]
+
+
+# ---------------------------------------------------------------- statistics
+def test_moments_and_beam_parameters_vs_fsum():
+    R, P, Q = gaussian_bunch(5000, seed=13)
+    m = O.calc_moments(R, P)
+    part = np.stack([R[:, 0], P[:, 0], R[:, 1], P[:, 1], R[:, 2], P[:, 2]], 1)
+    l = 6
+    for i in range(6):
+        assert abs(m[i] - math.fsum(part[:, i])) <= 1e-12 * np.abs(part[:, i]).sum()
+        for j in range(i + 1):
+            ex = math.fsum(part[:, i] * part[:, j])
+            assert abs(m[l] - ex) <= 1e-12 * np.abs(part[:, i] * part[:, j]).sum()
+            l += 1
+    s = O.beam_parameters(R, P)
+    for d in range(3):
+        x, p = part[:, 2 * d], part[:, 2 * d + 1]
+        assert abs(s["rrms"][d] - x.std()) < 1e-9 * x.std()
+        cov = np.cov(x, p, bias=True)
+        assert abs(s["eps_norm"][d] - math.sqrt(np.linalg.det(cov))) < 1e-8 * s["eps_norm"][d]
+    g = np.sqrt(1 + (P * P).sum(1)).mean()
+    assert abs(s["gamma_mean"] - g) < 1e-14
+
+
+def test_quaternion_aligns_mean_momentum_with_z():
+    u = np.array([0.01, -0.02, 0.8])
+    q = O.quaternion_to_z(u)
+    assert abs(np.dot(q, q) - 1) < 1e-15
+    w, x, y, z = q
+    M = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (w * y + x * z)],
+                  [2 * (w * z + x * y), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (w * x + y * z), 1 - 2 * (x * x + y * y)]])
+    v = M @ (u / np.linalg.norm(u))
+    assert np.allclose(v, [0, 0, 1], atol=1e-13)
+    assert np.allclose(O.quaternion_to_z([0, 0, 2.0]), [1, 0, 0, 0])
+
+
+# ---------------------------------------------------------------- tracker
+def test_track_zero_charge_is_ballistic_in_reference_frame():
+    N, steps, dT = 2000, 7, 1e-11
+    R, P, Q = gaussian_bunch(N, seed=14, p0=0.3)
+    Q[:] = 0.0
+    tr = O.make_tracker((16, 16, 16), dT, ref_p=(0, 0, 0.3))
+    R2, P2, _, _, stats = O.track(tr, R, P, Q, np.full(N, dT), steps)
+    g = np.sqrt(1 + (P * P).sum(1))
+    gref = math.sqrt(1 + 0.09)
+    expect = R + steps * C * dT * (P / g[:, None] - np.array([0, 0, 0.3 / gref]))
+    assert np.allclose(R2, expect, rtol=0, atol=1e-15)
+    assert np.array_equal(P2, P)
+    assert abs(tr.pathLength - steps * C * dT * 0.3 / gref) < 1e-15
+
+
+def test_track_single_particle_gyration_in_solenoid():
+    """No space charge, uniform Bz: the transverse momentum rotates by
+    2*atan(t) per step with |p| conserved (Boris rotation)."""
+    N, steps, dT, Bz = 1, 50, 1e-11, 0.5
+    R = np.zeros((1, 3))
+    P = np.array([[0.01, 0.0, 0.046181]])
+    tr = O.make_tracker((8, 8, 8), dT, space_charge=False, Bz=Bz, ref_p=(0, 0, 0.046181))
+    R2, P2, _, _, _ = O.track(tr, R, P, np.zeros(1), np.full(1, dT), steps)
+    gamma = math.sqrt(1 + (P * P).sum())
+    t = 0.5 * dT * 1.0 * C * C / (gamma * O.M_P_EV) * Bz
+    ang = steps * 2 * math.atan(t)
+    assert abs(np.hypot(*P2[0, :2]) - 0.01) < 1e-15
+    got = math.atan2(P2[0, 1], P2[0, 0])
+    assert abs((got - (-ang) + math.pi) % (2 * math.pi) - math.pi) < 1e-12
