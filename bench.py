@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark: OPAL-T space-charge PIC step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 1e8
+macroparticles, 6D Gaussian (sigma 1 mm, normalised momentum spread 1e-3)
+with 10% of the positions uniform in the 5-sigma ball, 250 MeV protons,
+10 nC, 256^3 grid (512^3 doubled-grid convolution), drift, dT = 1e-11 s.
+A step is one ParallelTTracker step over the resident bunch: push, beam-frame
+rotation, boundp, CIC scatter, IGF Green's function + Hockney convolution,
+E = -Grad(phi), gather, kick, push, reference-frame update, statistics.
+
+value = mixed-precision particle-steps/s (FP32 FFT/convolution/gather, FP64
+IGF, moments and particle state).  The plain FP64 step is timed on the same
+bunch for the speed ratio, and the E-field / emittance errors of the mixed
+step against the FP64 step are reported.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MP_EV = 0.93827204e+00 * 1e9
+
+
+def gamma_betagamma(ekin_mev):
+    g = 1.0 + ekin_mev * 1e6 / MP_EV
+    return g, math.sqrt(g * g - 1.0)
+
+
+CONFIGS = {
+    # the metric's configuration
+    "c3": dict(N=100_000_000, n=256, ekin=250.0, q=10e-9, halo=0.1, Bz=0.0, seed=3),
+    # plain FP64 drop-in configuration, used for smaller runs
+    "c2": dict(N=10_000_000, n=128, ekin=1.0, q=10e-9, halo=0.0, Bz=0.5, seed=2),
+    "c1": dict(N=100_000, n=32, ekin=1.0, q=1e-9, halo=0.0, Bz=0.0, seed=1),
+}
+
+
+def make_ctx(sc, cfg, precision, sort_interval):
+    n = cfg["n"]
+    c = sc.SpaceChargeContext((n, n, n), precision=precision, max_particles=cfg["N"],
+                              sort_interval=sort_interval)
+    g, bg = gamma_betagamma(cfg["ekin"])
+    c.generate_bunch(cfg["N"], cfg["seed"], (1e-3, 1e-3, 1e-3), (1e-3, 1e-3, 1e-3), bg, cfg["q"],
+                     halo_frac=cfg["halo"], halo_radius=5.0)
+    c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, Bz=cfg["Bz"], ref_p=(0.0, 0.0, bg))
+    return c
+
+
+def timed_run(c, steps, warmup):
+    if warmup:
+        c.track(warmup)
+    sc_sync = c._c  # noqa: F841
+    from src_amd.opalsc import lib
+    lib().opalsc_synchronize(c._h)
+    t0 = time.perf_counter()
+    stats = c.track(steps)
+    lib().opalsc_synchronize(c._h)
+    t1 = time.perf_counter()
+    return (t1 - t0), stats
+
+
+def algorithmic_bytes(cfg, wbytes):
+    """Per-launch algorithmic HBM bytes of each kernel class (one step)."""
+    N, n = cfg["N"], cfg["n"]
+    n3, K0 = n ** 3, n + 1
+    cw = 2 * wbytes  # complex word
+    return {
+        # read x,y,z,px,py,pz; write x,y,z
+        "push_rotate_bounds": N * 72,
+        # read x,y,z,q; read+write int64 rho once
+        "deposit": N * 32 + n3 * 16,
+        # read+write x,y,z,px,py,pz; read the E grid (4 words per cell) once
+        "gather_kick_push": N * 96 + n3 * 4 * wbytes,
+        # x pass: int64 rho -> A, and A -> phi (two launches; per-launch average)
+        "fft_x": (n3 * 8 + K0 * n * n * cw + K0 * n * n * cw + n3 * wbytes) / 2,
+        # y passes: A -> B, B -> A (two launches)
+        "fft_y": (K0 * n * n * cw + K0 * 2 * n * n * cw) * 2 / 2,
+        # z conv: B in/out + Ghat
+        "fft_z_conv": 2 * K0 * 2 * n * n * cw + (n + 1) ** 3 * wbytes,
+        "grad": n3 * wbytes + n3 * 4 * wbytes,
+        "zero_rho": n3 * 8,
+    }
+
+
+def cpu_baseline(budget_s=20.0):
+    """The oracle (single-threaded C restatement of the OPAL 2.0 step) on a
+    bounded sample: 1e6 particles of the benchmark distribution on a 64^3
+    grid (128^3 doubled convolution), glibc libm, timed per full step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    O.lib().orc_set_igf_libm.argtypes = [O.C.c_int]
+    O.lib().orc_set_igf_libm(0)
+    N, n = 1_000_000, 64
+    g, bg = gamma_betagamma(250.0)
+    rng = np.random.default_rng(3)
+    R = rng.normal(size=(N, 3)) * 1e-3
+    m = rng.random(N) < 0.1
+    d = rng.normal(size=(int(m.sum()), 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    R[m] = d * (5e-3 * rng.random(len(d)) ** (1 / 3))[:, None]
+    P = rng.normal(size=(N, 3)) * 1e-3
+    P[:, 2] += bg
+    Q = np.full(N, 10e-9 / N)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker((n, n, n), 1e-11, ref_p=(0, 0, bg))
+    steps, t = 0, 0.0
+    while t < budget_s and steps < 5:
+        t0 = time.perf_counter()
+        R, P, Q, dt, _ = O.track(tr, R, P, Q, dt, 1)
+        t += time.perf_counter() - t0
+        steps += 1
+    O.lib().orc_set_igf_libm(1)
+    return {"value": N * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle (CPU restatement of the OPAL 2.0 step, 1 thread, glibc libm): "
+                      "%d full steps of 1e6 particles of the benchmark distribution on a 64^3 "
+                      "grid (128^3 doubled FFT), %.1f s" % (steps, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (the space-charge step of one "
+              "bunch on one MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import src_amd as sc
+    cfg = CONFIGS[args.config]
+    N, n = cfg["N"], cfg["n"]
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    log("[bench] mixed precision, N=%d n=%d^3" % (N, n))
+    cm = make_ctx(sc, cfg, sc.MIXED, args.sort_interval)
+    t_mixed, st_mixed = timed_run(cm, args.steps, args.warmup)
+    ms_mixed = 1e3 * t_mixed / args.steps
+    value = N * args.steps / t_mixed
+    log("[bench] mixed: %.3f ms/step, %.3e particle-steps/s" % (ms_mixed, value))
+
+    kernels = {}
+    roofline = None
+    if not args.no_profile:
+        cm.profile_enable(True)
+        cm.track(max(3, min(args.steps, 10)))
+        prof = cm.profile_read()
+        cm.profile_enable(False)
+        ab = algorithmic_bytes(cfg, 4)
+        for name, (ms, nl) in prof.items():
+            if nl == 0:
+                continue
+            avg = ms / nl
+            k = {"avg_ms": avg, "launches": nl}
+            if name in ab:
+                k["algorithmic_GBs"] = ab[name] / (avg * 1e-3) / 1e9
+                k["frac_of_peak"] = k["algorithmic_GBs"] / HBM_PEAK_GBS
+            kernels[name] = k
+        dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+        a = kernels[dom]["algorithmic_GBs"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": a, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": a / HBM_PEAK_GBS, "traffic": None,
+                    "bytes_per_launch": ab[dom], "avg_ms": kernels[dom]["avg_ms"]}
+        log("[bench] kernels: " + json.dumps(kernels))
+
+    extra = {}
+    if not args.no_fp64:
+        log("[bench] FP64 step on the same bunch")
+        cd = make_ctx(sc, cfg, sc.FP64, args.sort_interval)
+        cm2 = make_ctx(sc, cfg, sc.MIXED, args.sort_interval)
+        # one step from the identical bunch: rho is bit-identical (int64), so the
+        # E difference is the precision of the FP32 solve and gather alone
+        s64 = cd.track(1)
+        s32 = cm2.track(1)
+        E64 = cd.get_grid(sc.GRID_EFIELD)
+        E32 = cm2.get_grid(sc.GRID_EFIELD)
+        e_rel = float(np.linalg.norm(E32 - E64) / np.linalg.norm(E64))
+        t64, st64 = timed_run(cd, args.steps, args.warmup)
+        _, st32 = timed_run(cm2, args.steps, args.warmup)
+        emit_err = max(abs(a["eps_norm"][d] / b["eps_norm"][d] - 1)
+                       for a, b in zip(st32, st64) for d in range(3))
+        ms64 = 1e3 * t64 / args.steps
+        extra = {"fp64_ms_per_step": ms64, "fp64_particle_steps_per_s": N * args.steps / t64,
+                 "mixed_over_fp64_speedup": ms64 / ms_mixed, "efield_rel_l2_mixed_vs_fp64": e_rel,
+                 "emittance_max_rel_err_mixed_vs_fp64": emit_err,
+                 "emittance_steps_compared": len(st64) + 1}
+        log("[bench] fp64: %.3f ms/step; speedup %.2fx; E rel L2 %.2e; emittance %.2e" %
+            (ms64, ms64 / ms_mixed, e_rel, emit_err))
+        del s64, s32
+        cd.close()
+        cm2.close()
+    cm.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle sample)")
+        cpu = cpu_baseline()
+        log("[bench] cpu: %.3e particle-steps/s" % cpu["value"])
+
+    out = {
+        "metric": "particle-steps/s @1e8 ptcl, 256^3 grid: mixed vs FP64 at field err<1e-5",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_mixed, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid / f64 particles",
+        "data": "synthetic (on-device seeded Gaussian bunch)",
+        "config": {"workload": "%s: %d macroparticles, %d^3 grid (%d^3 doubled FFT), %g MeV "
+                               "protons, %g nC, halo %g, sort every %d steps" %
+                               (args.config, N, n, 2 * n, cfg["ekin"], cfg["q"] * 1e9,
+                                cfg["halo"], args.sort_interval),
+                   "particles": N, "grid": [n, n, n], "precision": "mixed"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -290,13 +290,192 @@ __global__ __launch_bounds__(256) void k_deposit(int64_t N, const double *x, con
     if (nd) atomicAdd(dropped, nd);
 }
 
+// Sum the 8 corner contributions over runs of equal lgp key in consecutive
+// lanes (particles sorted by cell form long runs).  Returns true on the last
+// lane of each run, which then holds the run's sums.  Every lane of the
+// wave must call this.
+__device__ inline bool wave_run_sums(long long key, long long (&v)[8]) {
+    const int lane = threadIdx.x & 63;
+    const long long prev = __shfl_up(key, 1, 64);
+    const long long next = __shfl_down(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    const bool tail = lane == 63 || next != key;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __builtin_clzll(heads & upto);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        long long s = v[c];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long t = __shfl_up(s, off, 64);
+            if (lane >= off) s += t;
+        }
+        const long long before = __shfl(s, start > 0 ? start - 1 : 0, 64);
+        v[c] = start > 0 ? s - before : s;
+    }
+    return tail;
+}
+
+// LDS-tiled deposit.  A block owns a contiguous chunk of kChunk particles.
+// It bounds the chunk's cells with a block min/max of the positions; if the
+// target box fits kBoxCap cells, contributions are accumulated in an LDS copy
+// of the box (64-bit LDS atomics) and the non-zero cells are flushed to HBM
+// with one global atomic each; otherwise they go straight to HBM.  Before
+// either, each wave sums runs of equal cells (wave_run_sums), so a bunch
+// sorted by cell issues one atomic per run and corner instead of one per
+// particle.  Integer addition makes the result independent of all of this.
+constexpr int kPPT = 8;
+constexpr int kChunk = kThreads * kPPT;
+constexpr int kBoxCap = 5632;  // 44 KiB of int64
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x, const double *y,
+                                                     const double *z, double *q, const double *qdv,
+                                                     long long *rho, const Scalars *sc,
+                                                     unsigned long long *dropped) {
+    __shared__ long long box[kBoxCap];
+    __shared__ double red[4 * 6];
+    __shared__ int boxinfo[7];
+    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    const double e2q = sc->e2q, dT = sc->dT;
+    const int64_t base = (int64_t)blockIdx.x * kChunk;
+
+    // 1. bounding box of the chunk (positions, then a safe index margin)
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int k = 0; k < kPPT; ++k) {
+        const int64_t p = base + k * kThreads + threadIdx.x;
+        if (p < N) {
+            const double r[3] = {x[p], y[p], z[p]};
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = fmin(mn[d], r[d]);
+                mx[d] = fmax(mx[d], r[d]);
+            }
+        }
+    }
+    block_minmax(mn, mx, red);
+    if (threadIdx.x == 0) {
+        const double o[3] = {o0, o1, o2}, h[3] = {h0, h1, h2};
+        const int n[3] = {n0, n1, n2};
+        long long vol = 1;
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            const double tl = (mn[d] - o[d]) / h[d], th = (mx[d] - o[d]) / h[d];
+            if (!(tl > -1e9 && th < 1e9)) {
+                ok = false;
+                break;
+            }
+            int lo = (int)floor(tl) - 3, hi = (int)floor(th) + 3;
+            lo = lo < 0 ? 0 : lo;
+            hi = hi > n[d] - 1 ? n[d] - 1 : hi;
+            if (hi < lo) hi = lo;
+            boxinfo[d] = lo;
+            boxinfo[3 + d] = hi - lo + 1;
+            vol *= (hi - lo + 1);
+        }
+        boxinfo[6] = (ok && vol <= kBoxCap) ? (int)vol : -1;
+    }
+    __syncthreads();
+    const int vol = boxinfo[6];
+    const int bx0 = boxinfo[0], by0 = boxinfo[1], bz0 = boxinfo[2];
+    const int bx = boxinfo[3], by = boxinfo[4], bz = boxinfo[5];
+    const bool use_lds = vol > 0;
+    if (use_lds) {
+        for (int e = threadIdx.x; e < vol; e += kThreads) box[e] = 0;
+        __syncthreads();
+    }
+
+    // 2. exact CIC, run sums, atomics
+    unsigned long long nd = 0;
+    for (int k = 0; k < kPPT; ++k) {
+        const int64_t p = base + k * kThreads + threadIdx.x;
+        long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        long long key = -1 - (long long)threadIdx.x;  // unique: never merges
+        int l[3] = {0, 0, 0};
+        bool valid = false;
+        if (p < N) {
+            double qd;
+            if constexpr (MODE == 0) {
+                qd = qdv[p];
+            } else {
+                const double qv = q[p];
+                qd = qv * dT;
+                const double qn = qd / dT;
+                if (qn != qv) q[p] = qn;
+            }
+            double d[3];
+            valid = cic_dim(x[p], o0, h0, l[0], d[0]);
+            valid &= cic_dim(y[p], o1, h1, l[1], d[1]);
+            valid &= cic_dim(z[p], o2, h2, l[2], d[2]);
+            if (valid) {
+                // a stencil starting outside [-1, n-1] has no corner in the grid
+                const bool near = l[0] >= -1 && l[1] >= -1 && l[2] >= -1 && l[0] < n0 && l[1] < n1 && l[2] < n2;
+                if (near) {
+                    double w[8];
+                    cic_weights(d, w);
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) v[c] = (long long)rint(w[c] * qd * e2q);
+                    key = ((long long)(l[2] + 1) * (n1 + 1) + (l[1] + 1)) * (n0 + 1) + (l[0] + 1);
+                }
+            } else {
+                nd += 8;
+            }
+        }
+        const bool tail = wave_run_sums(key, v);
+        if (tail && valid) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), kk = l[2] + ((c >> 2) & 1);
+                if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
+                    if (v[c] == 0) continue;
+                    const int li = i - bx0, lj = j - by0, lk = kk - bz0;
+                    if (use_lds && li >= 0 && lj >= 0 && lk >= 0 && li < bx && lj < by && lk < bz)
+                        atomicAdd((unsigned long long *)&box[(lk * by + lj) * bx + li], (unsigned long long)v[c]);
+                    else
+                        atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], (unsigned long long)v[c]);
+                }
+            }
+        }
+        // out-of-grid corners of valid particles are dropped, counted per particle lane
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), kk = l[2] + ((c >> 2) & 1);
+                if (!(i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2)) ++nd;
+            }
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < vol; e += kThreads) {
+            const long long v = box[e];
+            if (v != 0) {
+                const int li = e % bx, lj = (e / bx) % by, lk = e / (bx * by);
+                atomicAdd((unsigned long long *)&rho[((size_t)(lk + bz0) * n1 + (lj + by0)) * n0 + (li + bx0)],
+                          (unsigned long long)v);
+            }
+        }
+    }
+    if (nd) atomicAdd(dropped, nd);
+}
+
 void launch_deposit(hipStream_t s, int mode, int64_t N, const double *x, const double *y,
                     const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
                     unsigned long long *dropped) {
     if (N <= 0) return;
-    int blocks = (int)std::min<int64_t>((N + kThreads - 1) / kThreads, 65536);
-    if (mode == 0) k_deposit<0><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
-    else k_deposit<1><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+    static const bool simple = getenv("OPALSC_DEPOSIT_SIMPLE") != nullptr;
+    if (simple) {
+        int blocks = (int)std::min<int64_t>((N + kThreads - 1) / kThreads, 65536);
+        if (mode == 0) k_deposit<0><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+        else k_deposit<1><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+        return;
+    }
+    const int64_t blocks = (N + kChunk - 1) / kChunk;
+    if (mode == 0) k_deposit_lds<0><<<(unsigned)blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+    else k_deposit_lds<1><<<(unsigned)blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
 }
 
 // ---- CIC gather ---------------------------------------------------------------

@@ -1,0 +1,78 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point that
+include/opalsc.h declares, validates arguments, and its correctly rounded
+atan/log (the same double-double code the IGF kernel runs) agree bit for bit
+with mpmath and with the oracle's binary128 evaluation."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+sc = pytest.importorskip("src_amd")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "opalsc.h")).read()
+    return sorted(set(re.findall(r"\b(opalsc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sc.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+    assert sorted(sc.EXPORTS) == names
+    assert L.opalsc_abi_version() == 1
+
+
+def test_create_validates_config_before_touching_a_device():
+    with pytest.raises(sc.OpalscError, match="precision"):
+        sc.SpaceChargeContext((32, 32, 32), precision=7)
+    with pytest.raises(sc.OpalscError, match="greens"):
+        sc.SpaceChargeContext((32, 32, 32), greens=5)
+    with pytest.raises(sc.OpalscError, match="max_particles"):
+        sc.SpaceChargeContext((32, 32, 32), max_particles=-1)
+
+
+def test_host_cr_math_equals_quad_oracle():
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.uniform(-5, 5, 100000), 10 ** rng.uniform(-30, 30, 100000),
+                        rng.uniform(0.999, 1.001, 20000)])
+    ya, yl = sc.host_atan_cr(x), sc.host_log_cr(np.abs(x))
+    L = O.lib()
+    L.orc_atan_cr.restype = L.orc_log_cr.restype = C.c_double
+    L.orc_atan_cr.argtypes = L.orc_log_cr.argtypes = [C.c_double]
+    qa = np.array([L.orc_atan_cr(float(v)) for v in x])
+    ql = np.array([L.orc_log_cr(float(abs(v))) for v in x])
+    assert np.array_equal(ya, qa)
+    assert np.array_equal(yl, ql)
+
+
+def test_host_cr_math_equals_mpmath_on_igf_arguments():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 160
+    rng = np.random.default_rng(22)
+    n, hr = 128, np.array([1.3e-4, 1.1e-4, 2.9e-4])
+    idx = rng.integers(0, n + 2, size=(3000, 3))
+    v = idx * hr - hr / 2
+    r = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+    a_args = np.concatenate([v[:, 0] * v[:, 1] / (v[:, 2] * r), v[:, 1] * v[:, 2] / (v[:, 0] * r)])
+    l_args = np.concatenate([v[:, 0] + r, v[:, 2] + r])
+    ya, yl = sc.host_atan_cr(a_args), sc.host_log_cr(l_args)
+    assert all(y == float(mpmath.atan(mpmath.mpf(float(a)))) for a, y in zip(a_args, ya))
+    assert all(y == float(mpmath.log(mpmath.mpf(float(a)))) for a, y in zip(l_args, yl))
+
+
+def test_special_values():
+    x = np.array([0.0, -0.0, np.inf, -np.inf, 1e-310, 1.0])
+    ya = sc.host_atan_cr(x)
+    assert ya[0] == 0.0 and ya[4] == 1e-310
+    assert ya[2] == np.arctan(np.inf) and ya[3] == -np.arctan(np.inf)
+    yl = sc.host_log_cr(np.array([1.0, 2.0, 1e-310, np.inf]))
+    assert yl[0] == 0.0 and yl[1] == np.log(2.0) and yl[3] == np.inf
+    assert yl[2] == float(np.log(np.float64(1e-310)))
