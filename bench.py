@@ -133,7 +133,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--sort-interval", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -157,8 +157,9 @@ def main():
     kernels = {}
     roofline = None
     if not args.no_profile:
+        # profile as many steps as the sort interval so the amortised re-sort is included
         cm.profile_enable(True)
-        cm.track(max(3, min(args.steps, 10)))
+        cm.track(max(args.sort_interval, 3))
         prof = cm.profile_read()
         cm.profile_enable(False)
         ab = algorithmic_bytes(cfg, 4)
@@ -177,6 +178,12 @@ def main():
                     "unit": "GB/s", "frac": a / HBM_PEAK_GBS, "traffic": None,
                     "bytes_per_launch": ab[dom], "avg_ms": kernels[dom]["avg_ms"]}
         log("[bench] kernels: " + json.dumps(kernels))
+    # achievable bandwidth of a plain copy kernel (also the byte-counter calibration)
+    copy_gbs = cm.copy_bandwidth(1 << 30, 20)
+    log("[bench] plain copy kernel: %.0f GB/s" % copy_gbs)
+    if roofline is not None:
+        roofline["copy_kernel_GBs"] = copy_gbs
+        roofline["frac_of_copy"] = roofline["achieved"] / copy_gbs
 
     extra = {}
     if not args.no_fp64:

@@ -125,10 +125,11 @@ int opalsc_boundp(opalsc_ctx *ctx, int64_t N, const double *R, double hr[3], dou
 int opalsc_cic_index(opalsc_ctx *ctx, int64_t N, const double *R, const double origin[3],
                      const double hr[3], int32_t *lgp, double *dpos);
 
-/* Stable sort of particles by the x-fastest linear index of their lgp cell:
- * perm[i] = original index of the i-th particle in cell order.  No
- * counterpart in the reference (its scatter is a serial loop); it is the
- * order the device scatter and gather run in. */
+/* Stable sort of particles by the cell of their lgp: perm[i] = original index
+ * of the i-th particle in cell order.  Cell order is the 3D Morton (Z-order)
+ * index of the lgp cell (bits of i, j, k interleaved, i lowest); particles
+ * outside the grid go last.  No counterpart in the reference (its scatter is a
+ * serial loop); it is the order the device scatter and gather run in. */
 int opalsc_sort_by_cell(opalsc_ctx *ctx, int64_t N, const double *R, const double origin[3],
                         const double hr[3], uint32_t *perm);
 
@@ -192,6 +193,12 @@ int opalsc_device_log_cr(opalsc_ctx *ctx, int64_t n, const double *x, double *y)
  * kernel classes (<= cap). */
 int opalsc_profile_enable(opalsc_ctx *ctx, int enable);
 int opalsc_profile_read(opalsc_ctx *ctx, int cap, const char **names, double *ms, int64_t *launches);
+
+/* Achievable HBM bandwidth on this device: a plain 16-byte-vector copy kernel
+ * (k_copy_calibration) over two buffers of `bytes` each, `reps` times; *gbs
+ * = 2*bytes per launch / average launch time.  No counterpart in the
+ * reference: it calibrates roofline fractions and hardware byte counters. */
+int opalsc_copy_bandwidth(opalsc_ctx *ctx, size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

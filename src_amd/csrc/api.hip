@@ -327,7 +327,7 @@ extern "C" int opalsc_cic_index(opalsc_ctx *c, int64_t N, const double *R, const
 }
 
 static int key_bits(const int n[3]) {
-    uint64_t m = (uint64_t)n[0] * n[1] * n[2] + 1;
+    uint64_t m = (uint64_t)cell_key_count(n[0], n[1], n[2]) + 1;
     int b = 1;
     while (b < 32 && (1ull << b) < m) ++b;
     return b;
@@ -727,6 +727,35 @@ extern "C" int opalsc_profile_enable(opalsc_ctx *c, int enable) {
         c->prof.ms[i] = 0.0;
         c->prof.launches[i] = 0;
     }
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_copy_bandwidth(opalsc_ctx *c, size_t bytes, int reps, double *gbs) {
+    if (!c || !gbs || reps <= 0 || bytes < 16) return OPALSC_EINVAL;
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) FAIL(c, OPALSC_ENOMEM, "copy buffer allocation failed");
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        FAIL(c, OPALSC_ENOMEM, "copy buffer allocation failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 1, bytes, c->stream);
+    launch_copy_calibration(c->stream, bytes, a, b);  // warm-up
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch_copy_calibration(c->stream, bytes, a, b);
+    (void)hipEventRecord(e1, c->stream);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) FAIL(c, OPALSC_EHIP, "copy calibration failed");
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
     return OPALSC_OK;
 }
 

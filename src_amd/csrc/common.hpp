@@ -15,7 +15,7 @@ namespace opalsc {
 constexpr double kC = 299792458.0;               // src/Classic/Physics/Physics.h:52
 constexpr double kPi = 3.14159265358979323846;   // src/Classic/Physics/Physics.h:31
 constexpr int kThreads = 256;                    // threads per block for particle kernels
-constexpr int kRedBlocksMax = 2048;              // reduction partial slots
+constexpr int kRedBlocksMax = 8192;              // reduction partial slots (blocks of the streaming kernels)
 constexpr int kMomentSlots = 32;                 // 27 moments + gamma sum + spare
 
 template <typename T> struct vec4_of;
@@ -78,6 +78,32 @@ __host__ __device__ inline bool cic_dim(double x, double o, double h, int &lgp, 
     lgp = ngp + off;
     dpos = (x - g) / delta;
     return true;
+}
+
+// Sort order of cells: the 3D Morton (Z-order) index, bits of i, j, k
+// interleaved (i lowest).  Any run of consecutive cells in this order spans a
+// box of about its own volume in every dimension, so a chunk of particles
+// sorted this way has a compact bounding box at every particle density (the
+// deposit's LDS box) and a compact cache footprint (the gather).
+// Cells up to 1024 per dimension; keys are < cell_key_count(n) = (2^b)^3 with
+// 2^b >= max(n).
+__host__ __device__ inline uint32_t morton_spread(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__host__ __device__ inline uint32_t cell_key(int i, int j, int k, int, int) {
+    return morton_spread((uint32_t)i) | (morton_spread((uint32_t)j) << 1) | (morton_spread((uint32_t)k) << 2);
+}
+__host__ __device__ inline uint32_t cell_key_count(int n0, int n1, int n2) {
+    int m = n0 > n1 ? n0 : n1;
+    m = m > n2 ? m : n2;
+    uint32_t b = 1;
+    while ((1u << b) < (uint32_t)m) ++b;
+    return 1u << (3 * b);
 }
 
 // CIC corner weights in the reference order 000,100,010,110,001,101,011,111,

@@ -121,6 +121,7 @@ void launch_generate(hipStream_t s, int64_t N, uint64_t seed, const double sr[3]
 void launch_absmax(hipStream_t s, int64_t N, const double *v, unsigned long long *out_bits);
 void launch_iota(hipStream_t s, int64_t N, uint32_t *v);
 void launch_fill_i64(hipStream_t s, size_t n, long long *p);
+void launch_copy_calibration(hipStream_t s, size_t bytes, const void *src, void *dst);
 void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y);
 
 // Tracker kernels

@@ -16,7 +16,8 @@ namespace opalsc {
 using crm::dd;
 using crm::dd_add;
 
-constexpr int kSlots = 64;  // doubles per block in the partials buffer
+constexpr int kSlots = 64;           // doubles per block in the partials buffer
+constexpr int kReduceThreads = 512;   // single-block final reductions of the partials
 
 int red_blocks(int64_t N) {
     int64_t b = (N + kThreads - 1) / kThreads;
@@ -203,13 +204,13 @@ __device__ inline void selffield_scalars(Scalars *sc, double psum_z, int64_t N, 
     sc->betaC = sqrt(gammaz * gammaz - 1.0) / gammaz / kC;
 }
 
-__global__ __launch_bounds__(256) void k_selffield_scalars(const double *partials, int nb, int64_t N,
+__global__ __launch_bounds__(512) void k_selffield_scalars(const double *partials, int nb, int64_t N,
                                                            double dT, double o0, double o1, double o2,
                                                            double h0, double h1, double h2, int n0,
                                                            int n1, int n2, double coupling,
                                                            const unsigned long long *qdmax_bits,
                                                            Scalars *sc) {
-    __shared__ double lds[4 * 2 * 2];
+    __shared__ double lds[16 * 2 * 2];
     double out[1];
     reduce_partials_dd<1>(partials, nb, out, lds);
     if (threadIdx.x != 0) return;
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(256) void k_selffield_scalars(const double *partial
 void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
                               const double origin[3], const double hr[3], const int n[3],
                               double coupling, const unsigned long long *qdmax_bits, Scalars *sc) {
-    k_selffield_scalars<<<1, kThreads, 0, s>>>(partials, nb, N, dT, origin[0], origin[1], origin[2],
+    k_selffield_scalars<<<1, kReduceThreads, 0, s>>>(partials, nb, N, dT, origin[0], origin[1], origin[2],
                                                hr[0], hr[1], hr[2], n[0], n[1], n[2], coupling,
                                                qdmax_bits, sc);
 }
@@ -301,6 +302,7 @@ __device__ inline bool wave_run_sums(long long key, long long (&v)[8]) {
     const bool head = lane == 0 || prev != key;
     const bool tail = lane == 63 || next != key;
     const unsigned long long heads = __ballot(head);
+    if (heads == ~0ull) return true;  // no two neighbours share a cell: nothing to merge
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const int start = 63 - __builtin_clzll(heads & upto);
 #pragma unroll
@@ -363,12 +365,15 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x,
         long long vol = 1;
         bool ok = true;
         for (int d = 0; d < 3; ++d) {
+            // (x-o)/h is monotonic in x, so the chunk's extreme cells follow from its
+            // extreme positions: lgp is in [floor(t)-1, floor(t)] for t >= 0 and
+            // >= floor(t)-2 below the origin; corners reach lgp+1.
             const double tl = (mn[d] - o[d]) / h[d], th = (mx[d] - o[d]) / h[d];
             if (!(tl > -1e9 && th < 1e9)) {
                 ok = false;
                 break;
             }
-            int lo = (int)floor(tl) - 3, hi = (int)floor(th) + 3;
+            int lo = (int)floor(tl) - (tl < 0.0 ? 2 : 1), hi = (int)floor(th) + 1;
             lo = lo < 0 ? 0 : lo;
             hi = hi > n[d] - 1 ? n[d] - 1 : hi;
             if (hi < lo) hi = lo;
@@ -584,9 +589,9 @@ void launch_bounds(hipStream_t s, int64_t N, const double *R, double *partials) 
     if (N > 0) k_bounds<<<red_blocks(N), kThreads, 0, s>>>(N, R, partials);
 }
 
-__global__ __launch_bounds__(256) void k_boundp_scalars(const double *partials, int nb, int n0, int n1,
+__global__ __launch_bounds__(512) void k_boundp_scalars(const double *partials, int nb, int n0, int n1,
                                                         int n2, double dh, Scalars *sc) {
-    __shared__ double lds[4 * 6];
+    __shared__ double lds[16 * 6];
     double mn[3], mx[3];
     reduce_partials_minmax(partials, nb, mn, mx, lds);
     if (threadIdx.x != 0) return;
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(256) void k_boundp_scalars(const double *partials, 
 
 void launch_boundp_scalars(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                            Scalars *sc) {
-    k_boundp_scalars<<<1, kThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, sc);
+    k_boundp_scalars<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, sc);
 }
 
 __global__ void k_push(int64_t N, double *R, const double *P, const double *dt) {
@@ -687,9 +692,9 @@ void launch_moments_soa(hipStream_t s, const Bunch &b, double *partials) {
     if (b.N > 0) k_moments_soa<<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, partials);
 }
 
-__global__ __launch_bounds__(256) void k_stats_from_partials(const double *partials, int nb, int64_t N,
+__global__ __launch_bounds__(512) void k_stats_from_partials(const double *partials, int nb, int64_t N,
                                                              double mass_eV, StatsOut *out) {
-    __shared__ double lds[4 * 28 * 2];
+    __shared__ double lds[16 * 28 * 2];
     double m[28];
     reduce_partials_dd<28>(partials, nb, m, lds);
     if (threadIdx.x != 0) return;
@@ -702,7 +707,7 @@ __global__ __launch_bounds__(256) void k_stats_from_partials(const double *parti
 }
 void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, int64_t N,
                                 double mass_eV, void *stats_out) {
-    k_stats_from_partials<<<1, kThreads, 0, s>>>(partials, nb, N, mass_eV, (StatsOut *)stats_out);
+    k_stats_from_partials<<<1, kReduceThreads, 0, s>>>(partials, nb, N, mass_eV, (StatsOut *)stats_out);
 }
 
 // ---- sorting support ------------------------------------------------------------
@@ -711,7 +716,7 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
-    const uint32_t out = (uint32_t)n0 * n1 * n2;
+    const uint32_t out = cell_key_count(n0, n1, n2);
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
         int l[3];
         double d[3];
@@ -720,7 +725,7 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
         ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
         uint32_t key = out;
         if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
-            key = ((uint32_t)l[2] * n1 + l[1]) * n0 + l[0];
+            key = cell_key(l[0], l[1], l[2], n0, n1);
         keys[p] = key;
         vals[p] = (uint32_t)p;
     }
@@ -750,6 +755,17 @@ __global__ void k_iota(int64_t N, uint32_t *v) {
 }
 void launch_iota(hipStream_t s, int64_t N, uint32_t *v) {
     if (N > 0) k_iota<<<stream_blocks(N), kThreads, 0, s>>>(N, v);
+}
+
+__global__ __launch_bounds__(256) void k_copy_calibration(size_t n16, const uint4 *__restrict__ src,
+                                                          uint4 *__restrict__ dst) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+void launch_copy_calibration(hipStream_t s, size_t bytes, const void *src, void *dst) {
+    const size_t n16 = bytes / 16;
+    int blocks = (int)std::min<size_t>((n16 + 255) / 256, 8192);
+    k_copy_calibration<<<blocks, kThreads, 0, s>>>(n16, (const uint4 *)src, (uint4 *)dst);
 }
 
 __global__ void k_fill_i64(size_t n, long long *p) {
@@ -915,9 +931,9 @@ void launch_track_alpha(hipStream_t s, Bunch &b, const Scalars *sc, const TrackS
         k_track_alpha<<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts, partials);
 }
 
-__global__ __launch_bounds__(256) void k_track_mesh(const double *partials, int nb, int n0, int n1,
+__global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int nb, int n0, int n1,
                                                     int n2, double dh, const TrackState *ts, Scalars *sc) {
-    __shared__ double lds[4 * 6];
+    __shared__ double lds[16 * 6];
     double mn[3], mx[3];
     reduce_partials_minmax(partials, nb, mn, mx, lds);
     if (threadIdx.x != 0) return;
@@ -935,7 +951,7 @@ __global__ __launch_bounds__(256) void k_track_mesh(const double *partials, int 
 }
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, Scalars *sc) {
-    k_track_mesh<<<1, kThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, sc);
+    k_track_mesh<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, sc);
 }
 
 // Gather + B from E (PartBunch.cpp:632-644), rotation back to the reference
@@ -1024,10 +1040,10 @@ void launch_track_gamma(hipStream_t s, bool fp64, Bunch &b, const void *Eg, cons
 
 // Step statistics (DataSink / calcBeamParameters) and the momentum sums the
 // next step's rotation and Lorentz factor need.
-__global__ __launch_bounds__(256) void k_track_post(const double *partials, int nb, TrackState *ts,
+__global__ __launch_bounds__(512) void k_track_post(const double *partials, int nb, TrackState *ts,
                                                     Scalars *sc, StatsOut *out,
                                                     const unsigned long long *dropped) {
-    __shared__ double lds[4 * 28 * 2];
+    __shared__ double lds[16 * 28 * 2];
     double m[28];
     reduce_partials_dd<28>(partials, nb, m, lds);
     if (threadIdx.x != 0) return;
@@ -1045,11 +1061,11 @@ __global__ __launch_bounds__(256) void k_track_post(const double *partials, int 
 }
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped) {
-    k_track_post<<<1, kThreads, 0, s>>>(partials, nb, ts, sc, (StatsOut *)stats_out, dropped);
+    k_track_post<<<1, kReduceThreads, 0, s>>>(partials, nb, ts, sc, (StatsOut *)stats_out, dropped);
 }
 void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,
                                Scalars *sc) {
-    k_track_post<<<1, kThreads, 0, s>>>(partials, nb, ts, sc, nullptr, nullptr);
+    k_track_post<<<1, kReduceThreads, 0, s>>>(partials, nb, ts, sc, nullptr, nullptr);
 }
 
 }  // namespace opalsc

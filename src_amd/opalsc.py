@@ -38,6 +38,7 @@ EXPORTS = [
     "opalsc_bunch_get", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
     "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
+    "opalsc_copy_bandwidth",
 ]
 
 
@@ -114,6 +115,7 @@ def lib():
         "opalsc_device_log_cr": ([vp, i64, dp, dp], C.c_int),
         "opalsc_profile_enable": ([vp, C.c_int], C.c_int),
         "opalsc_profile_read": ([vp, C.c_int, C.POINTER(C.c_char_p), dp, dp], C.c_int),
+        "opalsc_copy_bandwidth": ([vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -327,6 +329,11 @@ class SpaceChargeContext:
 
     def profile_enable(self, on=True):
         self._c(lib().opalsc_profile_enable(self._h, int(on)))
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        g = C.c_double()
+        self._c(lib().opalsc_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(g)))
+        return g.value
 
     def profile_read(self):
         names = (C.c_char_p * 32)()

@@ -56,7 +56,10 @@ def test_sort_by_cell_is_stable_and_bit_exact():
     R, P, Q = gaussian_bunch(100000, seed=2)
     hr, org = O.boundp(R, n)
     lgp, _, _ = O.cic_index(R, org, hr)
-    key = (lgp[:, 2].astype(np.int64) * n[1] + lgp[:, 1]) * n[0] + lgp[:, 0]
+    key = np.zeros(len(lgp), np.int64)
+    for b in range(10):  # 3D Morton order, i in the lowest bit of each triple
+        for d in range(3):
+            key |= ((lgp[:, d].astype(np.int64) >> b) & 1) << (3 * b + d)
     with ctx(n) as c:
         perm = c.sort_by_cell(R, org, hr)
     assert np.array_equal(perm.astype(np.int64), np.argsort(key, kind="stable"))
