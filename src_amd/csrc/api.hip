@@ -147,11 +147,15 @@ static int alloc_bunch(opalsc_ctx *c, int64_t cap) {
     b.sort_tmp_bytes = sort_temp_bytes(std::max<int64_t>(cap, 1));
     if (hipMalloc(&b.sort_tmp, std::max<size_t>(b.sort_tmp_bytes, 16)) != hipSuccess)
         FAIL(c, OPALSC_ENOMEM, "hipMalloc of sort scratch failed");
+    const size_t nchunks = (size_t)(std::max<int64_t>(cap, 1) + 2047) / 2048;
+    if (hipMalloc((void **)&b.chunkbox, nchunks * 6 * sizeof(double)) != hipSuccess)
+        FAIL(c, OPALSC_ENOMEM, "hipMalloc of chunk boxes failed");
     return 0;
 }
 
 static void free_bunch(Bunch &b) {
-    void *ps[] = {b.x, b.y, b.z, b.px, b.py, b.pz, b.q, b.tmp, b.qd, b.id, b.keys, b.vals, b.keys2, b.vals2, b.sort_tmp};
+    void *ps[] = {b.x, b.y, b.z, b.px, b.py, b.pz, b.q, b.tmp, b.qd, b.id, b.keys, b.vals, b.keys2, b.vals2, b.sort_tmp,
+                  b.chunkbox};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     b = Bunch();
@@ -382,10 +386,10 @@ extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double
     launch_selffield_prep(c->stream, N, c->aosP.as<double>(), b.q, c->dtb.as<double>(), b.qd, c->partials,
                           &c->d_counters[0]);
     launch_selffield_scalars(c->stream, c->partials, red_blocks(N), N, dT, origin, hr, c->cfg.n, coupling,
-                             &c->d_counters[0], c->d_sc);
+                             &c->d_counters[0], c->grid.fp64, c->d_sc);
     launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
-    launch_deposit(c->stream, 0, N, b.x, b.y, b.z, nullptr, b.qd, (long long *)c->grid.rho_i64, c->d_sc,
-                   &c->d_counters[1]);
+    launch_deposit(c->stream, 0, c->grid.fp64, N, b.x, b.y, b.z, nullptr, b.qd, (long long *)c->grid.rho_i64,
+                   c->d_sc, &c->d_counters[1], nullptr);
     c->grid.green(c->d_sc, c->grid.fp64);
     c->grid.solve(c->d_sc, 0, 1);
     c->grid.grad(c->d_sc);
@@ -611,25 +615,28 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps)) FAIL(c, OPALSC_ENOMEM, "stats buffer");
     Bunch &b = c->bunch;
     const int nb = red_blocks(b.N);
-    const bool cr = c->grid.fp64;
+    const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
+    const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     c->prof.start();
     for (int s = 0; s < nsteps; ++s) {
         launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N);
         c->prof.mark(PROF_SCALAR);
-        launch_track_alpha(c->stream, b, c->d_sc, c->d_ts, c->partials);
+        launch_track_alpha(c->stream, exact, b, c->d_sc, c->d_ts, c->partials);
         c->prof.mark(PROF_PUSH_BOUNDS);
-        launch_track_mesh(c->stream, c->partials, nb, c->cfg.n, c->dh, c->d_ts, c->d_sc);
+        launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, c->d_sc);
         c->prof.mark(PROF_SCALAR);
         const int interval = c->cfg.sort_interval;
+        bool sorted_now = false;
         if (interval > 0 && (c->steps_done % interval) == 0) {
             sort_bunch(c);
+            sorted_now = true;  // the push pass's chunk boxes no longer match the particle order
             c->prof.mark(PROF_SORT);
         }
         launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
         HIPCHK(c, hipMemsetAsync(&c->d_counters[1], 0, sizeof(unsigned long long), c->stream));
         c->prof.mark(PROF_ZERO);
-        launch_deposit(c->stream, 1, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
-                       c->d_sc, &c->d_counters[1]);
+        launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
+                       c->d_sc, &c->d_counters[1], sorted_now ? nullptr : b.chunkbox);
         c->prof.mark(PROF_DEPOSIT);
         c->grid.green(c->d_sc, cr);
         c->grid.solve(c->d_sc, 0, 1);

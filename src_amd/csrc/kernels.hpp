@@ -81,6 +81,7 @@ struct Bunch {
     uint32_t *id = nullptr;
     double *tmp = nullptr;       // permutation scratch
     double *qd = nullptr;        // Q*dt (self-field entry point)
+    double *chunkbox = nullptr;  // per-chunk beam-frame bounding boxes (6 doubles per 2048 particles)
     uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -94,10 +95,11 @@ void launch_selffield_prep(hipStream_t s, int64_t N, const double *P, double *Q,
                            double *qd, double *partials, unsigned long long *qdmax_bits);
 void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
                               const double origin[3], const double hr[3], const int n[3],
-                              double coupling, const unsigned long long *qdmax_bits, Scalars *sc);
-void launch_deposit(hipStream_t s, int mode, int64_t N, const double *x, const double *y,
+                              double coupling, const unsigned long long *qdmax_bits, bool exact,
+                              Scalars *sc);
+void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double *x, const double *y,
                     const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
-                    unsigned long long *dropped);
+                    unsigned long long *dropped, const double *chunkbox);
 void launch_gather_selffield(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y,
                              const double *z, const void *Eg, const Scalars *sc, double *Ef,
                              double *Bf);
@@ -126,10 +128,11 @@ void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double
 
 // Tracker kernels
 void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N);
-void launch_track_alpha(hipStream_t s, Bunch &b, const Scalars *sc, const TrackState *ts,
+int alpha_blocks(int64_t N);
+void launch_track_alpha(hipStream_t s, bool exact, Bunch &b, const Scalars *sc, const TrackState *ts,
                         double *partials);
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
-                       const TrackState *ts, Scalars *sc);
+                       const TrackState *ts, bool exact, Scalars *sc);
 void launch_track_gamma(hipStream_t s, bool fp64, Bunch &b, const void *Eg, const Scalars *sc,
                         const TrackState *ts, double *partials);
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,

@@ -168,15 +168,17 @@ void launch_selffield_prep(hipStream_t s, int64_t N, const double *P, double *Q,
     k_selffield_prep<<<red_blocks(N), kThreads, 0, s>>>(N, P, Q, dt, qd, partials, qdmax_bits);
 }
 
-// Fixed-point exponent: every cell sum is bounded by N*max|Q*dt|; scaling by
-// 2^e with N*max|qd|*2^e < 2^62 leaves headroom below 2^63 for rounding.
-__device__ inline void set_fixed_point(Scalars *sc, int64_t N, double qdmax) {
-    double B = (double)N * qdmax * (1.0 + 0x1p-40);
+// Fixed-point exponent.  FP64 path: every cell sum is bounded by N*max|Q*dt|;
+// scaling by 2^e with N*max|qd|*2^e < 2^62 leaves headroom below 2^63 for
+// rounding.  Mixed path: one deposit is below 2^24 (|qd|*2^e < 2^24), so 64
+// deposits sum in 32 bits and any cell sum stays far below 2^63.
+__device__ inline void set_fixed_point(Scalars *sc, int64_t N, double qdmax, bool exact) {
+    double B = (exact ? (double)N : 1.0) * qdmax * (1.0 + 0x1p-40);
     int e = 0;
     if (B > 0.0) {
         int E;
         (void)frexp(B, &E);
-        e = 62 - E;
+        e = (exact ? 62 : 24) - E;
         if (e > 1000) e = 1000;
     }
     sc->e2q = ldexp(1.0, e);
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(512) void k_selffield_scalars(const double *partial
                                                            double h0, double h1, double h2, int n0,
                                                            int n1, int n2, double coupling,
                                                            const unsigned long long *qdmax_bits,
-                                                           Scalars *sc) {
+                                                           int exact, Scalars *sc) {
     __shared__ double lds[16 * 2 * 2];
     double out[1];
     reduce_partials_dd<1>(partials, nb, out, lds);
@@ -227,15 +229,16 @@ __global__ __launch_bounds__(512) void k_selffield_scalars(const double *partial
     sc->mesh_ok = 1;
     sc->psum[2] = out[0];
     selffield_scalars(sc, out[0], N, dT, coupling);
-    set_fixed_point(sc, N, __longlong_as_double((long long)*qdmax_bits));
+    set_fixed_point(sc, N, __longlong_as_double((long long)*qdmax_bits), exact != 0);
 }
 
 void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
                               const double origin[3], const double hr[3], const int n[3],
-                              double coupling, const unsigned long long *qdmax_bits, Scalars *sc) {
+                              double coupling, const unsigned long long *qdmax_bits, bool exact,
+                              Scalars *sc) {
     k_selffield_scalars<<<1, kReduceThreads, 0, s>>>(partials, nb, N, dT, origin[0], origin[1], origin[2],
                                                hr[0], hr[1], hr[2], n[0], n[1], n[2], coupling,
-                                               qdmax_bits, sc);
+                                               qdmax_bits, exact ? 1 : 0, sc);
 }
 
 // ---- CIC scatter into int64 fixed point -------------------------------------
@@ -295,10 +298,45 @@ __global__ __launch_bounds__(256) void k_deposit(int64_t N, const double *x, con
 // lanes (particles sorted by cell form long runs).  Returns true on the last
 // lane of each run, which then holds the run's sums.  Every lane of the
 // wave must call this.
-__device__ inline bool wave_run_sums(long long key, long long (&v)[8]) {
+// Wave-wide inclusive prefix sum with DPP lane moves (no LDS traffic): four
+// row shifts build prefix sums inside each 16-lane row, two row broadcasts
+// carry the row totals forward (the GFX9 wave64 scan).  Lanes shifted in from
+// outside a row read 0.
+__device__ inline int dpp_shr(int v, int ctrl_sel) {
+    switch (ctrl_sel) {
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    case 4: return __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    case 8: return __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    case 15: return __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1,3
+    default: return __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2,3
+    }
+}
+__device__ inline int wave_incl_scan(int v) {
+    v += dpp_shr(v, 1);
+    v += dpp_shr(v, 2);
+    v += dpp_shr(v, 4);
+    v += dpp_shr(v, 8);
+    v += dpp_shr(v, 15);
+    v += dpp_shr(v, 31);
+    return v;
+}
+__device__ inline long long wave_incl_scan(long long v) {
+    const int steps[6] = {1, 2, 4, 8, 15, 31};
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const unsigned lo = (unsigned)dpp_shr((int)(unsigned)v, steps[s]);
+        const unsigned hi = (unsigned)dpp_shr((int)(unsigned)((unsigned long long)v >> 32), steps[s]);
+        v += (long long)(((unsigned long long)hi << 32) | lo);
+    }
+    return v;
+}
+
+template <typename V>
+__device__ inline bool wave_run_sums(int key, V (&v)[8]) {
     const int lane = threadIdx.x & 63;
-    const long long prev = __shfl_up(key, 1, 64);
-    const long long next = __shfl_down(key, 1, 64);
+    const int prev = __shfl_up(key, 1, 64);
+    const int next = __shfl_down(key, 1, 64);
     const bool head = lane == 0 || prev != key;
     const bool tail = lane == 63 || next != key;
     const unsigned long long heads = __ballot(head);
@@ -307,13 +345,13 @@ __device__ inline bool wave_run_sums(long long key, long long (&v)[8]) {
     const int start = 63 - __builtin_clzll(heads & upto);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        long long s = v[c];
+        V s = v[c];
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            const long long t = __shfl_up(s, off, 64);
+            const V t = __shfl_up(s, off, 64);
             if (lane >= off) s += t;
         }
-        const long long before = __shfl(s, start > 0 ? start - 1 : 0, 64);
+        const V before = __shfl(s, start > 0 ? start - 1 : 0, 64);
         v[c] = start > 0 ? s - before : s;
     }
     return tail;
@@ -331,16 +369,23 @@ constexpr int kPPT = 8;
 constexpr int kChunk = kThreads * kPPT;
 constexpr int kBoxCap = 5632;  // 44 KiB of int64
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x, const double *y,
-                                                     const double *z, double *q, const double *qdv,
-                                                     long long *rho, const Scalars *sc,
-                                                     unsigned long long *dropped) {
+// MODE 0: qd given; 1: qd = Q*dT with Q left as (Q*dT)/dT.  EXACT: the FP64
+// path (reference-order CIC, 64-bit fixed point per deposit); otherwise the
+// mixed path (CIC with 1/h, FP32 weights, |deposit| < 2^24 in 32-bit integers,
+// summed in 64 bits).  BOX: chunk bounding boxes supplied by the push pass.
+template <int MODE, bool EXACT, bool BOX>
+__global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     const double *__restrict__ z, double *q,
+                                                     const double *qdv, long long *rho,
+                                                     const Scalars *sc, unsigned long long *dropped,
+                                                     const double *chunkbox) {
     __shared__ long long box[kBoxCap];
     __shared__ double red[4 * 6];
     __shared__ int boxinfo[7];
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double e2q = sc->e2q, dT = sc->dT;
     const int64_t base = (int64_t)blockIdx.x * kChunk;
@@ -348,17 +393,24 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x,
     // 1. bounding box of the chunk (positions, then a safe index margin)
     double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
-    for (int k = 0; k < kPPT; ++k) {
-        const int64_t p = base + k * kThreads + threadIdx.x;
-        if (p < N) {
-            const double r[3] = {x[p], y[p], z[p]};
-            for (int d = 0; d < 3; ++d) {
-                mn[d] = fmin(mn[d], r[d]);
-                mx[d] = fmax(mx[d], r[d]);
+    if constexpr (BOX) {
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = chunkbox[blockIdx.x * 6 + d];
+            mx[d] = chunkbox[blockIdx.x * 6 + 3 + d];
+        }
+    } else {
+        for (int k = 0; k < kPPT; ++k) {
+            const int64_t p = base + k * kThreads + threadIdx.x;
+            if (p < N) {
+                const double r[3] = {x[p], y[p], z[p]};
+                for (int d = 0; d < 3; ++d) {
+                    mn[d] = fmin(mn[d], r[d]);
+                    mx[d] = fmax(mx[d], r[d]);
+                }
             }
         }
+        block_minmax(mn, mx, red);
     }
-    block_minmax(mn, mx, red);
     if (threadIdx.x == 0) {
         const double o[3] = {o0, o1, o2}, h[3] = {h0, h1, h2};
         const int n[3] = {n0, n1, n2};
@@ -393,12 +445,13 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x,
         __syncthreads();
     }
 
-    // 2. exact CIC, run sums, atomics
+    // 2. CIC, run sums, atomics
+    using V = typename std::conditional<EXACT, long long, int>::type;
     unsigned long long nd = 0;
     for (int k = 0; k < kPPT; ++k) {
         const int64_t p = base + k * kThreads + threadIdx.x;
-        long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        long long key = -1 - (long long)threadIdx.x;  // unique: never merges
+        V v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int key = -1 - (int)threadIdx.x;  // unique: never merges
         int l[3] = {0, 0, 0};
         bool valid = false;
         if (p < N) {
@@ -412,24 +465,45 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x,
                 if (qn != qv) q[p] = qn;
             }
             double d[3];
-            valid = cic_dim(x[p], o0, h0, l[0], d[0]);
-            valid &= cic_dim(y[p], o1, h1, l[1], d[1]);
-            valid &= cic_dim(z[p], o2, h2, l[2], d[2]);
+            if constexpr (EXACT) {
+                valid = cic_dim(x[p], o0, h0, l[0], d[0]);
+                valid &= cic_dim(y[p], o1, h1, l[1], d[1]);
+                valid &= cic_dim(z[p], o2, h2, l[2], d[2]);
+            } else {
+                valid = cic_dim_fast(x[p], o0, ih0, l[0], d[0]);
+                valid &= cic_dim_fast(y[p], o1, ih1, l[1], d[1]);
+                valid &= cic_dim_fast(z[p], o2, ih2, l[2], d[2]);
+            }
             if (valid) {
                 // a stencil starting outside [-1, n-1] has no corner in the grid
                 const bool near = l[0] >= -1 && l[1] >= -1 && l[2] >= -1 && l[0] < n0 && l[1] < n1 && l[2] < n2;
                 if (near) {
-                    double w[8];
-                    cic_weights(d, w);
+                    if constexpr (EXACT) {
+                        double w[8];
+                        cic_weights(d, w);
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) v[c] = (long long)rint(w[c] * qd * e2q);
-                    key = ((long long)(l[2] + 1) * (n1 + 1) + (l[1] + 1)) * (n0 + 1) + (l[0] + 1);
+                        for (int c = 0; c < 8; ++c) v[c] = (long long)rint(w[c] * qd * e2q);
+                    } else {
+                        const float a0 = 1.0f - (float)d[0], a1 = 1.0f - (float)d[1], a2 = 1.0f - (float)d[2];
+                        const float b0 = (float)d[0], b1 = (float)d[1], b2 = (float)d[2];
+                        const float qs = (float)(qd * e2q);
+                        const float a01 = a0 * a1 * qs, b01 = b0 * a1 * qs, a0b1 = a0 * b1 * qs, b0b1 = b0 * b1 * qs;
+                        v[0] = (int)rintf(a01 * a2);
+                        v[1] = (int)rintf(b01 * a2);
+                        v[2] = (int)rintf(a0b1 * a2);
+                        v[3] = (int)rintf(b0b1 * a2);
+                        v[4] = (int)rintf(a01 * b2);
+                        v[5] = (int)rintf(b01 * b2);
+                        v[6] = (int)rintf(a0b1 * b2);
+                        v[7] = (int)rintf(b0b1 * b2);
+                    }
+                    key = ((l[2] + 1) * (n1 + 1) + (l[1] + 1)) * (n0 + 1) + (l[0] + 1);
                 }
             } else {
                 nd += 8;
             }
         }
-        const bool tail = wave_run_sums(key, v);
+        const bool tail = wave_run_sums<V>(key, v);
         if (tail && valid) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
@@ -437,10 +511,11 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x,
                 if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
                     if (v[c] == 0) continue;
                     const int li = i - bx0, lj = j - by0, lk = kk - bz0;
+                    const unsigned long long add = (unsigned long long)(long long)v[c];
                     if (use_lds && li >= 0 && lj >= 0 && lk >= 0 && li < bx && lj < by && lk < bz)
-                        atomicAdd((unsigned long long *)&box[(lk * by + lj) * bx + li], (unsigned long long)v[c]);
+                        atomicAdd((unsigned long long *)&box[(lk * by + lj) * bx + li], add);
                     else
-                        atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], (unsigned long long)v[c]);
+                        atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], add);
                 }
             }
         }
@@ -467,20 +542,32 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *x,
     if (nd) atomicAdd(dropped, nd);
 }
 
-void launch_deposit(hipStream_t s, int mode, int64_t N, const double *x, const double *y,
+void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double *x, const double *y,
                     const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
-                    unsigned long long *dropped) {
+                    unsigned long long *dropped, const double *chunkbox) {
     if (N <= 0) return;
     static const bool simple = getenv("OPALSC_DEPOSIT_SIMPLE") != nullptr;
-    if (simple) {
+    if (simple && exact) {
         int blocks = (int)std::min<int64_t>((N + kThreads - 1) / kThreads, 65536);
         if (mode == 0) k_deposit<0><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
         else k_deposit<1><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
         return;
     }
-    const int64_t blocks = (N + kChunk - 1) / kChunk;
-    if (mode == 0) k_deposit_lds<0><<<(unsigned)blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
-    else k_deposit_lds<1><<<(unsigned)blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
+    const unsigned blocks = (unsigned)((N + kChunk - 1) / kChunk);
+    const bool box = chunkbox != nullptr;
+#define OPALSC_DEP(M_, E_, B_) \
+    k_deposit_lds<M_, E_, B_><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped, chunkbox)
+    if (mode == 0) {
+        if (exact) OPALSC_DEP(0, true, false);
+        else OPALSC_DEP(0, false, false);
+    } else if (box) {
+        if (exact) OPALSC_DEP(1, true, true);
+        else OPALSC_DEP(1, false, true);
+    } else {
+        if (exact) OPALSC_DEP(1, true, false);
+        else OPALSC_DEP(1, false, false);
+    }
+#undef OPALSC_DEP
 }
 
 // ---- CIC gather ---------------------------------------------------------------
@@ -512,6 +599,43 @@ __device__ inline void gather_E(const typename vec4_of<T>::type *Eg, const int l
     Ef[0] = (double)a0;
     Ef[1] = (double)a1;
     Ef[2] = (double)a2;
+}
+
+// Mixed-precision gather: the same stencil with FP32 weights and FMAs.
+__device__ inline void gather_E_fast(const float4 *Eg, const int l[3], const double d[3], int n0, int n1,
+                                     int n2, double Ef[3]) {
+    const float b0 = (float)d[0], b1 = (float)d[1], b2 = (float)d[2];
+    const float a0 = 1.0f - b0, a1 = 1.0f - b1, a2 = 1.0f - b2;
+    const float w[8] = {a0 * a1 * a2, b0 * a1 * a2, a0 * b1 * a2, b0 * b1 * a2,
+                        a0 * a1 * b2, b0 * a1 * b2, a0 * b1 * b2, b0 * b1 * b2};
+    const bool inside = l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] + 1 < n0 && l[1] + 1 < n1 && l[2] + 1 < n2;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    if (inside) {
+        const float4 *base = Eg + ((size_t)l[2] * n1 + l[1]) * n0 + l[0];
+        const size_t sy = n0, sz = (size_t)n0 * n1;
+        const float4 c[8] = {base[0], base[1], base[sy], base[sy + 1],
+                             base[sz], base[sz + 1], base[sz + sy], base[sz + sy + 1]};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            e0 = fmaf(w[k], c[k].x, e0);
+            e1 = fmaf(w[k], c[k].y, e1);
+            e2 = fmaf(w[k], c[k].z, e2);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = l[0] + (k & 1), j = l[1] + ((k >> 1) & 1), kk = l[2] + ((k >> 2) & 1);
+            if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
+                const float4 c = Eg[((size_t)kk * n1 + j) * n0 + i];
+                e0 = fmaf(w[k], c.x, e0);
+                e1 = fmaf(w[k], c.y, e1);
+                e2 = fmaf(w[k], c.z, e2);
+            }
+        }
+    }
+    Ef[0] = e0;
+    Ef[1] = e1;
+    Ef[2] = e2;
 }
 
 template <typename T>
@@ -894,45 +1018,84 @@ void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N) {
 }
 
 // push (timeIntegration1) + rotation into the beam frame + bounds (boundp's get_bounds).
-__global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *x, double *y, double *z,
-                                                     const double *px, const double *py,
-                                                     const double *pz, const Scalars *sc,
-                                                     const TrackState *ts, double *partials) {
+// Each block walks contiguous chunks of kChunk particles (the deposit's
+// chunks), writes every chunk's beam-frame bounding box to chunkbox (so the
+// deposit reads positions once), and leaves its running bounds in partials.
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restrict__ x,
+                                                     double *__restrict__ y, double *__restrict__ z,
+                                                     const double *__restrict__ px,
+                                                     const double *__restrict__ py,
+                                                     const double *__restrict__ pz, const Scalars *sc,
+                                                     const TrackState *ts, double *partials,
+                                                     double *chunkbox) {
     __shared__ double lds[4 * 6];
     const double dT = ts->dT;
+    const double half_cdt = 0.5 * (kC * dT);
     const bool sc_on = ts->space_charge != 0;
     double M[9], o[3];
     for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
     for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
-    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
-    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-        double r[3] = {x[i], y[i], z[i]};
-        push_half(r[0], r[1], r[2], px[i], py[i], pz[i], dT);
-        if (sc_on) transform_to(M, o, r);
-        x[i] = r[0];
-        y[i] = r[1];
-        z[i] = r[2];
-        for (int d = 0; d < 3; ++d) {
-            mn[d] = fmin(mn[d], r[d]);
-            mx[d] = fmax(mx[d], r[d]);
+    double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    const int64_t nchunks = (N + kChunk - 1) / kChunk;
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+        double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+        const int64_t base = c * kChunk + threadIdx.x;
+#pragma unroll 2
+        for (int k = 0; k < kPPT; ++k) {
+            const int64_t i = base + k * kThreads;
+            if (i >= N) break;
+            double r[3] = {x[i], y[i], z[i]};
+            if constexpr (EXACT) {
+                push_half(r[0], r[1], r[2], px[i], py[i], pz[i], dT);
+                if (sc_on) transform_to(M, o, r);
+            } else {
+                push_half_fast(r[0], r[1], r[2], px[i], py[i], pz[i], half_cdt);
+                if (sc_on) transform_to_fast(M, o, r);
+            }
+            x[i] = r[0];
+            y[i] = r[1];
+            z[i] = r[2];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = fmin(mn[d], r[d]);
+                mx[d] = fmax(mx[d], r[d]);
+            }
         }
+        block_minmax(mn, mx, lds);
+        if (threadIdx.x == 0)
+            for (int d = 0; d < 3; ++d) {
+                chunkbox[c * 6 + d] = mn[d];
+                chunkbox[c * 6 + 3 + d] = mx[d];
+                bmn[d] = fmin(bmn[d], mn[d]);
+                bmx[d] = fmax(bmx[d], mx[d]);
+            }
     }
-    block_minmax(mn, mx, lds);
     if (threadIdx.x == 0)
         for (int d = 0; d < 3; ++d) {
-            partials[blockIdx.x * kSlots + d] = mn[d];
-            partials[blockIdx.x * kSlots + 3 + d] = mx[d];
+            partials[blockIdx.x * kSlots + d] = bmn[d];
+            partials[blockIdx.x * kSlots + 3 + d] = bmx[d];
         }
 }
-void launch_track_alpha(hipStream_t s, Bunch &b, const Scalars *sc, const TrackState *ts,
+int alpha_blocks(int64_t N) {
+    int64_t nchunks = (N + kChunk - 1) / kChunk;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, kRedBlocksMax));
+}
+void launch_track_alpha(hipStream_t s, bool exact, Bunch &b, const Scalars *sc, const TrackState *ts,
                         double *partials) {
-    if (b.N > 0)
-        k_track_alpha<<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts, partials);
+    if (b.N <= 0) return;
+    if (exact)
+        k_track_alpha<true><<<alpha_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts,
+                                                                   partials, b.chunkbox);
+    else
+        k_track_alpha<false><<<alpha_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts,
+                                                                    partials, b.chunkbox);
 }
 
 __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int nb, int n0, int n1,
-                                                    int n2, double dh, const TrackState *ts, Scalars *sc) {
+                                                    int n2, double dh, const TrackState *ts, int exact,
+                                                    Scalars *sc) {
     __shared__ double lds[16 * 6];
     double mn[3], mx[3];
     reduce_partials_minmax(partials, nb, mn, mx, lds);
@@ -947,11 +1110,11 @@ __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int 
     const int64_t N = sc->N;
     selffield_scalars(sc, sc->psum[2], N, ts->dT, ts->coupling);
     // every particle deposits |Q*dT| <= max|Q|*dT*(1+2^-52)
-    set_fixed_point(sc, N, sc->qd_absmax);
+    set_fixed_point(sc, N, sc->qd_absmax, exact != 0);
 }
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
-                       const TrackState *ts, Scalars *sc) {
-    k_track_mesh<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, sc);
+                       const TrackState *ts, bool exact, Scalars *sc) {
+    k_track_mesh<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, exact ? 1 : 0, sc);
 }
 
 // Gather + B from E (PartBunch.cpp:632-644), rotation back to the reference
@@ -982,6 +1145,9 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
         Ee[d] = sc->Eext[d];
         Be[d] = sc->Bext[d];
     }
+    const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
+    const double e_kick = 0.5 * dT * charge * kC / mass, b_kick = 0.5 * dT * charge * kC * kC / mass;
+    const double half_cdt = 0.5 * (kC * dT);
     double m[28];
     for (int k = 0; k < 28; ++k) m[k] = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
@@ -991,19 +1157,32 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
         if (sc_on) {
             int l[3];
             double d[3];
-            bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
-            ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
-            ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
-            if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, Ef);
+            if constexpr (sizeof(T) == 8) {
+                bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
+                ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
+                ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
+                if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, Ef);
+            } else {
+                bool ok = cic_dim_fast(r[0], o0, ih0, l[0], d[0]);
+                ok &= cic_dim_fast(r[1], o1, ih1, l[1], d[1]);
+                ok &= cic_dim_fast(r[2], o2, ih2, l[2], d[2]);
+                if (ok) gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
+            }
             Bf[0] = Bf[0] - betaC * Ef[1];
             Bf[1] = Bf[1] + betaC * Ef[0];
-            transform_to(Mb, ob, r);
             double t3[3];
-            mat_vec(Mb, Ef, t3);
+            if constexpr (sizeof(T) == 8) {
+                transform_to(Mb, ob, r);
+                mat_vec(Mb, Ef, t3);
+            } else {
+                transform_to_fast(Mb, ob, r);
+                mat_vec_fast(Mb, Ef, t3);
+            }
             Ef[0] = t3[0];
             Ef[1] = t3[1];
             Ef[2] = t3[2];
-            mat_vec(Mb, Bf, t3);
+            if constexpr (sizeof(T) == 8) mat_vec(Mb, Bf, t3);
+            else mat_vec_fast(Mb, Bf, t3);
             Bf[0] = t3[0];
             Bf[1] = t3[1];
             Bf[2] = t3[2];
@@ -1012,11 +1191,18 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
             Ef[k] += Ee[k];
             Bf[k] += Be[k];
         }
-        boris_kick(p, Ef, Bf, dT, mass, charge);
-        push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
-        transform_to(Mu, ou, r);
         double pn[3];
-        mat_vec(Mu, p, pn);
+        if constexpr (sizeof(T) == 8) {
+            boris_kick(p, Ef, Bf, dT, mass, charge);
+            push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
+            transform_to(Mu, ou, r);
+            mat_vec(Mu, p, pn);
+        } else {
+            boris_kick_fast(p, Ef, Bf, e_kick, b_kick);
+            push_half_fast(r[0], r[1], r[2], p[0], p[1], p[2], half_cdt);
+            transform_to_fast(Mu, ou, r);
+            mat_vec_fast(Mu, p, pn);
+        }
         x[i] = r[0];
         y[i] = r[1];
         z[i] = r[2];
