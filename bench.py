@@ -76,8 +76,10 @@ def algorithmic_bytes(cfg, wbytes):
     n3, K0 = n ** 3, n + 1
     cw = 2 * wbytes  # complex word
     return {
-        # read x,y,z,px,py,pz; write x,y,z
+        # first step of a call: read x,y,z,px,py,pz; write x,y,z
         "push_rotate_bounds": N * 72,
+        # later steps (the previous gather pass applied the half push): read x,y,z
+        "rotate_bounds": N * 24,
         # read x,y,z,q; read+write int64 rho once
         "deposit": N * 32 + n3 * 16,
         # read+write x,y,z,px,py,pz; read the E grid (4 words per cell) once
@@ -190,8 +192,8 @@ def main():
         log("[bench] FP64 step on the same bunch")
         cd = make_ctx(sc, cfg, sc.FP64, args.sort_interval)
         cm2 = make_ctx(sc, cfg, sc.MIXED, args.sort_interval)
-        # one step from the identical bunch: rho is bit-identical (int64), so the
-        # E difference is the precision of the FP32 solve and gather alone
+        # one step from the identical bunch: the E-grid difference is the mixed
+        # path's error (2^24-scale deposit, FP32 convolution and Grad)
         s64 = cd.track(1)
         s32 = cm2.track(1)
         E64 = cd.get_grid(sc.GRID_EFIELD)

@@ -58,6 +58,7 @@ struct opalsc_ctx {
     DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
     double qmax = 0.0;        // max |Q| of the resident bunch
     bool have_bunch = false, have_track = false;
+    bool track_sc = false;    // the tracker computes space-charge fields
     Profiler prof;
     double dh = 0.02;
     int steps_done = 0;
@@ -84,7 +85,7 @@ namespace opalsc {
 const char *prof_name(int id) {
     static const char *names[PROF_COUNT] = {"push_rotate_bounds", "scalar", "sort", "zero_rho",
                                             "deposit", "green", "green_dct", "fft_x", "fft_y",
-                                            "fft_z_conv", "grad", "gather_kick_push"};
+                                            "fft_z_conv", "grad", "gather_kick_push", "rotate_bounds"};
     return (id >= 0 && id < PROF_COUNT) ? names[id] : "?";
 }
 void Profiler::start() {
@@ -353,7 +354,7 @@ extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, co
         s.origin[d] = origin[d];
     }
     upload_scalars(c, s);
-    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, b.keys, b.vals);
+    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, false, b.keys, b.vals);
     if (sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, key_bits(c->cfg.n)))
         FAIL(c, OPALSC_EHIP, "radix sort failed");
     CHECK_LAUNCH(c);
@@ -590,6 +591,7 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     CHECK_LAUNCH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_track = true;
+    c->track_sc = p->space_charge != 0;
     c->steps_done = 0;
     return OPALSC_OK;
 }
@@ -597,7 +599,7 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
 static void sort_bunch(opalsc_ctx *c) {
     Bunch &b = c->bunch;
     const int64_t N = b.N;
-    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, b.keys, b.vals);
+    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, true, b.keys, b.vals);
     sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, key_bits(c->cfg.n));
     double **arrs[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz, &b.q};
     for (double **a : arrs) {
@@ -618,30 +620,39 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     c->prof.start();
+    // Within one call the gather pass of step s also applies step s+1's first
+    // half push, so the bounds pass of step s+1 only reads positions.  The
+    // last step of a call leaves the bunch at its end-of-step state.
+    bool prepushed = false;
     for (int s = 0; s < nsteps; ++s) {
-        launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N);
+        launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1]);
         c->prof.mark(PROF_SCALAR);
-        launch_track_alpha(c->stream, exact, b, c->d_sc, c->d_ts, c->partials);
-        c->prof.mark(PROF_PUSH_BOUNDS);
-        launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, c->d_sc);
-        c->prof.mark(PROF_SCALAR);
-        const int interval = c->cfg.sort_interval;
-        bool sorted_now = false;
-        if (interval > 0 && (c->steps_done % interval) == 0) {
-            sort_bunch(c);
-            sorted_now = true;  // the push pass's chunk boxes no longer match the particle order
-            c->prof.mark(PROF_SORT);
+        if (c->track_sc || !prepushed) {
+            launch_track_alpha(c->stream, exact, !prepushed, b, c->d_sc, c->d_ts, c->partials);
+            c->prof.mark(prepushed ? PROF_ROTATE_BOUNDS : PROF_PUSH_BOUNDS);
         }
-        launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
-        HIPCHK(c, hipMemsetAsync(&c->d_counters[1], 0, sizeof(unsigned long long), c->stream));
-        c->prof.mark(PROF_ZERO);
-        launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
-                       c->d_sc, &c->d_counters[1], sorted_now ? nullptr : b.chunkbox);
-        c->prof.mark(PROF_DEPOSIT);
-        c->grid.green(c->d_sc, cr);
-        c->grid.solve(c->d_sc, 0, 1);
-        c->grid.grad(c->d_sc);
-        launch_track_gamma(c->stream, c->grid.fp64, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials);
+        if (c->track_sc) {
+            launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact,
+                              c->d_sc);
+            c->prof.mark(PROF_SCALAR);
+            const int interval = c->cfg.sort_interval;
+            bool sorted_now = false;
+            if (interval > 0 && (c->steps_done % interval) == 0) {
+                sort_bunch(c);
+                sorted_now = true;  // the bounds pass's chunk boxes no longer match the particle order
+                c->prof.mark(PROF_SORT);
+            }
+            launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+            c->prof.mark(PROF_ZERO);
+            launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
+                           c->d_sc, &c->d_counters[1], sorted_now ? nullptr : b.chunkbox);
+            c->prof.mark(PROF_DEPOSIT);
+            c->grid.green(c->d_sc, cr);
+            c->grid.solve(c->d_sc, 0, 1);
+            c->grid.grad(c->d_sc);
+        }
+        prepushed = s + 1 < nsteps;
+        launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials);
         c->prof.mark(PROF_GATHER_PUSH);
         launch_track_post(c->stream, c->partials, nb, c->d_ts, c->d_sc,
                           (char *)c->statsb.p + sizeof(opalsc_stats) * s, &c->d_counters[1]);

@@ -21,6 +21,7 @@ enum ProfId {
     PROF_FFT_Z,            // z: forward * Ghat * inverse
     PROF_GRAD,             // E = -Grad(phi)
     PROF_GATHER_PUSH,      // gather + kick + push + frame update + moments
+    PROF_ROTATE_BOUNDS,    // beam-frame rotation + bounds of already pushed positions
     PROF_COUNT
 };
 
@@ -115,7 +116,7 @@ void launch_moments(hipStream_t s, int64_t N, const double *R, const double *P, 
 void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, int64_t N,
                                 double mass_eV, void *stats_out);
 void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,
-                      const Scalars *sc, uint32_t *keys, uint32_t *vals);
+                      const Scalars *sc, bool rotate, uint32_t *keys, uint32_t *vals);
 void launch_permute(hipStream_t s, int64_t N, const uint32_t *perm, const double *src, double *dst);
 void launch_permute_u32(hipStream_t s, int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst);
 void launch_generate(hipStream_t s, int64_t N, uint64_t seed, const double sr[3], const double sp[3],
@@ -127,13 +128,13 @@ void launch_copy_calibration(hipStream_t s, size_t bytes, const void *src, void 
 void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y);
 
 // Tracker kernels
-void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N);
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped);
 int alpha_blocks(int64_t N);
-void launch_track_alpha(hipStream_t s, bool exact, Bunch &b, const Scalars *sc, const TrackState *ts,
-                        double *partials);
+void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Scalars *sc,
+                        const TrackState *ts, double *partials);
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, bool exact, Scalars *sc);
-void launch_track_gamma(hipStream_t s, bool fp64, Bunch &b, const void *Eg, const Scalars *sc,
+void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                         const TrackState *ts, double *partials);
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped);

@@ -247,52 +247,6 @@ void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int
 // integer atomics, so the charge grid does not depend on the order in which
 // particles are processed.  MODE 0: qd given; MODE 1: qd = Q*dT with Q
 // updated to (Q*dT)/dT as the reference leaves it (PartBunch.cpp:493-495).
-template <int MODE>
-__global__ __launch_bounds__(256) void k_deposit(int64_t N, const double *x, const double *y,
-                                                 const double *z, double *q, const double *qdv,
-                                                 long long *rho, const Scalars *sc,
-                                                 unsigned long long *dropped) {
-    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
-    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
-    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
-    const double e2q = sc->e2q, dT = sc->dT;
-    unsigned long long nd = 0;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
-        double qd;
-        if constexpr (MODE == 0) {
-            qd = qdv[p];
-        } else {
-            const double qv = q[p];
-            qd = qv * dT;
-            const double qn = qd / dT;
-            if (qn != qv) q[p] = qn;
-        }
-        int l[3];
-        double d[3];
-        bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
-        ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
-        ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
-        if (!ok) {
-            nd += 8;
-            continue;
-        }
-        double w[8];
-        cic_weights(d, w);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
-            if (i >= 0 && j >= 0 && k >= 0 && i < n0 && j < n1 && k < n2) {
-                const double v = w[c] * qd;
-                const long long iv = (long long)rint(v * e2q);
-                if (iv != 0)
-                    atomicAdd((unsigned long long *)&rho[((size_t)k * n1 + j) * n0 + i], (unsigned long long)iv);
-            } else {
-                ++nd;
-            }
-        }
-    }
-    if (nd) atomicAdd(dropped, nd);
-}
 
 // Sum the 8 corner contributions over runs of equal lgp key in consecutive
 // lanes (particles sorted by cell form long runs).  Returns true on the last
@@ -343,33 +297,41 @@ __device__ inline bool wave_run_sums(int key, V (&v)[8]) {
     if (heads == ~0ull) return true;  // no two neighbours share a cell: nothing to merge
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const int start = 63 - __builtin_clzll(heads & upto);
+    const int src = start > 0 ? start - 1 : 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        V s = v[c];
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const V t = __shfl_up(s, off, 64);
-            if (lane >= off) s += t;
-        }
-        const V before = __shfl(s, start > 0 ? start - 1 : 0, 64);
+        const V s = wave_incl_scan(v[c]);
+        const V before = __shfl(s, src, 64);
         v[c] = start > 0 ? s - before : s;
     }
     return tail;
 }
 
-// LDS-tiled deposit.  A block owns a contiguous chunk of kChunk particles.
-// It bounds the chunk's cells with a block min/max of the positions; if the
-// target box fits kBoxCap cells, contributions are accumulated in an LDS copy
-// of the box (64-bit LDS atomics) and the non-zero cells are flushed to HBM
-// with one global atomic each; otherwise they go straight to HBM.  Before
-// either, each wave sums runs of equal cells (wave_run_sums), so a bunch
-// sorted by cell issues one atomic per run and corner instead of one per
-// particle.  Integer addition makes the result independent of all of this.
+// LDS-tiled deposit.  A block owns a contiguous chunk of kChunk particles,
+// loads all of them into registers at once (kPPT loads per array in flight per
+// thread) and bounds the chunk's cells with a block min/max of the positions
+// (or takes the box the push pass recorded); if the target box fits kBoxCap
+// cells, contributions are accumulated in an LDS copy of the box (64-bit LDS
+// atomics) and the non-zero cells are flushed to HBM with one global atomic
+// each; otherwise they go straight to HBM.  Before either, each wave sums runs
+// of equal cells (wave_run_sums), so a bunch sorted by cell issues one atomic
+// per run and corner instead of one per particle.  Integer addition makes the
+// result independent of all of this.
 constexpr int kPPT = 8;
 constexpr int kChunk = kThreads * kPPT;
+
+// a / b for 0 <= a < 2^24, 0 < b, from a float estimate corrected to the exact quotient
+__device__ inline int small_div(int a, int b, float rb) {
+    int qt = (int)((float)a * rb);
+    if (qt * b > a) --qt;
+    else if ((qt + 1) * b <= a) ++qt;
+    return qt;
+}
 constexpr int kBoxCap = 5632;  // 44 KiB of int64
 
-// MODE 0: qd given; 1: qd = Q*dT with Q left as (Q*dT)/dT.  EXACT: the FP64
+// MODE 0: qd given; 1 (tracker): qd = Q*dT with Q left as (Q*dT)/dT, positions
+// given in the reference frame and rotated into the beam frame (M_rb, o_rb)
+// exactly as the bounds pass rotated them.  EXACT: the FP64
 // path (reference-order CIC, 64-bit fixed point per deposit); otherwise the
 // mixed path (CIC with 1/h, FP32 weights, |deposit| < 2^24 in 32-bit integers,
 // summed in 64 bits).  BOX: chunk bounding boxes supplied by the push pass.
@@ -390,7 +352,35 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     const double e2q = sc->e2q, dT = sc->dT;
     const int64_t base = (int64_t)blockIdx.x * kChunk;
 
-    // 1. bounding box of the chunk (positions, then a safe index margin)
+    // 1. the chunk's particles, all loads issued before any is used
+    double px[kPPT], py[kPPT], pz[kPPT], pq[kPPT];
+#pragma unroll
+    for (int k = 0; k < kPPT; ++k) {
+        const int64_t p = base + k * kThreads + threadIdx.x;
+        const bool in = p < N;
+        px[k] = in ? x[p] : 0.0;
+        py[k] = in ? y[p] : 0.0;
+        pz[k] = in ? z[p] : 0.0;
+        if constexpr (MODE == 0) pq[k] = in ? qdv[p] : 0.0;
+        else pq[k] = in ? q[p] : 0.0;
+    }
+    if constexpr (MODE == 1) {
+        // the tracker keeps positions in the reference frame; the mesh is in the beam frame
+        double M[9], o[3];
+        for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
+        for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k) {
+            double r[3] = {px[k], py[k], pz[k]};
+            if constexpr (EXACT) transform_to(M, o, r);
+            else transform_to_fast(M, o, r);
+            px[k] = r[0];
+            py[k] = r[1];
+            pz[k] = r[2];
+        }
+    }
+
+    // 2. bounding box of the chunk (positions, then a safe index margin)
     double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     if constexpr (BOX) {
@@ -399,14 +389,15 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
             mx[d] = chunkbox[blockIdx.x * 6 + 3 + d];
         }
     } else {
+#pragma unroll
         for (int k = 0; k < kPPT; ++k) {
-            const int64_t p = base + k * kThreads + threadIdx.x;
-            if (p < N) {
-                const double r[3] = {x[p], y[p], z[p]};
-                for (int d = 0; d < 3; ++d) {
-                    mn[d] = fmin(mn[d], r[d]);
-                    mx[d] = fmax(mx[d], r[d]);
-                }
+            if (base + k * kThreads + threadIdx.x < N) {
+                mn[0] = fmin(mn[0], px[k]);
+                mx[0] = fmax(mx[0], px[k]);
+                mn[1] = fmin(mn[1], py[k]);
+                mx[1] = fmax(mx[1], py[k]);
+                mn[2] = fmin(mn[2], pz[k]);
+                mx[2] = fmax(mx[2], pz[k]);
             }
         }
         block_minmax(mn, mx, red);
@@ -445,8 +436,9 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
         __syncthreads();
     }
 
-    // 2. CIC, run sums, atomics
+    // 3. CIC, run sums, atomics
     using V = typename std::conditional<EXACT, long long, int>::type;
+    const int sxy = bx * by;
     unsigned long long nd = 0;
     for (int k = 0; k < kPPT; ++k) {
         const int64_t p = base + k * kThreads + threadIdx.x;
@@ -457,22 +449,22 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
         if (p < N) {
             double qd;
             if constexpr (MODE == 0) {
-                qd = qdv[p];
+                qd = pq[k];
             } else {
-                const double qv = q[p];
+                const double qv = pq[k];
                 qd = qv * dT;
                 const double qn = qd / dT;
                 if (qn != qv) q[p] = qn;
             }
             double d[3];
             if constexpr (EXACT) {
-                valid = cic_dim(x[p], o0, h0, l[0], d[0]);
-                valid &= cic_dim(y[p], o1, h1, l[1], d[1]);
-                valid &= cic_dim(z[p], o2, h2, l[2], d[2]);
+                valid = cic_dim(px[k], o0, h0, l[0], d[0]);
+                valid &= cic_dim(py[k], o1, h1, l[1], d[1]);
+                valid &= cic_dim(pz[k], o2, h2, l[2], d[2]);
             } else {
-                valid = cic_dim_fast(x[p], o0, ih0, l[0], d[0]);
-                valid &= cic_dim_fast(y[p], o1, ih1, l[1], d[1]);
-                valid &= cic_dim_fast(z[p], o2, ih2, l[2], d[2]);
+                valid = cic_dim_fast(px[k], o0, ih0, l[0], d[0]);
+                valid &= cic_dim_fast(py[k], o1, ih1, l[1], d[1]);
+                valid &= cic_dim_fast(pz[k], o2, ih2, l[2], d[2]);
             }
             if (valid) {
                 // a stencil starting outside [-1, n-1] has no corner in the grid
@@ -504,23 +496,33 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
             }
         }
         const bool tail = wave_run_sums<V>(key, v);
-        if (tail && valid) {
+        const int li = l[0] - bx0, lj = l[1] - by0, lk = l[2] - bz0;
+        // the whole stencil inside the LDS box (hence inside the grid): the common case
+        const bool inbox = use_lds && li >= 0 && lj >= 0 && lk >= 0 && li < bx - 1 && lj < by - 1 && lk < bz - 1;
+        if (tail && valid && inbox) {
+            long long *b = &box[(lk * by + lj) * bx + li];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int off = (c & 1) + ((c >> 1) & 1) * bx + ((c >> 2) & 1) * sxy;
+                atomicAdd((unsigned long long *)(b + off), (unsigned long long)(long long)v[c]);
+            }
+        } else if (tail && valid) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), kk = l[2] + ((c >> 2) & 1);
                 if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
                     if (v[c] == 0) continue;
-                    const int li = i - bx0, lj = j - by0, lk = kk - bz0;
+                    const int ci = i - bx0, cj = j - by0, ck = kk - bz0;
                     const unsigned long long add = (unsigned long long)(long long)v[c];
-                    if (use_lds && li >= 0 && lj >= 0 && lk >= 0 && li < bx && lj < by && lk < bz)
-                        atomicAdd((unsigned long long *)&box[(lk * by + lj) * bx + li], add);
+                    if (use_lds && ci >= 0 && cj >= 0 && ck >= 0 && ci < bx && cj < by && ck < bz)
+                        atomicAdd((unsigned long long *)&box[(ck * by + cj) * bx + ci], add);
                     else
                         atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], add);
                 }
             }
         }
         // out-of-grid corners of valid particles are dropped, counted per particle lane
-        if (valid) {
+        if (valid && !(l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 - 1 && l[1] < n1 - 1 && l[2] < n2 - 1)) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), kk = l[2] + ((c >> 2) & 1);
@@ -530,10 +532,14 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     }
     if (use_lds) {
         __syncthreads();
+        const float rbx = 1.0f / (float)bx, rsxy = 1.0f / (float)sxy;
         for (int e = threadIdx.x; e < vol; e += kThreads) {
             const long long v = box[e];
             if (v != 0) {
-                const int li = e % bx, lj = (e / bx) % by, lk = e / (bx * by);
+                const int lk = small_div(e, sxy, rsxy);
+                const int r = e - lk * sxy;
+                const int lj = small_div(r, bx, rbx);
+                const int li = r - lj * bx;
                 atomicAdd((unsigned long long *)&rho[((size_t)(lk + bz0) * n1 + (lj + by0)) * n0 + (li + bx0)],
                           (unsigned long long)v);
             }
@@ -546,13 +552,6 @@ void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double
                     const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
                     unsigned long long *dropped, const double *chunkbox) {
     if (N <= 0) return;
-    static const bool simple = getenv("OPALSC_DEPOSIT_SIMPLE") != nullptr;
-    if (simple && exact) {
-        int blocks = (int)std::min<int64_t>((N + kThreads - 1) / kThreads, 65536);
-        if (mode == 0) k_deposit<0><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
-        else k_deposit<1><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped);
-        return;
-    }
     const unsigned blocks = (unsigned)((N + kChunk - 1) / kChunk);
     const bool box = chunkbox != nullptr;
 #define OPALSC_DEP(M_, E_, B_) \
@@ -835,18 +834,25 @@ void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, i
 }
 
 // ---- sorting support ------------------------------------------------------------
+// rotate: positions are in the reference frame; the cell is that of the
+// beam-frame position M_rb (r - o_rb) the deposit uses.
 __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const double *z,
-                            const Scalars *sc, uint32_t *keys, uint32_t *vals) {
+                            const Scalars *sc, int rotate, uint32_t *keys, uint32_t *vals) {
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const uint32_t out = cell_key_count(n0, n1, n2);
+    double M[9], o[3];
+    for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
+    for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
         int l[3];
         double d[3];
-        bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
-        ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
-        ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
+        double r[3] = {x[p], y[p], z[p]};
+        if (rotate) transform_to_fast(M, o, r);
+        bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
+        ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
+        ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
         uint32_t key = out;
         if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
             key = cell_key(l[0], l[1], l[2], n0, n1);
@@ -855,8 +861,8 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
     }
 }
 void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,
-                      const Scalars *sc, uint32_t *keys, uint32_t *vals) {
-    if (N > 0) k_cell_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, sc, keys, vals);
+                      const Scalars *sc, bool rotate, uint32_t *keys, uint32_t *vals) {
+    if (N > 0) k_cell_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, sc, rotate ? 1 : 0, keys, vals);
 }
 
 __global__ void k_permute(int64_t N, const uint32_t *perm, const double *src, double *dst) {
@@ -976,8 +982,9 @@ void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double
 // external fields at t + dT/2 (ParallelTTracker.cpp:630), then t += dT and the
 // reference-particle update whose trafo transformBunch applies at step end
 // (ParallelTTracker.cpp:362-367,1199-1213).
-__global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N) {
+__global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *dropped = 0;  // the step's count of deposits outside the grid
     TrackState s = *ts;
     sc->dT = s.dT;
     sc->coupling = s.coupling;
@@ -1013,15 +1020,18 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N) {
     s.step += 1;
     *ts = s;
 }
-void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N) {
-    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N);
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped) {
+    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N, dropped);
 }
 
-// push (timeIntegration1) + rotation into the beam frame + bounds (boundp's get_bounds).
-// Each block walks contiguous chunks of kChunk particles (the deposit's
-// chunks), writes every chunk's beam-frame bounding box to chunkbox (so the
-// deposit reads positions once), and leaves its running bounds in partials.
-template <bool EXACT>
+// PUSH: the step's first half push (timeIntegration1), written back; otherwise
+// the positions already hold it (the previous step's gather pass applied it).
+// Then the rotation into the beam frame (not stored: the deposit and the gather
+// pass repeat it bit for bit) and its bounds (boundp's get_bounds).  Each block
+// walks contiguous chunks of kChunk particles (the deposit's chunks), writes
+// every chunk's beam-frame bounding box to chunkbox (so the deposit reads
+// positions once), and leaves its running bounds in partials.
+template <bool EXACT, bool PUSH>
 __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restrict__ x,
                                                      double *__restrict__ y, double *__restrict__ z,
                                                      const double *__restrict__ px,
@@ -1048,16 +1058,17 @@ __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restri
             const int64_t i = base + k * kThreads;
             if (i >= N) break;
             double r[3] = {x[i], y[i], z[i]};
-            if constexpr (EXACT) {
-                push_half(r[0], r[1], r[2], px[i], py[i], pz[i], dT);
-                if (sc_on) transform_to(M, o, r);
-            } else {
-                push_half_fast(r[0], r[1], r[2], px[i], py[i], pz[i], half_cdt);
-                if (sc_on) transform_to_fast(M, o, r);
+            if constexpr (PUSH) {
+                if constexpr (EXACT) push_half(r[0], r[1], r[2], px[i], py[i], pz[i], dT);
+                else push_half_fast(r[0], r[1], r[2], px[i], py[i], pz[i], half_cdt);
+                x[i] = r[0];
+                y[i] = r[1];
+                z[i] = r[2];
             }
-            x[i] = r[0];
-            y[i] = r[1];
-            z[i] = r[2];
+            if (sc_on) {
+                if constexpr (EXACT) transform_to(M, o, r);
+                else transform_to_fast(M, o, r);
+            }
             for (int d = 0; d < 3; ++d) {
                 mn[d] = fmin(mn[d], r[d]);
                 mx[d] = fmax(mx[d], r[d]);
@@ -1082,15 +1093,20 @@ int alpha_blocks(int64_t N) {
     int64_t nchunks = (N + kChunk - 1) / kChunk;
     return (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, kRedBlocksMax));
 }
-void launch_track_alpha(hipStream_t s, bool exact, Bunch &b, const Scalars *sc, const TrackState *ts,
-                        double *partials) {
+void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Scalars *sc,
+                        const TrackState *ts, double *partials) {
     if (b.N <= 0) return;
-    if (exact)
-        k_track_alpha<true><<<alpha_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts,
-                                                                   partials, b.chunkbox);
-    else
-        k_track_alpha<false><<<alpha_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, ts,
-                                                                    partials, b.chunkbox);
+#define OPALSC_ALPHA(E_, P_)                                                                              \
+    k_track_alpha<E_, P_><<<alpha_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, sc, \
+                                                                 ts, partials, b.chunkbox)
+    if (exact) {
+        if (push) OPALSC_ALPHA(true, true);
+        else OPALSC_ALPHA(true, false);
+    } else {
+        if (push) OPALSC_ALPHA(false, true);
+        else OPALSC_ALPHA(false, false);
+    }
+#undef OPALSC_ALPHA
 }
 
 __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int nb, int n0, int n1,
@@ -1117,11 +1133,13 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
     k_track_mesh<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, exact ? 1 : 0, sc);
 }
 
-// Gather + B from E (PartBunch.cpp:632-644), rotation back to the reference
+// Rotation into the beam frame (the bounds pass's, repeated bit for bit),
+// gather + B from E (PartBunch.cpp:632-644), rotation back to the reference
 // frame (ParallelTTracker.cpp:586-591), external fields, kick + push
 // (timeIntegration2), transformBunch with the reference-frame update, and the
-// moment sums of the step's statistics.
-template <typename T>
+// moment sums of the step's statistics.  NEXT: also the next step's first half
+// push (timeIntegration1 with the updated P), so that pass reads positions only.
+template <typename T, bool NEXT>
 __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, double *y, double *z,
                                                      double *px, double *py, double *pz,
                                                      const typename vec4_of<T>::type *Eg,
@@ -1134,12 +1152,14 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double betaC = sc->betaC;
-    double Mb[9], ob[3], Mu[9], ou[3], Ee[3], Be[3];
+    double Mrb[9], orb[3], Mb[9], ob[3], Mu[9], ou[3], Ee[3], Be[3];
     for (int k = 0; k < 9; ++k) {
+        Mrb[k] = sc->M_rb[k];
         Mb[k] = sc->M_br[k];
         Mu[k] = sc->M_u[k];
     }
     for (int d = 0; d < 3; ++d) {
+        orb[d] = sc->o_rb[d];
         ob[d] = sc->o_br[d];
         ou[d] = sc->o_u[d];
         Ee[d] = sc->Eext[d];
@@ -1155,6 +1175,8 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
         double p[3] = {px[i], py[i], pz[i]};
         double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
         if (sc_on) {
+            if constexpr (sizeof(T) == 8) transform_to(Mrb, orb, r);
+            else transform_to_fast(Mrb, orb, r);
             int l[3];
             double d[3];
             if constexpr (sizeof(T) == 8) {
@@ -1203,25 +1225,34 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
             transform_to_fast(Mu, ou, r);
             mat_vec_fast(Mu, p, pn);
         }
+        accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+        if constexpr (NEXT) {
+            if constexpr (sizeof(T) == 8) push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
+            else push_half_fast(r[0], r[1], r[2], pn[0], pn[1], pn[2], half_cdt);
+        }
         x[i] = r[0];
         y[i] = r[1];
         z[i] = r[2];
         px[i] = pn[0];
         py[i] = pn[1];
         pz[i] = pn[2];
-        accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
     }
     store_moment_partials(m, partials, lds);
 }
-void launch_track_gamma(hipStream_t s, bool fp64, Bunch &b, const void *Eg, const Scalars *sc,
+void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                         const TrackState *ts, double *partials) {
     if (b.N <= 0) return;
-    if (fp64)
-        k_track_gamma<double><<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,
-                                                                   (const double4 *)Eg, sc, ts, partials);
-    else
-        k_track_gamma<float><<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,
-                                                                  (const float4 *)Eg, sc, ts, partials);
+#define OPALSC_GAMMA(T_, V_, N_)                                                                          \
+    k_track_gamma<T_, N_><<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,      \
+                                                               (const V_ *)Eg, sc, ts, partials)
+    if (fp64) {
+        if (next) OPALSC_GAMMA(double, double4, true);
+        else OPALSC_GAMMA(double, double4, false);
+    } else {
+        if (next) OPALSC_GAMMA(float, float4, true);
+        else OPALSC_GAMMA(float, float4, false);
+    }
+#undef OPALSC_GAMMA
 }
 
 // Step statistics (DataSink / calcBeamParameters) and the momentum sums the

@@ -182,6 +182,45 @@ def test_tracker_sorted_and_unsorted_agree():
         assert np.allclose(a["rrms"], b["rrms"], rtol=1e-13, atol=0)
 
 
+@pytest.mark.parametrize("precision", [sc.FP64, sc.MIXED])
+def test_tracker_one_call_equals_single_steps(precision):
+    """Inside one track() call the gather pass of step s also applies the first
+    half push of step s+1; a call of one step pushes in the bounds pass.  Both
+    schedules run the same per-particle operations, so they agree bit for bit."""
+    n, N, steps = (32, 32, 32), 150000, 5
+    R, P, Q = gaussian_bunch(N, seed=31, q_total=2e-9)
+    runs = []
+    for split in (False, True):
+        with ctx(n, precision=precision, N=N, sort_interval=2) as c:
+            c.set_bunch(R, P, Q)
+            c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5, Ez0=1e6, rf_freq=5e7)
+            st = [c.track(1)[0] for _ in range(steps)] if split else c.track(steps)
+            runs.append((c.get_bunch(), st, c.track_params()))
+    (Ra, Pa, Qa), sa, pa = runs[0]
+    (Rb, Pb, Qb), sb, pb = runs[1]
+    assert np.array_equal(Ra, Rb) and np.array_equal(Pa, Pb) and np.array_equal(Qa, Qb)
+    for a, b in zip(sa, sb):
+        for k in ("rmean", "pmean", "rrms", "prms", "eps_norm"):
+            assert np.array_equal(a[k], b[k]), k
+    assert pa.t == pb.t and pa.pathLength == pb.pathLength
+
+
+def test_tracker_without_space_charge_matches_oracle():
+    n, N, steps = (16, 16, 16), 50000, 4
+    R, P, Q = gaussian_bunch(N, seed=5, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5, space_charge=False)
+    Ro, Po, _, _, so = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, N=N) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5, space_charge=False)
+        sg = c.track(steps)
+        Rg, Pg, _ = c.get_bunch()
+    assert np.array_equal(Rg, Ro) and np.array_equal(Pg, Po)
+    for a, b in zip(sg, so):
+        assert np.allclose(a["rrms"], b["rrms"], rtol=1e-12, atol=0)
+
+
 def test_tracker_matches_oracle_solenoid_1e6_64():
     n, N, steps = (64, 64, 64), 1000000, 3
     R, P, Q = gaussian_bunch(N, seed=2, q_total=10e-9)
