@@ -1,28 +1,44 @@
 // LDS-resident batched 1D FFT for gfx950 (no vendor FFT library).
 //
-// A workgroup of kThreads threads transforms W lines of length L held in LDS
+// A workgroup of kFftThreads threads transforms W lines of length L held in LDS
 // with an interleaved layout: element (line, idx) lives at idx*P + line,
 // P = W + 1.  Consecutive threads work on consecutive lines, so every LDS
 // access of a stage touches consecutive addresses and the odd pitch keeps
-// the transposing global loads of the x pass conflict-free.
+// the transposing accesses of the x passes spread over the banks.  W is a
+// power of two chosen so that one row of a tile is a whole 128-byte line of
+// HBM in the strided passes.
 //
 // Each stage is a Stockham autosort radix-R step: thread-owned butterflies
 // are read into registers, twiddled, transformed, and written back after a
-// barrier.  E = W*L/kThreads elements per thread is a compile-time constant
+// barrier.  E = W*L/kFftThreads elements per thread is a compile-time constant
 // divisible by every radix used, so each thread owns exactly E/R butterflies
-// per stage.  Radices 2, 3, 4 and 8 are supported: L = 2^a or 3*2^a.
+// per stage.  Radices 8, 4 and 2 come first and radix 3 (L = 3*2^a) last, so
+// the span Ns of every stage is a power of two and all index arithmetic is
+// shifts and masks.  Twiddles come from LDS (FP32) or through the cache (FP64,
+// whose tiles leave no room for them).
 #pragma once
 #include "common.hpp"
 
 namespace opalsc {
 
+constexpr int kFftThreads = 512;  // threads per FFT workgroup
+
+// a / d for 0 <= a < 2^22, d > 0: float estimate, then the one correction it can need
+__device__ inline int div_small(int a, int d, float rd) {
+    int q = (int)((float)a * rd);
+    if (q * d > a) --q;
+    else if ((q + 1) * d <= a) ++q;
+    return q;
+}
+
 struct FftPlan {
     int L;           // transform length
-    int W;           // lines per workgroup
+    int W;           // lines per workgroup (power of two)
+    int lw;          // log2(W)
     int P;           // LDS pitch (W + 1)
     int E;           // elements per thread (template parameter of the kernels)
     int nst;         // number of stages
-    int radix[12];   // radix of each stage
+    int radix[16];   // radix of each stage
     const void *tw;  // exp(-2 pi i m / L), m in [0, L), complex T
 };
 
@@ -99,18 +115,18 @@ __device__ inline void dftR(C *v, bool inv) {
     else dft8(v, inv);
 }
 
-// One Stockham stage on LDS data; Ns = product of the radices already applied.
+// One Stockham stage on LDS data; Ns = 2^ls = product of the radices already applied.
 template <typename C, int E, int R>
-__device__ inline void fft_stage(C *buf, const C *tw, int L, int W, int P, int Ns, bool inv) {
+__device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int ls, bool inv) {
     constexpr int NB = E / R;
-    const int LR = L / R;
+    const int LR = L / R, Ns = 1 << ls, W = 1 << lw;
     const int twstep = L / (Ns * R);
     C v[NB][R];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int g = threadIdx.x + b * kThreads;
-        const int line = g % W, j = g / W;
-        const int k = j % Ns;
+        const int g = threadIdx.x + b * kFftThreads;
+        const int line = g & (W - 1), j = g >> lw;
+        const int k = j & (Ns - 1);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * LR) * P + line];
 #pragma unroll
@@ -124,35 +140,49 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int W, int P, int N
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int g = threadIdx.x + b * kThreads;
-        const int line = g % W, j = g / W;
-        const int k = j % Ns;
-        const int idxD = (j / Ns) * Ns * R + k;
+        const int g = threadIdx.x + b * kFftThreads;
+        const int line = g & (W - 1), j = g >> lw;
+        const int k = j & (Ns - 1);
+        const int idxD = (((j >> ls) * R) << ls) + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) buf[(idxD + r * Ns) * P + line] = v[b][r];
     }
     __syncthreads();
 }
 
-// Transform the W lines in buf (caller has synchronised after filling it).
+// Transform the W lines in buf (caller has synchronised after filling it and tw).
 template <typename C, int E>
-__device__ inline void fft_lds(C *buf, const FftPlan &pl, bool inv) {
-    const C *tw = (const C *)pl.tw;
-    int Ns = 1;
+__device__ inline void fft_lds(C *buf, const C *tw, const FftPlan &pl, bool inv) {
+    int ls = 0;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.radix[s];
         if constexpr (E % 8 == 0) {
-            if (R == 8) { fft_stage<C, E, 8>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 8; continue; }
+            if (R == 8) { fft_stage<C, E, 8>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); ls += 3; continue; }
         }
         if constexpr (E % 4 == 0) {
-            if (R == 4) { fft_stage<C, E, 4>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 4; continue; }
-        }
-        if constexpr (E % 3 == 0) {
-            if (R == 3) { fft_stage<C, E, 3>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 3; continue; }
+            if (R == 4) { fft_stage<C, E, 4>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); ls += 2; continue; }
         }
         if constexpr (E % 2 == 0) {
-            if (R == 2) { fft_stage<C, E, 2>(buf, tw, pl.L, pl.W, pl.P, Ns, inv); Ns *= 2; continue; }
+            if (R == 2) { fft_stage<C, E, 2>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); ls += 1; continue; }
         }
+        if constexpr (E % 3 == 0) {
+            if (R == 3) { fft_stage<C, E, 3>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); continue; }
+        }
+    }
+}
+
+// Twiddle source of a kernel: FP32 tiles copy the table into LDS after the
+// data (call before the barrier that follows the tile load); FP64 reads it
+// through the cache.
+template <typename C>
+__device__ inline const C *fft_twiddles(C *buf, const FftPlan &pl) {
+    if constexpr (sizeof(C) == 8) {
+        C *t = buf + pl.L * pl.P;
+        const C *src = (const C *)pl.tw;
+        for (int m = threadIdx.x; m < pl.L; m += kFftThreads) t[m] = src[m];
+        return t;
+    } else {
+        return (const C *)pl.tw;
     }
 }
 

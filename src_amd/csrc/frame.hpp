@@ -179,14 +179,14 @@ __host__ __device__ inline Trafo update_reference(TrackState &ts) {
 
 // Index of moment (i,j) in the 27-entry layout of PartBunchBase::calcMoments
 // (PartBunchBase.hpp:2185-2258): 6 first moments, then the lower triangle row by row.
-__host__ __device__ inline int mom_index(int i, int j) {
-    if (j > i) {
-        int t = i;
-        i = j;
-        j = t;
-    }
-    return 6 + i * (i + 1) / 2 + j;
-}
+// Moment sums that calcBeamParameters reads (PartBunchBase.hpp:1557-1611):
+// calcMoments (PartBunchBase.hpp:2185-2258) accumulates the full lower
+// triangle of part_i*part_j, but only the 9 entries below enter the
+// statistics, so only they are summed here (same products, same values):
+//   [0..5]   sum of x, px, y, py, z, pz
+//   [6+3d]   sum r_d^2,  [7+3d] sum p_d^2,  [8+3d] sum r_d*p_d   (d = 0..2)
+//   [15]     sum sqrt(1 + p.p)
+constexpr int kMom = 16;
 
 // calcBeamParameters from the moment sums (PartBunchBase.hpp:1557-1611) and
 // calcEMean (PartBunchBase.hpp:2099-2116).  Output order matches opalsc_stats.
@@ -198,8 +198,9 @@ struct StatsOut {
     int64_t dropped;
 };
 
-__host__ __device__ inline void beam_parameters(int64_t N, const double m[27], double gamma_sum,
-                                                double mass_eV, StatsOut &s) {
+__host__ __device__ inline void beam_parameters(int64_t N, const double m[kMom], double mass_eV,
+                                                StatsOut &s) {
+    const double gamma_sum = m[15];
     for (int i = 0; i < 3; ++i)
         s.rmean[i] = s.pmean[i] = s.rrms[i] = s.prms[i] = s.eps_norm[i] = s.eps[i] = s.rprms[i] = 0.0;
     s.gamma_mean = 0.0;
@@ -211,10 +212,10 @@ __host__ __device__ inline void beam_parameters(int64_t N, const double m[27], d
     for (int i = 0; i < 3; ++i) {
         s.rmean[i] = m[2 * i] / Nd;
         s.pmean[i] = m[2 * i + 1] / Nd;
-        rsq[i] = m[mom_index(2 * i, 2 * i)] - Nd * s.rmean[i] * s.rmean[i];
-        psq[i] = m[mom_index(2 * i + 1, 2 * i + 1)] - Nd * s.pmean[i] * s.pmean[i];
+        rsq[i] = m[6 + 3 * i] - Nd * s.rmean[i] * s.rmean[i];
+        psq[i] = m[7 + 3 * i] - Nd * s.pmean[i] * s.pmean[i];
         if (psq[i] < 0) psq[i] = 0;
-        rp[i] = m[mom_index(2 * i, 2 * i + 1)] - Nd * s.rmean[i] * s.pmean[i];
+        rp[i] = m[8 + 3 * i] - Nd * s.rmean[i] * s.pmean[i];
     }
     for (int i = 0; i < 3; ++i) eps2[i] = (rsq[i] * psq[i] - rp[i] * rp[i]) / (Nd * Nd);
     for (int i = 0; i < 3; ++i) rp[i] /= Nd;

@@ -44,6 +44,8 @@ struct Profiler {
 
 struct PoissonGrid {
     int n[3] = {0, 0, 0}, L[3] = {0, 0, 0}, K0 = 0;
+    int kp = 0;   // row pitch (complex elements) of A and B: K0 rounded up to whole tiles
+    int gp = 0;   // row pitch (real elements) of Ghat, >= kp
     bool fp64 = true;
     int greens = 0;
     size_t ncell = 0, noct = 0;

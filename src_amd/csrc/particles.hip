@@ -765,34 +765,36 @@ void launch_kick(hipStream_t s, int64_t N, double *P, const double *Ef, const do
 }
 
 // part = (x, px, y, py, z, pz): 6 sums, 21 products (calcMoments order), gamma sum.
-__device__ inline void accumulate_moments(double (&m)[28], double x, double y, double z, double px,
+__device__ inline void accumulate_moments(double (&m)[kMom], double x, double y, double z, double px,
                                           double py, double pz) {
     const double part[6] = {x, px, y, py, z, pz};
-    int l = 6;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        m[i] += part[i];
+    for (int i = 0; i < 6; ++i) m[i] += part[i];
 #pragma unroll
-        for (int j = 0; j <= i; ++j) m[l++] += part[i] * part[j];
+    for (int d = 0; d < 3; ++d) {
+        const double r = part[2 * d], p = part[2 * d + 1];
+        m[6 + 3 * d] += r * r;
+        m[7 + 3 * d] += p * p;
+        m[8 + 3 * d] += p * r;
     }
-    m[27] += sqrt(1.0 + (px * px + py * py + pz * pz));
+    m[15] += sqrt(1.0 + (px * px + py * py + pz * pz));
 }
 
-__device__ inline void store_moment_partials(double (&m)[28], double *partials, double *lds) {
-    double l[28];
-    for (int k = 0; k < 28; ++k) l[k] = 0.0;
-    block_sum_dd<28>(m, l, lds);
+__device__ inline void store_moment_partials(double (&m)[kMom], double *partials, double *lds) {
+    double l[kMom];
+    for (int k = 0; k < kMom; ++k) l[k] = 0.0;
+    block_sum_dd<kMom>(m, l, lds);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 28; ++k) {
+        for (int k = 0; k < kMom; ++k) {
             partials[blockIdx.x * kSlots + 2 * k] = m[k];
             partials[blockIdx.x * kSlots + 2 * k + 1] = l[k];
         }
 }
 
 __global__ __launch_bounds__(256) void k_moments(int64_t N, const double *R, const double *P, double *partials) {
-    __shared__ double lds[4 * 28 * 2];
-    double m[28];
-    for (int k = 0; k < 28; ++k) m[k] = 0.0;
+    __shared__ double lds[4 * kMom * 2];
+    double m[kMom];
+    for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
         accumulate_moments(m, R[3 * i], R[3 * i + 1], R[3 * i + 2], P[3 * i], P[3 * i + 1], P[3 * i + 2]);
     store_moment_partials(m, partials, lds);
@@ -804,9 +806,9 @@ void launch_moments(hipStream_t s, int64_t N, const double *R, const double *P, 
 __global__ __launch_bounds__(256) void k_moments_soa(int64_t N, const double *x, const double *y,
                                                      const double *z, const double *px,
                                                      const double *py, const double *pz, double *partials) {
-    __shared__ double lds[4 * 28 * 2];
-    double m[28];
-    for (int k = 0; k < 28; ++k) m[k] = 0.0;
+    __shared__ double lds[4 * kMom * 2];
+    double m[kMom];
+    for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
         accumulate_moments(m, x[i], y[i], z[i], px[i], py[i], pz[i]);
     store_moment_partials(m, partials, lds);
@@ -817,12 +819,12 @@ void launch_moments_soa(hipStream_t s, const Bunch &b, double *partials) {
 
 __global__ __launch_bounds__(512) void k_stats_from_partials(const double *partials, int nb, int64_t N,
                                                              double mass_eV, StatsOut *out) {
-    __shared__ double lds[16 * 28 * 2];
-    double m[28];
-    reduce_partials_dd<28>(partials, nb, m, lds);
+    __shared__ double lds[16 * kMom * 2];
+    double m[kMom];
+    reduce_partials_dd<kMom>(partials, nb, m, lds);
     if (threadIdx.x != 0) return;
     StatsOut s;
-    beam_parameters(N, m, m[27], mass_eV, s);
+    beam_parameters(N, m, mass_eV, s);
     s.t = 0.0;
     s.pathLength = 0.0;
     s.dropped = 0;
@@ -1139,13 +1141,15 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
 // (timeIntegration2), transformBunch with the reference-frame update, and the
 // moment sums of the step's statistics.  NEXT: also the next step's first half
 // push (timeIntegration1 with the updated P), so that pass reads positions only.
+// Four waves per SIMD (at most 128 VGPRs) keep enough loads in flight.
 template <typename T, bool NEXT>
-__global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, double *y, double *z,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma(
+    int64_t N, double *x, double *y, double *z,
                                                      double *px, double *py, double *pz,
                                                      const typename vec4_of<T>::type *Eg,
                                                      const Scalars *sc, const TrackState *ts,
                                                      double *partials) {
-    __shared__ double lds[4 * 28 * 2];
+    __shared__ double lds[4 * kMom * 2];
     const bool sc_on = ts->space_charge != 0;
     const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
@@ -1168,8 +1172,8 @@ __global__ __launch_bounds__(256) void k_track_gamma(int64_t N, double *x, doubl
     const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
     const double e_kick = 0.5 * dT * charge * kC / mass, b_kick = 0.5 * dT * charge * kC * kC / mass;
     const double half_cdt = 0.5 * (kC * dT);
-    double m[28];
-    for (int k = 0; k < 28; ++k) m[k] = 0.0;
+    double m[kMom];
+    for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         double r[3] = {x[i], y[i], z[i]};
         double p[3] = {px[i], py[i], pz[i]};
@@ -1260,16 +1264,16 @@ void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const voi
 __global__ __launch_bounds__(512) void k_track_post(const double *partials, int nb, TrackState *ts,
                                                     Scalars *sc, StatsOut *out,
                                                     const unsigned long long *dropped) {
-    __shared__ double lds[16 * 28 * 2];
-    double m[28];
-    reduce_partials_dd<28>(partials, nb, m, lds);
+    __shared__ double lds[16 * kMom * 2];
+    double m[kMom];
+    reduce_partials_dd<kMom>(partials, nb, m, lds);
     if (threadIdx.x != 0) return;
     sc->psum[0] = m[1];
     sc->psum[1] = m[3];
     sc->psum[2] = m[5];
     if (out) {
         StatsOut s;
-        beam_parameters(sc->N, m, m[27], ts->mass_eV, s);
+        beam_parameters(sc->N, m, ts->mass_eV, s);
         s.t = ts->t;
         s.pathLength = ts->pathLength;
         s.dropped = dropped ? (int64_t)*dropped : 0;

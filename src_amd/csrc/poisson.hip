@@ -6,7 +6,8 @@
 // function, multiplies, and c2r-transforms back: an exact circular
 // convolution.  Here the same convolution is computed with pruned passes that
 // never touch the zero half of the padded input or the unused half of the
-// output (all complex buffers are K0 = n0+1 wide in x):
+// output (complex buffers hold K0 = n0+1 frequencies in x, stored with a row
+// pitch kp >= K0 that is a whole number of 128-byte tiles; padding stays 0):
 //
 //   X-fwd  rho (n0*n1*n2)          -> A[z<n2][y<n1][kx<K0]     r2c along x
 //   Y-fwd  A                       -> B[z<n2][ky<2n1][kx]      along y, input y<n1
@@ -120,53 +121,84 @@ __global__ __launch_bounds__(256) void k_green_standard(double *G, int n0, int n
 // DCT-I passes (even extension of each octant line, length-2n FFT, real part)
 // ---------------------------------------------------------------------------
 
+// Tile loads: every kernel first issues all of its thread's global loads into
+// registers (the loop over q is unrolled; at most QH = E/2+1 elements per
+// thread for half-length and n+1-length lines, E for full lines), then writes
+// them to LDS, so the loads of a thread are in flight together.
+template <int E>
+constexpr int half_q() { return E / 2 + 1; }
+
+// Octant line `line` = (k, j) of G (pitch n0+1, FP64) -> row of Gh (pitch gp).
 template <typename T, int E>
-__global__ __launch_bounds__(256) void k_dct_x(const double *G, T *Gh, FftPlan pl, int n0,
-                                               int nlines) {
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(const double *G, T *Gh, FftPlan pl, int n0,
+                                                       int nlines, int gp) {
     using C = typename cplx_of<T>::type;
+    constexpr int Q = half_q<E>();
     extern __shared__ char smem[];
     C *buf = (C *)smem;
-    const int W = pl.W, P = pl.P, L = pl.L, m = n0 + 1;
+    const int W = pl.W, P = pl.P, L = pl.L, m = n0 + 1, total = W * m;
+    const float rm = 1.0f / (float)m;
     const int line0 = blockIdx.x * W;
-    for (int e = threadIdx.x; e < W * m; e += kThreads) {
-        int ln = e / m, i = e - ln * m;
-        int line = line0 + ln;
-        T v = line < nlines ? (T)G[(size_t)line * m + i] : T(0);
-        buf[i * P + ln] = C{v, T(0)};
-        if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{v, T(0)};
+    const C *tw = fft_twiddles(buf, pl);
+    T r[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = min(threadIdx.x + q * kFftThreads, total - 1);  // clamped: loads stay unconditional
+        const int ln = div_small(e, m, rm), i = e - ln * m;
+        r[q] = (T)G[(size_t)min(line0 + ln, nlines - 1) * m + i];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;
+        if (e < total) {
+            const int ln = div_small(e, m, rm), i = e - ln * m;
+            buf[i * P + ln] = C{r[q], T(0)};
+            if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{r[q], T(0)};
+        }
     }
     __syncthreads();
-    fft_lds<C, E>(buf, pl, false);
-    for (int e = threadIdx.x; e < W * m; e += kThreads) {
-        int ln = e / m, k = e - ln * m;
-        int line = line0 + ln;
-        if (line < nlines) Gh[(size_t)line * m + k] = buf[k * P + ln].x;
+    fft_lds<C, E>(buf, tw, pl, false);
+    for (int e = threadIdx.x; e < total; e += kFftThreads) {
+        const int ln = div_small(e, m, rm), k = e - ln * m;
+        if (line0 + ln < nlines) Gh[(size_t)(line0 + ln) * gp + k] = buf[k * P + ln].x;
     }
 }
 
 // DCT-I along a strided axis of Gh in place: element i of line (x, o) at
-// o*ostride + x + i*S, x in [0, nx), i in [0, npts).
+// o*ostride + x + i*S, x in [x0, x0+W) (whole tiles: the row pitch is a
+// multiple of W and the padding columns hold zeros), i in [0, npts).
 template <typename T, int E>
-__global__ __launch_bounds__(256) void k_dct_strided(T *Gh, FftPlan pl, int nx, int npts,
-                                                     size_t S, size_t ostride) {
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_strided(T *Gh, FftPlan pl, int npts, size_t S,
+                                                             size_t ostride) {
     using C = typename cplx_of<T>::type;
+    constexpr int Q = half_q<E>();
     extern __shared__ char smem[];
     C *buf = (C *)smem;
-    const int W = pl.W, P = pl.P, L = pl.L;
+    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * npts;
     const int x0 = blockIdx.x * W;
     const size_t base = (size_t)blockIdx.y * ostride + x0;
     const int nn = npts - 1;
-    for (int e = threadIdx.x; e < W * npts; e += kThreads) {
-        int ln = e % W, i = e / W;
-        T v = (x0 + ln < nx) ? Gh[base + ln + i * S] : T(0);
-        buf[i * P + ln] = C{v, T(0)};
-        if (i >= 1 && i <= nn - 1) buf[(L - i) * P + ln] = C{v, T(0)};
+    const C *tw = fft_twiddles(buf, pl);
+    T r[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = min(threadIdx.x + q * kFftThreads, total - 1);
+        r[q] = Gh[base + (e & (W - 1)) + (size_t)(e >> lw) * S];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;
+        if (e < total) {
+            const int ln = e & (W - 1), i = e >> lw;
+            buf[i * P + ln] = C{r[q], T(0)};
+            if (i >= 1 && i <= nn - 1) buf[(L - i) * P + ln] = C{r[q], T(0)};
+        }
     }
     __syncthreads();
-    fft_lds<C, E>(buf, pl, false);
-    for (int e = threadIdx.x; e < W * npts; e += kThreads) {
-        int ln = e % W, k = e / W;
-        if (x0 + ln < nx) Gh[base + ln + k * S] = buf[k * P + ln].x;
+    fft_lds<C, E>(buf, tw, pl, false);
+    for (int e = threadIdx.x; e < total; e += kFftThreads) {
+        const int ln = e & (W - 1), k = e >> lw;
+        Gh[base + ln + (size_t)k * S] = buf[k * P + ln].x;
     }
 }
 
@@ -177,12 +209,15 @@ __global__ __launch_bounds__(256) void k_dct_strided(T *Gh, FftPlan pl, int nx, 
 // SRC 0: int64 fixed-point charge (value = i*q2e, then /dT, *tmp2:
 //        PartBunch.cpp:494-510); SRC 1: double rho as given.
 template <typename T, int E, int SRC>
-__global__ __launch_bounds__(256) void k_xfwd(const void *src, typename cplx_of<T>::type *A,
-                                              FftPlan pl, int n0, int nlines, const Scalars *sc) {
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_xfwd(const void *src, typename cplx_of<T>::type *A,
+                                                      FftPlan pl, int n0, int nlines, int kp,
+                                                      const Scalars *sc) {
     using C = typename cplx_of<T>::type;
+    constexpr int Q = half_q<E>();
     extern __shared__ char smem[];
     C *buf = (C *)smem;
-    const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1;
+    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, K0 = n0 + 1, total = W * n0;
+    const float rn = 1.0f / (float)n0, rk = 1.0f / (float)K0;
     const int line0 = blockIdx.x * W;
     double q2e = 1.0, dT = 1.0, tmp2 = 1.0;
     if constexpr (SRC == 0) {
@@ -190,116 +225,152 @@ __global__ __launch_bounds__(256) void k_xfwd(const void *src, typename cplx_of<
         dT = sc->dT;
         tmp2 = sc->tmp2;
     }
-    for (int e = threadIdx.x; e < W * n0; e += kThreads) {
-        int ln = e / n0, x = e - ln * n0;
-        int line = line0 + ln;
-        T v = T(0);
-        if (line < nlines) {
+    const C *tw = fft_twiddles(buf, pl);
+    using S = typename std::conditional<SRC == 0, long long, double>::type;
+    const S *s = (const S *)src;
+    S r[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = min(threadIdx.x + q * kFftThreads, total - 1);
+        const int ln = div_small(e, n0, rn), x = e - ln * n0;
+        r[q] = s[(size_t)min(line0 + ln, nlines - 1) * n0 + x];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;
+        if (e < total) {
+            const int ln = div_small(e, n0, rn), x = e - ln * n0;
             double d;
             if constexpr (SRC == 0) {
-                d = (double)((const long long *)src)[(size_t)line * n0 + x] * q2e;
+                d = (double)r[q] * q2e;
                 d = d / dT;
                 d = d * tmp2;
             } else {
-                d = ((const double *)src)[(size_t)line * n0 + x];
+                d = r[q];
             }
-            v = (T)d;
+            buf[x * P + ln] = C{(T)d, T(0)};
         }
-        buf[x * P + ln] = C{v, T(0)};
     }
-    for (int e = threadIdx.x; e < W * (L - n0); e += kThreads) {
-        int ln = e % W, x = n0 + e / W;
+    for (int e = threadIdx.x; e < W * (L - n0); e += kFftThreads) {
+        const int ln = e & (W - 1), x = n0 + (e >> lw);
         buf[x * P + ln] = C{T(0), T(0)};
     }
     __syncthreads();
-    fft_lds<C, E>(buf, pl, false);
-    for (int e = threadIdx.x; e < W * K0; e += kThreads) {
-        int ln = e / K0, k = e - ln * K0;
-        int line = line0 + ln;
-        if (line < nlines) A[(size_t)line * K0 + k] = buf[k * P + ln];
+    fft_lds<C, E>(buf, tw, pl, false);
+    for (int e = threadIdx.x; e < W * K0; e += kFftThreads) {
+        const int ln = div_small(e, K0, rk), k = e - ln * K0;
+        if (line0 + ln < nlines) A[(size_t)(line0 + ln) * kp + k] = buf[k * P + ln];
     }
 }
 
-// Strided pass over one tile of W consecutive kx at fixed outer index:
-// element i of line kx at in[obase + i*inS + kx], i < nin (zero beyond), the
-// transformed line written to out[obase_out + i*outS + kx] for i < nout.
+// Strided pass over one tile of W consecutive kx at fixed outer index o:
+// element i of line kx at in[o*inO + i*inS + kx], i < nin (zero beyond), the
+// transformed line written to out[o*outO + i*outS + kx] for i < nout.  Tiles
+// are whole (the row pitch is a multiple of W, padding columns hold zeros and
+// stay zero), so every row of a tile is one aligned 128-byte line.
 // MODE 0: forward; MODE 1: inverse; MODE 2: forward, *Ghat, inverse (z-conv).
 template <typename T, int E, int MODE>
-__global__ __launch_bounds__(256) void k_strided(const typename cplx_of<T>::type *in,
-                                                 typename cplx_of<T>::type *out, FftPlan pl,
-                                                 int K0, int nin, int nout, size_t inS,
-                                                 size_t inO, size_t outS, size_t outO,
-                                                 const T *Gh, int n0, int n1, int n2, int L1) {
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_strided(const typename cplx_of<T>::type *in,
+                                                         typename cplx_of<T>::type *out, FftPlan pl,
+                                                         int nin, int nout, size_t inS, size_t inO,
+                                                         size_t outS, size_t outO, const T *Gh,
+                                                         int gp, int n1, int n2, int L1) {
     using C = typename cplx_of<T>::type;
+    constexpr int Q = MODE == 1 ? E : E / 2;  // the inverse y pass reads whole lines
     extern __shared__ char smem[];
     C *buf = (C *)smem;
-    const int W = pl.W, P = pl.P, L = pl.L;
+    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * nin;
     const int kx0 = blockIdx.x * W;
     const int o = blockIdx.y;
     const size_t ib = (size_t)o * inO + kx0, ob = (size_t)o * outO + kx0;
-    for (int e = threadIdx.x; e < W * nin; e += kThreads) {
-        int ln = e % W, i = e / W;
-        C v = C{T(0), T(0)};
-        if (kx0 + ln < K0) v = in[ib + i * inS + ln];
-        buf[i * P + ln] = v;
+    const C *tw = fft_twiddles(buf, pl);
+    C r[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = min(threadIdx.x + q * kFftThreads, total - 1);
+        r[q] = in[ib + (size_t)(e >> lw) * inS + (e & (W - 1))];
     }
-    for (int e = threadIdx.x; e < W * (L - nin); e += kThreads) {
-        int ln = e % W, i = nin + e / W;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;
+        if (e < total) buf[(e >> lw) * P + (e & (W - 1))] = r[q];
+    }
+    for (int e = threadIdx.x; e < W * (L - nin); e += kFftThreads) {
+        const int ln = e & (W - 1), i = nin + (e >> lw);
         buf[i * P + ln] = C{T(0), T(0)};
     }
     __syncthreads();
-    fft_lds<C, E>(buf, pl, MODE == 1);
     if constexpr (MODE == 2) {
-        // o is ky; Ghat folded into the octant (Ghat[2n-k] = Ghat[k])
+        // o is ky; Ghat folded into the octant (Ghat[2n-k] = Ghat[k]), all loads
+        // of a thread issued together
+        fft_lds<C, E>(buf, tw, pl, false);
         const int fy = o <= n1 ? o : L1 - o;
-        for (int e = threadIdx.x; e < W * L; e += kThreads) {
-            int ln = e % W, kz = e / W;
-            int fz = kz <= n2 ? kz : L - kz;
-            int kx = kx0 + ln;
-            if (kx < K0) {
-                T g = Gh[((size_t)fz * (n1 + 1) + fy) * (n0 + 1) + kx];
-                C v = buf[kz * P + ln];
-                buf[kz * P + ln] = C{v.x * g, v.y * g};
-            }
+        T g[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * kFftThreads;
+            const int kz = e >> lw;
+            const int fz = kz <= n2 ? kz : L - kz;
+            g[q] = Gh[((size_t)fz * (n1 + 1) + fy) * gp + kx0 + (e & (W - 1))];
+        }
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * kFftThreads;
+            const int idx = (e >> lw) * P + (e & (W - 1));
+            const C v = buf[idx];
+            buf[idx] = C{v.x * g[q], v.y * g[q]};
         }
         __syncthreads();
-        fft_lds<C, E>(buf, pl, true);
+        fft_lds<C, E>(buf, tw, pl, true);
+    } else {
+        fft_lds<C, E>(buf, tw, pl, MODE == 1);
     }
-    for (int e = threadIdx.x; e < W * nout; e += kThreads) {
-        int ln = e % W, i = e / W;
-        if (kx0 + ln < K0) out[ob + i * outS + ln] = buf[i * P + ln];
+    for (int e = threadIdx.x; e < W * nout; e += kFftThreads) {
+        const int ln = e & (W - 1), i = e >> lw;
+        out[ob + (size_t)i * outS + ln] = buf[i * P + ln];
     }
 }
 
 // c2r along x, keep x < n0, phi = ((raw*invM)*s1)*s2.  OUTSCALE 1: s1 = vol,
 // s2 = coupling (PartBunch.cpp:542-554); OUTSCALE 0: s1 = s2 = 1.
 template <typename T, int E, int OUTSCALE>
-__global__ __launch_bounds__(256) void k_xinv(const typename cplx_of<T>::type *A, T *phi,
-                                              FftPlan pl, int n0, int nlines, double invM,
-                                              const Scalars *sc) {
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_xinv(const typename cplx_of<T>::type *A, T *phi,
+                                                      FftPlan pl, int n0, int nlines, int kp,
+                                                      double invM, const Scalars *sc) {
     using C = typename cplx_of<T>::type;
+    constexpr int Q = half_q<E>();
     extern __shared__ char smem[];
     C *buf = (C *)smem;
-    const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1;
+    const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1, total = W * K0;
+    const float rn = 1.0f / (float)n0, rk = 1.0f / (float)K0;
     const int line0 = blockIdx.x * W;
-    for (int e = threadIdx.x; e < W * K0; e += kThreads) {
-        int ln = e / K0, k = e - ln * K0;
-        int line = line0 + ln;
-        C v = C{T(0), T(0)};
-        if (line < nlines) v = A[(size_t)line * K0 + k];
-        buf[k * P + ln] = v;
-        if (k >= 1 && k <= n0 - 1) buf[(L - k) * P + ln] = C{v.x, -v.y};
+    const C *tw = fft_twiddles(buf, pl);
+    C r[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = min(threadIdx.x + q * kFftThreads, total - 1);
+        const int ln = div_small(e, K0, rk), k = e - ln * K0;
+        r[q] = A[(size_t)min(line0 + ln, nlines - 1) * kp + k];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;
+        if (e < total) {
+            const int ln = div_small(e, K0, rk), k = e - ln * K0;
+            buf[k * P + ln] = r[q];
+            if (k >= 1 && k <= n0 - 1) buf[(L - k) * P + ln] = C{r[q].x, -r[q].y};
+        }
     }
     __syncthreads();
-    fft_lds<C, E>(buf, pl, true);
+    fft_lds<C, E>(buf, tw, pl, true);
     double s1 = 1.0, s2 = 1.0;
     if constexpr (OUTSCALE == 1) {
         s1 = sc->vol;
         s2 = sc->coupling;
     }
-    for (int e = threadIdx.x; e < W * n0; e += kThreads) {
-        int ln = e / n0, x = e - ln * n0;
-        int line = line0 + ln;
+    for (int e = threadIdx.x; e < W * n0; e += kFftThreads) {
+        const int ln = div_small(e, n0, rn), x = e - ln * n0;
+        const int line = line0 + ln;
         if (line < nlines) {
             if constexpr (sizeof(T) == 8) {
                 double d = buf[x * P + ln].x * invM;
@@ -375,25 +446,43 @@ __global__ void k_widen(const T *in, double *out, size_t n, int comps, int strid
 // Host side
 // ---------------------------------------------------------------------------
 
+// Tile shape for length L: the smallest E (elements per thread) whose tile has
+// W >= 128 bytes / complex size lines, within the register budget.
 static bool make_plan(int L, bool fp64, FftPlan &pl) {
     int a = 0, b = 0, m = L;
     while (m % 2 == 0) { m /= 2; ++a; }
     while (m % 3 == 0) { m /= 3; ++b; }
     if (m != 1 || b > 1 || a < 2) return false;
     pl.L = L;
-    if (b == 0) pl.E = fp64 ? 8 : 16;
-    else pl.E = fp64 ? 12 : 24;
-    if ((pl.E * kThreads) % L != 0) return false;
-    pl.W = pl.E * kThreads / L;
+    const int wmin = fp64 ? 8 : 16;
+    static const int c2[] = {4, 8, 16, 32}, c3[] = {6, 12, 24, 48};
+    const int *cand = b ? c3 : c2;
+    const int emax = fp64 ? (b ? 24 : 16) : (b ? 48 : 32);
+    pl.E = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int e = cand[c];
+        if (e > emax || (e * kFftThreads) % L != 0) continue;
+        pl.E = e;
+        if (e * kFftThreads / L >= wmin) break;
+    }
+    if (pl.E == 0) return false;
+    pl.W = pl.E * kFftThreads / L;
     if (pl.W < 1) return false;
+    pl.lw = 0;
+    while ((1 << pl.lw) < pl.W) ++pl.lw;
+    if ((1 << pl.lw) != pl.W) return false;
     pl.P = pl.W + 1;
     pl.nst = 0;
-    const bool use8 = (pl.E % 8) == 0;
+    // every radix must divide E (each thread owns E/R whole butterflies)
+    const bool use8 = (pl.E % 8) == 0, use4 = (pl.E % 4) == 0;
     int rem = a;
-    if (b) pl.radix[pl.nst++] = 3;
     while (rem >= 3 && use8) { pl.radix[pl.nst++] = 8; rem -= 3; }
-    while (rem >= 2) { pl.radix[pl.nst++] = 4; rem -= 2; }
-    if (rem == 1) pl.radix[pl.nst++] = 2;
+    while (rem >= 2 && use4) { pl.radix[pl.nst++] = 4; rem -= 2; }
+    while (rem >= 1) { pl.radix[pl.nst++] = 2; rem -= 1; }
+    if (b) pl.radix[pl.nst++] = 3;  // last: every earlier span is a power of two
+    if (pl.nst > 16) return false;
+    for (int s = 0; s < pl.nst; ++s)
+        if (pl.E % pl.radix[s] != 0) return false;
     return true;
 }
 
@@ -409,14 +498,21 @@ static void make_twiddles(int L, std::vector<char> &out) {
     }
 }
 
-static size_t lds_bytes(const FftPlan &pl, size_t csize) { return (size_t)pl.L * pl.P * csize; }
+// tile + (FP32) twiddle table
+static size_t lds_bytes(const FftPlan &pl, size_t csize) {
+    return (size_t)pl.L * pl.P * csize + (csize == 8 ? (size_t)pl.L * csize : 0);
+}
 
 #define OPALSC_DISPATCH_E(E_, ...)                             \
     switch (E_) {                                              \
+    case 4: { constexpr int EV = 4; __VA_ARGS__; } break;      \
+    case 6: { constexpr int EV = 6; __VA_ARGS__; } break;      \
     case 8: { constexpr int EV = 8; __VA_ARGS__; } break;      \
     case 12: { constexpr int EV = 12; __VA_ARGS__; } break;    \
     case 16: { constexpr int EV = 16; __VA_ARGS__; } break;    \
     case 24: { constexpr int EV = 24; __VA_ARGS__; } break;    \
+    case 32: if constexpr (sizeof(T) == 4) { constexpr int EV = 32; __VA_ARGS__; } break; \
+    case 48: if constexpr (sizeof(T) == 4) { constexpr int EV = 48; __VA_ARGS__; } break; \
     default: break;                                            \
     }
 
@@ -435,18 +531,42 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
     }
     K0 = n[0] + 1;
     const size_t cs = fp64 ? 16 : 8, rs = fp64 ? 8 : 4;
+    int lds_max = 0;
+    {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            lds_max = 65536;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (lds_bytes(plan[d], cs) > (size_t)lds_max) {
+            err = "grid size n=" + std::to_string(n[d]) + " needs more LDS per block than the device has";
+            return -1;
+        }
+    // tile-aligned pitches: whole 128-byte rows in every strided pass
+    const int tilew = std::max(std::max(plan[1].W, plan[2].W), (int)(128 / cs));
+    kp = (K0 + tilew - 1) / tilew * tilew;
+    const int gtile = (int)(128 / rs);
+    gp = (kp + gtile - 1) / gtile * gtile;
     ncell = (size_t)n[0] * n[1] * n[2];
     noct = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
-    const size_t szA = (size_t)K0 * n[1] * n[2] * cs;
-    const size_t szB = (size_t)K0 * L[1] * n[2] * cs;
+    const size_t szA = (size_t)kp * n[1] * n[2] * cs;
+    const size_t szB = (size_t)kp * L[1] * n[2] * cs;
+    const size_t szG = (size_t)gp * (n[1] + 1) * (n[2] + 1) * rs;
     auto alloc = [&](void **p, size_t bytes) -> bool {
         if (hipMalloc(p, bytes) != hipSuccess) { err = "hipMalloc failed (" + std::to_string(bytes) + " bytes)"; return false; }
         return true;
     };
     if (!alloc(&rho_i64, ncell * 8) || !alloc(&rho_f64, ncell * 8) || !alloc(&A, szA) ||
-        !alloc(&B, szB) || !alloc(&Goct, noct * 8) || !alloc(&Ghat, noct * rs) ||
+        !alloc(&B, szB) || !alloc(&Goct, noct * 8) || !alloc(&Ghat, szG) ||
         !alloc(&phi, ncell * rs) || !alloc(&Eg, ncell * 4 * rs))
         return -1;
+    // padding columns are read as whole tiles and must hold (and keep) zeros
+    if (hipMemset(A, 0, szA) != hipSuccess || hipMemset(B, 0, szB) != hipSuccess ||
+        hipMemset(Ghat, 0, szG) != hipSuccess) {
+        err = "hipMemset failed";
+        return -1;
+    }
     for (int d = 0; d < 3; ++d) {
         std::vector<char> tw;
         if (fp64) make_twiddles<double>(L[d], tw);
@@ -482,26 +602,25 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
+    const size_t gp = (size_t)g.gp;
     {
         const FftPlan &pl = g.plan[0];
         int nlines = (n1 + 1) * (n2 + 1);
         dim3 grid((nlines + pl.W - 1) / pl.W);
-        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                    (const double *)g.Goct, (T *)g.Ghat, pl, n0, nlines));
+        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (const double *)g.Goct, (T *)g.Ghat, pl, n0, nlines, g.gp));
     }
     {
         const FftPlan &pl = g.plan[1];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                    (T *)g.Ghat, pl, n0 + 1, n1 + 1, (size_t)(n0 + 1),
-                                    (size_t)(n0 + 1) * (n1 + 1)));
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (T *)g.Ghat, pl, n1 + 1, gp, gp * (n1 + 1)));
     }
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                    (T *)g.Ghat, pl, n0 + 1, n2 + 1,
-                                    (size_t)(n0 + 1) * (n1 + 1), (size_t)(n0 + 1)));
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (T *)g.Ghat, pl, n2 + 1, gp * (n1 + 1), gp));
     }
     g.profile_mark(PROF_DCT);
 }
@@ -516,42 +635,41 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], K0 = g.K0;
-    const int L1 = g.L[1], L2 = g.L[2];
+    const int L1 = g.L[1];
+    const size_t kp = (size_t)g.kp;
     C *A = (C *)g.A, *B = (C *)g.B;
     const int nlines = n1 * n2;
     {
         const FftPlan &pl = g.plan[0];
         dim3 grid((nlines + pl.W - 1) / pl.W);
         if (src_kind == 0)
-            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 0><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                        g.rho_i64, A, pl, n0, nlines, d_sc))
+            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        g.rho_i64, A, pl, n0, nlines, g.kp, d_sc))
         else
-            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 1><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                        g.rho_f64, A, pl, n0, nlines, d_sc))
+            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        g.rho_f64, A, pl, n0, nlines, g.kp, d_sc))
     }
     g.profile_mark(PROF_FFT_X);
     {
         const FftPlan &pl = g.plan[1];
         dim3 grid((K0 + pl.W - 1) / pl.W, n2);
-        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 0><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                    A, B, pl, K0, n1, L1, (size_t)K0, (size_t)K0 * n1, (size_t)K0,
-                                    (size_t)K0 * L1, nullptr, n0, n1, n2, L1));
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    A, B, pl, n1, L1, kp, kp * n1, kp, kp * L1, nullptr, 0, n1, n2, L1));
     }
     g.profile_mark(PROF_FFT_Y);
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((K0 + pl.W - 1) / pl.W, L1);
-        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                    B, B, pl, K0, n2, n2, (size_t)K0 * L1, (size_t)K0,
-                                    (size_t)K0 * L1, (size_t)K0, (const T *)g.Ghat, n0, n1, n2, L1));
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Ghat, g.gp,
+                                    n1, n2, L1));
     }
     g.profile_mark(PROF_FFT_Z);
     {
         const FftPlan &pl = g.plan[1];
         dim3 grid((K0 + pl.W - 1) / pl.W, n2);
-        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 1><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                    B, A, pl, K0, L1, n1, (size_t)K0, (size_t)K0 * L1, (size_t)K0,
-                                    (size_t)K0 * n1, nullptr, n0, n1, n2, L1));
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    B, A, pl, L1, n1, kp, kp * L1, kp, kp * n1, nullptr, 0, n1, n2, L1));
     }
     g.profile_mark(PROF_FFT_Y);
     {
@@ -559,14 +677,13 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
         dim3 grid((nlines + pl.W - 1) / pl.W);
         const double invM = 1.0 / ((double)(2 * n0) * (double)(2 * n1) * (double)(2 * n2));
         if (outscale)
-            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                        A, (T *)g.phi, pl, n0, nlines, invM, d_sc))
+            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        A, (T *)g.phi, pl, n0, nlines, g.kp, invM, d_sc))
         else
-            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 0><<<grid, 256, lds_bytes(pl, cs), g.stream>>>(
-                                        A, (T *)g.phi, pl, n0, nlines, invM, d_sc))
+            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        A, (T *)g.phi, pl, n0, nlines, g.kp, invM, d_sc))
     }
     g.profile_mark(PROF_FFT_X);
-    (void)L2;
 }
 
 void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale) {

@@ -66,7 +66,7 @@ def test_sort_by_cell_is_stable_and_bit_exact():
 
 
 @pytest.mark.parametrize("greens", [0, 1])
-@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12), (8, 48, 12), (64, 32, 96)])
 def test_compute_potential_fp64(n, greens):
     rng = np.random.default_rng(4)
     rho = rng.normal(size=n[0] * n[1] * n[2])
@@ -75,6 +75,18 @@ def test_compute_potential_fp64(n, greens):
     with ctx(n, greens=greens) as c:
         phi = c.computePotential(rho, hr)
     assert rel_l2(phi, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (8, 48, 12), (128, 64, 32)])
+def test_compute_potential_mixed(n):
+    """FP32 convolution (FP64 Green's function): tolerance 1e-5 relative L2."""
+    rng = np.random.default_rng(9)
+    rho = rng.normal(size=n[0] * n[1] * n[2])
+    hr = np.array([1.1e-4, 0.9e-4, 2.3e-4])
+    ref = O.compute_potential(n, rho, hr, 0)
+    with ctx(n, precision=sc.MIXED) as c:
+        phi = c.computePotential(rho, hr)
+    assert rel_l2(phi, ref) < 1e-5
 
 
 def _self_fields_case(n, N, seed, precision=sc.FP64):
