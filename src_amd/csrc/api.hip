@@ -616,7 +616,7 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     if (nsteps == 0) return OPALSC_OK;
     if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps)) FAIL(c, OPALSC_ENOMEM, "stats buffer");
     Bunch &b = c->bunch;
-    const int nb = red_blocks(b.N);
+    const int nb = gamma_blocks(c->grid.fp64, b.N);
     const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     c->prof.start();

@@ -54,8 +54,9 @@ struct PoissonGrid {
     void *rho_i64 = nullptr;  // long long[ncell]
     void *rho_f64 = nullptr;  // double[ncell] (compute_potential input)
     void *A = nullptr, *B = nullptr;
-    void *Goct = nullptr;     // double[noct]
-    void *Ghat = nullptr;     // T[noct]
+    void *Goct = nullptr;     // double[noct]: G on the octant
+    void *Tcorner = nullptr;  // double[(n0+2)(n1+2)(n2+2)]: IGF corner antiderivatives
+    void *Ghat = nullptr;     // T[n2+1][n1+1][gp]
     void *phi = nullptr;      // T[ncell]
     void *Eg = nullptr;       // vec4<T>[ncell]
     hipStream_t stream = nullptr;
@@ -136,6 +137,7 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
                         const TrackState *ts, double *partials);
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, bool exact, Scalars *sc);
+int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                         const TrackState *ts, double *partials);
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,

@@ -1174,6 +1174,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double half_cdt = 0.5 * (kC * dT);
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
+    // grid-stride: the whole grid streams one window of consecutive particles
+    // at a time (contiguous per-block ranges measured slower)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         double r[3] = {x[i], y[i], z[i]};
         double p[3] = {px[i], py[i], pz[i]};
@@ -1243,12 +1245,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     store_moment_partials(m, partials, lds);
 }
+// Blocks of `kernel` (kThreads each) that the device holds at once, per device.
+template <typename K>
+static int resident_blocks(K kernel) {
+    static int cache[64] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cache[dev] == 0) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0) != hipSuccess || per <= 0)
+            per = 1;
+        cache[dev] = cus * per;
+    }
+    return cache[dev];
+}
+
+// One wave of resident blocks (fewer partial sums for the step statistics).
+int gamma_blocks(bool fp64, int64_t N) {
+    const int r = fp64 ? resident_blocks(k_track_gamma<double, true>) : resident_blocks(k_track_gamma<float, true>);
+    const int b = red_blocks(N);
+    return b < r ? b : r;
+}
+
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                         const TrackState *ts, double *partials) {
     if (b.N <= 0) return;
+    const int blocks = gamma_blocks(fp64, b.N);
 #define OPALSC_GAMMA(T_, V_, N_)                                                                          \
-    k_track_gamma<T_, N_><<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,      \
-                                                               (const V_ *)Eg, sc, ts, partials)
+    k_track_gamma<T_, N_><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,               \
+                                                      (const V_ *)Eg, sc, ts, partials)
     if (fp64) {
         if (next) OPALSC_GAMMA(double, double4, true);
         else OPALSC_GAMMA(double, double4, false);

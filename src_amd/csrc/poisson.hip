@@ -66,37 +66,44 @@ __device__ inline double igf_corner(int i, int j, int k, double h0, double h1, d
     return t / cellVolume;
 }
 
-// G octant [0,n0]x[0,n1]x[0,n2] from corner antiderivatives T on an LDS tile
-// (8x8x4 G values from 9x9x5 corners), 8-corner difference in the reference's
-// order (FFTPoissonSolver.cpp:510-517) and G[0,0,0] := G[0,0,1]
-// (FFTPoissonSolver.cpp:657).  Index n uses T at n+1 from the same formula;
-// the physical potential never reads G at index n.
+// Corner antiderivatives T[k][j][i], i,j,k in [0, n+1], each evaluated once
+// (FFTPoissonSolver.cpp:472-491; the reference's tmpgreen field).  Grid:
+// x over the (n0+2)*(n1+2) corners of a plane, y over k.
 template <bool CR>
-__global__ __launch_bounds__(256) void k_igf_octant(double *G, int n0, int n1, int n2,
-                                                    const Scalars *sc) {
-    __shared__ double Tc[5][9][9];
+__global__ __launch_bounds__(256) void k_igf_corners(double *T, int n0, int n1, int n2,
+                                                     const Scalars *sc) {
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2];
     const double vol = h0 * h1 * h2;
-    const int gx0 = blockIdx.x * 8, gy0 = blockIdx.y * 8, gz0 = blockIdx.z * 4;
-    for (int c = threadIdx.x; c < 405; c += 256) {
-        int cx = c % 9, cy = (c / 9) % 9, cz = c / 81;
-        Tc[cz][cy][cx] = igf_corner<CR>(gx0 + cx, gy0 + cy, gz0 + cz, h0, h1, h2, vol);
-    }
-    __syncthreads();
-    const int tx = threadIdx.x & 7, ty = (threadIdx.x >> 3) & 7, tz = threadIdx.x >> 6;
-    const int i = gx0 + tx, j = gy0 + ty, k = gz0 + tz;
-    if (i > n0 || j > n1 || k > n2) return;
-    int ax = tx, ay = ty, az = tz;
-    if (i == 0 && j == 0 && k == 0) az = 1;  // quirk: G[0,0,0] = G[0,0,1]
-    double g = Tc[az + 1][ay + 1][ax + 1];
-    g += Tc[az][ay][ax + 1];
-    g += Tc[az][ay + 1][ax];
-    g += Tc[az + 1][ay][ax];
-    g -= Tc[az][ay + 1][ax + 1];
-    g -= Tc[az + 1][ay][ax + 1];
-    g -= Tc[az + 1][ay + 1][ax];
-    g -= Tc[az][ay][ax];
-    G[((size_t)k * (n1 + 1) + j) * (n0 + 1) + i] = g;
+    const int m0 = n0 + 2, plane = m0 * (n1 + 2);
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= plane) return;
+    const int j = div_small(c, m0, 1.0f / (float)m0), i = c - j * m0, k = blockIdx.y;
+    T[(size_t)k * plane + c] = igf_corner<CR>(i, j, k, h0, h1, h2, vol);
+}
+
+// G octant [0,n0]x[0,n1]x[0,n2] from T: the 8-corner difference in the
+// reference's order (FFTPoissonSolver.cpp:510-517) with G[0,0,0] := G[0,0,1]
+// (FFTPoissonSolver.cpp:657).  Index n uses T at n+1 from the same formula;
+// the physical potential never reads G at index n.  Grid: x over the
+// (n0+1)*(n1+1) points of a plane, y over k.
+__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1) {
+    const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= gplane) return;
+    const int j = div_small(c, g0, 1.0f / (float)g0), i = c - j * g0;
+    int k = blockIdx.y;
+    const int kq = (i == 0 && j == 0 && k == 0) ? 1 : k;  // quirk
+    const size_t m0 = (size_t)n0 + 2, plane = m0 * (n1 + 2);
+    const double *t = T + (size_t)kq * plane + (size_t)j * m0 + i;
+    double g = t[plane + m0 + 1];
+    g += t[1];
+    g += t[m0];
+    g += t[plane];
+    g -= t[m0 + 1];
+    g -= t[plane + 1];
+    g -= t[plane + m0];
+    g -= t[0];
+    G[(size_t)k * gplane + c] = g;
 }
 
 // STANDARD Green's function, SpecializedGreensFunction<3>::calculate
@@ -130,8 +137,8 @@ constexpr int half_q() { return E / 2 + 1; }
 
 // Octant line `line` = (k, j) of G (pitch n0+1, FP64) -> row of Gh (pitch gp).
 template <typename T, int E>
-__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(const double *G, T *Gh, FftPlan pl, int n0,
-                                                       int nlines, int gp) {
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(
+    const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
     extern __shared__ char smem[];
@@ -558,7 +565,8 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
         return true;
     };
     if (!alloc(&rho_i64, ncell * 8) || !alloc(&rho_f64, ncell * 8) || !alloc(&A, szA) ||
-        !alloc(&B, szB) || !alloc(&Goct, noct * 8) || !alloc(&Ghat, szG) ||
+        !alloc(&B, szB) || !alloc(&Goct, noct * 8) ||
+        !alloc(&Tcorner, (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8) || !alloc(&Ghat, szG) ||
         !alloc(&phi, ncell * rs) || !alloc(&Eg, ncell * 4 * rs))
         return -1;
     // padding columns are read as whole tiles and must hold (and keep) zeros
@@ -582,22 +590,25 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
 }
 
 void PoissonGrid::destroy() {
-    void *ps[] = {rho_i64, rho_f64, A, B, Goct, Ghat, phi, Eg, twd[0], twd[1], twd[2]};
+    void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    rho_i64 = rho_f64 = A = B = Goct = Ghat = phi = Eg = nullptr;
+    rho_i64 = rho_f64 = A = B = Goct = Tcorner = Ghat = phi = Eg = nullptr;
     twd[0] = twd[1] = twd[2] = nullptr;
 }
 
 template <typename T>
 static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    if (g.greens == 1) {
+    const bool igf = g.greens != 1;
+    if (!igf) {
         k_green_standard<<<1024, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
     } else {
-        dim3 grid((n0 + 1 + 7) / 8, (n1 + 1 + 7) / 8, (n2 + 1 + 3) / 4);
-        if (cr) k_igf_octant<true><<<grid, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
-        else k_igf_octant<false><<<grid, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
+        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
+        if (cr) k_igf_corners<true><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
+        else k_igf_corners<false><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
+        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
+        k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
