@@ -76,6 +76,9 @@ struct PoissonGrid {
     }
 };
 
+constexpr int kDepositChunk = 2048;  // particles per deposit / bounds chunk
+constexpr int kBoxStats = 12;        // per chunk: min[3], max[3], sum[3], sum of squares[3]
+
 // Particle buffers in structure-of-arrays form.
 struct Bunch {
     int64_t cap = 0, N = 0;
@@ -85,7 +88,7 @@ struct Bunch {
     uint32_t *id = nullptr;
     double *tmp = nullptr;       // permutation scratch
     double *qd = nullptr;        // Q*dt (self-field entry point)
-    double *chunkbox = nullptr;  // per-chunk beam-frame bounding boxes (6 doubles per 2048 particles)
+    double *chunkbox = nullptr;  // per-chunk beam-frame position statistics (kBoxStats doubles)
     uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;

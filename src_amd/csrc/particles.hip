@@ -83,6 +83,29 @@ __device__ inline void block_minmax(double (&mn)[3], double (&mx)[3], double *ld
     __syncthreads();
 }
 
+// Block min/max and sums of positions and squared positions (the chunk
+// statistics the deposit sizes its LDS box from); result valid in thread 0.
+__device__ inline void block_chunk_stats(double (&st)[kBoxStats], double *lds) {
+    for (int d = 0; d < 3; ++d) {
+        st[d] = wave_min(st[d]);
+        st[3 + d] = wave_max(st[3 + d]);
+    }
+    for (int k = 6; k < kBoxStats; ++k) st[k] = wave_sum(st[k]);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < kBoxStats; ++k) lds[wave * kBoxStats + k] = st[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < nw; ++w) {
+            for (int d = 0; d < 3; ++d) {
+                st[d] = fmin(st[d], lds[w * kBoxStats + d]);
+                st[3 + d] = fmax(st[3 + d], lds[w * kBoxStats + 3 + d]);
+            }
+            for (int k = 6; k < kBoxStats; ++k) st[k] += lds[w * kBoxStats + k];
+        }
+    __syncthreads();
+}
+
 // Sum K dd values over nb partial blocks (slots 2k, 2k+1) with one block of
 // 256 threads; result valid in thread 0.
 template <int K>
@@ -248,74 +271,16 @@ void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int
 // particles are processed.  MODE 0: qd given; MODE 1: qd = Q*dT with Q
 // updated to (Q*dT)/dT as the reference leaves it (PartBunch.cpp:493-495).
 
-// Sum the 8 corner contributions over runs of equal lgp key in consecutive
-// lanes (particles sorted by cell form long runs).  Returns true on the last
-// lane of each run, which then holds the run's sums.  Every lane of the
-// wave must call this.
-// Wave-wide inclusive prefix sum with DPP lane moves (no LDS traffic): four
-// row shifts build prefix sums inside each 16-lane row, two row broadcasts
-// carry the row totals forward (the GFX9 wave64 scan).  Lanes shifted in from
-// outside a row read 0.
-__device__ inline int dpp_shr(int v, int ctrl_sel) {
-    switch (ctrl_sel) {
-    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
-    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
-    case 4: return __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
-    case 8: return __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
-    case 15: return __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1,3
-    default: return __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2,3
-    }
-}
-__device__ inline int wave_incl_scan(int v) {
-    v += dpp_shr(v, 1);
-    v += dpp_shr(v, 2);
-    v += dpp_shr(v, 4);
-    v += dpp_shr(v, 8);
-    v += dpp_shr(v, 15);
-    v += dpp_shr(v, 31);
-    return v;
-}
-__device__ inline long long wave_incl_scan(long long v) {
-    const int steps[6] = {1, 2, 4, 8, 15, 31};
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const unsigned lo = (unsigned)dpp_shr((int)(unsigned)v, steps[s]);
-        const unsigned hi = (unsigned)dpp_shr((int)(unsigned)((unsigned long long)v >> 32), steps[s]);
-        v += (long long)(((unsigned long long)hi << 32) | lo);
-    }
-    return v;
-}
-
-template <typename V>
-__device__ inline bool wave_run_sums(int key, V (&v)[8]) {
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(key, 1, 64);
-    const int next = __shfl_down(key, 1, 64);
-    const bool head = lane == 0 || prev != key;
-    const bool tail = lane == 63 || next != key;
-    const unsigned long long heads = __ballot(head);
-    if (heads == ~0ull) return true;  // no two neighbours share a cell: nothing to merge
-    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-    const int start = 63 - __builtin_clzll(heads & upto);
-    const int src = start > 0 ? start - 1 : 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const V s = wave_incl_scan(v[c]);
-        const V before = __shfl(s, src, 64);
-        v[c] = start > 0 ? s - before : s;
-    }
-    return tail;
-}
-
-// LDS-tiled deposit.  A block owns a contiguous chunk of kChunk particles,
-// loads all of them into registers at once (kPPT loads per array in flight per
-// thread) and bounds the chunk's cells with a block min/max of the positions
-// (or takes the box the push pass recorded); if the target box fits kBoxCap
-// cells, contributions are accumulated in an LDS copy of the box (64-bit LDS
+// LDS-tiled deposit.  A block owns a contiguous chunk of kChunk particles;
+// each thread loads its kPPT consecutive particles into registers at once and
+// the block bounds the chunk's cells with a min/max of the positions (or takes
+// the box the bounds pass recorded).  If the target box fits kBoxCap cells,
+// contributions are accumulated in an LDS copy of the box (64-bit LDS
 // atomics) and the non-zero cells are flushed to HBM with one global atomic
-// each; otherwise they go straight to HBM.  Before either, each wave sums runs
-// of equal cells (wave_run_sums), so a bunch sorted by cell issues one atomic
-// per run and corner instead of one per particle.  Integer addition makes the
+// each; otherwise they go straight to HBM.  Before either, a thread sums its
+// consecutive particles of one cell in registers, so a bunch sorted by cell
+// issues one atomic per run and corner instead of one per particle, and lanes
+// of a wave rarely hit the same LDS address.  Integer addition makes the
 // result independent of all of this.
 constexpr int kPPT = 8;
 constexpr int kChunk = kThreads * kPPT;
@@ -334,7 +299,7 @@ constexpr int kBoxCap = 5632;  // 44 KiB of int64
 // exactly as the bounds pass rotated them.  EXACT: the FP64
 // path (reference-order CIC, 64-bit fixed point per deposit); otherwise the
 // mixed path (CIC with 1/h, FP32 weights, |deposit| < 2^24 in 32-bit integers,
-// summed in 64 bits).  BOX: chunk bounding boxes supplied by the push pass.
+// summed in 64 bits).  BOX: chunk statistics supplied by the bounds pass.
 template <int MODE, bool EXACT, bool BOX>
 __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__restrict__ x,
                                                      const double *__restrict__ y,
@@ -343,7 +308,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
                                                      const Scalars *sc, unsigned long long *dropped,
                                                      const double *chunkbox) {
     __shared__ long long box[kBoxCap];
-    __shared__ double red[4 * 6];
+    __shared__ double red[4 * kBoxStats];
     __shared__ int boxinfo[7];
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
@@ -352,17 +317,35 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     const double e2q = sc->e2q, dT = sc->dT;
     const int64_t base = (int64_t)blockIdx.x * kChunk;
 
-    // 1. the chunk's particles, all loads issued before any is used
+    // 1. the chunk's particles, all loads issued before any is used; thread t
+    // owns the kPPT consecutive particles p0 = base + t*kPPT ... p0 + kPPT-1
+    const int64_t p0 = base + (int64_t)threadIdx.x * kPPT;
+    const double *qsrc = MODE == 0 ? qdv : q;
     double px[kPPT], py[kPPT], pz[kPPT], pq[kPPT];
+    if (p0 + kPPT <= N) {
 #pragma unroll
-    for (int k = 0; k < kPPT; ++k) {
-        const int64_t p = base + k * kThreads + threadIdx.x;
-        const bool in = p < N;
-        px[k] = in ? x[p] : 0.0;
-        py[k] = in ? y[p] : 0.0;
-        pz[k] = in ? z[p] : 0.0;
-        if constexpr (MODE == 0) pq[k] = in ? qdv[p] : 0.0;
-        else pq[k] = in ? q[p] : 0.0;
+        for (int m = 0; m < kPPT / 2; ++m) {
+            const double2 tx = ((const double2 *)(x + p0))[m], ty = ((const double2 *)(y + p0))[m];
+            const double2 tz = ((const double2 *)(z + p0))[m], tq = ((const double2 *)(qsrc + p0))[m];
+            px[2 * m] = tx.x;
+            px[2 * m + 1] = tx.y;
+            py[2 * m] = ty.x;
+            py[2 * m + 1] = ty.y;
+            pz[2 * m] = tz.x;
+            pz[2 * m + 1] = tz.y;
+            pq[2 * m] = tq.x;
+            pq[2 * m + 1] = tq.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k) {
+            const int64_t p = p0 + k;
+            const bool in = p < N;
+            px[k] = in ? x[p] : 0.0;
+            py[k] = in ? y[p] : 0.0;
+            pz[k] = in ? z[p] : 0.0;
+            pq[k] = in ? qsrc[p] : 0.0;
+        }
     }
     if constexpr (MODE == 1) {
         // the tracker keeps positions in the reference frame; the mesh is in the beam frame
@@ -380,49 +363,83 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
         }
     }
 
-    // 2. bounding box of the chunk (positions, then a safe index margin)
-    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
-    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    // 2. the chunk's LDS box: its bounding box if that fits kBoxCap cells, else
+    // the largest box about the mean (mean +- a*rms per axis) that fits, with
+    // the particles outside it going to HBM directly
+    double st[kBoxStats];
     if constexpr (BOX) {
-        for (int d = 0; d < 3; ++d) {
-            mn[d] = chunkbox[blockIdx.x * 6 + d];
-            mx[d] = chunkbox[blockIdx.x * 6 + 3 + d];
-        }
+        for (int k = 0; k < kBoxStats; ++k) st[k] = chunkbox[blockIdx.x * kBoxStats + k];
     } else {
+        for (int d = 0; d < 3; ++d) {
+            st[d] = __builtin_inf();
+            st[3 + d] = -__builtin_inf();
+            st[6 + d] = st[9 + d] = 0.0;
+        }
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) {
-            if (base + k * kThreads + threadIdx.x < N) {
-                mn[0] = fmin(mn[0], px[k]);
-                mx[0] = fmax(mx[0], px[k]);
-                mn[1] = fmin(mn[1], py[k]);
-                mx[1] = fmax(mx[1], py[k]);
-                mn[2] = fmin(mn[2], pz[k]);
-                mx[2] = fmax(mx[2], pz[k]);
+            if (p0 + k < N) {
+                const double r[3] = {px[k], py[k], pz[k]};
+                for (int d = 0; d < 3; ++d) {
+                    st[d] = fmin(st[d], r[d]);
+                    st[3 + d] = fmax(st[3 + d], r[d]);
+                    st[6 + d] += r[d];
+                    st[9 + d] += r[d] * r[d];
+                }
             }
         }
-        block_minmax(mn, mx, red);
+        block_chunk_stats(st, red);
     }
     if (threadIdx.x == 0) {
         const double o[3] = {o0, o1, o2}, h[3] = {h0, h1, h2};
         const int n[3] = {n0, n1, n2};
-        long long vol = 1;
-        bool ok = true;
-        for (int d = 0; d < 3; ++d) {
-            // (x-o)/h is monotonic in x, so the chunk's extreme cells follow from its
-            // extreme positions: lgp is in [floor(t)-1, floor(t)] for t >= 0 and
-            // >= floor(t)-2 below the origin; corners reach lgp+1.
-            const double tl = (mn[d] - o[d]) / h[d], th = (mx[d] - o[d]) / h[d];
-            if (!(tl > -1e9 && th < 1e9)) {
-                ok = false;
-                break;
-            }
-            int lo = (int)floor(tl) - (tl < 0.0 ? 2 : 1), hi = (int)floor(th) + 1;
+        const double cnt = (double)(N - base < kChunk ? N - base : kChunk);
+        // cells [lo, hi] whose corners particles in [a, b] can reach: (x-o)/h is
+        // monotonic in x; lgp is in [floor(t)-1, floor(t)] for t >= 0 and
+        // >= floor(t)-2 below the origin; corners reach lgp+1
+        auto span = [&](int d, double a, double b, int &lo, int &hi) -> bool {
+            const double tl = (a - o[d]) / h[d], th = (b - o[d]) / h[d];
+            if (!(tl > -1e9 && th < 1e9)) return false;
+            lo = (int)floor(tl) - (tl < 0.0 ? 2 : 1);
+            hi = (int)floor(th) + 1;
             lo = lo < 0 ? 0 : lo;
             hi = hi > n[d] - 1 ? n[d] - 1 : hi;
             if (hi < lo) hi = lo;
-            boxinfo[d] = lo;
-            boxinfo[3 + d] = hi - lo + 1;
-            vol *= (hi - lo + 1);
+            return true;
+        };
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        long long vol = 1;
+        bool ok = true;
+        for (int d = 0; d < 3 && ok; ++d) {
+            ok = span(d, st[d], st[3 + d], lo[d], hi[d]);
+            vol *= (hi[d] - lo[d] + 1);
+        }
+        if (ok && vol > kBoxCap) {
+            static constexpr double kA[5] = {3.0, 2.0, 1.5, 1.0, 0.5};
+            ok = false;
+            for (int t = 0; t < 5 && !ok; ++t) {
+                int l2[3], h2[3];
+                long long v2 = 1;
+                bool fin = true;
+                for (int d = 0; d < 3; ++d) {
+                    const double mu = st[6 + d] / cnt;
+                    const double var = st[9 + d] / cnt - mu * mu;
+                    const double w = kA[t] * sqrt(var > 0.0 ? var : 0.0);
+                    fin &= span(d, fmax(mu - w, st[d]), fmin(mu + w, st[3 + d]), l2[d], h2[d]);
+                    v2 *= (h2[d] - l2[d] + 1);
+                }
+                if (fin && v2 <= kBoxCap) {
+                    ok = true;
+                    vol = v2;
+                    for (int d = 0; d < 3; ++d) {
+                        lo[d] = l2[d];
+                        hi[d] = h2[d];
+                    }
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            boxinfo[d] = lo[d];
+            boxinfo[3 + d] = hi[d] - lo[d] + 1;
         }
         boxinfo[6] = (ok && vol <= kBoxCap) ? (int)vol : -1;
     }
@@ -436,14 +453,44 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
         __syncthreads();
     }
 
-    // 3. CIC, run sums, atomics
+    // 3. CIC; a thread sums its consecutive particles of one cell in registers
+    // and adds each run to the box (or the grid) when the cell changes
     using V = typename std::conditional<EXACT, long long, int>::type;
     const int sxy = bx * by;
     unsigned long long nd = 0;
+    V acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cur_key = -100, cl[3] = {0, 0, 0};  // no particle key is -100
+    bool cur_live = false;
+    auto flush = [&]() {
+        const int li = cl[0] - bx0, lj = cl[1] - by0, lk = cl[2] - bz0;
+        // the whole stencil inside the LDS box (hence inside the grid): the common case
+        if (use_lds && li >= 0 && lj >= 0 && lk >= 0 && li < bx - 1 && lj < by - 1 && lk < bz - 1) {
+            long long *b = &box[(lk * by + lj) * bx + li];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int off = (c & 1) + ((c >> 1) & 1) * bx + ((c >> 2) & 1) * sxy;
+                atomicAdd((unsigned long long *)(b + off), (unsigned long long)(long long)acc[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int i = cl[0] + (c & 1), j = cl[1] + ((c >> 1) & 1), kk = cl[2] + ((c >> 2) & 1);
+                if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
+                    if (acc[c] == 0) continue;
+                    const int ci = i - bx0, cj = j - by0, ck = kk - bz0;
+                    const unsigned long long add = (unsigned long long)(long long)acc[c];
+                    if (use_lds && ci >= 0 && cj >= 0 && ck >= 0 && ci < bx && cj < by && ck < bz)
+                        atomicAdd((unsigned long long *)&box[(ck * by + cj) * bx + ci], add);
+                    else
+                        atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], add);
+                }
+            }
+        }
+    };
     for (int k = 0; k < kPPT; ++k) {
-        const int64_t p = base + k * kThreads + threadIdx.x;
+        const int64_t p = p0 + k;
         V v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int key = -1 - (int)threadIdx.x;  // unique: never merges
+        int key = -1 - k;  // unique within the thread: never merges
         int l[3] = {0, 0, 0};
         bool valid = false;
         if (p < N) {
@@ -495,33 +542,20 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
                 nd += 8;
             }
         }
-        const bool tail = wave_run_sums<V>(key, v);
-        const int li = l[0] - bx0, lj = l[1] - by0, lk = l[2] - bz0;
-        // the whole stencil inside the LDS box (hence inside the grid): the common case
-        const bool inbox = use_lds && li >= 0 && lj >= 0 && lk >= 0 && li < bx - 1 && lj < by - 1 && lk < bz - 1;
-        if (tail && valid && inbox) {
-            long long *b = &box[(lk * by + lj) * bx + li];
+        if (key == cur_key) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int off = (c & 1) + ((c >> 1) & 1) * bx + ((c >> 2) & 1) * sxy;
-                atomicAdd((unsigned long long *)(b + off), (unsigned long long)(long long)v[c]);
-            }
-        } else if (tail && valid) {
+            for (int c = 0; c < 8; ++c) acc[c] += v[c];
+        } else {
+            if (cur_live) flush();
+            cur_key = key;
+            cur_live = key >= 0;  // valid and near the grid
+            cl[0] = l[0];
+            cl[1] = l[1];
+            cl[2] = l[2];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), kk = l[2] + ((c >> 2) & 1);
-                if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
-                    if (v[c] == 0) continue;
-                    const int ci = i - bx0, cj = j - by0, ck = kk - bz0;
-                    const unsigned long long add = (unsigned long long)(long long)v[c];
-                    if (use_lds && ci >= 0 && cj >= 0 && ck >= 0 && ci < bx && cj < by && ck < bz)
-                        atomicAdd((unsigned long long *)&box[(ck * by + cj) * bx + ci], add);
-                    else
-                        atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], add);
-                }
-            }
+            for (int c = 0; c < 8; ++c) acc[c] = v[c];
         }
-        // out-of-grid corners of valid particles are dropped, counted per particle lane
+        // out-of-grid corners of valid particles are dropped, counted per particle
         if (valid && !(l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 - 1 && l[1] < n1 - 1 && l[2] < n2 - 1)) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
@@ -530,6 +564,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
             }
         }
     }
+    if (cur_live) flush();
     if (use_lds) {
         __syncthreads();
         const float rbx = 1.0f / (float)bx, rsxy = 1.0f / (float)sxy;
@@ -867,19 +902,34 @@ void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y
     if (N > 0) k_cell_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, sc, rotate ? 1 : 0, keys, vals);
 }
 
-__global__ void k_permute(int64_t N, const uint32_t *perm, const double *src, double *dst) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[perm[i]];
+// dst[i] = src[perm[i]]: each thread gathers kPermU elements with all of its
+// loads in flight before the first store.
+constexpr int kPermU = 8;
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute(int64_t N, const uint32_t *__restrict__ perm,
+                                                 const T *__restrict__ src, T *__restrict__ dst) {
+    const int64_t base = (int64_t)blockIdx.x * (kThreads * kPermU) + threadIdx.x;
+    uint32_t pi[kPermU];
+#pragma unroll
+    for (int u = 0; u < kPermU; ++u) {
+        const int64_t i = base + u * kThreads;
+        pi[u] = perm[i < N ? i : N - 1];
+    }
+    T v[kPermU];
+#pragma unroll
+    for (int u = 0; u < kPermU; ++u) v[u] = src[pi[u]];
+#pragma unroll
+    for (int u = 0; u < kPermU; ++u) {
+        const int64_t i = base + u * kThreads;
+        if (i < N) dst[i] = v[u];
+    }
 }
-__global__ void k_permute_u32(int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[perm[i]];
-}
+static unsigned permute_blocks(int64_t N) { return (unsigned)((N + kThreads * kPermU - 1) / (kThreads * kPermU)); }
 void launch_permute(hipStream_t s, int64_t N, const uint32_t *perm, const double *src, double *dst) {
-    if (N > 0) k_permute<<<stream_blocks(N), kThreads, 0, s>>>(N, perm, src, dst);
+    if (N > 0) k_permute<double><<<permute_blocks(N), kThreads, 0, s>>>(N, perm, src, dst);
 }
 void launch_permute_u32(hipStream_t s, int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst) {
-    if (N > 0) k_permute_u32<<<stream_blocks(N), kThreads, 0, s>>>(N, perm, src, dst);
+    if (N > 0) k_permute<uint32_t><<<permute_blocks(N), kThreads, 0, s>>>(N, perm, src, dst);
 }
 __global__ void k_iota(int64_t N, uint32_t *v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
@@ -1031,8 +1081,9 @@ void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, uns
 // Then the rotation into the beam frame (not stored: the deposit and the gather
 // pass repeat it bit for bit) and its bounds (boundp's get_bounds).  Each block
 // walks contiguous chunks of kChunk particles (the deposit's chunks), writes
-// every chunk's beam-frame bounding box to chunkbox (so the deposit reads
-// positions once), and leaves its running bounds in partials.
+// every chunk's beam-frame position statistics (bounds, sums, sums of squares)
+// to chunkbox (so the deposit reads positions once), and leaves its running
+// bounds in partials.
 template <bool EXACT, bool PUSH>
 __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restrict__ x,
                                                      double *__restrict__ y, double *__restrict__ z,
@@ -1041,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restri
                                                      const double *__restrict__ pz, const Scalars *sc,
                                                      const TrackState *ts, double *partials,
                                                      double *chunkbox) {
-    __shared__ double lds[4 * 6];
+    __shared__ double lds[4 * kBoxStats];
     const double dT = ts->dT;
     const double half_cdt = 0.5 * (kC * dT);
     const bool sc_on = ts->space_charge != 0;
@@ -1052,8 +1103,12 @@ __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restri
     double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     const int64_t nchunks = (N + kChunk - 1) / kChunk;
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
-        double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+        double st[kBoxStats];
+        for (int d = 0; d < 3; ++d) {
+            st[d] = __builtin_inf();
+            st[3 + d] = -__builtin_inf();
+            st[6 + d] = st[9 + d] = 0.0;
+        }
         const int64_t base = c * kChunk + threadIdx.x;
 #pragma unroll 2
         for (int k = 0; k < kPPT; ++k) {
@@ -1072,18 +1127,20 @@ __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restri
                 else transform_to_fast(M, o, r);
             }
             for (int d = 0; d < 3; ++d) {
-                mn[d] = fmin(mn[d], r[d]);
-                mx[d] = fmax(mx[d], r[d]);
+                st[d] = fmin(st[d], r[d]);
+                st[3 + d] = fmax(st[3 + d], r[d]);
+                st[6 + d] += r[d];
+                st[9 + d] += r[d] * r[d];
             }
         }
-        block_minmax(mn, mx, lds);
-        if (threadIdx.x == 0)
+        block_chunk_stats(st, lds);
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < kBoxStats; ++k) chunkbox[c * kBoxStats + k] = st[k];
             for (int d = 0; d < 3; ++d) {
-                chunkbox[c * 6 + d] = mn[d];
-                chunkbox[c * 6 + 3 + d] = mx[d];
-                bmn[d] = fmin(bmn[d], mn[d]);
-                bmx[d] = fmax(bmx[d], mx[d]);
+                bmn[d] = fmin(bmn[d], st[d]);
+                bmx[d] = fmax(bmx[d], st[3 + d]);
             }
+        }
     }
     if (threadIdx.x == 0)
         for (int d = 0; d < 3; ++d) {
