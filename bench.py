@@ -95,6 +95,31 @@ def algorithmic_bytes(cfg, wbytes):
     }
 
 
+# profiler kernel class -> kernel symbol of the mixed-precision step
+KERNEL_SYMBOL = {
+    "gather_kick_push": "k_track_gamma<float,true>",
+    "deposit": "k_deposit_lds<1,false,true>",
+    "rotate_bounds": "k_track_alpha<false,false>",
+    "push_rotate_bounds": "k_track_alpha<false,true>",
+    "fft_z_conv": "k_strided<float,16,2>",
+    "grad": "k_grad<float>",
+}
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated) of
+    `kernel`, from the committed rocprofv3 summary of this workload; None if absent."""
+    path = os.path.join(ROOT, "profiles", "%s_mixed_hbm_traffic.txt" % config)
+    sym = KERNEL_SYMBOL.get(kernel)
+    if sym is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == "traffic" and f[1] == sym:
+            return float(f[2]) + float(f[3])
+    return None
+
+
 def cpu_baseline(budget_s=20.0):
     """The oracle (single-threaded C restatement of the OPAL 2.0 step) on a
     bounded sample: 1e6 particles of the benchmark distribution on a 64^3
@@ -177,7 +202,8 @@ def main():
         dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
         a = kernels[dom]["algorithmic_GBs"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": a, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": a / HBM_PEAK_GBS, "traffic": None,
+                    "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(args.config, dom),
                     "bytes_per_launch": ab[dom], "avg_ms": kernels[dom]["avg_ms"]}
         log("[bench] kernels: " + json.dumps(kernels))
     # achievable bandwidth of a plain copy kernel (also the byte-counter calibration)

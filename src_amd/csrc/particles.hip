@@ -17,6 +17,7 @@ using crm::dd;
 using crm::dd_add;
 
 constexpr int kSlots = 64;           // doubles per block in the partials buffer
+constexpr int kXcds = 8;             // XCDs of an MI355X; workgroups are dealt to them round-robin
 constexpr int kReduceThreads = 512;   // single-block final reductions of the partials
 
 int red_blocks(int64_t N) {
@@ -1231,9 +1232,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double half_cdt = 0.5 * (kC * dT);
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
-    // grid-stride: the whole grid streams one window of consecutive particles
-    // at a time (contiguous per-block ranges measured slower)
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    // XCD-aware grid-stride: workgroups are dispatched to the XCDs round-robin
+    // (block b runs on XCD b % kXcds), so the blocks of one XCD stream one
+    // contiguous eighth of the bunch together.  A sorted bunch then gathers
+    // each field-grid line into one XCD's L2 instead of into several.
+    const int xcds = (gridDim.x % kXcds == 0) ? kXcds : 1;
+    const int64_t xper = ((N + xcds - 1) / xcds + kThreads - 1) / kThreads * kThreads;
+    const int64_t xbeg = (int64_t)(blockIdx.x % xcds) * xper;
+    const int64_t xend = xbeg + xper < N ? xbeg + xper : N;
+    const int64_t xstride = (int64_t)(gridDim.x / xcds) * kThreads;
+    for (int64_t i = xbeg + (int64_t)(blockIdx.x / xcds) * kThreads + threadIdx.x; i < xend; i += xstride) {
         double r[3] = {x[i], y[i], z[i]};
         double p[3] = {px[i], py[i], pz[i]};
         double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
