@@ -443,6 +443,103 @@ void orc_compute_potential(const int n[3], double *rho_inout, const double hr[3]
 }
 
 /* ------------------------------------------------------------------------ */
+/* Image-charge (shifted) solve                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Corner antiderivative of FFTPoissonSolver::shiftedIntGreensFunction,
+ * src/Solvers/FFTPoissonSolver.cpp:552-603: the IGF corner term with the z
+ * coordinate (k*h2 - h2/2) + zshift, and for the second table additionally
+ * - n2*h2 (left-to-right evaluation as in the reference). */
+static double igf_corner_shifted(int i, int j, int k, const double hr[3], double cellVolume,
+                                 double zshift, int second, int n2) {
+    double v0 = i * hr[0] - hr[0] / 2;
+    double v1 = j * hr[1] - hr[1] / 2;
+    double v2 = k * hr[2] - hr[2] / 2 + zshift;
+    if (second) v2 = v2 - n2 * hr[2];
+    double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    double t = -v2 * v2 * igf_atan(v0 * v1 / (v2 * r)) / 2;
+    t += -v1 * v1 * igf_atan(v0 * v2 / (v1 * r)) / 2;
+    t += -v0 * v0 * igf_atan(v1 * v2 / (v0 * r)) / 2;
+    t += v1 * v2 * igf_log(v0 + r);
+    t += v0 * v2 * igf_log(v1 + r);
+    t += v0 * v1 * igf_log(v2 + r);
+    return t / cellVolume;
+}
+
+/* The shifted Green's function on the doubled grid (2n0,2n1,2n2), x-fastest:
+ * two corner tables (FFTPoissonSolver.cpp:552-603), their 8-corner differences
+ * on [0,n]^3 (FFTPoissonSolver.cpp:621-639, no [0,0,0] quirk), the first for
+ * z in [0,n2], the second shifted to z in [n2+1, 2n2) (no z mirror), both
+ * mirrored evenly in x and y (mirrorRhoField(Field_t&), FFTPoissonSolver.cpp:666-691). */
+void orc_shifted_green_doubled(const int n[3], const double hr[3], double zshift, double *G2) {
+    const int t0 = n[0] + 2, t1 = n[1] + 2, t2 = n[2] + 2;
+    const int g0 = n[0] + 1, g1 = n[1] + 1, g2 = n[2] + 1;
+    const double cellVolume = hr[0] * hr[1] * hr[2];
+    const size_t nt = (size_t)t0 * t1 * t2, ng = (size_t)g0 * g1 * g2;
+    double *T = (double *)malloc(sizeof(double) * nt);
+    double *Ga = (double *)malloc(sizeof(double) * ng);
+    double *Gb = (double *)malloc(sizeof(double) * ng);
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k = 0; k < t2; ++k)
+            for (int j = 0; j < t1; ++j)
+                for (int i = 0; i < t0; ++i)
+                    T[((size_t)k * t1 + j) * t0 + i] = igf_corner_shifted(i, j, k, hr, cellVolume, zshift, pass, n[2]);
+        double *G = pass ? Gb : Ga;
+#define TT(a, b, c) T[((size_t)(c) * t1 + (b)) * t0 + (a)]
+        for (int k = 0; k < g2; ++k)
+            for (int j = 0; j < g1; ++j)
+                for (int i = 0; i < g0; ++i) {
+                    double g = TT(i + 1, j + 1, k + 1);
+                    g += TT(i + 1, j, k);
+                    g += TT(i, j + 1, k);
+                    g += TT(i, j, k + 1);
+                    g -= TT(i + 1, j + 1, k);
+                    g -= TT(i + 1, j, k + 1);
+                    g -= TT(i, j + 1, k + 1);
+                    g -= TT(i, j, k);
+                    G[((size_t)k * g1 + j) * g0 + i] = g;
+                }
+#undef TT
+    }
+    const int m0 = 2 * n[0], m1 = 2 * n[1], m2 = 2 * n[2];
+    for (int z = 0; z < m2; ++z) {
+        const double *G = z <= n[2] ? Ga : Gb;
+        const int fz = z <= n[2] ? z : z - n[2];
+        for (int y = 0; y < m1; ++y) {
+            const int fy = y <= n[1] ? y : m1 - y;
+            for (int x = 0; x < m0; ++x) {
+                const int fx = x <= n[0] ? x : m0 - x;
+                G2[((size_t)z * m1 + y) * m0 + x] = G[((size_t)fz * g1 + fy) * g0 + fx];
+            }
+        }
+    }
+    free(T);
+    free(Ga);
+    free(Gb);
+}
+
+/* FFTPoissonSolver::computePotential(rho, hr, zshift), FFTPoissonSolver.cpp:289-330:
+ * circular convolution with the shifted Green's function, negated (image
+ * charge), and the result flipped in z: phi[i][j][k] = conv[i][j][n2-1-k].
+ * In place: rho in, image potential out. */
+void orc_compute_potential_shifted(const int n[3], double *rho_inout, const double hr[3], double zshift) {
+    const size_t M = (size_t)8 * n[0] * n[1] * n[2];
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    double *G2 = (double *)malloc(M * sizeof(double));
+    orc_shifted_green_doubled(n, hr, zshift, G2);
+    for (size_t q = 0; q < M; ++q) G2[q] = -G2[q];  /* -(rho^ * G^) == rho^ * (-G)^ exactly */
+    double *conv = (double *)malloc(ncell * sizeof(double));
+    memcpy(conv, rho_inout, ncell * sizeof(double));
+    orc_convolve_doubled(n, conv, G2);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                rho_inout[((size_t)k * n[1] + j) * n[0] + i] = conv[((size_t)(n[2] - k - 1) * n[1] + j) * n[0] + i];
+    free(G2);
+    free(conv);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Gradient                                                                 */
 /* ------------------------------------------------------------------------ */
 
@@ -546,6 +643,135 @@ int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, dou
     }
     free(rho);
     free(E);
+    return dropped;
+}
+
+/* PartBunchBase::calcGammas, src/Classic/Algorithms/PartBunchBase.hpp:639-682:
+ * mean of sqrt(1 + P.P) over each bin's particles, summed sequentially in
+ * particle order; an empty bin gets 1.  Bins outside [0, nbins) are ignored. */
+void orc_calc_gammas(int64_t N, const double *P, const int *bin, int nbins, double *gamma) {
+    int64_t *cnt = (int64_t *)calloc((size_t)nbins, sizeof(int64_t));
+    for (int b = 0; b < nbins; ++b) gamma[b] = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        const int b = bin[i];
+        if (b < 0 || b >= nbins) continue;
+        gamma[b] += sqrt(1.0 + (P[3 * i] * P[3 * i] + P[3 * i + 1] * P[3 * i + 1] + P[3 * i + 2] * P[3 * i + 2]));
+        ++cnt[b];
+    }
+    for (int b = 0; b < nbins; ++b) gamma[b] = cnt[b] != 0 ? gamma[b] / (double)cnt[b] : 1.0;
+    free(cnt);
+}
+
+/* Field of one bin (and of its image charge) on the grid, AoS E, after the
+ * Lorentz scaling: the body of PartBunch::computeSelfFields(int),
+ * src/Classic/Algorithms/PartBunch.cpp:138-430, up to each gather.
+ * rho: the bin's charge density exactly as scattered ((qi*dt) deposits, then
+ * / dT); E_self and E_img receive -Grad(phi)*(g, g, 1/g). */
+static void binned_fields(const int n[3], const double *rho_scattered, const double origin[3],
+                          const double hr[3], double gammaz, double coupling, int greens, int image,
+                          double *E_self, double *E_img) {
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    const double scaleFactor = 1.0;
+    double *rho = (double *)malloc(ncell * sizeof(double));
+    double *img = (double *)malloc(ncell * sizeof(double));
+    double *E = (double *)malloc(3 * ncell * sizeof(double));
+    memcpy(rho, rho_scattered, ncell * sizeof(double));
+    /* PartBunch.cpp:181-182 */
+    double tmp2 = 1 / hr[0] * 1 / hr[1] * 1 / hr[2] / (scaleFactor * scaleFactor * scaleFactor) / gammaz;
+    for (size_t c = 0; c < ncell; ++c) rho[c] *= tmp2;
+    double hs[3] = {hr[0] * scaleFactor, hr[1] * scaleFactor, hr[2] * scaleFactor};
+    hs[2] *= gammaz;
+    memcpy(img, rho, ncell * sizeof(double));
+    orc_compute_potential(n, rho, hs, greens);
+    const double vol = hs[0] * hs[1] * hs[2];
+    const double sc[3] = {gammaz / scaleFactor, gammaz / scaleFactor, 1.0 / (scaleFactor * gammaz)};
+    for (size_t c = 0; c < ncell; ++c) rho[c] *= vol;
+    for (size_t c = 0; c < ncell; ++c) rho[c] *= coupling;
+    orc_grad(n, rho, hr, E);
+    for (size_t c = 0; c < ncell; ++c)
+        for (int d = 0; d < 3; ++d) E_self[3 * c + d] = -E[3 * c + d] * sc[d];
+    if (image) {
+        /* PartBunch.cpp:261-264: zshift from the mesh origin and the z extent */
+        const double hz = hr[2];
+        const double zshift = -(2 * origin[2] + (0 + (n[2] - 1) + 1) * hz) * gammaz * scaleFactor;
+        orc_compute_potential_shifted(n, img, hs, zshift);
+        for (size_t c = 0; c < ncell; ++c) img[c] *= vol;
+        for (size_t c = 0; c < ncell; ++c) img[c] *= coupling;
+        orc_grad(n, img, hr, E);
+        for (size_t c = 0; c < ncell; ++c)
+            for (int d = 0; d < 3; ++d) E_img[3 * c + d] = -E[3 * c + d] * sc[d];
+    }
+    free(rho);
+    free(img);
+    free(E);
+}
+
+/* The energy-binned self fields: the bin loop of
+ * ParallelTTracker::computeSpaceChargeFields (src/Algorithms/ParallelTTracker.cpp:567-579)
+ * over PartBunch::computeSelfFields(int) (PartBunch.cpp:138-430).  Bin b is
+ * given charge qi per particle (setBinCharge, PartBunchBase.hpp:716-724), its
+ * field and, if `image`, its image charge's field are gathered at every
+ * particle and added to Ef; Bf gets -/+ betaC*E_y, +/- betaC*E_x for the bunch
+ * and the image (opposite signs).  Ef is overwritten (the tracker zeroes it
+ * first), Bf accumulated.  gamma_in: per-bin Lorentz factors, or NULL to use
+ * calcGammas.  Returns the number of dropped deposits. */
+int64_t orc_compute_self_fields_binned(int64_t N, const double *R, const double *P, const int *bin,
+                                       int nbins, double qi, const double *dt, double dT,
+                                       const double origin[3], const double hr[3], const int n[3],
+                                       double coupling, int greens, int image,
+                                       const double *gamma_in, double *Ef, double *Bf) {
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    double *gam = (double *)malloc(sizeof(double) * (nbins > 0 ? nbins : 1));
+    if (gamma_in) memcpy(gam, gamma_in, sizeof(double) * nbins);
+    else orc_calc_gammas(N, P, bin, nbins, gam);
+    double *Qb = (double *)malloc(sizeof(double) * (N > 0 ? N : 1));
+    double *rho = (double *)malloc(ncell * sizeof(double));
+    double *Es = (double *)calloc(3 * ncell, sizeof(double));
+    double *Ei = (double *)calloc(3 * ncell, sizeof(double));
+    double *tmp = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+    int64_t dropped = 0;
+    memset(Ef, 0, sizeof(double) * 3 * N);
+    double *Rb = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+    for (int b = 0; b < nbins; ++b) {
+        /* the reference scatters every particle with Q = qi*dt in the bin and 0
+         * elsewhere; adding the zeros changes no value, so only the bin's
+         * particles are scattered (in particle order), and only their
+         * out-of-grid deposits are counted */
+        int64_t nb = 0;
+        for (int64_t i = 0; i < N; ++i)
+            if (bin[i] == b) {
+                Qb[nb] = qi;
+                Qb[nb] *= dt[i];
+                memcpy(Rb + 3 * nb, R + 3 * i, 3 * sizeof(double));
+                ++nb;
+            }
+        memset(rho, 0, ncell * sizeof(double));
+        dropped += orc_scatter(nb, Rb, Qb, origin, hr, n, rho);
+        for (size_t c = 0; c < ncell; ++c) rho[c] /= dT;
+        binned_fields(n, rho, origin, hr, gam[b], coupling, greens, image, Es, Ei);
+        const double betaC = sqrt(gam[b] * gam[b] - 1.0) / gam[b] / ORC_C;
+        orc_gather(N, R, Es, origin, hr, n, tmp);
+        for (int64_t i = 0; i < N; ++i) {
+            Bf[3 * i + 0] = Bf[3 * i + 0] - betaC * tmp[3 * i + 1];
+            Bf[3 * i + 1] = Bf[3 * i + 1] + betaC * tmp[3 * i + 0];
+        }
+        for (int64_t i = 0; i < 3 * N; ++i) Ef[i] += tmp[i];
+        if (image) {
+            orc_gather(N, R, Ei, origin, hr, n, tmp);
+            for (int64_t i = 0; i < N; ++i) {
+                Bf[3 * i + 0] = Bf[3 * i + 0] + betaC * tmp[3 * i + 1];
+                Bf[3 * i + 1] = Bf[3 * i + 1] - betaC * tmp[3 * i + 0];
+            }
+            for (int64_t i = 0; i < 3 * N; ++i) Ef[i] += tmp[i];
+        }
+    }
+    free(gam);
+    free(Qb);
+    free(Rb);
+    free(rho);
+    free(Es);
+    free(Ei);
+    free(tmp);
     return dropped;
 }
 
@@ -888,9 +1114,27 @@ static void update_ref(orc_tracker *tr, orc_trafo *upd) {
  * (src/Algorithms/ParallelTTracker.cpp:546-592) for a bunch without energy bins:
  * moments -> quaternion -> rotate R into the beam frame -> boundp ->
  * computeSelfFields -> rotate R, Ef, Bf back.  mesh_out (hr, origin) optional. */
+typedef struct {
+    const int *bin; /* per-particle energy bin, NULL: no bins */
+    int nbins;
+    double qi;      /* macro charge given to the particles of the bin being solved */
+    int image;
+} orc_bins;
+
+static int64_t space_charge_impl(orc_tracker *tr, int64_t N, double *R, const double *P, double *Q,
+                                 const double *dt, double *Ef, double *Bf, double mesh_out[6],
+                                 double *rho_out, double *phi_out, double *E_out, const orc_bins *bins);
+
 int64_t orc_space_charge(orc_tracker *tr, int64_t N, double *R, const double *P, double *Q,
                          const double *dt, double *Ef, double *Bf, double mesh_out[6],
                          double *rho_out, double *phi_out, double *E_out) {
+    return space_charge_impl(tr, N, R, P, Q, dt, Ef, Bf, mesh_out, rho_out, phi_out, E_out, NULL);
+}
+
+/* With energy bins: calcGammas, then the bin loop (ParallelTTracker.cpp:567-579). */
+static int64_t space_charge_impl(orc_tracker *tr, int64_t N, double *R, const double *P, double *Q,
+                                 const double *dt, double *Ef, double *Bf, double mesh_out[6],
+                                 double *rho_out, double *phi_out, double *E_out, const orc_bins *bins) {
     double m[27];
     orc_calc_moments(N, R, P, m);
     double pmean[3] = {m[1] / (double)N, m[3] / (double)N, m[5] / (double)N};
@@ -911,8 +1155,14 @@ int64_t orc_space_charge(orc_tracker *tr, int64_t N, double *R, const double *P,
         memcpy(mesh_out, hr, sizeof(hr));
         memcpy(mesh_out + 3, origin, sizeof(origin));
     }
-    int64_t dropped = orc_compute_self_fields(N, R, P, Q, dt, tr->dT, origin, hr, tr->n, tr->coupling,
-                                              tr->greens, Ef, Bf, rho_out, phi_out, E_out);
+    int64_t dropped;
+    if (bins && bins->bin)
+        dropped = orc_compute_self_fields_binned(N, R, P, bins->bin, bins->nbins, bins->qi, dt, tr->dT,
+                                                 origin, hr, tr->n, tr->coupling, tr->greens, bins->image,
+                                                 NULL, Ef, Bf);
+    else
+        dropped = orc_compute_self_fields(N, R, P, Q, dt, tr->dT, origin, hr, tr->n, tr->coupling,
+                                          tr->greens, Ef, Bf, rho_out, phi_out, E_out);
     orc_transform_to(&b2r, N, R);
     orc_rotate_to(&b2r, N, Ef);
     orc_rotate_to(&b2r, N, Bf);
@@ -922,8 +1172,23 @@ int64_t orc_space_charge(orc_tracker *tr, int64_t N, double *R, const double *P,
 /* ParallelTTracker::execute loop body, src/Algorithms/ParallelTTracker.cpp:343-388,
  * for nsteps steps with per-step statistics (DataSink stat quantities after
  * each step).  Returns 0 or a negative error. */
+static int track_impl(orc_tracker *tr, int64_t N, double *R, double *P, double *Q, double *dt, int nsteps,
+                      orc_stats *stats, const orc_bins *bins);
+
 int orc_track(orc_tracker *tr, int64_t N, double *R, double *P, double *Q, double *dt, int nsteps,
               orc_stats *stats) {
+    return track_impl(tr, N, R, P, Q, dt, nsteps, stats, NULL);
+}
+
+/* The same loop with energy bins (space charge by bin + image charge). */
+int orc_track_binned(orc_tracker *tr, int64_t N, double *R, double *P, double *Q, double *dt,
+                     const int *bin, int nbins, double qi, int image, int nsteps, orc_stats *stats) {
+    orc_bins b = {bin, nbins, qi, image};
+    return track_impl(tr, N, R, P, Q, dt, nsteps, stats, &b);
+}
+
+static int track_impl(orc_tracker *tr, int64_t N, double *R, double *P, double *Q, double *dt, int nsteps,
+                      orc_stats *stats, const orc_bins *bins) {
     double *Ef = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
     double *Bf = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
     for (int s = 0; s < nsteps; ++s) {
@@ -931,7 +1196,7 @@ int orc_track(orc_tracker *tr, int64_t N, double *R, double *P, double *Q, doubl
         memset(Ef, 0, sizeof(double) * 3 * N);
         memset(Bf, 0, sizeof(double) * 3 * N);
         if (tr->space_charge && N > 0) {
-            if (orc_space_charge(tr, N, R, P, Q, dt, Ef, Bf, NULL, NULL, NULL, NULL) < 0) {
+            if (space_charge_impl(tr, N, R, P, Q, dt, Ef, Bf, NULL, NULL, NULL, NULL, bins) < 0) {
                 free(Ef);
                 free(Bf);
                 return -1;

@@ -82,6 +82,16 @@ def lib():
     L.orc_space_charge.argtypes = [C.POINTER(Tracker), i64, PD, PD, PD, PD, PD, PD, PD,
                                    C.c_void_p, C.c_void_p, C.c_void_p]
     L.orc_space_charge.restype = i64
+    L.orc_shifted_green_doubled.argtypes = [PI32, PD, C.c_double, PD]
+    L.orc_compute_potential_shifted.argtypes = [PI32, PD, PD, C.c_double]
+    L.orc_calc_gammas.argtypes = [i64, PD, PI32, C.c_int, PD]
+    L.orc_compute_self_fields_binned.argtypes = [i64, PD, PD, PI32, C.c_int, C.c_double, PD, C.c_double,
+                                                 PD, PD, PI32, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                                 PD, PD]
+    L.orc_compute_self_fields_binned.restype = i64
+    L.orc_track_binned.argtypes = [C.POINTER(Tracker), i64, PD, PD, PD, PD, PI32, C.c_int, C.c_double,
+                                   C.c_int, C.c_int, C.POINTER(Stats)]
+    L.orc_track_binned.restype = C.c_int
     assert L.orc_sizeof_stats() == C.sizeof(Stats)
     assert L.orc_sizeof_tracker() == C.sizeof(Tracker)
     _lib = L
@@ -178,6 +188,38 @@ def compute_potential(n, rho, hr, greens=0):
     return phi
 
 
+def shifted_green_doubled(n, hr, zshift):
+    n = _n(n)
+    G2 = np.zeros(8 * int(np.prod(n)))
+    lib().orc_shifted_green_doubled(n, _d(hr), float(zshift), G2)
+    return G2.reshape(2 * int(n[2]), 2 * int(n[1]), 2 * int(n[0]))
+
+
+def compute_potential_shifted(n, rho, hr, zshift):
+    phi = _d(rho).ravel().copy()
+    lib().orc_compute_potential_shifted(_n(n), phi, _d(hr), float(zshift))
+    return phi
+
+
+def calc_gammas(P, bins, nbins):
+    g = np.zeros(nbins)
+    lib().orc_calc_gammas(len(P), _d(P), np.ascontiguousarray(bins, dtype=np.int32), nbins, g)
+    return g
+
+
+def compute_self_fields_binned(R, P, bins, nbins, qi, dt, dT, origin, hr, n, coupling=COUPLING,
+                               greens=0, image=True, gammas=None, Bf=None):
+    R, P = _d(R), _d(P)
+    N = len(R)
+    Ef = np.zeros((N, 3))
+    Bf = np.zeros((N, 3)) if Bf is None else _d(Bf).copy()
+    g = None if gammas is None else _d(gammas)
+    dropped = lib().orc_compute_self_fields_binned(
+        N, R, P, np.ascontiguousarray(bins, dtype=np.int32), nbins, qi, _d(dt), dT, _d(origin), _d(hr),
+        _n(n), coupling, greens, int(image), g.ctypes.data if g is not None else None, Ef, Bf)
+    return dict(Ef=Ef, Bf=Bf, dropped=dropped)
+
+
 def grad(n, phi, hr):
     n = _n(n)
     r = np.zeros(int(np.prod(n)) * 3)
@@ -262,4 +304,15 @@ def track(tracker, R, P, Q, dt, nsteps):
     rc = lib().orc_track(C.byref(tracker), len(R), R, P, Q, dt, nsteps, stats)
     if rc != 0:
         raise RuntimeError("orc_track failed: %d" % rc)
+    return R, P, Q, dt, [stats_dict(stats[i]) for i in range(nsteps)]
+
+
+def track_binned(tracker, R, P, Q, dt, bins, nbins, qi, nsteps, image=True):
+    R, P, Q, dt = (_d(a).copy() for a in (R, P, Q, dt))
+    stats = (Stats * max(nsteps, 1))()
+    rc = lib().orc_track_binned(C.byref(tracker), len(R), R, P, Q, dt,
+                                np.ascontiguousarray(bins, dtype=np.int32), nbins, qi, int(image),
+                                nsteps, stats)
+    if rc != 0:
+        raise RuntimeError("orc_track_binned failed: %d" % rc)
     return R, P, Q, dt, [stats_dict(stats[i]) for i in range(nsteps)]

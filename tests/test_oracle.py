@@ -366,3 +366,124 @@ def test_track_single_particle_gyration_in_solenoid():
     assert abs(np.hypot(*P2[0, :2]) - 0.01) < 1e-15
     got = math.atan2(P2[0, 1], P2[0, 0])
     assert abs((got - (-ang) + math.pi) % (2 * math.pi) - math.pi) < 1e-12
+
+
+# ---------------------------------------------------------------- image charge / energy bins
+def _pad_z(rho, n, m, reverse=False):
+    """rho (x-fastest n0*n1*n2) zero-padded to n2+m planes, optionally z-reversed."""
+    r = rho.reshape(n[2], n[1], n[0])
+    if reverse:
+        r = r[::-1]
+    out = np.zeros((n[2] + m, n[1], n[0]))
+    out[: n[2]] = r
+    return out.ravel()
+
+
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_shifted_solve_equals_standard_solve_on_extended_grid(sign):
+    """With zshift = +-m*h2 and m >= n2 no charge sits at zero distance from a
+    field point, so the shifted (non-mirrored in z), negated, z-flipped
+    convolution must equal the ordinary Hockney solve of the zero-padded
+    (for a negative shift: z-reversed) charge on a grid taller by m planes,
+    read m planes up.  Both solves share only the IGF corner formula; the
+    doubled-grid layout, the z wrap of the second table, the sign and the
+    flip are all different code."""
+    n = (6, 5, 4)
+    m = 5
+    rng = np.random.default_rng(11)
+    rho = rng.normal(size=int(np.prod(n)))
+    hr = np.array([1.3e-3, 0.8e-3, 1.1e-3])
+    img = O.compute_potential_shifted(n, rho, hr, sign * m * hr[2]).reshape(n[2], n[1], n[0])
+    ext = (n[0], n[1], n[2] + m)
+    phi = O.compute_potential(ext, _pad_z(rho, n, m, reverse=(sign < 0)), hr, 0).reshape(n[2] + m, n[1], n[0])
+    if sign > 0:
+        # img(k) = -conv(n2-1-k), conv(j) = phi_ext(j+m)
+        expect = -phi[m:][::-1]
+    else:
+        # img(k) = -phi_ext[reversed rho](k+m)
+        expect = -phi[m:]
+    assert rel_l2(img, expect) < 1e-10
+
+
+def test_shifted_solve_far_field_is_minus_total_charge_over_distance():
+    n = (8, 8, 8)
+    hr = np.array([1e-3, 1e-3, 1e-3])
+    rho = np.random.default_rng(12).random(512)
+    zs = -2.0  # 250x the bunch size
+    img = O.compute_potential_shifted(n, rho, hr, zs)
+    # every pair distance is |zs| up to O(size/zs): phi_img ~ -sum(rho)/|zs|
+    assert np.allclose(img, -rho.sum() / abs(zs), rtol=2e-2)
+    assert abs(img.mean() / (-rho.sum() / abs(zs)) - 1) < 1e-2
+
+
+def test_shifted_green_layout():
+    n = (4, 4, 4)
+    hr = np.array([1e-3, 1e-3, 1e-3])
+    G = O.shifted_green_doubled(n, hr, -0.01)
+    # even in x and y on the doubled grid, not in z
+    for x in range(1, 8):
+        assert np.array_equal(G[:, :, x], G[:, :, 8 - x])
+    for y in range(1, 8):
+        assert np.array_equal(G[:, y, :], G[:, 8 - y, :])
+    # z: the kernel at offsets d*h - 10 mm for d = z (z <= 4) and d = z - 8 (z > 4)
+    d = np.array([z if z <= 4 else z - 8 for z in range(8)])
+    dist = np.abs(d * 1e-3 - 0.01)
+    order = np.argsort(dist)
+    assert np.all(np.diff(G[order, 0, 0]) < 0)  # closer offsets -> larger kernel
+    assert np.allclose(G[:, 0, 0], 1.0 / dist, rtol=1e-3)
+
+
+def test_calc_gammas_vs_fsum():
+    rng = np.random.default_rng(13)
+    P = rng.normal(size=(1000, 3)) * [0.1, 0.1, 1.0] + [0, 0, 2.0]
+    bins = rng.integers(0, 4, 1000).astype(np.int32)
+    bins[bins == 2] = 3  # bin 2 empty
+    g = O.calc_gammas(P, bins, 4)
+    for b in range(4):
+        sel = bins == b
+        if sel.any():
+            exact = math.fsum(np.sqrt(1.0 + (P[sel] ** 2).sum(1))) / sel.sum()
+            assert abs(g[b] / exact - 1) < 1e-14
+        else:
+            assert g[b] == 1.0
+
+
+def test_binned_single_bin_without_image_equals_plain_self_fields():
+    """One bin holding every particle, no image, the plain path's Lorentz
+    factor passed in: the binned body reduces to computeSelfFields() up to the
+    order of the 1/h products in the density scale."""
+    N, n = 5000, (16, 16, 16)
+    R, P, _ = gaussian_bunch(N, seed=14, q_total=1e-9)
+    qi = 1e-9 / N
+    Q = np.full(N, qi)
+    dt = np.full(N, 1e-11)
+    hr, org = O.boundp(R, n)
+    gz = math.sqrt((P[:, 2].sum() / N) ** 2 + 1.0)
+    plain = O.compute_self_fields(R, P, Q, dt, 1e-11, org, hr, n)
+    binned = O.compute_self_fields_binned(R, P, np.zeros(N, np.int32), 1, qi, dt, 1e-11, org, hr, n,
+                                          image=False, gammas=[gz])
+    assert rel_l2(binned["Ef"], plain["Ef"]) < 1e-13
+    assert rel_l2(binned["Bf"], plain["Bf"]) < 1e-13
+
+
+def test_binned_fields_are_linear_in_the_bins():
+    """With one Lorentz factor for every bin, the sum over bins of the bin
+    fields (and of the image fields) equals the field of one bin holding all
+    particles."""
+    N, n = 6000, (16, 16, 16)
+    R, P, _ = gaussian_bunch(N, seed=15, q_total=1e-9)
+    qi = 1e-9 / N
+    dt = np.full(N, 1e-11)
+    R = R + np.array([0.0, 0.0, 0.05])  # bunch 5 cm downstream of the cathode plane z = 0
+    hr, org = O.boundp(R, n)
+    bins = np.random.default_rng(16).integers(0, 3, N).astype(np.int32)
+    g = [1.2, 1.2, 1.2]
+    three = O.compute_self_fields_binned(R, P, bins, 3, qi, dt, 1e-11, org, hr, n, gammas=g)
+    one = O.compute_self_fields_binned(R, P, np.zeros(N, np.int32), 1, qi, dt, 1e-11, org, hr, n, gammas=g[:1])
+    assert rel_l2(three["Ef"], one["Ef"]) < 1e-13
+    assert rel_l2(three["Bf"], one["Bf"]) < 1e-13
+    # the image field is present and opposes the bunch's own longitudinal push
+    noimg = O.compute_self_fields_binned(R, P, np.zeros(N, np.int32), 1, qi, dt, 1e-11, org, hr, n,
+                                         image=False, gammas=g[:1])
+    dEz = one["Ef"][:, 2] - noimg["Ef"][:, 2]
+    assert np.all(dEz < 0) or np.all(dEz > 0)
