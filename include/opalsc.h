@@ -114,6 +114,16 @@ int opalsc_synchronize(opalsc_ctx *ctx);
  * function on the doubled grid, without the h^3 and 1/(4 pi eps0) factors. */
 int opalsc_compute_potential(opalsc_ctx *ctx, double *rho_inout, const double hr[3]);
 
+/* FFTPoissonSolver::computePotential(Field_t& rho, Vector_t hr, double zshift),
+ * src/Solvers/FFTPoissonSolver.cpp:289-330: the image-charge potential.  In
+ * place: rho in; out: minus the circular convolution with the shifted
+ * integrated Green's function (shiftedIntGreensFunction,
+ * FFTPoissonSolver.cpp:546-639: not mirrored in z), read back flipped in z
+ * (phi[i][j][k] = conv[i][j][n2-1-k]).  The integrated Green's function is
+ * used whatever the context's greens setting. */
+int opalsc_compute_potential_shifted(opalsc_ctx *ctx, double *rho_inout, const double hr[3],
+                                     double zshift);
+
 /* --- mesh and cloud-in-cell ----------------------------------------------
  * PartBunchBase::boundp (src/Classic/Algorithms/PartBunchBase.hpp:802-883):
  * bounds of R, BBOXINCR padding, hr = (max-min)/(n-1), origin = min - hr/2. */

@@ -295,6 +295,32 @@ extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double
     return OPALSC_OK;
 }
 
+extern "C" int opalsc_compute_potential_shifted(opalsc_ctx *c, double *rho, const double hr[3], double zshift) {
+    if (!c || !rho || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    if (c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
+    Scalars s;
+    memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) {
+        s.n[d] = c->cfg.n[d];
+        s.hr[d] = hr[d];
+        s.hs[d] = hr[d];
+    }
+    s.dT = 1.0;
+    s.tmp2 = 1.0;
+    s.q2e = 1.0;
+    s.zshift = zshift;
+    upload_scalars(c, s);
+    const size_t nc = c->grid.ncell;
+    HIPCHK(c, hipMemcpyAsync(c->grid.rho_f64, rho, nc * 8, hipMemcpyHostToDevice, c->stream));
+    c->grid.green_image(c->d_sc, c->grid.fp64);
+    c->grid.solve(c->d_sc, 1, 0, true);
+    CHECK_LAUNCH(c);
+    c->grid.widen(1, (double *)c->grid.rho_f64);
+    HIPCHK(c, hipMemcpyAsync(rho, c->grid.rho_f64, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // mesh, CIC, sort
 // ---------------------------------------------------------------------------

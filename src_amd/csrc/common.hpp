@@ -43,6 +43,7 @@ struct Scalars {
     double M_br[9], o_br[3];   // beam -> reference
     double M_u[9], o_u[3];     // reference-frame update (transformBunch)
     double Eext[3], Bext[3];   // external field for the bunch at t + dT/2
+    double zshift;             // image-charge Green's function shift (FFTPoissonSolver.cpp:289)
     int mesh_ok;
     int64_t N;
 };

@@ -59,15 +59,22 @@ struct PoissonGrid {
     void *Ghat = nullptr;     // T[n2+1][n1+1][gp]
     void *phi = nullptr;      // T[ncell]
     void *Eg = nullptr;       // vec4<T>[ncell]
+    // image-charge Green's function (allocated on first use): the two shifted
+    // octant tables after the x and y DCTs (T[n2+1][n1+1][gp] each) and the
+    // transform over the doubled z (complex T[2 n2][n1+1][gp])
+    void *HimgA = nullptr, *HimgB = nullptr, *Gimg = nullptr;
     hipStream_t stream = nullptr;
     Profiler *prof = nullptr;
 
     int init(const int nn[3], bool fp64, int greens, hipStream_t s, std::string &err);
     void destroy();
     void green(const Scalars *d_sc, bool correctly_rounded);
+    int ensure_image(std::string &err);
+    void green_image(const Scalars *d_sc, bool correctly_rounded);  // shift: Scalars::zshift
     // src_kind 0: int64 fixed-point rho_i64 (self-field scaling); 1: rho_f64 as is
     // outscale 1: phi *= vol*coupling (self fields); 0: potential only
-    void solve(const Scalars *d_sc, int src_kind, int outscale);
+    // image: convolve with the image-charge Green's function (negated, z flipped)
+    void solve(const Scalars *d_sc, int src_kind, int outscale, bool image = false);
     void grad(const Scalars *d_sc);
     void rho_to_double(const Scalars *d_sc, double *out);
     void widen(int which, double *out);

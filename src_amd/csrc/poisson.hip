@@ -33,13 +33,10 @@ namespace opalsc {
 // Green's function octant (FP64 in both precisions)
 // ---------------------------------------------------------------------------
 
+// The IGF antiderivative at corner (v0, v1, v2): FFTPoissonSolver.cpp:479-489
+// (and, with shifted v2, FFTPoissonSolver.cpp:563-573, 589-599).
 template <bool CR>
-__device__ inline double igf_corner(int i, int j, int k, double h0, double h1, double h2,
-                                    double cellVolume) {
-    // FFTPoissonSolver::integratedGreensFunction, FFTPoissonSolver.cpp:472-491
-    double v0 = i * h0 - h0 / 2;
-    double v1 = j * h1 - h1 / 2;
-    double v2 = k * h2 - h2 / 2;
+__device__ inline double igf_term(double v0, double v1, double v2, double cellVolume) {
     double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
     double a0, a1, a2, l0, l1, l2;
     if constexpr (CR) {
@@ -66,6 +63,16 @@ __device__ inline double igf_corner(int i, int j, int k, double h0, double h1, d
     return t / cellVolume;
 }
 
+// FFTPoissonSolver::integratedGreensFunction, FFTPoissonSolver.cpp:472-491
+template <bool CR>
+__device__ inline double igf_corner(int i, int j, int k, double h0, double h1, double h2,
+                                    double cellVolume) {
+    double v0 = i * h0 - h0 / 2;
+    double v1 = j * h1 - h1 / 2;
+    double v2 = k * h2 - h2 / 2;
+    return igf_term<CR>(v0, v1, v2, cellVolume);
+}
+
 // Corner antiderivatives T[k][j][i], i,j,k in [0, n+1], each evaluated once
 // (FFTPoissonSolver.cpp:472-491; the reference's tmpgreen field).  Grid:
 // x over the (n0+2)*(n1+2) corners of a plane, y over k.
@@ -81,18 +88,38 @@ __global__ __launch_bounds__(256) void k_igf_corners(double *T, int n0, int n1, 
     T[(size_t)k * plane + c] = igf_corner<CR>(i, j, k, h0, h1, h2, vol);
 }
 
+// The two corner tables of FFTPoissonSolver::shiftedIntGreensFunction
+// (FFTPoissonSolver.cpp:552-603): z coordinate (k*h2 - h2/2) + zshift, and for
+// the second table also - n2*h2, evaluated left to right as there.
+template <bool CR>
+__global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, int n1, int n2,
+                                                             const Scalars *sc, int second) {
+    const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2], zshift = sc->zshift;
+    const double vol = h0 * h1 * h2;
+    const int m0 = n0 + 2, plane = m0 * (n1 + 2);
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= plane) return;
+    const int j = div_small(c, m0, 1.0f / (float)m0), i = c - j * m0, k = blockIdx.y;
+    double v0 = i * h0 - h0 / 2;
+    double v1 = j * h1 - h1 / 2;
+    double v2 = k * h2 - h2 / 2 + zshift;
+    if (second) v2 = v2 - n2 * h2;
+    T[(size_t)k * plane + c] = igf_term<CR>(v0, v1, v2, vol);
+}
+
 // G octant [0,n0]x[0,n1]x[0,n2] from T: the 8-corner difference in the
 // reference's order (FFTPoissonSolver.cpp:510-517) with G[0,0,0] := G[0,0,1]
 // (FFTPoissonSolver.cpp:657).  Index n uses T at n+1 from the same formula;
 // the physical potential never reads G at index n.  Grid: x over the
-// (n0+1)*(n1+1) points of a plane, y over k.
-__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1) {
+// (n0+1)*(n1+1) points of a plane, y over k.  The shifted (image-charge)
+// tables use the same difference without the quirk (FFTPoissonSolver.cpp:621-639).
+__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1, int quirk) {
     const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= gplane) return;
     const int j = div_small(c, g0, 1.0f / (float)g0), i = c - j * g0;
     int k = blockIdx.y;
-    const int kq = (i == 0 && j == 0 && k == 0) ? 1 : k;  // quirk
+    const int kq = (quirk && i == 0 && j == 0 && k == 0) ? 1 : k;
     const size_t m0 = (size_t)n0 + 2, plane = m0 * (n1 + 2);
     const double *t = T + (size_t)kq * plane + (size_t)j * m0 + i;
     double g = t[plane + m0 + 1];
@@ -206,6 +233,43 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     for (int e = threadIdx.x; e < total; e += kFftThreads) {
         const int ln = e & (W - 1), k = e >> lw;
         Gh[base + ln + (size_t)k * S] = buf[k * P + ln].x;
+    }
+}
+
+// z transform of the image-charge Green's function, which is even in x and y
+// but not in z (mirrorRhoField(Field_t&), FFTPoissonSolver.cpp:666-691): the
+// doubled z line of column (kx, ky) is Ha[z] for z <= n2 and Hb[z - n2] for
+// n2 < z < 2n2 (Ha, Hb: the two octant tables after the x and y DCTs, pitch
+// gp), transformed forward and stored complete (L2 = 2n2 complex values) in
+// Gi[kz][ky][kx] (pitch gp).
+template <typename T, int E>
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_green_zfft(
+    const T *Ha, const T *Hb, typename cplx_of<T>::type *Gi, FftPlan pl, int n1, int n2, int gp) {
+    using C = typename cplx_of<T>::type;
+    extern __shared__ char smem[];
+    C *buf = (C *)smem;
+    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * L;
+    const int x0 = blockIdx.x * W, ky = blockIdx.y;
+    const size_t S = (size_t)gp * (n1 + 1), base = (size_t)ky * gp + x0;
+    const C *tw = fft_twiddles(buf, pl);
+    T r[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;  // total = E * kFftThreads exactly
+        const int z = e >> lw;
+        r[q] = z <= n2 ? Ha[base + (e & (W - 1)) + (size_t)z * S] : Hb[base + (e & (W - 1)) + (size_t)(z - n2) * S];
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int e = threadIdx.x + q * kFftThreads;
+        buf[(e >> lw) * P + (e & (W - 1))] = C{r[q], T(0)};
+    }
+    __syncthreads();
+    fft_lds<C, E>(buf, tw, pl, false);
+    const size_t Sg = (size_t)gp * (n1 + 1);
+    for (int e = threadIdx.x; e < total; e += kFftThreads) {
+        const int ln = e & (W - 1), kz = e >> lw;
+        Gi[(size_t)kz * Sg + (size_t)ky * gp + x0 + ln] = buf[kz * P + ln];
     }
 }
 
@@ -329,12 +393,34 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         }
         __syncthreads();
         fft_lds<C, E>(buf, tw, pl, true);
+    } else if constexpr (MODE == 3) {
+        // image charge: times -Gi (complex, not even in z; y folded), FFTPoissonSolver.cpp:316
+        fft_lds<C, E>(buf, tw, pl, false);
+        const C *Gi = (const C *)Gh;
+        const int fy = o <= n1 ? o : L1 - o;
+        C g[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * kFftThreads;
+            g[q] = Gi[((size_t)(e >> lw) * (n1 + 1) + fy) * gp + kx0 + (e & (W - 1))];
+        }
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * kFftThreads;
+            const int idx = (e >> lw) * P + (e & (W - 1));
+            const C v = buf[idx];
+            buf[idx] = C{-(v.x * g[q].x - v.y * g[q].y), -(v.x * g[q].y + v.y * g[q].x)};
+        }
+        __syncthreads();
+        fft_lds<C, E>(buf, tw, pl, true);
     } else {
         fft_lds<C, E>(buf, tw, pl, MODE == 1);
     }
     for (int e = threadIdx.x; e < W * nout; e += kFftThreads) {
         const int ln = e & (W - 1), i = e >> lw;
-        out[ob + (size_t)i * outS + ln] = buf[i * P + ln];
+        // the image potential is read back flipped in z (FFTPoissonSolver.cpp:326-329)
+        const int io = MODE == 3 ? nout - 1 - i : i;
+        out[ob + (size_t)io * outS + ln] = buf[i * P + ln];
     }
 }
 
@@ -590,11 +676,34 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
 }
 
 void PoissonGrid::destroy() {
-    void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2]};
+    void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2],
+                  HimgA, HimgB, Gimg};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     rho_i64 = rho_f64 = A = B = Goct = Tcorner = Ghat = phi = Eg = nullptr;
+    HimgA = HimgB = Gimg = nullptr;
     twd[0] = twd[1] = twd[2] = nullptr;
+}
+
+// DCT-I along x (octant G, FP64, pitch n0+1 -> H, pitch gp) and along y, in place in H.
+template <typename T>
+static void dct_xy(PoissonGrid &g, const double *G, T *H) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C), gp = (size_t)g.gp;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    {
+        const FftPlan &pl = g.plan[0];
+        int nlines = (n1 + 1) * (n2 + 1);
+        dim3 grid((nlines + pl.W - 1) / pl.W);
+        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    G, H, pl, n0, nlines, g.gp));
+    }
+    {
+        const FftPlan &pl = g.plan[1];
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1);
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    H, pl, n1 + 1, gp, gp * (n1 + 1)));
+    }
 }
 
 template <typename T>
@@ -608,25 +717,13 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
         if (cr) k_igf_corners<true><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
         else k_igf_corners<false><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
         dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
-        k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1);
+        k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 1);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const size_t gp = (size_t)g.gp;
-    {
-        const FftPlan &pl = g.plan[0];
-        int nlines = (n1 + 1) * (n2 + 1);
-        dim3 grid((nlines + pl.W - 1) / pl.W);
-        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                    (const double *)g.Goct, (T *)g.Ghat, pl, n0, nlines, g.gp));
-    }
-    {
-        const FftPlan &pl = g.plan[1];
-        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                    (T *)g.Ghat, pl, n1 + 1, gp, gp * (n1 + 1)));
-    }
+    dct_xy<T>(g, (const double *)g.Goct, (T *)g.Ghat);
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
@@ -636,13 +733,64 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
     g.profile_mark(PROF_DCT);
 }
 
+// Transform of the image-charge Green's function (FFTPoissonSolver::shiftedIntGreensFunction,
+// FFTPoissonSolver.cpp:546-639) for the shift in the scalars: both corner
+// tables, their differences, DCT-I in x and y, complex FFT along the doubled z.
+template <typename T>
+static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C);
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    T *H[2] = {(T *)g.HimgA, (T *)g.HimgB};
+    for (int second = 0; second < 2; ++second) {
+        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
+        if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
+        else k_igf_corners_shifted<false><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
+        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
+        k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 0);
+        dct_xy<T>(g, (const double *)g.Goct, H[second]);
+    }
+    g.profile_mark(PROF_GREEN);
+    {
+        const FftPlan &pl = g.plan[2];
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
+        OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    H[0], H[1], (C *)g.Gimg, pl, n1, n2, g.gp));
+    }
+    g.profile_mark(PROF_DCT);
+}
+
 void PoissonGrid::green(const Scalars *d_sc, bool cr) {
     if (fp64) green_impl<double>(*this, d_sc, cr);
     else green_impl<float>(*this, d_sc, cr);
 }
 
+int PoissonGrid::ensure_image(std::string &err) {
+    if (Gimg) return 0;
+    const size_t rs = fp64 ? 8 : 4, cs = 2 * rs;
+    const size_t szH = (size_t)gp * (n[1] + 1) * (n[2] + 1) * rs;
+    const size_t szG = (size_t)L[2] * (n[1] + 1) * gp * cs;
+    if (hipMalloc(&HimgA, szH) != hipSuccess || hipMalloc(&HimgB, szH) != hipSuccess ||
+        hipMalloc(&Gimg, szG) != hipSuccess) {
+        err = "hipMalloc of the image-charge Green's function failed";
+        return -1;
+    }
+    // padding columns are read as whole tiles and must hold (and keep) zeros
+    if (hipMemset(HimgA, 0, szH) != hipSuccess || hipMemset(HimgB, 0, szH) != hipSuccess ||
+        hipMemset(Gimg, 0, szG) != hipSuccess) {
+        err = "hipMemset failed";
+        return -1;
+    }
+    return 0;
+}
+
+void PoissonGrid::green_image(const Scalars *d_sc, bool cr) {
+    if (fp64) green_image_impl<double>(*this, d_sc, cr);
+    else green_image_impl<float>(*this, d_sc, cr);
+}
+
 template <typename T>
-static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int outscale) {
+static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int outscale, bool image) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], K0 = g.K0;
@@ -671,9 +819,14 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((K0 + pl.W - 1) / pl.W, L1);
-        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                    B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Ghat, g.gp,
-                                    n1, n2, L1));
+        if (image)
+            OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 3><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Gimg, g.gp,
+                                        n1, n2, L1))
+        else
+            OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Ghat, g.gp,
+                                        n1, n2, L1))
     }
     g.profile_mark(PROF_FFT_Z);
     {
@@ -697,9 +850,9 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
     g.profile_mark(PROF_FFT_X);
 }
 
-void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale) {
-    if (fp64) solve_impl<double>(*this, d_sc, src_kind, outscale);
-    else solve_impl<float>(*this, d_sc, src_kind, outscale);
+void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale, bool image) {
+    if (fp64) solve_impl<double>(*this, d_sc, src_kind, outscale, image);
+    else solve_impl<float>(*this, d_sc, src_kind, outscale, image);
 }
 
 void PoissonGrid::grad(const Scalars *d_sc) {

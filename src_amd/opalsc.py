@@ -32,7 +32,8 @@ M_P_EV = 0.93827204e+00 * 1e9              # src/Classic/Physics/Physics.h:88 (G
 
 EXPORTS = [
     "opalsc_abi_version", "opalsc_create", "opalsc_destroy", "opalsc_last_error",
-    "opalsc_synchronize", "opalsc_compute_potential", "opalsc_boundp", "opalsc_cic_index",
+    "opalsc_synchronize", "opalsc_compute_potential", "opalsc_compute_potential_shifted",
+    "opalsc_boundp", "opalsc_cic_index",
     "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_get_grid", "opalsc_push_half",
     "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
     "opalsc_bunch_get", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
@@ -93,6 +94,7 @@ def lib():
         "opalsc_last_error": ([vp], C.c_char_p),
         "opalsc_synchronize": ([vp], C.c_int),
         "opalsc_compute_potential": ([vp, dp, dp], C.c_int),
+        "opalsc_compute_potential_shifted": ([vp, dp, dp, C.c_double], C.c_int),
         "opalsc_boundp": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_cic_index": ([vp, i64, dp, dp, dp, dp, dp], C.c_int),
         "opalsc_sort_by_cell": ([vp, i64, dp, dp, dp, dp], C.c_int),
@@ -206,6 +208,13 @@ class SpaceChargeContext:
         phi = _d(rho).ravel().copy()
         assert phi.size == self.ncell
         self._c(lib().opalsc_compute_potential(self._h, _ptr(phi), _ptr(_d(hr))))
+        return phi
+
+    # FFTPoissonSolver::computePotential(rho, hr, zshift): image-charge potential
+    def computePotentialShifted(self, rho, hr, zshift):
+        phi = _d(rho).ravel().copy()
+        assert phi.size == self.ncell
+        self._c(lib().opalsc_compute_potential_shifted(self._h, _ptr(phi), _ptr(_d(hr)), float(zshift)))
         return phi
 
     def boundp(self, R):
@@ -351,5 +360,9 @@ class FFTPoissonSolver:
         self._ctx = SpaceChargeContext(n, precision=precision, greens=g, max_particles=0,
                                        device=device)
 
-    def computePotential(self, rho, hr):
-        return self._ctx.computePotential(rho, hr)
+    def computePotential(self, rho, hr, zshift=None):
+        """In place in the reference (rho in, phi out); here rho is returned as phi.
+        With zshift: the image-charge overload (FFTPoissonSolver.cpp:289-330)."""
+        if zshift is None:
+            return self._ctx.computePotential(rho, hr)
+        return self._ctx.computePotentialShifted(rho, hr, zshift)

@@ -280,3 +280,18 @@ def test_compute_self_fields_fp64_c2_full_size():
         err = rel_l2(a, b)
         print("C2 %s rel L2 %.3e" % (name, err))
         assert err < 1e-12, name
+
+
+@pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-12), (sc.MIXED, 1e-5)])
+@pytest.mark.parametrize("n,zshift", [((16, 16, 16), -0.021), ((32, 16, 24), 0.013), ((12, 24, 8), -0.35)])
+def test_compute_potential_shifted(n, zshift, precision, tol):
+    """Image-charge solve (shifted Green's function, negated, z flipped)."""
+    rng = np.random.default_rng(17)
+    rho = rng.normal(size=n[0] * n[1] * n[2])
+    hr = np.array([1.1e-4, 0.9e-4, 2.3e-4])
+    ref = O.compute_potential_shifted(n, rho, hr, zshift)
+    with ctx(n, precision=precision) as c:
+        phi = c.computePotentialShifted(rho, hr, zshift)
+        phi0 = c.computePotential(rho, hr)  # the plain solve still works after the image one
+    assert rel_l2(phi, ref) < tol
+    assert rel_l2(phi0, O.compute_potential(n, rho, hr, 0)) < max(tol, 1e-12)
