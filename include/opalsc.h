@@ -154,6 +154,25 @@ int opalsc_compute_self_fields(opalsc_ctx *ctx, int64_t N, const double *R, cons
                                double *Q, const double *dt, double dT, const double origin[3],
                                const double hr[3], double coupling, double *Ef, double *Bf);
 
+/* Energy-binned self fields: the bin loop of
+ * ParallelTTracker::computeSpaceChargeFields (src/Algorithms/ParallelTTracker.cpp:567-579)
+ * over PartBunch::computeSelfFields(int) (src/Classic/Algorithms/PartBunch.cpp:138-430).
+ * Particle i belongs to bin[i] (0 <= bin < nbins <= 64; other values belong
+ * to no bin).  Bin b is scattered with charge qi*dt per particle
+ * (setBinCharge, PartBunchBase.hpp:716-724), rho /= dT, and solved on the
+ * mesh (origin, hr) with its own Lorentz factor gamma_bin[b] (NULL: the mean
+ * of sqrt(1+P.P) over the bin, calcGammas, PartBunchBase.hpp:639-682; an
+ * empty bin has no field).  If image != 0 its image charge in the cathode
+ * plane z = 0 (shifted integrated Green's function, PartBunch.cpp:252-303) is
+ * solved too.  Ef (overwritten) = sum over bins of E_bin + E_image;
+ * Bf (accumulated) += sum over bins of betaC_b * (-Ey, Ex, 0) of the bin and
+ * betaC_b * (Ey, -Ex, 0) of its image. */
+int opalsc_compute_self_fields_binned(opalsc_ctx *ctx, int64_t N, const double *R, const double *P,
+                                      const int32_t *bin, int nbins, double qi, const double *dt,
+                                      double dT, const double origin[3], const double hr[3],
+                                      double coupling, const double *gamma_bin, int image,
+                                      double *Ef, double *Bf);
+
 /* Grids of the most recent compute_self_fields / track step: OPALSC_GRID_RHO
  * (charge density entering the solver, n0*n1*n2), OPALSC_GRID_PHI (potential
  * in V, n0*n1*n2), OPALSC_GRID_EFIELD (AoS [n0*n1*n2][3] after the Lorentz

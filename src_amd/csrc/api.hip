@@ -56,6 +56,7 @@ struct opalsc_ctx {
     double *partials = nullptr;
     unsigned long long *d_counters = nullptr;  // [0] qd max bits, [1] dropped
     DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
+    DevBuf binb, gamb, cntb;  // energy bins: per-particle bin, per-bin gamma, per-bin count
     double qmax = 0.0;        // max |Q| of the resident bunch
     bool have_bunch = false, have_track = false;
     bool track_sc = false;    // the tracker computes space-charge fields
@@ -426,6 +427,93 @@ extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double
     HIPCHK(c, hipMemcpyAsync(Ef, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(Q, b.q, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_bunch = false;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const double *R, const double *P,
+                                                 const int32_t *bin, int nbins, double qi, const double *dt,
+                                                 double dT, const double origin[3], const double hr[3],
+                                                 double coupling, const double *gamma_bin, int image,
+                                                 double *Ef, double *Bf) {
+    if (!c || !origin || !hr) return OPALSC_EINVAL;
+    if (N > 0 && (!R || !P || !bin || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
+    if (nbins < 1 || nbins > kMaxBins) FAIL(c, OPALSC_EINVAL, "nbins must be in [1, 64]");
+    if (int e = check_n(c, N)) return e;
+    if (N == 0) return OPALSC_OK;
+    if (c->grid.ensure_sums(c->err)) return OPALSC_ENOMEM;
+    if (image && c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
+    Bunch &b = c->bunch;
+    const bool exact = c->grid.fp64;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosBf, Bf, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->dtb, dt, (size_t)N * 8)) return e;
+    if (int e = upload(c, c->binb, bin, (size_t)N * 4)) return e;
+    if (!c->aosEf.ensure((size_t)N * 24) || !c->gamb.ensure(kMaxBins * 8) || !c->cntb.ensure(kMaxBins * 8))
+        FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    if (gamma_bin)
+        HIPCHK(c, hipMemcpyAsync(c->gamb.p, gamma_bin, (size_t)nbins * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 2, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->cntb.p, 0, kMaxBins * 8, c->stream));
+    Scalars s;
+    memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) {
+        s.n[d] = c->cfg.n[d];
+        s.hr[d] = hr[d];
+        s.origin[d] = origin[d];
+    }
+    s.dT = dT;
+    s.coupling = coupling;
+    s.N = N;
+    upload_scalars(c, s);
+    // order by bin: each bin's particles become one range of the deposit
+    launch_soa_from_aos(c->stream, N, c->aosR.as<double>(), b.x, b.y, b.z);
+    launch_bin_count(c->stream, N, c->binb.as<int32_t>(), nbins, c->cntb.as<unsigned long long>());
+    launch_bin_keys(c->stream, N, c->binb.as<int32_t>(), nbins, b.keys, b.vals);
+    int bits = 1;
+    while ((1 << bits) <= nbins) ++bits;
+    if (sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, bits))
+        FAIL(c, OPALSC_EHIP, "radix sort failed");
+    launch_bin_order(c->stream, N, b.vals2, b.x, b.y, b.z, c->dtb.as<double>(), qi, b.px, b.py, b.pz, b.qd);
+    launch_absmax(c->stream, N, b.qd, &c->d_counters[0]);
+    unsigned long long cnt[kMaxBins];
+    HIPCHK(c, hipMemcpyAsync(cnt, c->cntb.p, (size_t)nbins * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t start[kMaxBins];
+    int64_t acc = 0;
+    for (int k = 0; k < nbins; ++k) {
+        start[k] = acc;
+        acc += (int64_t)cnt[k];
+    }
+    if (!gamma_bin)
+        for (int k = 0; k < nbins; ++k)
+            launch_bin_gamma(c->stream, (int64_t)cnt[k], b.vals2 + start[k], c->aosP.as<double>(), c->partials, k,
+                             c->gamb.as<double>());
+    c->grid.clear_sums();
+    for (int k = 0; k < nbins; ++k) {
+        if (cnt[k] == 0) continue;  // an empty bin has no field
+        const int64_t s0 = start[k], nk = (int64_t)cnt[k];
+        launch_bin_scalars(c->stream, c->d_sc, c->gamb.as<double>(), k, nk, &c->d_counters[0], exact);
+        launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+        launch_deposit(c->stream, 0, exact, nk, b.px + s0, b.py + s0, b.pz + s0, nullptr, b.qd + s0,
+                       (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], nullptr);
+        c->grid.green(c->d_sc, exact);
+        c->grid.solve(c->d_sc, 0, 1);
+        c->grid.grad_accum(c->d_sc, -1.0);
+        if (image) {
+            c->grid.green_image(c->d_sc, exact);
+            c->grid.solve(c->d_sc, 0, 1, true);
+            c->grid.grad_accum(c->d_sc, +1.0);
+        }
+        CHECK_LAUNCH(c);
+    }
+    launch_gather_binned(c->stream, c->grid.fp64, N, b.x, b.y, b.z, c->grid.Esum, c->grid.Bsum, c->d_sc,
+                         c->aosEf.as<double>(), c->aosBf.as<double>());
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(Ef, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_bunch = false;
     return OPALSC_OK;

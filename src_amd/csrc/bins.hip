@@ -1,0 +1,179 @@
+// Energy-binned self fields, gfx950: ParallelTTracker::computeSpaceChargeFields
+// with energy bins (src/Algorithms/ParallelTTracker.cpp:567-579) over
+// PartBunch::computeSelfFields(int) (src/Classic/Algorithms/PartBunch.cpp:138-430).
+//
+// Bin b gives its particles the macro charge qi (setBinCharge,
+// PartBunchBase.hpp:716-724) and is solved with its own Lorentz factor
+// (calcGammas, PartBunchBase.hpp:639-682), once for the bunch and once for its
+// image charge.  The particles are ordered by bin, so each bin's deposit is
+// one contiguous range.  The fields of all bins are summed on the grid and
+// gathered once (CIC gathering is linear).
+#include "common.hpp"
+#include "gather.hpp"
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+namespace opalsc {
+
+// keys for ordering by bin: bins outside [0, nbins) sort last and are ignored
+__global__ void k_bin_keys(int64_t N, const int32_t *bin, int nbins, uint32_t *keys, uint32_t *vals) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = bin[i];
+        keys[i] = (b >= 0 && b < nbins) ? (uint32_t)b : (uint32_t)nbins;
+        vals[i] = (uint32_t)i;
+    }
+}
+void launch_bin_keys(hipStream_t s, int64_t N, const int32_t *bin, int nbins, uint32_t *keys, uint32_t *vals) {
+    if (N > 0) k_bin_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, bin, nbins, keys, vals);
+}
+
+// particles per bin (LDS histogram per block)
+__global__ __launch_bounds__(256) void k_bin_count(int64_t N, const int32_t *bin, int nbins,
+                                                   unsigned long long *counts) {
+    __shared__ unsigned int h[kMaxBins];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = bin[i];
+        if (b >= 0 && b < nbins) atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (h[b]) atomicAdd(&counts[b], (unsigned long long)h[b]);
+}
+void launch_bin_count(hipStream_t s, int64_t N, const int32_t *bin, int nbins, unsigned long long *counts) {
+    if (N > 0) k_bin_count<<<std::min(stream_blocks(N), 4096), kThreads, 0, s>>>(N, bin, nbins, counts);
+}
+
+// the bin-ordered copies the deposit reads: positions and qd = qi*dt
+__global__ void k_bin_order(int64_t N, const uint32_t *perm, const double *x, const double *y,
+                            const double *z, const double *dt, double qi, double *ox, double *oy,
+                            double *oz, double *oqd) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = perm[i];
+        ox[i] = x[j];
+        oy[i] = y[j];
+        oz[i] = z[j];
+        double qd = qi;  // Q = qi (setBinCharge), Q *= dt (PartBunch.cpp:154)
+        qd *= dt[j];
+        oqd[i] = qd;
+    }
+}
+void launch_bin_order(hipStream_t s, int64_t N, const uint32_t *perm, const double *x, const double *y,
+                      const double *z, const double *dt, double qi, double *ox, double *oy, double *oz,
+                      double *oqd) {
+    if (N > 0) k_bin_order<<<stream_blocks(N), kThreads, 0, s>>>(N, perm, x, y, z, dt, qi, ox, oy, oz, oqd);
+}
+
+// calcGammas: per bin, the sum of sqrt(1 + P.P) over its particles (double-double
+// across threads and blocks), then the mean; an empty bin gets 1.
+__global__ __launch_bounds__(256) void k_bin_gamma_partials(int64_t N, const uint32_t *perm, const double *P,
+                                                            double *partials) {
+    __shared__ double lds[4 * 2];
+    double h[1] = {0.0}, l[1] = {0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const double *p = P + 3 * (size_t)perm[i];
+        const dd r = dd_add(dd{h[0], l[0]}, dd{sqrt(1.0 + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2])), 0.0});
+        h[0] = r.hi;
+        l[0] = r.lo;
+    }
+    block_sum_dd<1>(h, l, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x * kSlots] = h[0];
+        partials[blockIdx.x * kSlots + 1] = l[0];
+    }
+}
+__global__ __launch_bounds__(kReduceThreads) void k_bin_gamma_final(const double *partials, int nb, int64_t count,
+                                                                    int b, double *gammas) {
+    __shared__ double lds[16 * 2];
+    double out[1];
+    reduce_partials_dd<1>(partials, nb, out, lds);
+    if (threadIdx.x == 0) gammas[b] = count != 0 ? out[0] / (double)count : 1.0;
+}
+void launch_bin_gamma(hipStream_t s, int64_t count, const uint32_t *perm, const double *P, double *partials,
+                      int b, double *gammas) {
+    const int nb = count > 0 ? red_blocks(count) : 0;
+    if (nb > 0) k_bin_gamma_partials<<<nb, kThreads, 0, s>>>(count, perm, P, partials);
+    k_bin_gamma_final<<<1, kReduceThreads, 0, s>>>(partials, nb, count, b, gammas);
+}
+
+// The scalars of bin b (PartBunch.cpp:172-191,215,236,261-264): Lorentz factor,
+// density scale 1/h0*1/h1*1/h2/gamma, solver spacing (h0, h1, h2*gamma), field
+// scale (gamma, gamma, 1/gamma), betaC and the image shift; fixed-point scale
+// for count deposits of at most qdmax.  Mesh, n, dT and coupling are already set.
+__global__ void k_bin_scalars(Scalars *sc, const double *gammas, int b, int64_t count,
+                              const unsigned long long *qdmax_bits, int exact) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double scaleFactor = 1.0;
+    const double gammaz = gammas[b];
+    const double *hr = sc->hr;
+    sc->gammaz = gammaz;
+    sc->tmp2 = 1 / hr[0] * 1 / hr[1] * 1 / hr[2] / (scaleFactor * scaleFactor * scaleFactor) / gammaz;
+    double hs[3] = {hr[0] * scaleFactor, hr[1] * scaleFactor, hr[2] * scaleFactor};
+    hs[2] *= gammaz;
+    for (int d = 0; d < 3; ++d) sc->hs[d] = hs[d];
+    sc->vol = hs[0] * hs[1] * hs[2];
+    sc->gscale[0] = gammaz / scaleFactor;
+    sc->gscale[1] = gammaz / scaleFactor;
+    sc->gscale[2] = 1.0 / (scaleFactor * gammaz);
+    sc->betaC = sqrt(gammaz * gammaz - 1.0) / gammaz / kC;
+    const double hz = hr[2];
+    sc->zshift = -(2 * sc->origin[2] + (0 + (sc->n[2] - 1) + 1) * hz) * gammaz * scaleFactor;
+    // fixed point: count deposits of |qd| <= qdmax (FP64: sum < 2^62; mixed: each < 2^24)
+    const double qdmax = __longlong_as_double((long long)*qdmax_bits);
+    const double B = (exact ? (double)(count > 0 ? count : 1) : 1.0) * qdmax * (1.0 + 0x1p-40);
+    int e = 0;
+    if (B > 0.0) {
+        int E;
+        (void)frexp(B, &E);
+        e = (exact ? 62 : 24) - E;
+        if (e > 1000) e = 1000;
+    }
+    sc->e2q = ldexp(1.0, e);
+    sc->q2e = ldexp(1.0, -e);
+    sc->qd_absmax = qdmax;
+}
+void launch_bin_scalars(hipStream_t s, Scalars *sc, const double *gammas, int b, int64_t count,
+                        const unsigned long long *qdmax_bits, bool exact) {
+    k_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, b, count, qdmax_bits, exact ? 1 : 0);
+}
+
+// Ef = gather(sum over bins of E), Bf += gather(sum over bins of the implied B)
+// (PartBunch.cpp:238-248,419-426 for every bin), AoS outputs.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_binned(int64_t N, const double *x, const double *y, const double *z,
+                                                       const typename vec4_of<T>::type *Es,
+                                                       const typename vec4_of<T>::type *Bs, const Scalars *sc,
+                                                       double *Ef, double *Bf) {
+    const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+    const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        int l[3];
+        double d[3], e[3] = {0.0, 0.0, 0.0}, bb[3] = {0.0, 0.0, 0.0};
+        bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
+        ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
+        ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
+        if (ok) {
+            gather_E<T>(Es, l, d, n0, n1, n2, e);
+            gather_E<T>(Bs, l, d, n0, n1, n2, bb);
+        }
+        Ef[3 * p] = e[0];
+        Ef[3 * p + 1] = e[1];
+        Ef[3 * p + 2] = e[2];
+        Bf[3 * p] = Bf[3 * p] + bb[0];
+        Bf[3 * p + 1] = Bf[3 * p + 1] + bb[1];
+    }
+}
+void launch_gather_binned(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y, const double *z,
+                          const void *Es, const void *Bs, const Scalars *sc, double *Ef, double *Bf) {
+    if (N <= 0) return;
+    if (fp64)
+        k_gather_binned<double><<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, (const double4 *)Es,
+                                                                      (const double4 *)Bs, sc, Ef, Bf);
+    else
+        k_gather_binned<float><<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, (const float4 *)Es,
+                                                                     (const float4 *)Bs, sc, Ef, Bf);
+}
+
+}  // namespace opalsc

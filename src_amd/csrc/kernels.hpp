@@ -63,6 +63,9 @@ struct PoissonGrid {
     // octant tables after the x and y DCTs (T[n2+1][n1+1][gp] each) and the
     // transform over the doubled z (complex T[2 n2][n1+1][gp])
     void *HimgA = nullptr, *HimgB = nullptr, *Gimg = nullptr;
+    // energy bins (allocated on first use): sums over bins of E and of the
+    // magnetic field it implies, vec4<T>[ncell] each
+    void *Esum = nullptr, *Bsum = nullptr;
     hipStream_t stream = nullptr;
     Profiler *prof = nullptr;
 
@@ -76,6 +79,10 @@ struct PoissonGrid {
     // image: convolve with the image-charge Green's function (negated, z flipped)
     void solve(const Scalars *d_sc, int src_kind, int outscale, bool image = false);
     void grad(const Scalars *d_sc);
+    int ensure_sums(std::string &err);
+    void clear_sums();
+    // Esum += E, Bsum += (s*betaC*Ey, -s*betaC*Ex): s = -1 a bin's field, +1 its image
+    void grad_accum(const Scalars *d_sc, double sign);
     void rho_to_double(const Scalars *d_sc, double *out);
     void widen(int which, double *out);
     void profile_mark(int id) {
@@ -155,6 +162,20 @@ void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState
 void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,
                                Scalars *sc);
 void launch_moments_soa(hipStream_t s, const Bunch &b, double *partials);
+
+// Energy bins (bins.hip)
+constexpr int kMaxBins = 64;
+void launch_bin_keys(hipStream_t s, int64_t N, const int32_t *bin, int nbins, uint32_t *keys, uint32_t *vals);
+void launch_bin_count(hipStream_t s, int64_t N, const int32_t *bin, int nbins, unsigned long long *counts);
+void launch_bin_order(hipStream_t s, int64_t N, const uint32_t *perm, const double *x, const double *y,
+                      const double *z, const double *dt, double qi, double *ox, double *oy, double *oz,
+                      double *oqd);
+void launch_bin_gamma(hipStream_t s, int64_t count, const uint32_t *perm, const double *P, double *partials,
+                      int b, double *gammas);
+void launch_bin_scalars(hipStream_t s, Scalars *sc, const double *gammas, int b, int64_t count,
+                        const unsigned long long *qdmax_bits, bool exact);
+void launch_gather_binned(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y, const double *z,
+                          const void *Es, const void *Bs, const Scalars *sc, double *Ef, double *Bf);
 
 // Sorting (sort.hip)
 size_t sort_temp_bytes(int64_t N);

@@ -10,131 +10,17 @@
 #include "crmath.hpp"
 #include "frame.hpp"
 #include "kernels.hpp"
+#include "gather.hpp"
+#include "reduce.hpp"
 
 namespace opalsc {
 
-using crm::dd;
-using crm::dd_add;
-
-constexpr int kSlots = 64;           // doubles per block in the partials buffer
 constexpr int kXcds = 8;             // XCDs of an MI355X; workgroups are dealt to them round-robin
-constexpr int kReduceThreads = 512;   // single-block final reductions of the partials
 
 int red_blocks(int64_t N) {
     int64_t b = (N + kThreads - 1) / kThreads;
     if (b > kRedBlocksMax) b = kRedBlocksMax;
     return (int)b;
-}
-
-static int stream_blocks(int64_t N) {
-    int64_t b = (N + kThreads - 1) / kThreads;
-    if (b > 65536) b = 65536;
-    if (b < 1) b = 1;
-    return (int)b;
-}
-
-// ---- deterministic block reductions ---------------------------------------
-template <int K>
-__device__ inline void block_sum_dd(double (&h)[K], double (&l)[K], double *lds) {
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double oh = __shfl_xor(h[k], off, 64), ol = __shfl_xor(l[k], off, 64);
-            dd r = dd_add(dd{h[k], l[k]}, dd{oh, ol});
-            h[k] = r.hi;
-            l[k] = r.lo;
-        }
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < K; ++k) {
-            lds[(wave * K + k) * 2] = h[k];
-            lds[(wave * K + k) * 2 + 1] = l[k];
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < K; ++k) {
-            dd a = {lds[2 * k], lds[2 * k + 1]};
-            for (int w = 1; w < nw; ++w) a = dd_add(a, dd{lds[(w * K + k) * 2], lds[(w * K + k) * 2 + 1]});
-            h[k] = a.hi;
-            l[k] = a.lo;
-        }
-    }
-    __syncthreads();
-}
-
-__device__ inline void block_minmax(double (&mn)[3], double (&mx)[3], double *lds) {
-    for (int d = 0; d < 3; ++d) {
-        mn[d] = wave_min(mn[d]);
-        mx[d] = wave_max(mx[d]);
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    if (lane == 0)
-        for (int d = 0; d < 3; ++d) {
-            lds[wave * 6 + d] = mn[d];
-            lds[wave * 6 + 3 + d] = mx[d];
-        }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int w = 1; w < nw; ++w)
-            for (int d = 0; d < 3; ++d) {
-                mn[d] = fmin(mn[d], lds[w * 6 + d]);
-                mx[d] = fmax(mx[d], lds[w * 6 + 3 + d]);
-            }
-    __syncthreads();
-}
-
-// Block min/max and sums of positions and squared positions (the chunk
-// statistics the deposit sizes its LDS box from); result valid in thread 0.
-__device__ inline void block_chunk_stats(double (&st)[kBoxStats], double *lds) {
-    for (int d = 0; d < 3; ++d) {
-        st[d] = wave_min(st[d]);
-        st[3 + d] = wave_max(st[3 + d]);
-    }
-    for (int k = 6; k < kBoxStats; ++k) st[k] = wave_sum(st[k]);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < kBoxStats; ++k) lds[wave * kBoxStats + k] = st[k];
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int w = 1; w < nw; ++w) {
-            for (int d = 0; d < 3; ++d) {
-                st[d] = fmin(st[d], lds[w * kBoxStats + d]);
-                st[3 + d] = fmax(st[3 + d], lds[w * kBoxStats + 3 + d]);
-            }
-            for (int k = 6; k < kBoxStats; ++k) st[k] += lds[w * kBoxStats + k];
-        }
-    __syncthreads();
-}
-
-// Sum K dd values over nb partial blocks (slots 2k, 2k+1) with one block of
-// 256 threads; result valid in thread 0.
-template <int K>
-__device__ inline void reduce_partials_dd(const double *partials, int nb, double (&out)[K], double *lds) {
-    double h[K], l[K];
-    for (int k = 0; k < K; ++k) h[k] = l[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        for (int k = 0; k < K; ++k) {
-            dd r = dd_add(dd{h[k], l[k]}, dd{partials[b * kSlots + 2 * k], partials[b * kSlots + 2 * k + 1]});
-            h[k] = r.hi;
-            l[k] = r.lo;
-        }
-    block_sum_dd<K>(h, l, lds);
-    for (int k = 0; k < K; ++k) out[k] = h[k] + l[k];
-}
-
-__device__ inline void reduce_partials_minmax(const double *partials, int nb, double (&mn)[3],
-                                              double (&mx)[3], double *lds) {
-    for (int d = 0; d < 3; ++d) {
-        mn[d] = __builtin_inf();
-        mx[d] = -__builtin_inf();
-    }
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        for (int d = 0; d < 3; ++d) {
-            mn[d] = fmin(mn[d], partials[b * kSlots + d]);
-            mx[d] = fmax(mx[d], partials[b * kSlots + 3 + d]);
-        }
-    block_minmax(mn, mx, lds);
 }
 
 // ---- layout conversion ------------------------------------------------------
@@ -606,73 +492,6 @@ void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double
 }
 
 // ---- CIC gather ---------------------------------------------------------------
-// IntCICImpl<3>::gather (ippl/src/Particle/IntCIC.h:933-977): sum over the 8
-// corners in reference order of ((a*b)*c)*E; corners outside the grid read 0.
-template <typename T>
-__device__ inline void gather_E(const typename vec4_of<T>::type *Eg, const int l[3], const double d[3],
-                                int n0, int n1, int n2, double Ef[3]) {
-    double w[8];
-    cic_weights(d, w);
-    T a0 = T(0), a1 = T(0), a2 = T(0);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
-        typename vec4_of<T>::type e;
-        if (i >= 0 && j >= 0 && k >= 0 && i < n0 && j < n1 && k < n2) e = Eg[((size_t)k * n1 + j) * n0 + i];
-        else e.x = e.y = e.z = e.w = T(0);
-        const T wc = (T)w[c];
-        if (c == 0) {
-            a0 = wc * e.x;
-            a1 = wc * e.y;
-            a2 = wc * e.z;
-        } else {
-            a0 = a0 + wc * e.x;
-            a1 = a1 + wc * e.y;
-            a2 = a2 + wc * e.z;
-        }
-    }
-    Ef[0] = (double)a0;
-    Ef[1] = (double)a1;
-    Ef[2] = (double)a2;
-}
-
-// Mixed-precision gather: the same stencil with FP32 weights and FMAs.
-__device__ inline void gather_E_fast(const float4 *Eg, const int l[3], const double d[3], int n0, int n1,
-                                     int n2, double Ef[3]) {
-    const float b0 = (float)d[0], b1 = (float)d[1], b2 = (float)d[2];
-    const float a0 = 1.0f - b0, a1 = 1.0f - b1, a2 = 1.0f - b2;
-    const float w[8] = {a0 * a1 * a2, b0 * a1 * a2, a0 * b1 * a2, b0 * b1 * a2,
-                        a0 * a1 * b2, b0 * a1 * b2, a0 * b1 * b2, b0 * b1 * b2};
-    const bool inside = l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] + 1 < n0 && l[1] + 1 < n1 && l[2] + 1 < n2;
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-    if (inside) {
-        const float4 *base = Eg + ((size_t)l[2] * n1 + l[1]) * n0 + l[0];
-        const size_t sy = n0, sz = (size_t)n0 * n1;
-        const float4 c[8] = {base[0], base[1], base[sy], base[sy + 1],
-                             base[sz], base[sz + 1], base[sz + sy], base[sz + sy + 1]};
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            e0 = fmaf(w[k], c[k].x, e0);
-            e1 = fmaf(w[k], c[k].y, e1);
-            e2 = fmaf(w[k], c[k].z, e2);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int i = l[0] + (k & 1), j = l[1] + ((k >> 1) & 1), kk = l[2] + ((k >> 2) & 1);
-            if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
-                const float4 c = Eg[((size_t)kk * n1 + j) * n0 + i];
-                e0 = fmaf(w[k], c.x, e0);
-                e1 = fmaf(w[k], c.y, e1);
-                e2 = fmaf(w[k], c.z, e2);
-            }
-        }
-    }
-    Ef[0] = e0;
-    Ef[1] = e1;
-    Ef[2] = e2;
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather_selffield(int64_t N, const double *x, const double *y,
                                                           const double *z,

@@ -484,12 +484,19 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 // E = -Grad(phi) * (gz, gz, 1/gz)   (PartBunch.cpp:573-575; Grad Cell->Cell,
 // ippl/src/Meshes/UniformCartesian.hpp:2127-2200; ZeroFace guards)
 // ---------------------------------------------------------------------------
-template <typename T>
+// ACC (energy bins): instead of storing E, add it to Eg and add the magnetic
+// field it implies to Bg: Bx += s*betaC*Ey, By -= s*betaC*Ex, with s = -1 for a
+// bin's own field and +1 for its image charge (PartBunch.cpp:246-247,423-424);
+// CIC gathering is linear, so gathering these sums once equals the
+// reference's per-bin gathers added at each particle.
+template <typename T, bool ACC = false>
 __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>::type *Eg, int n0,
-                                              int n1, int n2, const Scalars *sc) {
+                                              int n1, int n2, const Scalars *sc,
+                                              typename vec4_of<T>::type *Bg = nullptr, double sign = 0.0) {
     const size_t total = (size_t)n0 * n1 * n2;
     const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
     const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
+    const T bs = (T)(sign * sc->betaC);
     const size_t sy = n0, sz = (size_t)n0 * n1;
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total;
          c += (size_t)gridDim.x * blockDim.x) {
@@ -512,7 +519,18 @@ __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>:
         e.y = -r1 * g1;
         e.z = -r2 * g2;
         e.w = T(0);
-        Eg[c] = e;
+        if constexpr (ACC) {
+            typename vec4_of<T>::type a = Eg[c], b = Bg[c];
+            a.x += e.x;
+            a.y += e.y;
+            a.z += e.z;
+            b.x += bs * e.y;
+            b.y -= bs * e.x;
+            Eg[c] = a;
+            Bg[c] = b;
+        } else {
+            Eg[c] = e;
+        }
     }
 }
 
@@ -677,11 +695,11 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
 
 void PoissonGrid::destroy() {
     void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2],
-                  HimgA, HimgB, Gimg};
+                  HimgA, HimgB, Gimg, Esum, Bsum};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     rho_i64 = rho_f64 = A = B = Goct = Tcorner = Ghat = phi = Eg = nullptr;
-    HimgA = HimgB = Gimg = nullptr;
+    HimgA = HimgB = Gimg = Esum = Bsum = nullptr;
     twd[0] = twd[1] = twd[2] = nullptr;
 }
 
@@ -859,6 +877,33 @@ void PoissonGrid::grad(const Scalars *d_sc) {
     const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
     if (fp64) k_grad<double><<<blocks, 256, 0, stream>>>((const double *)phi, (double4 *)Eg, n[0], n[1], n[2], d_sc);
     else k_grad<float><<<blocks, 256, 0, stream>>>((const float *)phi, (float4 *)Eg, n[0], n[1], n[2], d_sc);
+    profile_mark(PROF_GRAD);
+}
+
+int PoissonGrid::ensure_sums(std::string &err) {
+    if (Esum) return 0;
+    const size_t bytes = ncell * 4 * (fp64 ? 8 : 4);
+    if (hipMalloc(&Esum, bytes) != hipSuccess || hipMalloc(&Bsum, bytes) != hipSuccess) {
+        err = "hipMalloc of the field sums failed";
+        return -1;
+    }
+    return 0;
+}
+
+void PoissonGrid::clear_sums() {
+    const size_t words = ncell * (fp64 ? 4 : 2);  // one 4-vector per cell, in 8-byte words
+    launch_fill_i64(stream, words, (long long *)Esum);
+    launch_fill_i64(stream, words, (long long *)Bsum);
+}
+
+void PoissonGrid::grad_accum(const Scalars *d_sc, double sign) {
+    const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
+    if (fp64)
+        k_grad<double, true><<<blocks, 256, 0, stream>>>((const double *)phi, (double4 *)Esum, n[0], n[1], n[2],
+                                                         d_sc, (double4 *)Bsum, sign);
+    else
+        k_grad<float, true><<<blocks, 256, 0, stream>>>((const float *)phi, (float4 *)Esum, n[0], n[1], n[2],
+                                                        d_sc, (float4 *)Bsum, sign);
     profile_mark(PROF_GRAD);
 }
 

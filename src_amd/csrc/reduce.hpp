@@ -1,0 +1,128 @@
+// Deterministic block reductions shared by the particle kernels: per-thread
+// values are combined across a wave with butterfly shuffles and across waves
+// in a fixed order, sums as double-double, so results do not depend on timing.
+#pragma once
+#include "common.hpp"
+#include "crmath.hpp"
+#include "kernels.hpp"
+
+namespace opalsc {
+
+using crm::dd;
+using crm::dd_add;
+
+constexpr int kSlots = 64;           // doubles per block in the partials buffer
+constexpr int kReduceThreads = 512;   // single-block final reductions of the partials
+
+inline int stream_blocks(int64_t N) {
+    int64_t b = (N + kThreads - 1) / kThreads;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---- deterministic block reductions ---------------------------------------
+template <int K>
+__device__ inline void block_sum_dd(double (&h)[K], double (&l)[K], double *lds) {
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double oh = __shfl_xor(h[k], off, 64), ol = __shfl_xor(l[k], off, 64);
+            dd r = dd_add(dd{h[k], l[k]}, dd{oh, ol});
+            h[k] = r.hi;
+            l[k] = r.lo;
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) {
+            lds[(wave * K + k) * 2] = h[k];
+            lds[(wave * K + k) * 2 + 1] = l[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; ++k) {
+            dd a = {lds[2 * k], lds[2 * k + 1]};
+            for (int w = 1; w < nw; ++w) a = dd_add(a, dd{lds[(w * K + k) * 2], lds[(w * K + k) * 2 + 1]});
+            h[k] = a.hi;
+            l[k] = a.lo;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline void block_minmax(double (&mn)[3], double (&mx)[3], double *lds) {
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = wave_min(mn[d]);
+        mx[d] = wave_max(mx[d]);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) {
+            lds[wave * 6 + d] = mn[d];
+            lds[wave * 6 + 3 + d] = mx[d];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < nw; ++w)
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = fmin(mn[d], lds[w * 6 + d]);
+                mx[d] = fmax(mx[d], lds[w * 6 + 3 + d]);
+            }
+    __syncthreads();
+}
+
+// Block min/max and sums of positions and squared positions (the chunk
+// statistics the deposit sizes its LDS box from); result valid in thread 0.
+__device__ inline void block_chunk_stats(double (&st)[kBoxStats], double *lds) {
+    for (int d = 0; d < 3; ++d) {
+        st[d] = wave_min(st[d]);
+        st[3 + d] = wave_max(st[3 + d]);
+    }
+    for (int k = 6; k < kBoxStats; ++k) st[k] = wave_sum(st[k]);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < kBoxStats; ++k) lds[wave * kBoxStats + k] = st[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < nw; ++w) {
+            for (int d = 0; d < 3; ++d) {
+                st[d] = fmin(st[d], lds[w * kBoxStats + d]);
+                st[3 + d] = fmax(st[3 + d], lds[w * kBoxStats + 3 + d]);
+            }
+            for (int k = 6; k < kBoxStats; ++k) st[k] += lds[w * kBoxStats + k];
+        }
+    __syncthreads();
+}
+
+// Sum K dd values over nb partial blocks (slots 2k, 2k+1) with one block of
+// 256 threads; result valid in thread 0.
+template <int K>
+__device__ inline void reduce_partials_dd(const double *partials, int nb, double (&out)[K], double *lds) {
+    double h[K], l[K];
+    for (int k = 0; k < K; ++k) h[k] = l[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int k = 0; k < K; ++k) {
+            dd r = dd_add(dd{h[k], l[k]}, dd{partials[b * kSlots + 2 * k], partials[b * kSlots + 2 * k + 1]});
+            h[k] = r.hi;
+            l[k] = r.lo;
+        }
+    block_sum_dd<K>(h, l, lds);
+    for (int k = 0; k < K; ++k) out[k] = h[k] + l[k];
+}
+
+__device__ inline void reduce_partials_minmax(const double *partials, int nb, double (&mn)[3],
+                                              double (&mx)[3], double *lds) {
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = __builtin_inf();
+        mx[d] = -__builtin_inf();
+    }
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], partials[b * kSlots + d]);
+            mx[d] = fmax(mx[d], partials[b * kSlots + 3 + d]);
+        }
+    block_minmax(mn, mx, lds);
+}
+
+}  // namespace opalsc

@@ -34,7 +34,8 @@ EXPORTS = [
     "opalsc_abi_version", "opalsc_create", "opalsc_destroy", "opalsc_last_error",
     "opalsc_synchronize", "opalsc_compute_potential", "opalsc_compute_potential_shifted",
     "opalsc_boundp", "opalsc_cic_index",
-    "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_get_grid", "opalsc_push_half",
+    "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_compute_self_fields_binned",
+    "opalsc_get_grid", "opalsc_push_half",
     "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
     "opalsc_bunch_get", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
@@ -100,6 +101,9 @@ def lib():
         "opalsc_sort_by_cell": ([vp, i64, dp, dp, dp, dp], C.c_int),
         "opalsc_compute_self_fields": ([vp, i64, dp, dp, dp, dp, C.c_double, dp, dp, C.c_double,
                                         dp, dp], C.c_int),
+        "opalsc_compute_self_fields_binned": ([vp, i64, dp, dp, dp, C.c_int, C.c_double, dp,
+                                               C.c_double, dp, dp, C.c_double, dp, C.c_int, dp, dp],
+                                              C.c_int),
         "opalsc_get_grid": ([vp, C.c_int, dp], C.c_int),
         "opalsc_push_half": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_kick": ([vp, i64, dp, dp, dp, dp, C.c_double, C.c_double], C.c_int),
@@ -250,6 +254,26 @@ class SpaceChargeContext:
                                                  dT, _ptr(_d(origin)), _ptr(_d(hr)), coupling,
                                                  _ptr(Ef), _ptr(Bf)))
         return Ef, Bf, Q
+
+    # ParallelTTracker::computeSpaceChargeFields with energy bins over
+    # PartBunch::computeSelfFields(int bin)
+    def computeSelfFieldsBinned(self, R, P, bin, nbins, qi, dt, dT, origin, hr,
+                                coupling=COUPLING, gammas=None, image=False, Bf=None):
+        R, P = _d(R, (-1, 3)), _d(P, (-1, 3))
+        N = len(R)
+        bin = np.ascontiguousarray(bin, dtype=np.int32)
+        if bin.shape != (N,):
+            raise ValueError("bin must hold one entry per particle")
+        Ef = np.zeros((N, 3))
+        Bf = np.zeros((N, 3)) if Bf is None else _d(Bf, (-1, 3)).copy()
+        g = None if gammas is None else _d(gammas)
+        if g is not None and len(g) < nbins:
+            raise ValueError("gammas must hold nbins entries")
+        self._c(lib().opalsc_compute_self_fields_binned(
+            self._h, N, _ptr(R), _ptr(P), _ptr(bin), int(nbins), float(qi), _ptr(_d(dt)), dT,
+            _ptr(_d(origin)), _ptr(_d(hr)), coupling, None if g is None else _ptr(g),
+            1 if image else 0, _ptr(Ef), _ptr(Bf)))
+        return Ef, Bf
 
     def get_grid(self, which):
         out = np.zeros(self.ncell * (3 if which == GRID_EFIELD else 1))

@@ -295,3 +295,59 @@ def test_compute_potential_shifted(n, zshift, precision, tol):
         phi0 = c.computePotential(rho, hr)  # the plain solve still works after the image one
     assert rel_l2(phi, ref) < tol
     assert rel_l2(phi0, O.compute_potential(n, rho, hr, 0)) < max(tol, 1e-12)
+
+
+def _binned_case(N, n, nbins, seed, precision, image, given_gammas):
+    R, P, _ = gaussian_bunch(N, seed=seed, q_total=1e-9)
+    R = R + np.array([0.0, 0.0, 0.004])  # bunch 4 mm downstream of the cathode plane z = 0
+    P[:, 2] += np.linspace(0.0, 0.3, N)  # energy spread across the bins
+    rng = np.random.default_rng(seed + 1)
+    dt = rng.uniform(0.8e-11, 1.2e-11, N)
+    # bins by energy; bin 1 left empty, a few particles in no bin
+    bins = np.minimum((np.argsort(np.argsort(P[:, 2])) * nbins) // N, nbins - 1).astype(np.int32)
+    bins[bins == 1] = 2
+    bins[rng.random(N) < 0.01] = -1
+    bins[:3] = nbins  # out of range: no bin
+    qi = 1e-9 / N
+    hr, org = O.boundp(R, n)
+    Bf0 = rng.normal(size=(N, 3)) * 1e-3
+    g = None
+    if given_gammas:
+        g = 1.0 + rng.uniform(0.01, 0.5, nbins)
+    ref = O.compute_self_fields_binned(R, P, bins, nbins, qi, dt, 1e-11, org, hr, n, image=image,
+                                       gammas=g, Bf=Bf0)
+    with ctx(n, precision=precision, N=N) as c:
+        Ef, Bf = c.computeSelfFieldsBinned(R, P, bins, nbins, qi, dt, 1e-11, org, hr, gammas=g,
+                                           image=image, Bf=Bf0)
+        # a second call on the same context gives the same fields
+        Ef2, Bf2 = c.computeSelfFieldsBinned(R, P, bins, nbins, qi, dt, 1e-11, org, hr, gammas=g,
+                                             image=image, Bf=Bf0)
+    assert np.array_equal(Ef, Ef2) and np.array_equal(Bf, Bf2)
+    return ref, Ef, Bf, Bf0
+
+
+@pytest.mark.parametrize("image", [True, False])
+@pytest.mark.parametrize("given_gammas", [True, False])
+def test_compute_self_fields_binned_fp64(image, given_gammas):
+    """Energy-binned self fields with the cathode image charge, FP64 path."""
+    ref, Ef, Bf, Bf0 = _binned_case(60000, (16, 24, 32), 5, 21, sc.FP64, image, given_gammas)
+    assert ref["dropped"] == 0
+    assert rel_l2(Ef, ref["Ef"]) < 1e-12
+    assert rel_l2(Bf - Bf0, ref["Bf"] - Bf0) < 1e-12
+    assert np.array_equal(Bf[:, 2], Bf0[:, 2])
+
+
+def test_compute_self_fields_binned_mixed():
+    ref, Ef, Bf, Bf0 = _binned_case(60000, (32, 32, 32), 4, 22, sc.MIXED, True, False)
+    assert rel_l2(Ef, ref["Ef"]) < 1e-5
+    assert rel_l2(Bf - Bf0, ref["Bf"] - Bf0) < 1e-5
+
+
+def test_compute_self_fields_binned_rejects_bad_nbins():
+    R, P, _ = gaussian_bunch(100, seed=3)
+    dt = np.full(100, 1e-11)
+    hr, org = O.boundp(R, (8, 8, 8))
+    with ctx((8, 8, 8), N=100) as c:
+        for nb in (0, 65):
+            with pytest.raises(sc.OpalscError):
+                c.computeSelfFieldsBinned(R, P, np.zeros(100, np.int32), nb, 1e-12, dt, 1e-11, org, hr)
