@@ -201,6 +201,19 @@ int opalsc_bunch_set(opalsc_ctx *ctx, int64_t N, const double *R, const double *
 int opalsc_bunch_generate(opalsc_ctx *ctx, const opalsc_gauss_params *g);
 /* Download R, P, Q in the order the bunch was set (any may be NULL). */
 int opalsc_bunch_get(opalsc_ctx *ctx, double *R, double *P, double *Q);
+/* Energy bins of the resident bunch: the Bin particle attribute with
+ * setEnergyBins and the macro charge qi_m (PartBunchBase.hpp:450-456,716-724).
+ * bin[i] belongs to the i-th particle as given to bunch_set (values outside
+ * [0, nbins) belong to no bin and carry no charge).  With bins the tracker's
+ * space-charge step is ParallelTTracker::computeSpaceChargeFields' bin loop
+ * (ParallelTTracker.cpp:567-579): calcGammas, then per bin the charge qi*dT
+ * of its particles solved in the bin's Lorentz frame and, if image != 0, its
+ * image charge (PartBunch::computeSelfFields(int), PartBunch.cpp:138-430).
+ * nbins = 0 removes the bins.  Bins stay until the bunch is set again. */
+int opalsc_bunch_set_bins(opalsc_ctx *ctx, const int32_t *bin, int nbins, double qi, int image);
+/* Number of bins, particles per bin (counts[nbins]) and the Lorentz factors of
+ * the most recent tracker step (gammas[nbins]); counts/gammas may be NULL. */
+int opalsc_bunch_get_bins(opalsc_ctx *ctx, int *nbins, int64_t *counts, double *gammas);
 int opalsc_track_init(opalsc_ctx *ctx, const opalsc_track_params *p);
 /* Run nsteps steps; per_step (may be NULL) receives nsteps statistics. */
 int opalsc_track(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);

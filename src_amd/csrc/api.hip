@@ -57,6 +57,13 @@ struct opalsc_ctx {
     unsigned long long *d_counters = nullptr;  // [0] qd max bits, [1] dropped
     DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
     DevBuf binb, gamb, cntb;  // energy bins: per-particle bin, per-bin gamma, per-bin count
+    // tracker energy bins: the resident bunch is ordered by bin
+    BinRanges bins;
+    double bin_qi = 0.0;
+    bool bin_image = true;
+    DevBuf binsorted;         // int32 bin of each resident particle (bin order)
+    DevBuf binpart;           // calcGammas partial sums
+    DevBuf scb;               // Scalars per bin
     double qmax = 0.0;        // max |Q| of the resident bunch
     bool have_bunch = false, have_track = false;
     bool track_sc = false;    // the tracker computes space-charge fields
@@ -230,7 +237,8 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->grid.destroy();
     free_bunch(c->bunch);
-    DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb, &c->statsb, &c->tmpd};
+    DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb,      &c->statsb,  &c->tmpd,
+                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb};
     for (DevBuf *b : bs) b->release();
     void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
     for (void *p : ps)
@@ -387,7 +395,8 @@ extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, co
     CHECK_LAUNCH(c);
     HIPCHK(c, hipMemcpyAsync(perm, b.vals2, (size_t)N * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->have_bunch = false;  // SoA buffers were reused
+    c->have_bunch = c->have_track = false;  // SoA buffers were reused
+    c->bins.nbins = 0;
     return OPALSC_OK;
 }
 
@@ -428,7 +437,8 @@ extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double
     HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(Q, b.q, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->have_bunch = false;
+    c->have_bunch = c->have_track = false;  // SoA buffers were reused
+    c->bins.nbins = 0;
     return OPALSC_OK;
 }
 
@@ -515,7 +525,8 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
     HIPCHK(c, hipMemcpyAsync(Ef, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->have_bunch = false;
+    c->have_bunch = c->have_track = false;  // SoA buffers were reused
+    c->bins.nbins = 0;
     return OPALSC_OK;
 }
 
@@ -619,6 +630,7 @@ extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const
     c->qmax = qm;
     c->have_bunch = true;
     c->have_track = false;
+    c->bins.nbins = 0;
     return OPALSC_OK;
 }
 
@@ -635,6 +647,7 @@ extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g
     c->qmax = std::fabs(qeach);
     c->have_bunch = true;
     c->have_track = false;
+    c->bins.nbins = 0;
     return OPALSC_OK;
 }
 
@@ -710,11 +723,16 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     return OPALSC_OK;
 }
 
-static void sort_bunch(opalsc_ctx *c) {
+static int bin_bits(int nbins) {  // bits of the values 0 .. nbins
+    int bits = 1;
+    while ((1 << bits) <= nbins) ++bits;
+    return bits;
+}
+
+// Reorders every particle array of the resident bunch by perm (b.vals2).
+static void permute_bunch(opalsc_ctx *c) {
     Bunch &b = c->bunch;
     const int64_t N = b.N;
-    launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, true, b.keys, b.vals);
-    sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, key_bits(c->cfg.n));
     // one gather pass per array (measured faster than gathering all arrays in one pass)
     double **arrs[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz, &b.q};
     for (double **a : arrs) {
@@ -723,6 +741,77 @@ static void sort_bunch(opalsc_ctx *c) {
     }
     launch_permute_u32(c->stream, N, b.vals2, b.id, b.keys);  // the sort's input keys are no longer needed
     std::swap(b.id, b.keys);
+}
+
+// Cell order (3D Morton) of the resident bunch; with energy bins, cell order
+// within each bin, so the bins keep their ranges.
+static void sort_bunch(opalsc_ctx *c) {
+    Bunch &b = c->bunch;
+    const int64_t N = b.N;
+    int bits = key_bits(c->cfg.n);
+    if (c->bins.nbins > 0) {
+        const int bb = bin_bits(c->bins.nbins);
+        const int drop = bits + bb > 32 ? bits + bb - 32 : 0;
+        launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, true, b.keys, b.vals, c->binsorted.as<int32_t>(),
+                         bits - drop, drop);
+        bits = bits - drop + bb;
+    } else {
+        launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, true, b.keys, b.vals);
+    }
+    sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, bits);
+    permute_bunch(c);
+}
+
+extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbins, double qi, int image) {
+    if (!c) return OPALSC_EINVAL;
+    if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "set_bins before bunch_set/bunch_generate");
+    if (nbins == 0) {
+        c->bins.nbins = 0;
+        return OPALSC_OK;
+    }
+    if (nbins < 0 || nbins > kMaxBins) FAIL(c, OPALSC_EINVAL, "nbins must be in [0, 64]");
+    Bunch &b = c->bunch;
+    const int64_t N = b.N;
+    if (N > 0 && !bin) FAIL(c, OPALSC_EINVAL, "null bin array");
+    if (!c->binsorted.ensure((size_t)(N > 0 ? N : 1) * 4) || !c->cntb.ensure(kMaxBins * 8) ||
+        !c->gamb.ensure(kMaxBins * 8) || !c->scb.ensure(sizeof(Scalars) * kMaxBins) ||
+        !c->binpart.ensure(sizeof(double) * kMaxBins * kBinGammaBlocks * kSlots))
+        FAIL(c, OPALSC_ENOMEM, "bin buffers");
+    unsigned long long cnt[kMaxBins] = {0};
+    if (N > 0) {
+        if (int e = upload(c, c->binb, bin, (size_t)N * 4)) return e;
+        // bin order, stable: particles keep their relative order within a bin
+        launch_bin_keys_by_id(c->stream, N, c->binb.as<int32_t>(), b.id, nbins, b.keys, b.vals);
+        if (sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N,
+                       bin_bits(nbins)))
+            FAIL(c, OPALSC_EHIP, "radix sort failed");
+        HIPCHK(c, hipMemcpyAsync(c->binsorted.p, b.keys2, (size_t)N * 4, hipMemcpyDeviceToDevice, c->stream));
+        permute_bunch(c);
+        HIPCHK(c, hipMemsetAsync(c->cntb.p, 0, kMaxBins * 8, c->stream));
+        launch_bin_count(c->stream, N, c->binsorted.as<int32_t>(), nbins, c->cntb.as<unsigned long long>());
+        CHECK_LAUNCH(c);
+        HIPCHK(c, hipMemcpyAsync(cnt, c->cntb.p, (size_t)nbins * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->bins.nbins = nbins;
+    c->bins.start[0] = 0;
+    for (int k = 0; k < nbins; ++k) c->bins.start[k + 1] = c->bins.start[k] + (int64_t)cnt[k];
+    c->bin_qi = qi;
+    c->bin_image = image != 0;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_bunch_get_bins(opalsc_ctx *c, int *nbins, int64_t *counts, double *gammas) {
+    if (!c) return OPALSC_EINVAL;
+    const int nb = c->bins.nbins;
+    if (nbins) *nbins = nb;
+    if (counts)
+        for (int k = 0; k < nb; ++k) counts[k] = c->bins.start[k + 1] - c->bins.start[k];
+    if (gammas && nb > 0) {
+        HIPCHK(c, hipMemcpyAsync(gammas, c->gamb.p, (size_t)nb * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return OPALSC_OK;
 }
 
 extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
@@ -734,6 +823,11 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     const int nb = gamma_blocks(c->grid.fp64, b.N);
     const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
+    const bool binned = c->bins.nbins > 0;
+    if (binned && c->track_sc) {
+        if (c->grid.ensure_sums(c->err)) return OPALSC_ENOMEM;
+        if (c->bin_image && c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
+    }
     c->prof.start();
     // Within one call the gather pass of step s also applies step s+1's first
     // half push, so the bounds pass of step s+1 only reads positions.  The
@@ -757,17 +851,48 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
                 sorted_now = true;  // the bounds pass's chunk boxes no longer match the particle order
                 c->prof.mark(PROF_SORT);
             }
-            launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
-            c->prof.mark(PROF_ZERO);
-            launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
-                           c->d_sc, &c->d_counters[1], sorted_now ? nullptr : b.chunkbox);
-            c->prof.mark(PROF_DEPOSIT);
-            c->grid.green(c->d_sc, cr);
-            c->grid.solve(c->d_sc, 0, 1);
-            c->grid.grad(c->d_sc);
+            if (binned) {
+                // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its image
+                launch_track_bin_scalars(c->stream, b, c->bins, c->bin_qi, exact, c->d_sc, c->binpart.as<double>(),
+                                         c->gamb.as<double>(), c->scb.as<Scalars>());
+                c->grid.clear_sums();
+                c->prof.mark(PROF_SCALAR);
+                for (int k = 0; k < c->bins.nbins; ++k) {
+                    const int64_t lo = c->bins.start[k], hi = c->bins.start[k + 1];
+                    if (hi == lo) continue;  // an empty bin has no charge
+                    const Scalars *sk = c->scb.as<Scalars>() + k;
+                    launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+                    c->prof.mark(PROF_ZERO);
+                    launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
+                                         (long long *)c->grid.rho_i64, sk, &c->d_counters[1],
+                                         sorted_now ? nullptr : b.chunkbox);
+                    c->prof.mark(PROF_DEPOSIT);
+                    c->grid.green(sk, cr);
+                    c->grid.solve(sk, 0, 1);
+                    c->grid.grad_accum(sk, -1.0);
+                    if (c->bin_image) {
+                        c->grid.green_image(sk, cr);
+                        c->grid.solve(sk, 0, 1, true);
+                        c->grid.grad_accum(sk, +1.0);
+                    }
+                }
+            } else {
+                launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+                c->prof.mark(PROF_ZERO);
+                launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
+                               c->d_sc, &c->d_counters[1], sorted_now ? nullptr : b.chunkbox);
+                c->prof.mark(PROF_DEPOSIT);
+                c->grid.green(c->d_sc, cr);
+                c->grid.solve(c->d_sc, 0, 1);
+                c->grid.grad(c->d_sc);
+            }
         }
         prepushed = s + 1 < nsteps;
-        launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials);
+        if (binned && c->track_sc)
+            launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
+                               c->grid.Bsum);
+        else
+            launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials);
         c->prof.mark(PROF_GATHER_PUSH);
         launch_track_post(c->stream, c->partials, nb, c->d_ts, c->d_sc,
                           (char *)c->statsb.p + sizeof(opalsc_stats) * s, &c->d_counters[1]);

@@ -97,15 +97,13 @@ void launch_bin_gamma(hipStream_t s, int64_t count, const uint32_t *perm, const 
     k_bin_gamma_final<<<1, kReduceThreads, 0, s>>>(partials, nb, count, b, gammas);
 }
 
-// The scalars of bin b (PartBunch.cpp:172-191,215,236,261-264): Lorentz factor,
-// density scale 1/h0*1/h1*1/h2/gamma, solver spacing (h0, h1, h2*gamma), field
-// scale (gamma, gamma, 1/gamma), betaC and the image shift; fixed-point scale
-// for count deposits of at most qdmax.  Mesh, n, dT and coupling are already set.
-__global__ void k_bin_scalars(Scalars *sc, const double *gammas, int b, int64_t count,
-                              const unsigned long long *qdmax_bits, int exact) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The scalars of one bin (PartBunch.cpp:172-191,215,236,261-264): Lorentz
+// factor, density scale 1/h0*1/h1*1/h2/gamma, solver spacing (h0, h1, h2*gamma),
+// field scale (gamma, gamma, 1/gamma), betaC and the image shift; fixed-point
+// scale for count deposits of at most qdmax.  Mesh, n, dT and coupling are
+// already set in sc.
+__device__ inline void bin_scalars(Scalars *sc, double gammaz, int64_t count, double qdmax, bool exact) {
     const double scaleFactor = 1.0;
-    const double gammaz = gammas[b];
     const double *hr = sc->hr;
     sc->gammaz = gammaz;
     sc->tmp2 = 1 / hr[0] * 1 / hr[1] * 1 / hr[2] / (scaleFactor * scaleFactor * scaleFactor) / gammaz;
@@ -120,7 +118,6 @@ __global__ void k_bin_scalars(Scalars *sc, const double *gammas, int b, int64_t 
     const double hz = hr[2];
     sc->zshift = -(2 * sc->origin[2] + (0 + (sc->n[2] - 1) + 1) * hz) * gammaz * scaleFactor;
     // fixed point: count deposits of |qd| <= qdmax (FP64: sum < 2^62; mixed: each < 2^24)
-    const double qdmax = __longlong_as_double((long long)*qdmax_bits);
     const double B = (exact ? (double)(count > 0 ? count : 1) : 1.0) * qdmax * (1.0 + 0x1p-40);
     int e = 0;
     if (B > 0.0) {
@@ -133,9 +130,88 @@ __global__ void k_bin_scalars(Scalars *sc, const double *gammas, int b, int64_t 
     sc->q2e = ldexp(1.0, -e);
     sc->qd_absmax = qdmax;
 }
+
+__global__ void k_bin_scalars(Scalars *sc, const double *gammas, int b, int64_t count,
+                              const unsigned long long *qdmax_bits, int exact) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    bin_scalars(sc, gammas[b], count, __longlong_as_double((long long)*qdmax_bits), exact != 0);
+}
 void launch_bin_scalars(hipStream_t s, Scalars *sc, const double *gammas, int b, int64_t count,
                         const unsigned long long *qdmax_bits, bool exact) {
     k_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, b, count, qdmax_bits, exact ? 1 : 0);
+}
+
+// ---- tracker with energy bins: the bunch is ordered by bin, bin b holding
+// particles start[b] <= i < start[b+1]
+
+// calcGammas over the bin ranges: block (x, b) sums sqrt(1 + P.P) over a
+// grid-stride share of bin b (double-double), then one block per bin
+// reduces the kBinGammaBlocks partials to the mean (1 for an empty bin).
+__global__ __launch_bounds__(256) void k_track_bin_gamma_partials(const double *px, const double *py,
+                                                                  const double *pz, BinRanges r,
+                                                                  double *partials) {
+    __shared__ double lds[4 * 2];
+    const int b = blockIdx.y;
+    double h[1] = {0.0}, l[1] = {0.0};
+    for (int64_t i = r.start[b] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < r.start[b + 1];
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double p0 = px[i], p1 = py[i], p2 = pz[i];
+        const dd v = dd_add(dd{h[0], l[0]}, dd{sqrt(1.0 + (p0 * p0 + p1 * p1 + p2 * p2)), 0.0});
+        h[0] = v.hi;
+        l[0] = v.lo;
+    }
+    block_sum_dd<1>(h, l, lds);
+    if (threadIdx.x == 0) {
+        double *o = partials + ((size_t)b * gridDim.x + blockIdx.x) * kSlots;
+        o[0] = h[0];
+        o[1] = l[0];
+    }
+}
+__global__ __launch_bounds__(kReduceThreads) void k_track_bin_gamma_final(const double *partials, BinRanges r,
+                                                                          double *gammas) {
+    __shared__ double lds[16 * 2];
+    const int b = blockIdx.x;
+    double out[1];
+    reduce_partials_dd<1>(partials + (size_t)b * kBinGammaBlocks * kSlots, kBinGammaBlocks, out, lds);
+    const int64_t count = r.start[b + 1] - r.start[b];
+    if (threadIdx.x == 0) gammas[b] = count != 0 ? out[0] / (double)count : 1.0;
+}
+
+// Per-bin scalars of the step: copies of the step's scalars (mesh, frames)
+// with bin b's Lorentz-frame values and charge qi.
+__global__ void k_track_bin_scalars(const Scalars *sc, const double *gammas, BinRanges r, double qi,
+                                    int exact, Scalars *scb) {
+    const int b = threadIdx.x;
+    if (b >= r.nbins) return;
+    Scalars s = *sc;
+    s.qi = qi;
+    // every deposit is |qi*dT| <= |qi|*dT*(1+2^-52)
+    bin_scalars(&s, gammas[b], r.start[b + 1] - r.start[b], fabs(qi) * s.dT * (1.0 + 0x1p-50), exact != 0);
+    scb[b] = s;
+}
+
+void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges &r, double qi, bool exact,
+                              const Scalars *sc, double *partials, double *gammas, Scalars *scb) {
+    if (r.nbins <= 0) return;
+    k_track_bin_gamma_partials<<<dim3(kBinGammaBlocks, r.nbins), kThreads, 0, s>>>(bunch.px, bunch.py, bunch.pz,
+                                                                                  r, partials);
+    k_track_bin_gamma_final<<<r.nbins, kReduceThreads, 0, s>>>(partials, r, gammas);
+    k_track_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, r, qi, exact ? 1 : 0, scb);
+}
+
+// bin keys of the resident bunch for its first bin-major order: the bin of
+// particle id[i] (bins outside [0, nbins) become nbins and sort last)
+__global__ void k_bin_keys_by_id(int64_t N, const int32_t *bin, const uint32_t *id, int nbins, uint32_t *keys,
+                                 uint32_t *vals) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = bin[id[i]];
+        keys[i] = (b >= 0 && b < nbins) ? (uint32_t)b : (uint32_t)nbins;
+        vals[i] = (uint32_t)i;
+    }
+}
+void launch_bin_keys_by_id(hipStream_t s, int64_t N, const int32_t *bin, const uint32_t *id, int nbins,
+                           uint32_t *keys, uint32_t *vals) {
+    if (N > 0) k_bin_keys_by_id<<<stream_blocks(N), kThreads, 0, s>>>(N, bin, id, nbins, keys, vals);
 }
 
 // Ef = gather(sum over bins of E), Bf += gather(sum over bins of the implied B)

@@ -44,6 +44,7 @@ struct Scalars {
     double M_u[9], o_u[3];     // reference-frame update (transformBunch)
     double Eext[3], Bext[3];   // external field for the bunch at t + dT/2
     double zshift;             // image-charge Green's function shift (FFTPoissonSolver.cpp:289)
+    double qi;                 // energy bins: macro charge of the bin's particles (setBinCharge)
     int mesh_ok;
     int64_t N;
 };

@@ -90,6 +90,7 @@ struct PoissonGrid {
     }
 };
 
+constexpr int kSlots = 64;           // doubles per block in a partial-sums buffer
 constexpr int kDepositChunk = 2048;  // particles per deposit / bounds chunk
 constexpr int kBoxStats = 12;        // per chunk: min[3], max[3], sum[3], sum of squares[3]
 
@@ -121,6 +122,11 @@ void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int
 void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double *x, const double *y,
                     const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
                     unsigned long long *dropped, const double *chunkbox);
+// deposit of particles lo <= p < hi of an Nall-particle bunch (chunkbox indexed over the bunch)
+void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64_t hi, int64_t Nall,
+                          const double *x, const double *y, const double *z, double *q, const double *qd,
+                          long long *rho, const Scalars *sc, unsigned long long *dropped,
+                          const double *chunkbox);
 void launch_gather_selffield(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y,
                              const double *z, const void *Eg, const Scalars *sc, double *Ef,
                              double *Bf);
@@ -136,7 +142,8 @@ void launch_moments(hipStream_t s, int64_t N, const double *R, const double *P, 
 void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, int64_t N,
                                 double mass_eV, void *stats_out);
 void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,
-                      const Scalars *sc, bool rotate, uint32_t *keys, uint32_t *vals);
+                      const Scalars *sc, bool rotate, uint32_t *keys, uint32_t *vals,
+                      const int32_t *bin = nullptr, int bshift = 0, int drop = 0);
 void launch_permute(hipStream_t s, int64_t N, const uint32_t *perm, const double *src, double *dst);
 void launch_permute_u32(hipStream_t s, int64_t N, const uint32_t *perm, const uint32_t *src, uint32_t *dst);
 void launch_generate(hipStream_t s, int64_t N, uint64_t seed, const double sr[3], const double sp[3],
@@ -155,8 +162,9 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, bool exact, Scalars *sc);
 int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
+// Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered)
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials);
+                        const TrackState *ts, double *partials, const void *Bg = nullptr);
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped);
 void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,
@@ -176,6 +184,17 @@ void launch_bin_scalars(hipStream_t s, Scalars *sc, const double *gammas, int b,
                         const unsigned long long *qdmax_bits, bool exact);
 void launch_gather_binned(hipStream_t s, bool fp64, int64_t N, const double *x, const double *y, const double *z,
                           const void *Es, const void *Bs, const Scalars *sc, double *Ef, double *Bf);
+// tracker: bin b holds the particles start[b] <= i < start[b+1] of the bin-ordered bunch
+struct BinRanges {
+    int nbins = 0;
+    int64_t start[kMaxBins + 1] = {0};
+};
+constexpr int kBinGammaBlocks = 64;  // partial-sum blocks per bin for calcGammas
+// per-bin calcGammas and the per-bin scalars scb[b] (partials: kMaxBins*kBinGammaBlocks*kSlots doubles)
+void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges &r, double qi, bool exact,
+                              const Scalars *sc, double *partials, double *gammas, Scalars *scb);
+void launch_bin_keys_by_id(hipStream_t s, int64_t N, const int32_t *bin, const uint32_t *id, int nbins,
+                           uint32_t *keys, uint32_t *vals);
 
 // Sorting (sort.hip)
 size_t sort_temp_bytes(int64_t N);

@@ -183,12 +183,16 @@ constexpr int kBoxCap = 5632;  // 44 KiB of int64
 
 // MODE 0: qd given; 1 (tracker): qd = Q*dT with Q left as (Q*dT)/dT, positions
 // given in the reference frame and rotated into the beam frame (M_rb, o_rb)
-// exactly as the bounds pass rotated them.  EXACT: the FP64
-// path (reference-order CIC, 64-bit fixed point per deposit); otherwise the
-// mixed path (CIC with 1/h, FP32 weights, |deposit| < 2^24 in 32-bit integers,
-// summed in 64 bits).  BOX: chunk statistics supplied by the bounds pass.
+// exactly as the bounds pass rotated them; 2 (tracker, energy bin): as 1 with
+// qd = qi*dT for every particle (setBinCharge) and Q untouched.  EXACT: the
+// FP64 path (reference-order CIC, 64-bit fixed point per deposit); otherwise
+// the mixed path (CIC with 1/h, FP32 weights, |deposit| < 2^24 in 32-bit
+// integers, summed in 64 bits).  BOX: chunk statistics supplied by the bounds
+// pass.  Only particles lo <= p < N deposit; block b owns chunk chunk0 + b of
+// the Nall-particle bunch.
 template <int MODE, bool EXACT, bool BOX>
-__global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__restrict__ x,
+__global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int64_t Nall, int64_t chunk0,
+                                                     const double *__restrict__ x,
                                                      const double *__restrict__ y,
                                                      const double *__restrict__ z, double *q,
                                                      const double *qdv, long long *rho,
@@ -202,39 +206,49 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double e2q = sc->e2q, dT = sc->dT;
-    const int64_t base = (int64_t)blockIdx.x * kChunk;
+    const int64_t chunk = chunk0 + blockIdx.x;
+    const int64_t base = chunk * kChunk;
 
     // 1. the chunk's particles, all loads issued before any is used; thread t
     // owns the kPPT consecutive particles p0 = base + t*kPPT ... p0 + kPPT-1
     const int64_t p0 = base + (int64_t)threadIdx.x * kPPT;
     const double *qsrc = MODE == 0 ? qdv : q;
     double px[kPPT], py[kPPT], pz[kPPT], pq[kPPT];
-    if (p0 + kPPT <= N) {
+    if (MODE == 2) {
+        double qb = sc->qi;  // Q = qi (setBinCharge), Q *= dt (PartBunch.cpp:154)
+        qb *= dT;
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k) pq[k] = qb;
+    }
+    if (p0 >= lo && p0 + kPPT <= N) {
 #pragma unroll
         for (int m = 0; m < kPPT / 2; ++m) {
             const double2 tx = ((const double2 *)(x + p0))[m], ty = ((const double2 *)(y + p0))[m];
-            const double2 tz = ((const double2 *)(z + p0))[m], tq = ((const double2 *)(qsrc + p0))[m];
+            const double2 tz = ((const double2 *)(z + p0))[m];
             px[2 * m] = tx.x;
             px[2 * m + 1] = tx.y;
             py[2 * m] = ty.x;
             py[2 * m + 1] = ty.y;
             pz[2 * m] = tz.x;
             pz[2 * m + 1] = tz.y;
-            pq[2 * m] = tq.x;
-            pq[2 * m + 1] = tq.y;
+            if constexpr (MODE != 2) {
+                const double2 tq = ((const double2 *)(qsrc + p0))[m];
+                pq[2 * m] = tq.x;
+                pq[2 * m + 1] = tq.y;
+            }
         }
     } else {
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) {
             const int64_t p = p0 + k;
-            const bool in = p < N;
+            const bool in = p >= lo && p < N;
             px[k] = in ? x[p] : 0.0;
             py[k] = in ? y[p] : 0.0;
             pz[k] = in ? z[p] : 0.0;
-            pq[k] = in ? qsrc[p] : 0.0;
+            if constexpr (MODE != 2) pq[k] = in ? qsrc[p] : 0.0;
         }
     }
-    if constexpr (MODE == 1) {
+    if constexpr (MODE != 0) {
         // the tracker keeps positions in the reference frame; the mesh is in the beam frame
         double M[9], o[3];
         for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
@@ -255,7 +269,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     // the particles outside it going to HBM directly
     double st[kBoxStats];
     if constexpr (BOX) {
-        for (int k = 0; k < kBoxStats; ++k) st[k] = chunkbox[blockIdx.x * kBoxStats + k];
+        for (int k = 0; k < kBoxStats; ++k) st[k] = chunkbox[chunk * kBoxStats + k];
     } else {
         for (int d = 0; d < 3; ++d) {
             st[d] = __builtin_inf();
@@ -264,7 +278,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
         }
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) {
-            if (p0 + k < N) {
+            if (p0 + k >= lo && p0 + k < N) {
                 const double r[3] = {px[k], py[k], pz[k]};
                 for (int d = 0; d < 3; ++d) {
                     st[d] = fmin(st[d], r[d]);
@@ -279,7 +293,10 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     if (threadIdx.x == 0) {
         const double o[3] = {o0, o1, o2}, h[3] = {h0, h1, h2};
         const int n[3] = {n0, n1, n2};
-        const double cnt = (double)(N - base < kChunk ? N - base : kChunk);
+        // particles the statistics cover: the whole chunk (bounds pass) or its part in [lo, N)
+        const int64_t c0 = BOX ? base : (lo > base ? lo : base);
+        const int64_t c1 = BOX ? (Nall < base + kChunk ? Nall : base + kChunk) : (N < base + kChunk ? N : base + kChunk);
+        const double cnt = (double)(c1 - c0);
         // cells [lo, hi] whose corners particles in [a, b] can reach: (x-o)/h is
         // monotonic in x; lgp is in [floor(t)-1, floor(t)] for t >= 0 and
         // >= floor(t)-2 below the origin; corners reach lgp+1
@@ -380,9 +397,9 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
         int key = -1 - k;  // unique within the thread: never merges
         int l[3] = {0, 0, 0};
         bool valid = false;
-        if (p < N) {
+        if (p >= lo && p < N) {
             double qd;
-            if constexpr (MODE == 0) {
+            if constexpr (MODE != 1) {
                 qd = pq[k];
             } else {
                 const double qv = pq[k];
@@ -470,25 +487,43 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t N, const double *__
     if (nd) atomicAdd(dropped, nd);
 }
 
-void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double *x, const double *y,
-                    const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
-                    unsigned long long *dropped, const double *chunkbox) {
-    if (N <= 0) return;
-    const unsigned blocks = (unsigned)((N + kChunk - 1) / kChunk);
+void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64_t hi, int64_t Nall,
+                          const double *x, const double *y, const double *z, double *q, const double *qd,
+                          long long *rho, const Scalars *sc, unsigned long long *dropped,
+                          const double *chunkbox) {
+    if (hi <= lo) return;
+    const int64_t chunk0 = lo / kChunk;
+    const unsigned blocks = (unsigned)((hi - 1) / kChunk - chunk0 + 1);
     const bool box = chunkbox != nullptr;
-#define OPALSC_DEP(M_, E_, B_) \
-    k_deposit_lds<M_, E_, B_><<<blocks, kThreads, 0, s>>>(N, x, y, z, q, qd, rho, sc, dropped, chunkbox)
+#define OPALSC_DEP(M_, E_, B_)                                                                               \
+    k_deposit_lds<M_, E_, B_><<<blocks, kThreads, 0, s>>>(lo, hi, Nall, chunk0, x, y, z, q, qd, rho, sc, \
+                                                          dropped, chunkbox)
     if (mode == 0) {
         if (exact) OPALSC_DEP(0, true, false);
         else OPALSC_DEP(0, false, false);
-    } else if (box) {
-        if (exact) OPALSC_DEP(1, true, true);
-        else OPALSC_DEP(1, false, true);
+    } else if (mode == 1) {
+        if (box) {
+            if (exact) OPALSC_DEP(1, true, true);
+            else OPALSC_DEP(1, false, true);
+        } else {
+            if (exact) OPALSC_DEP(1, true, false);
+            else OPALSC_DEP(1, false, false);
+        }
     } else {
-        if (exact) OPALSC_DEP(1, true, false);
-        else OPALSC_DEP(1, false, false);
+        if (box) {
+            if (exact) OPALSC_DEP(2, true, true);
+            else OPALSC_DEP(2, false, true);
+        } else {
+            if (exact) OPALSC_DEP(2, true, false);
+            else OPALSC_DEP(2, false, false);
+        }
     }
 #undef OPALSC_DEP
+}
+void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double *x, const double *y,
+                    const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
+                    unsigned long long *dropped, const double *chunkbox) {
+    launch_deposit_range(s, mode, exact, 0, N, N, x, y, z, q, qd, rho, sc, dropped, chunkbox);
 }
 
 // ---- CIC gather ---------------------------------------------------------------
@@ -693,8 +728,10 @@ void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, i
 // ---- sorting support ------------------------------------------------------------
 // rotate: positions are in the reference frame; the cell is that of the
 // beam-frame position M_rb (r - o_rb) the deposit uses.
+// With energy bins (bin != nullptr) the key is bin-major: (bin << bshift) | (cell key >> drop).
 __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const double *z,
-                            const Scalars *sc, int rotate, uint32_t *keys, uint32_t *vals) {
+                            const Scalars *sc, int rotate, const int32_t *bin, int bshift, int drop,
+                            uint32_t *keys, uint32_t *vals) {
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
@@ -713,13 +750,17 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
         uint32_t key = out;
         if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
             key = cell_key(l[0], l[1], l[2], n0, n1);
+        if (bin) key = ((uint32_t)bin[p] << bshift) | (key >> drop);
         keys[p] = key;
         vals[p] = (uint32_t)p;
     }
 }
 void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,
-                      const Scalars *sc, bool rotate, uint32_t *keys, uint32_t *vals) {
-    if (N > 0) k_cell_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, sc, rotate ? 1 : 0, keys, vals);
+                      const Scalars *sc, bool rotate, uint32_t *keys, uint32_t *vals, const int32_t *bin,
+                      int bshift, int drop) {
+    if (N > 0)
+        k_cell_keys<<<stream_blocks(N), kThreads, 0, s>>>(N, x, y, z, sc, rotate ? 1 : 0, bin, bshift, drop, keys,
+                                                          vals);
 }
 
 // dst[i] = src[perm[i]]: each thread gathers kPermU elements with all of its
@@ -1018,14 +1059,15 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
 // (timeIntegration2), transformBunch with the reference-frame update, and the
 // moment sums of the step's statistics.  NEXT: also the next step's first half
 // push (timeIntegration1 with the updated P), so that pass reads positions only.
-// Four waves per SIMD (at most 128 VGPRs) keep enough loads in flight.
-template <typename T, bool NEXT>
+// BINNED (energy bins): Eg and Bg hold the sums over the bins of E and of the
+// magnetic field each bin and its image imply (ParallelTTracker.cpp:567-579),
+// both gathered.  Four waves per SIMD (at most 128 VGPRs) keep enough loads in
+// flight.
+template <typename T, bool NEXT, bool BINNED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma(
-    int64_t N, double *x, double *y, double *z,
-                                                     double *px, double *py, double *pz,
-                                                     const typename vec4_of<T>::type *Eg,
-                                                     const Scalars *sc, const TrackState *ts,
-                                                     double *partials) {
+    int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz,
+    const typename vec4_of<T>::type *Eg, const typename vec4_of<T>::type *Bg, const Scalars *sc,
+    const TrackState *ts, double *partials) {
     __shared__ double lds[4 * kMom * 2];
     const bool sc_on = ts->space_charge != 0;
     const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
@@ -1073,15 +1115,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
                 ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
                 ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
-                if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, Ef);
+                if (ok) {
+                    gather_E<T>(Eg, l, d, n0, n1, n2, Ef);
+                    if constexpr (BINNED) gather_E<T>(Bg, l, d, n0, n1, n2, Bf);
+                }
             } else {
                 bool ok = cic_dim_fast(r[0], o0, ih0, l[0], d[0]);
                 ok &= cic_dim_fast(r[1], o1, ih1, l[1], d[1]);
                 ok &= cic_dim_fast(r[2], o2, ih2, l[2], d[2]);
-                if (ok) gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
+                if (ok) {
+                    gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
+                    if constexpr (BINNED) gather_E_fast(Bg, l, d, n0, n1, n2, Bf);
+                }
             }
-            Bf[0] = Bf[0] - betaC * Ef[1];
-            Bf[1] = Bf[1] + betaC * Ef[0];
+            if constexpr (!BINNED) {
+                Bf[0] = Bf[0] - betaC * Ef[1];
+                Bf[1] = Bf[1] + betaC * Ef[0];
+            }
             double t3[3];
             if constexpr (sizeof(T) == 8) {
                 transform_to(Mb, ob, r);
@@ -1149,25 +1199,30 @@ static int resident_blocks(K kernel) {
 
 // One wave of resident blocks (fewer partial sums for the step statistics).
 int gamma_blocks(bool fp64, int64_t N) {
-    const int r = fp64 ? resident_blocks(k_track_gamma<double, true>) : resident_blocks(k_track_gamma<float, true>);
+    const int r = fp64 ? resident_blocks(k_track_gamma<double, true, false>)
+                       : resident_blocks(k_track_gamma<float, true, false>);
     const int b = red_blocks(N);
     return b < r ? b : r;
 }
 
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials) {
+                        const TrackState *ts, double *partials, const void *Bg) {
     if (b.N <= 0) return;
     const int blocks = gamma_blocks(fp64, b.N);
-#define OPALSC_GAMMA(T_, V_, N_)                                                                          \
-    k_track_gamma<T_, N_><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz,               \
-                                                      (const V_ *)Eg, sc, ts, partials)
+#define OPALSC_GAMMA(T_, V_, N_, B_)                                                                         \
+    k_track_gamma<T_, N_, B_><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const V_ *)Eg, \
+                                                          (const V_ *)Bg, sc, ts, partials)
+#define OPALSC_GAMMA2(T_, V_, N_)            \
+    if (Bg) OPALSC_GAMMA(T_, V_, N_, true); \
+    else OPALSC_GAMMA(T_, V_, N_, false)
     if (fp64) {
-        if (next) OPALSC_GAMMA(double, double4, true);
-        else OPALSC_GAMMA(double, double4, false);
+        if (next) OPALSC_GAMMA2(double, double4, true);
+        else OPALSC_GAMMA2(double, double4, false);
     } else {
-        if (next) OPALSC_GAMMA(float, float4, true);
-        else OPALSC_GAMMA(float, float4, false);
+        if (next) OPALSC_GAMMA2(float, float4, true);
+        else OPALSC_GAMMA2(float, float4, false);
     }
+#undef OPALSC_GAMMA2
 #undef OPALSC_GAMMA
 }
 

@@ -11,7 +11,6 @@ namespace opalsc {
 using crm::dd;
 using crm::dd_add;
 
-constexpr int kSlots = 64;           // doubles per block in the partials buffer
 constexpr int kReduceThreads = 512;   // single-block final reductions of the partials
 
 inline int stream_blocks(int64_t N) {

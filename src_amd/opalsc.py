@@ -37,7 +37,7 @@ EXPORTS = [
     "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_compute_self_fields_binned",
     "opalsc_get_grid", "opalsc_push_half",
     "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
-    "opalsc_bunch_get", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
+    "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
     "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
     "opalsc_copy_bandwidth",
@@ -111,6 +111,8 @@ def lib():
         "opalsc_bunch_set": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_bunch_generate": ([vp, C.POINTER(GaussParams)], C.c_int),
         "opalsc_bunch_get": ([vp, dp, dp, dp], C.c_int),
+        "opalsc_bunch_set_bins": ([vp, dp, C.c_int, C.c_double, C.c_int], C.c_int),
+        "opalsc_bunch_get_bins": ([vp, C.POINTER(C.c_int), dp, dp], C.c_int),
         "opalsc_track_init": ([vp, C.POINTER(TrackParams)], C.c_int),
         "opalsc_track": ([vp, C.c_int, dp], C.c_int),
         "opalsc_track_get_params": ([vp, C.POINTER(TrackParams)], C.c_int),
@@ -332,6 +334,24 @@ class SpaceChargeContext:
         Q = np.zeros(self.N)
         self._c(lib().opalsc_bunch_get(self._h, _ptr(R), _ptr(P), _ptr(Q)))
         return R, P, Q
+
+    def set_bins(self, bin, nbins, qi, image=True):
+        """Energy bins of the resident bunch (bin[i] for the i-th particle given
+        to set_bunch); the tracker then solves each bin in its own frame and,
+        with image, its image charge.  nbins=0 removes the bins."""
+        bin = np.ascontiguousarray(bin if bin is not None else np.zeros(0), dtype=np.int32)
+        if nbins and bin.shape != (self.N,):
+            raise ValueError("bin must hold one entry per particle")
+        self._c(lib().opalsc_bunch_set_bins(self._h, _ptr(bin) if nbins else None, int(nbins), float(qi),
+                                            1 if image else 0))
+
+    def get_bins(self):
+        """(counts, gammas) of the energy bins; gammas are those of the most recent step."""
+        nb = C.c_int(0)
+        counts = np.zeros(64, np.int64)
+        gammas = np.zeros(64)
+        self._c(lib().opalsc_bunch_get_bins(self._h, C.byref(nb), _ptr(counts), _ptr(gammas)))
+        return counts[:nb.value], gammas[:nb.value]
 
     def track_init(self, dT, charge=1.0, mass_eV=M_P_EV, coupling=COUPLING, Bz=0.0, Ez0=0.0,
                    rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), space_charge=True):

@@ -351,3 +351,63 @@ def test_compute_self_fields_binned_rejects_bad_nbins():
         for nb in (0, 65):
             with pytest.raises(sc.OpalscError):
                 c.computeSelfFieldsBinned(R, P, np.zeros(100, np.int32), nb, 1e-12, dt, 1e-11, org, hr)
+
+
+def geometric_energy_bins(P, nbins):
+    """Bins by longitudinal momentum rank with populations in the ratio 2:1
+    from one bin to the next (bin 0 the largest, holding the lowest pz)."""
+    N = len(P)
+    w = 0.5 ** np.arange(nbins)
+    edges = np.floor(np.cumsum(w) / w.sum() * N).astype(np.int64)
+    edges[-1] = N
+    rank = np.empty(N, np.int64)
+    rank[np.argsort(P[:, 2], kind="stable")] = np.arange(N)
+    return np.searchsorted(edges, rank, side="right").astype(np.int32)
+
+
+def _binned_track_case(n, N, nbins, steps, precision=sc.FP64, sort_interval=0, image=True, nobin=0.0, seed=31):
+    R, P, Q = gaussian_bunch(N, seed=seed, q_total=1e-9, sigma_p=(1e-3, 1e-3, 5e-3))
+    R[:, 2] += 0.005  # 5 sigma downstream of the cathode plane
+    bins = geometric_energy_bins(P, nbins)
+    if nobin:
+        bins[np.random.default_rng(seed).random(N) < nobin] = -1
+    qi = 1e-9 / N
+    dt = np.full(N, 1e-11)
+    kw = dict(Bz=1.0, Ez0=1e6, rf_freq=50e6, ref_p=(0.0, 0.0, 0.046181))
+    tr = O.make_tracker(n, 1e-11, **kw)
+    ref = O.track_binned(tr, R, P, Q, dt, bins, nbins, qi, steps, image=image)
+    with ctx(n, precision=precision, N=N, sort_interval=sort_interval) as c:
+        c.set_bunch(R, P, Q)
+        c.set_bins(bins, nbins, qi, image=image)
+        counts, _ = c.get_bins()
+        c.track_init(1e-11, **kw)
+        sg = c.track(steps)
+        Rg, Pg, Qg = c.get_bunch()
+        _, gammas = c.get_bins()
+    assert np.array_equal(counts, np.bincount(bins[bins >= 0], minlength=nbins))
+    assert np.array_equal(Qg, Q)  # the bin loop restores Q
+    return R, P, ref, (Rg, Pg, sg), gammas
+
+
+@pytest.mark.parametrize("sort_interval", [0, 2])
+def test_tracker_binned_matches_oracle(sort_interval):
+    """Energy bins + image charge + solenoid + RF field, FP64 path."""
+    R, P, ref, gpu, gammas = _binned_track_case((16, 16, 32), 60000, 6, 3, sort_interval=sort_interval,
+                                                nobin=0.02)
+    Ro, Po, _, _, so = ref
+    Rg, Pg, sg = gpu
+    assert np.all(gammas > 1.0)
+    assert rel_l2(Rg, Ro) < 1e-12
+    assert rel_l2(Pg - P, Po - P) < 1e-9
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm", "prms"):
+            assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])
+
+
+def test_tracker_binned_without_image_and_mixed():
+    R, P, ref, gpu, _ = _binned_track_case((32, 32, 32), 80000, 10, 3, precision=sc.MIXED, image=False)
+    Ro, Po, _, _, so = ref
+    Rg, Pg, sg = gpu
+    assert rel_l2(Pg - P, Po - P) < 1e-5
+    for a, b in zip(sg, so):
+        assert np.allclose(a["eps_norm"], b["eps_norm"], rtol=1e-6, atol=0)
