@@ -45,6 +45,12 @@ CONFIGS = {
     "c1": dict(N=100_000, n=32, ekin=1.0, q=1e-9, halo=0.0, Bz=0.0, seed=1),
 }
 
+# energy-binned configuration: 5e7 particles in 10 bins of populations in the
+# ratio 2:1 from bin to bin, 128^3 grid per bin with its image charge, solenoid
+# plus RF cavity field, mixed precision
+C5 = dict(N=50_000_000, n=128, ekin=1.0, q=10e-9, nbins=10, sigma_p=5e-3, Bz=1.0, Ez0=1e6, rf=50e6,
+          seed=5)
+
 
 def make_ctx(sc, cfg, precision, sort_interval):
     n = cfg["n"]
@@ -68,6 +74,55 @@ def timed_run(c, steps, warmup):
     lib().opalsc_synchronize(c._h)
     t1 = time.perf_counter()
     return (t1 - t0), stats
+
+
+def geometric_bins_by_pz(pz, p0, sigma, nbins):
+    """Energy bins from the longitudinal momentum: thresholds at the Gaussian
+    quantiles that give bin k the fraction 2^-k / sum_j 2^-j (lowest pz in bin 0)."""
+    from statistics import NormalDist
+    w = 0.5 ** np.arange(nbins)
+    frac = np.cumsum(w)[:-1] / w.sum()
+    edges = np.array([p0 + sigma * NormalDist().inv_cdf(float(f)) for f in frac])
+    return np.searchsorted(edges, pz, side="right").astype(np.int32)
+
+
+def run_c5(sc, steps, warmup, sort_interval, log):
+    """The energy-binned step (one Poisson solve per bin and one for its image)."""
+    from src_amd.opalsc import lib, _ptr
+    cfg = C5
+    N, n = cfg["N"], cfg["n"]
+    c = sc.SpaceChargeContext((n, n, n), precision=sc.MIXED, max_particles=N, sort_interval=sort_interval)
+    g, bg = gamma_betagamma(cfg["ekin"])
+    sp = cfg["sigma_p"]
+    c.generate_bunch(N, cfg["seed"], (1e-3, 1e-3, 1e-3), (sp, sp, sp), bg, cfg["q"])
+    P = np.zeros((N, 3))
+    c._c(lib().opalsc_bunch_get(c._h, None, _ptr(P), None))
+    bins = geometric_bins_by_pz(P[:, 2], bg, sp, cfg["nbins"])
+    del P
+    c.set_bins(bins, cfg["nbins"], cfg["q"] / N, image=True)
+    counts, _ = c.get_bins()
+    del bins
+    c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, Bz=cfg["Bz"], Ez0=cfg["Ez0"], rf_freq=cfg["rf"],
+                 ref_p=(0.0, 0.0, bg))
+    t, st = timed_run(c, steps, warmup)
+    ms = 1e3 * t / steps
+    c.profile_enable(True)
+    c.track(max(sort_interval, 3))
+    prof = c.profile_read()
+    c.profile_enable(False)
+    _, gammas = c.get_bins()
+    c.close()
+    log("[bench] c5 binned: %.3f ms/step" % ms)
+    return {"workload": "5e7 macroparticles in %d energy bins (populations %s), %d^3 grid per bin "
+                        "(%d^3 doubled FFT) plus its image charge, %g MeV protons, sigma_p %g, Bz %g T, "
+                        "Ez %g V/m at %g MHz, mixed precision, sort every %d steps" %
+                        (cfg["nbins"], "/".join(str(int(x)) for x in counts), n, 2 * n, cfg["ekin"], sp,
+                         cfg["Bz"], cfg["Ez0"], cfg["rf"] / 1e6, sort_interval),
+            "ms_per_step": ms, "particle_steps_per_s": N * steps / t, "steps": steps,
+            "poisson_solves_per_step": 2 * int((counts > 0).sum()),
+            "bin_gammas": [float(x) for x in gammas],
+            "emittance_last": [float(x) for x in st[-1]["eps_norm"]],
+            "kernels": {k: {"avg_ms": ms_ / nl, "launches": nl} for k, (ms_, nl) in prof.items() if nl}}
 
 
 def algorithmic_bytes(cfg, wbytes):
@@ -164,6 +219,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="skip the energy-binned measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the space-charge step of one "
@@ -240,6 +296,10 @@ def main():
         cd.close()
         cm2.close()
     cm.close()
+
+    if not args.no_c5:
+        log("[bench] energy-binned step (5e7 particles, 10 bins, 128^3 + image per bin)")
+        extra["c5_binned"] = run_c5(sc, args.steps, args.warmup, args.sort_interval, log)
 
     cpu = None
     if not args.no_cpu:

@@ -215,7 +215,7 @@ extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
     if (alloc_bunch(c, cfg->max_particles) != 0) return bail(OPALSC_ENOMEM);
     if (hipMalloc((void **)&c->d_sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc((void **)&c->d_ts, sizeof(TrackState)) != hipSuccess ||
-        hipMalloc((void **)&c->partials, sizeof(double) * kRedBlocksMax * 64) != hipSuccess ||
+        hipMalloc((void **)&c->partials, sizeof(double) * kSlots * kRedBlocksMax) != hipSuccess ||
         hipMalloc((void **)&c->d_counters, sizeof(unsigned long long) * 4) != hipSuccess) {
         c->err = "hipMalloc of scalar buffers failed";
         return bail(OPALSC_ENOMEM);
@@ -775,7 +775,7 @@ extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbin
     if (N > 0 && !bin) FAIL(c, OPALSC_EINVAL, "null bin array");
     if (!c->binsorted.ensure((size_t)(N > 0 ? N : 1) * 4) || !c->cntb.ensure(kMaxBins * 8) ||
         !c->gamb.ensure(kMaxBins * 8) || !c->scb.ensure(sizeof(Scalars) * kMaxBins) ||
-        !c->binpart.ensure(sizeof(double) * kMaxBins * kBinGammaBlocks * kSlots))
+        !c->binpart.ensure(sizeof(double) * kMaxBins * kBinGammaBlocks * 2))
         FAIL(c, OPALSC_ENOMEM, "bin buffers");
     unsigned long long cnt[kMaxBins] = {0};
     if (N > 0) {

@@ -145,8 +145,9 @@ void launch_bin_scalars(hipStream_t s, Scalars *sc, const double *gammas, int b,
 // particles start[b] <= i < start[b+1]
 
 // calcGammas over the bin ranges: block (x, b) sums sqrt(1 + P.P) over a
-// grid-stride share of bin b (double-double), then one block per bin
-// reduces the kBinGammaBlocks partials to the mean (1 for an empty bin).
+// grid-stride share of bin b (double-double); gridDim.x is sized for the most
+// populated bin.  Then one block per bin reduces its gridDim.x partials to the
+// mean (1 for an empty bin).
 __global__ __launch_bounds__(256) void k_track_bin_gamma_partials(const double *px, const double *py,
                                                                   const double *pz, BinRanges r,
                                                                   double *partials) {
@@ -162,19 +163,25 @@ __global__ __launch_bounds__(256) void k_track_bin_gamma_partials(const double *
     }
     block_sum_dd<1>(h, l, lds);
     if (threadIdx.x == 0) {
-        double *o = partials + ((size_t)b * gridDim.x + blockIdx.x) * kSlots;
+        double *o = partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
         o[0] = h[0];
         o[1] = l[0];
     }
 }
-__global__ __launch_bounds__(kReduceThreads) void k_track_bin_gamma_final(const double *partials, BinRanges r,
-                                                                          double *gammas) {
+__global__ __launch_bounds__(kReduceThreads) void k_track_bin_gamma_final(const double *partials, int nx,
+                                                                          BinRanges r, double *gammas) {
     __shared__ double lds[16 * 2];
     const int b = blockIdx.x;
-    double out[1];
-    reduce_partials_dd<1>(partials + (size_t)b * kBinGammaBlocks * kSlots, kBinGammaBlocks, out, lds);
+    const double *pb = partials + (size_t)b * nx * 2;
+    double h[1] = {0.0}, l[1] = {0.0};
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        const dd v = dd_add(dd{h[0], l[0]}, dd{pb[2 * x], pb[2 * x + 1]});
+        h[0] = v.hi;
+        l[0] = v.lo;
+    }
+    block_sum_dd<1>(h, l, lds);
     const int64_t count = r.start[b + 1] - r.start[b];
-    if (threadIdx.x == 0) gammas[b] = count != 0 ? out[0] / (double)count : 1.0;
+    if (threadIdx.x == 0) gammas[b] = count != 0 ? (h[0] + l[0]) / (double)count : 1.0;
 }
 
 // Per-bin scalars of the step: copies of the step's scalars (mesh, frames)
@@ -193,9 +200,12 @@ __global__ void k_track_bin_scalars(const Scalars *sc, const double *gammas, Bin
 void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges &r, double qi, bool exact,
                               const Scalars *sc, double *partials, double *gammas, Scalars *scb) {
     if (r.nbins <= 0) return;
-    k_track_bin_gamma_partials<<<dim3(kBinGammaBlocks, r.nbins), kThreads, 0, s>>>(bunch.px, bunch.py, bunch.pz,
-                                                                                  r, partials);
-    k_track_bin_gamma_final<<<r.nbins, kReduceThreads, 0, s>>>(partials, r, gammas);
+    // blocks per bin: about 8 particles per thread in the most populated bin
+    int64_t most = 0;
+    for (int k = 0; k < r.nbins; ++k) most = std::max(most, r.start[k + 1] - r.start[k]);
+    const int nx = (int)std::min<int64_t>(kBinGammaBlocks, std::max<int64_t>(1, (most + 2047) / 2048));
+    k_track_bin_gamma_partials<<<dim3(nx, r.nbins), kThreads, 0, s>>>(bunch.px, bunch.py, bunch.pz, r, partials);
+    k_track_bin_gamma_final<<<r.nbins, kReduceThreads, 0, s>>>(partials, nx, r, gammas);
     k_track_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, r, qi, exact ? 1 : 0, scb);
 }
 

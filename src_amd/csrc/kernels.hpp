@@ -189,8 +189,8 @@ struct BinRanges {
     int nbins = 0;
     int64_t start[kMaxBins + 1] = {0};
 };
-constexpr int kBinGammaBlocks = 64;  // partial-sum blocks per bin for calcGammas
-// per-bin calcGammas and the per-bin scalars scb[b] (partials: kMaxBins*kBinGammaBlocks*kSlots doubles)
+constexpr int kBinGammaBlocks = 4096;  // at most this many partial-sum blocks per bin for calcGammas
+// per-bin calcGammas and the per-bin scalars scb[b] (partials: kMaxBins*kBinGammaBlocks*2 doubles)
 void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges &r, double qi, bool exact,
                               const Scalars *sc, double *partials, double *gammas, Scalars *scb);
 void launch_bin_keys_by_id(hipStream_t s, int64_t N, const int32_t *bin, const uint32_t *id, int nbins,

@@ -1096,7 +1096,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // XCD-aware grid-stride: workgroups are dispatched to the XCDs round-robin
     // (block b runs on XCD b % kXcds), so the blocks of one XCD stream one
     // contiguous eighth of the bunch together.  A sorted bunch then gathers
-    // each field-grid line into one XCD's L2 instead of into several.
+    // each field-grid line into one XCD's L2 instead of into several.  (One
+    // block per 2048-particle chunk in the same XCD order measured slower:
+    // 2.81 against 2.45 ms at 1e8 particles; the per-block moment reduction
+    // and set-up outweigh the streaming gain.)
     const int xcds = (gridDim.x % kXcds == 0) ? kXcds : 1;
     const int64_t xper = ((N + xcds - 1) / xcds + kThreads - 1) / kThreads * kThreads;
     const int64_t xbeg = (int64_t)(blockIdx.x % xcds) * xper;
