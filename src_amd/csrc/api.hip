@@ -67,6 +67,7 @@ struct opalsc_ctx {
     double qmax = 0.0;        // max |Q| of the resident bunch
     bool have_bunch = false, have_track = false;
     bool track_sc = false;    // the tracker computes space-charge fields
+    bool track_ext = false;   // the tracker has a nonzero external field
     Profiler prof;
     double dh = 0.02;
     int steps_done = 0;
@@ -719,6 +720,7 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_track = true;
     c->track_sc = p->space_charge != 0;
+    c->track_ext = p->Bz != 0.0 || p->Ez0 != 0.0;
     c->steps_done = 0;
     return OPALSC_OK;
 }
@@ -890,9 +892,10 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
         prepushed = s + 1 < nsteps;
         if (binned && c->track_sc)
             launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
-                               c->grid.Bsum);
+                               c->grid.Bsum, c->track_ext);
         else
-            launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials);
+            launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials,
+                               nullptr, c->track_ext);
         c->prof.mark(PROF_GATHER_PUSH);
         launch_track_post(c->stream, c->partials, nb, c->d_ts, c->d_sc,
                           (char *)c->statsb.p + sizeof(opalsc_stats) * s, &c->d_counters[1]);

@@ -43,6 +43,12 @@ struct Scalars {
     double M_br[9], o_br[3];   // beam -> reference
     double M_u[9], o_u[3];     // reference-frame update (transformBunch)
     double Eext[3], Bext[3];   // external field for the bunch at t + dT/2
+    // mixed-precision gather pass: kick and push in the beam frame, then one map
+    // to the updated reference frame.  r_b' = M_rb (r - o_rb2) is the beam-frame
+    // position relative to the mesh origin, ih = 1/hr, M_c = M_u M_br maps
+    // beam-frame vectors to the updated reference frame, r_new = M_c (r_b' - o_c2),
+    // Eext_b/Bext_b the external fields in the beam frame.
+    double o_rb2[3], ih[3], M_c[9], o_c[3], o_c2[3], Eext_b[3], Bext_b[3];
     double zshift;             // image-charge Green's function shift (FFTPoissonSolver.cpp:289)
     double qi;                 // energy bins: macro charge of the bin's particles (setBinCharge)
     int mesh_ok;

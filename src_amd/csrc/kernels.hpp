@@ -162,9 +162,10 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, bool exact, Scalars *sc);
 int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
-// Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered)
+// Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
+// ext: the tracker has a nonzero external field
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials, const void *Bg = nullptr);
+                        const TrackState *ts, double *partials, const void *Bg, bool ext);
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped);
 void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,

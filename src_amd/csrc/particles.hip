@@ -918,6 +918,10 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
             sc->o_br[d] = b2r.origin[d];
             sc->o_rb[d] = r2b.origin[d];
         }
+    } else {
+        // no beam frame: the identity (only the mixed-precision gather pass reads it then)
+        for (int k = 0; k < 9; ++k) sc->M_br[k] = sc->M_rb[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        for (int d = 0; d < 3; ++d) sc->o_br[d] = sc->o_rb[d] = 0.0;
     }
     ext_fields(s, s.t + 0.5 * s.dT, sc->Eext, sc->Bext);
     s.t += s.dT;
@@ -930,6 +934,21 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
     }
     for (int k = 0; k < 9; ++k) sc->M_u[k] = upd.M[k];
     for (int d = 0; d < 3; ++d) sc->o_u[d] = upd.origin[d];
+    // composed maps of the mixed-precision gather pass: r_ref = M_br (r_b - o_br)
+    // and r_new = M_u (r_ref - o_u) give r_new = M_c (r_b - o_c) with M_c = M_u M_br,
+    // o_c = o_br + M_br^T o_u (M_br orthogonal)
+    const double *Mb = sc->M_br, *Mu = sc->M_u, *Mr = sc->M_rb;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            sc->M_c[3 * i + j] = Mu[3 * i] * Mb[j] + Mu[3 * i + 1] * Mb[3 + j] + Mu[3 * i + 2] * Mb[6 + j];
+    for (int d = 0; d < 3; ++d) {
+        sc->o_c[d] = sc->o_br[d] + (Mb[d] * upd.origin[0] + Mb[3 + d] * upd.origin[1] + Mb[6 + d] * upd.origin[2]);
+        sc->o_c2[d] = sc->o_c[d];
+        sc->o_rb2[d] = sc->o_rb[d];
+        sc->ih[d] = 0.0;
+    }
+    mat_vec(Mr, sc->Eext, sc->Eext_b);
+    mat_vec(Mr, sc->Bext, sc->Bext_b);
     s.step += 1;
     *ts = s;
 }
@@ -1043,6 +1062,13 @@ __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int 
         sc->rmax[d] = mx[d];
     }
     sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin) ? 1 : 0;
+    // mixed-precision gather pass: beam-frame positions relative to the mesh origin
+    const double *Mr = sc->M_rb, *og = sc->origin;
+    for (int d = 0; d < 3; ++d) {
+        sc->o_rb2[d] = sc->o_rb[d] + (Mr[d] * og[0] + Mr[3 + d] * og[1] + Mr[6 + d] * og[2]);
+        sc->o_c2[d] = sc->o_c[d] - og[d];
+        sc->ih[d] = 1.0 / sc->hr[d];
+    }
     const int64_t N = sc->N;
     selffield_scalars(sc, sc->psum[2], N, ts->dT, ts->coupling);
     // every particle deposits |Q*dT| <= max|Q|*dT*(1+2^-52)
@@ -1063,11 +1089,10 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
 // magnetic field each bin and its image imply (ParallelTTracker.cpp:567-579),
 // both gathered.  Four waves per SIMD (at most 128 VGPRs) keep enough loads in
 // flight.
-template <typename T, bool NEXT, bool BINNED>
+template <bool NEXT, bool BINNED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma(
-    int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz,
-    const typename vec4_of<T>::type *Eg, const typename vec4_of<T>::type *Bg, const Scalars *sc,
-    const TrackState *ts, double *partials) {
+    int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz, const double4 *Eg,
+    const double4 *Bg, const Scalars *sc, const TrackState *ts, double *partials) {
     __shared__ double lds[4 * kMom * 2];
     const bool sc_on = ts->space_charge != 0;
     const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
@@ -1088,9 +1113,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         Ee[d] = sc->Eext[d];
         Be[d] = sc->Bext[d];
     }
-    const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
-    const double e_kick = 0.5 * dT * charge * kC / mass, b_kick = 0.5 * dT * charge * kC * kC / mass;
-    const double half_cdt = 0.5 * (kC * dT);
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     // XCD-aware grid-stride: workgroups are dispatched to the XCDs round-robin
@@ -1110,44 +1132,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         double p[3] = {px[i], py[i], pz[i]};
         double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
         if (sc_on) {
-            if constexpr (sizeof(T) == 8) transform_to(Mrb, orb, r);
-            else transform_to_fast(Mrb, orb, r);
+            transform_to(Mrb, orb, r);
             int l[3];
             double d[3];
-            if constexpr (sizeof(T) == 8) {
-                bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
-                ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
-                ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
-                if (ok) {
-                    gather_E<T>(Eg, l, d, n0, n1, n2, Ef);
-                    if constexpr (BINNED) gather_E<T>(Bg, l, d, n0, n1, n2, Bf);
-                }
-            } else {
-                bool ok = cic_dim_fast(r[0], o0, ih0, l[0], d[0]);
-                ok &= cic_dim_fast(r[1], o1, ih1, l[1], d[1]);
-                ok &= cic_dim_fast(r[2], o2, ih2, l[2], d[2]);
-                if (ok) {
-                    gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
-                    if constexpr (BINNED) gather_E_fast(Bg, l, d, n0, n1, n2, Bf);
-                }
+            bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
+            ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
+            ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
+            if (ok) {
+                gather_E<double>(Eg, l, d, n0, n1, n2, Ef);
+                if constexpr (BINNED) gather_E<double>(Bg, l, d, n0, n1, n2, Bf);
             }
             if constexpr (!BINNED) {
                 Bf[0] = Bf[0] - betaC * Ef[1];
                 Bf[1] = Bf[1] + betaC * Ef[0];
             }
             double t3[3];
-            if constexpr (sizeof(T) == 8) {
-                transform_to(Mb, ob, r);
-                mat_vec(Mb, Ef, t3);
-            } else {
-                transform_to_fast(Mb, ob, r);
-                mat_vec_fast(Mb, Ef, t3);
-            }
+            transform_to(Mb, ob, r);
+            mat_vec(Mb, Ef, t3);
             Ef[0] = t3[0];
             Ef[1] = t3[1];
             Ef[2] = t3[2];
-            if constexpr (sizeof(T) == 8) mat_vec(Mb, Bf, t3);
-            else mat_vec_fast(Mb, Bf, t3);
+            mat_vec(Mb, Bf, t3);
             Bf[0] = t3[0];
             Bf[1] = t3[1];
             Bf[2] = t3[2];
@@ -1157,22 +1162,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             Bf[k] += Be[k];
         }
         double pn[3];
-        if constexpr (sizeof(T) == 8) {
-            boris_kick(p, Ef, Bf, dT, mass, charge);
-            push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
-            transform_to(Mu, ou, r);
-            mat_vec(Mu, p, pn);
-        } else {
-            boris_kick_fast(p, Ef, Bf, e_kick, b_kick);
-            push_half_fast(r[0], r[1], r[2], p[0], p[1], p[2], half_cdt);
-            transform_to_fast(Mu, ou, r);
-            mat_vec_fast(Mu, p, pn);
-        }
+        boris_kick(p, Ef, Bf, dT, mass, charge);
+        push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
+        transform_to(Mu, ou, r);
+        mat_vec(Mu, p, pn);
         accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
-        if constexpr (NEXT) {
-            if constexpr (sizeof(T) == 8) push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
-            else push_half_fast(r[0], r[1], r[2], pn[0], pn[1], pn[2], half_cdt);
+        if constexpr (NEXT) push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
+        x[i] = r[0];
+        y[i] = r[1];
+        z[i] = r[2];
+        px[i] = pn[0];
+        py[i] = pn[1];
+        pz[i] = pn[2];
+    }
+    store_moment_partials(m, partials, lds);
+}
+
+// The mixed-precision gather pass: the same step with FP32 field grids, the
+// kick and push done in the beam frame (Boris kick and drift commute with
+// rotations) and one composed map M_c to the updated reference frame, so the
+// loop needs 33 frame constants instead of 42 and no round trip of the
+// position through the reference frame.  Positions, momenta and moments stay
+// in FP64.  EXT: a nonzero external field (without one the loop holds 27 frame
+// constants, which fit the scalar registers).
+template <bool NEXT, bool BINNED, bool EXT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma_mixed(
+    int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz, const float4 *Eg,
+    const float4 *Bg, const Scalars *sc, const TrackState *ts, double *partials) {
+    __shared__ double lds[4 * kMom * 2];
+    const bool sc_on = ts->space_charge != 0;
+    const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
+    const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+    const double betaC = sc->betaC;
+    double Mr[9], Mc[9], orb[3], oc[3], ih[3], Ee[3], Be[3];
+    for (int k = 0; k < 9; ++k) {
+        Mr[k] = sc->M_rb[k];
+        Mc[k] = sc->M_c[k];
+    }
+    for (int d = 0; d < 3; ++d) {
+        orb[d] = sc->o_rb2[d];
+        oc[d] = sc->o_c2[d];
+        ih[d] = sc->ih[d];
+        Ee[d] = sc->Eext_b[d];
+        Be[d] = sc->Bext_b[d];
+    }
+    const double e_kick = 0.5 * dT * charge * kC / mass, b_kick = 0.5 * dT * charge * kC * kC / mass;
+    const double half_cdt = 0.5 * (kC * dT);
+    double m[kMom];
+    for (int k = 0; k < kMom; ++k) m[k] = 0.0;
+    // XCD-aware grid-stride, as in k_track_gamma
+    const int xcds = (gridDim.x % kXcds == 0) ? kXcds : 1;
+    const int64_t xper = ((N + xcds - 1) / xcds + kThreads - 1) / kThreads * kThreads;
+    const int64_t xbeg = (int64_t)(blockIdx.x % xcds) * xper;
+    const int64_t xend = xbeg + xper < N ? xbeg + xper : N;
+    const int64_t xstride = (int64_t)(gridDim.x / xcds) * kThreads;
+    for (int64_t i = xbeg + (int64_t)(blockIdx.x / xcds) * kThreads + threadIdx.x; i < xend; i += xstride) {
+        double r[3] = {x[i], y[i], z[i]};
+        double p[3] = {px[i], py[i], pz[i]};
+        transform_to_fast(Mr, orb, r);  // beam frame, relative to the mesh origin
+        double pb[3];
+        mat_vec_fast(Mr, p, pb);
+        double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
+        if (sc_on) {
+            int l[3];
+            double d[3];
+            bool ok = cic_dim_fast(r[0], 0.0, ih[0], l[0], d[0]);
+            ok &= cic_dim_fast(r[1], 0.0, ih[1], l[1], d[1]);
+            ok &= cic_dim_fast(r[2], 0.0, ih[2], l[2], d[2]);
+            if (ok) {
+                gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
+                if constexpr (BINNED) gather_E_fast(Bg, l, d, n0, n1, n2, Bf);
+            }
+            if constexpr (!BINNED) {
+                Bf[0] = -betaC * Ef[1];
+                Bf[1] = betaC * Ef[0];
+            }
         }
+        if constexpr (EXT)
+            for (int k = 0; k < 3; ++k) {
+                Ef[k] += Ee[k];
+                Bf[k] += Be[k];
+            }
+        boris_kick_fast(pb, Ef, Bf, e_kick, b_kick);
+        push_half_fast(r[0], r[1], r[2], pb[0], pb[1], pb[2], half_cdt);
+        transform_to_fast(Mc, oc, r);
+        double pn[3];
+        mat_vec_fast(Mc, pb, pn);
+        accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+        if constexpr (NEXT) push_half_fast(r[0], r[1], r[2], pn[0], pn[1], pn[2], half_cdt);
         x[i] = r[0];
         y[i] = r[1];
         z[i] = r[2];
@@ -1202,30 +1279,44 @@ static int resident_blocks(K kernel) {
 
 // One wave of resident blocks (fewer partial sums for the step statistics).
 int gamma_blocks(bool fp64, int64_t N) {
-    const int r = fp64 ? resident_blocks(k_track_gamma<double, true, false>)
-                       : resident_blocks(k_track_gamma<float, true, false>);
+    const int r = fp64 ? resident_blocks(k_track_gamma<true, false>)
+                       : resident_blocks(k_track_gamma_mixed<true, false, true>);
     const int b = red_blocks(N);
     return b < r ? b : r;
 }
 
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials, const void *Bg) {
+                        const TrackState *ts, double *partials, const void *Bg, bool ext) {
     if (b.N <= 0) return;
     const int blocks = gamma_blocks(fp64, b.N);
-#define OPALSC_GAMMA(T_, V_, N_, B_)                                                                         \
-    k_track_gamma<T_, N_, B_><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const V_ *)Eg, \
-                                                          (const V_ *)Bg, sc, ts, partials)
-#define OPALSC_GAMMA2(T_, V_, N_)            \
-    if (Bg) OPALSC_GAMMA(T_, V_, N_, true); \
-    else OPALSC_GAMMA(T_, V_, N_, false)
+#define OPALSC_GAMMA(K_, V_, ...)                                                                       \
+    K_<__VA_ARGS__><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const V_ *)Eg, \
+                                                (const V_ *)Bg, sc, ts, partials)
     if (fp64) {
-        if (next) OPALSC_GAMMA2(double, double4, true);
-        else OPALSC_GAMMA2(double, double4, false);
+        if (next) {
+            if (Bg) OPALSC_GAMMA(k_track_gamma, double4, true, true);
+            else OPALSC_GAMMA(k_track_gamma, double4, true, false);
+        } else {
+            if (Bg) OPALSC_GAMMA(k_track_gamma, double4, false, true);
+            else OPALSC_GAMMA(k_track_gamma, double4, false, false);
+        }
+    } else if (ext) {
+        if (next) {
+            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, true, true, true);
+            else OPALSC_GAMMA(k_track_gamma_mixed, float4, true, false, true);
+        } else {
+            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, false, true, true);
+            else OPALSC_GAMMA(k_track_gamma_mixed, float4, false, false, true);
+        }
     } else {
-        if (next) OPALSC_GAMMA2(float, float4, true);
-        else OPALSC_GAMMA2(float, float4, false);
+        if (next) {
+            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, true, true, false);
+            else OPALSC_GAMMA(k_track_gamma_mixed, float4, true, false, false);
+        } else {
+            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, false, true, false);
+            else OPALSC_GAMMA(k_track_gamma_mixed, float4, false, false, false);
+        }
     }
-#undef OPALSC_GAMMA2
 #undef OPALSC_GAMMA
 }
 
