@@ -152,12 +152,12 @@ def algorithmic_bytes(cfg, wbytes):
 
 # profiler kernel class -> kernel symbol of the mixed-precision step
 KERNEL_SYMBOL = {
-    "gather_kick_push": "k_track_gamma<float,true>",
+    "gather_kick_push": "k_track_gamma_mixed<true,false,false>",
     "deposit": "k_deposit_lds<1,false,true>",
     "rotate_bounds": "k_track_alpha<false,false>",
     "push_rotate_bounds": "k_track_alpha<false,true>",
     "fft_z_conv": "k_strided<float,16,2>",
-    "grad": "k_grad<float>",
+    "grad": "k_grad<float,false>",
 }
 
 
@@ -220,12 +220,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="skip the energy-binned measurement")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="run only the energy-binned measurement and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the space-charge step of one "
               "bunch on one MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import src_amd as sc
+    if args.c5_only:
+        log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+        print(json.dumps(run_c5(sc, args.steps, args.warmup, args.sort_interval, log)))
+        return
     cfg = CONFIGS[args.config]
     N, n = cfg["N"], cfg["n"]
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731

@@ -800,15 +800,20 @@ void launch_iota(hipStream_t s, int64_t N, uint32_t *v) {
     if (N > 0) k_iota<<<stream_blocks(N), kThreads, 0, s>>>(N, v);
 }
 
+// Plain copy, one 16-byte element per thread over a full grid: the fastest
+// copy form measured on MI355X (a persistent grid-stride loop reaches ~25 %
+// less), used as the achievable-bandwidth reference and to calibrate the
+// FETCH_SIZE / WRITE_SIZE counters.
 __global__ __launch_bounds__(256) void k_copy_calibration(size_t n16, const uint4 *__restrict__ src,
                                                           uint4 *__restrict__ dst) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
 }
 void launch_copy_calibration(hipStream_t s, size_t bytes, const void *src, void *dst) {
     const size_t n16 = bytes / 16;
-    int blocks = (int)std::min<size_t>((n16 + 255) / 256, 8192);
-    k_copy_calibration<<<blocks, kThreads, 0, s>>>(n16, (const uint4 *)src, (uint4 *)dst);
+    const size_t blocks = (n16 + 255) / 256;
+    if (blocks == 0 || blocks > 0x7fffffff) return;
+    k_copy_calibration<<<(unsigned)blocks, kThreads, 0, s>>>(n16, (const uint4 *)src, (uint4 *)dst);
 }
 
 __global__ void k_fill_i64(size_t n, long long *p) {
