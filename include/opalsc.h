@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define OPALSC_ABI_VERSION 1
+#define OPALSC_ABI_VERSION 2
 
 enum {
     OPALSC_OK = 0,
@@ -46,6 +46,7 @@ enum {
 
 enum { OPALSC_FP64 = 0, OPALSC_MIXED = 1 };
 enum { OPALSC_GREENS_INTEGRATED = 0, OPALSC_GREENS_STANDARD = 1 };
+enum { OPALSC_BC_OPEN = 0, OPALSC_BC_PERIODIC = 1 };
 enum { OPALSC_GRID_RHO = 0, OPALSC_GRID_PHI = 1, OPALSC_GRID_EFIELD = 2 };
 
 typedef struct opalsc_ctx opalsc_ctx;
@@ -59,6 +60,10 @@ typedef struct {
     int64_t max_particles; /* capacity of the device particle buffers */
     int device;            /* HIP device ordinal */
     int sort_interval;     /* tracker: re-sort the bunch by cell every k steps (0: never) */
+    int bcz;               /* BCFFTZ (FieldSolver.cpp:153,428-448): OPALSC_BC_OPEN, or
+                              OPALSC_BC_PERIODIC (DC beams: the solver is periodic in z,
+                              FFTPoissonSolver.cpp:60,130-155; opalsc_compute_potential
+                              only, n[2] = 2^a or 3*2^a with a >= 2) */
 } opalsc_config;
 
 /* DataSink statistics (src/Structure/DataSink.cpp:360-391) as computed by
@@ -113,6 +118,10 @@ int opalsc_synchronize(opalsc_ctx *ctx);
  * circular convolution of the zero-padded rho with the (mirrored) Green's
  * function on the doubled grid, without the h^3 and 1/(4 pi eps0) factors. */
 int opalsc_compute_potential(opalsc_ctx *ctx, double *rho_inout, const double hr[3]);
+/* With bcz = OPALSC_BC_PERIODIC the same call is the periodic-z solve: circular
+ * convolution on the (2 n0, 2 n1, n2) grid with the Green's function mirrored in
+ * x and y only and read one-sided in z (mirrorRhoField, FFTPoissonSolver.cpp:654-664),
+ * normalised by 1/(2 n0 2 n1 n2). */
 
 /* FFTPoissonSolver::computePotential(Field_t& rho, Vector_t hr, double zshift),
  * src/Solvers/FFTPoissonSolver.cpp:289-330: the image-charge potential.  In

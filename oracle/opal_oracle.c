@@ -443,6 +443,62 @@ void orc_compute_potential(const int n[3], double *rho_inout, const double hr[3]
 }
 
 /* ------------------------------------------------------------------------ */
+/* Periodic-z solve (DC beams, FFTPoissonSolver with bcz_m)                 */
+/* ------------------------------------------------------------------------ */
+
+/* The Green's function on the (2n0, 2n1, n2) grid of the periodic-z solver:
+ * the octant (with the [0,0,0] quirk) mirrored in x and y only, z read one-sided
+ * for k in [0, n2) (FFTPoissonSolver::initializeFields, FFTPoissonSolver.cpp:130-155:
+ * z not doubled; mirrorRhoField, FFTPoissonSolver.cpp:661-662: no z mirror). */
+void orc_mirror_green_periodic_z(const int n[3], const double *Goct, double *G2) {
+    const int m0 = 2 * n[0], m1 = 2 * n[1], m2 = n[2];
+    const int g0 = n[0] + 1, g1 = n[1] + 1;
+    for (int z = 0; z < m2; ++z)
+        for (int y = 0; y < m1; ++y) {
+            int fy = y <= n[1] ? y : m1 - y;
+            for (int x = 0; x < m0; ++x) {
+                int fx = x <= n[0] ? x : m0 - x;
+                G2[((size_t)z * m1 + y) * m0 + x] = Goct[((size_t)z * g1 + fy) * g0 + fx];
+            }
+        }
+}
+
+/* FFTPoissonSolver::computePotential(rho, hr) with bcz_m
+ * (FFTPoissonSolver.cpp:337-382 over the grid of FFTPoissonSolver.cpp:130-155):
+ * circular convolution on (2n0, 2n1, n2) of rho zero-padded in x and y with the
+ * Green's function above, scaled by 1/(2n0 2n1 n2).  In place: rho in, phi out. */
+void orc_compute_potential_periodic_z(const int n[3], double *rho_inout, const double hr[3], int greens) {
+    const int m[3] = {2 * n[0], 2 * n[1], n[2]};
+    const size_t go = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    const size_t M = (size_t)m[0] * m[1] * m[2];
+    double *Goct = (double *)malloc(go * sizeof(double));
+    double *G2 = (double *)malloc(M * sizeof(double));
+    if (greens == 1) orc_green_standard_octant(n, hr, Goct);
+    else orc_igf_octant(n, hr, Goct, 1);
+    orc_mirror_green_periodic_z(n, Goct, G2);
+    cplx *a = (cplx *)calloc(M, sizeof(cplx));
+    cplx *g = (cplx *)malloc(M * sizeof(cplx));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                a[((size_t)k * m[1] + j) * m[0] + i] = rho_inout[((size_t)k * n[1] + j) * n[0] + i];
+    for (size_t q = 0; q < M; ++q) g[q] = G2[q];
+    orc_fft3d(a, m, -1);
+    orc_fft3d(g, m, -1);
+    for (size_t q = 0; q < M; ++q) a[q] *= g[q];
+    orc_fft3d(a, m, +1);
+    double inv = 1.0 / (double)M;
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                rho_inout[((size_t)k * n[1] + j) * n[0] + i] = creal(a[((size_t)k * m[1] + j) * m[0] + i]) * inv;
+    free(a);
+    free(g);
+    free(Goct);
+    free(G2);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Image-charge (shifted) solve                                             */
 /* ------------------------------------------------------------------------ */
 

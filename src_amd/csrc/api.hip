@@ -189,6 +189,10 @@ extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
         g_create_error = "max_particles out of range (0 .. 2^32-16)";
         return OPALSC_EINVAL;
     }
+    if (cfg->bcz != OPALSC_BC_OPEN && cfg->bcz != OPALSC_BC_PERIODIC) {
+        g_create_error = "bcz must be OPALSC_BC_OPEN or OPALSC_BC_PERIODIC";
+        return OPALSC_EINVAL;
+    }
     opalsc_ctx *c = new (std::nothrow) opalsc_ctx();
     if (!c) {
         g_create_error = "out of host memory";
@@ -210,7 +214,8 @@ extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
         return bail(OPALSC_EHIP);
     }
     c->prof.stream = c->stream;
-    if (c->grid.init(cfg->n, cfg->precision == OPALSC_FP64, cfg->greens, c->stream, c->err) != 0)
+    if (c->grid.init(cfg->n, cfg->precision == OPALSC_FP64, cfg->greens, cfg->bcz == OPALSC_BC_PERIODIC, c->stream,
+                     c->err) != 0)
         return bail(OPALSC_EINVAL);
     c->grid.prof = &c->prof;
     if (alloc_bunch(c, cfg->max_particles) != 0) return bail(OPALSC_ENOMEM);
@@ -262,6 +267,12 @@ static int check_n(opalsc_ctx *c, int64_t N) {
     return 0;
 }
 
+// The periodic-z (DC beam) context provides the Poisson solve only.
+static int reject_periodic_z(opalsc_ctx *c) {
+    if (c->grid.pz) FAIL(c, OPALSC_EINVAL, "a periodic-z context provides opalsc_compute_potential only");
+    return 0;
+}
+
 static int upload(opalsc_ctx *c, DevBuf &b, const void *src, size_t bytes) {
     if (!b.ensure(bytes)) FAIL(c, OPALSC_ENOMEM, "hipMalloc of staging buffer failed");
     if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
@@ -307,6 +318,7 @@ extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double
 
 extern "C" int opalsc_compute_potential_shifted(opalsc_ctx *c, double *rho, const double hr[3], double zshift) {
     if (!c || !rho || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    if (int e = reject_periodic_z(c)) return e;
     if (c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
     Scalars s;
     memset(&s, 0, sizeof(s));
@@ -409,6 +421,7 @@ extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double
                                           const double origin[3], const double hr[3],
                                           double coupling, double *Ef, double *Bf) {
     if (!c || !origin || !hr) return OPALSC_EINVAL;
+    if (int e = reject_periodic_z(c)) return e;
     if (N > 0 && (!R || !P || !Q || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
     if (int e = check_n(c, N)) return e;
     if (N == 0) FAIL(c, OPALSC_EINVAL, "computeSelfFields needs particles (the Lorentz factor divides by N)");
@@ -449,6 +462,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
                                                  double coupling, const double *gamma_bin, int image,
                                                  double *Ef, double *Bf) {
     if (!c || !origin || !hr) return OPALSC_EINVAL;
+    if (int e = reject_periodic_z(c)) return e;
     if (N > 0 && (!R || !P || !bin || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
     if (nbins < 1 || nbins > kMaxBins) FAIL(c, OPALSC_EINVAL, "nbins must be in [1, 64]");
     if (int e = check_n(c, N)) return e;
@@ -685,6 +699,7 @@ extern "C" int opalsc_bunch_get(opalsc_ctx *c, double *R, double *P, double *Q) 
 
 extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     if (!c || !p) return OPALSC_EINVAL;
+    if (int e = reject_periodic_z(c)) return e;
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "track_init before bunch_set/bunch_generate");
     if (!(p->dT > 0.0)) FAIL(c, OPALSC_EINVAL, "dT must be positive");
     TrackState ts;

@@ -47,6 +47,7 @@ struct PoissonGrid {
     int kp = 0;   // row pitch (complex elements) of A and B: K0 rounded up to whole tiles
     int gp = 0;   // row pitch (real elements) of Ghat, >= kp
     bool fp64 = true;
+    bool pz = false;  // periodic in z (z not doubled): the DC-beam solver
     int greens = 0;
     size_t ncell = 0, noct = 0;
     FftPlan plan[3];
@@ -66,10 +67,12 @@ struct PoissonGrid {
     // energy bins (allocated on first use): sums over bins of E and of the
     // magnetic field it implies, vec4<T>[ncell] each
     void *Esum = nullptr, *Bsum = nullptr;
+    // periodic z: the Green's function transform, complex T[n2][n1+1][gp] (not even in z)
+    void *Gper = nullptr;
     hipStream_t stream = nullptr;
     Profiler *prof = nullptr;
 
-    int init(const int nn[3], bool fp64, int greens, hipStream_t s, std::string &err);
+    int init(const int nn[3], bool fp64, int greens, bool periodic_z, hipStream_t s, std::string &err);
     void destroy();
     void green(const Scalars *d_sc, bool correctly_rounded);
     int ensure_image(std::string &err);

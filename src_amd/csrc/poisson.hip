@@ -339,7 +339,8 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 // transformed line written to out[o*outO + i*outS + kx] for i < nout.  Tiles
 // are whole (the row pitch is a multiple of W, padding columns hold zeros and
 // stay zero), so every row of a tile is one aligned 128-byte line.
-// MODE 0: forward; MODE 1: inverse; MODE 2: forward, *Ghat, inverse (z-conv).
+// MODE 0: forward; MODE 1: inverse; MODE 2: forward, *Ghat, inverse (z-conv);
+// MODE 3: image charge; MODE 4: periodic z (whole lines, complex Gper, no flip).
 template <typename T, int E, int MODE>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_strided(const typename cplx_of<T>::type *in,
                                                          typename cplx_of<T>::type *out, FftPlan pl,
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
                                                          size_t outS, size_t outO, const T *Gh,
                                                          int gp, int n1, int n2, int L1) {
     using C = typename cplx_of<T>::type;
-    constexpr int Q = MODE == 1 ? E : E / 2;  // the inverse y pass reads whole lines
+    constexpr int Q = (MODE == 1 || MODE == 4) ? E : E / 2;  // whole-line passes read E per thread
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * nin;
@@ -393,8 +394,9 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         }
         __syncthreads();
         fft_lds<C, E>(buf, tw, pl, true);
-    } else if constexpr (MODE == 3) {
-        // image charge: times -Gi (complex, not even in z; y folded), FFTPoissonSolver.cpp:316
+    } else if constexpr (MODE == 3 || MODE == 4) {
+        // image charge: times -Gi (complex, not even in z; y folded), FFTPoissonSolver.cpp:316;
+        // periodic z: times Gper (complex, one-sided in z; y folded)
         fft_lds<C, E>(buf, tw, pl, false);
         const C *Gi = (const C *)Gh;
         const int fy = o <= n1 ? o : L1 - o;
@@ -409,7 +411,8 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
             const int e = threadIdx.x + q * kFftThreads;
             const int idx = (e >> lw) * P + (e & (W - 1));
             const C v = buf[idx];
-            buf[idx] = C{-(v.x * g[q].x - v.y * g[q].y), -(v.x * g[q].y + v.y * g[q].x)};
+            const C p = C{v.x * g[q].x - v.y * g[q].y, v.x * g[q].y + v.y * g[q].x};
+            buf[idx] = MODE == 3 ? C{-p.x, -p.y} : p;
         }
         __syncthreads();
         fft_lds<C, E>(buf, tw, pl, true);
@@ -627,16 +630,21 @@ static size_t lds_bytes(const FftPlan &pl, size_t csize) {
     default: break;                                            \
     }
 
-int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, std::string &err) {
+int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z, hipStream_t s,
+                      std::string &err) {
     stream = s;
     fp64 = fp64_;
     greens = greens_;
+    pz = periodic_z;
     for (int d = 0; d < 3; ++d) {
         n[d] = nn[d];
-        L[d] = 2 * nn[d];
+        // periodic z: the z transform has the grid's own length (FFTPoissonSolver.cpp:130-155)
+        L[d] = (pz && d == 2) ? nn[d] : 2 * nn[d];
         if (nn[d] < 4) { err = "n must be >= 4 in every dimension"; return -1; }
         if (!make_plan(L[d], fp64, plan[d])) {
-            err = "unsupported grid size n=" + std::to_string(nn[d]) + " (need n = 2^a or 3*2^a, a>=1)";
+            err = "unsupported grid size n=" + std::to_string(nn[d]) +
+                  ((pz && d == 2) ? " for periodic z (need n = 2^a or 3*2^a, a>=2)"
+                                  : " (need n = 2^a or 3*2^a, a>=1)");
             return -1;
         }
     }
@@ -679,6 +687,14 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
         err = "hipMemset failed";
         return -1;
     }
+    if (pz) {
+        const size_t szP = (size_t)L[2] * (n[1] + 1) * gp * cs;
+        if (!alloc(&Gper, szP)) return -1;
+        if (hipMemset(Gper, 0, szP) != hipSuccess) {
+            err = "hipMemset failed";
+            return -1;
+        }
+    }
     for (int d = 0; d < 3; ++d) {
         std::vector<char> tw;
         if (fp64) make_twiddles<double>(L[d], tw);
@@ -695,11 +711,11 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, hipStream_t s, s
 
 void PoissonGrid::destroy() {
     void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2],
-                  HimgA, HimgB, Gimg, Esum, Bsum};
+                  HimgA, HimgB, Gimg, Esum, Bsum, Gper};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     rho_i64 = rho_f64 = A = B = Goct = Tcorner = Ghat = phi = Eg = nullptr;
-    HimgA = HimgB = Gimg = Esum = Bsum = nullptr;
+    HimgA = HimgB = Gimg = Esum = Bsum = Gper = nullptr;
     twd[0] = twd[1] = twd[2] = nullptr;
 }
 
@@ -742,7 +758,14 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
     const size_t cs = sizeof(C);
     const size_t gp = (size_t)g.gp;
     dct_xy<T>(g, (const double *)g.Goct, (T *)g.Ghat);
-    {
+    if (g.pz) {
+        // periodic z: the z line is G[k] for k in [0, n2), one-sided (mirrorRhoField,
+        // FFTPoissonSolver.cpp:661-662), so its transform is complex
+        const FftPlan &pl = g.plan[2];
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
+        OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (const T *)g.Ghat, (const T *)g.Ghat, (C *)g.Gper, pl, n1, n2, g.gp));
+    } else {
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
         OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
@@ -837,7 +860,11 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((K0 + pl.W - 1) / pl.W, L1);
-        if (image)
+        if (g.pz)
+            OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 4><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                        B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Gper, g.gp,
+                                        n1, n2, L1))
+        else if (image)
             OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 3><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
                                         B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Gimg, g.gp,
                                         n1, n2, L1))
@@ -857,7 +884,7 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
     {
         const FftPlan &pl = g.plan[0];
         dim3 grid((nlines + pl.W - 1) / pl.W);
-        const double invM = 1.0 / ((double)(2 * n0) * (double)(2 * n1) * (double)(2 * n2));
+        const double invM = 1.0 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]);
         if (outscale)
             OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
                                         A, (T *)g.phi, pl, n0, nlines, g.kp, invM, d_sc))

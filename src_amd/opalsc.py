@@ -23,6 +23,7 @@ LIB_PATH = os.environ.get("OPALSC_LIB", os.path.join(_HERE, "libopalsc.so"))
 FP64, MIXED = 0, 1
 GREENS_INTEGRATED, GREENS_STANDARD = 0, 1
 GRID_RHO, GRID_PHI, GRID_EFIELD = 0, 1, 2
+BC_OPEN, BC_PERIODIC = 0, 1
 
 CLIGHT = 299792458.0                       # src/Classic/Physics/Physics.h:52
 EPSILON_0 = 8.854187817e-12                # src/Classic/Physics/Physics.h:58
@@ -47,7 +48,7 @@ EXPORTS = [
 class Config(C.Structure):
     _fields_ = [("n", C.c_int * 3), ("greens", C.c_int), ("precision", C.c_int),
                 ("bboxincr", C.c_double), ("max_particles", C.c_int64), ("device", C.c_int),
-                ("sort_interval", C.c_int)]
+                ("sort_interval", C.c_int), ("bcz", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -176,8 +177,9 @@ class SpaceChargeContext:
     """One opalsc context: a FIELDSOLVER of n cells per dimension on one GPU."""
 
     def __init__(self, n, precision=FP64, greens=GREENS_INTEGRATED, bboxincr=2.0,
-                 max_particles=1 << 20, device=0, sort_interval=0):
+                 max_particles=1 << 20, device=0, sort_interval=0, bcz=BC_OPEN):
         cfg = Config()
+        cfg.bcz = bcz
         cfg.n[:] = [int(v) for v in n]
         cfg.greens, cfg.precision, cfg.bboxincr = greens, precision, bboxincr
         cfg.max_particles, cfg.device, cfg.sort_interval = int(max_particles), device, sort_interval
@@ -399,10 +401,11 @@ class SpaceChargeContext:
 class FFTPoissonSolver:
     """PoissonSolver interface (src/Solvers/PoissonSolver.h:16-70) over the HIP solver."""
 
-    def __init__(self, n, greens="INTEGRATED", precision=FP64, device=0):
+    def __init__(self, n, greens="INTEGRATED", precision=FP64, device=0, bcz="OPEN"):
         g = GREENS_INTEGRATED if greens.upper() == "INTEGRATED" else GREENS_STANDARD
+        b = BC_PERIODIC if str(bcz).upper() == "PERIODIC" else BC_OPEN
         self._ctx = SpaceChargeContext(n, precision=precision, greens=g, max_particles=0,
-                                       device=device)
+                                       device=device, bcz=b)
 
     def computePotential(self, rho, hr, zshift=None):
         """In place in the reference (rho in, phi out); here rho is returned as phi.

@@ -84,6 +84,8 @@ def lib():
     L.orc_space_charge.restype = i64
     L.orc_shifted_green_doubled.argtypes = [PI32, PD, C.c_double, PD]
     L.orc_compute_potential_shifted.argtypes = [PI32, PD, PD, C.c_double]
+    L.orc_compute_potential_periodic_z.argtypes = [PI32, PD, PD, C.c_int]
+    L.orc_mirror_green_periodic_z.argtypes = [PI32, PD, PD]
     L.orc_calc_gammas.argtypes = [i64, PD, PI32, C.c_int, PD]
     L.orc_compute_self_fields_binned.argtypes = [i64, PD, PD, PI32, C.c_int, C.c_double, PD, C.c_double,
                                                  PD, PD, PI32, C.c_double, C.c_int, C.c_int, C.c_void_p,
@@ -186,6 +188,19 @@ def compute_potential(n, rho, hr, greens=0):
     phi = _d(rho).ravel().copy()
     lib().orc_compute_potential(_n(n), phi, _d(hr), greens)
     return phi
+
+
+def compute_potential_periodic_z(n, rho, hr, greens=0):
+    phi = _d(rho).ravel().copy()
+    lib().orc_compute_potential_periodic_z(_n(n), phi, _d(hr), greens)
+    return phi
+
+
+def mirror_green_periodic_z(n, Goct):
+    n = _n(n)
+    G2 = np.zeros(4 * int(n[0]) * int(n[1]) * int(n[2]))
+    lib().orc_mirror_green_periodic_z(n, _d(Goct).ravel(), G2)
+    return G2.reshape(int(n[2]), 2 * int(n[1]), 2 * int(n[0]))
 
 
 def shifted_green_doubled(n, hr, zshift):

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert missing == []
     assert sorted(sc.EXPORTS) == names
-    assert L.opalsc_abi_version() == 1
+    assert L.opalsc_abi_version() == 2
 
 
 def test_create_validates_config_before_touching_a_device():
@@ -37,6 +37,17 @@ def test_create_validates_config_before_touching_a_device():
         sc.SpaceChargeContext((32, 32, 32), greens=5)
     with pytest.raises(sc.OpalscError, match="max_particles"):
         sc.SpaceChargeContext((32, 32, 32), max_particles=-1)
+    with pytest.raises(sc.OpalscError, match="bcz"):
+        sc.SpaceChargeContext((32, 32, 32), bcz=3)
+
+
+def test_config_struct_matches_header():
+    """The ctypes mirror of opalsc_config has the header's fields in order."""
+    text = open(os.path.join(ROOT, "include", "opalsc.h")).read()
+    body = re.search(r"typedef struct \{(.*?)\} opalsc_config;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"\b([a-z_]+)(?:\[3\])?;", body)
+    assert fields == [f[0] for f in sc.opalsc.Config._fields_]
 
 
 def test_host_cr_math_equals_quad_oracle():

@@ -411,3 +411,37 @@ def test_tracker_binned_without_image_and_mixed():
     assert rel_l2(Pg - P, Po - P) < 1e-5
     for a, b in zip(sg, so):
         assert np.allclose(a["eps_norm"], b["eps_norm"], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-12), (sc.MIXED, 1e-5)])
+@pytest.mark.parametrize("greens", [0, 1])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 24, 8), (64, 32, 96)])
+def test_compute_potential_periodic_z(n, greens, precision, tol):
+    """DC-beam solver (BCFFTZ=PERIODIC): x and y doubled, z periodic with the
+    one-sided Green's function."""
+    rng = np.random.default_rng(19)
+    rho = rng.normal(size=n[0] * n[1] * n[2])
+    hr = np.array([1.1e-4, 0.9e-4, 2.3e-4])
+    ref = O.compute_potential_periodic_z(n, rho, hr, greens)
+    with sc.SpaceChargeContext(n, precision=precision, greens=greens, max_particles=0,
+                               bcz=sc.BC_PERIODIC) as c:
+        phi = c.computePotential(rho, hr)
+        phi2 = c.computePotential(rho, hr)
+    assert np.array_equal(phi, phi2)
+    assert rel_l2(phi, ref) < tol
+
+
+def test_periodic_z_context_rejects_particle_entry_points():
+    R, P, Q = gaussian_bunch(100, seed=3)
+    dt = np.full(100, 1e-11)
+    with sc.SpaceChargeContext((8, 8, 8), max_particles=100, bcz=sc.BC_PERIODIC) as c:
+        hr, org = c.boundp(R)  # the mesh entry points do not depend on the solver
+        with pytest.raises(sc.OpalscError, match="periodic"):
+            c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr)
+        with pytest.raises(sc.OpalscError, match="periodic"):
+            c.computePotentialShifted(np.zeros(512), hr, 0.1)
+        c.set_bunch(R, P, Q)
+        with pytest.raises(sc.OpalscError, match="periodic"):
+            c.track_init(1e-11)
+    with pytest.raises(sc.OpalscError, match="periodic z"):
+        sc.SpaceChargeContext((8, 8, 6), max_particles=0, bcz=sc.BC_PERIODIC)

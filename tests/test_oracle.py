@@ -226,6 +226,62 @@ def test_point_charge_potential_is_green_function():
         assert abs(phi[z, y, x] - g) < 1e-14 * G.max()
 
 
+# ------------------------------------------------------- periodic-z (DC beam) solve
+def test_periodic_z_green_layout():
+    """FFTPoissonSolver with bcz_m: the grid is doubled in x and y only and the
+    octant is mirrored in x and y only (FFTPoissonSolver.cpp:130-155,661-662),
+    so z reads G one-sided for k in [0, n2)."""
+    n, hr = (3, 4, 5), np.array([1.0, 0.8, 1.3])
+    Goct = O.igf_octant(n, hr)
+    G2 = O.mirror_green_periodic_z(n, Goct)
+    assert G2.shape == (5, 8, 6)
+    f = lambda i, m: i if i <= m else 2 * m - i  # noqa: E731
+    for z in range(5):
+        for y in range(8):
+            for x in range(6):
+                assert G2[z, y, x] == Goct[z, f(y, 4), f(x, 3)]
+
+
+def test_periodic_z_solve_equals_brute_force():
+    """Circular convolution on (2n0, 2n1, n2), Green's function built here from
+    the octant, summed directly."""
+    rng = np.random.default_rng(7)
+    n, hr = (4, 5, 6), np.array([1.0, 0.7, 1.4])
+    rho = rng.normal(size=np.prod(n))
+    Goct = O.igf_octant(n, hr)
+    f = lambda i, m: i if i <= m else 2 * m - i  # noqa: E731
+    M = (n[2], 2 * n[1], 2 * n[0])
+    G = np.zeros(M)
+    for z in range(M[0]):
+        for y in range(M[1]):
+            for x in range(M[2]):
+                G[z, y, x] = Goct[z, f(y, n[1]), f(x, n[0])]
+    r3 = rho.reshape(n[2], n[1], n[0])
+    zz, yy, xx = np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")
+    bf = np.zeros_like(r3)
+    for z in range(n[2]):
+        for y in range(n[1]):
+            for x in range(n[0]):
+                bf[z, y, x] = (G[(z - zz) % M[0], (y - yy) % M[1], (x - xx) % M[2]] * r3).sum()
+    phi = O.compute_potential_periodic_z(n, rho, hr)
+    assert rel_l2(phi, bf) < 1e-14
+
+
+@pytest.mark.parametrize("greens", [0, 1])
+def test_periodic_z_point_charge(greens):
+    """A point charge at c: phi(x, y, z) = G(|x-cx|, |y-cy|, (z-cz) mod n2) --
+    z wraps with the one-sided table; x and y see the free-space value."""
+    n, hr = (6, 6, 8), np.array([1.0, 1.2, 0.9])
+    rho = np.zeros(np.prod(n))
+    c = (2, 4, 6)
+    rho[(c[2] * n[1] + c[1]) * n[0] + c[0]] = 1.0
+    phi = O.compute_potential_periodic_z(n, rho, hr, greens).reshape(n[2], n[1], n[0])
+    G = O.igf_octant(n, hr) if greens == 0 else O.green_standard_octant(n, hr)
+    for (x, y, z) in [(2, 4, 6), (5, 0, 0), (0, 2, 7), (3, 3, 1)]:
+        g = G[(z - c[2]) % n[2], abs(y - c[1]), abs(x - c[0])]
+        assert abs(phi[z, y, x] - g) < 1e-13 * np.abs(G).max()
+
+
 def test_compute_potential_matches_numpy_fft():
     rng = np.random.default_rng(6)
     n, hr = (12, 8, 10), np.array([1e-4, 2e-4, 1.5e-4])
