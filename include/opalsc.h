@@ -226,6 +226,25 @@ int opalsc_bunch_get_bins(opalsc_ctx *ctx, int *nbins, int64_t *counts, double *
 int opalsc_track_init(opalsc_ctx *ctx, const opalsc_track_params *p);
 /* Run nsteps steps; per_step (may be NULL) receives nsteps statistics. */
 int opalsc_track(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);
+
+/* --- many independent bunches --------------------------------------------
+ * No counterpart in the reference, which tracks one bunch per run
+ * (ParallelTTracker::execute): a batch steps several independent bunches,
+ * each a context that is already set up (bunch_set/bunch_generate,
+ * optional set_bins, track_init), together.  The contexts must stay alive
+ * while the batch exists and may differ in grid size and precision. */
+typedef struct opalsc_batch opalsc_batch;
+int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, int nctx);
+int opalsc_batch_destroy(opalsc_batch *batch);
+const char *opalsc_batch_last_error(const opalsc_batch *batch);
+/* nsteps steps of every bunch.  use_graph != 0: one step of every bunch is
+ * captured once as a HIP graph (the bunches as parallel branches) and
+ * replayed per step; it is captured again when a context's bunch, bins or
+ * tracker set-up changed.  use_graph = 0: the same kernels launched on the
+ * bunches' streams.  Each step is the one opalsc_track(ctx, 1, ...) runs (its
+ * own first half push, no re-sort), so results equal single-step calls bit
+ * for bit.  per_step (may be NULL) receives [nsteps][nctx] statistics. */
+int opalsc_step_batched(opalsc_batch *batch, int nsteps, opalsc_stats *per_step, int use_graph);
 int opalsc_track_get_params(opalsc_ctx *ctx, opalsc_track_params *out);
 /* Mesh (hr, origin) of the most recent track step or boundp call. */
 int opalsc_get_mesh(opalsc_ctx *ctx, double hr[3], double origin[3]);

@@ -1,5 +1,5 @@
 """MI355X-native space-charge PIC step (OPAL-T drop-in) -- HIP kernels behind a C ABI."""
-from .opalsc import (BC_OPEN, BC_PERIODIC, COUPLING, CLIGHT, EXPORTS, FP64, MIXED,  # noqa: F401
+from .opalsc import (BC_OPEN, BC_PERIODIC, Batch, COUPLING, CLIGHT, EXPORTS, FP64, MIXED,  # noqa: F401
                      GREENS_INTEGRATED,
                      GREENS_STANDARD, GRID_EFIELD, GRID_PHI, GRID_RHO, M_P_EV, FFTPoissonSolver,
                      OpalscError, SpaceChargeContext, host_atan_cr, host_log_cr, lib)

@@ -71,6 +71,7 @@ struct opalsc_ctx {
     Profiler prof;
     double dh = 0.02;
     int steps_done = 0;
+    uint64_t gen = 0;  // bumped whenever buffers or sizes a captured step refers to change
 };
 
 #define FAIL(ctx, code, msg)   \
@@ -409,6 +410,7 @@ extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, co
     HIPCHK(c, hipMemcpyAsync(perm, b.vals2, (size_t)N * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_bunch = c->have_track = false;  // SoA buffers were reused
+    ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
 }
@@ -452,6 +454,7 @@ extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double
     HIPCHK(c, hipMemcpyAsync(Q, b.q, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_bunch = c->have_track = false;  // SoA buffers were reused
+    ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
 }
@@ -541,6 +544,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
     HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_bunch = c->have_track = false;  // SoA buffers were reused
+    ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
 }
@@ -645,6 +649,7 @@ extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const
     c->qmax = qm;
     c->have_bunch = true;
     c->have_track = false;
+    ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
 }
@@ -662,6 +667,7 @@ extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g
     c->qmax = std::fabs(qeach);
     c->have_bunch = true;
     c->have_track = false;
+    ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
 }
@@ -734,6 +740,7 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     CHECK_LAUNCH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_track = true;
+    ++c->gen;
     c->track_sc = p->space_charge != 0;
     c->track_ext = p->Bz != 0.0 || p->Ez0 != 0.0;
     c->steps_done = 0;
@@ -748,6 +755,7 @@ static int bin_bits(int nbins) {  // bits of the values 0 .. nbins
 
 // Reorders every particle array of the resident bunch by perm (b.vals2).
 static void permute_bunch(opalsc_ctx *c) {
+    ++c->gen;  // the arrays trade places with the permutation buffer
     Bunch &b = c->bunch;
     const int64_t N = b.N;
     // one gather pass per array (measured faster than gathering all arrays in one pass)
@@ -784,6 +792,7 @@ extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbin
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "set_bins before bunch_set/bunch_generate");
     if (nbins == 0) {
         c->bins.nbins = 0;
+        ++c->gen;
         return OPALSC_OK;
     }
     if (nbins < 0 || nbins > kMaxBins) FAIL(c, OPALSC_EINVAL, "nbins must be in [0, 64]");
@@ -810,6 +819,7 @@ extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbin
         HIPCHK(c, hipMemcpyAsync(cnt, c->cntb.p, (size_t)nbins * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    ++c->gen;
     c->bins.nbins = nbins;
     c->bins.start[0] = 0;
     for (int k = 0; k < nbins; ++k) c->bins.start[k + 1] = c->bins.start[k] + (int64_t)cnt[k];
@@ -831,90 +841,103 @@ extern "C" int opalsc_bunch_get_bins(opalsc_ctx *c, int *nbins, int64_t *counts,
     return OPALSC_OK;
 }
 
+// Buffers the space-charge step of c needs beyond those of create.
+static int prepare_track(opalsc_ctx *c) {
+    if (c->bins.nbins > 0 && c->track_sc) {
+        if (c->grid.ensure_sums(c->err)) return OPALSC_ENOMEM;
+        if (c->bin_image && c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
+    }
+    (void)gamma_blocks(c->grid.fp64, c->bunch.N);  // occupancy query outside any stream capture
+    return OPALSC_OK;
+}
+
+// One ParallelTTracker step of c on its stream (kernel launches only, so a
+// step can be captured into a graph).  push: the step's first half push
+// (otherwise the previous step's gather pass applied it); next: the gather
+// pass also applies the next step's first half push; sort_now: re-sort the
+// bunch by cell after this step's mesh.  Statistics go to the call's slot
+// ts->step - ts->call_step0 - 1 of statsb.
+static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now) {
+    Bunch &b = c->bunch;
+    const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
+    const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
+    const bool binned = c->bins.nbins > 0;
+    launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1]);
+    c->prof.mark(PROF_SCALAR);
+    if (c->track_sc || push) {
+        launch_track_alpha(c->stream, exact, push, b, c->d_sc, c->d_ts, c->partials);
+        c->prof.mark(push ? PROF_PUSH_BOUNDS : PROF_ROTATE_BOUNDS);
+    }
+    if (c->track_sc) {
+        launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, c->d_sc);
+        c->prof.mark(PROF_SCALAR);
+        if (sort_now) {
+            sort_bunch(c);  // the bounds pass's chunk boxes no longer match the particle order
+            c->prof.mark(PROF_SORT);
+        }
+        if (binned) {
+            // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its image
+            launch_track_bin_scalars(c->stream, b, c->bins, c->bin_qi, exact, c->d_sc, c->binpart.as<double>(),
+                                     c->gamb.as<double>(), c->scb.as<Scalars>());
+            c->grid.clear_sums();
+            c->prof.mark(PROF_SCALAR);
+            for (int k = 0; k < c->bins.nbins; ++k) {
+                const int64_t lo = c->bins.start[k], hi = c->bins.start[k + 1];
+                if (hi == lo) continue;  // an empty bin has no charge
+                const Scalars *sk = c->scb.as<Scalars>() + k;
+                launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+                c->prof.mark(PROF_ZERO);
+                launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
+                                     (long long *)c->grid.rho_i64, sk, &c->d_counters[1],
+                                     sort_now ? nullptr : b.chunkbox);
+                c->prof.mark(PROF_DEPOSIT);
+                c->grid.green(sk, cr);
+                c->grid.solve(sk, 0, 1);
+                c->grid.grad_accum(sk, -1.0);
+                if (c->bin_image) {
+                    c->grid.green_image(sk, cr);
+                    c->grid.solve(sk, 0, 1, true);
+                    c->grid.grad_accum(sk, +1.0);
+                }
+            }
+        } else {
+            launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+            c->prof.mark(PROF_ZERO);
+            launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
+                           c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
+            c->prof.mark(PROF_DEPOSIT);
+            c->grid.green(c->d_sc, cr);
+            c->grid.solve(c->d_sc, 0, 1);
+            c->grid.grad(c->d_sc);
+        }
+    }
+    if (binned && c->track_sc)
+        launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
+                           c->grid.Bsum, c->track_ext);
+    else
+        launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
+                           c->track_ext);
+    c->prof.mark(PROF_GATHER_PUSH);
+    launch_track_post(c->stream, c->partials, gamma_blocks(c->grid.fp64, b.N), c->d_ts, c->d_sc, c->statsb.p,
+                      &c->d_counters[1]);
+    c->prof.mark(PROF_SCALAR);
+}
+
 extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     if (!c || nsteps < 0) return OPALSC_EINVAL;
     if (!c->have_track) FAIL(c, OPALSC_ESTATE, "track before track_init");
     if (nsteps == 0) return OPALSC_OK;
     if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps)) FAIL(c, OPALSC_ENOMEM, "stats buffer");
-    Bunch &b = c->bunch;
-    const int nb = gamma_blocks(c->grid.fp64, b.N);
-    const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
-    const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
-    const bool binned = c->bins.nbins > 0;
-    if (binned && c->track_sc) {
-        if (c->grid.ensure_sums(c->err)) return OPALSC_ENOMEM;
-        if (c->bin_image && c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
-    }
+    if (int e = prepare_track(c)) return e;
     c->prof.start();
+    launch_track_begin_call(c->stream, c->d_ts);
     // Within one call the gather pass of step s also applies step s+1's first
     // half push, so the bounds pass of step s+1 only reads positions.  The
     // last step of a call leaves the bunch at its end-of-step state.
-    bool prepushed = false;
     for (int s = 0; s < nsteps; ++s) {
-        launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1]);
-        c->prof.mark(PROF_SCALAR);
-        if (c->track_sc || !prepushed) {
-            launch_track_alpha(c->stream, exact, !prepushed, b, c->d_sc, c->d_ts, c->partials);
-            c->prof.mark(prepushed ? PROF_ROTATE_BOUNDS : PROF_PUSH_BOUNDS);
-        }
-        if (c->track_sc) {
-            launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact,
-                              c->d_sc);
-            c->prof.mark(PROF_SCALAR);
-            const int interval = c->cfg.sort_interval;
-            bool sorted_now = false;
-            if (interval > 0 && (c->steps_done % interval) == 0) {
-                sort_bunch(c);
-                sorted_now = true;  // the bounds pass's chunk boxes no longer match the particle order
-                c->prof.mark(PROF_SORT);
-            }
-            if (binned) {
-                // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its image
-                launch_track_bin_scalars(c->stream, b, c->bins, c->bin_qi, exact, c->d_sc, c->binpart.as<double>(),
-                                         c->gamb.as<double>(), c->scb.as<Scalars>());
-                c->grid.clear_sums();
-                c->prof.mark(PROF_SCALAR);
-                for (int k = 0; k < c->bins.nbins; ++k) {
-                    const int64_t lo = c->bins.start[k], hi = c->bins.start[k + 1];
-                    if (hi == lo) continue;  // an empty bin has no charge
-                    const Scalars *sk = c->scb.as<Scalars>() + k;
-                    launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
-                    c->prof.mark(PROF_ZERO);
-                    launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
-                                         (long long *)c->grid.rho_i64, sk, &c->d_counters[1],
-                                         sorted_now ? nullptr : b.chunkbox);
-                    c->prof.mark(PROF_DEPOSIT);
-                    c->grid.green(sk, cr);
-                    c->grid.solve(sk, 0, 1);
-                    c->grid.grad_accum(sk, -1.0);
-                    if (c->bin_image) {
-                        c->grid.green_image(sk, cr);
-                        c->grid.solve(sk, 0, 1, true);
-                        c->grid.grad_accum(sk, +1.0);
-                    }
-                }
-            } else {
-                launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
-                c->prof.mark(PROF_ZERO);
-                launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
-                               c->d_sc, &c->d_counters[1], sorted_now ? nullptr : b.chunkbox);
-                c->prof.mark(PROF_DEPOSIT);
-                c->grid.green(c->d_sc, cr);
-                c->grid.solve(c->d_sc, 0, 1);
-                c->grid.grad(c->d_sc);
-            }
-        }
-        prepushed = s + 1 < nsteps;
-        if (binned && c->track_sc)
-            launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
-                               c->grid.Bsum, c->track_ext);
-        else
-            launch_track_gamma(c->stream, c->grid.fp64, prepushed, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials,
-                               nullptr, c->track_ext);
-        c->prof.mark(PROF_GATHER_PUSH);
-        launch_track_post(c->stream, c->partials, nb, c->d_ts, c->d_sc,
-                          (char *)c->statsb.p + sizeof(opalsc_stats) * s, &c->d_counters[1]);
-        c->prof.mark(PROF_SCALAR);
+        const int interval = c->cfg.sort_interval;
+        const bool sort_now = c->track_sc && interval > 0 && (c->steps_done % interval) == 0;
+        enqueue_step(c, s == 0, s + 1 < nsteps, sort_now);
         ++c->steps_done;
         CHECK_LAUNCH(c);
     }
@@ -923,6 +946,163 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
                                  hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->prof.flush();
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// many independent bunches: one graph of one step of every bunch
+// ---------------------------------------------------------------------------
+struct opalsc_batch {
+    std::vector<opalsc_ctx *> ctx;
+    std::vector<uint64_t> gen;         // generations the graph was captured at
+    std::vector<void *> stats_ptr;     // statistics buffers it writes
+    hipStream_t master = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+};
+
+#define BFAIL(b, code, msg) \
+    do {                    \
+        (b)->err = (msg);   \
+        return (code);      \
+    } while (0)
+#define BHIP(b, call)                                                                                         \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess) BFAIL(b, OPALSC_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+static void batch_drop_graph(opalsc_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+extern "C" int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, int n) {
+    if (!out || !ctxs || n < 1) return OPALSC_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return OPALSC_EINVAL;
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return OPALSC_EINVAL;  // each bunch once
+    }
+    opalsc_batch *b = new (std::nothrow) opalsc_batch();
+    if (!b) return OPALSC_ENOMEM;
+    b->ctx.assign(ctxs, ctxs + n);
+    b->gen.assign(n, ~(uint64_t)0);
+    b->stats_ptr.assign(n, nullptr);
+    b->join.assign(n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < n; ++i) ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        if (b->master) (void)hipStreamDestroy(b->master);
+        if (b->fork) (void)hipEventDestroy(b->fork);
+        for (hipEvent_t e : b->join)
+            if (e) (void)hipEventDestroy(e);
+        delete b;
+        return OPALSC_EHIP;
+    }
+    *out = b;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_batch_destroy(opalsc_batch *b) {
+    if (!b) return OPALSC_OK;
+    if (b->master) (void)hipStreamSynchronize(b->master);
+    batch_drop_graph(b);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->master) (void)hipStreamDestroy(b->master);
+    delete b;
+    return OPALSC_OK;
+}
+
+extern "C" const char *opalsc_batch_last_error(const opalsc_batch *b) { return b ? b->err.c_str() : ""; }
+
+// Captures one step of every bunch (push, no fused next push, no sort) as the
+// parallel branches of one graph: each bunch's kernels on its own stream,
+// forked from and joined back into the batch's stream.
+static int batch_capture(opalsc_batch *b) {
+    batch_drop_graph(b);
+    const int n = (int)b->ctx.size();
+    BHIP(b, hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipEventRecord(b->fork, b->master);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        opalsc_ctx *c = b->ctx[i];
+        e = hipStreamWaitEvent(c->stream, b->fork, 0);
+        if (e != hipSuccess) break;
+        enqueue_step(c, true, false, false);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(b->join[i], c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->master, b->join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->master, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        BFAIL(b, OPALSC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    for (int i = 0; i < n; ++i) {
+        b->gen[i] = b->ctx[i]->gen;
+        b->stats_ptr[i] = b->ctx[i]->statsb.p;
+    }
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *per_step, int use_graph) {
+    if (!b || nsteps < 0) return OPALSC_EINVAL;
+    if (nsteps == 0) return OPALSC_OK;
+    const int n = (int)b->ctx.size();
+    bool recapture = !b->exec;
+    for (int i = 0; i < n; ++i) {
+        opalsc_ctx *c = b->ctx[i];
+        if (!c->have_track) BFAIL(b, OPALSC_ESTATE, "bunch " + std::to_string(i) + ": step before track_init");
+        if (c->prof.enabled) BFAIL(b, OPALSC_ESTATE, "bunch " + std::to_string(i) + ": profiling is on");
+        if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps))
+            BFAIL(b, OPALSC_ENOMEM, "bunch " + std::to_string(i) + ": stats buffer");
+        if (prepare_track(c) != OPALSC_OK) BFAIL(b, OPALSC_ENOMEM, "bunch " + std::to_string(i) + ": " + c->err);
+        recapture |= b->gen[i] != c->gen || b->stats_ptr[i] != c->statsb.p;
+    }
+    // every bunch starts its call on its own stream; the batch's stream waits for them
+    for (int i = 0; i < n; ++i) {
+        opalsc_ctx *c = b->ctx[i];
+        launch_track_begin_call(c->stream, c->d_ts);
+        BHIP(b, hipGetLastError());
+        BHIP(b, hipEventRecord(b->join[i], c->stream));
+        BHIP(b, hipStreamWaitEvent(b->master, b->join[i], 0));
+    }
+    if (use_graph) {
+        if (recapture)
+            if (int e = batch_capture(b)) return e;
+        for (int s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->exec, b->master));
+        // later work on every bunch's stream follows the graph
+        BHIP(b, hipEventRecord(b->fork, b->master));
+        for (int i = 0; i < n; ++i) BHIP(b, hipStreamWaitEvent(b->ctx[i]->stream, b->fork, 0));
+    } else {
+        for (int s = 0; s < nsteps; ++s)
+            for (int i = 0; i < n; ++i) {
+                enqueue_step(b->ctx[i], true, false, false);
+                BHIP(b, hipGetLastError());
+            }
+    }
+    for (int i = 0; i < n; ++i) {
+        opalsc_ctx *c = b->ctx[i];
+        c->steps_done += nsteps;
+        BHIP(b, hipStreamSynchronize(c->stream));
+        if (per_step) {
+            std::vector<opalsc_stats> st((size_t)nsteps);
+            BHIP(b, hipMemcpy(st.data(), c->statsb.p, sizeof(opalsc_stats) * (size_t)nsteps, hipMemcpyDeviceToHost));
+            for (int s = 0; s < nsteps; ++s) per_step[(size_t)s * n + i] = st[s];
+        }
+    }
     return OPALSC_OK;
 }
 

@@ -63,6 +63,7 @@ struct TrackState {
     double RefPartR[3], RefPartP[3];
     int space_charge;
     int step;
+    int call_step0;  // step at the start of the current tracking call (statistics slot = step - call_step0 - 1)
 };
 
 // ---- cloud-in-cell lookup -------------------------------------------------

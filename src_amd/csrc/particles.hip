@@ -1343,9 +1343,13 @@ __global__ __launch_bounds__(512) void k_track_post(const double *partials, int 
         s.t = ts->t;
         s.pathLength = ts->pathLength;
         s.dropped = dropped ? (int64_t)*dropped : 0;
-        *out = s;
+        out[ts->step - ts->call_step0 - 1] = s;  // the step's slot, so that a captured step replays unchanged
     }
 }
+__global__ void k_track_begin_call(TrackState *ts) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ts->call_step0 = ts->step;
+}
+void launch_track_begin_call(hipStream_t s, TrackState *ts) { k_track_begin_call<<<1, 64, 0, s>>>(ts); }
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped) {
     k_track_post<<<1, kReduceThreads, 0, s>>>(partials, nb, ts, sc, (StatsOut *)stats_out, dropped);

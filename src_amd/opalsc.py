@@ -38,7 +38,8 @@ EXPORTS = [
     "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_compute_self_fields_binned",
     "opalsc_get_grid", "opalsc_push_half",
     "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
-    "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init", "opalsc_track", "opalsc_track_get_params",
+    "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init",
+    "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
     "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
     "opalsc_copy_bandwidth",
@@ -116,6 +117,10 @@ def lib():
         "opalsc_bunch_get_bins": ([vp, C.POINTER(C.c_int), dp, dp], C.c_int),
         "opalsc_track_init": ([vp, C.POINTER(TrackParams)], C.c_int),
         "opalsc_track": ([vp, C.c_int, dp], C.c_int),
+        "opalsc_batch_create": ([C.POINTER(vp), C.POINTER(vp), C.c_int], C.c_int),
+        "opalsc_batch_destroy": ([vp], C.c_int),
+        "opalsc_batch_last_error": ([vp], C.c_char_p),
+        "opalsc_step_batched": ([vp, C.c_int, dp, C.c_int], C.c_int),
         "opalsc_track_get_params": ([vp, C.POINTER(TrackParams)], C.c_int),
         "opalsc_get_mesh": ([vp, dp, dp], C.c_int),
         "opalsc_host_atan_cr": ([i64, dp, dp], C.c_int),
@@ -396,6 +401,45 @@ class SpaceChargeContext:
         launches = np.zeros(32, np.int64)
         n = lib().opalsc_profile_read(self._h, 32, names, _ptr(ms), _ptr(launches))
         return {names[i].decode(): (float(ms[i]), int(launches[i])) for i in range(n)}
+
+
+class Batch:
+    """Several independent bunches stepped together (opalsc_step_batched).
+    The contexts must be set up for tracking and outlive the batch."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        h = C.c_void_p()
+        rc = lib().opalsc_batch_create(C.byref(h), arr, len(self.contexts))
+        if rc != 0:
+            raise OpalscError("opalsc_batch_create failed: %d" % rc)
+        self._h = h
+
+    def step(self, nsteps, graph=True, want_stats=True):
+        n = len(self.contexts)
+        stats = (Stats * max(nsteps * n, 1))() if want_stats else None
+        rc = lib().opalsc_step_batched(self._h, int(nsteps), C.cast(stats, C.c_void_p) if want_stats else None,
+                                       1 if graph else 0)
+        if rc != 0:
+            msg = lib().opalsc_batch_last_error(self._h)
+            raise OpalscError("opalsc error %d: %s" % (rc, msg.decode() if msg else ""))
+        if not want_stats:
+            return None
+        return [[stats_to_dict(stats[s * n + i]) for i in range(n)] for s in range(nsteps)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().opalsc_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class FFTPoissonSolver:

@@ -445,3 +445,72 @@ def test_periodic_z_context_rejects_particle_entry_points():
             c.track_init(1e-11)
     with pytest.raises(sc.OpalscError, match="periodic z"):
         sc.SpaceChargeContext((8, 8, 6), max_particles=0, bcz=sc.BC_PERIODIC)
+
+
+def _batch_setups():
+    """Four independent bunches: different sizes, grids, precisions, one with energy bins."""
+    out = []
+    specs = [(3000, (16, 16, 16), sc.FP64, 0.0, 0), (20000, (32, 32, 32), sc.MIXED, 0.5, 0),
+             (5000, (12, 24, 16), sc.FP64, 1.0, 3), (60000, (16, 32, 24), sc.MIXED, 0.0, 0)]
+    for k, (N, n, prec, Bz, nbins) in enumerate(specs):
+        R, P, Q = gaussian_bunch(N, seed=40 + k, q_total=1e-9 * (k + 1))
+        bins = None
+        if nbins:
+            R[:, 2] += 0.005
+            bins = geometric_energy_bins(P, nbins)
+        out.append(dict(N=N, n=n, prec=prec, Bz=Bz, R=R, P=P, Q=Q, bins=bins, nbins=nbins))
+    return out
+
+
+def _make_batch_ctx(u):
+    c = sc.SpaceChargeContext(u["n"], precision=u["prec"], max_particles=u["N"])
+    c.set_bunch(u["R"], u["P"], u["Q"])
+    if u["nbins"]:
+        c.set_bins(u["bins"], u["nbins"], 1e-9 / u["N"], image=True)
+    c.track_init(1e-11, Bz=u["Bz"], ref_p=(0.0, 0.0, 0.046181))
+    return c
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_single_steps(graph):
+    """A batched step is opalsc_track(ctx, 1): same bunch and statistics bit for bit."""
+    setups = _batch_setups()
+    steps = 4
+    ref = []
+    for u in setups:
+        c = _make_batch_ctx(u)
+        st = [c.track(1)[0] for _ in range(steps)]
+        ref.append((c.get_bunch(), st))
+        c.close()
+    ctxs = [_make_batch_ctx(u) for u in setups]
+    with sc.Batch(ctxs) as b:
+        st1 = b.step(2, graph=graph)
+        st2 = b.step(2, graph=graph)  # the second call replays the captured graph
+    sts = st1 + st2
+    for i, (c, (bunch_ref, st_ref)) in enumerate(zip(ctxs, ref)):
+        R, P, Q = c.get_bunch()
+        assert np.array_equal(R, bunch_ref[0]) and np.array_equal(P, bunch_ref[1]), i
+        for s in range(steps):
+            for k in ("rrms", "prms", "eps_norm", "rmean", "pmean"):
+                assert np.array_equal(sts[s][i][k], st_ref[s][k]), (i, s, k)
+        c.close()
+
+
+def test_batch_recaptures_after_a_new_bunch():
+    setups = _batch_setups()[:2]
+    ctxs = [_make_batch_ctx(u) for u in setups]
+    with sc.Batch(ctxs) as b:
+        b.step(1)
+        # a new bunch in context 0 (other size): the graph must not replay stale buffers
+        R, P, Q = gaussian_bunch(1500, seed=99, q_total=1e-9)
+        ctxs[0].set_bunch(R, P, Q)
+        ctxs[0].track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+        st = b.step(2)
+    c = sc.SpaceChargeContext(setups[0]["n"], precision=setups[0]["prec"], max_particles=3000)
+    c.set_bunch(R, P, Q)
+    c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+    ref = [c.track(1)[0] for _ in range(2)]
+    assert np.array_equal(ctxs[0].get_bunch()[0], c.get_bunch()[0])
+    assert np.array_equal(st[1][0]["eps_norm"], ref[1]["eps_norm"])
+    for x in ctxs + [c]:
+        x.close()
