@@ -125,6 +125,61 @@ def run_c5(sc, steps, warmup, sort_interval, log):
             "kernels": {k: {"avg_ms": ms_ / nl, "launches": nl} for k, (ms_, nl) in prof.items() if nl}}
 
 
+def run_c4(sc, steps, warmup, log):
+    """64 independent bunches (particle counts log-uniform in [1e4, 1e6], grids
+    from {16, 32, 48, 64}^3, sigma uniform in [0.5, 2] mm, 1-100 MeV protons,
+    per-bunch seeds, mixed precision), stepped one bunch after another with
+    opalsc_track, then together with opalsc_step_batched without and with the
+    HIP graph."""
+    rng = np.random.default_rng(4)
+    specs = []
+    for k in range(64):
+        N = int(10 ** rng.uniform(4, 6))
+        n = int(rng.choice([16, 32, 48, 64]))
+        sig = float(rng.uniform(0.5e-3, 2e-3))
+        _, bg = gamma_betagamma(float(rng.uniform(1.0, 100.0)))
+        specs.append((N, n, sig, bg))
+    total = sum(s[0] for s in specs)
+
+    def make():
+        cs = []
+        for k, (N, n, sig, bg) in enumerate(specs):
+            c = sc.SpaceChargeContext((n, n, n), precision=sc.MIXED, max_particles=N)
+            c.generate_bunch(N, 1000 + k, (sig, sig, sig), (1e-3, 1e-3, 1e-3), bg, 1e-9)
+            c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, ref_p=(0.0, 0.0, bg))
+            cs.append(c)
+        return cs
+
+    from src_amd.opalsc import lib
+    out = {"workload": "64 bunches, %d particles in total (1e4-1e6 each), grids 16/32/48/64^3, mixed precision"
+                       % total, "steps": steps}
+    cs = make()
+    for c in cs:
+        c.track(warmup, want_stats=False)
+    t0 = time.perf_counter()
+    for c in cs:
+        c.track(steps, want_stats=False)
+    t_seq = time.perf_counter() - t0
+    for c in cs:
+        c.close()
+    for mode, graph in (("batched_streams", False), ("batched_graph", True)):
+        cs = make()
+        with sc.Batch(cs) as b:
+            b.step(warmup, graph=graph, want_stats=False)
+            t0 = time.perf_counter()
+            b.step(steps, graph=graph, want_stats=False)
+            t = time.perf_counter() - t0
+        for c in cs:
+            c.close()
+        out[mode + "_ms_per_step"] = 1e3 * t / steps
+        out[mode + "_particle_steps_per_s"] = total * steps / t
+    out["sequential_ms_per_step"] = 1e3 * t_seq / steps
+    out["sequential_particle_steps_per_s"] = total * steps / t_seq
+    log("[bench] c4: sequential %.2f ms/step, batched streams %.2f, batched graph %.2f" %
+        (out["sequential_ms_per_step"], out["batched_streams_ms_per_step"], out["batched_graph_ms_per_step"]))
+    return out
+
+
 def algorithmic_bytes(cfg, wbytes):
     """Per-launch algorithmic HBM bytes of each kernel class (one step)."""
     N, n = cfg["N"], cfg["n"]
@@ -222,15 +277,21 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the energy-binned measurement")
     ap.add_argument("--c5-only", action="store_true",
                     help="run only the energy-binned measurement and print its record (for profiling)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 64-bunch batched measurement")
+    ap.add_argument("--c4-only", action="store_true",
+                    help="run only the 64-bunch batched measurement and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the space-charge step of one "
               "bunch on one MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import src_amd as sc
-    if args.c5_only:
+    if args.c5_only or args.c4_only:
         log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
-        print(json.dumps(run_c5(sc, args.steps, args.warmup, args.sort_interval, log)))
+        if args.c5_only:
+            print(json.dumps(run_c5(sc, args.steps, args.warmup, args.sort_interval, log)))
+        else:
+            print(json.dumps(run_c4(sc, args.steps, args.warmup, log)))
         return
     cfg = CONFIGS[args.config]
     N, n = cfg["N"], cfg["n"]
@@ -306,6 +367,9 @@ def main():
     if not args.no_c5:
         log("[bench] energy-binned step (5e7 particles, 10 bins, 128^3 + image per bin)")
         extra["c5_binned"] = run_c5(sc, args.steps, args.warmup, args.sort_interval, log)
+    if not args.no_c4:
+        log("[bench] 64 independent bunches, batched")
+        extra["c4_batched"] = run_c4(sc, args.steps, args.warmup, log)
 
     cpu = None
     if not args.no_cpu:
