@@ -190,8 +190,9 @@ def algorithmic_bytes(cfg, wbytes):
         "push_rotate_bounds": N * 72,
         # later steps (the previous gather pass applied the half push): read x,y,z
         "rotate_bounds": N * 24,
-        # read x,y,z,q; read+write int64 rho once
-        "deposit": N * 32 + n3 * 16,
+        # read x,y,z (the synthetic bunch's particles all carry the same charge,
+        # kept as one number); read+write int64 rho once
+        "deposit": N * 24 + n3 * 16,
         # read+write x,y,z,px,py,pz; read the E grid (4 words per cell) once
         "gather_kick_push": N * 96 + n3 * 4 * wbytes,
         # x pass: int64 rho -> A, and A -> phi (two launches; per-launch average)
@@ -208,7 +209,7 @@ def algorithmic_bytes(cfg, wbytes):
 # profiler kernel class -> kernel symbol of the mixed-precision step
 KERNEL_SYMBOL = {
     "gather_kick_push": "k_track_gamma_mixed<true,false,false>",
-    "deposit": "k_deposit_lds<1,false,true>",
+    "deposit": "k_deposit_lds<3,false,true>",
     "rotate_bounds": "k_track_alpha<false,false>",
     "push_rotate_bounds": "k_track_alpha<false,true>",
     "fft_z_conv": "k_strided<float,16,2>",

@@ -65,6 +65,8 @@ struct opalsc_ctx {
     DevBuf binpart;           // calcGammas partial sums
     DevBuf scb;               // Scalars per bin
     double qmax = 0.0;        // max |Q| of the resident bunch
+    bool q_uniform = false;   // every particle of the resident bunch carries the charge q_value:
+    double q_value = 0.0;     // the tracker then keeps Q as that one number (no per-particle array)
     bool have_bunch = false, have_track = false;
     bool track_sc = false;    // the tracker computes space-charge fields
     bool track_ext = false;   // the tracker has a nonzero external field
@@ -645,8 +647,14 @@ extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const
         c->aosP.release();
     }
     double qm = 0.0;
-    for (int64_t i = 0; i < N; ++i) qm = std::max(qm, std::fabs(Q[i]));
+    bool uni = true;
+    for (int64_t i = 0; i < N; ++i) {
+        qm = std::max(qm, std::fabs(Q[i]));
+        uni = uni && Q[i] == Q[0];
+    }
     c->qmax = qm;
+    c->q_uniform = uni && N > 0;
+    c->q_value = N > 0 ? Q[0] : 0.0;
     c->have_bunch = true;
     c->have_track = false;
     ++c->gen;
@@ -665,6 +673,8 @@ extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g
     CHECK_LAUNCH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->qmax = std::fabs(qeach);
+    c->q_uniform = true;
+    c->q_value = qeach;
     c->have_bunch = true;
     c->have_track = false;
     ++c->gen;
@@ -698,8 +708,17 @@ extern "C" int opalsc_bunch_get(opalsc_ctx *c, double *R, double *P, double *Q) 
         if (int e = fetch(b.py, P, 3, 1)) return e;
         if (int e = fetch(b.pz, P, 3, 2)) return e;
     }
-    if (Q)
+    if (Q && c->q_uniform) {
+        double qv = c->q_value;
+        if (c->have_track) {
+            Scalars s;
+            if (int e = download_scalars(c, s)) return e;
+            qv = s.qu;
+        }
+        for (int64_t i = 0; i < N; ++i) Q[i] = qv;
+    } else if (Q) {
         if (int e = fetch(b.q, Q, 1, 0)) return e;
+    }
     return OPALSC_OK;
 }
 
@@ -732,6 +751,7 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     for (int d = 0; d < 3; ++d) s.n[d] = c->cfg.n[d];
     s.N = c->bunch.N;
     s.qd_absmax = c->qmax * p->dT * (1.0 + 0x1p-50);
+    s.qu = c->q_value;
     s.dT = p->dT;
     s.coupling = p->coupling;
     upload_scalars(c, s);
@@ -761,6 +781,7 @@ static void permute_bunch(opalsc_ctx *c) {
     // one gather pass per array (measured faster than gathering all arrays in one pass)
     double **arrs[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz, &b.q};
     for (double **a : arrs) {
+        if (*a == b.q && c->q_uniform) continue;  // the charge is one number
         launch_permute(c->stream, N, b.vals2, *a, b.tmp);
         std::swap(*a, b.tmp);
     }
@@ -869,7 +890,8 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now) {
         c->prof.mark(push ? PROF_PUSH_BOUNDS : PROF_ROTATE_BOUNDS);
     }
     if (c->track_sc) {
-        launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, c->d_sc);
+        launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, !binned,
+                          c->d_sc);
         c->prof.mark(PROF_SCALAR);
         if (sort_now) {
             sort_bunch(c);  // the bounds pass's chunk boxes no longer match the particle order
@@ -903,8 +925,8 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now) {
         } else {
             launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
             c->prof.mark(PROF_ZERO);
-            launch_deposit(c->stream, 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr, (long long *)c->grid.rho_i64,
-                           c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
+            launch_deposit(c->stream, c->q_uniform ? 3 : 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr,
+                           (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
             c->prof.mark(PROF_DEPOSIT);
             c->grid.green(c->d_sc, cr);
             c->grid.solve(c->d_sc, 0, 1);

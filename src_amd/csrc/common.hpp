@@ -51,6 +51,7 @@ struct Scalars {
     double o_rb2[3], ih[3], M_c[9], o_c[3], o_c2[3], Eext_b[3], Bext_b[3];
     double zshift;             // image-charge Green's function shift (FFTPoissonSolver.cpp:289)
     double qi;                 // energy bins: macro charge of the bin's particles (setBinCharge)
+    double qu, qd_u;           // tracker, every particle of charge qu: qu and this step's qu*dT
     int mesh_ok;
     int64_t N;
 };

@@ -162,8 +162,9 @@ void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, uns
 int alpha_blocks(int64_t N);
 void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Scalars *sc,
                         const TrackState *ts, double *partials);
+// q_roundtrip: the step scatters the bunch's own charge (plain, not binned): Q*dT, then /dT
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
-                       const TrackState *ts, bool exact, Scalars *sc);
+                       const TrackState *ts, bool exact, bool q_roundtrip, Scalars *sc);
 int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field

@@ -181,10 +181,12 @@ __device__ inline int small_div(int a, int b, float rb) {
 }
 constexpr int kBoxCap = 5632;  // 44 KiB of int64
 
-// MODE 0: qd given; 1 (tracker): qd = Q*dT with Q left as (Q*dT)/dT, positions
+// MODE 0: qd given; 1 (tracker): qd = Q*dT with Q left as (Q*dT)/dT (FP64 path), positions
 // given in the reference frame and rotated into the beam frame (M_rb, o_rb)
 // exactly as the bounds pass rotated them; 2 (tracker, energy bin): as 1 with
-// qd = qi*dT for every particle (setBinCharge) and Q untouched.  EXACT: the
+// qd = qi*dT for every particle (setBinCharge) and Q untouched; 3 (tracker, a
+// bunch whose particles all carry the same Q): as 1 with qd = Scalars::qd_u,
+// the charge kept as one number (Scalars::qu).  EXACT: the
 // FP64 path (reference-order CIC, 64-bit fixed point per deposit); otherwise
 // the mixed path (CIC with 1/h, FP32 weights, |deposit| < 2^24 in 32-bit
 // integers, summed in 64 bits).  BOX: chunk statistics supplied by the bounds
@@ -214,9 +216,12 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
     const int64_t p0 = base + (int64_t)threadIdx.x * kPPT;
     const double *qsrc = MODE == 0 ? qdv : q;
     double px[kPPT], py[kPPT], pz[kPPT], pq[kPPT];
-    if (MODE == 2) {
-        double qb = sc->qi;  // Q = qi (setBinCharge), Q *= dt (PartBunch.cpp:154)
-        qb *= dT;
+    if (MODE >= 2) {
+        double qb = sc->qd_u;
+        if (MODE == 2) {
+            qb = sc->qi;  // Q = qi (setBinCharge), Q *= dt (PartBunch.cpp:154)
+            qb *= dT;
+        }
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) pq[k] = qb;
     }
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
             py[2 * m + 1] = ty.y;
             pz[2 * m] = tz.x;
             pz[2 * m + 1] = tz.y;
-            if constexpr (MODE != 2) {
+            if constexpr (MODE < 2) {
                 const double2 tq = ((const double2 *)(qsrc + p0))[m];
                 pq[2 * m] = tq.x;
                 pq[2 * m + 1] = tq.y;
@@ -245,7 +250,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
             px[k] = in ? x[p] : 0.0;
             py[k] = in ? y[p] : 0.0;
             pz[k] = in ? z[p] : 0.0;
-            if constexpr (MODE != 2) pq[k] = in ? qsrc[p] : 0.0;
+            if constexpr (MODE < 2) pq[k] = in ? qsrc[p] : 0.0;
         }
     }
     if constexpr (MODE != 0) {
@@ -404,8 +409,12 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
             } else {
                 const double qv = pq[k];
                 qd = qv * dT;
-                const double qn = qd / dT;
-                if (qn != qv) q[p] = qn;
+                if constexpr (EXACT) {
+                    // the FP64 path leaves Q as (Q*dT)/dT like the reference; the
+                    // mixed path keeps Q (they differ by at most an ulp)
+                    const double qn = qd / dT;
+                    if (qn != qv) q[p] = qn;
+                }
             }
             double d[3];
             if constexpr (EXACT) {
@@ -509,13 +518,21 @@ void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64
             if (exact) OPALSC_DEP(1, true, false);
             else OPALSC_DEP(1, false, false);
         }
-    } else {
+    } else if (mode == 2) {
         if (box) {
             if (exact) OPALSC_DEP(2, true, true);
             else OPALSC_DEP(2, false, true);
         } else {
             if (exact) OPALSC_DEP(2, true, false);
             else OPALSC_DEP(2, false, false);
+        }
+    } else {
+        if (box) {
+            if (exact) OPALSC_DEP(3, true, true);
+            else OPALSC_DEP(3, false, true);
+        } else {
+            if (exact) OPALSC_DEP(3, true, false);
+            else OPALSC_DEP(3, false, false);
         }
     }
 #undef OPALSC_DEP
@@ -1055,7 +1072,7 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
 
 __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int nb, int n0, int n1,
                                                     int n2, double dh, const TrackState *ts, int exact,
-                                                    Scalars *sc) {
+                                                    int q_roundtrip, Scalars *sc) {
     __shared__ double lds[16 * 6];
     double mn[3], mx[3];
     reduce_partials_minmax(partials, nb, mn, mx, lds);
@@ -1075,13 +1092,21 @@ __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int 
         sc->ih[d] = 1.0 / sc->hr[d];
     }
     const int64_t N = sc->N;
+    // a bunch of equal charges: Q *= dt for the scatter, Q /= dt after it
+    // (PartBunch.cpp:493-495), once for all particles; the mixed path keeps Q,
+    // and so does the energy-binned step (its loop restores Q)
+    if (q_roundtrip) {
+        sc->qd_u = sc->qu * ts->dT;
+        if (exact) sc->qu = sc->qd_u / ts->dT;
+    }
     selffield_scalars(sc, sc->psum[2], N, ts->dT, ts->coupling);
     // every particle deposits |Q*dT| <= max|Q|*dT*(1+2^-52)
     set_fixed_point(sc, N, sc->qd_absmax, exact != 0);
 }
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
-                       const TrackState *ts, bool exact, Scalars *sc) {
-    k_track_mesh<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, exact ? 1 : 0, sc);
+                       const TrackState *ts, bool exact, bool q_roundtrip, Scalars *sc) {
+    k_track_mesh<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, ts, exact ? 1 : 0,
+                                              q_roundtrip ? 1 : 0, sc);
 }
 
 // Rotation into the beam frame (the bounds pass's, repeated bit for bit),

@@ -514,3 +514,42 @@ def test_batch_recaptures_after_a_new_bunch():
     assert np.array_equal(st[1][0]["eps_norm"], ref[1]["eps_norm"])
     for x in ctxs + [c]:
         x.close()
+
+
+@pytest.mark.parametrize("precision,tol_r,tol_s", [(sc.FP64, 1e-12, 1e-10), (sc.MIXED, 1e-9, 1e-6)])
+def test_tracker_with_unequal_charges(precision, tol_r, tol_s):
+    """Per-particle macro charges (the deposit reads Q and, on the FP64 path,
+    leaves it as (Q*dT)/dT); bunches of equal charges keep Q as one number."""
+    n, N, steps = (32, 32, 32), 80000, 4
+    R, P, Q = gaussian_bunch(N, seed=12, q_total=1e-9)
+    Q = Q * np.random.default_rng(13).uniform(0.7, 1.3, N)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5)
+    Ro, Po, Qo, _, so = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, precision=precision, N=N, sort_interval=2) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5)
+        sg = c.track(steps)
+        Rg, Pg, Qg = c.get_bunch()
+    assert rel_l2(Rg, Ro) < tol_r
+    if precision == sc.FP64:
+        assert np.array_equal(Qg, Qo)
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm"):
+            assert np.allclose(a[k], b[k], rtol=tol_s, atol=0), (k, a[k], b[k])
+
+
+def test_tracker_equal_charges_round_trip_like_the_reference():
+    """A bunch of equal charges: Q after tracking equals the oracle's per-particle (Q*dT)/dT."""
+    n, N, steps = (16, 16, 16), 20000, 3
+    R, P, Q = gaussian_bunch(N, seed=14, q_total=3e-9)
+    Q[:] = 8.294255678822374e-13  # (Q*dT)/dT rounds to the next double
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181))
+    _, _, Qo, _, _ = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, N=N) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+        c.track(steps)
+        _, _, Qg = c.get_bunch()
+    assert np.array_equal(Qg, Qo)
