@@ -396,6 +396,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
             }
         }
     };
+#pragma unroll
     for (int k = 0; k < kPPT; ++k) {
         const int64_t p = p0 + k;
         V v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
