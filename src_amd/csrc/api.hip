@@ -49,6 +49,8 @@ struct opalsc_ctx {
     opalsc_config cfg;
     std::string err;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // the tracker's Green's function, concurrent with the deposit
+    hipEvent_t ev_mesh = nullptr, ev_green = nullptr;
     PoissonGrid grid;
     Bunch bunch;
     Scalars *d_sc = nullptr;
@@ -253,6 +255,10 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     for (void *p : ps)
         if (p) (void)hipFree(p);
     c->prof.destroy();
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->ev_mesh) (void)hipEventDestroy(c->ev_mesh);
+    if (c->ev_green) (void)hipEventDestroy(c->ev_green);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OPALSC_OK;
@@ -877,8 +883,10 @@ static int prepare_track(opalsc_ctx *c) {
 // (otherwise the previous step's gather pass applied it); next: the gather
 // pass also applies the next step's first half push; sort_now: re-sort the
 // bunch by cell after this step's mesh.  Statistics go to the call's slot
-// ts->step - ts->call_step0 - 1 of statsb.
-static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now) {
+// ts->step - ts->call_step0 - 1 of statsb.  overlap: the Green's function may
+// run on the context's second stream (a single bunch; a batch already runs
+// bunches side by side).
+static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, bool overlap) {
     Bunch &b = c->bunch;
     const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
@@ -923,13 +931,24 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now) {
                 }
             }
         } else {
+            // The Green's function needs only the mesh: without profiling it runs
+            // on the second stream, beside the deposit and the x and y passes,
+            // and the z pass waits for it.
+            // (on small grids the extra synchronisation costs more than it hides)
+            overlap = overlap && !c->prof.enabled && c->stream2 && c->grid.ncell >= (size_t)96 * 96 * 96;
+            if (overlap) {
+                (void)hipEventRecord(c->ev_mesh, c->stream);
+                (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
+                c->grid.green(c->d_sc, cr, c->stream2);
+                (void)hipEventRecord(c->ev_green, c->stream2);
+            }
             launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
             c->prof.mark(PROF_ZERO);
             launch_deposit(c->stream, c->q_uniform ? 3 : 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr,
                            (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
             c->prof.mark(PROF_DEPOSIT);
-            c->grid.green(c->d_sc, cr);
-            c->grid.solve(c->d_sc, 0, 1);
+            if (!overlap) c->grid.green(c->d_sc, cr);
+            c->grid.solve(c->d_sc, 0, 1, false, overlap ? c->ev_green : nullptr);
             c->grid.grad(c->d_sc);
         }
     }
@@ -951,6 +970,14 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     if (nsteps == 0) return OPALSC_OK;
     if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps)) FAIL(c, OPALSC_ENOMEM, "stats buffer");
     if (int e = prepare_track(c)) return e;
+    // the second stream is made on first use: streams are spread over the
+    // device's hardware queues in creation order, and contexts that are only
+    // ever stepped in a batch keep one stream each
+    if (!c->stream2) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_mesh, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_green, hipEventDisableTiming));
+    }
     c->prof.start();
     launch_track_begin_call(c->stream, c->d_ts);
     // Within one call the gather pass of step s also applies step s+1's first
@@ -959,7 +986,7 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     for (int s = 0; s < nsteps; ++s) {
         const int interval = c->cfg.sort_interval;
         const bool sort_now = c->track_sc && interval > 0 && (c->steps_done % interval) == 0;
-        enqueue_step(c, s == 0, s + 1 < nsteps, sort_now);
+        enqueue_step(c, s == 0, s + 1 < nsteps, sort_now, true);
         ++c->steps_done;
         CHECK_LAUNCH(c);
     }
@@ -1059,7 +1086,7 @@ static int batch_capture(opalsc_batch *b) {
         opalsc_ctx *c = b->ctx[i];
         e = hipStreamWaitEvent(c->stream, b->fork, 0);
         if (e != hipSuccess) break;
-        enqueue_step(c, true, false, false);
+        enqueue_step(c, true, false, false, false);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(b->join[i], c->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(b->master, b->join[i], 0);
@@ -1111,7 +1138,7 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
     } else {
         for (int s = 0; s < nsteps; ++s)
             for (int i = 0; i < n; ++i) {
-                enqueue_step(b->ctx[i], true, false, false);
+                enqueue_step(b->ctx[i], true, false, false, false);
                 BHIP(b, hipGetLastError());
             }
     }

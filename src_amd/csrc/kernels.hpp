@@ -74,13 +74,14 @@ struct PoissonGrid {
 
     int init(const int nn[3], bool fp64, int greens, bool periodic_z, hipStream_t s, std::string &err);
     void destroy();
-    void green(const Scalars *d_sc, bool correctly_rounded);
+    void green(const Scalars *d_sc, bool correctly_rounded, hipStream_t s = nullptr);  // s: default stream
     int ensure_image(std::string &err);
     void green_image(const Scalars *d_sc, bool correctly_rounded);  // shift: Scalars::zshift
     // src_kind 0: int64 fixed-point rho_i64 (self-field scaling); 1: rho_f64 as is
     // outscale 1: phi *= vol*coupling (self fields); 0: potential only
     // image: convolve with the image-charge Green's function (negated, z flipped)
-    void solve(const Scalars *d_sc, int src_kind, int outscale, bool image = false);
+    // green_ready: the z pass waits for it (Green's function computed on another stream)
+    void solve(const Scalars *d_sc, int src_kind, int outscale, bool image = false, hipEvent_t green_ready = nullptr);
     void grad(const Scalars *d_sc);
     int ensure_sums(std::string &err);
     void clear_sums();

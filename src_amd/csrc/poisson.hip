@@ -721,7 +721,7 @@ void PoissonGrid::destroy() {
 
 // DCT-I along x (octant G, FP64, pitch n0+1 -> H, pitch gp) and along y, in place in H.
 template <typename T>
-static void dct_xy(PoissonGrid &g, const double *G, T *H) {
+static void dct_xy(PoissonGrid &g, const double *G, T *H, hipStream_t st) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C), gp = (size_t)g.gp;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -729,46 +729,46 @@ static void dct_xy(PoissonGrid &g, const double *G, T *H) {
         const FftPlan &pl = g.plan[0];
         int nlines = (n1 + 1) * (n2 + 1);
         dim3 grid((nlines + pl.W - 1) / pl.W);
-        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
                                     G, H, pl, n0, nlines, g.gp));
     }
     {
         const FftPlan &pl = g.plan[1];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
                                     H, pl, n1 + 1, gp, gp * (n1 + 1)));
     }
 }
 
 template <typename T>
-static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
+static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t st) {
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const bool igf = g.greens != 1;
     if (!igf) {
-        k_green_standard<<<1024, 256, 0, g.stream>>>((double *)g.Goct, n0, n1, n2, d_sc);
+        k_green_standard<<<1024, 256, 0, st>>>((double *)g.Goct, n0, n1, n2, d_sc);
     } else {
         dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
-        if (cr) k_igf_corners<true><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
-        else k_igf_corners<false><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
+        if (cr) k_igf_corners<true><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
+        else k_igf_corners<false><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
         dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
-        k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 1);
+        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 1);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const size_t gp = (size_t)g.gp;
-    dct_xy<T>(g, (const double *)g.Goct, (T *)g.Ghat);
+    dct_xy<T>(g, (const double *)g.Goct, (T *)g.Ghat, st);
     if (g.pz) {
         // periodic z: the z line is G[k] for k in [0, n2), one-sided (mirrorRhoField,
         // FFTPoissonSolver.cpp:661-662), so its transform is complex
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+        OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
                                     (const T *)g.Ghat, (const T *)g.Ghat, (C *)g.Gper, pl, n1, n2, g.gp));
     } else {
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+        OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
                                     (T *)g.Ghat, pl, n2 + 1, gp * (n1 + 1), gp));
     }
     g.profile_mark(PROF_DCT);
@@ -789,7 +789,7 @@ static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
         else k_igf_corners_shifted<false><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
         dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
         k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 0);
-        dct_xy<T>(g, (const double *)g.Goct, H[second]);
+        dct_xy<T>(g, (const double *)g.Goct, H[second], g.stream);
     }
     g.profile_mark(PROF_GREEN);
     {
@@ -801,9 +801,10 @@ static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
     g.profile_mark(PROF_DCT);
 }
 
-void PoissonGrid::green(const Scalars *d_sc, bool cr) {
-    if (fp64) green_impl<double>(*this, d_sc, cr);
-    else green_impl<float>(*this, d_sc, cr);
+void PoissonGrid::green(const Scalars *d_sc, bool cr, hipStream_t s) {
+    if (!s) s = stream;
+    if (fp64) green_impl<double>(*this, d_sc, cr, s);
+    else green_impl<float>(*this, d_sc, cr, s);
 }
 
 int PoissonGrid::ensure_image(std::string &err) {
@@ -831,7 +832,8 @@ void PoissonGrid::green_image(const Scalars *d_sc, bool cr) {
 }
 
 template <typename T>
-static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int outscale, bool image) {
+static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int outscale, bool image,
+                       hipEvent_t green_ready) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], K0 = g.K0;
@@ -857,6 +859,8 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
                                     A, B, pl, n1, L1, kp, kp * n1, kp, kp * L1, nullptr, 0, n1, n2, L1));
     }
     g.profile_mark(PROF_FFT_Y);
+    // the Green's function, computed on another stream, is read from here on
+    if (green_ready) (void)hipStreamWaitEvent(g.stream, green_ready, 0);
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((K0 + pl.W - 1) / pl.W, L1);
@@ -895,9 +899,9 @@ static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int ou
     g.profile_mark(PROF_FFT_X);
 }
 
-void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale, bool image) {
-    if (fp64) solve_impl<double>(*this, d_sc, src_kind, outscale, image);
-    else solve_impl<float>(*this, d_sc, src_kind, outscale, image);
+void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale, bool image, hipEvent_t green_ready) {
+    if (fp64) solve_impl<double>(*this, d_sc, src_kind, outscale, image, green_ready);
+    else solve_impl<float>(*this, d_sc, src_kind, outscale, image, green_ready);
 }
 
 void PoissonGrid::grad(const Scalars *d_sc) {
