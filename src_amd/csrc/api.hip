@@ -50,7 +50,7 @@ struct opalsc_ctx {
     std::string err;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // the tracker's Green's function, concurrent with the deposit
-    hipEvent_t ev_mesh = nullptr, ev_green = nullptr;
+    hipEvent_t ev_mesh = nullptr, ev_green = nullptr, ev_gimg = nullptr;
     PoissonGrid grid;
     Bunch bunch;
     Scalars *d_sc = nullptr;
@@ -258,6 +258,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->ev_mesh) (void)hipEventDestroy(c->ev_mesh);
     if (c->ev_green) (void)hipEventDestroy(c->ev_green);
+    if (c->ev_gimg) (void)hipEventDestroy(c->ev_gimg);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -911,24 +912,39 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                                      c->gamb.as<double>(), c->scb.as<Scalars>());
             c->grid.clear_sums();
             c->prof.mark(PROF_SCALAR);
+            // A bin's Green's functions need only its scalars: they run on the
+            // second stream beside the bin's deposit and x/y passes, after the
+            // previous bin's last z pass has read the shared tables.
+            const bool ov = overlap && !c->prof.enabled && c->stream2 && c->grid.ncell >= (size_t)96 * 96 * 96;
+            if (ov) (void)hipEventRecord(c->ev_mesh, c->stream);
             for (int k = 0; k < c->bins.nbins; ++k) {
                 const int64_t lo = c->bins.start[k], hi = c->bins.start[k + 1];
                 if (hi == lo) continue;  // an empty bin has no charge
                 const Scalars *sk = c->scb.as<Scalars>() + k;
+                if (ov) {
+                    (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
+                    c->grid.green(sk, cr, c->stream2);
+                    (void)hipEventRecord(c->ev_green, c->stream2);
+                    if (c->bin_image) {
+                        c->grid.green_image(sk, cr, c->stream2);
+                        (void)hipEventRecord(c->ev_gimg, c->stream2);
+                    }
+                }
                 launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
                 c->prof.mark(PROF_ZERO);
                 launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
                                      (long long *)c->grid.rho_i64, sk, &c->d_counters[1],
                                      sort_now ? nullptr : b.chunkbox);
                 c->prof.mark(PROF_DEPOSIT);
-                c->grid.green(sk, cr);
-                c->grid.solve(sk, 0, 1);
+                if (!ov) c->grid.green(sk, cr);
+                c->grid.solve(sk, 0, 1, false, ov ? c->ev_green : nullptr);
                 c->grid.grad_accum(sk, -1.0);
                 if (c->bin_image) {
-                    c->grid.green_image(sk, cr);
-                    c->grid.solve(sk, 0, 1, true);
+                    if (!ov) c->grid.green_image(sk, cr);
+                    c->grid.solve(sk, 0, 1, true, ov ? c->ev_gimg : nullptr);
                     c->grid.grad_accum(sk, +1.0);
                 }
+                if (ov) (void)hipEventRecord(c->ev_mesh, c->stream);  // this bin's z passes are issued
             }
         } else {
             // The Green's function needs only the mesh: without profiling it runs
@@ -977,6 +993,7 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
         HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_mesh, hipEventDisableTiming));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_green, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_gimg, hipEventDisableTiming));
     }
     c->prof.start();
     launch_track_begin_call(c->stream, c->d_ts);

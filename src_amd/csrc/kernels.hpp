@@ -76,7 +76,7 @@ struct PoissonGrid {
     void destroy();
     void green(const Scalars *d_sc, bool correctly_rounded, hipStream_t s = nullptr);  // s: default stream
     int ensure_image(std::string &err);
-    void green_image(const Scalars *d_sc, bool correctly_rounded);  // shift: Scalars::zshift
+    void green_image(const Scalars *d_sc, bool correctly_rounded, hipStream_t s = nullptr);  // shift: Scalars::zshift
     // src_kind 0: int64 fixed-point rho_i64 (self-field scaling); 1: rho_f64 as is
     // outscale 1: phi *= vol*coupling (self fields); 0: potential only
     // image: convolve with the image-charge Green's function (negated, z flipped)

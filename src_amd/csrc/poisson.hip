@@ -778,24 +778,24 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t
 // FFTPoissonSolver.cpp:546-639) for the shift in the scalars: both corner
 // tables, their differences, DCT-I in x and y, complex FFT along the doubled z.
 template <typename T>
-static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr) {
+static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t st) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     T *H[2] = {(T *)g.HimgA, (T *)g.HimgB};
     for (int second = 0; second < 2; ++second) {
         dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
-        if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
-        else k_igf_corners_shifted<false><<<grid, 256, 0, g.stream>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
+        if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
+        else k_igf_corners_shifted<false><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
         dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
-        k_igf_diff<<<grid2, 256, 0, g.stream>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 0);
-        dct_xy<T>(g, (const double *)g.Goct, H[second], g.stream);
+        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 0);
+        dct_xy<T>(g, (const double *)g.Goct, H[second], st);
     }
     g.profile_mark(PROF_GREEN);
     {
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
-        OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+        OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
                                     H[0], H[1], (C *)g.Gimg, pl, n1, n2, g.gp));
     }
     g.profile_mark(PROF_DCT);
@@ -826,9 +826,10 @@ int PoissonGrid::ensure_image(std::string &err) {
     return 0;
 }
 
-void PoissonGrid::green_image(const Scalars *d_sc, bool cr) {
-    if (fp64) green_image_impl<double>(*this, d_sc, cr);
-    else green_image_impl<float>(*this, d_sc, cr);
+void PoissonGrid::green_image(const Scalars *d_sc, bool cr, hipStream_t s) {
+    if (!s) s = stream;
+    if (fp64) green_image_impl<double>(*this, d_sc, cr, s);
+    else green_image_impl<float>(*this, d_sc, cr, s);
 }
 
 template <typename T>

@@ -553,3 +553,29 @@ def test_tracker_equal_charges_round_trip_like_the_reference():
         c.track(steps)
         _, _, Qg = c.get_bunch()
     assert np.array_equal(Qg, Qo)
+
+
+@pytest.mark.parametrize("precision", [sc.FP64, sc.MIXED])
+@pytest.mark.parametrize("nbins", [0, 4])
+def test_green_function_overlap_changes_nothing(precision, nbins):
+    """On large grids the tracker computes the Green's functions on a second
+    stream beside the deposit; with profiling on it runs them in line.  Both
+    schedules give the same bunch and statistics bit for bit."""
+    n, N, steps = (96, 96, 96), 100000, 3
+    R, P, Q = gaussian_bunch(N, seed=15, q_total=1e-9, sigma_p=(1e-3, 1e-3, 5e-3))
+    R[:, 2] += 0.005
+    out = []
+    for prof in (False, True):
+        with ctx(n, precision=precision, N=N) as c:
+            c.set_bunch(R, P, Q)
+            if nbins:
+                c.set_bins(geometric_energy_bins(P, nbins), nbins, 1e-9 / N, image=True)
+            c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+            c.profile_enable(prof)
+            st = c.track(steps)
+            out.append((c.get_bunch(), st))
+    (R0, P0, _), s0 = out[0]
+    (R1, P1, _), s1 = out[1]
+    assert np.array_equal(R0, R1) and np.array_equal(P0, P1)
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a["eps_norm"], b["eps_norm"])
