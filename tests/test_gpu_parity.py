@@ -579,3 +579,65 @@ def test_green_function_overlap_changes_nothing(precision, nbins):
     assert np.array_equal(R0, R1) and np.array_equal(P0, P1)
     for a, b in zip(s0, s1):
         assert np.array_equal(a["eps_norm"], b["eps_norm"])
+
+
+def test_compute_self_fields_particles_outside_the_mesh():
+    """A mesh (from boundp of the core) that leaves a tenth of the bunch outside:
+    their deposits are dropped and they gather zero field (ZeroFace)."""
+    n, N = (32, 32, 32), 50000
+    R, P, Q = gaussian_bunch(N, seed=23, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    r2 = (R ** 2).sum(1)
+    core = R[r2 < np.quantile(r2, 0.9)]
+    hr, org = O.boundp(core, n)
+    ref = O.compute_self_fields(R, P, Q, dt, 1e-11, org, hr, n, want_grids=True)
+    assert ref["dropped"] > 0
+    with ctx(n, N=N) as c:
+        Ef, Bf, Q2 = c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr)
+        rho = c.get_grid(sc.GRID_RHO)
+    assert rel_l2(rho, ref["rho"]) < 1e-12
+    assert rel_l2(Ef, ref["Ef"]) < 1e-12
+    assert rel_l2(Bf, ref["Bf"]) < 1e-12
+    t = (R - org) / hr  # cell coordinates: more than two cells outside the mesh reads no field
+    far = ((t < -2) | (t > np.array(n) + 1)).any(axis=1)
+    assert far.sum() > 0
+    assert np.all(Ef[far] == 0.0) and np.all(ref["Ef"][far] == 0.0)
+
+
+@pytest.mark.parametrize("N", [1, 7])
+def test_compute_self_fields_tiny_bunches(N):
+    """One particle (boundp's degenerate-length branch in every dimension) and a few."""
+    n = (8, 8, 8)
+    R, P, Q = gaussian_bunch(N, seed=24, q_total=1e-12)
+    dt = np.full(N, 1e-11)
+    hr, org = O.boundp(R, n)
+    ref = O.compute_self_fields(R, P, Q, dt, 1e-11, org, hr, n, want_grids=True)
+    with ctx(n, N=16) as c:
+        hr2, org2 = c.boundp(R)
+        Ef, Bf, Q2 = c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr)
+        rho = c.get_grid(sc.GRID_RHO)
+        E = c.get_grid(sc.GRID_EFIELD)
+    assert np.array_equal(hr2, hr) and np.array_equal(org2, org)
+    assert np.array_equal(Q2, ref["Q"])
+    assert rel_l2(rho, ref["rho"]) < 1e-12
+    assert rel_l2(E, ref["E"]) < 1e-12
+    # a lone particle sits where its own field cancels: compare on the grid's scale
+    scale = np.abs(ref["E"]).max()
+    assert np.abs(Ef - ref["Ef"]).max() <= 1e-12 * scale
+
+
+def test_tracker_few_particles():
+    n, N, steps = (8, 8, 8), 5, 4
+    R, P, Q = gaussian_bunch(N, seed=25, q_total=1e-12)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.3)
+    Ro, Po, _, _, so = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, N=N) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.3)
+        sg = c.track(steps)
+        Rg, Pg, _ = c.get_bunch()
+    assert rel_l2(Rg, Ro) < 1e-12
+    assert rel_l2(Pg - P, Po - P) < 1e-9
+    for a, b in zip(sg, so):
+        assert np.allclose(a["rrms"], b["rrms"], rtol=1e-10, atol=0)
