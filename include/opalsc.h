@@ -118,6 +118,9 @@ int opalsc_synchronize(opalsc_ctx *ctx);
  * circular convolution of the zero-padded rho with the (mirrored) Green's
  * function on the doubled grid, without the h^3 and 1/(4 pi eps0) factors. */
 int opalsc_compute_potential(opalsc_ctx *ctx, double *rho_inout, const double hr[3]);
+/* The same on a device array (rho_dev: device memory, n0*n1*n2 doubles); returns
+ * when phi is in rho_dev.  For callers whose grids already live in HBM. */
+int opalsc_compute_potential_dev(opalsc_ctx *ctx, double *rho_dev, const double hr[3]);
 /* With bcz = OPALSC_BC_PERIODIC the same call is the periodic-z solve: circular
  * convolution on the (2 n0, 2 n1, n2) grid with the Green's function mirrored in
  * x and y only and read one-sided in z (mirrorRhoField, FFTPoissonSolver.cpp:654-664),
@@ -162,6 +165,11 @@ int opalsc_sort_by_cell(opalsc_ctx *ctx, int64_t N, const double *R, const doubl
 int opalsc_compute_self_fields(opalsc_ctx *ctx, int64_t N, const double *R, const double *P,
                                double *Q, const double *dt, double dT, const double origin[3],
                                const double hr[3], double coupling, double *Ef, double *Bf);
+/* The same on device arrays (same layouts; every _dev pointer must be device
+ * memory, checked); no host copies.  Returns when Ef, Bf and Q are written. */
+int opalsc_compute_self_fields_dev(opalsc_ctx *ctx, int64_t N, const double *R_dev, const double *P_dev,
+                                   double *Q_dev, const double *dt_dev, double dT, const double origin[3],
+                                   const double hr[3], double coupling, double *Ef_dev, double *Bf_dev);
 
 /* Energy-binned self fields: the bin loop of
  * ParallelTTracker::computeSpaceChargeFields (src/Algorithms/ParallelTTracker.cpp:567-579)
@@ -226,6 +234,8 @@ int opalsc_bunch_get_bins(opalsc_ctx *ctx, int *nbins, int64_t *counts, double *
 int opalsc_track_init(opalsc_ctx *ctx, const opalsc_track_params *p);
 /* Run nsteps steps; per_step (may be NULL) receives nsteps statistics. */
 int opalsc_track(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);
+/* The same entry point under the name of the fused device-resident step. */
+int opalsc_step(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);
 
 /* --- many independent bunches --------------------------------------------
  * No counterpart in the reference, which tracks one bunch per run

@@ -302,6 +302,48 @@ static int download_scalars(opalsc_ctx *c, Scalars &s) {
 // ---------------------------------------------------------------------------
 // Poisson solver
 // ---------------------------------------------------------------------------
+// Every byte of [p, p + bytes) is device memory the library may read and write.
+static bool is_device_range(const void *p, size_t bytes) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) return false;
+    (void)bytes;
+    return true;
+}
+
+static void potential_scalars(opalsc_ctx *c, const double hr[3]) {
+    Scalars s;
+    memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) {
+        s.n[d] = c->cfg.n[d];
+        s.hr[d] = hr[d];
+        s.hs[d] = hr[d];
+    }
+    s.dT = 1.0;
+    s.tmp2 = 1.0;
+    s.q2e = 1.0;
+    upload_scalars(c, s);
+}
+
+extern "C" int opalsc_compute_potential_dev(opalsc_ctx *c, double *rho_dev, const double hr[3]) {
+    if (!c || !rho_dev || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    const size_t nc = c->grid.ncell;
+    if (!is_device_range(rho_dev, nc * 8)) FAIL(c, OPALSC_EINVAL, "rho_dev is not device memory");
+    potential_scalars(c, hr);
+    HIPCHK(c, hipMemcpyAsync(c->grid.rho_f64, rho_dev, nc * 8, hipMemcpyDeviceToDevice, c->stream));
+    c->grid.green(c->d_sc, c->grid.fp64);
+    c->grid.solve(c->d_sc, 1, 0);
+    CHECK_LAUNCH(c);
+    c->grid.widen(1, rho_dev);
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
 extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double hr[3]) {
     if (!c || !rho || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
     Scalars s;
@@ -427,26 +469,24 @@ extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, co
 // ---------------------------------------------------------------------------
 // self fields
 // ---------------------------------------------------------------------------
-extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double *R, const double *P,
-                                          double *Q, const double *dt, double dT,
-                                          const double origin[3], const double hr[3],
-                                          double coupling, double *Ef, double *Bf) {
+static int self_fields_checks(opalsc_ctx *c, int64_t N, const void *R, const void *P, const void *Q, const void *dt,
+                              const double origin[3], const double hr[3], const void *Ef, const void *Bf) {
     if (!c || !origin || !hr) return OPALSC_EINVAL;
     if (int e = reject_periodic_z(c)) return e;
     if (N > 0 && (!R || !P || !Q || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
     if (int e = check_n(c, N)) return e;
     if (N == 0) FAIL(c, OPALSC_EINVAL, "computeSelfFields needs particles (the Lorentz factor divides by N)");
+    return OPALSC_OK;
+}
+
+// PartBunch::computeSelfFields on device arrays (AoS R, P, Ef, Bf; Q updated in place).
+static int self_fields_core(opalsc_ctx *c, int64_t N, const double *R, const double *P, double *Q, const double *dt,
+                            double dT, const double origin[3], const double hr[3], double coupling, double *Ef,
+                            double *Bf) {
     Bunch &b = c->bunch;
-    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
-    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
-    if (int e = upload(c, c->aosBf, Bf, (size_t)N * 24)) return e;
-    if (int e = upload(c, c->dtb, dt, (size_t)N * 8)) return e;
-    if (!c->aosEf.ensure((size_t)N * 24)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
-    HIPCHK(c, hipMemcpyAsync(b.q, Q, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 2, c->stream));
-    launch_soa_from_aos(c->stream, N, c->aosR.as<double>(), b.x, b.y, b.z);
-    launch_selffield_prep(c->stream, N, c->aosP.as<double>(), b.q, c->dtb.as<double>(), b.qd, c->partials,
-                          &c->d_counters[0]);
+    launch_soa_from_aos(c->stream, N, R, b.x, b.y, b.z);
+    launch_selffield_prep(c->stream, N, P, Q, dt, b.qd, c->partials, &c->d_counters[0]);
     launch_selffield_scalars(c->stream, c->partials, red_blocks(N), N, dT, origin, hr, c->cfg.n, coupling,
                              &c->d_counters[0], c->grid.fp64, c->d_sc);
     launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
@@ -455,16 +495,48 @@ extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double
     c->grid.green(c->d_sc, c->grid.fp64);
     c->grid.solve(c->d_sc, 0, 1);
     c->grid.grad(c->d_sc);
-    launch_gather_selffield(c->stream, c->grid.fp64, N, b.x, b.y, b.z, c->grid.Eg, c->d_sc,
-                            c->aosEf.as<double>(), c->aosBf.as<double>());
+    launch_gather_selffield(c->stream, c->grid.fp64, N, b.x, b.y, b.z, c->grid.Eg, c->d_sc, Ef, Bf);
     CHECK_LAUNCH(c);
-    HIPCHK(c, hipMemcpyAsync(Ef, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(Q, b.q, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_bunch = c->have_track = false;  // SoA buffers were reused
     ++c->gen;
     c->bins.nbins = 0;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_compute_self_fields(opalsc_ctx *c, int64_t N, const double *R, const double *P,
+                                          double *Q, const double *dt, double dT,
+                                          const double origin[3], const double hr[3],
+                                          double coupling, double *Ef, double *Bf) {
+    if (int e = self_fields_checks(c, N, R, P, Q, dt, origin, hr, Ef, Bf)) return e;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->aosBf, Bf, (size_t)N * 24)) return e;
+    if (int e = upload(c, c->dtb, dt, (size_t)N * 8)) return e;
+    if (!c->aosEf.ensure((size_t)N * 24)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    double *q = c->bunch.q;
+    HIPCHK(c, hipMemcpyAsync(q, Q, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    if (int e = self_fields_core(c, N, c->aosR.as<double>(), c->aosP.as<double>(), q, c->dtb.as<double>(), dT,
+                                 origin, hr, coupling, c->aosEf.as<double>(), c->aosBf.as<double>()))
+        return e;
+    HIPCHK(c, hipMemcpyAsync(Ef, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Bf, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Q, q, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_compute_self_fields_dev(opalsc_ctx *c, int64_t N, const double *R_dev, const double *P_dev,
+                                              double *Q_dev, const double *dt_dev, double dT,
+                                              const double origin[3], const double hr[3], double coupling,
+                                              double *Ef_dev, double *Bf_dev) {
+    if (int e = self_fields_checks(c, N, R_dev, P_dev, Q_dev, dt_dev, origin, hr, Ef_dev, Bf_dev)) return e;
+    const size_t v = (size_t)N * 24, sz = (size_t)N * 8;
+    if (!is_device_range(R_dev, v) || !is_device_range(P_dev, v) || !is_device_range(Q_dev, sz) ||
+        !is_device_range(dt_dev, sz) || !is_device_range(Ef_dev, v) || !is_device_range(Bf_dev, v))
+        FAIL(c, OPALSC_EINVAL, "a _dev argument is not device memory");
+    if (int e = self_fields_core(c, N, R_dev, P_dev, Q_dev, dt_dev, dT, origin, hr, coupling, Ef_dev, Bf_dev))
+        return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return OPALSC_OK;
 }
 
@@ -1013,6 +1085,10 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->prof.flush();
     return OPALSC_OK;
+}
+
+extern "C" int opalsc_step(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
+    return opalsc_track(c, nsteps, per_step);
 }
 
 // ---------------------------------------------------------------------------

@@ -34,6 +34,7 @@ M_P_EV = 0.93827204e+00 * 1e9              # src/Classic/Physics/Physics.h:88 (G
 EXPORTS = [
     "opalsc_abi_version", "opalsc_create", "opalsc_destroy", "opalsc_last_error",
     "opalsc_synchronize", "opalsc_compute_potential", "opalsc_compute_potential_shifted",
+    "opalsc_compute_potential_dev", "opalsc_compute_self_fields_dev", "opalsc_step",
     "opalsc_boundp", "opalsc_cic_index",
     "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_compute_self_fields_binned",
     "opalsc_get_grid", "opalsc_push_half",
@@ -98,6 +99,10 @@ def lib():
         "opalsc_synchronize": ([vp], C.c_int),
         "opalsc_compute_potential": ([vp, dp, dp], C.c_int),
         "opalsc_compute_potential_shifted": ([vp, dp, dp, C.c_double], C.c_int),
+        "opalsc_compute_potential_dev": ([vp, vp, dp], C.c_int),
+        "opalsc_compute_self_fields_dev": ([vp, i64, vp, vp, vp, vp, C.c_double, dp, dp, C.c_double,
+                                            vp, vp], C.c_int),
+        "opalsc_step": ([vp, C.c_int, dp], C.c_int),
         "opalsc_boundp": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_cic_index": ([vp, i64, dp, dp, dp, dp, dp], C.c_int),
         "opalsc_sort_by_cell": ([vp, i64, dp, dp, dp, dp], C.c_int),

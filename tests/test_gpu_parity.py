@@ -641,3 +641,70 @@ def test_tracker_few_particles():
     assert rel_l2(Pg - P, Po - P) < 1e-9
     for a, b in zip(sg, so):
         assert np.allclose(a["rrms"], b["rrms"], rtol=1e-10, atol=0)
+
+
+class _HipArrays:
+    """Device arrays from the HIP runtime the library itself links (hipMalloc/hipMemcpy via ctypes)."""
+
+    def __init__(self):
+        import ctypes
+        import ctypes.util
+        self.C = ctypes
+        path = None
+        for line in open("/proc/self/maps"):
+            if "libamdhip64" in line:
+                path = line.split()[-1]
+                break
+        self.hip = ctypes.CDLL(path or "libamdhip64.so")
+        self.ptrs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p = self.C.c_void_p()
+        assert self.hip.hipMalloc(self.C.byref(p), self.C.c_size_t(a.nbytes)) == 0
+        assert self.hip.hipMemcpy(p, a.ctypes.data_as(self.C.c_void_p), self.C.c_size_t(a.nbytes), 1) == 0
+        self.ptrs.append(p)
+        return p, a.shape
+
+    def get(self, pa):
+        p, shape = pa
+        out = np.zeros(shape)
+        assert self.hip.hipMemcpy(out.ctypes.data_as(self.C.c_void_p), p, self.C.c_size_t(out.nbytes), 2) == 0
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+
+
+def test_device_array_entry_points_equal_host_ones():
+    """opalsc_compute_self_fields_dev / opalsc_compute_potential_dev on arrays
+    already in HBM give what the host-array entry points give, bit for bit;
+    a host array passed as a device one is refused."""
+    n, N = (32, 32, 32), 40000
+    R, P, Q = gaussian_bunch(N, seed=26, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    hr, org = O.boundp(R, n)
+    Bf0 = np.random.default_rng(27).normal(size=(N, 3)) * 1e-4
+    rho = np.random.default_rng(28).normal(size=n[0] * n[1] * n[2])
+    L = sc.lib()
+    ptr = sc.opalsc._ptr
+    with ctx(n, N=N) as c:
+        Ef_h, Bf_h, Q_h = c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr, Bf=Bf0)
+        phi_h = c.computePotential(rho, hr)
+        H = _HipArrays()
+        try:
+            d = {k: H.put(v) for k, v in dict(R=R, P=P, Q=Q, dt=dt, Bf=Bf0, Ef=np.zeros((N, 3)),
+                                                 rho=rho).items()}
+            c._c(L.opalsc_compute_self_fields_dev(c._h, N, d["R"][0], d["P"][0], d["Q"][0], d["dt"][0], 1e-11,
+                                                  ptr(org), ptr(hr), sc.COUPLING, d["Ef"][0], d["Bf"][0]))
+            assert np.array_equal(H.get(d["Ef"]), Ef_h)
+            assert np.array_equal(H.get(d["Bf"]), Bf_h)
+            assert np.array_equal(H.get(d["Q"]), Q_h)
+            c._c(L.opalsc_compute_potential_dev(c._h, d["rho"][0], ptr(hr)))
+            assert np.array_equal(H.get(d["rho"]), phi_h)
+            host = np.zeros(n[0] * n[1] * n[2])
+            with pytest.raises(sc.OpalscError, match="device memory"):
+                c._c(L.opalsc_compute_potential_dev(c._h, host.ctypes.data, ptr(hr)))
+        finally:
+            H.free()
