@@ -279,6 +279,7 @@ def main():
     ap.add_argument("--c5-only", action="store_true",
                     help="run only the energy-binned measurement and print its record (for profiling)")
     ap.add_argument("--no-c4", action="store_true", help="skip the 64-bunch batched measurement")
+    ap.add_argument("--no-c2", action="store_true", help="skip the FP64 1e7-particle measurement")
     ap.add_argument("--c4-only", action="store_true",
                     help="run only the 64-bunch batched measurement and print its record (for profiling)")
     args = ap.parse_args()
@@ -371,6 +372,18 @@ def main():
     if not args.no_c4:
         log("[bench] 64 independent bunches, batched")
         extra["c4_batched"] = run_c4(sc, args.steps, args.warmup, log)
+    if not args.no_c2:
+        # the plain FP64 drop-in configuration: 1e7 particles, 128^3, 1 MeV, Bz 0.5 T
+        c2 = CONFIGS["c2"]
+        log("[bench] FP64 drop-in step, 1e7 particles, 128^3")
+        cc = make_ctx(sc, c2, sc.FP64, args.sort_interval)
+        t2, _ = timed_run(cc, args.steps, args.warmup)
+        cc.close()
+        extra["c2_fp64"] = {"workload": "1e7 macroparticles, 128^3 grid (256^3 doubled FFT), 1 MeV protons, "
+                                        "10 nC, Bz 0.5 T, FP64, sort every %d steps" % args.sort_interval,
+                            "ms_per_step": 1e3 * t2 / args.steps,
+                            "particle_steps_per_s": c2["N"] * args.steps / t2}
+        log("[bench] c2 fp64: %.3f ms/step" % extra["c2_fp64"]["ms_per_step"])
 
     cpu = None
     if not args.no_cpu:
