@@ -92,6 +92,12 @@ struct opalsc_ctx {
 
 #define CHECK_LAUNCH(ctx) HIPCHK(ctx, hipGetLastError())
 
+// Every entry point works on the device the context was created on.
+static int use_device(opalsc_ctx *c) {
+    if (hipSetDevice(c->cfg.device) != hipSuccess) FAIL(c, OPALSC_EHIP, "hipSetDevice failed");
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // profiler
 // ---------------------------------------------------------------------------
@@ -245,6 +251,7 @@ extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
 
 extern "C" int opalsc_destroy(opalsc_ctx *c) {
     if (!c) return OPALSC_OK;
+    (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->grid.destroy();
     free_bunch(c->bunch);
@@ -267,6 +274,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
 
 extern "C" int opalsc_synchronize(opalsc_ctx *c) {
     if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return OPALSC_OK;
 }
@@ -331,6 +339,7 @@ static void potential_scalars(opalsc_ctx *c, const double hr[3]) {
 
 extern "C" int opalsc_compute_potential_dev(opalsc_ctx *c, double *rho_dev, const double hr[3]) {
     if (!c || !rho_dev || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     const size_t nc = c->grid.ncell;
     if (!is_device_range(rho_dev, nc * 8)) FAIL(c, OPALSC_EINVAL, "rho_dev is not device memory");
     potential_scalars(c, hr);
@@ -346,6 +355,7 @@ extern "C" int opalsc_compute_potential_dev(opalsc_ctx *c, double *rho_dev, cons
 
 extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double hr[3]) {
     if (!c || !rho || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     Scalars s;
     memset(&s, 0, sizeof(s));
     for (int d = 0; d < 3; ++d) {
@@ -370,6 +380,7 @@ extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double
 
 extern "C" int opalsc_compute_potential_shifted(opalsc_ctx *c, double *rho, const double hr[3], double zshift) {
     if (!c || !rho || !hr) return c ? (c->err = "null argument", OPALSC_EINVAL) : OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = reject_periodic_z(c)) return e;
     if (c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
     Scalars s;
@@ -400,6 +411,7 @@ extern "C" int opalsc_compute_potential_shifted(opalsc_ctx *c, double *rho, cons
 // ---------------------------------------------------------------------------
 extern "C" int opalsc_boundp(opalsc_ctx *c, int64_t N, const double *R, double hr[3], double origin[3]) {
     if (!c || (!R && N > 0) || !hr || !origin) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
     launch_bounds(c->stream, N, c->aosR.as<double>(), c->partials);
@@ -418,6 +430,7 @@ extern "C" int opalsc_boundp(opalsc_ctx *c, int64_t N, const double *R, double h
 extern "C" int opalsc_cic_index(opalsc_ctx *c, int64_t N, const double *R, const double origin[3],
                                 const double hr[3], int32_t *lgp, double *dpos) {
     if (!c || (N > 0 && (!R || !lgp || !dpos)) || !origin || !hr) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     if (N == 0) return OPALSC_OK;
     if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
@@ -441,6 +454,7 @@ static int key_bits(const int n[3]) {
 extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, const double origin[3],
                                    const double hr[3], uint32_t *perm) {
     if (!c || (N > 0 && (!R || !perm)) || !origin || !hr) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     if (N == 0) return OPALSC_OK;
     if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
@@ -472,6 +486,7 @@ extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, co
 static int self_fields_checks(opalsc_ctx *c, int64_t N, const void *R, const void *P, const void *Q, const void *dt,
                               const double origin[3], const double hr[3], const void *Ef, const void *Bf) {
     if (!c || !origin || !hr) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = reject_periodic_z(c)) return e;
     if (N > 0 && (!R || !P || !Q || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
     if (int e = check_n(c, N)) return e;
@@ -546,6 +561,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
                                                  double coupling, const double *gamma_bin, int image,
                                                  double *Ef, double *Bf) {
     if (!c || !origin || !hr) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = reject_periodic_z(c)) return e;
     if (N > 0 && (!R || !P || !bin || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
     if (nbins < 1 || nbins > kMaxBins) FAIL(c, OPALSC_EINVAL, "nbins must be in [1, 64]");
@@ -632,6 +648,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
 
 extern "C" int opalsc_get_grid(opalsc_ctx *c, int which, double *out) {
     if (!c || !out) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     const size_t nc = c->grid.ncell;
     if (which == OPALSC_GRID_RHO) {
         c->grid.rho_to_double(c->d_sc, (double *)c->grid.rho_f64);
@@ -656,6 +673,7 @@ extern "C" int opalsc_get_grid(opalsc_ctx *c, int which, double *out) {
 // ---------------------------------------------------------------------------
 extern "C" int opalsc_push_half(opalsc_ctx *c, int64_t N, double *R, const double *P, const double *dt) {
     if (!c || (N > 0 && (!R || !P || !dt))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     if (N == 0) return OPALSC_OK;
     if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
@@ -671,6 +689,7 @@ extern "C" int opalsc_push_half(opalsc_ctx *c, int64_t N, double *R, const doubl
 extern "C" int opalsc_kick(opalsc_ctx *c, int64_t N, double *P, const double *Ef, const double *Bf,
                            const double *dt, double charge, double mass_eV) {
     if (!c || (N > 0 && (!P || !Ef || !Bf || !dt))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     if (N == 0) return OPALSC_OK;
     if (int e = upload(c, c->aosP, P, (size_t)N * 24)) return e;
@@ -688,6 +707,7 @@ extern "C" int opalsc_kick(opalsc_ctx *c, int64_t N, double *P, const double *Ef
 extern "C" int opalsc_beam_parameters(opalsc_ctx *c, int64_t N, const double *R, const double *P,
                                       double mass_eV, opalsc_stats *out) {
     if (!c || !out || (N > 0 && (!R || !P))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     if (N == 0) {
         memset(out, 0, sizeof(*out));
@@ -709,6 +729,7 @@ extern "C" int opalsc_beam_parameters(opalsc_ctx *c, int64_t N, const double *R,
 // ---------------------------------------------------------------------------
 extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const double *P, const double *Q) {
     if (!c || (N > 0 && (!R || !P || !Q))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, N)) return e;
     Bunch &b = c->bunch;
     b.N = N;
@@ -743,6 +764,7 @@ extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const
 
 extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g) {
     if (!c || !g) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = check_n(c, g->n)) return e;
     Bunch &b = c->bunch;
     b.N = g->n;
@@ -763,6 +785,7 @@ extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g
 
 extern "C" int opalsc_bunch_get(opalsc_ctx *c, double *R, double *P, double *Q) {
     if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "no resident bunch");
     Bunch &b = c->bunch;
     const int64_t N = b.N;
@@ -803,6 +826,7 @@ extern "C" int opalsc_bunch_get(opalsc_ctx *c, double *R, double *P, double *Q) 
 
 extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     if (!c || !p) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (int e = reject_periodic_z(c)) return e;
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "track_init before bunch_set/bunch_generate");
     if (!(p->dT > 0.0)) FAIL(c, OPALSC_EINVAL, "dT must be positive");
@@ -889,6 +913,7 @@ static void sort_bunch(opalsc_ctx *c) {
 
 extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbins, double qi, int image) {
     if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "set_bins before bunch_set/bunch_generate");
     if (nbins == 0) {
         c->bins.nbins = 0;
@@ -930,6 +955,7 @@ extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbin
 
 extern "C" int opalsc_bunch_get_bins(opalsc_ctx *c, int *nbins, int64_t *counts, double *gammas) {
     if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     const int nb = c->bins.nbins;
     if (nbins) *nbins = nb;
     if (counts)
@@ -1054,6 +1080,7 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
 
 extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     if (!c || nsteps < 0) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (!c->have_track) FAIL(c, OPALSC_ESTATE, "track before track_init");
     if (nsteps == 0) return OPALSC_OK;
     if (!c->statsb.ensure(sizeof(opalsc_stats) * (size_t)nsteps)) FAIL(c, OPALSC_ENOMEM, "stats buffer");
@@ -1129,9 +1156,11 @@ extern "C" int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, 
     *out = nullptr;
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return OPALSC_EINVAL;
+        if (ctxs[i]->cfg.device != ctxs[0]->cfg.device) return OPALSC_EINVAL;  // one device per batch
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == ctxs[i]) return OPALSC_EINVAL;  // each bunch once
     }
+    if (hipSetDevice(ctxs[0]->cfg.device) != hipSuccess) return OPALSC_EHIP;
     opalsc_batch *b = new (std::nothrow) opalsc_batch();
     if (!b) return OPALSC_ENOMEM;
     b->ctx.assign(ctxs, ctxs + n);
@@ -1203,6 +1232,7 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
     if (!b || nsteps < 0) return OPALSC_EINVAL;
     if (nsteps == 0) return OPALSC_OK;
     const int n = (int)b->ctx.size();
+    BHIP(b, hipSetDevice(b->ctx[0]->cfg.device));
     bool recapture = !b->exec;
     for (int i = 0; i < n; ++i) {
         opalsc_ctx *c = b->ctx[i];
@@ -1250,6 +1280,7 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
 
 extern "C" int opalsc_track_get_params(opalsc_ctx *c, opalsc_track_params *out) {
     if (!c || !out) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (!c->have_track) FAIL(c, OPALSC_ESTATE, "no tracker state");
     TrackState ts;
     HIPCHK(c, hipMemcpyAsync(&ts, c->d_ts, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
@@ -1274,6 +1305,7 @@ extern "C" int opalsc_track_get_params(opalsc_ctx *c, opalsc_track_params *out) 
 
 extern "C" int opalsc_get_mesh(opalsc_ctx *c, double hr[3], double origin[3]) {
     if (!c || !hr || !origin) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     Scalars s;
     if (int e = download_scalars(c, s)) return e;
     for (int d = 0; d < 3; ++d) {
@@ -1298,6 +1330,7 @@ extern "C" int opalsc_host_log_cr(int64_t n, const double *x, double *y) {
 }
 static int device_cr(opalsc_ctx *c, int which, int64_t n, const double *x, double *y) {
     if (!c || n < 0 || (n > 0 && (!x || !y))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     if (n == 0) return OPALSC_OK;
     if (int e = upload(c, c->aosR, x, (size_t)n * 8)) return e;
     if (!c->aosP.ensure((size_t)n * 8)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
@@ -1319,6 +1352,7 @@ extern "C" int opalsc_device_log_cr(opalsc_ctx *c, int64_t n, const double *x, d
 // ---------------------------------------------------------------------------
 extern "C" int opalsc_profile_enable(opalsc_ctx *c, int enable) {
     if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     c->prof.flush();
     c->prof.enabled = enable != 0;
     for (int i = 0; i < PROF_COUNT; ++i) {
@@ -1330,6 +1364,7 @@ extern "C" int opalsc_profile_enable(opalsc_ctx *c, int enable) {
 
 extern "C" int opalsc_copy_bandwidth(opalsc_ctx *c, size_t bytes, int reps, double *gbs) {
     if (!c || !gbs || reps <= 0 || bytes < 16) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     bytes &= ~(size_t)15;
     void *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, bytes) != hipSuccess) FAIL(c, OPALSC_ENOMEM, "copy buffer allocation failed");
@@ -1359,6 +1394,7 @@ extern "C" int opalsc_copy_bandwidth(opalsc_ctx *c, size_t bytes, int reps, doub
 
 extern "C" int opalsc_profile_read(opalsc_ctx *c, int cap, const char **names, double *ms, int64_t *launches) {
     if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
     c->prof.flush();
     int n = std::min<int>(cap, PROF_COUNT);
     for (int i = 0; i < n; ++i) {
