@@ -62,8 +62,9 @@ typedef struct {
     int sort_interval;     /* tracker: re-sort the bunch by cell every k steps (0: never) */
     int bcz;               /* BCFFTZ (FieldSolver.cpp:153,428-448): OPALSC_BC_OPEN, or
                               OPALSC_BC_PERIODIC (DC beams: the solver is periodic in z,
-                              FFTPoissonSolver.cpp:60,130-155; opalsc_compute_potential
-                              only, n[2] = 2^a or 3*2^a with a >= 2) */
+                              FFTPoissonSolver.cpp:60,130-155, the field's z faces
+                              periodic and particles wrapped in z, PartBunch.cpp:1079-1098;
+                              no energy bins; n[2] = 2^a or 3*2^a with a >= 2) */
 } opalsc_config;
 
 /* DataSink statistics (src/Structure/DataSink.cpp:360-391) as computed by
@@ -138,8 +139,24 @@ int opalsc_compute_potential_shifted(opalsc_ctx *ctx, double *rho_inout, const d
 
 /* --- mesh and cloud-in-cell ----------------------------------------------
  * PartBunchBase::boundp (src/Classic/Algorithms/PartBunchBase.hpp:802-883):
- * bounds of R, BBOXINCR padding, hr = (max-min)/(n-1), origin = min - hr/2. */
+ * bounds of R, BBOXINCR padding, hr = (max-min)/(n-1), origin = min - hr/2.
+ * In a periodic-z context (DC beam) z is not padded: its extent is the period
+ * length, hr[2] = period/(n2-1) (PartBunchBase.hpp:814,840-843). */
 int opalsc_boundp(opalsc_ctx *ctx, int64_t N, const double *R, double hr[3], double origin[3]);
+
+/* PartBunchBase::setBeamFrequency (src/Classic/Algorithms/PartBunchBase.hpp:2534-2537):
+ * the DC beam's period length becomes c/f (default c / 1 GHz).  Applies to
+ * opalsc_boundp and to the tracker (also after opalsc_track_init). */
+int opalsc_set_beam_frequency(opalsc_ctx *ctx, double f_hz);
+
+/* The particle boundary condition of a DC beam, which boundp's update()
+ * applies after the mesh is set (ippl/src/Particle/ParticleSpatialLayout.hpp:296-297,
+ * ParticlePeriodicBCond, ippl/src/Particle/ParticleBConds.h:38-45): R[i][2]
+ * (AoS, in place) wrapped into [origin[2], n2*hr[2] + origin[2]); x and y
+ * unchanged.  The self-field entry points do not wrap: a caller follows
+ * opalsc_boundp with this call as the reference's boundp does.  (The tracker
+ * applies it itself.) */
+int opalsc_wrap_periodic_z(opalsc_ctx *ctx, int64_t N, double *R, const double origin[3], const double hr[3]);
 
 /* IntCICImpl<3> cell lookup (ippl/src/Particle/IntCIC.h:816-851;
  * ippl/src/Meshes/UniformCartesian.hpp:1025-1039,1084-1097): per particle the
@@ -161,7 +178,10 @@ int opalsc_sort_by_cell(opalsc_ctx *ctx, int64_t N, const double *R, const doubl
  * E = -Grad(phi) scaled by (gz, gz, 1/gz), CIC gather into Ef (overwritten),
  * Bf += (-betaC*Ey, betaC*Ex, 0) (accumulated).  Q is updated in place to
  * (Q*dt)/dt exactly as the reference leaves it.  origin/hr are the mesh
- * set by boundp. */
+ * set by boundp.  In a periodic-z context the solve is periodic in z and the
+ * gather reads the periodic images of E beyond the z faces (PeriodicFace
+ * guard layers); deposits beyond the faces are dropped as on an open mesh
+ * (BareField::accumGuardCells has no neighbour on one rank). */
 int opalsc_compute_self_fields(opalsc_ctx *ctx, int64_t N, const double *R, const double *P,
                                double *Q, const double *dt, double dT, const double origin[3],
                                const double hr[3], double coupling, double *Ef, double *Bf);

@@ -60,13 +60,15 @@ void orc_get_bounds(int64_t N, const double *R, double rmin[3], double rmax[3]) 
 }
 
 /* PartBunchBase<T,Dim>::boundp, src/Classic/Algorithms/PartBunchBase.hpp:802-883
- * (non-DC beam, not emitting).  dh = BBOXINCR/100 (src/Structure/FieldSolver.cpp:487).
- * Returns 0, or -1 when h0*h1*h2 <= 0 ("h<0, can not build a mesh"). */
-int orc_boundp(int64_t N, const double *R, const int n[3], double dh,
-               double hr[3], double origin[3]) {
+ * (not emitting).  dh = BBOXINCR/100 (src/Structure/FieldSolver.cpp:487).  dc:
+ * DC beam, z not padded but given the period length (PartBunchBase.hpp:814,
+ * 840-843).  Returns 0, or -1 when h0*h1*h2 <= 0 ("h<0, can not build a mesh"). */
+static int boundp_impl(int64_t N, const double *R, const int n[3], double dh, int dc, double period,
+                       double hr[3], double origin[3]) {
     double rmin[3], rmax[3];
     orc_get_bounds(N, R, rmin, rmax);
-    for (int i = 0; i < 3; ++i) {
+    const int dims = dc ? 2 : 3;
+    for (int i = 0; i < dims; ++i) {
         double length = fabs(rmax[i] - rmin[i]);
         if (length < 1e-10) {
             rmax[i] += 1e-10;
@@ -77,9 +79,40 @@ int orc_boundp(int64_t N, const double *R, const int n[3], double dh,
         }
         hr[i] = (rmax[i] - rmin[i]) / (n[i] - 1);
     }
+    if (dc) {
+        rmax[2] = rmin[2] + period;
+        hr[2] = period / (n[2] - 1);
+    }
     if (hr[0] * hr[1] * hr[2] <= 0) return -1;
     for (int i = 0; i < 3; ++i) origin[i] = rmin[i] - hr[i] / 2.0;
     return 0;
+}
+
+int orc_boundp(int64_t N, const double *R, const int n[3], double dh,
+               double hr[3], double origin[3]) {
+    return boundp_impl(N, R, n, dh, 0, 0.0, hr, origin);
+}
+
+int orc_boundp_dc(int64_t N, const double *R, const int n[3], double dh, double period,
+                  double hr[3], double origin[3]) {
+    return boundp_impl(N, R, n, dh, 1, period, hr, origin);
+}
+
+/* boundp's update() for a DC beam (ippl/src/Particle/ParticleSpatialLayout.hpp:296-297):
+ * ParticlePeriodicBCond (ippl/src/Particle/ParticleBConds.h:38-45, applied by
+ * ParticleBConds::apply, ParticleBConds.h:113-120) on z over the region of the
+ * mesh's cells, min = 0*h + origin, max = n*h + origin (RegionLayout::convert_index,
+ * ippl/src/Region/RegionLayout.hpp:526-556, getVertexPosition,
+ * ippl/src/Meshes/UniformCartesian.hpp:1054-1055).  x and y: ParticleNoBCond. */
+void orc_wrap_periodic_z(int64_t N, double *R, const double origin[3], const double hr[3], const int n[3]) {
+    const double m1 = 0 * hr[2] + origin[2], m2 = n[2] * hr[2] + origin[2];
+    for (int64_t i = 0; i < N; ++i) {
+        const double t = R[3 * i + 2];
+        if (t < m1)
+            R[3 * i + 2] = m2 - (m1 - t);
+        else if (t >= m2)
+            R[3 * i + 2] = m1 + (t - m2);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -172,9 +205,12 @@ int64_t orc_scatter(int64_t N, const double *R, const double *pdata, const doubl
 
 /* IntCICImpl<3>::gather, ippl/src/Particle/IntCIC.h:933-977 for a Vektor field:
  * Ef = sum over the 8 corners, in order 000,100,...,111, of ((a*b)*c)*E.
- * Corners outside the interior read the ZeroFace guard value 0. */
-void orc_gather(int64_t N, const double *R, const double *E, const double origin[3],
-                const double hr[3], const int n[3], double *Ef) {
+ * Corners outside the interior read the guard value: 0 for ZeroFace; with
+ * periodic z faces (pz, a DC beam's PeriodicFace, filled by Field::fillGuardCells
+ * before the gather, ippl/src/Field/Field.hpp:366-381) the z guard layers hold
+ * the opposite cells: k = n2 reads k = 0, k = -1 reads k = n2-1. */
+static void gather_impl(int64_t N, const double *R, const double *E, const double origin[3],
+                        const double hr[3], const int n[3], int pz, double *Ef) {
     for (int64_t p = 0; p < N; ++p) {
         int l[3];
         double dp[3];
@@ -199,6 +235,8 @@ void orc_gather(int64_t N, const double *R, const double *E, const double origin
             double acc = 0.0;
             for (int c = 0; c < 8; ++c) {
                 int i = l[0] + (c & 1), j = l[1] + ((c >> 1) & 1), k = l[2] + ((c >> 2) & 1);
+                if (pz && k == n[2]) k = 0;
+                else if (pz && k == -1) k = n[2] - 1;
                 double e = in_grid(i, j, k, n) ? E[3 * (((int64_t)k * n[1] + j) * n[0] + i) + d] : 0.0;
                 double term = w[c] * e;
                 acc = (c == 0) ? term : acc + term;
@@ -206,6 +244,16 @@ void orc_gather(int64_t N, const double *R, const double *E, const double origin
             Ef[3 * p + d] = acc;
         }
     }
+}
+
+void orc_gather(int64_t N, const double *R, const double *E, const double origin[3],
+                const double hr[3], const int n[3], double *Ef) {
+    gather_impl(N, R, E, origin, hr, n, 0, Ef);
+}
+
+void orc_gather_periodic_z(int64_t N, const double *R, const double *E, const double origin[3],
+                           const double hr[3], const int n[3], double *Ef) {
+    gather_impl(N, R, E, origin, hr, n, 1, Ef);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -653,9 +701,9 @@ void orc_grad(const int n[3], const double *x, const double hr[3], double *r) {
  * (scaled charge density that enters the solver), phi_out (potential in V)
  * and E_out (AoS grid field after the Lorentz scaling) may be NULL.
  * Returns the number of dropped deposits. */
-int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, double *Q,
+static int64_t self_fields_impl(int64_t N, const double *R, const double *P, double *Q,
                                 const double *dt, double dT, const double origin[3],
-                                const double hr[3], const int n[3], double coupling, int greens,
+                                const double hr[3], const int n[3], double coupling, int greens, int pz,
                                 double *Ef, double *Bf, double *rho_out, double *phi_out,
                                 double *E_out) {
     const size_t ncell = (size_t)n[0] * n[1] * n[2];
@@ -677,7 +725,10 @@ int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, dou
     for (size_t c = 0; c < ncell; ++c) rho[c] *= tmp2;
     if (rho_out) memcpy(rho_out, rho, ncell * sizeof(double));
 
-    orc_compute_potential(n, rho, hs, greens);
+    if (pz)
+        orc_compute_potential_periodic_z(n, rho, hs, greens);
+    else
+        orc_compute_potential(n, rho, hs, greens);
 
     double vol = hs[0] * hs[1] * hs[2];
     for (size_t c = 0; c < ncell; ++c) rho[c] *= vol;
@@ -690,7 +741,7 @@ int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, dou
         for (int d = 0; d < 3; ++d) E[3 * c + d] = -E[3 * c + d] * sc[d];
     if (E_out) memcpy(E_out, E, 3 * ncell * sizeof(double));
 
-    orc_gather(N, R, E, origin, hr, n, Ef);
+    gather_impl(N, R, E, origin, hr, n, pz, Ef);
 
     double betaC = sqrt(gammaz * gammaz - 1.0) / gammaz / ORC_C;
     for (int64_t i = 0; i < N; ++i) {
@@ -700,6 +751,28 @@ int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, dou
     free(rho);
     free(E);
     return dropped;
+}
+
+int64_t orc_compute_self_fields(int64_t N, const double *R, const double *P, double *Q,
+                                const double *dt, double dT, const double origin[3],
+                                const double hr[3], const int n[3], double coupling, int greens,
+                                double *Ef, double *Bf, double *rho_out, double *phi_out,
+                                double *E_out) {
+    return self_fields_impl(N, R, P, Q, dt, dT, origin, hr, n, coupling, greens, 0, Ef, Bf, rho_out,
+                            phi_out, E_out);
+}
+
+/* The same for a DC beam (setBCForDCBeam, src/Classic/Algorithms/PartBunch.cpp:1079-1098):
+ * periodic-z solve (FFTPoissonSolver.cpp:130-155) and periodic z guards for
+ * the gather; deposits beyond the z faces are dropped (accumGuardCells has no
+ * neighbour field on one rank, ippl/src/Field/BareField.hpp:764-960). */
+int64_t orc_compute_self_fields_periodic_z(int64_t N, const double *R, const double *P, double *Q,
+                                           const double *dt, double dT, const double origin[3],
+                                           const double hr[3], const int n[3], double coupling, int greens,
+                                           double *Ef, double *Bf, double *rho_out, double *phi_out,
+                                           double *E_out) {
+    return self_fields_impl(N, R, P, Q, dt, dT, origin, hr, n, coupling, greens, 1, Ef, Bf, rho_out,
+                            phi_out, E_out);
 }
 
 /* PartBunchBase::calcGammas, src/Classic/Algorithms/PartBunchBase.hpp:639-682:
@@ -1118,6 +1191,9 @@ typedef struct {
     /* tracker state */
     double t, pathLength;
     double RefPartR[3], RefPartP[3];
+    /* DC beam (BCFFTZ=PERIODIC): periodic z and its period length */
+    int bcz;
+    double period;
 } orc_tracker;
 
 static void ext_fields(const orc_tracker *tr, double t, double E[3], double B[3]) {
@@ -1206,7 +1282,9 @@ static int64_t space_charge_impl(orc_tracker *tr, int64_t N, double *R, const do
     orc_trafo_inverted(&b2r, &r2b);
     orc_transform_to(&r2b, N, R);
     double hr[3], origin[3];
-    if (orc_boundp(N, R, tr->n, tr->dh, hr, origin) != 0) return -1;
+    if (tr->bcz && bins && bins->bin) return -1; /* no energy bins for a DC beam */
+    if (boundp_impl(N, R, tr->n, tr->dh, tr->bcz, tr->period, hr, origin) != 0) return -1;
+    if (tr->bcz) orc_wrap_periodic_z(N, R, origin, hr, tr->n);
     if (mesh_out) {
         memcpy(mesh_out, hr, sizeof(hr));
         memcpy(mesh_out + 3, origin, sizeof(origin));
@@ -1217,8 +1295,8 @@ static int64_t space_charge_impl(orc_tracker *tr, int64_t N, double *R, const do
                                                  origin, hr, tr->n, tr->coupling, tr->greens, bins->image,
                                                  NULL, Ef, Bf);
     else
-        dropped = orc_compute_self_fields(N, R, P, Q, dt, tr->dT, origin, hr, tr->n, tr->coupling,
-                                          tr->greens, Ef, Bf, rho_out, phi_out, E_out);
+        dropped = self_fields_impl(N, R, P, Q, dt, tr->dT, origin, hr, tr->n, tr->coupling, tr->greens,
+                                   tr->bcz, Ef, Bf, rho_out, phi_out, E_out);
     orc_transform_to(&b2r, N, R);
     orc_rotate_to(&b2r, N, Ef);
     orc_rotate_to(&b2r, N, Bf);

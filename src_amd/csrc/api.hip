@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -74,6 +75,7 @@ struct opalsc_ctx {
     bool track_ext = false;   // the tracker has a nonzero external field
     Profiler prof;
     double dh = 0.02;
+    double period = kC / 1e9;  // DC beam period length (PartBunchBase.hpp:132: c / 1 GHz)
     int steps_done = 0;
     uint64_t gen = 0;  // bumped whenever buffers or sizes a captured step refers to change
 };
@@ -285,9 +287,10 @@ static int check_n(opalsc_ctx *c, int64_t N) {
     return 0;
 }
 
-// The periodic-z (DC beam) context provides the Poisson solve only.
+// A periodic-z (DC beam) context has no energy bins or image charge (the
+// shifted Green's function is an open-boundary construction).
 static int reject_periodic_z(opalsc_ctx *c) {
-    if (c->grid.pz) FAIL(c, OPALSC_EINVAL, "a periodic-z context provides opalsc_compute_potential only");
+    if (c->grid.pz) FAIL(c, OPALSC_EINVAL, "energy bins and image charge need an open-boundary context");
     return 0;
 }
 
@@ -415,7 +418,8 @@ extern "C" int opalsc_boundp(opalsc_ctx *c, int64_t N, const double *R, double h
     if (int e = check_n(c, N)) return e;
     if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
     launch_bounds(c->stream, N, c->aosR.as<double>(), c->partials);
-    launch_boundp_scalars(c->stream, c->partials, N > 0 ? red_blocks(N) : 0, c->cfg.n, c->dh, c->d_sc);
+    launch_boundp_scalars(c->stream, c->partials, N > 0 ? red_blocks(N) : 0, c->cfg.n, c->dh, c->grid.pz, c->period,
+                          c->d_sc);
     CHECK_LAUNCH(c);
     Scalars s;
     if (int e = download_scalars(c, s)) return e;
@@ -424,6 +428,37 @@ extern "C" int opalsc_boundp(opalsc_ctx *c, int64_t N, const double *R, double h
         hr[d] = s.hr[d];
         origin[d] = s.origin[d];
     }
+    return OPALSC_OK;
+}
+
+// PartBunchBase::setBeamFrequency (src/Classic/Algorithms/PartBunchBase.hpp:2534-2537):
+// the DC beam's period length becomes c / f.
+extern "C" int opalsc_set_beam_frequency(opalsc_ctx *c, double f) {
+    if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if (!(f > 0.0) || !std::isfinite(f)) FAIL(c, OPALSC_EINVAL, "beam frequency must be positive and finite");
+    c->period = kC / f;
+    if (c->have_track)
+        HIPCHK(c, hipMemcpyAsync((char *)c->d_ts + offsetof(TrackState, period), &c->period, sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
+// The particle boundary condition boundp's update() applies to a DC beam
+// (ParticleSpatialLayout.hpp:296-297): z wrapped into [origin, n*h + origin).
+extern "C" int opalsc_wrap_periodic_z(opalsc_ctx *c, int64_t N, double *R, const double origin[3],
+                                      const double hr[3]) {
+    if (!c || (!R && N > 0) || !origin || !hr) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if (int e = check_n(c, N)) return e;
+    const double lo = 0 * hr[2] + origin[2], hi = c->cfg.n[2] * hr[2] + origin[2];
+    if (!(hi > lo)) FAIL(c, OPALSC_EINVAL, "the mesh has no extent in z");
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    launch_wrap_periodic_z(c->stream, N, c->aosR.as<double>(), lo, hi);
+    CHECK_LAUNCH(c);
+    if (N > 0) HIPCHK(c, hipMemcpyAsync(R, c->aosR.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return OPALSC_OK;
 }
 
@@ -487,7 +522,6 @@ static int self_fields_checks(opalsc_ctx *c, int64_t N, const void *R, const voi
                               const double origin[3], const double hr[3], const void *Ef, const void *Bf) {
     if (!c || !origin || !hr) return OPALSC_EINVAL;
     if (int e_ = use_device(c)) return e_;
-    if (int e = reject_periodic_z(c)) return e;
     if (N > 0 && (!R || !P || !Q || !dt || !Ef || !Bf)) FAIL(c, OPALSC_EINVAL, "null particle array");
     if (int e = check_n(c, N)) return e;
     if (N == 0) FAIL(c, OPALSC_EINVAL, "computeSelfFields needs particles (the Lorentz factor divides by N)");
@@ -503,7 +537,7 @@ static int self_fields_core(opalsc_ctx *c, int64_t N, const double *R, const dou
     launch_soa_from_aos(c->stream, N, R, b.x, b.y, b.z);
     launch_selffield_prep(c->stream, N, P, Q, dt, b.qd, c->partials, &c->d_counters[0]);
     launch_selffield_scalars(c->stream, c->partials, red_blocks(N), N, dT, origin, hr, c->cfg.n, coupling,
-                             &c->d_counters[0], c->grid.fp64, c->d_sc);
+                             &c->d_counters[0], c->grid.fp64, c->grid.pz, c->d_sc);
     launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
     launch_deposit(c->stream, 0, c->grid.fp64, N, b.x, b.y, b.z, nullptr, b.qd, (long long *)c->grid.rho_i64,
                    c->d_sc, &c->d_counters[1], nullptr);
@@ -827,7 +861,8 @@ extern "C" int opalsc_bunch_get(opalsc_ctx *c, double *R, double *P, double *Q) 
 extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     if (!c || !p) return OPALSC_EINVAL;
     if (int e_ = use_device(c)) return e_;
-    if (int e = reject_periodic_z(c)) return e;
+    if (c->bins.nbins > 0)
+        if (int e = reject_periodic_z(c)) return e;
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "track_init before bunch_set/bunch_generate");
     if (!(p->dT > 0.0)) FAIL(c, OPALSC_EINVAL, "dT must be positive");
     TrackState ts;
@@ -848,6 +883,8 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     }
     ts.space_charge = p->space_charge;
     ts.step = 0;
+    ts.pz = c->grid.pz ? 1 : 0;
+    ts.period = c->period;
     HIPCHK(c, hipMemcpyAsync(c->d_ts, &ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     Scalars s;
     memset(&s, 0, sizeof(s));
@@ -915,6 +952,8 @@ extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbin
     if (!c) return OPALSC_EINVAL;
     if (int e_ = use_device(c)) return e_;
     if (!c->have_bunch) FAIL(c, OPALSC_ESTATE, "set_bins before bunch_set/bunch_generate");
+    if (nbins > 0)
+        if (int e = reject_periodic_z(c)) return e;
     if (nbins == 0) {
         c->bins.nbins = 0;
         ++c->gen;

@@ -54,6 +54,10 @@ struct Scalars {
     double qu, qd_u;           // tracker, every particle of charge qu: qu and this step's qu*dT
     int mesh_ok;
     int64_t N;
+    // DC beam (periodic z): period length, and the z range the particles are
+    // wrapped into after boundp, [wrap_lo, wrap_hi) in the beam frame
+    int pz;
+    double period, wrap_lo, wrap_hi;
 };
 
 // Tracker state carried across steps on the device.
@@ -65,7 +69,34 @@ struct TrackState {
     int space_charge;
     int step;
     int call_step0;  // step at the start of the current tracking call (statistics slot = step - call_step0 - 1)
+    int pz;          // DC beam: periodic z (mesh rule, particle wrap, periodic guards and solve)
+    double period;   // its period length (PartBunchBase::setBeamFrequency: c / f)
 };
+
+// The particle boundary condition of a DC beam in z, applied by boundp's
+// update() (ParticleSpatialLayout.hpp:296-297): ParticlePeriodicBCond
+// (ippl/src/Particle/ParticleBConds.h:38-45) over the region of the mesh's
+// cells, [origin, n*h + origin) (RegionLayout::convert_index,
+// ippl/src/Region/RegionLayout.hpp:526-556; getVertexPosition,
+// ippl/src/Meshes/UniformCartesian.hpp:1054-1055).
+__host__ __device__ inline double wrap_periodic(double t, double lo, double hi) {
+    if (t < lo) return hi - (lo - t);
+    if (t >= hi) return lo + (t - hi);
+    return t;
+}
+
+// Cell index of a stencil corner along z: with a periodic z face the guard
+// layers hold the opposite cells (PeriodicFace, filled before the gather by
+// Field::fillGuardCells, ippl/src/Field/Field.hpp:366-381), otherwise zero.
+// Returns -1 for a corner that reads 0.
+__host__ __device__ inline int corner_z(int k, int n2, bool pz) {
+    if (k >= 0 && k < n2) return k;
+    if (pz) {
+        if (k == n2) return 0;
+        if (k == -1) return n2 - 1;
+    }
+    return -1;
+}
 
 // ---- cloud-in-cell lookup -------------------------------------------------
 // One dimension of IntCICImpl<3>::scatter/gather (ippl/src/Particle/IntCIC.h:826-851):

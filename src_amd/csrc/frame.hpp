@@ -12,10 +12,12 @@ namespace opalsc {
 // non-DC beam, not emitting; returns false for "h<0, can not build a mesh".
 __host__ __device__ inline bool boundp_mesh(const double rmin_in[3], const double rmax_in[3],
                                             const int n[3], double dh, double hr[3],
-                                            double origin[3]) {
+                                            double origin[3], bool dc = false, double period = 0.0) {
     double rmin[3] = {rmin_in[0], rmin_in[1], rmin_in[2]};
     double rmax[3] = {rmax_in[0], rmax_in[1], rmax_in[2]};
-    for (int i = 0; i < 3; ++i) {
+    // a DC beam (periodic in z) leaves z to the period (PartBunchBase.hpp:814,840-843)
+    const int dims = dc ? 2 : 3;
+    for (int i = 0; i < dims; ++i) {
         double length = fabs(rmax[i] - rmin[i]);
         if (length < 1e-10) {
             rmax[i] += 1e-10;
@@ -25,6 +27,10 @@ __host__ __device__ inline bool boundp_mesh(const double rmin_in[3], const doubl
             rmin[i] -= dh * length;
         }
         hr[i] = (rmax[i] - rmin[i]) / (n[i] - 1);
+    }
+    if (dc) {
+        rmax[2] = rmin[2] + period;
+        hr[2] = period / (n[2] - 1);
     }
     if (hr[0] * hr[1] * hr[2] <= 0) return false;
     for (int i = 0; i < 3; ++i) origin[i] = rmin[i] - hr[i] / 2.0;

@@ -122,7 +122,7 @@ void launch_selffield_prep(hipStream_t s, int64_t N, const double *P, double *Q,
 void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
                               const double origin[3], const double hr[3], const int n[3],
                               double coupling, const unsigned long long *qdmax_bits, bool exact,
-                              Scalars *sc);
+                              bool pz, Scalars *sc);
 void launch_deposit(hipStream_t s, int mode, bool exact, int64_t N, const double *x, const double *y,
                     const double *z, double *q, const double *qd, long long *rho, const Scalars *sc,
                     unsigned long long *dropped, const double *chunkbox);
@@ -137,8 +137,11 @@ void launch_gather_selffield(hipStream_t s, bool fp64, int64_t N, const double *
 void launch_cic_index(hipStream_t s, int64_t N, const double *R, const double origin[3],
                       const double hr[3], int32_t *lgp, double *dpos);
 void launch_bounds(hipStream_t s, int64_t N, const double *R, double *partials);
+// pz: DC beam (z left to the period length, PartBunchBase.hpp:840-843)
 void launch_boundp_scalars(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
-                           Scalars *sc);
+                           bool pz, double period, Scalars *sc);
+// ParticlePeriodicBCond in z over [origin, n*h + origin) (AoS R, in place)
+void launch_wrap_periodic_z(hipStream_t s, int64_t N, double *R, double lo, double hi);
 void launch_push(hipStream_t s, int64_t N, double *R, const double *P, const double *dt);
 void launch_kick(hipStream_t s, int64_t N, double *P, const double *Ef, const double *Bf,
                  const double *dt, double charge, double mass);

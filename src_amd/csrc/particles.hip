@@ -121,7 +121,7 @@ __global__ __launch_bounds__(512) void k_selffield_scalars(const double *partial
                                                            double h0, double h1, double h2, int n0,
                                                            int n1, int n2, double coupling,
                                                            const unsigned long long *qdmax_bits,
-                                                           int exact, Scalars *sc) {
+                                                           int exact, int pz, Scalars *sc) {
     __shared__ double lds[16 * 2 * 2];
     double out[1];
     reduce_partials_dd<1>(partials, nb, out, lds);
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(512) void k_selffield_scalars(const double *partial
     sc->hr[2] = h2;
     sc->N = N;
     sc->mesh_ok = 1;
+    sc->pz = pz;
     sc->psum[2] = out[0];
     selffield_scalars(sc, out[0], N, dT, coupling);
     set_fixed_point(sc, N, __longlong_as_double((long long)*qdmax_bits), exact != 0);
@@ -145,10 +146,10 @@ __global__ __launch_bounds__(512) void k_selffield_scalars(const double *partial
 void launch_selffield_scalars(hipStream_t s, const double *partials, int nb, int64_t N, double dT,
                               const double origin[3], const double hr[3], const int n[3],
                               double coupling, const unsigned long long *qdmax_bits, bool exact,
-                              Scalars *sc) {
+                              bool pz, Scalars *sc) {
     k_selffield_scalars<<<1, kReduceThreads, 0, s>>>(partials, nb, N, dT, origin[0], origin[1], origin[2],
                                                hr[0], hr[1], hr[2], n[0], n[1], n[2], coupling,
-                                               qdmax_bits, exact ? 1 : 0, sc);
+                                               qdmax_bits, exact ? 1 : 0, pz ? 1 : 0, sc);
 }
 
 // ---- CIC scatter into int64 fixed point -------------------------------------
@@ -262,6 +263,8 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
         double M[9], o[3];
         for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
         for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
+        const bool dcz = sc->pz != 0;
+        const double wlo = sc->wrap_lo, whi = sc->wrap_hi;
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) {
             double r[3] = {px[k], py[k], pz[k]};
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
             else transform_to_fast(M, o, r);
             px[k] = r[0];
             py[k] = r[1];
-            pz[k] = r[2];
+            pz[k] = dcz ? wrap_periodic(r[2], wlo, whi) : r[2];  // boundp's update() of a DC beam
         }
     }
 
@@ -558,13 +561,14 @@ __global__ __launch_bounds__(256) void k_gather_selffield(int64_t N, const doubl
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double betaC = sc->betaC;
+    const bool pz = sc->pz != 0;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
         int l[3];
         double d[3], e[3] = {0.0, 0.0, 0.0};
         bool ok = cic_dim(x[p], o0, h0, l[0], d[0]);
         ok &= cic_dim(y[p], o1, h1, l[1], d[1]);
         ok &= cic_dim(z[p], o2, h2, l[2], d[2]);
-        if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, e);
+        if (ok) gather_E<T>(Eg, l, d, n0, n1, n2, e, pz);
         Ef[3 * p] = e[0];
         Ef[3 * p + 1] = e[1];
         Ef[3 * p + 2] = e[2];
@@ -625,7 +629,7 @@ void launch_bounds(hipStream_t s, int64_t N, const double *R, double *partials) 
 }
 
 __global__ __launch_bounds__(512) void k_boundp_scalars(const double *partials, int nb, int n0, int n1,
-                                                        int n2, double dh, Scalars *sc) {
+                                                        int n2, double dh, int pz, double period, Scalars *sc) {
     __shared__ double lds[16 * 6];
     double mn[3], mx[3];
     reduce_partials_minmax(partials, nb, mn, mx, lds);
@@ -638,12 +642,20 @@ __global__ __launch_bounds__(512) void k_boundp_scalars(const double *partials, 
         sc->rmin[d] = mn[d];
         sc->rmax[d] = mx[d];
     }
-    sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin) ? 1 : 0;
+    sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin, pz != 0, period) ? 1 : 0;
 }
 
 void launch_boundp_scalars(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
-                           Scalars *sc) {
-    k_boundp_scalars<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, sc);
+                           bool pz, double period, Scalars *sc) {
+    k_boundp_scalars<<<1, kReduceThreads, 0, s>>>(partials, nb, n[0], n[1], n[2], dh, pz ? 1 : 0, period, sc);
+}
+
+__global__ void k_wrap_periodic_z(int64_t N, double *R, double lo, double hi) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        R[3 * i + 2] = wrap_periodic(R[3 * i + 2], lo, hi);
+}
+void launch_wrap_periodic_z(hipStream_t s, int64_t N, double *R, double lo, double hi) {
+    if (N > 0) k_wrap_periodic_z<<<stream_blocks(N), kThreads, 0, s>>>(N, R, lo, hi);
 }
 
 __global__ void k_push(int64_t N, double *R, const double *P, const double *dt) {
@@ -1088,7 +1100,11 @@ __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int 
         sc->rmin[d] = mn[d];
         sc->rmax[d] = mx[d];
     }
-    sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin) ? 1 : 0;
+    sc->mesh_ok = boundp_mesh(mn, mx, n, dh, sc->hr, sc->origin, ts->pz != 0, ts->period) ? 1 : 0;
+    sc->pz = ts->pz;
+    sc->period = ts->period;
+    sc->wrap_lo = 0 * sc->hr[2] + sc->origin[2];
+    sc->wrap_hi = n2 * sc->hr[2] + sc->origin[2];
     // mixed-precision gather pass: beam-frame positions relative to the mesh origin
     const double *Mr = sc->M_rb, *og = sc->origin;
     for (int d = 0; d < 3; ++d) {
@@ -1135,6 +1151,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double betaC = sc->betaC;
+    const bool dcz = sc->pz != 0;
+    const double wlo = sc->wrap_lo, whi = sc->wrap_hi;
     double Mrb[9], orb[3], Mb[9], ob[3], Mu[9], ou[3], Ee[3], Be[3];
     for (int k = 0; k < 9; ++k) {
         Mrb[k] = sc->M_rb[k];
@@ -1168,13 +1186,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
         if (sc_on) {
             transform_to(Mrb, orb, r);
+            if (dcz) r[2] = wrap_periodic(r[2], wlo, whi);  // boundp's update() of a DC beam
             int l[3];
             double d[3];
             bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
             ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
             ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
             if (ok) {
-                gather_E<double>(Eg, l, d, n0, n1, n2, Ef);
+                gather_E<double>(Eg, l, d, n0, n1, n2, Ef, dcz);
                 if constexpr (BINNED) gather_E<double>(Bg, l, d, n0, n1, n2, Bf);
             }
             if constexpr (!BINNED) {
@@ -1229,6 +1248,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double betaC = sc->betaC;
+    const bool dcz = sc->pz != 0;
+    const double zper = n2 * sc->hr[2];  // period of the mesh-relative z
     double Mr[9], Mc[9], orb[3], oc[3], ih[3], Ee[3], Be[3];
     for (int k = 0; k < 9; ++k) {
         Mr[k] = sc->M_rb[k];
@@ -1279,6 +1300,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             nr[5] = pz[j];
         }
         transform_to_fast(Mr, orb, r);  // beam frame, relative to the mesh origin
+        if (dcz && sc_on) r[2] = wrap_periodic(r[2], 0.0, zper);  // boundp's update() of a DC beam
         double pb[3];
         mat_vec_fast(Mr, p, pb);
         double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
@@ -1289,7 +1311,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             ok &= cic_dim_fast(r[1], 0.0, ih[1], l[1], d[1]);
             ok &= cic_dim_fast(r[2], 0.0, ih[2], l[2], d[2]);
             if (ok) {
-                gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
+                gather_E_fast(Eg, l, d, n0, n1, n2, Ef, dcz);
                 if constexpr (BINNED) gather_E_fast(Bg, l, d, n0, n1, n2, Bf);
             }
             if constexpr (!BINNED) {

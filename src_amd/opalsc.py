@@ -35,7 +35,7 @@ EXPORTS = [
     "opalsc_abi_version", "opalsc_create", "opalsc_destroy", "opalsc_last_error",
     "opalsc_synchronize", "opalsc_compute_potential", "opalsc_compute_potential_shifted",
     "opalsc_compute_potential_dev", "opalsc_compute_self_fields_dev", "opalsc_step",
-    "opalsc_boundp", "opalsc_cic_index",
+    "opalsc_boundp", "opalsc_set_beam_frequency", "opalsc_wrap_periodic_z", "opalsc_cic_index",
     "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_compute_self_fields_binned",
     "opalsc_get_grid", "opalsc_push_half",
     "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
@@ -104,6 +104,8 @@ def lib():
                                             vp, vp], C.c_int),
         "opalsc_step": ([vp, C.c_int, dp], C.c_int),
         "opalsc_boundp": ([vp, i64, dp, dp, dp], C.c_int),
+        "opalsc_set_beam_frequency": ([vp, C.c_double], C.c_int),
+        "opalsc_wrap_periodic_z": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_cic_index": ([vp, i64, dp, dp, dp, dp, dp], C.c_int),
         "opalsc_sort_by_cell": ([vp, i64, dp, dp, dp, dp], C.c_int),
         "opalsc_compute_self_fields": ([vp, i64, dp, dp, dp, dp, C.c_double, dp, dp, C.c_double,
@@ -240,6 +242,16 @@ class SpaceChargeContext:
         hr, origin = np.zeros(3), np.zeros(3)
         self._c(lib().opalsc_boundp(self._h, len(R), _ptr(R), _ptr(hr), _ptr(origin)))
         return hr, origin
+
+    # PartBunchBase::setBeamFrequency: the DC beam's period length is c / f
+    def setBeamFrequency(self, f):
+        self._c(lib().opalsc_set_beam_frequency(self._h, float(f)))
+
+    # boundp's update() for a DC beam: z wrapped into [origin, n*h + origin)
+    def wrap_periodic_z(self, R, origin, hr):
+        R = _d(R, (-1, 3)).copy()
+        self._c(lib().opalsc_wrap_periodic_z(self._h, len(R), _ptr(R), _ptr(_d(origin)), _ptr(_d(hr))))
+        return R
 
     def cic_index(self, R, origin, hr):
         R = _d(R, (-1, 3))

@@ -38,7 +38,7 @@ class Tracker(C.Structure):
                 ("mass_eV", C.c_double), ("coupling", C.c_double), ("Bz", C.c_double),
                 ("Ez0", C.c_double), ("rf_freq", C.c_double), ("rf_phase", C.c_double),
                 ("t", C.c_double), ("pathLength", C.c_double), ("RefPartR", C.c_double * 3),
-                ("RefPartP", C.c_double * 3)]
+                ("RefPartP", C.c_double * 3), ("bcz", C.c_int), ("period", C.c_double)]
 
 
 _lib = None
@@ -60,6 +60,10 @@ def lib():
     L.orc_scatter.argtypes = [i64, PD, PD, PD, PD, PI32, PD]
     L.orc_scatter.restype = i64
     L.orc_gather.argtypes = [i64, PD, PD, PD, PD, PI32, PD]
+    L.orc_gather_periodic_z.argtypes = [i64, PD, PD, PD, PD, PI32, PD]
+    L.orc_boundp_dc.argtypes = [i64, PD, PI32, C.c_double, C.c_double, PD, PD]
+    L.orc_boundp_dc.restype = C.c_int
+    L.orc_wrap_periodic_z.argtypes = [i64, PD, PD, PD, PI32]
     L.orc_fft3d.argtypes = [PC, PI32, C.c_int]
     L.orc_fft1d.argtypes = [PC, C.c_int, C.c_int]
     L.orc_igf_octant.argtypes = [PI32, PD, PD, C.c_int]
@@ -72,6 +76,8 @@ def lib():
                                           C.c_double, C.c_int, PD, PD, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
     L.orc_compute_self_fields.restype = i64
+    L.orc_compute_self_fields_periodic_z.argtypes = L.orc_compute_self_fields.argtypes
+    L.orc_compute_self_fields_periodic_z.restype = i64
     L.orc_push.argtypes = [i64, PD, PD, PD]
     L.orc_kick.argtypes = [i64, PD, PD, PD, PD, C.c_double, C.c_double]
     L.orc_calc_moments.argtypes = [i64, PD, PD, PD]
@@ -115,13 +121,23 @@ def get_bounds(R):
     return lo, hi
 
 
-def boundp(R, n, dh=0.02):
+def boundp(R, n, dh=0.02, period=None):
+    """period: DC beam (periodic z) with this period length."""
     R = _d(R)
     hr, origin = np.zeros(3), np.zeros(3)
-    rc = lib().orc_boundp(len(R), R, _n(n), dh, hr, origin)
+    if period is None:
+        rc = lib().orc_boundp(len(R), R, _n(n), dh, hr, origin)
+    else:
+        rc = lib().orc_boundp_dc(len(R), R, _n(n), dh, float(period), hr, origin)
     if rc != 0:
         raise ValueError("h<0, can not build a mesh")
     return hr, origin
+
+
+def wrap_periodic_z(R, origin, hr, n):
+    R = _d(R).copy()
+    lib().orc_wrap_periodic_z(len(R), R, _d(origin), _d(hr), _n(n))
+    return R
 
 
 def cic_index(R, origin, hr):
@@ -143,10 +159,11 @@ def scatter(R, q, origin, hr, n, rho=None):
     return rho, dropped
 
 
-def gather(R, E, origin, hr, n):
+def gather(R, E, origin, hr, n, periodic_z=False):
     R = _d(R)
     Ef = np.zeros((len(R), 3))
-    lib().orc_gather(len(R), R, _d(E), _d(origin), _d(hr), _n(n), Ef)
+    f = lib().orc_gather_periodic_z if periodic_z else lib().orc_gather
+    f(len(R), R, _d(E), _d(origin), _d(hr), _n(n), Ef)
     return Ef
 
 
@@ -243,7 +260,7 @@ def grad(n, phi, hr):
 
 
 def compute_self_fields(R, P, Q, dt, dT, origin, hr, n, coupling=COUPLING, greens=0, Bf=None,
-                        want_grids=False):
+                        want_grids=False, periodic_z=False):
     R, P = _d(R), _d(P)
     Q = _d(Q).copy()
     N = len(R)
@@ -255,8 +272,9 @@ def compute_self_fields(R, P, Q, dt, dT, origin, hr, n, coupling=COUPLING, green
     phi = np.zeros(ncell) if want_grids else None
     E = np.zeros(3 * ncell) if want_grids else None
     ptr = (lambda a: a.ctypes.data if a is not None else None)
-    dropped = lib().orc_compute_self_fields(N, R, P, Q, _d(dt), dT, _d(origin), _d(hr), n,
-                                            coupling, greens, Ef, Bf, ptr(rho), ptr(phi), ptr(E))
+    f = lib().orc_compute_self_fields_periodic_z if periodic_z else lib().orc_compute_self_fields
+    dropped = f(N, R, P, Q, _d(dt), dT, _d(origin), _d(hr), n, coupling, greens, Ef, Bf, ptr(rho), ptr(phi),
+                ptr(E))
     out = dict(Ef=Ef, Bf=Bf, Q=Q, dropped=dropped)
     if want_grids:
         out.update(rho=rho, phi=phi, E=E.reshape(-1, 3))
@@ -301,7 +319,8 @@ def quaternion_to_z(u):
 
 
 def make_tracker(n, dT, charge=1.0, mass=M_P_EV, dh=0.02, greens=0, space_charge=True, Bz=0.0,
-                 Ez0=0.0, rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), coupling=COUPLING):
+                 Ez0=0.0, rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), coupling=COUPLING,
+                 periodic_z=False, period=CLIGHT / 1e9):
     t = Tracker()
     t.n[:] = list(n)
     t.dh, t.greens, t.space_charge, t.dT = dh, greens, int(space_charge), dT
@@ -310,6 +329,7 @@ def make_tracker(n, dT, charge=1.0, mass=M_P_EV, dh=0.02, greens=0, space_charge
     t.t, t.pathLength = 0.0, 0.0
     t.RefPartR[:] = [0.0, 0.0, 0.0]
     t.RefPartP[:] = list(ref_p)
+    t.bcz, t.period = int(periodic_z), float(period)
     return t
 
 

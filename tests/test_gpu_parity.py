@@ -431,20 +431,110 @@ def test_compute_potential_periodic_z(n, greens, precision, tol):
     assert rel_l2(phi, ref) < tol
 
 
-def test_periodic_z_context_rejects_particle_entry_points():
+def test_periodic_z_context_rejects_bins_and_image_charge():
     R, P, Q = gaussian_bunch(100, seed=3)
     dt = np.full(100, 1e-11)
     with sc.SpaceChargeContext((8, 8, 8), max_particles=100, bcz=sc.BC_PERIODIC) as c:
-        hr, org = c.boundp(R)  # the mesh entry points do not depend on the solver
-        with pytest.raises(sc.OpalscError, match="periodic"):
-            c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr)
-        with pytest.raises(sc.OpalscError, match="periodic"):
+        hr, org = c.boundp(R)
+        with pytest.raises(sc.OpalscError, match="open-boundary"):
             c.computePotentialShifted(np.zeros(512), hr, 0.1)
+        with pytest.raises(sc.OpalscError, match="open-boundary"):
+            c.computeSelfFieldsBinned(R, P, np.zeros(100, np.int32), 1, 1e-12, dt, 1e-11, org, hr)
         c.set_bunch(R, P, Q)
-        with pytest.raises(sc.OpalscError, match="periodic"):
-            c.track_init(1e-11)
+        with pytest.raises(sc.OpalscError, match="open-boundary"):
+            c.set_bins(np.zeros(100, np.int32), 1, 1e-12)
+        with pytest.raises(sc.OpalscError, match="beam frequency"):
+            c.setBeamFrequency(0.0)
     with pytest.raises(sc.OpalscError, match="periodic z"):
         sc.SpaceChargeContext((8, 8, 6), max_particles=0, bcz=sc.BC_PERIODIC)
+
+
+def _dc_bunch(N, seed, zlen):
+    """Round beam, uniform in z over zlen, <P> exactly along z (momenta in +- pairs)."""
+    rng = np.random.default_rng(seed)
+    R = np.column_stack([rng.normal(0, 1e-3, N), rng.normal(0, 1e-3, N), rng.uniform(-zlen / 2, zlen / 2, N)])
+    P = np.zeros((N, 3))
+    for d in (0, 1):
+        P[:, d] = rng.normal(0, 1e-3, N // 2).repeat(2) * np.tile([1.0, -1.0], N // 2)
+    P[:, 2] = 0.046181 + rng.normal(0, 1e-3, N)
+    return R, P, np.full(N, 1e-9 / N)
+
+
+@pytest.mark.parametrize("freq", [1e9, 2.5e10])
+def test_dc_boundp_and_wrap_bit_exact(freq):
+    """DC-beam mesh rule (z given the period c/f) and the particle wrap, against
+    the oracle bit for bit; the second frequency makes the period shorter than
+    the bunch, so particles wrap."""
+    n = (16, 16, 32)
+    R, P, Q = _dc_bunch(20000, seed=12, zlen=0.02)
+    period = O.CLIGHT / freq
+    hro, orgo = O.boundp(R, n, period=period)
+    Rwo = O.wrap_periodic_z(R, orgo, hro, n)
+    with sc.SpaceChargeContext(n, max_particles=len(R), bcz=sc.BC_PERIODIC) as c:
+        c.setBeamFrequency(freq)
+        hr, org = c.boundp(R)
+        Rw = c.wrap_periodic_z(R, org, hr)
+    assert np.array_equal(hr, hro) and np.array_equal(org, orgo)
+    assert np.array_equal(Rw, Rwo)
+    assert (not np.array_equal(Rw, R)) == (period < 0.02)
+
+
+@pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-12), (sc.MIXED, 1e-5)])
+def test_dc_self_fields_match_oracle(precision, tol):
+    """computeSelfFields of a DC beam: periodic-z solve and periodic gather."""
+    n, N, freq = (16, 16, 32), 60000, 2.5e10
+    R, P, Q = _dc_bunch(N, seed=13, zlen=0.02)
+    dt = np.full(N, 1e-11)
+    hr, org = O.boundp(R, n, period=O.CLIGHT / freq)
+    Rw = O.wrap_periodic_z(R, org, hr, n)
+    ref = O.compute_self_fields(Rw, P, Q, dt, 1e-11, org, hr, n, want_grids=True, periodic_z=True)
+    with sc.SpaceChargeContext(n, precision=precision, max_particles=N, bcz=sc.BC_PERIODIC) as c:
+        Ef, Bf, Q2 = c.computeSelfFields(Rw, P, Q, dt, 1e-11, org, hr)
+        E = c.get_grid(sc.GRID_EFIELD)
+    assert rel_l2(E, ref["E"]) < tol
+    assert rel_l2(Ef, ref["Ef"]) < tol
+    assert rel_l2(Bf, ref["Bf"]) < tol
+    # the top layer's particles read layer 0 through the periodic guard
+    top = np.floor((Rw[:, 2] - org[2]) / hr[2] - 0.5) >= n[2] - 1
+    assert top.any()
+
+
+def test_dc_tracker_matches_oracle():
+    """The DC-beam tracker (period shorter than the bunch: particles wrap in the
+    beam frame every step) against the oracle's ParallelTTracker loop."""
+    n, N, steps, freq = (16, 16, 32), 50000, 4, 2.5e10
+    R, P, Q = _dc_bunch(N, seed=14, zlen=0.02)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5, periodic_z=True,
+                        period=O.CLIGHT / freq)
+    Ro, Po, Qo, _, so = O.track(tr, R, P, Q, dt, steps)
+    with sc.SpaceChargeContext(n, max_particles=N, bcz=sc.BC_PERIODIC, sort_interval=2) as c:
+        c.setBeamFrequency(freq)
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5)
+        sg = c.track(steps)
+        Rg, Pg, Qg = c.get_bunch()
+    assert np.abs(Ro[:, 2] - R[:, 2]).max() > 1e-3  # some particles wrapped
+    assert rel_l2(Rg, Ro) < 1e-12
+    assert rel_l2(Pg - P, Po - P) < 1e-9
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm", "prms"):
+            assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])
+
+
+def test_dc_tracker_mixed_close_to_fp64():
+    n, N, steps, freq = (16, 16, 32), 50000, 4, 2.5e10
+    R, P, Q = _dc_bunch(N, seed=15, zlen=0.02)
+    out = []
+    for prec in (sc.FP64, sc.MIXED):
+        with sc.SpaceChargeContext(n, precision=prec, max_particles=N, bcz=sc.BC_PERIODIC) as c:
+            c.setBeamFrequency(freq)
+            c.set_bunch(R, P, Q)
+            c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+            out.append(c.track(steps))
+    for a, b in zip(out[0], out[1]):
+        for k in ("rrms", "eps_norm"):
+            assert np.allclose(a[k], b[k], rtol=1e-4, atol=0), (k, a[k], b[k])
 
 
 def _batch_setups():

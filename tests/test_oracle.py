@@ -543,3 +543,103 @@ def test_binned_fields_are_linear_in_the_bins():
                                          image=False, gammas=g[:1])
     dEz = one["Ef"][:, 2] - noimg["Ef"][:, 2]
     assert np.all(dEz < 0) or np.all(dEz > 0)
+
+
+# --- DC beam (BCFFTZ=PERIODIC): particle side -----------------------------------
+
+def test_boundp_dc_rule_by_hand():
+    """boundp of a DC beam (PartBunchBase.hpp:814,840-843): x and y as for an
+    open beam, z not padded: hz = period/(n2-1), origin_z = zmin - hz/2."""
+    R = np.array([[0.0, 0.0, 0.0], [1e-3, 2e-3, 5e-3], [-1e-3, -2e-3, 1e-3]])
+    n, period = (8, 8, 16), 0.01
+    hr, org = O.boundp(R, n, period=period)
+    hro, orgo = O.boundp(R, n)
+    assert np.array_equal(hr[:2], hro[:2]) and np.array_equal(org[:2], orgo[:2])
+    assert hr[2] == period / 15
+    assert org[2] == 0.0 - hr[2] / 2.0
+    assert hro[2] != hr[2]  # the open rule pads the z extent instead
+
+
+def test_wrap_periodic_z_by_hand():
+    """ParticlePeriodicBCond over [origin, n*h + origin): below the region
+    t -> max - (min - t), at or above it t -> min + (t - max); x, y untouched."""
+    origin, hr, n = np.array([0.0, 0.0, -0.5]), np.array([1.0, 1.0, 0.25]), (4, 4, 8)
+    z = np.array([-0.75, 1.5, 1.75, 0.3, -0.5, 1.25])
+    R = np.zeros((len(z), 3))
+    R[:, 0], R[:, 1], R[:, 2] = 7.0, -3.0, z
+    W = O.wrap_periodic_z(R, origin, hr, n)
+    assert np.array_equal(W[:, 2], [1.25, -0.5, -0.25, 0.3, -0.5, 1.25])
+    assert np.all(W[:, 0] == 7.0) and np.all(W[:, 1] == -3.0)
+
+
+def test_periodic_gather_reads_the_opposite_layer():
+    """With periodic z faces the gather's z guard layers hold the opposite
+    cells: a particle in the top cell reads layer 0 with its upper z weight, one
+    below the first cell centre reads layer n2-1 with its lower z weight; with
+    ZeroFace both read 0 there.  Weights 0.5 (x, y) and 0.25/0.75 (z) are exact."""
+    n = (4, 4, 6)
+    E = np.zeros((n[2], n[1], n[0], 3))
+    E[0] = (1.0, 2.0, 3.0)
+    E[n[2] - 1] = (4.0, 5.0, 6.0)
+    E = E.reshape(-1, 3)
+    origin, hr = np.zeros(3), np.ones(3)
+    R = np.array([[2.0, 2.0, 5.75],    # lgp = (1, 1, 5), dz = 0.25: upper corners at k = 6 -> 0
+                  [2.0, 2.0, 0.25]])   # lgp = (1, 1, -1), dz = 0.75: lower corners at k = -1 -> 5
+    Ep = O.gather(R, E, origin, hr, n, periodic_z=True)
+    Eo = O.gather(R, E, origin, hr, n)
+    assert np.array_equal(Ep[0], 0.25 * np.array([1.0, 2.0, 3.0]) + 0.75 * np.array([4.0, 5.0, 6.0]))
+    assert np.array_equal(Ep[1], 0.25 * np.array([4.0, 5.0, 6.0]) + 0.75 * np.array([1.0, 2.0, 3.0]))
+    assert np.array_equal(Eo[0], 0.75 * np.array([4.0, 5.0, 6.0]))
+    assert np.array_equal(Eo[1], 0.75 * np.array([1.0, 2.0, 3.0]))
+
+
+def test_dc_self_fields_of_a_long_beam_have_less_axial_field():
+    """A beam uniform in z over the period: with the periodic solve and the
+    periodic gather its middle sees a much weaker E_z than the same particles
+    on an open mesh, which see the end fields of a finite cylinder.  (The DC
+    mesh is not exactly periodic for the particles: its period is
+    n2*hz = period*n2/(n2-1), the top layer's upper deposits are dropped and
+    Grad is one-sided at the z faces, so E_z does not vanish near the ends.)"""
+    rng = np.random.default_rng(4)
+    N, n, period = 40000, (16, 16, 32), 0.02
+    R = np.column_stack([rng.normal(0, 1e-3, N), rng.normal(0, 1e-3, N), rng.uniform(0, period, N)])
+    P = np.zeros((N, 3))
+    P[:, 2] = 0.046181
+    Q, dt = np.full(N, 1e-9 / N), np.full(N, 1e-11)
+    hr, org = O.boundp(R, n, period=period)
+    Rw = O.wrap_periodic_z(R, org, hr, n)
+    assert np.array_equal(Rw, R)  # every particle lies inside the region already
+    dc = O.compute_self_fields(Rw, P, Q, dt, 1e-11, org, hr, n, periodic_z=True)
+    hro, orgo = O.boundp(R, n)
+    op = O.compute_self_fields(R, P, Q, dt, 1e-11, orgo, hro, n)
+    mid = np.abs(R[:, 2] - period / 2) < 0.1 * period
+    def rms_z(Ef):
+        return np.sqrt(np.mean(Ef[mid, 2] ** 2))
+    def rms_r(Ef):
+        return np.sqrt(np.mean(Ef[mid, 0] ** 2 + Ef[mid, 1] ** 2))
+    assert rms_z(dc["Ef"]) < 0.5 * rms_z(op["Ef"])
+    assert rms_r(dc["Ef"]) > 0.5 * rms_r(op["Ef"])  # the transverse field is the same order
+
+
+def test_dc_tracker_wraps_in_the_beam_frame():
+    """In the tracker's space-charge step the beam-frame z of every particle is
+    wrapped into the mesh's region after boundp; with <P> along z the beam frame
+    is the reference frame shifted by the path length (0 here)."""
+    rng = np.random.default_rng(8)
+    N, n, period = 4000, (8, 8, 16), 0.01
+    R = np.column_stack([rng.normal(0, 1e-3, N), rng.normal(0, 1e-3, N), rng.uniform(-0.008, 0.008, N)])
+    P = np.zeros((N, 3))
+    P[:, 0] = rng.normal(0, 1e-3, N // 2).repeat(2) * np.tile([1.0, -1.0], N // 2)
+    P[:, 1] = rng.normal(0, 1e-3, N // 2).repeat(2) * np.tile([1.0, -1.0], N // 2)
+    P[:, 2] = 0.046181
+    Q, dt = np.full(N, 1e-9 / N), np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), periodic_z=True, period=period)
+    R2, Ef, Bf = R.copy(), np.zeros((N, 3)), np.zeros((N, 3))
+    mesh = np.zeros(6)
+    import ctypes as C
+    O.lib().orc_space_charge(C.byref(tr), N, R2, P, Q.copy(), dt, Ef, Bf, mesh, None, None, None)
+    hr, org = mesh[:3], mesh[3:]
+    lo, hi = org[2], n[2] * hr[2] + org[2]
+    assert np.all(R2[:, 2] >= lo) and np.all(R2[:, 2] < hi + 1e-15)
+    moved = R2[:, 2] != R[:, 2]
+    assert moved.any() and np.allclose(R[moved, 2] - R2[moved, 2], hi - lo, rtol=0, atol=1e-15)
