@@ -263,6 +263,51 @@ int opalsc_step(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);
  * each a context that is already set up (bunch_set/bunch_generate,
  * optional set_bins, track_init), together.  The contexts must stay alive
  * while the batch exists and may differ in grid size and precision. */
+/* --- file formats -----------------------------------------------------------
+ * FROMFILE distributions (Distribution::getNumberOfParticlesInFile and
+ * createDistributionFromFile, src/Distribution/Distribution.cpp:1073-1100,1103-1250):
+ * an ASCII file whose first line that is not empty once '#' comments are
+ * removed starts with the particle count, followed by x px y py z pz per
+ * particle, whitespace separated.  momenta_in_ev != 0 is INPUTMOUNITS=EV: each
+ * momentum component converted with converteVToBetaGamma (Distribution.cpp:1047-1054)
+ * for the rest energy mass_eV.  As in the reference, values the stream cannot
+ * read leave zeros.  opalsc_fromfile_read fills R and P (AoS) for N particles,
+ * N the count in the file.  No context: errors are read with
+ * opalsc_last_error(NULL). */
+int opalsc_fromfile_count(const char *path, int64_t *N);
+int opalsc_fromfile_read(const char *path, int64_t N, int momenta_in_ev, double mass_eV, double *R, double *P);
+
+/* One line of the DataSink statistics file (src/Structure/DataSink.cpp:281-430),
+ * in its column order.  energy is the mean kinetic energy, max_r get_bounds'
+ * maximum, corr the x-px correlations (rprms), Dx..DDy the dispersion moments
+ * (calcBeamParameters, PartBunchBase.hpp:1529-1613), B_ref/E_ref (T, MV/m) the
+ * external field at the reference particle, R0/P0 the first particle set. */
+typedef struct {
+    double t_ns, s;
+    int64_t num_particles;
+    double charge, energy_MeV;
+    double rms_r[3], rms_p[3], emit[3], mean_r[3], ref_r[3], ref_p[3], max_r[3], corr[3];
+    double Dx, DDx, Dy, DDy;
+    double B_ref[3], E_ref[3];
+    double dE_MeV, dt_ns;
+    int64_t parts_outside; /* 0: no halo boundary (Options::beamHaloBoundary = 0) */
+    int has_particle0;     /* the R0/P0 columns are written */
+    double R0[3], P0[3];
+} opalsc_stat_row;
+
+/* The statistics line of the resident bunch after opalsc_track, as
+ * ParallelTTracker::writePhaseSpace passes it to DataSink (ParallelTTracker.cpp:1018-1071):
+ * external field at the reference particle at t - dT/2.  The reference
+ * particle is the tracker's (its frame; OPAL writes the lab frame, which the
+ * analytic element fields here do not distinguish). */
+int opalsc_stat_row_get(opalsc_ctx *ctx, opalsc_stat_row *row);
+
+/* DataSink::writeSDDSHeader + doWriteStatData (DataSink.cpp:281-430,585-925):
+ * new_file != 0 creates the file and writes the SDDS header (input_fn in its
+ * description), then the row is appended: each value followed by a tab padded
+ * to width 10, doubles in scientific notation with 15 digits. */
+int opalsc_stat_write(const char *path, const char *input_fn, int new_file, const opalsc_stat_row *row);
+
 typedef struct opalsc_batch opalsc_batch;
 int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, int nctx);
 int opalsc_batch_destroy(opalsc_batch *batch);

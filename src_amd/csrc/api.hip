@@ -46,6 +46,10 @@ std::string g_create_error;
 
 }  // namespace
 
+// Errors of the entry points without a context (creation, file I/O), read by
+// opalsc_last_error(NULL).
+void opalsc::set_global_error(const std::string &msg) { g_create_error = msg; }
+
 struct opalsc_ctx {
     opalsc_config cfg;
     std::string err;
@@ -1314,6 +1318,74 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
             for (int s = 0; s < nsteps; ++s) per_step[(size_t)s * n + i] = st[s];
         }
     }
+    return OPALSC_OK;
+}
+
+// One line of the DataSink statistics file for the resident bunch after
+// opalsc_track: calcBeamParameters (PartBunchBase.hpp:1529-1613) with the
+// dispersion moments and the energy spread, get_bounds' maximum, sum(Q), the
+// tracker's time, path length and reference particle, the external field at
+// the reference particle at t - dT/2 (ParallelTTracker::writePhaseSpace,
+// ParallelTTracker.cpp:1030-1040) and the first particle set (R[0], P[0]).
+extern "C" int opalsc_stat_row_get(opalsc_ctx *c, opalsc_stat_row *row) {
+    if (!c || !row) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if (!c->have_track) FAIL(c, OPALSC_ESTATE, "stat_row_get needs a tracked bunch (opalsc_track_init)");
+    const Bunch &b = c->bunch;
+    const int64_t N = b.N;
+    memset(row, 0, sizeof(*row));
+    TrackState ts;
+    HIPCHK(c, hipMemcpyAsync(&ts, c->d_ts, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    row->t_ns = ts.t * 1e9;
+    row->s = ts.pathLength;
+    row->num_particles = N;
+    for (int d = 0; d < 3; ++d) {
+        row->ref_r[d] = ts.RefPartR[d];
+        row->ref_p[d] = ts.RefPartP[d];
+    }
+    double E[3], B[3];
+    ext_fields(ts, ts.t - 0.5 * ts.dT, E, B);
+    for (int d = 0; d < 3; ++d) {
+        row->B_ref[d] = B[d];
+        row->E_ref[d] = E[d] * 1e-6;
+    }
+    row->dt_ns = ts.dT * 1e9;
+    if (N == 0) return OPALSC_OK;
+    if (!c->statsb.ensure(sizeof(StatsOut)) || !c->tmpd.ensure(16 * sizeof(double)))
+        FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    launch_moments_soa(c->stream, b, c->partials);
+    launch_stats_from_partials(c->stream, c->partials, red_blocks(N), N, ts.mass_eV, c->statsb.p);
+    CHECK_LAUNCH(c);
+    opalsc_stats st;
+    HIPCHK(c, hipMemcpyAsync(&st, c->statsb.p, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    launch_stat_extra(c->stream, b, c->q_uniform, c->q_value, c->partials, c->tmpd.as<double>());
+    CHECK_LAUNCH(c);
+    double ex[14];
+    HIPCHK(c, hipMemcpyAsync(ex, c->tmpd.p, sizeof(ex), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double Nd = (double)N;
+    row->charge = ex[4];
+    row->energy_MeV = st.ekin_MeV;
+    for (int d = 0; d < 3; ++d) {
+        row->rms_r[d] = st.rrms[d];
+        row->rms_p[d] = st.prms[d];
+        row->emit[d] = st.eps_norm[d];
+        row->mean_r[d] = st.rmean[d];
+        row->max_r[d] = ex[5 + d];
+        row->corr[d] = st.rprms[d];
+        row->R0[d] = ex[8 + d];
+        row->P0[d] = ex[11 + d];
+    }
+    row->Dx = ex[0] / Nd;
+    row->DDx = ex[1] / Nd;
+    row->Dy = ex[2] / Nd;
+    row->DDy = ex[3] / Nd;
+    // energy spread from the longitudinal momentum spread (PartBunchBase.hpp:1595-1604)
+    const double m0 = ts.mass_eV * 1.E-6;
+    const double tmp = 1.0 / std::pow(st.ekin_MeV / m0 + 1., 2.0);
+    row->dE_MeV = st.prms[2] * m0 * std::sqrt(1.0 - tmp);
+    row->has_particle0 = 1;
     return OPALSC_OK;
 }
 

@@ -8,6 +8,9 @@
 
 namespace opalsc {
 
+void set_global_error(const std::string &msg);
+
+
 enum ProfId {
     PROF_PUSH_BOUNDS = 0,  // push + beam-frame rotation + bounds
     PROF_SCALAR,           // single-thread scalar kernels
@@ -146,6 +149,11 @@ void launch_push(hipStream_t s, int64_t N, double *R, const double *P, const dou
 void launch_kick(hipStream_t s, int64_t N, double *P, const double *Ef, const double *Bf,
                  const double *dt, double charge, double mass);
 void launch_moments(hipStream_t s, int64_t N, const double *R, const double *P, double *partials);
+// statistics-file extras of the resident bunch: out[0..4] = sums of x*pz,
+// px*pz, y*pz, py*pz, Q; out[5..7] = max of x, y, z; out[8..13] = R, P of the
+// particle with id 0
+void launch_stat_extra(hipStream_t s, const Bunch &b, bool q_uniform, double q_value, double *partials,
+                       double *out);
 void launch_stats_from_partials(hipStream_t s, const double *partials, int nb, int64_t N,
                                 double mass_eV, void *stats_out);
 void launch_cell_keys(hipStream_t s, int64_t N, const double *x, const double *y, const double *z,

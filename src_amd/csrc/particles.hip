@@ -741,6 +741,80 @@ void launch_moments_soa(hipStream_t s, const Bunch &b, double *partials) {
     if (b.N > 0) k_moments_soa<<<red_blocks(b.N), kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, partials);
 }
 
+// Statistics file extras (calcBeamParameters' dispersion moments and the
+// charge sum, get_bounds' maximum, PartBunchBase.hpp:1529-1613,1178-1218,1727-1729):
+// per block double-double sums of x*pz, px*pz, y*pz, py*pz and Q (slots 0-9)
+// and the maxima of x, y, z (slots 10-12); the particle whose id is 0 (the
+// first one set) copied to p0[0..5].
+__global__ __launch_bounds__(256) void k_stat_extra(int64_t N, const double *x, const double *y, const double *z,
+                                                    const double *px, const double *py, const double *pz,
+                                                    const double *q, double q_uniform, const uint32_t *id,
+                                                    double *partials, double *p0) {
+    __shared__ double lds[4 * 5 * 2];
+    __shared__ double lmm[4 * 6];
+    double h[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, l[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = __builtin_inf();
+        mx[d] = -__builtin_inf();
+    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v[5] = {x[i] * pz[i], px[i] * pz[i], y[i] * pz[i], py[i] * pz[i], q ? q[i] : q_uniform};
+        for (int k = 0; k < 5; ++k) {
+            dd r = dd_add(dd{h[k], l[k]}, dd{v[k], 0.0});
+            h[k] = r.hi;
+            l[k] = r.lo;
+        }
+        const double r3[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; ++d) mx[d] = fmax(mx[d], r3[d]);
+        if (id[i] == 0) {
+            p0[0] = x[i];
+            p0[1] = y[i];
+            p0[2] = z[i];
+            p0[3] = px[i];
+            p0[4] = py[i];
+            p0[5] = pz[i];
+        }
+    }
+    block_sum_dd<5>(h, l, lds);
+    block_minmax(mn, mx, lmm);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 5; ++k) {
+            partials[blockIdx.x * kSlots + 2 * k] = h[k];
+            partials[blockIdx.x * kSlots + 2 * k + 1] = l[k];
+        }
+        for (int d = 0; d < 3; ++d) partials[blockIdx.x * kSlots + 10 + d] = mx[d];
+    }
+}
+
+__global__ __launch_bounds__(512) void k_stat_extra_final(const double *partials, int nb, double *out) {
+    __shared__ double lds[16 * 5 * 2];
+    double s[5];
+    reduce_partials_dd<5>(partials, nb, s, lds);
+    __shared__ double lmm[16 * 6];
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = __builtin_inf();
+        mx[d] = -__builtin_inf();
+    }
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int d = 0; d < 3; ++d) mx[d] = fmax(mx[d], partials[b * kSlots + 10 + d]);
+    block_minmax(mn, mx, lmm);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 5; ++k) out[k] = s[k];
+        for (int d = 0; d < 3; ++d) out[5 + d] = mx[d];
+    }
+}
+
+void launch_stat_extra(hipStream_t s, const Bunch &b, bool q_uniform, double q_value, double *partials,
+                       double *out) {
+    if (b.N <= 0) return;
+    const int nb = red_blocks(b.N);
+    k_stat_extra<<<nb, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, q_uniform ? nullptr : b.q,
+                                         q_value, b.id, partials, out + 8);
+    k_stat_extra_final<<<1, kReduceThreads, 0, s>>>(partials, nb, out);
+}
+
 __global__ __launch_bounds__(512) void k_stats_from_partials(const double *partials, int nb, int64_t N,
                                                              double mass_eV, StatsOut *out) {
     __shared__ double lds[16 * kMom * 2];

@@ -43,7 +43,8 @@ EXPORTS = [
     "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
     "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
-    "opalsc_copy_bandwidth",
+    "opalsc_copy_bandwidth", "opalsc_fromfile_count", "opalsc_fromfile_read", "opalsc_stat_row_get",
+    "opalsc_stat_write",
 ]
 
 
@@ -58,6 +59,18 @@ class Stats(C.Structure):
                 ("rmean", "pmean", "rrms", "prms", "eps_norm", "eps", "rprms")] + [
         ("gamma_mean", C.c_double), ("ekin_MeV", C.c_double), ("t", C.c_double),
         ("pathLength", C.c_double), ("n", C.c_int64), ("dropped", C.c_int64)]
+
+
+class StatRow(C.Structure):
+    """opalsc_stat_row: one line of the DataSink statistics file."""
+    _fields_ = [("t_ns", C.c_double), ("s", C.c_double), ("num_particles", C.c_int64),
+                ("charge", C.c_double), ("energy_MeV", C.c_double)] + [
+        (name, C.c_double * 3) for name in
+        ("rms_r", "rms_p", "emit", "mean_r", "ref_r", "ref_p", "max_r", "corr")] + [
+        ("Dx", C.c_double), ("DDx", C.c_double), ("Dy", C.c_double), ("DDy", C.c_double),
+        ("B_ref", C.c_double * 3), ("E_ref", C.c_double * 3), ("dE_MeV", C.c_double),
+        ("dt_ns", C.c_double), ("parts_outside", C.c_int64), ("has_particle0", C.c_int),
+        ("R0", C.c_double * 3), ("P0", C.c_double * 3)]
 
 
 class TrackParams(C.Structure):
@@ -131,6 +144,10 @@ def lib():
         "opalsc_track_get_params": ([vp, C.POINTER(TrackParams)], C.c_int),
         "opalsc_get_mesh": ([vp, dp, dp], C.c_int),
         "opalsc_host_atan_cr": ([i64, dp, dp], C.c_int),
+        "opalsc_fromfile_count": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
+        "opalsc_fromfile_read": ([C.c_char_p, i64, C.c_int, C.c_double, dp, dp], C.c_int),
+        "opalsc_stat_row_get": ([vp, C.POINTER(StatRow)], C.c_int),
+        "opalsc_stat_write": ([C.c_char_p, C.c_char_p, C.c_int, C.POINTER(StatRow)], C.c_int),
         "opalsc_host_log_cr": ([i64, dp, dp], C.c_int),
         "opalsc_device_atan_cr": ([vp, i64, dp, dp], C.c_int),
         "opalsc_device_log_cr": ([vp, i64, dp, dp], C.c_int),
@@ -163,6 +180,24 @@ def stats_to_dict(s):
     d.update(gamma_mean=s.gamma_mean, ekin_MeV=s.ekin_MeV, t=s.t, pathLength=s.pathLength,
              n=s.n, dropped=s.dropped)
     return d
+
+
+def fromfile_read(path, momenta_in_ev=False, mass_eV=M_P_EV):
+    """Distribution TYPE=FROMFILE: (R, P) as OPAL reads them (INPUTMOUNITS=EV
+    with momenta_in_ev)."""
+    n = C.c_int64()
+    p = str(path).encode()
+    _check(None, lib().opalsc_fromfile_count(p, C.byref(n)))
+    R = np.zeros((n.value, 3))
+    P = np.zeros((n.value, 3))
+    _check(None, lib().opalsc_fromfile_read(p, n.value, int(bool(momenta_in_ev)), float(mass_eV), _ptr(R), _ptr(P)))
+    return R, P
+
+
+def stat_write(path, row, input_fn="", new_file=False):
+    """Append one StatRow to a DataSink .stat file (new_file: header first)."""
+    _check(None, lib().opalsc_stat_write(str(path).encode(), str(input_fn).encode(), int(bool(new_file)),
+                                         C.byref(row)))
 
 
 def host_atan_cr(x):
@@ -393,6 +428,12 @@ class SpaceChargeContext:
         self._c(lib().opalsc_track(self._h, int(nsteps),
                                    C.cast(stats, C.c_void_p) if want_stats else None))
         return [stats_to_dict(stats[i]) for i in range(nsteps)] if want_stats else None
+
+    def stat_row(self):
+        """The DataSink statistics line of the resident bunch (after track)."""
+        row = StatRow()
+        self._c(lib().opalsc_stat_row_get(self._h, C.byref(row)))
+        return row
 
     def track_params(self):
         p = TrackParams()
