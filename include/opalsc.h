@@ -263,6 +263,27 @@ int opalsc_step(opalsc_ctx *ctx, int nsteps, opalsc_stats *per_step);
  * each a context that is already set up (bunch_set/bunch_generate,
  * optional set_bins, track_init), together.  The contexts must stay alive
  * while the batch exists and may differ in grid size and precision. */
+/* --- external field maps -------------------------------------------------------
+ * Elements on a straight beamline along z whose fields the tracker adds in
+ * every step (ParallelTTracker::computeExternalFields, ParallelTTracker.cpp:595-640,
+ * and updateReferenceParticle for the reference particle, 1134-1155): a
+ * Solenoid with a 2D magnetostatic map (FM2DMagnetoStatic; scale = KS) or an
+ * RFCavity with a 2D dynamic map (FM2DDynamic; scale = VOLT, freq_hz = FREQ,
+ * phase = LAG; the map's frequency when freq_hz is 0 or more than 1 % away,
+ * RFCavity.cpp:290-305).  The map's first z point sits at elemedge + zbegin in
+ * the lab frame, which is the reference frame moved by (0, 0, pathLength).
+ * Files are read as the reference reads them (Fieldmap.cpp:519-556,
+ * FM2DMagnetoStatic.cpp:11-140, FM2DDynamic.cpp:13-150).  At most 8 maps;
+ * opalsc_fieldmap_add returns the element's index.  A particle beyond a map's
+ * radius gets no field from it or from later maps (the reference removes it). */
+int opalsc_fieldmap_add(opalsc_ctx *ctx, const char *path, double elemedge, double scale, double freq_hz,
+                        double phase);
+int opalsc_fieldmap_clear(opalsc_ctx *ctx);
+/* The maps' field at lab positions R (AoS) and time t into E, B (overwritten);
+ * lost (may be NULL) counts particles beyond a map's radius. */
+int opalsc_fieldmap_eval(opalsc_ctx *ctx, int64_t N, const double *R, double t, double *E, double *B,
+                         int64_t *lost);
+
 /* --- file formats -----------------------------------------------------------
  * FROMFILE distributions (Distribution::getNumberOfParticlesInFile and
  * createDistributionFromFile, src/Distribution/Distribution.cpp:1073-1100,1103-1250):

@@ -1176,6 +1176,96 @@ void orc_rotate_to(const orc_trafo *t, int64_t N, double *V) {
 /* Tracker                                                                  */
 /* ------------------------------------------------------------------------ */
 
+/* A Solenoid (kind 0, FM2DMagnetoStatic map: a = Bz, b = Br) or an RFCavity
+ * (kind 1, FM2DDynamic map: a = Ez, b = Er, c = Bt), grids nz*nr indexed
+ * i + j*nz, in SI units after the map's normalisation. */
+typedef struct orc_fieldmap_s {
+    int kind, nz, nr;
+    double hz, hr, zstart, length, scale, omega, phase;
+    const double *a, *b, *c;
+} orc_fieldmap;
+
+/* FM2DMagnetoStatic::getFieldstrength (src/Classic/Fields/FM2DMagnetoStatic.cpp:161-190),
+ * FM2DDynamic::getFieldstrength (src/Classic/Fields/FM2DDynamic.cpp:169-205). */
+static int fieldmap_2d(const orc_fieldmap *m, double x, double y, double z, double e[3], double b[3]) {
+    const double RR = sqrt(x * x + y * y);
+    int indexr, indexz;
+    if (m->kind == 0) {
+        indexr = abs((int)floor(RR / m->hr));
+        indexz = abs((int)floor(z / m->hz));
+    } else {
+        indexr = (int)floor(RR / m->hr);
+        indexz = (int)floor(z / m->hz);
+    }
+    const double leverr = RR / m->hr - indexr;
+    const double leverz = z / m->hz - indexz;
+    if ((indexz < 0) || (indexz + 2 > m->nz)) return 0;
+    if (indexr + 2 > m->nr) return 1;
+    const int index1 = indexz + indexr * m->nz;
+    const int index2 = index1 + m->nz;
+#define ORC_LERP(f) ((1.0 - leverz) * (1.0 - leverr) * (f)[index1] + leverz * (1.0 - leverr) * (f)[index1 + 1] + \
+                     (1.0 - leverz) * leverr * (f)[index2] + leverz * leverr * (f)[index2 + 1])
+    if (m->kind == 0) {
+        const double BfieldR = ORC_LERP(m->b), BfieldZ = ORC_LERP(m->a);
+        if (RR != 0) {
+            b[0] += BfieldR * x / RR;
+            b[1] += BfieldR * y / RR;
+        }
+        b[2] += BfieldZ;
+    } else {
+        const double EfieldR = ORC_LERP(m->b), EfieldZ = ORC_LERP(m->a), BfieldT = ORC_LERP(m->c);
+        if (RR > 1e-10) {
+            e[0] += EfieldR * x / RR;
+            e[1] += EfieldR * y / RR;
+            b[0] -= BfieldT * y / RR;
+            b[1] += BfieldT * x / RR;
+        }
+        e[2] += EfieldZ;
+    }
+#undef ORC_LERP
+    return 0;
+}
+
+/* Solenoid::apply (src/Classic/AbsBeamline/Solenoid.cpp:167-179) and
+ * RFCavity::apply (src/Classic/AbsBeamline/RFCavity.cpp:232-250) at the lab
+ * position (x, y, zlab): 1 when the particle is beyond the map's radius. */
+int orc_fieldmap_apply(const orc_fieldmap *m, double x, double y, double zlab, double t, double E[3], double B[3]) {
+    const double z = zlab - m->zstart;
+    if (!(z >= 0.0 && z < m->length)) return 0;
+    double te[3] = {0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0};
+    if (fieldmap_2d(m, x, y, z, te, tb)) return 1;
+    if (m->kind == 0) {
+        for (int d = 0; d < 3; ++d) B[d] += m->scale * tb[d];
+    } else {
+        const double ce = m->scale * cos(m->omega * t + m->phase), sb = m->scale * sin(m->omega * t + m->phase);
+        for (int d = 0; d < 3; ++d) E[d] += ce * te[d];
+        for (int d = 0; d < 3; ++d) B[d] -= sb * tb[d];
+    }
+    return 0;
+}
+
+/* Every map at lab positions R (AoS) at time t into E, B (overwritten), maps
+ * in order, a particle beyond one map's radius skipping the later ones
+ * (computeExternalFields, src/Algorithms/ParallelTTracker.cpp:614-640).
+ * Returns the number of such particles. */
+int64_t orc_fieldmap_eval(int nmaps, const orc_fieldmap *maps, int64_t N, const double *R, double t, double *E,
+                          double *B) {
+    int64_t lost = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        double e[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+        int out = 0;
+        for (int k = 0; k < nmaps && !out; ++k) out = orc_fieldmap_apply(&maps[k], R[3 * i], R[3 * i + 1], R[3 * i + 2], t, e, b);
+        for (int d = 0; d < 3; ++d) {
+            E[3 * i + d] = e[d];
+            B[3 * i + d] = b[d];
+        }
+        lost += out;
+    }
+    return lost;
+}
+
+size_t orc_sizeof_fieldmap(void) { return sizeof(orc_fieldmap); }
+
 typedef struct {
     int n[3];
     double dh;        /* BBOXINCR/100 */
@@ -1194,6 +1284,10 @@ typedef struct {
     /* DC beam (BCFFTZ=PERIODIC): periodic z and its period length */
     int bcz;
     double period;
+    /* field maps on the beamline (see orc_fieldmap); the lab frame is the
+     * reference frame moved by (0, 0, pathLength) */
+    int nmaps;
+    const struct orc_fieldmap_s *maps;
 } orc_tracker;
 
 static void ext_fields(const orc_tracker *tr, double t, double E[3], double B[3]) {
@@ -1222,6 +1316,9 @@ static void update_ref(orc_tracker *tr, orc_trafo *upd) {
     ext_fields(tr, tr->t - 0.5 * dt, E, B);
     double Ef[3] = {0.0 + E[0], 0.0 + E[1], 0.0 + E[2]};
     double Bf[3] = {0.0 + B[0], 0.0 + B[1], 0.0 + B[2]};
+    /* elements at the reference particle (ParallelTTracker.cpp:1134-1155) */
+    for (int k = 0; k < tr->nmaps; ++k)
+        if (orc_fieldmap_apply(&tr->maps[k], R[0], R[1], R[2] + tr->pathLength, tr->t - 0.5 * dt, Ef, Bf)) break;
     boris_kick1(P, Ef, Bf, dt, tr->mass_eV, tr->charge);
     for (int d = 0; d < 3; ++d) R[d] /= scale;
     {
@@ -1343,6 +1440,11 @@ static int track_impl(orc_tracker *tr, int64_t N, double *R, double *P, double *
                 Ef[3 * i + d] += E[d];
                 Bf[3 * i + d] += B[d];
             }
+            /* computeExternalFields (ParallelTTracker.cpp:595-640) */
+            for (int k = 0; k < tr->nmaps; ++k)
+                if (orc_fieldmap_apply(&tr->maps[k], R[3 * i], R[3 * i + 1], R[3 * i + 2] + tr->pathLength,
+                                       tr->t + 0.5 * dt[i], &Ef[3 * i], &Bf[3 * i]))
+                    break;
         }
         orc_kick(N, P, Ef, Bf, dt, tr->charge, tr->mass_eV);
         orc_push(N, R, P, dt);

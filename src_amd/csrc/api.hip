@@ -63,6 +63,8 @@ struct opalsc_ctx {
     double *partials = nullptr;
     unsigned long long *d_counters = nullptr;  // [0] qd max bits, [1] dropped
     DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
+    std::vector<FieldMapHost> fmaps;  // field maps on the beamline, in the order they were added
+    DevBuf fm_desc, fm_data;
     DevBuf binb, gamb, cntb;  // energy bins: per-particle bin, per-bin gamma, per-bin count
     // tracker energy bins: the resident bunch is ordered by bin
     BinRanges bins;
@@ -262,7 +264,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     c->grid.destroy();
     free_bunch(c->bunch);
     DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb,      &c->statsb,  &c->tmpd,
-                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb};
+                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->fm_desc, &c->fm_data};
     for (DevBuf *b : bs) b->release();
     void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
     for (void *p : ps)
@@ -432,6 +434,116 @@ extern "C" int opalsc_boundp(opalsc_ctx *c, int64_t N, const double *R, double h
         hr[d] = s.hr[d];
         origin[d] = s.origin[d];
     }
+    return OPALSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field maps on the beamline
+// ---------------------------------------------------------------------------
+// Uploads every map (descriptors with offsets into one data array) and points
+// the tracker state at them.
+static int upload_fieldmaps(opalsc_ctx *c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<FieldMapDesc> desc;
+    std::vector<double> data;
+    for (const FieldMapHost &h : c->fmaps) {
+        FieldMapDesc d = h.d;
+        const int64_t base = (int64_t)data.size();
+        d.a = base;
+        data.insert(data.end(), h.a.begin(), h.a.end());
+        d.b = (int64_t)data.size();
+        data.insert(data.end(), h.b.begin(), h.b.end());
+        d.c = (int64_t)data.size();
+        data.insert(data.end(), h.c.begin(), h.c.end());
+        desc.push_back(d);
+    }
+    if (!c->fm_desc.ensure(sizeof(FieldMapDesc) * kMaxFieldMaps) || !c->fm_data.ensure(data.size() * 8))
+        FAIL(c, OPALSC_ENOMEM, "field map allocation failed");
+    if (!desc.empty())
+        HIPCHK(c, hipMemcpyAsync(c->fm_desc.p, desc.data(), desc.size() * sizeof(FieldMapDesc),
+                                 hipMemcpyHostToDevice, c->stream));
+    if (!data.empty())
+        HIPCHK(c, hipMemcpyAsync(c->fm_data.p, data.data(), data.size() * 8, hipMemcpyHostToDevice, c->stream));
+    if (c->have_track) {
+        struct {
+            int nmaps;
+            const FieldMapDesc *maps;
+            const double *mapdata;
+        } f = {(int)desc.size(), c->fm_desc.as<FieldMapDesc>(), c->fm_data.as<double>()};
+        static_assert(offsetof(TrackState, maps) - offsetof(TrackState, nmaps) == offsetof(decltype(f), maps),
+                      "field map members of TrackState");
+        static_assert(offsetof(TrackState, mapdata) - offsetof(TrackState, nmaps) == offsetof(decltype(f), mapdata),
+                      "field map members of TrackState");
+        HIPCHK(c, hipMemcpyAsync((char *)c->d_ts + offsetof(TrackState, nmaps), &f, sizeof(f), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    ++c->gen;
+    return OPALSC_OK;
+}
+
+// A Solenoid (2D magnetostatic map) or RFCavity (2D dynamic map) whose map
+// starts at elemedge on the beamline (Solenoid::initialise, Solenoid.cpp:195-214;
+// RFCavity::initialise, RFCavity.cpp:285-310): scale is KS or VOLT, freq_hz
+// the cavity's frequency (the map's when 0 or more than 1 % away from it, as
+// there), phase its LAG.  Returns the element's index or a negative error.
+extern "C" int opalsc_fieldmap_add(opalsc_ctx *c, const char *path, double elemedge, double scale,
+                                   double freq_hz, double phase) {
+    if (!c || !path) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if ((int)c->fmaps.size() >= kMaxFieldMaps) FAIL(c, OPALSC_EINVAL, "at most 8 field maps");
+    FieldMapHost h;
+    std::string err;
+    if (parse_fieldmap_2d(path, h, err) != 0) FAIL(c, OPALSC_EINVAL, err);
+    h.d.zstart = elemedge + h.d.zstart;
+    h.d.scale = scale;
+    if (h.d.kind == 1) {
+        const double two_pi = 2.0 * kPi;
+        double omega = freq_hz * two_pi;
+        if (!(omega > 0.0) || std::abs((omega - h.frequency) / omega) > 0.01) omega = h.frequency;
+        h.d.omega = omega;
+        h.d.phase = phase;
+    }
+    c->fmaps.push_back(std::move(h));
+    if (int e = upload_fieldmaps(c)) {
+        c->fmaps.pop_back();
+        return e;
+    }
+    return (int)c->fmaps.size() - 1;
+}
+
+extern "C" int opalsc_fieldmap_clear(opalsc_ctx *c) {
+    if (!c) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    c->fmaps.clear();
+    return upload_fieldmaps(c);
+}
+
+// The field of every map at lab positions R (AoS) and time t, summed in the
+// order the maps were added into E and B (overwritten); lost (may be NULL)
+// counts the particles beyond a map's radius.
+extern "C" int opalsc_fieldmap_eval(opalsc_ctx *c, int64_t N, const double *R, double t, double *E, double *B,
+                                    int64_t *lost) {
+    if (!c || (N > 0 && (!R || !E || !B))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if (int e = check_n(c, N)) return e;
+    if (!c->fm_desc.p && upload_fieldmaps(c)) return OPALSC_ENOMEM;
+    if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
+    if (!c->aosEf.ensure((size_t)N * 24 + 8) || !c->aosBf.ensure((size_t)N * 24 + 8))
+        FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 2, c->stream));
+    launch_fieldmap_eval(c->stream, N, c->aosR.as<double>(), t, c->fm_desc.as<FieldMapDesc>(),
+                         (int)c->fmaps.size(), c->fm_data.as<double>(), c->aosEf.as<double>(),
+                         c->aosBf.as<double>(), &c->d_counters[1]);
+    CHECK_LAUNCH(c);
+    unsigned long long nl = 0;
+    if (N > 0) {
+        HIPCHK(c, hipMemcpyAsync(E, c->aosEf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(B, c->aosBf.p, (size_t)N * 24, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(&nl, &c->d_counters[1], sizeof(nl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (lost) *lost = (int64_t)nl;
     return OPALSC_OK;
 }
 
@@ -888,6 +1000,9 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     ts.space_charge = p->space_charge;
     ts.step = 0;
     ts.pz = c->grid.pz ? 1 : 0;
+    ts.nmaps = (int)c->fmaps.size();
+    ts.maps = c->fm_desc.as<FieldMapDesc>();
+    ts.mapdata = c->fm_data.as<double>();
     ts.period = c->period;
     HIPCHK(c, hipMemcpyAsync(c->d_ts, &ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     Scalars s;
@@ -1109,12 +1224,13 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             c->grid.grad(c->d_sc);
         }
     }
+    const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
     if (binned && c->track_sc)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
-                           c->grid.Bsum, c->track_ext);
+                           c->grid.Bsum, ext);
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
-                           c->track_ext);
+                           ext);
     c->prof.mark(PROF_GATHER_PUSH);
     launch_track_post(c->stream, c->partials, gamma_blocks(c->grid.fp64, b.N), c->d_ts, c->d_sc, c->statsb.p,
                       &c->d_counters[1]);

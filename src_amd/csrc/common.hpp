@@ -58,7 +58,94 @@ struct Scalars {
     // wrapped into after boundp, [wrap_lo, wrap_hi) in the beam frame
     int pz;
     double period, wrap_lo, wrap_hi;
+    // field maps: the path length the step's lab frame is offset by, and the
+    // time of the bunch's field evaluation (t + dT/2)
+    double fm_s0, fm_t;
 };
+
+// ---- external field maps ------------------------------------------------------
+// A Solenoid with a 2D magnetostatic map (FM2DMagnetoStatic) or an RFCavity
+// with a 2D dynamic map (FM2DDynamic) on the beamline's z axis.  Grids are
+// nz*nr doubles indexed i + j*nz (z index i, r index j), as the reference
+// stores them (FM2DMagnetoStatic.cpp:110-130, FM2DDynamic.cpp:118-140):
+// magnetostatic a = Bz, b = Br; dynamic a = Ez, b = Er, c = Bt (SI units).
+constexpr int kMaxFieldMaps = 8;
+struct FieldMapDesc {
+    int kind;          // 0: solenoid (2D magnetostatic), 1: RF cavity (2D dynamic)
+    int nz, nr;        // grid points
+    double hz, hr;     // grid spacings (m)
+    double zstart;     // lab z of the map's first z point: element edge + zbegin
+    double length;     // zend - zbegin (m)
+    double scale;      // KS (solenoid) or VOLT (cavity)
+    double omega, phase;
+    int64_t a, b, c;   // offsets of the grids in the map data array
+};
+
+// Bilinear field of one map at (x, y, z) in its own frame (z from the map's
+// first point): FM2DMagnetoStatic::getFieldstrength (FM2DMagnetoStatic.cpp:161-190)
+// and FM2DDynamic::getFieldstrength (FM2DDynamic.cpp:169-205), accumulated into
+// e, b.  Returns true when the particle is beyond the map's radius.
+__host__ __device__ inline bool fieldmap_2d(const FieldMapDesc &m, const double *data, double x, double y, double z,
+                                            double e[3], double bf[3]) {
+    const double RR = sqrt(x * x + y * y);
+    int indexr, indexz;
+    if (m.kind == 0) {
+        indexr = abs((int)floor(RR / m.hr));
+        indexz = abs((int)floor(z / m.hz));
+    } else {
+        indexr = (int)floor(RR / m.hr);
+        indexz = (int)floor(z / m.hz);
+    }
+    const double leverr = RR / m.hr - indexr;
+    const double leverz = z / m.hz - indexz;
+    if ((indexz < 0) || (indexz + 2 > m.nz)) return false;
+    if (indexr + 2 > m.nr) return true;
+    const int index1 = indexz + indexr * m.nz;
+    const int index2 = index1 + m.nz;
+    auto lerp = [&](const double *f) {
+        return (1.0 - leverz) * (1.0 - leverr) * f[index1] + leverz * (1.0 - leverr) * f[index1 + 1] +
+               (1.0 - leverz) * leverr * f[index2] + leverz * leverr * f[index2 + 1];
+    };
+    if (m.kind == 0) {
+        const double BfieldR = lerp(data + m.b), BfieldZ = lerp(data + m.a);
+        if (RR != 0) {
+            bf[0] += BfieldR * x / RR;
+            bf[1] += BfieldR * y / RR;
+        }
+        bf[2] += BfieldZ;
+    } else {
+        const double EfieldR = lerp(data + m.b), EfieldZ = lerp(data + m.a), BfieldT = lerp(data + m.c);
+        if (RR > 1e-10) {
+            e[0] += EfieldR * x / RR;
+            e[1] += EfieldR * y / RR;
+            bf[0] -= BfieldT * y / RR;
+            bf[1] += BfieldT * x / RR;
+        }
+        e[2] += EfieldZ;
+    }
+    return false;
+}
+
+// Solenoid::apply / RFCavity::apply (Solenoid.cpp:167-179, RFCavity.cpp:232-250)
+// at lab position (x, y, zlab) and time t: inside [0, length) of the map the
+// scaled field (a cavity's E times cos(omega t + phase), its B times
+// -sin(omega t + phase)) is added to E, B.  Returns true for a particle beyond
+// the map's radius (the reference removes it; it gets no field here).
+__host__ __device__ inline bool fieldmap_apply(const FieldMapDesc &m, const double *data, double x, double y,
+                                               double zlab, double t, double E[3], double B[3]) {
+    const double z = zlab - m.zstart;
+    if (!(z >= 0.0 && z < m.length)) return false;
+    double te[3] = {0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0};
+    if (fieldmap_2d(m, data, x, y, z, te, tb)) return true;
+    if (m.kind == 0) {
+        for (int d = 0; d < 3; ++d) B[d] += m.scale * tb[d];
+    } else {
+        const double ce = m.scale * cos(m.omega * t + m.phase), sb = m.scale * sin(m.omega * t + m.phase);
+        for (int d = 0; d < 3; ++d) E[d] += ce * te[d];
+        for (int d = 0; d < 3; ++d) B[d] -= sb * tb[d];
+    }
+    return false;
+}
 
 // Tracker state carried across steps on the device.
 struct TrackState {
@@ -71,6 +158,11 @@ struct TrackState {
     int call_step0;  // step at the start of the current tracking call (statistics slot = step - call_step0 - 1)
     int pz;          // DC beam: periodic z (mesh rule, particle wrap, periodic guards and solve)
     double period;   // its period length (PartBunchBase::setBeamFrequency: c / f)
+    // field maps on the beamline (device arrays); the lab frame is the
+    // reference frame moved by (0, 0, pathLength)
+    int nmaps;
+    const FieldMapDesc *maps;
+    const double *mapdata;
 };
 
 // The particle boundary condition of a DC beam in z, applied by boundp's

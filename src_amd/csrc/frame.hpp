@@ -163,8 +163,11 @@ __host__ __device__ inline Trafo update_reference(TrackState &ts) {
     push_half(R[0], R[1], R[2], P[0], P[1], P[2], dt);
     double E[3], B[3];
     ext_fields(ts, ts.t - 0.5 * dt, E, B);
-    const double Ef[3] = {0.0 + E[0], 0.0 + E[1], 0.0 + E[2]};
-    const double Bf[3] = {0.0 + B[0], 0.0 + B[1], 0.0 + B[2]};
+    double Ef[3] = {0.0 + E[0], 0.0 + E[1], 0.0 + E[2]};
+    double Bf[3] = {0.0 + B[0], 0.0 + B[1], 0.0 + B[2]};
+    // field maps (applyToReferenceParticle) in the lab frame of the step
+    for (int k = 0; k < ts.nmaps; ++k)
+        if (fieldmap_apply(ts.maps[k], ts.mapdata, R[0], R[1], R[2] + ts.pathLength, ts.t - 0.5 * dt, Ef, Bf)) break;
     boris_kick(P, Ef, Bf, dt, ts.mass_eV, ts.charge);
     push_half(R[0], R[1], R[2], P[0], P[1], P[2], dt);
     (void)scale;

@@ -1,7 +1,9 @@
 // Host-side file formats around the space-charge step: FROMFILE particle
 // distributions (read) and the DataSink statistics file (write).
 #include "../../include/opalsc.h"
+#include "kernels.hpp"
 
+#include <cctype>
 #include <cmath>
 #include <ctime>
 #include <fstream>
@@ -10,10 +12,6 @@
 #include <regex>
 #include <sstream>
 #include <string>
-
-namespace opalsc {
-void set_global_error(const std::string &msg);
-}
 
 using opalsc::set_global_error;
 
@@ -247,3 +245,179 @@ extern "C" int opalsc_stat_write(const char *path, const char *input_fn, int new
     if (!os) return io_fail(std::string("write to '") + path + "' failed");
     return OPALSC_OK;
 }
+
+// ---- 2D field maps ------------------------------------------------------------
+namespace opalsc {
+
+namespace {
+
+// Fieldmap::getLine (src/Classic/Fields/Fieldmap.cpp:519-540): the next line
+// with an alphanumeric character once a '#' comment is cut, trimmed to its
+// first and last such character.
+const char *kAlnum = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ0123456789.-+\211";
+
+bool get_line(std::ifstream &in, std::string &buffer) {
+    size_t firstof = 0, lastof;
+    do {
+        if (!std::getline(in, buffer)) {
+            buffer.clear();
+            return false;
+        }
+        buffer = buffer.substr(0, buffer.find('#'));
+        lastof = buffer.find_last_of(kAlnum);
+        firstof = buffer.find_first_of(kAlnum);
+    } while (lastof == std::string::npos);
+    buffer = buffer.substr(firstof, lastof - firstof + 1);
+    return true;
+}
+
+// Fieldmap::interpreteLine: the line must hold exactly the values asked for.
+template <typename... T>
+bool interprete_line(std::ifstream &in, T &...values) {
+    std::string buffer;
+    if (!get_line(in, buffer)) return false;
+    std::istringstream interpreter(buffer);
+    bool ok = true;
+    ((ok = ok && static_cast<bool>(interpreter >> values)), ...);
+    if (!ok) return false;
+    std::string rest;
+    return !(interpreter >> rest);
+}
+
+// Fieldmap::interpreteEOF (Fieldmap.cpp:542-556): nothing but comments left.
+bool only_comments_left(std::ifstream &in) {
+    std::string line;
+    while (std::getline(in, line)) {
+        line = line.substr(0, line.find('#'));
+        if (line.find_first_of(kAlnum) != std::string::npos) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// The constructors and readMap of FM2DMagnetoStatic (FM2DMagnetoStatic.cpp:11-140)
+// and FM2DDynamic (FM2DDynamic.cpp:13-150): first line "<type> XZ|ZX" (the type
+// from its first four characters, Fieldmap.cpp:318-330); then z and r ranges in
+// cm with the number of spacings (r first for ZX), a dynamic map's frequency
+// (MHz) between them; then (nz+1)(nr+1) lines of 2 (magnetostatic: Bz Br, or
+// Br Bz for ZX) or 4 values (dynamic: Ez Er - Bt, or Er Ez Bt - for ZX),
+// z fastest for XZ, r fastest for ZX.  Fields are normalised to the largest
+// on-axis |Bz| / |Ez| (the default of Fieldmap::normalize_m); a dynamic map's E
+// is converted from MV/m, its H to B with mu_0.  A third word on the first
+// line makes the reference's parse fail (it overwrites the orientation), and
+// so it does here.
+int parse_fieldmap_2d(const char *path, FieldMapHost &out, std::string &err) {
+    std::ifstream file(path ? path : "");
+    if (!path || !file.good()) {
+        err = std::string("field map '") + (path ? path : "") + "' cannot be opened";
+        return -1;
+    }
+    std::string line;
+    if (!get_line(file, line)) {
+        err = std::string("field map '") + path + "' is empty";
+        return -1;
+    }
+    const std::string magic = line.substr(0, 4);
+    int kind;
+    if (magic == "2DMa") kind = 0;
+    else if (magic == "2DDy") kind = 1;
+    else {
+        err = std::string("field map '") + path + "' is not a 2D magnetostatic or 2D dynamic map";
+        return -1;
+    }
+    std::istringstream first(line);
+    std::string w1, w2, w3;
+    first >> w1 >> w2;
+    if (first >> w3) {
+        for (char &ch : w3) ch = (char)std::toupper((unsigned char)ch);
+        if (w3 != "TRUE" && w3 != "FALSE")
+            err = "The third string on the first line of 2D field maps has to be either TRUE or FALSE";
+        else
+            err = std::string("An error occured when reading the fieldmap '") + path + "'";
+        return -1;
+    }
+    double zbegin = 0, zend = 0, rbegin = 0, rend = 0, freq = 0;
+    int nz = 0, nr = 0;
+    bool swap, ok = true;
+    if (w2 == "ZX") {
+        swap = true;
+        ok = interprete_line(file, rbegin, rend, nr);
+        if (kind == 1) ok = ok && interprete_line(file, freq);
+        ok = ok && interprete_line(file, zbegin, zend, nz);
+    } else if (w2 == "XZ") {
+        swap = false;
+        ok = interprete_line(file, zbegin, zend, nz);
+        if (kind == 1) ok = ok && interprete_line(file, freq);
+        ok = ok && interprete_line(file, rbegin, rend, nr);
+    } else {
+        err = std::string("An error occured when reading the fieldmap '") + path + "'";
+        return -1;
+    }
+    if (!ok || nz < 1 || nr < 1) {
+        err = std::string("An error occured when reading the fieldmap '") + path + "'";
+        return -1;
+    }
+    const int gz = nz + 1, gr = nr + 1;
+    out.a.assign((size_t)gz * gr, 0.0);
+    out.b.assign((size_t)gz * gr, 0.0);
+    out.c.assign(kind == 1 ? (size_t)gz * gr : 0, 0.0);
+    for (int outer = 0; outer < (swap ? gz : gr) && ok; ++outer)
+        for (int inner = 0; inner < (swap ? gr : gz) && ok; ++inner) {
+            const int i = swap ? outer : inner, j = swap ? inner : outer;
+            const size_t k = (size_t)i + (size_t)j * gz;
+            double v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+            if (kind == 0) {
+                ok = interprete_line(file, v1, v2);
+                out.a[k] = swap ? v2 : v1;  // Bz
+                out.b[k] = swap ? v1 : v2;  // Br
+            } else {
+                ok = interprete_line(file, v1, v2, v3, v4);
+                out.a[k] = swap ? v2 : v1;  // Ez
+                out.b[k] = swap ? v1 : v2;  // Er
+                out.c[k] = swap ? v3 : v4;  // Bt (H)
+            }
+        }
+    if (!ok || !only_comments_left(file)) {
+        err = std::string("An error occured when reading the fieldmap '") + path + "'";
+        return -1;
+    }
+    double maxv = 0.0;
+    for (int i = 0; i < gz; ++i)
+        if (std::abs(out.a[i]) > maxv) maxv = std::abs(out.a[i]);
+    if (kind == 0) {
+        for (size_t k = 0; k < out.a.size(); ++k) {
+            out.a[k] /= maxv;
+            out.b[k] /= maxv;
+        }
+    } else {
+        const double mu_0 = 1.256637061e-06;  // src/Classic/Physics/Physics.h:55
+        for (size_t k = 0; k < out.a.size(); ++k) {
+            out.a[k] *= 1.e6 / maxv;
+            out.b[k] *= 1.e6 / maxv;
+            out.c[k] *= mu_0 / maxv;
+        }
+    }
+    FieldMapDesc &d = out.d;
+    d.kind = kind;
+    rbegin /= 100.;
+    rend /= 100.;
+    zbegin /= 100.;
+    zend /= 100.;
+    d.hr = (rend - rbegin) / nr;
+    d.hz = (zend - zbegin) / nz;
+    d.nz = gz;
+    d.nr = gr;
+    d.zstart = zbegin;
+    d.length = zend - zbegin;
+    d.scale = 1.0;
+    d.omega = 0.0;
+    d.phase = 0.0;
+    d.a = 0;
+    d.b = (int64_t)out.a.size();
+    d.c = kind == 1 ? 2 * (int64_t)out.a.size() : 0;
+    out.frequency = freq * (2.0 * 3.14159265358979323846) * 1e6;
+    return 0;
+}
+
+}  // namespace opalsc

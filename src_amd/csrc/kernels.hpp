@@ -10,6 +10,15 @@ namespace opalsc {
 
 void set_global_error(const std::string &msg);
 
+// A parsed 2D field map (host), FieldMapDesc offsets relative to its own grids.
+struct FieldMapHost {
+    FieldMapDesc d;
+    double frequency;  // the map file's frequency (rad/s; dynamic maps)
+    std::vector<double> a, b, c;
+};
+// FM2DMagnetoStatic / FM2DDynamic file reading (io.cpp); 0 or -1 with err set.
+int parse_fieldmap_2d(const char *path, FieldMapHost &out, std::string &err);
+
 
 enum ProfId {
     PROF_PUSH_BOUNDS = 0,  // push + beam-frame rotation + bounds
@@ -145,6 +154,9 @@ void launch_boundp_scalars(hipStream_t s, const double *partials, int nb, const 
                            bool pz, double period, Scalars *sc);
 // ParticlePeriodicBCond in z over [origin, n*h + origin) (AoS R, in place)
 void launch_wrap_periodic_z(hipStream_t s, int64_t N, double *R, double lo, double hi);
+// every field map at lab positions R (AoS) and time t into E, B (overwritten)
+void launch_fieldmap_eval(hipStream_t s, int64_t N, const double *R, double t, const FieldMapDesc *maps, int nmaps,
+                          const double *data, double *E, double *B, unsigned long long *lost);
 void launch_push(hipStream_t s, int64_t N, double *R, const double *P, const double *dt);
 void launch_kick(hipStream_t s, int64_t N, double *P, const double *Ef, const double *Bf,
                  const double *dt, double charge, double mass);
@@ -181,7 +193,7 @@ int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamm
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials, const void *Bg, bool ext);
+                        const TrackState *ts, double *partials, const void *Bg, int ext);
 // stats_out: the call's statistics array, written at slot ts->step - ts->call_step0 - 1
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped);

@@ -658,6 +658,25 @@ void launch_wrap_periodic_z(hipStream_t s, int64_t N, double *R, double lo, doub
     if (N > 0) k_wrap_periodic_z<<<stream_blocks(N), kThreads, 0, s>>>(N, R, lo, hi);
 }
 
+__global__ void k_fieldmap_eval(int64_t N, const double *R, double t, const FieldMapDesc *maps, int nmaps,
+                                const double *data, double *E, double *B, unsigned long long *lost) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double e[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+        bool out = false;  // a particle beyond one map's radius sees no later map (computeExternalFields)
+        for (int k = 0; k < nmaps && !out; ++k)
+            out = fieldmap_apply(maps[k], data, R[3 * i], R[3 * i + 1], R[3 * i + 2], t, e, b);
+        for (int d = 0; d < 3; ++d) {
+            E[3 * i + d] = e[d];
+            B[3 * i + d] = b[d];
+        }
+        if (out) atomicAdd(lost, 1ull);
+    }
+}
+void launch_fieldmap_eval(hipStream_t s, int64_t N, const double *R, double t, const FieldMapDesc *maps, int nmaps,
+                          const double *data, double *E, double *B, unsigned long long *lost) {
+    if (N > 0) k_fieldmap_eval<<<stream_blocks(N), kThreads, 0, s>>>(N, R, t, maps, nmaps, data, E, B, lost);
+}
+
 __global__ void k_push(int64_t N, double *R, const double *P, const double *dt) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         double x = R[3 * i], y = R[3 * i + 1], z = R[3 * i + 2];
@@ -1037,6 +1056,8 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
         for (int d = 0; d < 3; ++d) sc->o_br[d] = sc->o_rb[d] = 0.0;
     }
     ext_fields(s, s.t + 0.5 * s.dT, sc->Eext, sc->Bext);
+    sc->fm_s0 = s.pathLength;  // the lab frame of the field maps during this step
+    sc->fm_t = s.t + 0.5 * s.dT;
     s.t += s.dT;
     Trafo upd;
     if (s.t > 0.0) {
@@ -1214,7 +1235,7 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
 // magnetic field each bin and its image imply (ParallelTTracker.cpp:567-579),
 // both gathered.  Four waves per SIMD (at most 128 VGPRs) keep enough loads in
 // flight.
-template <bool NEXT, bool BINNED>
+template <bool NEXT, bool BINNED, bool MAPS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma(
     int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz, const double4 *Eg,
     const double4 *Bg, const Scalars *sc, const TrackState *ts, double *partials) {
@@ -1289,6 +1310,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             Ef[k] += Ee[k];
             Bf[k] += Be[k];
         }
+        if constexpr (MAPS) {
+            // computeExternalFields (ParallelTTracker.cpp:595-640) at the
+            // position the space-charge rotation left, in the step's lab frame
+            const double zlab = r[2] + sc->fm_s0, tf = sc->fm_t;
+            for (int k = 0; k < ts->nmaps; ++k)
+                if (fieldmap_apply(ts->maps[k], ts->mapdata, r[0], r[1], zlab, tf, Ef, Bf)) break;
+        }
         double pn[3];
         boris_kick(p, Ef, Bf, dT, mass, charge);
         push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
@@ -1313,7 +1341,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // position through the reference frame.  Positions, momenta and moments stay
 // in FP64.  EXT: a nonzero external field (without one the loop holds 27 frame
 // constants, which fit the scalar registers).
-template <bool NEXT, bool BINNED, bool EXT>
+template <bool NEXT, bool BINNED, int EXT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_track_gamma_mixed(
     int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz, const float4 *Eg,
     const float4 *Bg, const Scalars *sc, const TrackState *ts, double *partials) {
@@ -1364,6 +1392,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (; i < xend; i += xstride) {
         double r[3] = {nr[0], nr[1], nr[2]};
         double p[3] = {nr[3], nr[4], nr[5]};
+        double em[3] = {0.0, 0.0, 0.0}, bm[3] = {0.0, 0.0, 0.0};
+        if constexpr (EXT == 2) {
+            // field maps at the reference-frame position in the step's lab frame
+            const double zlab = r[2] + sc->fm_s0, tf = sc->fm_t;
+            for (int k = 0; k < ts->nmaps; ++k)
+                if (fieldmap_apply(ts->maps[k], ts->mapdata, r[0], r[1], zlab, tf, em, bm)) break;
+        }
         const int64_t j = i + xstride;
         if (j < xend) {
             nr[0] = x[j];
@@ -1393,11 +1428,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 Bf[1] = betaC * Ef[0];
             }
         }
-        if constexpr (EXT)
+        if constexpr (EXT != 0)
             for (int k = 0; k < 3; ++k) {
                 Ef[k] += Ee[k];
                 Bf[k] += Be[k];
             }
+        if constexpr (EXT == 2) {
+            double t3[3];
+            mat_vec_fast(Mr, em, t3);
+            for (int k = 0; k < 3; ++k) Ef[k] += t3[k];
+            mat_vec_fast(Mr, bm, t3);
+            for (int k = 0; k < 3; ++k) Bf[k] += t3[k];
+        }
         boris_kick_fast(pb, Ef, Bf, e_kick, b_kick);
         push_half_fast(r[0], r[1], r[2], pb[0], pb[1], pb[2], half_cdt);
         transform_to_fast(Mc, oc, r);
@@ -1434,44 +1476,44 @@ static int resident_blocks(K kernel) {
 
 // One wave of resident blocks (fewer partial sums for the step statistics).
 int gamma_blocks(bool fp64, int64_t N) {
-    const int r = fp64 ? resident_blocks(k_track_gamma<true, false>)
-                       : resident_blocks(k_track_gamma_mixed<true, false, true>);
+    const int r = fp64 ? resident_blocks(k_track_gamma<true, false, false>)
+                       : resident_blocks(k_track_gamma_mixed<true, false, 1>);
     const int b = red_blocks(N);
     return b < r ? b : r;
 }
 
+// ext: 0 no external field, 1 the analytic fields, 2 those and field maps
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials, const void *Bg, bool ext) {
+                        const TrackState *ts, double *partials, const void *Bg, int ext) {
     if (b.N <= 0) return;
     const int blocks = gamma_blocks(fp64, b.N);
 #define OPALSC_GAMMA(K_, V_, ...)                                                                       \
     K_<__VA_ARGS__><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const V_ *)Eg, \
                                                 (const V_ *)Bg, sc, ts, partials)
+#define OPALSC_GAMMA_NB(K_, V_, N_, ...)                                                                \
+    do {                                                                                                 \
+        if (Bg) OPALSC_GAMMA(K_, V_, N_, true, __VA_ARGS__);                                           \
+        else OPALSC_GAMMA(K_, V_, N_, false, __VA_ARGS__);                                             \
+    } while (0)
     if (fp64) {
-        if (next) {
-            if (Bg) OPALSC_GAMMA(k_track_gamma, double4, true, true);
-            else OPALSC_GAMMA(k_track_gamma, double4, true, false);
+        if (ext == 2) {
+            if (next) OPALSC_GAMMA_NB(k_track_gamma, double4, true, true);
+            else OPALSC_GAMMA_NB(k_track_gamma, double4, false, true);
         } else {
-            if (Bg) OPALSC_GAMMA(k_track_gamma, double4, false, true);
-            else OPALSC_GAMMA(k_track_gamma, double4, false, false);
+            if (next) OPALSC_GAMMA_NB(k_track_gamma, double4, true, false);
+            else OPALSC_GAMMA_NB(k_track_gamma, double4, false, false);
         }
-    } else if (ext) {
-        if (next) {
-            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, true, true, true);
-            else OPALSC_GAMMA(k_track_gamma_mixed, float4, true, false, true);
-        } else {
-            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, false, true, true);
-            else OPALSC_GAMMA(k_track_gamma_mixed, float4, false, false, true);
-        }
+    } else if (ext == 2) {
+        if (next) OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, true, 2);
+        else OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, false, 2);
+    } else if (ext == 1) {
+        if (next) OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, true, 1);
+        else OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, false, 1);
     } else {
-        if (next) {
-            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, true, true, false);
-            else OPALSC_GAMMA(k_track_gamma_mixed, float4, true, false, false);
-        } else {
-            if (Bg) OPALSC_GAMMA(k_track_gamma_mixed, float4, false, true, false);
-            else OPALSC_GAMMA(k_track_gamma_mixed, float4, false, false, false);
-        }
+        if (next) OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, true, 0);
+        else OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, false, 0);
     }
+#undef OPALSC_GAMMA_NB
 #undef OPALSC_GAMMA
 }
 

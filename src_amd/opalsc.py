@@ -44,7 +44,7 @@ EXPORTS = [
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
     "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
     "opalsc_copy_bandwidth", "opalsc_fromfile_count", "opalsc_fromfile_read", "opalsc_stat_row_get",
-    "opalsc_stat_write",
+    "opalsc_stat_write", "opalsc_fieldmap_add", "opalsc_fieldmap_clear", "opalsc_fieldmap_eval",
 ]
 
 
@@ -147,6 +147,9 @@ def lib():
         "opalsc_fromfile_count": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
         "opalsc_fromfile_read": ([C.c_char_p, i64, C.c_int, C.c_double, dp, dp], C.c_int),
         "opalsc_stat_row_get": ([vp, C.POINTER(StatRow)], C.c_int),
+        "opalsc_fieldmap_add": ([vp, C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+        "opalsc_fieldmap_clear": ([vp], C.c_int),
+        "opalsc_fieldmap_eval": ([vp, i64, dp, C.c_double, dp, dp, C.POINTER(C.c_int64)], C.c_int),
         "opalsc_stat_write": ([C.c_char_p, C.c_char_p, C.c_int, C.POINTER(StatRow)], C.c_int),
         "opalsc_host_log_cr": ([i64, dp, dp], C.c_int),
         "opalsc_device_atan_cr": ([vp, i64, dp, dp], C.c_int),
@@ -428,6 +431,25 @@ class SpaceChargeContext:
         self._c(lib().opalsc_track(self._h, int(nsteps),
                                    C.cast(stats, C.c_void_p) if want_stats else None))
         return [stats_to_dict(stats[i]) for i in range(nsteps)] if want_stats else None
+
+    # a Solenoid (2D magnetostatic map, scale = KS) or RFCavity (2D dynamic map,
+    # scale = VOLT, freq_hz = FREQ, phase = LAG) whose map starts at elemedge
+    def add_fieldmap(self, path, elemedge=0.0, scale=1.0, freq_hz=0.0, phase=0.0):
+        rc = lib().opalsc_fieldmap_add(self._h, str(path).encode(), float(elemedge), float(scale),
+                                       float(freq_hz), float(phase))
+        if rc < 0:
+            self._c(rc)
+        return rc
+
+    def clear_fieldmaps(self):
+        self._c(lib().opalsc_fieldmap_clear(self._h))
+
+    def fieldmap_eval(self, R, t):
+        R = _d(R, (-1, 3))
+        E, B = np.zeros_like(R), np.zeros_like(R)
+        lost = C.c_int64()
+        self._c(lib().opalsc_fieldmap_eval(self._h, len(R), _ptr(R), float(t), _ptr(E), _ptr(B), C.byref(lost)))
+        return E, B, lost.value
 
     def stat_row(self):
         """The DataSink statistics line of the resident bunch (after track)."""

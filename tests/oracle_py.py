@@ -38,7 +38,14 @@ class Tracker(C.Structure):
                 ("mass_eV", C.c_double), ("coupling", C.c_double), ("Bz", C.c_double),
                 ("Ez0", C.c_double), ("rf_freq", C.c_double), ("rf_phase", C.c_double),
                 ("t", C.c_double), ("pathLength", C.c_double), ("RefPartR", C.c_double * 3),
-                ("RefPartP", C.c_double * 3), ("bcz", C.c_int), ("period", C.c_double)]
+                ("RefPartP", C.c_double * 3), ("bcz", C.c_int), ("period", C.c_double),
+                ("nmaps", C.c_int), ("maps", C.c_void_p)]
+
+
+class FieldMap(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nz", C.c_int), ("nr", C.c_int), ("hz", C.c_double), ("hr", C.c_double),
+                ("zstart", C.c_double), ("length", C.c_double), ("scale", C.c_double), ("omega", C.c_double),
+                ("phase", C.c_double), ("a", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p)]
 
 
 _lib = None
@@ -100,6 +107,9 @@ def lib():
     L.orc_track_binned.argtypes = [C.POINTER(Tracker), i64, PD, PD, PD, PD, PI32, C.c_int, C.c_double,
                                    C.c_int, C.c_int, C.POINTER(Stats)]
     L.orc_track_binned.restype = C.c_int
+    L.orc_fieldmap_eval.argtypes = [C.c_int, C.c_void_p, i64, PD, C.c_double, PD, PD]
+    L.orc_fieldmap_eval.restype = i64
+    assert L.orc_sizeof_fieldmap() == C.sizeof(FieldMap)
     assert L.orc_sizeof_stats() == C.sizeof(Stats)
     assert L.orc_sizeof_tracker() == C.sizeof(Tracker)
     _lib = L
@@ -320,7 +330,7 @@ def quaternion_to_z(u):
 
 def make_tracker(n, dT, charge=1.0, mass=M_P_EV, dh=0.02, greens=0, space_charge=True, Bz=0.0,
                  Ez0=0.0, rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), coupling=COUPLING,
-                 periodic_z=False, period=CLIGHT / 1e9):
+                 periodic_z=False, period=CLIGHT / 1e9, maps=None):
     t = Tracker()
     t.n[:] = list(n)
     t.dh, t.greens, t.space_charge, t.dT = dh, greens, int(space_charge), dT
@@ -330,6 +340,10 @@ def make_tracker(n, dT, charge=1.0, mass=M_P_EV, dh=0.02, greens=0, space_charge
     t.RefPartR[:] = [0.0, 0.0, 0.0]
     t.RefPartP[:] = list(ref_p)
     t.bcz, t.period = int(periodic_z), float(period)
+    if maps:
+        arr = make_fieldmaps(maps)
+        t.nmaps, t.maps = len(maps), C.cast(arr, C.c_void_p)
+        t._maps = arr
     return t
 
 
@@ -351,3 +365,110 @@ def track_binned(tracker, R, P, Q, dt, bins, nbins, qi, nsteps, image=True):
     if rc != 0:
         raise RuntimeError("orc_track_binned failed: %d" % rc)
     return R, P, Q, dt, [stats_dict(stats[i]) for i in range(nsteps)]
+
+
+# --- 2D field maps -----------------------------------------------------------------
+_ALNUM = set("abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ0123456789.-+\x89")
+
+
+def _content_lines(path):
+    """Fieldmap::getLine (src/Classic/Fields/Fieldmap.cpp:519-540): lines with an
+    alphanumeric character once a '#' comment is cut, trimmed to the first and
+    last such character."""
+    out = []
+    for line in open(path, encoding="latin-1").read().split("\n"):
+        line = line.split("#", 1)[0]
+        idx = [k for k, ch in enumerate(line) if ch in _ALNUM]
+        if idx:
+            out.append(line[idx[0]:idx[-1] + 1])
+    return out
+
+
+def parse_fieldmap_2d(path):
+    """FM2DMagnetoStatic / FM2DDynamic file reading, restated: returns a dict
+    with kind (0 magnetostatic, 1 dynamic), nz, nr (points), hz, hr (m), zbegin,
+    length (m), frequency (rad/s) and the grids a, b[, c] (index i + j*nz),
+    normalised as the reference does.  Raises ValueError where it throws."""
+    lines = _content_lines(path)
+    head = lines[0].split()
+    kind = {"2DMa": 0, "2DDy": 1}.get(lines[0][:4])
+    if kind is None or len(head) != 2 or head[1] not in ("XZ", "ZX"):
+        raise ValueError("An error occured when reading the fieldmap")
+    swap = head[1] == "ZX"
+    k = 1
+    def ints(line):
+        a, b, n = line.split()
+        return float(a), float(b), int(n)
+    if swap:
+        rb, re_, nr = ints(lines[k]); k += 1
+        freq = 0.0
+        if kind == 1:
+            freq = float(lines[k]); k += 1
+        zb, ze, nz = ints(lines[k]); k += 1
+    else:
+        zb, ze, nz = ints(lines[k]); k += 1
+        freq = 0.0
+        if kind == 1:
+            freq = float(lines[k]); k += 1
+        rb, re_, nr = ints(lines[k]); k += 1
+    gz, gr = nz + 1, nr + 1
+    nv = 2 if kind == 0 else 4
+    rows = [list(map(float, l.split())) for l in lines[k:]]
+    if len(rows) != gz * gr or any(len(r) != nv for r in rows):
+        raise ValueError("An error occured when reading the fieldmap")
+    a = np.zeros(gz * gr); b = np.zeros(gz * gr); c = np.zeros(gz * gr)
+    it = iter(rows)
+    outer, inner = (gz, gr) if swap else (gr, gz)
+    for o in range(outer):
+        for n in range(inner):
+            i, j = (o, n) if swap else (n, o)
+            v = next(it)
+            q = i + j * gz
+            if kind == 0:
+                a[q], b[q] = (v[1], v[0]) if swap else (v[0], v[1])
+            else:
+                if swap:
+                    b[q], a[q], c[q] = v[0], v[1], v[2]
+                else:
+                    a[q], b[q], c[q] = v[0], v[1], v[3]
+    vmax = np.max(np.abs(a[:gz]))
+    if kind == 0:
+        a = a / vmax
+        b = b / vmax
+    else:
+        mu_0 = 1.256637061e-06
+        a = a * (1.e6 / vmax)
+        b = b * (1.e6 / vmax)
+        c = c * (mu_0 / vmax)
+    rb, re_, zb, ze = rb / 100., re_ / 100., zb / 100., ze / 100.
+    return dict(kind=kind, nz=gz, nr=gr, hr=(re_ - rb) / nr, hz=(ze - zb) / nz, zbegin=zb, length=ze - zb,
+                frequency=freq * (2.0 * PI) * 1e6, a=a, b=b, c=c)
+
+
+def make_fieldmaps(specs):
+    """specs: list of (parsed map dict, elemedge, scale, freq_hz, phase).  Returns
+    the ctypes array (keep it and the grids alive while it is used)."""
+    arr = (FieldMap * max(len(specs), 1))()
+    keep = []
+    for k, (m, edge, scale, freq_hz, phase) in enumerate(specs):
+        f = arr[k]
+        f.kind, f.nz, f.nr, f.hz, f.hr = m["kind"], m["nz"], m["nr"], m["hz"], m["hr"]
+        f.zstart, f.length, f.scale = edge + m["zbegin"], m["length"], scale
+        omega = 0.0
+        if m["kind"] == 1:
+            omega = freq_hz * (2.0 * PI)
+            if not omega > 0.0 or abs((omega - m["frequency"]) / omega) > 0.01:
+                omega = m["frequency"]
+        f.omega, f.phase = omega, (phase if m["kind"] == 1 else 0.0)
+        grids = [np.ascontiguousarray(m[g], dtype=np.float64) for g in ("a", "b", "c")]
+        keep += grids
+        f.a, f.b, f.c = (g.ctypes.data for g in grids)
+    arr._keep = keep
+    return arr
+
+
+def fieldmap_eval(maps, n, R, t):
+    R = _d(R)
+    E = np.zeros((len(R), 3)); B = np.zeros((len(R), 3))
+    lost = lib().orc_fieldmap_eval(n, C.cast(maps, C.c_void_p), len(R), R, float(t), E, B)
+    return E, B, lost
