@@ -1227,10 +1227,10 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
     if (binned && c->track_sc)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
-                           c->grid.Bsum, ext);
+                           c->grid.Bsum, ext, false);
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
-                           ext);
+                           ext, c->grid.pz);
     c->prof.mark(PROF_GATHER_PUSH);
     launch_track_post(c->stream, c->partials, gamma_blocks(c->grid.fp64, b.N), c->d_ts, c->d_sc, c->statsb.p,
                       &c->d_counters[1]);

@@ -36,8 +36,11 @@ __device__ inline void gather_E(const typename vec4_of<T>::type *Eg, const int l
 }
 
 // Mixed-precision gather: the same stencil with FP32 weights and FMAs.
+// PZ (a compile-time choice: the periodic corner lookup in the edge path
+// measurably slows the common case): z faces periodic.
+template <bool PZ = false>
 __device__ inline void gather_E_fast(const float4 *Eg, const int l[3], const double d[3], int n0, int n1,
-                                     int n2, double Ef[3], bool pz = false) {
+                                     int n2, double Ef[3]) {
     const float b0 = (float)d[0], b1 = (float)d[1], b2 = (float)d[2];
     const float a0 = 1.0f - b0, a1 = 1.0f - b1, a2 = 1.0f - b2;
     const float w[8] = {a0 * a1 * a2, b0 * a1 * a2, a0 * b1 * a2, b0 * b1 * a2,
@@ -58,8 +61,9 @@ __device__ inline void gather_E_fast(const float4 *Eg, const int l[3], const dou
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int i = l[0] + (k & 1), j = l[1] + ((k >> 1) & 1), kk = corner_z(l[2] + ((k >> 2) & 1), n2, pz);
-            if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1) {
+            const int i = l[0] + (k & 1), j = l[1] + ((k >> 1) & 1);
+            const int kk = PZ ? corner_z(l[2] + ((k >> 2) & 1), n2, true) : l[2] + ((k >> 2) & 1);
+            if (i >= 0 && j >= 0 && kk >= 0 && i < n0 && j < n1 && kk < n2) {
                 const float4 c = Eg[((size_t)kk * n1 + j) * n0 + i];
                 e0 = fmaf(w[k], c.x, e0);
                 e1 = fmaf(w[k], c.y, e1);

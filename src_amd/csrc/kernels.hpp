@@ -193,7 +193,7 @@ int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamm
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials, const void *Bg, int ext);
+                        const TrackState *ts, double *partials, const void *Bg, int ext, bool dc);
 // stats_out: the call's statistics array, written at slot ts->step - ts->call_step0 - 1
 void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState *ts, Scalars *sc,
                        void *stats_out, const unsigned long long *dropped);

@@ -1339,9 +1339,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // rotations) and one composed map M_c to the updated reference frame, so the
 // loop needs 33 frame constants instead of 42 and no round trip of the
 // position through the reference frame.  Positions, momenta and moments stay
-// in FP64.  EXT: a nonzero external field (without one the loop holds 27 frame
-// constants, which fit the scalar registers).
-template <bool NEXT, bool BINNED, int EXT>
+// in FP64.  EXT: 0 no external field (the loop's 27 frame constants then fit
+// the scalar registers), 1 the analytic fields, 2 those and field maps.  DC: a
+// DC beam (periodic z), a separate instance so the common case carries none of it.
+template <bool NEXT, bool BINNED, int EXT, bool DC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_track_gamma_mixed(
     int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz, const float4 *Eg,
     const float4 *Bg, const Scalars *sc, const TrackState *ts, double *partials) {
@@ -1350,8 +1351,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double betaC = sc->betaC;
-    const bool dcz = sc->pz != 0;
-    const double zper = n2 * sc->hr[2];  // period of the mesh-relative z
+    const double zper = DC ? n2 * sc->hr[2] : 0.0;  // DC beam: period of the mesh-relative z
     double Mr[9], Mc[9], orb[3], oc[3], ih[3], Ee[3], Be[3];
     for (int k = 0; k < 9; ++k) {
         Mr[k] = sc->M_rb[k];
@@ -1409,7 +1409,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             nr[5] = pz[j];
         }
         transform_to_fast(Mr, orb, r);  // beam frame, relative to the mesh origin
-        if (dcz && sc_on) r[2] = wrap_periodic(r[2], 0.0, zper);  // boundp's update() of a DC beam
+        if constexpr (DC)
+            if (sc_on) r[2] = wrap_periodic(r[2], 0.0, zper);  // boundp's update() of a DC beam
         double pb[3];
         mat_vec_fast(Mr, p, pb);
         double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
@@ -1420,7 +1421,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             ok &= cic_dim_fast(r[1], 0.0, ih[1], l[1], d[1]);
             ok &= cic_dim_fast(r[2], 0.0, ih[2], l[2], d[2]);
             if (ok) {
-                gather_E_fast(Eg, l, d, n0, n1, n2, Ef, dcz);
+                gather_E_fast<DC>(Eg, l, d, n0, n1, n2, Ef);
                 if constexpr (BINNED) gather_E_fast(Bg, l, d, n0, n1, n2, Bf);
             }
             if constexpr (!BINNED) {
@@ -1477,44 +1478,62 @@ static int resident_blocks(K kernel) {
 // One wave of resident blocks (fewer partial sums for the step statistics).
 int gamma_blocks(bool fp64, int64_t N) {
     const int r = fp64 ? resident_blocks(k_track_gamma<true, false, false>)
-                       : resident_blocks(k_track_gamma_mixed<true, false, 1>);
+                       : resident_blocks(k_track_gamma_mixed<true, false, 1, false>);
     const int b = red_blocks(N);
     return b < r ? b : r;
 }
 
-// ext: 0 no external field, 1 the analytic fields, 2 those and field maps
+namespace {
+struct GammaArgs {
+    unsigned blocks;
+    hipStream_t s;
+    Bunch *b;
+    const void *Eg, *Bg;
+    const Scalars *sc;
+    const TrackState *ts;
+    double *partials;
+};
+template <bool NEXT, bool BINNED, int EXT, bool DC>
+void launch_mixed(const GammaArgs &a) {
+    k_track_gamma_mixed<NEXT, BINNED, EXT, DC><<<a.blocks, kThreads, 0, a.s>>>(
+        a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const float4 *)a.Eg, (const float4 *)a.Bg, a.sc,
+        a.ts, a.partials);
+}
+template <bool NEXT, int EXT>
+void launch_mixed_nb(const GammaArgs &a, bool dc) {
+    if (a.Bg) launch_mixed<NEXT, true, EXT, false>(a);  // energy bins never come with a DC beam
+    else if (dc) launch_mixed<NEXT, false, EXT, true>(a);
+    else launch_mixed<NEXT, false, EXT, false>(a);
+}
+template <bool NEXT, bool MAPS>
+void launch_fp64(const GammaArgs &a) {
+    if (a.Bg)
+        k_track_gamma<NEXT, true, MAPS><<<a.blocks, kThreads, 0, a.s>>>(
+            a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const double4 *)a.Eg, (const double4 *)a.Bg,
+            a.sc, a.ts, a.partials);
+    else
+        k_track_gamma<NEXT, false, MAPS><<<a.blocks, kThreads, 0, a.s>>>(
+            a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const double4 *)a.Eg, nullptr, a.sc, a.ts,
+            a.partials);
+}
+}  // namespace
+
+// ext: 0 no external field, 1 the analytic fields, 2 those and field maps;
+// dc: a DC beam (periodic z)
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                        const TrackState *ts, double *partials, const void *Bg, int ext) {
+                        const TrackState *ts, double *partials, const void *Bg, int ext, bool dc) {
     if (b.N <= 0) return;
-    const int blocks = gamma_blocks(fp64, b.N);
-#define OPALSC_GAMMA(K_, V_, ...)                                                                       \
-    K_<__VA_ARGS__><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const V_ *)Eg, \
-                                                (const V_ *)Bg, sc, ts, partials)
-#define OPALSC_GAMMA_NB(K_, V_, N_, ...)                                                                \
-    do {                                                                                                 \
-        if (Bg) OPALSC_GAMMA(K_, V_, N_, true, __VA_ARGS__);                                           \
-        else OPALSC_GAMMA(K_, V_, N_, false, __VA_ARGS__);                                             \
-    } while (0)
+    const GammaArgs a{(unsigned)gamma_blocks(fp64, b.N), s, &b, Eg, Bg, sc, ts, partials};
     if (fp64) {
-        if (ext == 2) {
-            if (next) OPALSC_GAMMA_NB(k_track_gamma, double4, true, true);
-            else OPALSC_GAMMA_NB(k_track_gamma, double4, false, true);
-        } else {
-            if (next) OPALSC_GAMMA_NB(k_track_gamma, double4, true, false);
-            else OPALSC_GAMMA_NB(k_track_gamma, double4, false, false);
-        }
+        if (ext == 2) next ? launch_fp64<true, true>(a) : launch_fp64<false, true>(a);
+        else next ? launch_fp64<true, false>(a) : launch_fp64<false, false>(a);
     } else if (ext == 2) {
-        if (next) OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, true, 2);
-        else OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, false, 2);
+        next ? launch_mixed_nb<true, 2>(a, dc) : launch_mixed_nb<false, 2>(a, dc);
     } else if (ext == 1) {
-        if (next) OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, true, 1);
-        else OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, false, 1);
+        next ? launch_mixed_nb<true, 1>(a, dc) : launch_mixed_nb<false, 1>(a, dc);
     } else {
-        if (next) OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, true, 0);
-        else OPALSC_GAMMA_NB(k_track_gamma_mixed, float4, false, 0);
+        next ? launch_mixed_nb<true, 0>(a, dc) : launch_mixed_nb<false, 0>(a, dc);
     }
-#undef OPALSC_GAMMA_NB
-#undef OPALSC_GAMMA
 }
 
 // Step statistics (DataSink / calcBeamParameters) and the momentum sums the
