@@ -181,10 +181,14 @@ __device__ inline int small_div(int a, int b, float rb) {
     else if ((qt + 1) * b <= a) ++qt;
     return qt;
 }
-// 38 KiB of int64: four blocks per CU fit the 160 KiB of LDS, so the deposit
-// runs at four waves per SIMD (measured 0.98 -> 0.91 ms at 1e8 particles
-// against a 44 KiB box at three; a 31 KiB box at five waves spills and slows)
-constexpr int kBoxCap = 4864;
+// LDS box capacities (int64 cells): 38 KiB lets four blocks share a CU's
+// 160 KiB of LDS, so a large bunch's deposit runs at four waves per SIMD
+// (0.98 -> 0.91 ms at 1e8 particles against 44 KiB at three; 31 KiB at five
+// waves spills and slows); a bunch of a few million particles or fewer fills
+// the device with fewer blocks than that, and there the larger box (fewer
+// chunks falling back to global atomics) wins (64 small bunches: 6.2 -> 5.9 ms).
+constexpr int kBoxCapSmall = 4864, kBoxCapLarge = 5632;
+constexpr int64_t kLargeBoxMaxParticles = int64_t(8) << 20;
 
 // MODE 0: qd given; 1 (tracker): qd = Q*dT with Q left as (Q*dT)/dT (FP64 path), positions
 // given in the reference frame and rotated into the beam frame (M_rb, o_rb)
@@ -197,7 +201,7 @@ constexpr int kBoxCap = 4864;
 // integers, summed in 64 bits).  BOX: chunk statistics supplied by the bounds
 // pass.  Only particles lo <= p < N deposit; block b owns chunk chunk0 + b of
 // the Nall-particle bunch.
-template <int MODE, bool EXACT, bool BOX>
+template <int MODE, bool EXACT, bool BOX, int kBoxCap>
 __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int64_t Nall, int64_t chunk0,
                                                      const double *__restrict__ x,
                                                      const double *__restrict__ y,
@@ -512,9 +516,16 @@ void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64
     const int64_t chunk0 = lo / kChunk;
     const unsigned blocks = (unsigned)((hi - 1) / kChunk - chunk0 + 1);
     const bool box = chunkbox != nullptr;
-#define OPALSC_DEP(M_, E_, B_)                                                                               \
-    k_deposit_lds<M_, E_, B_><<<blocks, kThreads, 0, s>>>(lo, hi, Nall, chunk0, x, y, z, q, qd, rho, sc, \
-                                                          dropped, chunkbox)
+    const bool large = hi - lo <= kLargeBoxMaxParticles;
+#define OPALSC_DEP(M_, E_, B_)                                                                              \
+    do {                                                                                                    \
+        if (large)                                                                                          \
+            k_deposit_lds<M_, E_, B_, kBoxCapLarge><<<blocks, kThreads, 0, s>>>(                            \
+                lo, hi, Nall, chunk0, x, y, z, q, qd, rho, sc, dropped, chunkbox);                          \
+        else                                                                                                \
+            k_deposit_lds<M_, E_, B_, kBoxCapSmall><<<blocks, kThreads, 0, s>>>(                            \
+                lo, hi, Nall, chunk0, x, y, z, q, qd, rho, sc, dropped, chunkbox);                          \
+    } while (0)
     if (mode == 0) {
         if (exact) OPALSC_DEP(0, true, false);
         else OPALSC_DEP(0, false, false);
