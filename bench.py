@@ -208,7 +208,7 @@ def algorithmic_bytes(cfg, wbytes):
 
 # profiler kernel class -> kernel symbol of the mixed-precision step
 KERNEL_SYMBOL = {
-    "gather_kick_push": "k_track_gamma_mixed<true,false,0,false>",
+    "gather_kick_push": "k_track_gamma_mixed_os<true,0>",
     "deposit": "k_deposit_lds<3,false,true,4864>",
     "rotate_bounds": "k_track_alpha<false,false>",
     "push_rotate_bounds": "k_track_alpha<false,true>",

@@ -65,6 +65,7 @@ struct opalsc_ctx {
     DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
     std::vector<FieldMapHost> fmaps;  // field maps on the beamline, in the order they were added
     DevBuf fm_desc, fm_data;
+    DevBuf os_dense;  // per-block moment sums of the one-shot gather pass
     DevBuf binb, gamb, cntb;  // energy bins: per-particle bin, per-bin gamma, per-bin count
     // tracker energy bins: the resident bunch is ordered by bin
     BinRanges bins;
@@ -264,7 +265,8 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     c->grid.destroy();
     free_bunch(c->bunch);
     DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb,      &c->statsb,  &c->tmpd,
-                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->fm_desc, &c->fm_data};
+                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->fm_desc, &c->fm_data,
+                    &c->os_dense};
     for (DevBuf *b : bs) b->release();
     void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
     for (void *p : ps)
@@ -1018,6 +1020,8 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     launch_track_init_moments(c->stream, c->partials, red_blocks(c->bunch.N), c->d_ts, c->d_sc);
     CHECK_LAUNCH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->grid.fp64 && !c->os_dense.ensure((size_t)os_blocks(std::max<int64_t>(c->bunch.cap, 1)) * kMom * 8))
+        c->os_dense.release();  // the persistent gather pass then serves
     c->have_track = true;
     ++c->gen;
     c->track_sc = p->space_charge != 0;
@@ -1225,15 +1229,20 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
         }
     }
     const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
+    int nparts = gamma_blocks(c->grid.fp64, b.N);
+    // the common mixed case in blocks that run once (2.30 -> 2.20 ms at 1e8 particles)
+    const bool oneshot = !c->grid.fp64 && !binned && !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
     if (binned && c->track_sc)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
                            c->grid.Bsum, ext, false);
+    else if (oneshot)
+        nparts = launch_track_gamma_os(c->stream, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->os_dense.as<double>(),
+                                       c->partials, ext);
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
                            ext, c->grid.pz);
     c->prof.mark(PROF_GATHER_PUSH);
-    launch_track_post(c->stream, c->partials, gamma_blocks(c->grid.fp64, b.N), c->d_ts, c->d_sc, c->statsb.p,
-                      &c->d_counters[1]);
+    launch_track_post(c->stream, c->partials, nparts, c->d_ts, c->d_sc, c->statsb.p, &c->d_counters[1]);
     c->prof.mark(PROF_SCALAR);
 }
 

@@ -190,6 +190,11 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, bool exact, bool q_roundtrip, Scalars *sc);
 int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
+// the mixed gather pass in blocks that run once (no bins, DC beam or maps):
+// os_blocks(N) rows of kMom doubles in dense; returns the partial blocks for k_track_post
+int os_blocks(int64_t N);
+int launch_track_gamma_os(hipStream_t s, bool next, Bunch &b, const void *Eg, const Scalars *sc,
+                          const TrackState *ts, double *dense, double *partials, int ext);
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
