@@ -1020,7 +1020,8 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     launch_track_init_moments(c->stream, c->partials, red_blocks(c->bunch.N), c->d_ts, c->d_sc);
     CHECK_LAUNCH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (!c->grid.fp64 && !c->os_dense.ensure((size_t)os_blocks(std::max<int64_t>(c->bunch.cap, 1)) * kMom * 8))
+    const int64_t cap1 = std::max<int64_t>(c->bunch.cap, 1);
+    if (!c->os_dense.ensure((size_t)std::max(os_blocks(cap1), os_blocks64(cap1)) * kMom * 8))
         c->os_dense.release();  // the persistent gather pass then serves
     c->have_track = true;
     ++c->gen;
@@ -1231,13 +1232,13 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
     int nparts = gamma_blocks(c->grid.fp64, b.N);
     // the common mixed case in blocks that run once (2.30 -> 2.20 ms at 1e8 particles)
-    const bool oneshot = !c->grid.fp64 && !binned && !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
+    const bool oneshot = !binned && !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
     if (binned && c->track_sc)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
                            c->grid.Bsum, ext, false);
     else if (oneshot)
-        nparts = launch_track_gamma_os(c->stream, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->os_dense.as<double>(),
-                                       c->partials, ext);
+        nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts,
+                                       c->os_dense.as<double>(), c->partials, ext);
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
                            ext, c->grid.pz);

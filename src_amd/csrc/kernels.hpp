@@ -193,7 +193,8 @@ int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamm
 // the mixed gather pass in blocks that run once (no bins, DC beam or maps):
 // os_blocks(N) rows of kMom doubles in dense; returns the partial blocks for k_track_post
 int os_blocks(int64_t N);
-int launch_track_gamma_os(hipStream_t s, bool next, Bunch &b, const void *Eg, const Scalars *sc,
+int os_blocks64(int64_t N);
+int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                           const TrackState *ts, double *dense, double *partials, int ext);
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field

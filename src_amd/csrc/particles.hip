@@ -1245,7 +1245,8 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
 // BINNED (energy bins): Eg and Bg hold the sums over the bins of E and of the
 // magnetic field each bin and its image imply (ParallelTTracker.cpp:567-579),
 // both gathered.  Four waves per SIMD (at most 128 VGPRs) keep enough loads in
-// flight.
+// flight.  The plain FP64 case (no bins, DC beam or field maps) runs in
+// k_track_gamma_os instead.
 template <bool NEXT, bool BINNED, bool MAPS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma(
     int64_t N, double *x, double *y, double *z, double *px, double *py, double *pz, const double4 *Eg,
@@ -1580,6 +1581,107 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
     }
 }
 
+// The FP64 gather pass (no energy bins, DC beam or field maps) in blocks that
+// run once, kOsU64 particles per thread, the per-particle sequence of
+// k_track_gamma unchanged; moment sums as in k_track_gamma_mixed_os.
+constexpr int kOsU64 = 2;
+constexpr int kOsChunk64 = kThreads * kOsU64;
+int os_blocks64(int64_t N) {
+    const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
+    const int64_t per = (nchunks + kXcds - 1) / kXcds;
+    return (int)(per * kXcds);
+}
+
+template <bool NEXT>
+__global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, double *y, double *z, double *px,
+                                                        double *py, double *pz, const double4 *Eg, const Scalars *sc,
+                                                        const TrackState *ts, double *dense) {
+    __shared__ double wsum[kThreads / 64][kMom];
+    const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
+    const int64_t per = (nchunks + kXcds - 1) / kXcds;
+    const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    double m[kMom];
+    for (int k = 0; k < kMom; ++k) m[k] = 0.0;
+    if (chunk < nchunks) {
+        const bool sc_on = ts->space_charge != 0;
+        const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
+        const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
+        const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
+        const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
+        const double betaC = sc->betaC;
+        const int64_t base = chunk * kOsChunk64 + threadIdx.x;
+        double rr[kOsU64][6];
+#pragma unroll
+        for (int u = 0; u < kOsU64; ++u) {
+            const int64_t i = base + u * kThreads;
+            const int64_t j = i < N ? i : N - 1;
+            rr[u][0] = x[j];
+            rr[u][1] = y[j];
+            rr[u][2] = z[j];
+            rr[u][3] = px[j];
+            rr[u][4] = py[j];
+            rr[u][5] = pz[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kOsU64; ++u) {
+            const int64_t i = base + u * kThreads;
+            if (i >= N) break;
+            double r[3] = {rr[u][0], rr[u][1], rr[u][2]};
+            double p[3] = {rr[u][3], rr[u][4], rr[u][5]};
+            double Ef[3] = {0.0, 0.0, 0.0}, Bf[3] = {0.0, 0.0, 0.0};
+            if (sc_on) {
+                transform_to(sc->M_rb, sc->o_rb, r);
+                int l[3];
+                double d[3];
+                bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
+                ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
+                ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
+                if (ok) gather_E<double>(Eg, l, d, n0, n1, n2, Ef);
+                Bf[0] = Bf[0] - betaC * Ef[1];
+                Bf[1] = Bf[1] + betaC * Ef[0];
+                double t3[3];
+                transform_to(sc->M_br, sc->o_br, r);
+                mat_vec(sc->M_br, Ef, t3);
+                Ef[0] = t3[0];
+                Ef[1] = t3[1];
+                Ef[2] = t3[2];
+                mat_vec(sc->M_br, Bf, t3);
+                Bf[0] = t3[0];
+                Bf[1] = t3[1];
+                Bf[2] = t3[2];
+            }
+            for (int k = 0; k < 3; ++k) {
+                Ef[k] += sc->Eext[k];
+                Bf[k] += sc->Bext[k];
+            }
+            double pn[3];
+            boris_kick(p, Ef, Bf, dT, mass, charge);
+            push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
+            transform_to(sc->M_u, sc->o_u, r);
+            mat_vec(sc->M_u, p, pn);
+            accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+            if constexpr (NEXT) push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
+            x[i] = r[0];
+            y[i] = r[1];
+            z[i] = r[2];
+            px[i] = pn[0];
+            py[i] = pn[1];
+            pz[i] = pn[2];
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < kMom; ++k) m[k] = wave_sum(m[k]);
+    if (lane == 0)
+        for (int k = 0; k < kMom; ++k) wsum[wave][k] = m[k];
+    __syncthreads();
+    if (threadIdx.x < kMom) {
+        double t = wsum[0][threadIdx.x];
+        for (int w = 1; w < kThreads / 64; ++w) t += wsum[w][threadIdx.x];
+        dense[(int64_t)blockIdx.x * kMom + threadIdx.x] = t;
+    }
+}
+
 // Dense per-block moment sums -> at most kOsReduceBlocks double-double partials
 // (rows in a fixed order per block), the layout k_track_post reads.
 constexpr int kOsReduceBlocks = 256;
@@ -1603,8 +1705,15 @@ __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64
         }
 }
 
-int launch_track_gamma_os(hipStream_t s, bool next, Bunch &b, const void *Eg, const Scalars *sc,
+int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                           const TrackState *ts, double *dense, double *partials, int ext) {
+    if (fp64) {
+        const int blocks64 = os_blocks64(b.N);
+        if (next) k_track_gamma_os<true><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense);
+        else k_track_gamma_os<false><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense);
+        k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks64, partials);
+        return kOsReduceBlocks;
+    }
     const int blocks = os_blocks(b.N);
     if (next) {
         if (ext) k_track_gamma_mixed_os<true, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense);
