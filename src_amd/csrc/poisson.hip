@@ -111,26 +111,40 @@ __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, 
 // reference's order (FFTPoissonSolver.cpp:510-517) with G[0,0,0] := G[0,0,1]
 // (FFTPoissonSolver.cpp:657).  Index n uses T at n+1 from the same formula;
 // the physical potential never reads G at index n.  Grid: x over the
-// (n0+1)*(n1+1) points of a plane, y over k.  The shifted (image-charge)
-// tables use the same difference without the quirk (FFTPoissonSolver.cpp:621-639).
-__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1, int quirk) {
+// (n0+1)*(n1+1) points of a plane, y over runs of kIgfDiffK planes; each
+// thread walks its column up the run and keeps the four corners of the plane
+// below in registers, so every T value is loaded once per run instead of
+// twice.  The shifted (image-charge) tables use the same difference without
+// the quirk (FFTPoissonSolver.cpp:621-639).
+constexpr int kIgfDiffK = 8;
+__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1, int n2, int quirk) {
     const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= gplane) return;
     const int j = div_small(c, g0, 1.0f / (float)g0), i = c - j * g0;
-    int k = blockIdx.y;
-    const int kq = (quirk && i == 0 && j == 0 && k == 0) ? 1 : k;
+    const int kbeg = blockIdx.y * kIgfDiffK;
+    const int kend = kbeg + kIgfDiffK < n2 + 1 ? kbeg + kIgfDiffK : n2 + 1;
     const size_t m0 = (size_t)n0 + 2, plane = m0 * (n1 + 2);
-    const double *t = T + (size_t)kq * plane + (size_t)j * m0 + i;
-    double g = t[plane + m0 + 1];
-    g += t[1];
-    g += t[m0];
-    g += t[plane];
-    g -= t[m0 + 1];
-    g -= t[plane + 1];
-    g -= t[plane + m0];
-    g -= t[0];
-    G[(size_t)k * gplane + c] = g;
+    const double *t = T + (size_t)kbeg * plane + (size_t)j * m0 + i;
+    double a00 = t[0], a01 = t[1], a10 = t[m0], a11 = t[m0 + 1];
+    for (int k = kbeg; k < kend; ++k) {
+        t += plane;
+        const double b00 = t[0], b01 = t[1], b10 = t[m0], b11 = t[m0 + 1];
+        double g = b11;
+        g += a01;
+        g += a10;
+        g += b00;
+        g -= a11;
+        g -= b01;
+        g -= b10;
+        g -= a00;
+        G[(size_t)k * gplane + c] = g;
+        if (quirk && c == 0 && k == 1) G[0] = g;
+        a00 = b00;
+        a01 = b01;
+        a10 = b10;
+        a11 = b11;
+    }
 }
 
 // STANDARD Green's function, SpecializedGreensFunction<3>::calculate
@@ -750,8 +764,8 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t
         dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
         if (cr) k_igf_corners<true><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
         else k_igf_corners<false><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
-        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
-        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 1);
+        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK);
+        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, n2, 1);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
@@ -787,8 +801,8 @@ static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipSt
         dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
         if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
         else k_igf_corners_shifted<false><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
-        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, n2 + 1);
-        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, 0);
+        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK);
+        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, n2, 0);
         dct_xy<T>(g, (const double *)g.Goct, H[second], st);
     }
     g.profile_mark(PROF_GREEN);

@@ -733,6 +733,29 @@ def test_tracker_few_particles():
         assert np.allclose(a["rrms"], b["rrms"], rtol=1e-10, atol=0)
 
 
+@pytest.mark.parametrize("N", [511, 513, 4097])
+def test_tracker_ragged_block_edges(N):
+    """Bunch sizes one either side of the gather pass's 512- and 4096-particle
+    block edges (one-shot blocks with a partial last block, and XCD shares of
+    unequal length)."""
+    n, steps = (8, 8, 8), 3
+    R, P, Q = gaussian_bunch(N, seed=31, q_total=1e-11)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.3)
+    Ro, Po, _, _, so = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, N=N) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.3)
+        sg = c.track(steps)
+        Rg, Pg, _ = c.get_bunch()
+    assert rel_l2(Rg, Ro) < 1e-12
+    assert rel_l2(Pg - P, Po - P) < 1e-9
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm", "prms"):
+            assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])
+        assert np.all(np.abs(np.asarray(a["rmean"]) - b["rmean"]) <= 1e-12 * np.asarray(b["rrms"]))
+
+
 class _HipArrays:
     """Device arrays from the HIP runtime the library itself links (hipMalloc/hipMemcpy via ctypes)."""
 
