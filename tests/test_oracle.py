@@ -331,6 +331,148 @@ def test_self_field_of_gaussian_bunch_matches_erf_formula():
     assert errs[1] < 0.03 and errs[1] < 0.6 * errs[0], errs
 
 
+def _gaussian_field(X, sig, Qt):
+    """E at points X of a Gaussian charge Qt with per-axis sigmas sig, from
+    1/r = (2/sqrt(pi)) int_0^inf exp(-r^2 s^2) ds: convolving the heat kernel
+    with the Gaussian gives a one-dimensional integral per point,
+    E_i = k Qt (2/sqrt(pi)) int_0^inf 2 s^2 x_i / a_i prod_j a_j^(-1/2)
+    exp(-s^2 x_j^2 / a_j) ds with a_j = 1 + 2 s^2 sig_j^2, evaluated by
+    400-point Gauss-Legendre in s = tan(theta)/min(sig)."""
+    th, w = np.polynomial.legendre.leggauss(400)
+    th, w = (th + 1) * (np.pi / 4), w * (np.pi / 4)
+    s0 = 1.0 / min(sig)
+    s, ds = np.tan(th) * s0, s0 / np.cos(th) ** 2
+    sig = np.asarray(sig, dtype=np.float64)
+    E = np.zeros_like(X)
+    for k in range(len(s)):
+        a = 1 + 2 * s[k] ** 2 * sig ** 2
+        f = np.prod(a ** -0.5) * np.exp(-(s[k] ** 2 * X ** 2 / a).sum(axis=1))
+        E += (w[k] * ds[k] * 2 * s[k] ** 2) * f[:, None] * X / a
+    return O.COUPLING * Qt * (2 / np.sqrt(np.pi)) * E
+
+
+def test_gaussian_field_helper_against_erf_and_quad():
+    """The 1D-integral field of a Gaussian equals the erf closed form for a
+    sphere and scipy's adaptive quadrature for an elongated bunch."""
+    from scipy import integrate
+    from scipy.special import erf
+    s, Qt = 1e-3, 1e-9
+    X = np.random.default_rng(0).normal(size=(50, 3)) * s
+    r = np.linalg.norm(X, axis=1)
+    x = r / (math.sqrt(2) * s)
+    Er = O.COUPLING * Qt / r ** 2 * (erf(x) - np.sqrt(2 / np.pi) * r / s * np.exp(-x * x))
+    E = _gaussian_field(X, (s, s, s), Qt)
+    assert np.max(np.abs((E * X).sum(1) / r / Er - 1)) < 1e-12
+    sig = np.array([s, s, 2 * s])
+    for p in X[:3]:
+        def f(t, i):
+            a = 1 + 2 * t * t * sig ** 2
+            return 2 * t * t * p[i] / a[i] * np.prod(a ** -0.5) * np.exp(-(t * t * p ** 2 / a).sum())
+        q = [integrate.quad(f, 0, np.inf, args=(i,), limit=200, epsabs=0, epsrel=1e-12)[0] for i in range(3)]
+        Eq = O.COUPLING * Qt * 2 / np.sqrt(np.pi) * np.array(q)
+        assert np.max(np.abs(_gaussian_field(p[None], sig, Qt)[0] / Eq - 1)) < 1e-12
+
+
+def _boosted_gaussian_fields(R, s, Qt, g):
+    """Lab-frame E and B of a spherical Gaussian (sigma s) moving along z
+    with Lorentz factor g: the rest-frame field of the bunch stretched to
+    (s, s, g s), evaluated at (x, y, g z), boosted as E_perp = g E'_perp,
+    E_z = E'_z, B = v x E / c^2."""
+    X = R.copy()
+    X[:, 2] *= g
+    E = _gaussian_field(X, (s, s, g * s), Qt) * np.array([g, g, 1.0])
+    beta = math.sqrt(g * g - 1) / g
+    B = np.stack([-beta / C * E[:, 1], beta / C * E[:, 0], np.zeros(len(E))], axis=1)
+    return E, B
+
+
+def test_self_fields_of_moving_gaussian_match_the_boosted_rest_frame_field():
+    """PartBunch::computeSelfFields at gamma = 2 (PartBunch.cpp:497-511,573-575,
+    641-644): the mesh stretch h_z*gamma, the 1/gamma density, the
+    (gamma, gamma, 1/gamma) field scaling and B = beta/c x E together must give
+    the field of a moving charge.  The oracle converges to the boosted closed
+    form at O(h^2) (8 % at 32^3, 2.3 % at 64^3); swapping gamma and 1/gamma
+    gives a 70 % error and the unboosted field 33 %."""
+    N, s, Qt, g = 400000, 1e-3, 1e-9, 2.0
+    R, P, Q = gaussian_bunch(N, seed=21, q_total=Qt, p0=math.sqrt(g * g - 1), sigma_p=(0, 0, 0))
+    r = np.linalg.norm(R, axis=1)
+    sel = (r > 0.5 * s) & (r < 2.5 * s)
+    Ea, Ba = _boosted_gaussian_fields(R[sel], s, Qt, g)
+    Xw = R[sel].copy()
+    Xw[:, 2] /= g
+    Ewrong = _gaussian_field(Xw, (s, s, s / g), Qt) * np.array([1 / g, 1 / g, 1.0])
+    errs = []
+    for n in ((32, 32, 32), (64, 64, 64)):
+        hr, origin = O.boundp(R, n)
+        out = O.compute_self_fields(R, P, Q, np.full(N, 1e-11), 1e-11, origin, hr, n)
+        e = rel_l2(out["Ef"][sel], Ea)
+        errs.append(e)
+        assert rel_l2(out["Bf"][sel], Ba) < 1.05 * e + 1e-3
+        assert rel_l2(out["Ef"][sel], Ewrong) > 0.5
+    assert errs[1] < 0.035 and errs[1] < 0.4 * errs[0], errs
+
+
+def test_binned_self_fields_of_moving_gaussian_match_the_boosted_field():
+    """The binned path's own Lorentz handling (calcGammas, the /gamma density,
+    h_z*gamma, PartBunch.cpp:155-182,239-248) at gamma = 2: one bin holding the
+    cold bunch gives the boosted closed-form field, as the plain path does."""
+    N, s, Qt, g = 400000, 1e-3, 1e-9, 2.0
+    R, P, _ = gaussian_bunch(N, seed=22, q_total=Qt, p0=math.sqrt(g * g - 1), sigma_p=(0, 0, 0))
+    assert abs(O.calc_gammas(P, np.zeros(N, np.int32), 1)[0] - g) < 1e-14
+    r = np.linalg.norm(R, axis=1)
+    sel = (r > 0.5 * s) & (r < 2.5 * s)
+    Ea, Ba = _boosted_gaussian_fields(R[sel], s, Qt, g)
+    n = (64, 64, 64)
+    hr, origin = O.boundp(R, n)
+    out = O.compute_self_fields_binned(R, P, np.zeros(N, np.int32), 1, Qt / N, np.full(N, 1e-11), 1e-11,
+                                       origin, hr, n, image=False)
+    e = rel_l2(out["Ef"][sel], Ea)
+    assert e < 0.035, e
+    assert rel_l2(out["Bf"][sel], Ba) < 1.05 * e + 1e-3
+
+
+@pytest.mark.parametrize("g", [1.0, 1.7])
+def test_image_charge_is_the_mirrored_bunch(g):
+    """The image solve (zshift = -(2 o_z + n h_z) gamma, PartBunch.cpp:252-264;
+    FFTPoissonSolver.cpp:289-330) gives the field of the bunch mirrored in the
+    cathode plane z = 0 with opposite charge.  Independent construction: on a
+    mesh with 2 o_z / h_z = 20 the mirrored cells fall on the bunch's own
+    lattice, so one open-boundary solve on a z-extended grid that holds the
+    negated, mirrored charge below the bunch's cells gives the image field on
+    those cells.  Particles whose stencils avoid the border cells (where Grad
+    is one-sided) must agree to solver rounding; the image moved by one cell
+    is 5 % off."""
+    n, M = (16, 16, 16), 20
+    h = np.array([0.5e-3, 0.5e-3, 0.5e-3])
+    org = np.array([-8 * h[0], -8 * h[1], (M / 2) * h[2]])
+    R, P, _ = gaussian_bunch(20000, seed=31, sigma_r=(0.8e-3, 0.8e-3, 0.8e-3), p0=1e-6, sigma_p=(0, 0, 0))
+    R[:, 2] += org[2] + 8 * h[2]
+    lgp, _, _ = O.cic_index(R, org, h)
+    keep = np.all((lgp >= 1) & (lgp <= np.array(n) - 3), axis=1)
+    R, P = R[keep], P[keep]
+    N = len(R)
+    qi, dT = 1e-9 / N, 1e-11
+    dt = np.full(N, dT)
+    bins = np.zeros(N, np.int32)
+    a = O.compute_self_fields_binned(R, P, bins, 1, qi, dt, dT, org, h, n, image=True, gammas=[g])
+    b = O.compute_self_fields_binned(R, P, bins, 1, qi, dt, dT, org, h, n, image=False, gammas=[g])
+    Eimg = a["Ef"] - b["Ef"]
+    assert np.all(Eimg[:, 2] < 0)  # the image attracts the bunch towards the cathode
+    rho, _ = O.scatter(R, np.full(N, qi * dT), org, h, n)
+    rho = (rho / dT * (1 / h[0] * 1 / h[1] * 1 / h[2] / g)).reshape(n[2], n[1], n[0])
+    nzb = M + 2 * n[2]  # image cells m = -(M+n) .. -(M+1) at z = -z_k', bunch cells m = 0 .. n-1
+    big = np.zeros((nzb, n[1], n[0]))
+    big[:n[2]] = -rho[::-1]  # image of cell k' at m = -M-1-k', i.e. index n-1-k'
+    hs = h * np.array([1.0, 1.0, g])
+    nb = (n[0], n[1], nzb)
+    phi = O.compute_potential(nb, big.ravel(), hs) * np.prod(hs) * O.COUPLING
+    E = (-O.grad(nb, phi, h) * np.array([g, g, 1 / g])).reshape(nzb, n[1], n[0], 3)
+    Eg = O.gather(R, np.ascontiguousarray(E[M + n[2]:]).reshape(-1, 3), org, h, n)
+    assert rel_l2(Eimg, Eg) < 1e-9
+    Eoff = O.gather(R, np.ascontiguousarray(E[M + n[2] - 1:M + 2 * n[2] - 1]).reshape(-1, 3), org, h, n)
+    assert rel_l2(Eimg, Eoff) > 0.01
+
+
 # ---------------------------------------------------------------- pusher
 def test_push_is_half_drift():
     rng = np.random.default_rng(7)
