@@ -125,12 +125,10 @@ def run_c5(sc, steps, warmup, sort_interval, log):
             "kernels": {k: {"avg_ms": ms_ / nl, "launches": nl} for k, (ms_, nl) in prof.items() if nl}}
 
 
-def run_c4(sc, steps, warmup, log):
-    """64 independent bunches (particle counts log-uniform in [1e4, 1e6], grids
-    from {16, 32, 48, 64}^3, sigma uniform in [0.5, 2] mm, 1-100 MeV protons,
-    per-bunch seeds, mixed precision), stepped one bunch after another with
-    opalsc_track, then together with opalsc_step_batched without and with the
-    HIP graph."""
+def c4_specs():
+    """The 64 independent bunches: (N, n, sigma, beta*gamma) with particle
+    counts log-uniform in [1e4, 1e6], grids from {16, 32, 48, 64}^3, sigma
+    uniform in [0.5, 2] mm and 1-100 MeV protons, from one seeded generator."""
     rng = np.random.default_rng(4)
     specs = []
     for k in range(64):
@@ -139,16 +137,27 @@ def run_c4(sc, steps, warmup, log):
         sig = float(rng.uniform(0.5e-3, 2e-3))
         _, bg = gamma_betagamma(float(rng.uniform(1.0, 100.0)))
         specs.append((N, n, sig, bg))
+    return specs
+
+
+def c4_context(sc, k, spec, precision):
+    """Bunch k of the 64-bunch set (seed 1000 + k, 1 nC) ready to track."""
+    N, n, sig, bg = spec
+    c = sc.SpaceChargeContext((n, n, n), precision=precision, max_particles=N)
+    c.generate_bunch(N, 1000 + k, (sig, sig, sig), (1e-3, 1e-3, 1e-3), bg, 1e-9)
+    c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, ref_p=(0.0, 0.0, bg))
+    return c
+
+
+def run_c4(sc, steps, warmup, log):
+    """64 independent bunches (c4_specs, per-bunch seeds, mixed precision),
+    stepped one bunch after another with opalsc_track, then together with
+    opalsc_step_batched without and with the HIP graph."""
+    specs = c4_specs()
     total = sum(s[0] for s in specs)
 
     def make():
-        cs = []
-        for k, (N, n, sig, bg) in enumerate(specs):
-            c = sc.SpaceChargeContext((n, n, n), precision=sc.MIXED, max_particles=N)
-            c.generate_bunch(N, 1000 + k, (sig, sig, sig), (1e-3, 1e-3, 1e-3), bg, 1e-9)
-            c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, ref_p=(0.0, 0.0, bg))
-            cs.append(c)
-        return cs
+        return [c4_context(sc, k, s, sc.MIXED) for k, s in enumerate(specs)]
 
     from src_amd.opalsc import lib
     out = {"workload": "64 bunches, %d particles in total (1e4-1e6 each), grids 16/32/48/64^3, mixed precision"

@@ -656,7 +656,7 @@ static int self_fields_core(opalsc_ctx *c, int64_t N, const double *R, const dou
     launch_selffield_prep(c->stream, N, P, Q, dt, b.qd, c->partials, &c->d_counters[0]);
     launch_selffield_scalars(c->stream, c->partials, red_blocks(N), N, dT, origin, hr, c->cfg.n, coupling,
                              &c->d_counters[0], c->grid.fp64, c->grid.pz, c->d_sc);
-    launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+    c->grid.zero_rho(c->stream);
     launch_deposit(c->stream, 0, c->grid.fp64, N, b.x, b.y, b.z, nullptr, b.qd, (long long *)c->grid.rho_i64,
                    c->d_sc, &c->d_counters[1], nullptr);
     c->grid.green(c->d_sc, c->grid.fp64);
@@ -773,7 +773,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
         if (cnt[k] == 0) continue;  // an empty bin has no field
         const int64_t s0 = start[k], nk = (int64_t)cnt[k];
         launch_bin_scalars(c->stream, c->d_sc, c->gamb.as<double>(), k, nk, &c->d_counters[0], exact);
-        launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+        c->grid.zero_rho(c->stream);
         launch_deposit(c->stream, 0, exact, nk, b.px + s0, b.py + s0, b.pz + s0, nullptr, b.qd + s0,
                        (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], nullptr);
         c->grid.green(c->d_sc, exact);
@@ -1191,7 +1191,7 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                         (void)hipEventRecord(c->ev_gimg, c->stream2);
                     }
                 }
-                launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+                c->grid.zero_rho(c->stream);
                 c->prof.mark(PROF_ZERO);
                 launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
                                      (long long *)c->grid.rho_i64, sk, &c->d_counters[1],
@@ -1219,7 +1219,7 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                 c->grid.green(c->d_sc, cr, c->stream2);
                 (void)hipEventRecord(c->ev_green, c->stream2);
             }
-            launch_fill_i64(c->stream, c->grid.ncell, (long long *)c->grid.rho_i64);
+            c->grid.zero_rho(c->stream);
             c->prof.mark(PROF_ZERO);
             launch_deposit(c->stream, c->q_uniform ? 3 : 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr,
                            (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);

@@ -117,18 +117,8 @@ __device__ inline void bin_scalars(Scalars *sc, double gammaz, int64_t count, do
     sc->betaC = sqrt(gammaz * gammaz - 1.0) / gammaz / kC;
     const double hz = hr[2];
     sc->zshift = -(2 * sc->origin[2] + (0 + (sc->n[2] - 1) + 1) * hz) * gammaz * scaleFactor;
-    // fixed point: count deposits of |qd| <= qdmax (FP64: sum < 2^62; mixed: each < 2^24)
-    const double B = (exact ? (double)(count > 0 ? count : 1) : 1.0) * qdmax * (1.0 + 0x1p-40);
-    int e = 0;
-    if (B > 0.0) {
-        int E;
-        (void)frexp(B, &E);
-        e = (exact ? 62 : 24) - E;
-        if (e > 1000) e = 1000;
-    }
-    sc->e2q = ldexp(1.0, e);
-    sc->q2e = ldexp(1.0, -e);
-    sc->qd_absmax = qdmax;
+    // fixed point: count deposits of |qd| <= qdmax
+    fixed_point_scale(sc, count, qdmax, exact);
 }
 
 __global__ void k_bin_scalars(Scalars *sc, const double *gammas, int b, int64_t count,

@@ -17,6 +17,7 @@ constexpr double kPi = 3.14159265358979323846;   // src/Classic/Physics/Physics.
 constexpr int kThreads = 256;                    // threads per block for particle kernels
 constexpr int kRedBlocksMax = 8192;              // reduction partial slots (blocks of the streaming kernels)
 constexpr int kMomentSlots = 32;                 // 27 moments + gamma sum + spare
+constexpr int kDepositChunk = 2048;              // particles per deposit / bounds chunk
 
 template <typename T> struct vec4_of;
 template <> struct vec4_of<float> { using type = float4; };
@@ -31,8 +32,12 @@ struct Scalars {
     int n[3];
     double hr[3], origin[3];   // mesh of the beam frame (boundp)
     double rmin[3], rmax[3];   // particle bounds
-    // deposit fixed point: value = int * q2e; e chosen from N*max|Q*dt|
-    double q2e, e2q;
+    // deposit fixed point (fixed_point_scale): each deposit is rounded to an
+    // integer at the scale e2q.  Mixed path: one word per cell, value = int * q2e.
+    // FP64 path: two words per cell, value = hi * q2e + lo * q2e_lo, where a
+    // flushed sum v is split into hi = v >> lo_shift and lo = v - (hi << lo_shift).
+    double q2e, e2q, q2e_lo;
+    int lo_shift;
     double qd_absmax;
     // computeSelfFields scalars (src/Classic/Algorithms/PartBunch.cpp:496-644)
     double dT, gammaz, betaC, hs[3], tmp2, vol, coupling;
@@ -62,6 +67,39 @@ struct Scalars {
     // time of the bunch's field evaluation (t + dT/2)
     double fm_s0, fm_t;
 };
+
+// Fixed-point scale of the charge deposits of `count` particles, each
+// depositing at most |qdmax| (IntCIC weights are <= 1).
+// Mixed path: one deposit below 2^24 (|qd| * 2^e < 2^24), so 64 deposits sum
+// in 32 bits and any cell sum stays far below 2^63.
+// FP64 path: the high word's unit 2^-e with count * qdmax * 2^e < 2^62 bounds
+// every cell sum of the high words; each deposit is rounded at the finer scale
+// 2^(e+s) with 2^s <= count / kDepositChunk, which bounds the sum of one
+// deposit chunk (kDepositChunk particles) in an int64 LDS cell.  A chunk's
+// flushed cell sum v is split exactly into hi = v >> s (added to the high word)
+// and lo = v - (hi << s) in [0, 2^s) (added to the low word, which sums at most
+// 8 * count such terms, < 2^56).  The rounding error per deposit is then
+// 2^-(e+s) instead of 2^-e: at 1e8 particles 15 bits finer than one global
+// scale for the whole bunch.
+__host__ __device__ inline void fixed_point_scale(Scalars *sc, int64_t count, double qdmax, bool exact) {
+    const double B = (exact ? (double)(count > 0 ? count : 1) : 1.0) * qdmax * (1.0 + 0x1p-40);
+    int e = 0, s = 0;
+    if (B > 0.0) {
+        int E;
+        (void)frexp(B, &E);
+        e = (exact ? 62 : 24) - E;
+    }
+    if (exact)
+        while (((int64_t)kDepositChunk << (s + 1)) <= count) ++s;
+    if (e + s > 1000) {  // tiny charges: keep the scales finite
+        e = 1000 - s;
+    }
+    sc->e2q = ldexp(1.0, e + s);
+    sc->q2e = ldexp(1.0, -e);
+    sc->q2e_lo = ldexp(1.0, -(e + s));
+    sc->lo_shift = s;
+    sc->qd_absmax = qdmax;
+}
 
 // ---- external field maps ------------------------------------------------------
 // A Solenoid with a 2D magnetostatic map (FM2DMagnetoStatic) or an RFCavity

@@ -64,7 +64,7 @@ struct PoissonGrid {
     size_t ncell = 0, noct = 0;
     FftPlan plan[3];
     void *twd[3] = {nullptr, nullptr, nullptr};
-    void *rho_i64 = nullptr;  // long long[ncell]
+    void *rho_i64 = nullptr;  // fixed-point charge: long long[ncell] (mixed), [ncell][2] hi/lo (FP64)
     void *rho_f64 = nullptr;  // double[ncell] (compute_potential input)
     void *A = nullptr, *B = nullptr;
     void *Goct = nullptr;     // double[noct]: G on the octant
@@ -100,6 +100,8 @@ struct PoissonGrid {
     // Esum += E, Bsum += (s*betaC*Ey, -s*betaC*Ex): s = -1 a bin's field, +1 its image
     void grad_accum(const Scalars *d_sc, double sign);
     void rho_to_double(const Scalars *d_sc, double *out);
+    size_t rho_words() const { return fp64 ? 2 * ncell : ncell; }
+    void zero_rho(hipStream_t s);  // clears every word of rho_i64
     void widen(int which, double *out);
     void profile_mark(int id) {
         if (prof) prof->mark(id);
@@ -107,7 +109,6 @@ struct PoissonGrid {
 };
 
 constexpr int kSlots = 64;           // doubles per block in a partial-sums buffer
-constexpr int kDepositChunk = 2048;  // particles per deposit / bounds chunk
 constexpr int kBoxStats = 12;        // per chunk: min[3], max[3], sum[3], sum of squares[3]
 
 // Particle buffers in structure-of-arrays form.

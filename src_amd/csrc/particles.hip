@@ -78,22 +78,9 @@ void launch_selffield_prep(hipStream_t s, int64_t N, const double *P, double *Q,
     k_selffield_prep<<<red_blocks(N), kThreads, 0, s>>>(N, P, Q, dt, qd, partials, qdmax_bits);
 }
 
-// Fixed-point exponent.  FP64 path: every cell sum is bounded by N*max|Q*dt|;
-// scaling by 2^e with N*max|qd|*2^e < 2^62 leaves headroom below 2^63 for
-// rounding.  Mixed path: one deposit is below 2^24 (|qd|*2^e < 2^24), so 64
-// deposits sum in 32 bits and any cell sum stays far below 2^63.
+// Fixed-point scale of a deposit of all N particles (fixed_point_scale).
 __device__ inline void set_fixed_point(Scalars *sc, int64_t N, double qdmax, bool exact) {
-    double B = (exact ? (double)N : 1.0) * qdmax * (1.0 + 0x1p-40);
-    int e = 0;
-    if (B > 0.0) {
-        int E;
-        (void)frexp(B, &E);
-        e = (exact ? 62 : 24) - E;
-        if (e > 1000) e = 1000;
-    }
-    sc->e2q = ldexp(1.0, e);
-    sc->q2e = ldexp(1.0, -e);
-    sc->qd_absmax = qdmax;
+    fixed_point_scale(sc, N, qdmax, exact);
 }
 
 // The computeSelfFields scalars (PartBunch.cpp:497-511,542-554,573-575,641).
@@ -190,6 +177,21 @@ __device__ inline int small_div(int a, int b, float rb) {
 constexpr int kBoxCapSmall = 4864, kBoxCapLarge = 5632;
 constexpr int64_t kLargeBoxMaxParticles = int64_t(8) << 20;
 
+// Adds a cell sum v (deposit scale) to the charge grid: one word per cell on
+// the mixed path; on the FP64 path the exact split of fixed_point_scale,
+// hi = floor(v / 2^s) to word 2c and lo = v - hi * 2^s in [0, 2^s) to word 2c+1.
+template <bool EXACT>
+__device__ inline void rho_add(long long *rho, size_t c, long long v, int s) {
+    if constexpr (EXACT) {
+        const long long hi = v >> s;
+        const long long lo = v - hi * ((long long)1 << s);
+        if (hi) atomicAdd((unsigned long long *)&rho[2 * c], (unsigned long long)hi);
+        if (lo) atomicAdd((unsigned long long *)&rho[2 * c + 1], (unsigned long long)lo);
+    } else {
+        atomicAdd((unsigned long long *)&rho[c], (unsigned long long)v);
+    }
+}
+
 // MODE 0: qd given; 1 (tracker): qd = Q*dT with Q left as (Q*dT)/dT (FP64 path), positions
 // given in the reference frame and rotated into the beam frame (M_rb, o_rb)
 // exactly as the bounds pass rotated them; 2 (tracker, energy bin): as 1 with
@@ -217,6 +219,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
     const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double e2q = sc->e2q, dT = sc->dT;
+    const int lsh = sc->lo_shift;
     const int64_t chunk = chunk0 + blockIdx.x;
     const int64_t base = chunk * kChunk;
 
@@ -402,7 +405,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
                     if (use_lds && ci >= 0 && cj >= 0 && ck >= 0 && ci < bx && cj < by && ck < bz)
                         atomicAdd((unsigned long long *)&box[(ck * by + cj) * bx + ci], add);
                     else
-                        atomicAdd((unsigned long long *)&rho[((size_t)kk * n1 + j) * n0 + i], add);
+                        rho_add<EXACT>(rho, ((size_t)kk * n1 + j) * n0 + i, (long long)acc[c], lsh);
                 }
             }
         }
@@ -500,8 +503,7 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
                 const int r = e - lk * sxy;
                 const int lj = small_div(r, bx, rbx);
                 const int li = r - lj * bx;
-                atomicAdd((unsigned long long *)&rho[((size_t)(lk + bz0) * n1 + (lj + by0)) * n0 + (li + bx0)],
-                          (unsigned long long)v);
+                rho_add<EXACT>(rho, ((size_t)(lk + bz0) * n1 + (lj + by0)) * n0 + (li + bx0), v, lsh);
             }
         }
     }

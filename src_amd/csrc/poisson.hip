@@ -117,6 +117,11 @@ __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, 
 // twice.  The shifted (image-charge) tables use the same difference without
 // the quirk (FFTPoissonSolver.cpp:621-639).
 constexpr int kIgfDiffK = 8;
+// The quirk G[0,0,0] := G[0,0,1] is written by the thread of column 0 in the
+// run that holds planes 0 and 1 (k = 1 is computed after k = 0 in the same
+// thread, so no other block writes G[0] afterwards): the first run must hold
+// both planes, and n2 >= 4 (PoissonGrid::init) guarantees plane 1 exists.
+static_assert(kIgfDiffK >= 2, "the run of planes 0..kIgfDiffK-1 must include plane 1 for the G[0,0,0] quirk");
 __global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1, int n2, int quirk) {
     const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
     const int c = blockIdx.x * 256 + threadIdx.x;
@@ -311,7 +316,13 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         tmp2 = sc->tmp2;
     }
     const C *tw = fft_twiddles(buf, pl);
-    using S = typename std::conditional<SRC == 0, long long, double>::type;
+    // SRC 0: the fixed-point charge grid, two words per cell (hi, lo) on the
+    // FP64 path and one on the mixed path
+    constexpr bool kTwoWords = SRC == 0 && std::is_same<T, double>::value;
+    using S = typename std::conditional<SRC == 0, typename std::conditional<kTwoWords, longlong2, long long>::type,
+                                        double>::type;
+    double q2e_lo = 0.0;
+    if constexpr (kTwoWords) q2e_lo = sc->q2e_lo;
     const S *s = (const S *)src;
     S r[Q];
 #pragma unroll
@@ -326,7 +337,11 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         if (e < total) {
             const int ln = div_small(e, n0, rn), x = e - ln * n0;
             double d;
-            if constexpr (SRC == 0) {
+            if constexpr (kTwoWords) {
+                d = (double)r[q].x * q2e + (double)r[q].y * q2e_lo;
+                d = d / dT;
+                d = d * tmp2;
+            } else if constexpr (SRC == 0) {
                 d = (double)r[q] * q2e;
                 d = d / dT;
                 d = d * tmp2;
@@ -552,11 +567,11 @@ __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>:
 }
 
 // rho entering the solver, for diagnostics
-__global__ void k_rho_to_double(const long long *rho, double *out, size_t n, const Scalars *sc) {
-    const double q2e = sc->q2e, dT = sc->dT, tmp2 = sc->tmp2;
+__global__ void k_rho_to_double(const long long *rho, double *out, size_t n, const Scalars *sc, int two_words) {
+    const double q2e = sc->q2e, q2e_lo = sc->q2e_lo, dT = sc->dT, tmp2 = sc->tmp2;
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
          c += (size_t)gridDim.x * blockDim.x) {
-        double d = (double)rho[c] * q2e;
+        double d = two_words ? (double)rho[2 * c] * q2e + (double)rho[2 * c + 1] * q2e_lo : (double)rho[c] * q2e;
         d = d / dT;
         d = d * tmp2;
         out[c] = d;
@@ -690,7 +705,7 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
         if (hipMalloc(p, bytes) != hipSuccess) { err = "hipMalloc failed (" + std::to_string(bytes) + " bytes)"; return false; }
         return true;
     };
-    if (!alloc(&rho_i64, ncell * 8) || !alloc(&rho_f64, ncell * 8) || !alloc(&A, szA) ||
+    if (!alloc(&rho_i64, rho_words() * 8) || !alloc(&rho_f64, ncell * 8) || !alloc(&A, szA) ||
         !alloc(&B, szB) || !alloc(&Goct, noct * 8) ||
         !alloc(&Tcorner, (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8) || !alloc(&Ghat, szG) ||
         !alloc(&phi, ncell * rs) || !alloc(&Eg, ncell * 4 * rs))
@@ -953,8 +968,10 @@ void PoissonGrid::grad_accum(const Scalars *d_sc, double sign) {
     profile_mark(PROF_GRAD);
 }
 
+void PoissonGrid::zero_rho(hipStream_t s) { launch_fill_i64(s, rho_words(), (long long *)rho_i64); }
+
 void PoissonGrid::rho_to_double(const Scalars *d_sc, double *out) {
-    k_rho_to_double<<<1024, 256, 0, stream>>>((const long long *)rho_i64, out, ncell, d_sc);
+    k_rho_to_double<<<1024, 256, 0, stream>>>((const long long *)rho_i64, out, ncell, d_sc, fp64 ? 1 : 0);
 }
 
 void PoissonGrid::widen(int which, double *out) {

@@ -177,6 +177,18 @@ def _d(a, shape=None):
     return a
 
 
+def _per_particle(a, N, name, vec=False):
+    """a as a contiguous float64 array of N scalars or N 3-vectors (AoS); a
+    ValueError otherwise, before any pointer reaches the C side."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    want = (N, 3) if vec else (N,)
+    if vec and a.ndim == 1 and a.size == 3 * N:
+        a = a.reshape(N, 3)
+    if a.shape != want:
+        raise ValueError("%s must have shape %s, got %s" % (name, want, a.shape))
+    return a
+
+
 def stats_to_dict(s):
     d = {k: np.array(getattr(s, k)[:]) for k in
          ("rmean", "pmean", "rrms", "prms", "eps_norm", "eps", "rprms")}
@@ -308,12 +320,13 @@ class SpaceChargeContext:
 
     # PartBunch::computeSelfFields
     def computeSelfFields(self, R, P, Q, dt, dT, origin, hr, coupling=COUPLING, Bf=None):
-        R, P = _d(R, (-1, 3)), _d(P, (-1, 3))
+        R = _d(R, (-1, 3))
         N = len(R)
-        Q = _d(Q).copy()
-        dt = _d(dt)
+        P = _per_particle(P, N, "P", vec=True)
+        Q = _per_particle(Q, N, "Q").copy()
+        dt = _per_particle(dt, N, "dt")
         Ef = np.zeros((N, 3))
-        Bf = np.zeros((N, 3)) if Bf is None else _d(Bf, (-1, 3)).copy()
+        Bf = np.zeros((N, 3)) if Bf is None else _per_particle(Bf, N, "Bf", vec=True).copy()
         self._c(lib().opalsc_compute_self_fields(self._h, N, _ptr(R), _ptr(P), _ptr(Q), _ptr(dt),
                                                  dT, _ptr(_d(origin)), _ptr(_d(hr)), coupling,
                                                  _ptr(Ef), _ptr(Bf)))
@@ -323,18 +336,20 @@ class SpaceChargeContext:
     # PartBunch::computeSelfFields(int bin)
     def computeSelfFieldsBinned(self, R, P, bin, nbins, qi, dt, dT, origin, hr,
                                 coupling=COUPLING, gammas=None, image=False, Bf=None):
-        R, P = _d(R, (-1, 3)), _d(P, (-1, 3))
+        R = _d(R, (-1, 3))
         N = len(R)
+        P = _per_particle(P, N, "P", vec=True)
+        dt = _per_particle(dt, N, "dt")
         bin = np.ascontiguousarray(bin, dtype=np.int32)
         if bin.shape != (N,):
             raise ValueError("bin must hold one entry per particle")
         Ef = np.zeros((N, 3))
-        Bf = np.zeros((N, 3)) if Bf is None else _d(Bf, (-1, 3)).copy()
+        Bf = np.zeros((N, 3)) if Bf is None else _per_particle(Bf, N, "Bf", vec=True).copy()
         g = None if gammas is None else _d(gammas)
         if g is not None and len(g) < nbins:
             raise ValueError("gammas must hold nbins entries")
         self._c(lib().opalsc_compute_self_fields_binned(
-            self._h, N, _ptr(R), _ptr(P), _ptr(bin), int(nbins), float(qi), _ptr(_d(dt)), dT,
+            self._h, N, _ptr(R), _ptr(P), _ptr(bin), int(nbins), float(qi), _ptr(dt), dT,
             _ptr(_d(origin)), _ptr(_d(hr)), coupling, None if g is None else _ptr(g),
             1 if image else 0, _ptr(Ef), _ptr(Bf)))
         return Ef, Bf
@@ -346,18 +361,25 @@ class SpaceChargeContext:
 
     def pushParticles(self, R, P, dt):
         R = _d(R, (-1, 3)).copy()
-        self._c(lib().opalsc_push_half(self._h, len(R), _ptr(R), _ptr(_d(P, (-1, 3))), _ptr(_d(dt))))
+        N = len(R)
+        P = _per_particle(P, N, "P", vec=True)
+        dt = _per_particle(dt, N, "dt")
+        self._c(lib().opalsc_push_half(self._h, N, _ptr(R), _ptr(P), _ptr(dt)))
         return R
 
     def kickParticles(self, P, Ef, Bf, dt, charge=1.0, mass_eV=M_P_EV):
         P = _d(P, (-1, 3)).copy()
-        self._c(lib().opalsc_kick(self._h, len(P), _ptr(P), _ptr(_d(Ef, (-1, 3))),
-                                  _ptr(_d(Bf, (-1, 3))), _ptr(_d(dt)), charge, mass_eV))
+        N = len(P)
+        Ef = _per_particle(Ef, N, "Ef", vec=True)
+        Bf = _per_particle(Bf, N, "Bf", vec=True)
+        dt = _per_particle(dt, N, "dt")
+        self._c(lib().opalsc_kick(self._h, N, _ptr(P), _ptr(Ef), _ptr(Bf), _ptr(dt), charge, mass_eV))
         return P
 
     def calcBeamParameters(self, R, P, mass_eV=M_P_EV):
         s = Stats()
-        R, P = _d(R, (-1, 3)), _d(P, (-1, 3))
+        R = _d(R, (-1, 3))
+        P = _per_particle(P, len(R), "P", vec=True)
         self._c(lib().opalsc_beam_parameters(self._h, len(R), _ptr(R), _ptr(P), mass_eV,
                                              C.byref(s)))
         return stats_to_dict(s)
@@ -376,7 +398,9 @@ class SpaceChargeContext:
 
     # ---- device-resident tracker (ParallelTTracker::execute) ----
     def set_bunch(self, R, P, Q):
-        R, P, Q = _d(R, (-1, 3)), _d(P, (-1, 3)), _d(Q)
+        R = _d(R, (-1, 3))
+        P = _per_particle(P, len(R), "P", vec=True)
+        Q = _per_particle(Q, len(R), "Q")
         self._c(lib().opalsc_bunch_set(self._h, len(R), _ptr(R), _ptr(P), _ptr(Q)))
         self.N = len(R)
 

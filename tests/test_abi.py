@@ -87,3 +87,29 @@ def test_special_values():
     yl = sc.host_log_cr(np.array([1.0, 2.0, 1e-310, np.inf]))
     assert yl[0] == 0.0 and yl[1] == np.log(2.0) and yl[3] == np.inf
     assert yl[2] == float(np.log(np.float64(1e-310)))
+
+
+def test_mirror_rejects_short_per_particle_arrays_before_the_c_call():
+    """Every per-particle argument of the Python mirror must hold N scalars or
+    N 3-vectors; a scalar Q or a short dt raises ValueError before a pointer
+    reaches the library (the method never touches the context's handle)."""
+    c = object.__new__(sc.SpaceChargeContext)  # no device context behind it
+    R = np.zeros((4, 3))
+    good = dict(P=np.zeros((4, 3)), Q=np.zeros(4), dt=np.zeros(4))
+    bad = [("Q", 1e-12), ("dt", np.zeros(3)), ("P", np.zeros((3, 3)))]
+    for name, val in bad:
+        args = dict(good, **{name: val})
+        with pytest.raises(ValueError, match=name):
+            c.computeSelfFields(R, args["P"], args["Q"], args["dt"], 1e-11, np.zeros(3), np.ones(3))
+    with pytest.raises(ValueError, match="Bf"):
+        c.computeSelfFields(R, good["P"], good["Q"], good["dt"], 1e-11, np.zeros(3), np.ones(3),
+                            Bf=np.zeros((5, 3)))
+    with pytest.raises(ValueError, match="dt"):
+        c.pushParticles(R, good["P"], np.zeros(2))
+    with pytest.raises(ValueError, match="Ef"):
+        c.kickParticles(good["P"], np.zeros((2, 3)), np.zeros((4, 3)), good["dt"])
+    with pytest.raises(ValueError, match="Q"):
+        c.set_bunch(R, good["P"], np.zeros(5))
+    with pytest.raises(ValueError, match="dt"):
+        c.computeSelfFieldsBinned(R, good["P"], np.zeros(4, np.int32), 1, 1e-12, np.zeros(1), 1e-11,
+                                  np.zeros(3), np.ones(3))

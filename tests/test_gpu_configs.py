@@ -1,0 +1,190 @@
+"""The benchmark configurations at their own sizes, on the GPU.
+
+C1: 1e5 particles, 32^3, 100 steps, FP64 against the oracle step by step.
+C3: 1e8 particles, 256^3 (10 % halo, 250 MeV): the FP64 charge grid against
+    the oracle's scatter of the same positions, and the mixed-precision step
+    against the FP64 step (E-field and emittance gates).
+C4: the 64-bunch batch against per-bunch steps, and three of its bunches in
+    FP64 against the oracle.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import gaussian_bunch, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+sc = pytest.importorskip("src_amd")
+import bench  # noqa: E402  (the benchmark's own configurations)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def test_c1_full_run_matches_oracle_every_step():
+    """C1 (1e5 particles, 1 MeV, 1 nC, 32^3, drift, dT = 1e-11 s) for all 100
+    steps: per-step beam sizes and transverse emittances within 1e-10 of the
+    oracle.  The longitudinal emittance subtracts N<pz>^2 from sum pz^2 with
+    (<pz>/sigma_pz)^2 ~ 2e3, so it carries ~1e-12 * 2e3 of the positions'
+    rounding differences and is held to 1e-8."""
+    n, N, steps = (32, 32, 32), 100_000, 100
+    R, P, Q = gaussian_bunch(N, seed=1, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181))
+    t0 = time.time()
+    Ro, Po, _, _, so = O.track(tr, R, P, Q, dt, steps)
+    log("oracle C1 100 steps: %.1f s" % (time.time() - t0))
+    with sc.SpaceChargeContext(n, max_particles=N, sort_interval=20) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181))
+        sg = c.track(steps)
+        Rg, Pg, _ = c.get_bunch()
+    assert rel_l2(Rg, Ro) < 1e-12
+    assert rel_l2(Pg - P, Po - P) < 1e-9
+    worst = 0.0
+    for a, b in zip(sg, so):
+        for d in range(3):
+            worst = max(worst, abs(a["rrms"][d] / b["rrms"][d] - 1))
+            assert abs(a["rrms"][d] / b["rrms"][d] - 1) < 1e-10, ("rrms", d)
+        for d in range(2):
+            assert abs(a["eps_norm"][d] / b["eps_norm"][d] - 1) < 1e-10, ("eps_norm", d)
+        assert abs(a["eps_norm"][2] / b["eps_norm"][2] - 1) < 1e-8
+    log("C1 worst rrms relative difference over 100 steps: %.2e" % worst)
+
+
+@pytest.fixture(scope="module")
+def c3_bunch():
+    """C3's bunch as host arrays: 1e8 particles, 90 % Gaussian (1 mm, momentum
+    spread 1e-3) and 10 % uniform in the 5-sigma ball, 250 MeV protons, 10 nC."""
+    cfg = bench.CONFIGS["c3"]
+    _, bg = bench.gamma_betagamma(cfg["ekin"])
+    t0 = time.time()
+    R, P, Q = gaussian_bunch(cfg["N"], seed=cfg["seed"], p0=bg, q_total=cfg["q"], halo_frac=cfg["halo"])
+    log("C3 host bunch: %.1f s" % (time.time() - t0))
+    return R, P, Q
+
+
+def test_c3_fp64_charge_grid_and_cells_match_oracle(c3_bunch):
+    """FP64 computeSelfFields at C3's size: the mesh and every particle's cell
+    bit-exact, and rho (after /dT and the Lorentz-scaled 1/h^3) within 1e-12 of
+    the oracle's sequential scatter of the same positions (IntCIC.h:816-860)."""
+    R, P, Q = c3_bunch
+    N, n = len(R), (256, 256, 256)
+    dt = np.full(N, 1e-11)
+    hr, org = O.boundp(R, n)
+    t0 = time.time()
+    lgp_o, dpos_o, _ = O.cic_index(R, org, hr)
+    with sc.SpaceChargeContext(n, max_particles=N) as c:
+        hr_g, org_g = c.boundp(R)
+        assert np.array_equal(hr_g, hr) and np.array_equal(org_g, org)
+        lgp, dpos = c.cic_index(R, org, hr)
+        assert np.array_equal(lgp, lgp_o) and np.array_equal(dpos, dpos_o)
+        del lgp, dpos, lgp_o, dpos_o
+        c.computeSelfFields(R, P, Q, dt, 1e-11, org, hr)
+        rho = c.get_grid(sc.GRID_RHO)
+    log("C3 GPU side: %.1f s" % (time.time() - t0))
+    rho_o, dropped = O.scatter(R, Q * dt, org, hr, n)
+    assert dropped == 0
+    gz = np.sqrt((P[:, 2].sum() / N) ** 2 + 1.0)
+    rho_o = rho_o / 1e-11 * (1 / hr[0] * 1 / hr[1] * 1 / (hr[2] * gz))
+    err = rel_l2(rho, rho_o)
+    log("C3 FP64 rho rel L2 vs oracle scatter: %.3e" % err)
+    assert err < 1e-12
+
+
+def test_c3_mixed_step_against_fp64_step():
+    """The mixed-precision gates at C3 on the benchmark's own bunch: E-field
+    relative L2 of one step from the same bunch <= 1e-5, and the per-step
+    normalised emittance within 1e-6 of the FP64 step over 21 steps (one, then
+    twenty in one call)."""
+    cfg = bench.CONFIGS["c3"]
+    cd = bench.make_ctx(sc, cfg, sc.FP64, 20)
+    cm = bench.make_ctx(sc, cfg, sc.MIXED, 20)
+    try:
+        s64 = cd.track(1)
+        s32 = cm.track(1)
+        E64 = cd.get_grid(sc.GRID_EFIELD)
+        E32 = cm.get_grid(sc.GRID_EFIELD)
+        e_rel = rel_l2(E32, E64)
+        del E64, E32
+        s64 += cd.track(20)
+        s32 += cm.track(20)
+    finally:
+        cd.close()
+        cm.close()
+    emit = max(abs(a["eps_norm"][d] / b["eps_norm"][d] - 1) for a, b in zip(s32, s64) for d in range(3))
+    rrms = max(abs(a["rrms"][d] / b["rrms"][d] - 1) for a, b in zip(s32, s64) for d in range(3))
+    log("C3 mixed vs FP64: E rel L2 %.3e, emittance %.3e, rms size %.3e over %d steps" %
+        (e_rel, emit, rrms, len(s64)))
+    assert len(s64) == 21
+    assert e_rel <= 1e-5
+    assert emit <= 1e-6
+    assert rrms <= 1e-6
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_c4_batch_equals_per_bunch_steps(graph):
+    """The 64-bunch set (1.57e7 particles on 16-64^3 grids, mixed precision):
+    three batched steps give every bunch the state and statistics that three
+    single-step calls of its own context give, bit for bit."""
+    specs = bench.c4_specs()
+    steps = 3
+    single = []
+    for k, s in enumerate(specs):
+        c = bench.c4_context(sc, k, s, sc.MIXED)
+        st = [c.track(1)[0] for _ in range(steps)]
+        single.append((c.get_bunch(), st))
+        c.close()
+    cs = [bench.c4_context(sc, k, s, sc.MIXED) for k, s in enumerate(specs)]
+    try:
+        with sc.Batch(cs) as b:
+            sb = b.step(steps, graph=graph)
+        for k, c in enumerate(cs):
+            (R1, P1, Q1), st1 = single[k]
+            R2, P2, Q2 = c.get_bunch()
+            assert np.array_equal(R1, R2) and np.array_equal(P1, P2) and np.array_equal(Q1, Q2), k
+            for s in range(steps):
+                for key in ("rrms", "eps_norm", "rmean", "pmean"):
+                    assert np.array_equal(st1[s][key], sb[s][k][key]), (k, s, key)
+    finally:
+        for c in cs:
+            c.close()
+
+
+def test_c4_bunches_fp64_match_oracle():
+    """Three bunches of the 64-bunch set (a 16^3, a 32^3 and a 48^3 one) in a
+    batch, FP64, three steps: positions within 1e-12 and statistics within
+    1e-10 of the oracle's tracker on the same particles."""
+    specs = bench.c4_specs()
+    pick = []
+    for n in (16, 32, 48):
+        k = min((k for k, s in enumerate(specs) if s[1] == n), key=lambda k: specs[k][0])
+        pick.append(k)
+    steps = 3
+    cs = [bench.c4_context(sc, k, specs[k], sc.FP64) for k in pick]
+    try:
+        init = [c.get_bunch() for c in cs]
+        with sc.Batch(cs) as b:
+            sb = b.step(steps, graph=True)
+        for j, (k, c) in enumerate(zip(pick, cs)):
+            N, n, sig, bg = specs[k]
+            R, P, Q = init[j]
+            tr = O.make_tracker((n, n, n), 1e-11, ref_p=(0.0, 0.0, bg))
+            Ro, Po, _, _, so = O.track(tr, R, P, Q, np.full(N, 1e-11), steps)
+            Rg, Pg, _ = c.get_bunch()
+            assert rel_l2(Rg, Ro) < 1e-12, k
+            assert rel_l2(Pg - P, Po - P) < 1e-9, k
+            for s in range(steps):
+                a, o = sb[s][j], so[s]
+                for key in ("rrms", "eps_norm"):
+                    for d in range(3):
+                        tol = 1e-8 if (key == "eps_norm" and d == 2) else 1e-10
+                        assert abs(a[key][d] / o[key][d] - 1) < tol, (k, s, key, d)
+    finally:
+        for c in cs:
+            c.close()

@@ -733,27 +733,33 @@ def test_tracker_few_particles():
         assert np.allclose(a["rrms"], b["rrms"], rtol=1e-10, atol=0)
 
 
-@pytest.mark.parametrize("N", [511, 513, 4097])
-def test_tracker_ragged_block_edges(N):
-    """Bunch sizes one either side of the gather pass's 512- and 4096-particle
-    block edges (one-shot blocks with a partial last block, and XCD shares of
-    unequal length)."""
+@pytest.mark.parametrize("precision,N", [(sc.FP64, 511), (sc.FP64, 513), (sc.FP64, 4097),
+                                         (sc.MIXED, 1023), (sc.MIXED, 1025), (sc.MIXED, 8193)])
+def test_tracker_ragged_block_edges(precision, N):
+    """Bunch sizes one either side of the gather pass's block edges: FP64
+    one-shot blocks of 512 particles (4096 per XCD share), mixed one-shot
+    blocks of 1024 (8192 per XCD share, with the N-1 clamp of their loads);
+    a partial last block and XCD shares of unequal length.  Mixed tolerances:
+    positions 1e-9, statistics 1e-6 relative."""
     n, steps = (8, 8, 8), 3
     R, P, Q = gaussian_bunch(N, seed=31, q_total=1e-11)
     dt = np.full(N, 1e-11)
     tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.3)
     Ro, Po, _, _, so = O.track(tr, R, P, Q, dt, steps)
-    with ctx(n, N=N) as c:
+    fp64 = precision == sc.FP64
+    with ctx(n, precision=precision, N=N) as c:
         c.set_bunch(R, P, Q)
         c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.3)
         sg = c.track(steps)
         Rg, Pg, _ = c.get_bunch()
-    assert rel_l2(Rg, Ro) < 1e-12
-    assert rel_l2(Pg - P, Po - P) < 1e-9
+    assert rel_l2(Rg, Ro) < (1e-12 if fp64 else 1e-9)
+    assert rel_l2(Pg - P, Po - P) < (1e-9 if fp64 else 1e-6)
+    rtol = 1e-10 if fp64 else 1e-6
     for a, b in zip(sg, so):
         for k in ("rrms", "eps_norm", "prms"):
-            assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])
-        assert np.all(np.abs(np.asarray(a["rmean"]) - b["rmean"]) <= 1e-12 * np.asarray(b["rrms"]))
+            assert np.allclose(a[k], b[k], rtol=rtol, atol=0), (k, a[k], b[k])
+        tol_mean = 1e-12 if fp64 else 1e-6
+        assert np.all(np.abs(np.asarray(a["rmean"]) - b["rmean"]) <= tol_mean * np.asarray(b["rrms"]))
 
 
 class _HipArrays:
