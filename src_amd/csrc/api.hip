@@ -322,17 +322,7 @@ static int download_scalars(opalsc_ctx *c, Scalars &s) {
 // Poisson solver
 // ---------------------------------------------------------------------------
 // Every byte of [p, p + bytes) is device memory the library may read and write.
-static bool is_device_range(const void *p, size_t bytes) {
-    if (!p) return false;
-    hipPointerAttribute_t a;
-    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) return false;
-    (void)bytes;
-    return true;
-}
+static bool is_device_range(const void *p, size_t bytes) { return device_range_ok(p, bytes); }
 
 static void potential_scalars(opalsc_ctx *c, const double hr[3]) {
     Scalars s;

@@ -10,6 +10,26 @@ namespace opalsc {
 
 void set_global_error(const std::string &msg);
 
+// Every byte of [p, p + bytes) is device (or managed) memory: the pointer's
+// attributes, then the extent of the allocation that holds it.  Allocates
+// nothing.
+inline bool device_range_ok(const void *p, size_t bytes) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;  // memory from a pool whose extent the runtime does not report
+    }
+    return (const char *)p + bytes <= (const char *)base + size;
+}
+
 // A parsed 2D field map (host), FieldMapDesc offsets relative to its own grids.
 struct FieldMapHost {
     FieldMapDesc d;
