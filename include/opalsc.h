@@ -230,6 +230,19 @@ int opalsc_kick(opalsc_ctx *ctx, int64_t N, double *P, const double *Ef, const d
 int opalsc_beam_parameters(opalsc_ctx *ctx, int64_t N, const double *R, const double *P,
                            double mass_eV, opalsc_stats *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): opalsc_beam_parameters on device arrays R_dev, P_dev (AoS
+ * double[N][3]) into out_dev (device memory, one opalsc_stats; t, pathLength
+ * and dropped are 0), enqueued on `stream` (a hipStream_t, NULL: the null
+ * stream) of the current device, without synchronising and without
+ * allocating.  The reduction's partial sums live in a 4 MiB file-scope device
+ * array (at most 8192 rows: every N up to 2^32-16 fits); each GPU holds its
+ * copy until the process ends, and calls that overlap on one GPU race on it.
+ * Errors are read with opalsc_last_error(NULL). */
+int opalsc_beam_parameters_noalloc(int64_t N, const double *R_dev, const double *P_dev, double mass_eV,
+                                   opalsc_stats *out_dev, void *stream);
+
 /* --- device-resident tracker --------------------------------------------
  * The ParallelTTracker::execute step loop (ParallelTTracker.cpp:343-388) with
  * the bunch kept in HBM: push, beam-frame rotation, boundp, self fields,

@@ -25,7 +25,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
-SOURCES = ["api.hip", "bins.hip", "particles.hip", "poisson.hip", "sort.hip", "io.cpp"]
+SOURCES = ["api.hip", "bins.hip", "particles.hip", "poisson.hip", "sort.hip", "noalloc.hip", "io.cpp"]
 
 
 def _compile(src, verbose):

@@ -38,7 +38,8 @@ EXPORTS = [
     "opalsc_boundp", "opalsc_set_beam_frequency", "opalsc_wrap_periodic_z", "opalsc_cic_index",
     "opalsc_sort_by_cell", "opalsc_compute_self_fields", "opalsc_compute_self_fields_binned",
     "opalsc_get_grid", "opalsc_push_half",
-    "opalsc_kick", "opalsc_beam_parameters", "opalsc_bunch_set", "opalsc_bunch_generate",
+    "opalsc_kick", "opalsc_beam_parameters", "opalsc_beam_parameters_noalloc", "opalsc_bunch_set",
+    "opalsc_bunch_generate",
     "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init",
     "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
@@ -130,6 +131,7 @@ def lib():
         "opalsc_push_half": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_kick": ([vp, i64, dp, dp, dp, dp, C.c_double, C.c_double], C.c_int),
         "opalsc_beam_parameters": ([vp, i64, dp, dp, C.c_double, C.POINTER(Stats)], C.c_int),
+        "opalsc_beam_parameters_noalloc": ([i64, vp, vp, C.c_double, vp, vp], C.c_int),
         "opalsc_bunch_set": ([vp, i64, dp, dp, dp], C.c_int),
         "opalsc_bunch_generate": ([vp, C.POINTER(GaussParams)], C.c_int),
         "opalsc_bunch_get": ([vp, dp, dp, dp], C.c_int),
