@@ -197,8 +197,9 @@ def algorithmic_bytes(cfg, wbytes):
     return {
         # first step of a call: read x,y,z,px,py,pz; write x,y,z
         "push_rotate_bounds": N * 72,
-        # later steps (the previous gather pass applied the half push): read x,y,z
-        "rotate_bounds": N * 24,
+        # later steps of a call take their bounds from the previous gather pass's
+        # chunk statistics and rotate only the chunks that can hold an extreme:
+        # no fixed per-particle traffic, reported by time only
         # read x,y,z (the synthetic bunch's particles all carry the same charge,
         # kept as one number); read+write int64 rho once
         "deposit": N * 24 + n3 * 16,

@@ -61,11 +61,12 @@ struct opalsc_ctx {
     Scalars *d_sc = nullptr;
     TrackState *d_ts = nullptr;
     double *partials = nullptr;
-    unsigned long long *d_counters = nullptr;  // [0] qd max bits, [1] dropped
+    unsigned long long *d_counters = nullptr;  // [0] qd max bits, [1] dropped, [8..13] bound keys
     DevBuf aosR, aosP, aosEf, aosBf, dtb, statsb, tmpd;
     std::vector<FieldMapHost> fmaps;  // field maps on the beamline, in the order they were added
     DevBuf fm_desc, fm_data;
     DevBuf os_dense;  // per-block moment sums of the one-shot gather pass
+    DevBuf osbox;     // per one-shot block: statistics of the positions it wrote (the next bounds)
     DevBuf binb, gamb, cntb;  // energy bins: per-particle bin, per-bin gamma, per-bin count
     // tracker energy bins: the resident bunch is ordered by bin
     BinRanges bins;
@@ -242,7 +243,7 @@ extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
     if (hipMalloc((void **)&c->d_sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc((void **)&c->d_ts, sizeof(TrackState)) != hipSuccess ||
         hipMalloc((void **)&c->partials, sizeof(double) * kSlots * kRedBlocksMax) != hipSuccess ||
-        hipMalloc((void **)&c->d_counters, sizeof(unsigned long long) * 4) != hipSuccess) {
+        hipMalloc((void **)&c->d_counters, sizeof(unsigned long long) * 16) != hipSuccess) {
         c->err = "hipMalloc of scalar buffers failed";
         return bail(OPALSC_ENOMEM);
     }
@@ -250,7 +251,7 @@ extern "C" int opalsc_create(opalsc_ctx **out, const opalsc_config *cfg) {
     memset(&z, 0, sizeof(z));
     for (int d = 0; d < 3; ++d) z.n[d] = cfg->n[d];
     if (hipMemcpy(c->d_sc, &z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 4) != hipSuccess) {
+        hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 16) != hipSuccess) {
         c->err = "initialisation copy failed";
         return bail(OPALSC_EHIP);
     }
@@ -266,7 +267,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     free_bunch(c->bunch);
     DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb,      &c->statsb,  &c->tmpd,
                     &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->fm_desc, &c->fm_data,
-                    &c->os_dense};
+                    &c->os_dense, &c->osbox};
     for (DevBuf *b : bs) b->release();
     void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
     for (void *p : ps)
@@ -1013,6 +1014,8 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
     const int64_t cap1 = std::max<int64_t>(c->bunch.cap, 1);
     if (!c->os_dense.ensure((size_t)std::max(os_blocks(cap1), os_blocks64(cap1)) * kMom * 8))
         c->os_dense.release();  // the persistent gather pass then serves
+    if (!c->osbox.ensure((size_t)std::max(os_box_count(true, cap1), os_box_count(false, cap1)) * kOsBoxStats * 8))
+        c->osbox.release();  // every step then runs the full bounds pass
     c->have_track = true;
     ++c->gen;
     c->track_sc = p->space_charge != 0;
@@ -1138,14 +1141,20 @@ static int prepare_track(opalsc_ctx *c) {
 // ts->step - ts->call_step0 - 1 of statsb.  overlap: the Green's function may
 // run on the context's second stream (a single bunch; a batch already runs
 // bunches side by side).
-static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, bool overlap) {
+static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, bool overlap, bool boxes) {
     Bunch &b = c->bunch;
     const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     const bool binned = c->bins.nbins > 0;
-    launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1]);
+    unsigned long long *bkeys = &c->d_counters[8];
+    launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
     c->prof.mark(PROF_SCALAR);
-    if (c->track_sc || push) {
+    if (c->track_sc && !push && boxes) {
+        // the previous gather pass left the statistics of these positions
+        launch_bounds_from_boxes(c->stream, c->grid.fp64, b, c->osbox.as<double>(), c->d_sc, bkeys);
+        launch_bounds_scan(c->stream, exact, b, c->d_sc, bkeys, c->partials);
+        c->prof.mark(PROF_ROTATE_BOUNDS);
+    } else if (c->track_sc || push) {
         launch_track_alpha(c->stream, exact, push, b, c->d_sc, c->d_ts, c->partials);
         c->prof.mark(push ? PROF_PUSH_BOUNDS : PROF_ROTATE_BOUNDS);
     }
@@ -1223,18 +1232,22 @@ static void enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     int nparts = gamma_blocks(c->grid.fp64, b.N);
     // the common mixed case in blocks that run once (2.30 -> 2.20 ms at 1e8 particles)
     const bool oneshot = !binned && !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
+    // the next step of this call takes its bounds from the statistics this pass leaves
+    const bool boxes_out = oneshot && next && c->track_sc && c->osbox.p;
     if (binned && c->track_sc)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
                            c->grid.Bsum, ext, false);
     else if (oneshot)
         nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts,
-                                       c->os_dense.as<double>(), c->partials, ext);
+                                       c->os_dense.as<double>(), c->partials, ext,
+                                       boxes_out ? c->osbox.as<double>() : nullptr);
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
                            ext, c->grid.pz);
     c->prof.mark(PROF_GATHER_PUSH);
     launch_track_post(c->stream, c->partials, nparts, c->d_ts, c->d_sc, c->statsb.p, &c->d_counters[1]);
     c->prof.mark(PROF_SCALAR);
+    return boxes_out;
 }
 
 extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
@@ -1256,12 +1269,14 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
     c->prof.start();
     launch_track_begin_call(c->stream, c->d_ts);
     // Within one call the gather pass of step s also applies step s+1's first
-    // half push, so the bounds pass of step s+1 only reads positions.  The
+    // half push, so the bounds pass of step s+1 only reads positions (or, after
+    // a one-shot gather pass, only the chunks that can hold an extreme).  The
     // last step of a call leaves the bunch at its end-of-step state.
+    bool boxes = false;
     for (int s = 0; s < nsteps; ++s) {
         const int interval = c->cfg.sort_interval;
         const bool sort_now = c->track_sc && interval > 0 && (c->steps_done % interval) == 0;
-        enqueue_step(c, s == 0, s + 1 < nsteps, sort_now, true);
+        boxes = enqueue_step(c, s == 0, s + 1 < nsteps, sort_now, true, boxes);
         ++c->steps_done;
         CHECK_LAUNCH(c);
     }
@@ -1367,7 +1382,7 @@ static int batch_capture(opalsc_batch *b) {
         opalsc_ctx *c = b->ctx[i];
         e = hipStreamWaitEvent(c->stream, b->fork, 0);
         if (e != hipSuccess) break;
-        enqueue_step(c, true, false, false, false);
+        (void)enqueue_step(c, true, false, false, false, false);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(b->join[i], c->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(b->master, b->join[i], 0);
@@ -1420,7 +1435,7 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
     } else {
         for (int s = 0; s < nsteps; ++s)
             for (int i = 0; i < n; ++i) {
-                enqueue_step(b->ctx[i], true, false, false, false);
+                (void)enqueue_step(b->ctx[i], true, false, false, false, false);
                 BHIP(b, hipGetLastError());
             }
     }

@@ -130,6 +130,7 @@ struct PoissonGrid {
 
 constexpr int kSlots = 64;           // doubles per block in a partial-sums buffer
 constexpr int kBoxStats = 12;        // per chunk: min[3], max[3], sum[3], sum of squares[3]
+constexpr int kOsBoxStats = 6;       // per one-shot gather block: min[3], max[3] of the positions written
 
 // Particle buffers in structure-of-arrays form.
 struct Bunch {
@@ -203,7 +204,9 @@ void launch_copy_calibration(hipStream_t s, size_t bytes, const void *src, void 
 void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y);
 
 // Tracker kernels
-void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped);
+// bkeys (may be null): the six bound keys of launch_bounds_from_boxes, reset for the step
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
+                      unsigned long long *bkeys);
 int alpha_blocks(int64_t N);
 void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Scalars *sc,
                         const TrackState *ts, double *partials);
@@ -215,8 +218,20 @@ int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamm
 // os_blocks(N) rows of kMom doubles in dense; returns the partial blocks for k_track_post
 int os_blocks(int64_t N);
 int os_blocks64(int64_t N);
+// osbox (used with next): per one-shot block the bounds (kOsBoxStats doubles) of the
+// positions written, which the next step's bounds come from
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                          const TrackState *ts, double *dense, double *partials, int ext);
+                          const TrackState *ts, double *dense, double *partials, int ext, double *osbox);
+// blocks of statistics launch_track_gamma_os leaves (one per one-shot block)
+int64_t os_box_count(bool fp64, int64_t N);
+// The bounds pass from the previous gather pass's statistics: the beam-frame
+// chunk statistics into b.chunkbox and the six bound keys (reset by
+// launch_track_pre), then the exact bounds of the chunks that can hold an
+// extreme into partials (alpha_blocks(N) rows, as launch_track_alpha leaves them).
+void launch_bounds_from_boxes(hipStream_t s, bool fp64, const Bunch &b, const double *osbox, const Scalars *sc,
+                              unsigned long long *bkeys);
+void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc,
+                        const unsigned long long *bkeys, double *partials);
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,

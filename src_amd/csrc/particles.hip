@@ -16,6 +16,11 @@
 namespace opalsc {
 
 constexpr int kXcds = 8;             // XCDs of an MI355X; workgroups are dealt to them round-robin
+// particles per thread and per block of the one-shot gather passes (mixed, FP64)
+constexpr int kOsU = 4;
+constexpr int kOsChunk = kThreads * kOsU;
+constexpr int kOsU64 = 2;
+constexpr int kOsChunk64 = kThreads * kOsU64;
 
 int red_blocks(int64_t N) {
     int64_t b = (N + kThreads - 1) / kThreads;
@@ -204,7 +209,7 @@ __device__ inline void rho_add(long long *rho, size_t c, long long v, int s) {
 // pass.  Only particles lo <= p < N deposit; block b owns chunk chunk0 + b of
 // the Nall-particle bunch.
 template <int MODE, bool EXACT, bool BOX, int kBoxCap>
-__global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int64_t Nall, int64_t chunk0,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_deposit_lds(int64_t lo, int64_t N, int64_t Nall, int64_t chunk0,
                                                      const double *__restrict__ x,
                                                      const double *__restrict__ y,
                                                      const double *__restrict__ z, double *q,
@@ -286,49 +291,54 @@ __global__ __launch_bounds__(256) void k_deposit_lds(int64_t lo, int64_t N, int6
     // 2. the chunk's LDS box: its bounding box if that fits kBoxCap cells, else
     // the largest box about the mean (mean +- a*rms per axis) that fits, with
     // the particles outside it going to HBM directly
+    const double ob[3] = {o0, o1, o2}, hb[3] = {h0, h1, h2};
+    const int nb[3] = {n0, n1, n2};
+    // cells [lo, hi] whose corners particles in [a, b] can reach: (x-o)/h is
+    // monotonic in x; lgp is in [floor(t)-1, floor(t)] for t >= 0 and
+    // >= floor(t)-2 below the origin; corners reach lgp+1
+    auto span = [&](int d, double a, double b, int &lo, int &hi) -> bool {
+        const double tl = (a - ob[d]) / hb[d], th = (b - ob[d]) / hb[d];
+        if (!(tl > -1e9 && th < 1e9)) return false;
+        lo = (int)floor(tl) - (tl < 0.0 ? 2 : 1);
+        hi = (int)floor(th) + 1;
+        lo = lo < 0 ? 0 : lo;
+        hi = hi > nb[d] - 1 ? nb[d] - 1 : hi;
+        if (hi < lo) hi = lo;
+        return true;
+    };
     double st[kBoxStats];
+    // own: the block forms its chunk's statistics itself (no statistics supplied,
+    // or only a bounding box, NaN sums, that overflows the LDS box)
+    bool own = !BOX;
     if constexpr (BOX) {
         for (int k = 0; k < kBoxStats; ++k) st[k] = chunkbox[chunk * kBoxStats + k];
-    } else {
-        for (int d = 0; d < 3; ++d) {
-            st[d] = __builtin_inf();
-            st[3 + d] = -__builtin_inf();
-            st[6 + d] = st[9 + d] = 0.0;
+        if (!(st[6] == st[6])) {
+            int l0 = 0, h0_ = 0;
+            long long v = 1;
+            bool fin = true;
+            for (int d = 0; d < 3 && fin; ++d) {
+                fin = span(d, st[d], st[3 + d], l0, h0_);
+                v *= (h0_ - l0 + 1);
+            }
+            own = !fin || v > kBoxCap;  // the same for every thread of the block
         }
+    }
+    if (own) {
+        box_stats_init(st);
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) {
             if (p0 + k >= lo && p0 + k < N) {
                 const double r[3] = {px[k], py[k], pz[k]};
-                for (int d = 0; d < 3; ++d) {
-                    st[d] = fmin(st[d], r[d]);
-                    st[3 + d] = fmax(st[3 + d], r[d]);
-                    st[6 + d] += r[d];
-                    st[9 + d] += r[d] * r[d];
-                }
+                box_stats_add(st, r);
             }
         }
         block_chunk_stats(st, red);
     }
     if (threadIdx.x == 0) {
-        const double o[3] = {o0, o1, o2}, h[3] = {h0, h1, h2};
-        const int n[3] = {n0, n1, n2};
         // particles the statistics cover: the whole chunk (bounds pass) or its part in [lo, N)
-        const int64_t c0 = BOX ? base : (lo > base ? lo : base);
-        const int64_t c1 = BOX ? (Nall < base + kChunk ? Nall : base + kChunk) : (N < base + kChunk ? N : base + kChunk);
+        const int64_t c0 = !own ? base : (lo > base ? lo : base);
+        const int64_t c1 = !own ? (Nall < base + kChunk ? Nall : base + kChunk) : (N < base + kChunk ? N : base + kChunk);
         const double cnt = (double)(c1 - c0);
-        // cells [lo, hi] whose corners particles in [a, b] can reach: (x-o)/h is
-        // monotonic in x; lgp is in [floor(t)-1, floor(t)] for t >= 0 and
-        // >= floor(t)-2 below the origin; corners reach lgp+1
-        auto span = [&](int d, double a, double b, int &lo, int &hi) -> bool {
-            const double tl = (a - o[d]) / h[d], th = (b - o[d]) / h[d];
-            if (!(tl > -1e9 && th < 1e9)) return false;
-            lo = (int)floor(tl) - (tl < 0.0 ? 2 : 1);
-            hi = (int)floor(th) + 1;
-            lo = lo < 0 ? 0 : lo;
-            hi = hi > n[d] - 1 ? n[d] - 1 : hi;
-            if (hi < lo) hi = lo;
-            return true;
-        };
         int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         long long vol = 1;
         bool ok = true;
@@ -1040,9 +1050,27 @@ void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double
 // external fields at t + dT/2 (ParallelTTracker.cpp:630), then t += dT and the
 // reference-particle update whose trafo transformBunch applies at step end
 // (ParallelTTracker.cpp:362-367,1199-1213).
-__global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped) {
+// Order-preserving map of doubles onto unsigned 64-bit keys, so that the
+// minimum or maximum of doubles is an integer atomicMin/atomicMax (exact and
+// independent of the order of the atomics).
+__device__ inline unsigned long long ord_key(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double ord_val(unsigned long long k) {
+    const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+__global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
+                            unsigned long long *bkeys) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     *dropped = 0;  // the step's count of deposits outside the grid
+    if (bkeys)
+        for (int d = 0; d < 3; ++d) {
+            bkeys[d] = ord_key(__builtin_inf());       // bound from above of the minimum of beam-frame coordinate d
+            bkeys[3 + d] = ord_key(-__builtin_inf());  // bound from below of its maximum
+        }
     TrackState s = *ts;
     sc->dT = s.dT;
     sc->coupling = s.coupling;
@@ -1099,8 +1127,9 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
     s.step += 1;
     *ts = s;
 }
-void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped) {
-    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N, dropped);
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
+                      unsigned long long *bkeys) {
+    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N, dropped, bkeys);
 }
 
 // PUSH: the step's first half push (timeIntegration1), written back; otherwise
@@ -1193,6 +1222,158 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
         else OPALSC_ALPHA(false, false);
     }
 #undef OPALSC_ALPHA
+}
+
+// ---- bounds of the beam-frame positions from the previous gather pass's boxes
+// The one-shot gather pass leaves, per block of consecutive particles, the
+// statistics (min, max, sum, sum of squares per axis) of the positions it
+// wrote, which are the positions the next step's bounds pass reads.  Rotated
+// into this step's beam frame, each 2048-particle chunk's box gives an interval
+// [L_d, U_d] that holds every particle's beam-frame coordinate d, and a particle
+// on the box's face bounds the chunk's minimum from above (its maximum from
+// below).  Only chunks whose interval reaches past the best such bound over all
+// chunks can hold a global extreme; k_bounds_scan rotates their particles
+// exactly as the full bounds pass does, so the bounds (hence the mesh) are the
+// same bit for bit while a few percent of the positions are read.  The
+// intervals are widened by 16 ulps of sum_j |M_dj| max|r_j - o_j|, more than the
+// rounding of either the interval or a particle's rotation can move them.
+constexpr double kBoxMarginUlps = 16.0 * 0x1p-53;
+
+// OSPC: one-shot gather blocks per 2048-particle chunk (2 mixed, 4 FP64).  Also
+// writes each chunk's widened beam-frame interval as the deposit's bounding
+// box, with NaN sums: a deposit block whose box overflows its LDS box forms
+// the chunk's statistics itself.
+template <int OSPC>
+__global__ __launch_bounds__(256) void k_bounds_from_boxes(int64_t N, const double *__restrict__ osbox,
+                                                           const Scalars *sc, double *chunkbox,
+                                                           unsigned long long *bkeys) {
+    __shared__ double lds[4 * 6];
+    constexpr int kOs = kChunk / OSPC;
+    const int64_t nchunks = (N + kChunk - 1) / kChunk;
+    const int64_t nos = (N + kOs - 1) / kOs;
+    double M[9], o[3];
+    for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
+    for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
+    double umin[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double lmax[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nchunks; c += (int64_t)gridDim.x * blockDim.x) {
+        double lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = __builtin_inf();
+            hi[d] = -__builtin_inf();
+        }
+        for (int q = 0; q < OSPC; ++q) {
+            const int64_t k = c * OSPC + q;
+            if (k >= nos) break;
+            const double *b = osbox + k * kOsBoxStats;
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], b[d]);
+                hi[d] = fmax(hi[d], b[3 + d]);
+            }
+        }
+        double wl[3], wh[3];
+        for (int j = 0; j < 3; ++j) {
+            wl[j] = lo[j] - o[j];
+            wh[j] = hi[j] - o[j];
+        }
+        double *cb = chunkbox + c * kBoxStats;
+        for (int d = 0; d < 3; ++d) {
+            double a[3], b[3], S = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                a[j] = M[3 * d + j] * wl[j];
+                b[j] = M[3 * d + j] * wh[j];
+                S += fabs(M[3 * d + j]) * fmax(fabs(wl[j]), fabs(wh[j]));
+            }
+            const double mg = kBoxMarginUlps * S + 1e-300;
+            double L = 0.0, U = 0.0, Lx = 0.0, Ux = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                L += fmin(a[j], b[j]);
+                U += fmax(a[j], b[j]);
+                if (j != d) {
+                    Lx += fmin(a[j], b[j]);
+                    Ux += fmax(a[j], b[j]);
+                }
+            }
+            const bool pos = M[4 * d] >= 0.0;
+            umin[d] = fmin(umin[d], (pos ? a[d] : b[d]) + Ux + mg);
+            lmax[d] = fmax(lmax[d], (pos ? b[d] : a[d]) + Lx - mg);
+            cb[d] = L - mg;
+            cb[3 + d] = U + mg;
+            cb[6 + d] = cb[9 + d] = __builtin_nan("");
+        }
+    }
+    block_minmax(umin, lmax, lds);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&bkeys[d], ord_key(umin[d]));
+            atomicMax(&bkeys[3 + d], ord_key(lmax[d]));
+        }
+}
+
+// The exact bounds over the chunks that can hold an extreme (see above), in
+// the full bounds pass's partials layout (min in slots 0-2, max in 3-5).
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_bounds_scan(int64_t N, const double *__restrict__ x,
+                                                     const double *__restrict__ y, const double *__restrict__ z,
+                                                     const Scalars *sc, const double *chunkbox,
+                                                     const unsigned long long *bkeys, double *partials) {
+    __shared__ double lds[4 * 6];
+    double M[9], o[3], umin[3], lmax[3];
+    for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
+    for (int d = 0; d < 3; ++d) {
+        o[d] = sc->o_rb[d];
+        umin[d] = ord_val(bkeys[d]);
+        lmax[d] = ord_val(bkeys[3 + d]);
+    }
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    const int64_t nchunks = (N + kChunk - 1) / kChunk;
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const double *cb = chunkbox + c * kBoxStats;
+        bool cand = false;
+        for (int d = 0; d < 3; ++d) cand |= (cb[d] <= umin[d]) || (cb[3 + d] >= lmax[d]);
+        if (!cand) continue;
+        const int64_t base = c * kChunk + threadIdx.x;
+#pragma unroll 2
+        for (int k = 0; k < kPPT; ++k) {
+            const int64_t i = base + k * kThreads;
+            if (i >= N) break;
+            double r[3] = {x[i], y[i], z[i]};
+            if constexpr (EXACT) transform_to(M, o, r);
+            else transform_to_fast(M, o, r);
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = fmin(mn[d], r[d]);
+                mx[d] = fmax(mx[d], r[d]);
+            }
+        }
+    }
+    block_minmax(mn, mx, lds);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            partials[blockIdx.x * kSlots + d] = mn[d];
+            partials[blockIdx.x * kSlots + 3 + d] = mx[d];
+        }
+}
+
+int64_t os_box_count(bool fp64, int64_t N) {
+    const int64_t k = fp64 ? kOsChunk64 : kOsChunk;
+    return (N + k - 1) / k;
+}
+void launch_bounds_from_boxes(hipStream_t s, bool fp64, const Bunch &b, const double *osbox, const Scalars *sc,
+                              unsigned long long *bkeys) {
+    if (b.N <= 0) return;
+    const int64_t nchunks = (b.N + kChunk - 1) / kChunk;
+    const int blocks = (int)std::min<int64_t>((nchunks + kThreads - 1) / kThreads, 1024);
+    static_assert(kChunk == 2 * kOsChunk && kChunk == 4 * kOsChunk64, "one-shot blocks tile the chunks");
+    if (fp64) k_bounds_from_boxes<4><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, bkeys);
+    else k_bounds_from_boxes<2><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, bkeys);
+}
+void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc,
+                        const unsigned long long *bkeys, double *partials) {
+    if (b.N <= 0) return;
+    const int blocks = alpha_blocks(b.N);  // the partial rows launch_track_mesh reduces
+    if (exact) k_bounds_scan<true><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
+    else k_bounds_scan<false><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
 }
 
 __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int nb, int n0, int n1,
@@ -1479,8 +1660,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // butterflies, then the four waves in order).  A stream of short-lived blocks
 // keeps more particle loads in flight than a persistent grid-stride loop
 // (2.30 -> 2.20 ms at 1e8 particles; 2 or 8 particles per thread: 2.27, 2.31 ms).
-constexpr int kOsU = 4;
-constexpr int kOsChunk = kThreads * kOsU;
 int os_blocks(int64_t N) {
     const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
@@ -1491,13 +1670,16 @@ template <bool NEXT, int EXT>
 __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double *x, double *y, double *z, double *px,
                                                               double *py, double *pz, const float4 *Eg,
                                                               const Scalars *sc, const TrackState *ts,
-                                                              double *dense) {
+                                                              double *dense, double *osbox) {
     __shared__ double wsum[kThreads / 64][kMom];
+    __shared__ double lbox[4 * 6];
     const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
     const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
+    double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     if (chunk < nchunks) {
         const bool sc_on = ts->space_charge != 0;
         const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
@@ -1561,7 +1743,13 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
             double pn[3];
             mat_vec_fast(Mc, pb, pn);
             accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
-            if constexpr (NEXT) push_half_fast(r[0], r[1], r[2], pn[0], pn[1], pn[2], half_cdt);
+            if constexpr (NEXT) {
+                push_half_fast(r[0], r[1], r[2], pn[0], pn[1], pn[2], half_cdt);
+                for (int d = 0; d < 3; ++d) {
+                    bmn[d] = fmin(bmn[d], r[d]);
+                    bmx[d] = fmax(bmx[d], r[d]);
+                }
+            }
             x[i] = r[0];
             y[i] = r[1];
             z[i] = r[2];
@@ -1581,13 +1769,21 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
         for (int w = 1; w < kThreads / 64; ++w) t += wsum[w][threadIdx.x];
         dense[(int64_t)blockIdx.x * kMom + threadIdx.x] = t;
     }
+    if constexpr (NEXT) {
+        if (osbox) {
+            block_minmax(bmn, bmx, lbox);
+            if (threadIdx.x == 0 && chunk < nchunks)
+                for (int d = 0; d < 3; ++d) {
+                    osbox[chunk * kOsBoxStats + d] = bmn[d];
+                    osbox[chunk * kOsBoxStats + 3 + d] = bmx[d];
+                }
+        }
+    }
 }
 
 // The FP64 gather pass (no energy bins, DC beam or field maps) in blocks that
 // run once, kOsU64 particles per thread, the per-particle sequence of
 // k_track_gamma unchanged; moment sums as in k_track_gamma_mixed_os.
-constexpr int kOsU64 = 2;
-constexpr int kOsChunk64 = kThreads * kOsU64;
 int os_blocks64(int64_t N) {
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
@@ -1597,13 +1793,16 @@ int os_blocks64(int64_t N) {
 template <bool NEXT>
 __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, double *y, double *z, double *px,
                                                         double *py, double *pz, const double4 *Eg, const Scalars *sc,
-                                                        const TrackState *ts, double *dense) {
+                                                        const TrackState *ts, double *dense, double *osbox) {
     __shared__ double wsum[kThreads / 64][kMom];
+    __shared__ double lbox[4 * 6];
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
     const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
+    double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     if (chunk < nchunks) {
         const bool sc_on = ts->space_charge != 0;
         const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
@@ -1662,7 +1861,13 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
             transform_to(sc->M_u, sc->o_u, r);
             mat_vec(sc->M_u, p, pn);
             accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
-            if constexpr (NEXT) push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
+            if constexpr (NEXT) {
+                push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
+                for (int d = 0; d < 3; ++d) {
+                    bmn[d] = fmin(bmn[d], r[d]);
+                    bmx[d] = fmax(bmx[d], r[d]);
+                }
+            }
             x[i] = r[0];
             y[i] = r[1];
             z[i] = r[2];
@@ -1681,6 +1886,16 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
         double t = wsum[0][threadIdx.x];
         for (int w = 1; w < kThreads / 64; ++w) t += wsum[w][threadIdx.x];
         dense[(int64_t)blockIdx.x * kMom + threadIdx.x] = t;
+    }
+    if constexpr (NEXT) {
+        if (osbox) {
+            block_minmax(bmn, bmx, lbox);
+            if (threadIdx.x == 0 && chunk < nchunks)
+                for (int d = 0; d < 3; ++d) {
+                    osbox[chunk * kOsBoxStats + d] = bmn[d];
+                    osbox[chunk * kOsBoxStats + 3 + d] = bmx[d];
+                }
+        }
     }
 }
 
@@ -1708,21 +1923,21 @@ __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64
 }
 
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                          const TrackState *ts, double *dense, double *partials, int ext) {
+                          const TrackState *ts, double *dense, double *partials, int ext, double *osbox) {
     if (fp64) {
         const int blocks64 = os_blocks64(b.N);
-        if (next) k_track_gamma_os<true><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense);
-        else k_track_gamma_os<false><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense);
+        if (next) k_track_gamma_os<true><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense, osbox);
+        else k_track_gamma_os<false><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense, nullptr);
         k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks64, partials);
         return kOsReduceBlocks;
     }
     const int blocks = os_blocks(b.N);
     if (next) {
-        if (ext) k_track_gamma_mixed_os<true, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense);
-        else k_track_gamma_mixed_os<true, 0><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense);
+        if (ext) k_track_gamma_mixed_os<true, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, osbox);
+        else k_track_gamma_mixed_os<true, 0><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, osbox);
     } else {
-        if (ext) k_track_gamma_mixed_os<false, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense);
-        else k_track_gamma_mixed_os<false, 0><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense);
+        if (ext) k_track_gamma_mixed_os<false, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, nullptr);
+        else k_track_gamma_mixed_os<false, 0><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, nullptr);
     }
     k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks, partials);
     return kOsReduceBlocks;

@@ -71,6 +71,23 @@ __device__ inline void block_minmax(double (&mn)[3], double (&mx)[3], double *ld
     __syncthreads();
 }
 
+// Per-thread chunk statistics of positions: min[3], max[3], sum[3], sum of squares[3].
+__device__ inline void box_stats_init(double (&st)[kBoxStats]) {
+    for (int d = 0; d < 3; ++d) {
+        st[d] = __builtin_inf();
+        st[3 + d] = -__builtin_inf();
+        st[6 + d] = st[9 + d] = 0.0;
+    }
+}
+__device__ inline void box_stats_add(double (&st)[kBoxStats], const double r[3]) {
+    for (int d = 0; d < 3; ++d) {
+        st[d] = fmin(st[d], r[d]);
+        st[3 + d] = fmax(st[3 + d], r[d]);
+        st[6 + d] += r[d];
+        st[9 + d] += r[d] * r[d];
+    }
+}
+
 // Block min/max and sums of positions and squared positions (the chunk
 // statistics the deposit sizes its LDS box from); result valid in thread 0.
 __device__ inline void block_chunk_stats(double (&st)[kBoxStats], double *lds) {

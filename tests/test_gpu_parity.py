@@ -217,6 +217,39 @@ def test_tracker_one_call_equals_single_steps(precision):
     assert pa.t == pb.t and pa.pathLength == pb.pathLength
 
 
+@pytest.mark.parametrize("precision", [sc.FP64, sc.MIXED])
+def test_tracker_tilted_bunch_one_call_equals_single_steps(precision):
+    """A bunch whose mean momentum points well away from z (beam frame rotated
+    by ~50 degrees) and a halo: inside one call the bounds of steps 2.. come
+    from the previous gather pass's chunk statistics and an exact scan of the
+    chunks that can hold an extreme; single-step calls run the full bounds
+    pass.  Meshes, particles and statistics agree bit for bit."""
+    n, N, steps = (24, 32, 16), 300000, 6
+    R, P, Q = gaussian_bunch(N, seed=33, q_total=3e-9, halo_frac=0.1, sigma_r=(1e-3, 2e-3, 0.5e-3))
+    P += np.array([0.35, -0.2, 0.3])
+    runs = []
+    for split in (False, True):
+        with ctx(n, precision=precision, N=N, sort_interval=3) as c:
+            c.set_bunch(R, P, Q)
+            c.track_init(1e-11, ref_p=(0.35, -0.2, 0.346181), Bz=0.2)
+            st, meshes = [], []
+            if split:
+                for _ in range(steps):
+                    st += c.track(1)
+                    meshes.append(c.mesh())
+            else:
+                st = c.track(steps)
+                meshes.append(c.mesh())
+            runs.append((c.get_bunch(), st, meshes[-1]))
+    (Ra, Pa, Qa), sa, ma = runs[0]
+    (Rb, Pb, Qb), sb, mb = runs[1]
+    assert np.array_equal(ma[0], mb[0]) and np.array_equal(ma[1], mb[1])
+    assert np.array_equal(Ra, Rb) and np.array_equal(Pa, Pb)
+    for a, b in zip(sa, sb):
+        for k in ("rmean", "pmean", "rrms", "prms", "eps_norm"):
+            assert np.array_equal(a[k], b[k]), k
+
+
 def test_tracker_without_space_charge_matches_oracle():
     n, N, steps = (16, 16, 16), 50000, 4
     R, P, Q = gaussian_bunch(N, seed=5, q_total=1e-9)
