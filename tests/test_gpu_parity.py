@@ -65,6 +65,37 @@ def test_sort_by_cell_is_stable_and_bit_exact():
     assert np.array_equal(perm.astype(np.int64), np.argsort(key, kind="stable"))
 
 
+def _morton_keys(lgp, n):
+    m = 1
+    while (1 << m) < max(n):
+        m += 1
+    key = np.zeros(len(lgp), np.int64)
+    for b in range(m):
+        for d in range(3):
+            key |= ((lgp[:, d].astype(np.int64) >> b) & 1) << (3 * b + d)
+    inside = np.all((lgp >= 0) & (lgp < np.asarray(n)), axis=1)
+    key[~inside] = 1 << (3 * m)  # outside the grid: after every cell
+    return key
+
+
+@pytest.mark.parametrize("n,N", [((256, 256, 256), 1_000_003), ((48, 24, 96), 300_001), ((8, 8, 8), 5000)])
+def test_sort_by_cell_large_keys_ragged_and_ties(n, N):
+    """The radix sort over the key widths the grids need (25 bits for 256^3:
+    three digit passes; 22 bits for 96 cells: three; 10 bits for 8^3: two
+    passes), a particle count that is no multiple of the 2048-key blocks,
+    a dense clump (thousands of equal keys) and particles outside the mesh:
+    the permutation is numpy's stable argsort of the Morton keys."""
+    rng = np.random.default_rng(N)
+    R = rng.normal(size=(N, 3)) * 1e-3
+    R[: N // 4] *= 0.02  # a clump: long runs of equal keys
+    hr, org = O.boundp(R[: N // 2], n)  # the mesh of half the bunch: the rest may fall outside
+    lgp, _, _ = O.cic_index(R, org, hr)
+    key = _morton_keys(lgp, n)
+    with ctx(n, N=N) as c:
+        perm = c.sort_by_cell(R, org, hr)
+    assert np.array_equal(perm.astype(np.int64), np.argsort(key, kind="stable"))
+
+
 @pytest.mark.parametrize("greens", [0, 1])
 @pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12), (8, 48, 12), (64, 32, 96)])
 def test_compute_potential_fp64(n, greens):
