@@ -32,6 +32,7 @@ constexpr int kRsMaxBits = 9;
 constexpr int kRsBuckets = 1 << kRsMaxBits;
 constexpr int kScanIPT = 8;
 constexpr int kScanSeg = kRsThreads * kScanIPT;  // entries per block of the prefix sum
+static_assert(kRsIPT == 8, "the histogram loads eight keys per thread as two 16-byte words");
 
 int64_t rs_blocks(int64_t N) { return (N + kRsTile - 1) / kRsTile; }
 
@@ -64,16 +65,34 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t *__restri
     const uint32_t mask = (uint32_t)nb - 1u;
     for (int d = threadIdx.x; d < nb; d += kRsThreads) h[d] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kRsTile + threadIdx.x;
+    // thread t counts the kRsIPT consecutive keys from base: a run of equal
+    // digits (the common case in the high digits of a bunch sorted a few steps
+    // ago) is one LDS atomic instead of one per key
+    const int64_t base = (int64_t)blockIdx.x * kRsTile + (int64_t)threadIdx.x * kRsIPT;
     uint32_t kk[kRsIPT];
+    if (base + kRsIPT <= N) {
+        const uint4 a = ((const uint4 *)(keys + base))[0], b = ((const uint4 *)(keys + base))[1];
+        kk[0] = a.x; kk[1] = a.y; kk[2] = a.z; kk[3] = a.w;
+        kk[4] = b.x; kk[5] = b.y; kk[6] = b.z; kk[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRsIPT; ++k) kk[k] = base + k < N ? keys[base + k] : 0u;
+    }
+    const int cnt = base >= N ? 0 : (N - base < kRsIPT ? (int)(N - base) : kRsIPT);
+    uint32_t run_d = 0, run_n = 0;
 #pragma unroll
     for (int k = 0; k < kRsIPT; ++k) {
-        const int64_t i = base + k * kRsThreads;
-        kk[k] = i < N ? keys[i] : 0u;
+        if (k < cnt) {
+            const uint32_t d = (kk[k] >> shift) & mask;
+            if (run_n && d != run_d) {
+                atomicAdd(&h[run_d], run_n);
+                run_n = 0;
+            }
+            run_d = d;
+            ++run_n;
+        }
     }
-#pragma unroll
-    for (int k = 0; k < kRsIPT; ++k)
-        if (base + k * kRsThreads < N) atomicAdd(&h[(kk[k] >> shift) & mask], 1u);
+    if (run_n) atomicAdd(&h[run_d], run_n);
     __syncthreads();
     for (int d = threadIdx.x; d < nb; d += kRsThreads) hist[(int64_t)d * nblocks + blockIdx.x] = h[d];
 }
@@ -234,7 +253,8 @@ size_t sort_temp_bytes(int64_t N) {
     if (N <= 0) return 0;
     const int64_t L = (int64_t)kRsBuckets * rs_blocks(N);
     const int64_t nseg = (L + kScanSeg - 1) / kScanSeg;
-    return (size_t)N * 8 + (size_t)L * 4 + (size_t)(nseg + 1) * 4 + 256;
+    const size_t n4 = (size_t)(N + 3) / 4 * 4;  // whole 16-byte words per key/value buffer
+    return n4 * 8 + (size_t)L * 4 + (size_t)(nseg + 1) * 4 + 256;
 }
 
 int sort_pairs(hipStream_t s, void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout,
@@ -250,8 +270,9 @@ int sort_pairs(hipStream_t s, void *tmp, size_t tmp_bytes, const uint32_t *kin, 
     const int64_t nblocks = rs_blocks(N);
     const int passes = (end_bit + kRsMaxBits - 1) / kRsMaxBits;
     const int width = (end_bit + passes - 1) / passes;
-    uint32_t *tk = (uint32_t *)tmp, *tv = tk + N;
-    uint32_t *hist = tv + N;
+    const int64_t n4 = (N + 3) / 4 * 4;  // keeps every buffer 16-byte aligned
+    uint32_t *tk = (uint32_t *)tmp, *tv = tk + n4;
+    uint32_t *hist = tv + n4;
     // (input, output) of each pass: A -> B, B -> T, T -> B, ... for an odd
     // number of passes, A -> T, T -> B, B -> T, ... for an even one
     const uint32_t *sk = kin, *sv = vin;
