@@ -122,9 +122,15 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
     const int LR = L / R, Ns = 1 << ls, W = 1 << lw;
     const int twstep = L / (Ns * R);
     C v[NB][R];
+    // the lane's element index, opaque to loop-invariant code motion: hoisted
+    // out of the stage loop, the E load and store addresses stay live across
+    // every stage (and both transforms of the z pass) and spill at E = 16;
+    // recomputed here they cost a few integer operations per stage
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int g = threadIdx.x + b * kFftThreads;
+        const int g = t + b * kFftThreads;
         const int line = g & (W - 1), j = g >> lw;
         const int k = j & (Ns - 1);
 #pragma unroll
@@ -140,7 +146,7 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int g = threadIdx.x + b * kFftThreads;
+        const int g = t + b * kFftThreads;
         const int line = g & (W - 1), j = g >> lw;
         const int k = j & (Ns - 1);
         const int idxD = (((j >> ls) * R) << ls) + k;

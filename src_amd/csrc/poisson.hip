@@ -406,20 +406,26 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         // of a thread issued together
         fft_lds<C, E>(buf, tw, pl, false);
         const int fy = o <= n1 ? o : L1 - o;
-        T g[E];
+        // in halves of E/2 loads in flight: the whole E would hold E more
+        // registers across the multiply and spill at E = 16
+        constexpr int H = E >= 8 ? E / 2 : E;
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = threadIdx.x + q * kFftThreads;
-            const int kz = e >> lw;
-            const int fz = kz <= n2 ? kz : L - kz;
-            g[q] = Gh[((size_t)fz * (n1 + 1) + fy) * gp + kx0 + (e & (W - 1))];
-        }
+        for (int h = 0; h < E; h += H) {
+            T g[H];
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = threadIdx.x + q * kFftThreads;
-            const int idx = (e >> lw) * P + (e & (W - 1));
-            const C v = buf[idx];
-            buf[idx] = C{v.x * g[q], v.y * g[q]};
+            for (int q = 0; q < H; ++q) {
+                const int e = threadIdx.x + (h + q) * kFftThreads;
+                const int kz = e >> lw;
+                const int fz = kz <= n2 ? kz : L - kz;
+                g[q] = Gh[((size_t)fz * (n1 + 1) + fy) * gp + kx0 + (e & (W - 1))];
+            }
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                const int e = threadIdx.x + (h + q) * kFftThreads;
+                const int idx = (e >> lw) * P + (e & (W - 1));
+                const C v = buf[idx];
+                buf[idx] = C{v.x * g[q], v.y * g[q]};
+            }
         }
         __syncthreads();
         fft_lds<C, E>(buf, tw, pl, true);
