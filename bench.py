@@ -120,6 +120,7 @@ def run_c5(sc, steps, warmup, sort_interval, log):
                          cfg["Bz"], cfg["Ez0"], cfg["rf"] / 1e6, sort_interval),
             "ms_per_step": ms, "particle_steps_per_s": N * steps / t, "steps": steps,
             "poisson_solves_per_step": 2 * int((counts > 0).sum()),
+            "roofline_step": step_roofline(step_bytes(N, n, 4, solves=2 * int((counts > 0).sum())), ms),
             "bin_gammas": [float(x) for x in gammas],
             "emittance_last": [float(x) for x in st[-1]["eps_norm"]],
             "kernels": {k: {"avg_ms": ms_ / nl, "launches": nl} for k, (ms_, nl) in prof.items() if nl}}
@@ -182,11 +183,56 @@ def run_c4(sc, steps, warmup, log):
             c.close()
         out[mode + "_ms_per_step"] = 1e3 * t / steps
         out[mode + "_particle_steps_per_s"] = total * steps / t
+    c4_bytes = sum(step_bytes(Nb, nb, 4) for Nb, nb, _, _ in specs)
+    for mode in ("batched_streams", "batched_graph"):
+        out[mode + "_roofline_step"] = step_roofline(c4_bytes, out[mode + "_ms_per_step"])
     out["sequential_ms_per_step"] = 1e3 * t_seq / steps
     out["sequential_particle_steps_per_s"] = total * steps / t_seq
     log("[bench] c4: sequential %.2f ms/step, batched streams %.2f, batched graph %.2f" %
         (out["sequential_ms_per_step"], out["batched_streams_ms_per_step"], out["batched_graph_ms_per_step"]))
     return out
+
+
+def step_bytes(N, n, wbytes, solves=1):
+    """Algorithmic HBM bytes of one step of the reference algorithm: per
+    particle R and P read and written (96 B; the bunch's equal charges are one
+    number), and per Poisson solve on an n^3 grid: the int64 charge written and
+    read, phi and the 3-component E written and read, the doubled-grid
+    transforms of rho and of G (M = (2n)^3 real points, Mc = (2n)^2 (n+1)
+    complex), the real G, the spectral product, the inverse transform and the
+    (n+1)^3 corner table; wbytes is the word size of the grids (8 FP64, 4
+    mixed)."""
+    n3, M, Mc = n ** 3, (2 * n) ** 3, (2 * n) ** 2 * (n + 1)
+    grid = (n3 * (2 * 8 + 2 * wbytes + 6 * wbytes) + 2 * Mc * 2 * wbytes + 2 * M * wbytes + 3 * Mc * 2 * wbytes
+            + Mc * 2 * wbytes + (n + 1) ** 3 * 8)
+    return N * 96 + solves * grid
+
+
+def step_roofline(bytes_per_step, ms):
+    a = bytes_per_step / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+            "bytes_per_step": bytes_per_step}
+
+
+def profile_kernels(c, steps, ab):
+    """Per-kernel average durations (HIP events on the library's stream) over
+    `steps` tracker steps, with GB/s and fraction of peak where ab gives the
+    kernel's algorithmic bytes per launch."""
+    c.profile_enable(True)
+    c.track(steps)
+    prof = c.profile_read()
+    c.profile_enable(False)
+    kernels = {}
+    for name, (ms, nl) in prof.items():
+        if nl == 0:
+            continue
+        avg = ms / nl
+        k = {"avg_ms": avg, "launches": nl}
+        if name in ab:
+            k["algorithmic_GBs"] = ab[name] / (avg * 1e-3) / 1e9
+            k["frac_of_peak"] = k["algorithmic_GBs"] / HBM_PEAK_GBS
+        kernels[name] = k
+    return kernels
 
 
 def algorithmic_bytes(cfg, wbytes):
@@ -241,38 +287,81 @@ def measured_traffic(config, kernel):
     return None
 
 
-def cpu_baseline(budget_s=20.0):
-    """The oracle (single-threaded C restatement of the OPAL 2.0 step) on a
-    bounded sample: 1e6 particles of the benchmark distribution on a 64^3
-    grid (128^3 doubled convolution), glibc libm, timed per full step."""
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """The host threads this process may use: OMP_NUM_THREADS when set, else its CPU affinity."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    if v.isdigit() and int(v) > 0:
+        return int(v)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(config, log):
+    """The oracle (the C restatement of the OPAL 2.0 step: serial-order CIC,
+    mixed-radix FFT convolution on the doubled grid, IGF rebuilt every step,
+    host libm as the reference binary uses) timed on this host: one full step
+    of the benchmark workload itself (1e8 particles, 256^3 grid) on all host
+    threads (OpenMP), and the small configuration (1e5 particles, 32^3) on one
+    core.  Its bunch is drawn with numpy from the same distribution as the
+    GPU's; setup is not timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes
     import oracle_py as O
-    O.lib().orc_set_igf_libm.argtypes = [O.C.c_int]
-    O.lib().orc_set_igf_libm(0)
-    N, n = 1_000_000, 64
-    g, bg = gamma_betagamma(250.0)
-    rng = np.random.default_rng(3)
-    R = rng.normal(size=(N, 3)) * 1e-3
-    m = rng.random(N) < 0.1
-    d = rng.normal(size=(int(m.sum()), 3))
-    d /= np.linalg.norm(d, axis=1)[:, None]
-    R[m] = d * (5e-3 * rng.random(len(d)) ** (1 / 3))[:, None]
-    P = rng.normal(size=(N, 3)) * 1e-3
-    P[:, 2] += bg
-    Q = np.full(N, 10e-9 / N)
-    dt = np.full(N, 1e-11)
-    tr = O.make_tracker((n, n, n), 1e-11, ref_p=(0, 0, bg))
-    steps, t = 0, 0.0
-    while t < budget_s and steps < 5:
+    L = O.lib()
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_igf_libm.argtypes = [ctypes.c_int]
+    L.orc_set_igf_libm(0)
+    threads = host_threads()
+    model = host_cpu_model()
+    out = {"unit": "particle-steps/s", "cores": threads, "kind": "port", "cpu_model": model}
+
+    def bunch(cfg):
+        _, bg = gamma_betagamma(cfg["ekin"])
+        rng = np.random.default_rng(cfg["seed"])
+        N = cfg["N"]
+        R = rng.normal(size=(N, 3)) * 1e-3
+        if cfg["halo"] > 0:
+            m = rng.random(N) < cfg["halo"]
+            d = rng.normal(size=(int(m.sum()), 3))
+            d /= np.linalg.norm(d, axis=1)[:, None]
+            R[m] = d * (5e-3 * rng.random(len(d)) ** (1 / 3))[:, None]
+        P = rng.normal(size=(N, 3)) * 1e-3
+        P[:, 2] += bg
+        return R, P, np.full(N, cfg["q"] / N), np.full(N, 1e-11), bg
+
+    def timed(cfg, steps, nthreads):
+        R, P, Q, dt, bg = bunch(cfg)
+        n = cfg["n"]
+        tr = O.make_tracker((n, n, n), 1e-11, ref_p=(0.0, 0.0, bg), Bz=cfg["Bz"])
+        L.orc_set_threads(nthreads)
         t0 = time.perf_counter()
-        R, P, Q, dt, _ = O.track(tr, R, P, Q, dt, 1)
-        t += time.perf_counter() - t0
-        steps += 1
-    O.lib().orc_set_igf_libm(1)
-    return {"value": N * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle (CPU restatement of the OPAL 2.0 step, 1 thread, glibc libm): "
-                      "%d full steps of 1e6 particles of the benchmark distribution on a 64^3 "
-                      "grid (128^3 doubled FFT), %.1f s" % (steps, t)}
+        O.track(tr, R, P, Q, dt, steps)
+        t = time.perf_counter() - t0
+        return cfg["N"] * steps / t, t
+
+    cfg = CONFIGS[config]
+    v, t = timed(cfg, 1, threads)
+    log("[bench] cpu: %s, %d threads: one %s step %.1f s" % (model, threads, config, t))
+    out["value"] = v
+    out["sample"] = ("oracle (CPU restatement of the OPAL 2.0 step, OpenMP, host libm) on %s: one full step "
+                     "of the benchmark workload (%d particles, %d^3 grid, %d^3 doubled FFT) on %d threads, %.1f s"
+                     % (model, cfg["N"], cfg["n"], 2 * cfg["n"], threads, t))
+    v1, t1 = timed(CONFIGS["c1"], 20, 1)
+    log("[bench] cpu: c1 on one core: 20 steps %.1f s" % t1)
+    out["c1_single_core"] = {"value": v1, "unit": "particle-steps/s", "cores": 1,
+                             "sample": "20 steps of 1e5 particles on a 32^3 grid, %.1f s" % t1}
+    L.orc_set_threads(1)
+    L.orc_set_igf_libm(1)
+    return out
 
 
 def main():
@@ -320,20 +409,8 @@ def main():
     roofline = None
     if not args.no_profile:
         # profile as many steps as the sort interval so the amortised re-sort is included
-        cm.profile_enable(True)
-        cm.track(max(args.sort_interval, 3))
-        prof = cm.profile_read()
-        cm.profile_enable(False)
         ab = algorithmic_bytes(cfg, 4)
-        for name, (ms, nl) in prof.items():
-            if nl == 0:
-                continue
-            avg = ms / nl
-            k = {"avg_ms": avg, "launches": nl}
-            if name in ab:
-                k["algorithmic_GBs"] = ab[name] / (avg * 1e-3) / 1e9
-                k["frac_of_peak"] = k["algorithmic_GBs"] / HBM_PEAK_GBS
-            kernels[name] = k
+        kernels = profile_kernels(cm, max(args.sort_interval, 3), ab)
         dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
         a = kernels[dom]["algorithmic_GBs"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": a, "peak": HBM_PEAK_GBS,
@@ -368,7 +445,17 @@ def main():
         extra = {"fp64_ms_per_step": ms64, "fp64_particle_steps_per_s": N * args.steps / t64,
                  "mixed_over_fp64_speedup": ms64 / ms_mixed, "efield_rel_l2_mixed_vs_fp64": e_rel,
                  "emittance_max_rel_err_mixed_vs_fp64": emit_err,
-                 "emittance_steps_compared": len(st64) + 1}
+                 "emittance_steps_compared": len(st64) + 1,
+                 "fp64_roofline_step": step_roofline(step_bytes(N, n, 8), ms64)}
+        if not args.no_profile:
+            k64 = profile_kernels(cd, max(args.sort_interval, 3), algorithmic_bytes(cfg, 8))
+            extra["fp64_kernels"] = k64
+            g = k64.get("gather_kick_push")
+            if g:
+                extra["fp64_roofline"] = {"bound": "hbm", "kernel": "gather_kick_push",
+                                          "achieved": g["algorithmic_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                          "frac": g["frac_of_peak"], "avg_ms": g["avg_ms"],
+                                          "bytes_per_launch": algorithmic_bytes(cfg, 8)["gather_kick_push"]}
         log("[bench] fp64: %.3f ms/step; speedup %.2fx; E rel L2 %.2e; emittance %.2e" %
             (ms64, ms64 / ms_mixed, e_rel, emit_err))
         del s64, s32
@@ -397,8 +484,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle sample)")
-        cpu = cpu_baseline()
+        log("[bench] CPU baseline (the oracle on the host cores)")
+        cpu = cpu_baseline(args.config, log)
         log("[bench] cpu: %.3e particle-steps/s" % cpu["value"])
 
     out = {
@@ -413,6 +500,7 @@ def main():
                                 cfg["halo"], args.sort_interval),
                    "particles": N, "grid": [n, n, n], "precision": "mixed"},
         "roofline": roofline,
+        "roofline_step": step_roofline(step_bytes(N, n, 4), ms_mixed),
         "cpu_baseline": cpu,
         "kernels": kernels,
     }

@@ -1,7 +1,7 @@
 /*
  * opal_oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
- * A single-threaded FP64 CPU restatement of OPAL 2.0's space-charge PIC step
+ * An FP64 CPU restatement of OPAL 2.0's space-charge PIC step
  * (cloud-in-cell scatter, Hockney open-boundary FFT Poisson solve with the
  * integrated Green's function, CIC gather, Boris kick/push, bunch moments,
  * beam-frame rotation and reference-frame update).  It is the checker that the
@@ -12,6 +12,14 @@
  * cites (file:line relative to the OPAL source tree), because bit-exact
  * agreement of cell indices and interpolation weights depends on that order.
  * Build with -ffp-contract=off (see oracle/Makefile): no FMA contraction.
+ *
+ * Threads: sequential by default (what the tests use).  orc_set_threads(n)
+ * runs the independent loops (FFT lines, Green's function corners, CIC gather,
+ * Grad, push, kick, coordinate transformations) on n OpenMP threads with
+ * results identical to the sequential ones; the scatter and the moment sums,
+ * which accumulate, then split the particles into n contiguous ranges added in
+ * range order, so they round differently (deterministically) for n > 1.  This
+ * serves the CPU baseline timing on all host cores.
  *
  * Layouts (same as the reference): particle vectors are AoS double[N][3];
  * fields are x-fastest, interior cells only (no guard layer); the vector
@@ -25,6 +33,7 @@
  */
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <quadmath.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -37,6 +46,12 @@
 typedef double complex cplx;
 
 int orc_version(void) { return 1; }
+
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int orc_get_threads(void) { return g_threads; }
+/* an independent loop on g_threads threads (sequential when g_threads is 1) */
+#define ORC_PAR _Pragma("omp parallel for num_threads(g_threads) if(g_threads > 1) schedule(static)")
 
 /* ------------------------------------------------------------------------ */
 /* Bounds and mesh                                                          */
@@ -146,6 +161,7 @@ static int cic_dim(double x, double o, double h, int *lgp, double *dpos) {
  * and normalised offset dpos; valid[i] = 0 for dropped particles. */
 void orc_cic_index(int64_t N, const double *R, const double origin[3], const double hr[3],
                    int32_t *lgp, double *dpos, uint8_t *valid) {
+    ORC_PAR
     for (int64_t p = 0; p < N; ++p) {
         int ok = 1;
         for (int d = 0; d < 3; ++d) {
@@ -170,10 +186,10 @@ static inline int in_grid(int i, int j, int k, const int n[3]) {
  * accumGuardCells has no neighbour to add guard cells into
  * (ippl/src/Field/BareField.hpp:764+).  Returns the number of dropped
  * (particle, corner) deposits. */
-int64_t orc_scatter(int64_t N, const double *R, const double *pdata, const double origin[3],
-                    const double hr[3], const int n[3], double *rho) {
+static int64_t scatter_range(int64_t p0, int64_t p1, const double *R, const double *pdata,
+                             const double origin[3], const double hr[3], const int n[3], double *rho) {
     int64_t dropped = 0;
-    for (int64_t p = 0; p < N; ++p) {
+    for (int64_t p = p0; p < p1; ++p) {
         int l[3];
         double dp[3];
         int ok = 1;
@@ -203,6 +219,25 @@ int64_t orc_scatter(int64_t N, const double *R, const double *pdata, const doubl
     return dropped;
 }
 
+int64_t orc_scatter(int64_t N, const double *R, const double *pdata, const double origin[3],
+                    const double hr[3], const int n[3], double *rho) {
+    const int T = g_threads;
+    if (T <= 1 || N < 4096) return scatter_range(0, N, R, pdata, origin, hr, n, rho);
+    /* thread t deposits its contiguous range into a grid of its own; the grids
+     * are then added to rho in thread order */
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    double *priv = (double *)calloc((size_t)T * ncell, sizeof(double));
+    int64_t dropped = 0;
+#pragma omp parallel for num_threads(T) schedule(static, 1) reduction(+ : dropped)
+    for (int t = 0; t < T; ++t)
+        dropped += scatter_range(N * t / T, N * (t + 1) / T, R, pdata, origin, hr, n, priv + (size_t)t * ncell);
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (size_t c = 0; c < ncell; ++c)
+        for (int t = 0; t < T; ++t) rho[c] += priv[(size_t)t * ncell + c];
+    free(priv);
+    return dropped;
+}
+
 /* IntCICImpl<3>::gather, ippl/src/Particle/IntCIC.h:933-977 for a Vektor field:
  * Ef = sum over the 8 corners, in order 000,100,...,111, of ((a*b)*c)*E.
  * Corners outside the interior read the guard value: 0 for ZeroFace; with
@@ -211,6 +246,7 @@ int64_t orc_scatter(int64_t N, const double *R, const double *pdata, const doubl
  * the opposite cells: k = n2 reads k = 0, k = -1 reads k = n2-1. */
 static void gather_impl(int64_t N, const double *R, const double *E, const double origin[3],
                         const double hr[3], const int n[3], int pz, double *Ef) {
+    ORC_PAR
     for (int64_t p = 0; p < N; ++p) {
         int l[3];
         double dp[3];
@@ -310,8 +346,6 @@ void orc_fft1d(cplx *line, int n, int sign) {
 void orc_fft3d(cplx *a, const int m[3], int sign) {
     int maxm = m[0] > m[1] ? m[0] : m[1];
     if (m[2] > maxm) maxm = m[2];
-    cplx *line = (cplx *)malloc(sizeof(cplx) * maxm);
-    cplx *in = (cplx *)malloc(sizeof(cplx) * maxm);
     cplx *tw = (cplx *)malloc(sizeof(cplx) * maxm);
     const size_t stride[3] = {1, (size_t)m[0], (size_t)m[0] * m[1]};
     for (int ax = 0; ax < 3; ++ax) {
@@ -321,16 +355,23 @@ void orc_fft3d(cplx *a, const int m[3], int sign) {
             tw[k] = cos(ang) + sign * sin(ang) * I;
         }
         int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
-        for (int u = 0; u < m[o2]; ++u)
-            for (int v = 0; v < m[o1]; ++v) {
+        const int64_t nlines = (int64_t)m[o2] * m[o1];
+#pragma omp parallel num_threads(g_threads) if (g_threads > 1)
+        {
+            cplx *line = (cplx *)malloc(sizeof(cplx) * maxm);
+            cplx *in = (cplx *)malloc(sizeof(cplx) * maxm);
+#pragma omp for schedule(static)
+            for (int64_t uv = 0; uv < nlines; ++uv) {
+                const int u = (int)(uv / m[o1]), v = (int)(uv % m[o1]);
                 size_t base = (size_t)u * stride[o2] + (size_t)v * stride[o1];
                 for (int k = 0; k < L; ++k) in[k] = a[base + (size_t)k * stride[ax]];
                 fft_rec(in, 1, line, L, tw, 1);
                 for (int k = 0; k < L; ++k) a[base + (size_t)k * stride[ax]] = line[k];
             }
+            free(line);
+            free(in);
+        }
     }
-    free(line);
-    free(in);
     free(tw);
 }
 
@@ -386,12 +427,14 @@ void orc_igf_octant(const int n[3], const double hr[3], double *G, int apply_qui
     const int t0 = n[0] + 2, t1 = n[1] + 2, t2 = n[2] + 2;
     double cellVolume = hr[0] * hr[1] * hr[2];
     double *T = (double *)malloc(sizeof(double) * (size_t)t0 * t1 * t2);
+    ORC_PAR
     for (int k = 0; k < t2; ++k)
         for (int j = 0; j < t1; ++j)
             for (int i = 0; i < t0; ++i)
                 T[((size_t)k * t1 + j) * t0 + i] = igf_corner(i, j, k, hr, cellVolume);
 #define TT(a, b, c) T[((size_t)(c) * t1 + (b)) * t0 + (a)]
     const int g0 = n[0] + 1, g1 = n[1] + 1, g2 = n[2] + 1;
+    ORC_PAR
     for (int k = 0; k < g2; ++k)
         for (int j = 0; j < g1; ++j)
             for (int i = 0; i < g0; ++i) {
@@ -430,6 +473,7 @@ void orc_green_standard_octant(const int n[3], const double hr[3], double *G) {
 void orc_mirror_green(const int n[3], const double *Goct, double *G2) {
     const int m0 = 2 * n[0], m1 = 2 * n[1], m2 = 2 * n[2];
     const int g0 = n[0] + 1, g1 = n[1] + 1;
+    ORC_PAR
     for (int z = 0; z < m2; ++z) {
         int fz = z <= n[2] ? z : m2 - z;
         for (int y = 0; y < m1; ++y) {
@@ -456,16 +500,20 @@ void orc_convolve_doubled(const int n[3], double *rho_inout, const double *G2) {
     const size_t M = (size_t)m[0] * m[1] * m[2];
     cplx *a = (cplx *)calloc(M, sizeof(cplx));
     cplx *g = (cplx *)malloc(M * sizeof(cplx));
+    ORC_PAR
     for (int k = 0; k < n[2]; ++k)
         for (int j = 0; j < n[1]; ++j)
             for (int i = 0; i < n[0]; ++i)
                 a[((size_t)k * m[1] + j) * m[0] + i] = rho_inout[((size_t)k * n[1] + j) * n[0] + i];
+    ORC_PAR
     for (size_t q = 0; q < M; ++q) g[q] = G2[q];
     orc_fft3d(a, m, -1);
     orc_fft3d(g, m, -1);
+    ORC_PAR
     for (size_t q = 0; q < M; ++q) a[q] *= g[q];
     orc_fft3d(a, m, +1);
     double inv = 1.0 / (double)M;
+    ORC_PAR
     for (int k = 0; k < n[2]; ++k)
         for (int j = 0; j < n[1]; ++j)
             for (int i = 0; i < n[0]; ++i)
@@ -656,6 +704,7 @@ void orc_grad(const int n[3], const double *x, const double hr[3], double *r) {
     const double id0 = 0.5 / hr[0], id1 = 0.5 / hr[1], id2 = 0.5 / hr[2];
 #define XV(i, j, k) (((i) < 0 || (j) < 0 || (k) < 0 || (i) >= n0 || (j) >= n1 || (k) >= n2) ? 0.0 : x[((size_t)(k) * n1 + (j)) * n0 + (i)])
 #define RR(i, j, k, d) r[3 * (((size_t)(k) * n1 + (j)) * n0 + (i)) + (d)]
+    ORC_PAR
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n0; ++i) {
@@ -913,15 +962,18 @@ int64_t orc_compute_self_fields_binned(int64_t N, const double *R, const double 
  * BorisPusher::push (src/Steppers/BorisPusher.h:116-124),
  * switchOffUnitlessPositions(true) (PartBunchBase.hpp:493-513). */
 void orc_push(int64_t N, double *R, const double *P, const double *dt) {
+    ORC_PAR
     for (int64_t i = 0; i < N; ++i) {
         double s = ORC_C * dt[i];
         for (int d = 0; d < 3; ++d) R[3 * i + d] /= s;
     }
+    ORC_PAR
     for (int64_t i = 0; i < N; ++i) {
         const double *p = P + 3 * i;
         double g = sqrt(1.0 + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
         for (int d = 0; d < 3; ++d) R[3 * i + d] += 0.5 * p[d] / g;
     }
+    ORC_PAR
     for (int64_t i = 0; i < N; ++i) {
         double s = ORC_C * dt[i];
         for (int d = 0; d < 3; ++d) R[3 * i + d] *= s;
@@ -949,6 +1001,7 @@ static void boris_kick1(double P[3], const double Ef[3], const double Bf[3], dou
 /* ParallelTTracker::kickParticles, src/Algorithms/ParallelTTracker.h:442-446. */
 void orc_kick(int64_t N, double *P, const double *Ef, const double *Bf, const double *dt,
               double charge, double mass) {
+    ORC_PAR
     for (int64_t i = 0; i < N; ++i) boris_kick1(P + 3 * i, Ef + 3 * i, Bf + 3 * i, dt[i], mass, charge);
 }
 
@@ -966,9 +1019,9 @@ typedef struct {
 /* PartBunchBase::calcMoments, src/Classic/Algorithms/PartBunchBase.hpp:2185-2258
  * (sequential accumulation; out[0..5] first moments, out[6..26] lower triangle
  * row by row). */
-void orc_calc_moments(int64_t N, const double *R, const double *P, double out[27]) {
+static void moments_range(int64_t k0, int64_t k1, const double *R, const double *P, double out[27]) {
     for (int q = 0; q < 27; ++q) out[q] = 0.0;
-    for (int64_t k = 0; k < N; ++k) {
+    for (int64_t k = k0; k < k1; ++k) {
         double part[6] = {R[3 * k], P[3 * k], R[3 * k + 1], P[3 * k + 1], R[3 * k + 2], P[3 * k + 2]};
         int l = 6;
         for (int i = 0; i < 6; ++i) {
@@ -976,6 +1029,22 @@ void orc_calc_moments(int64_t N, const double *R, const double *P, double out[27
             for (int j = 0; j <= i; ++j) out[l++] += part[i] * part[j];
         }
     }
+}
+
+void orc_calc_moments(int64_t N, const double *R, const double *P, double out[27]) {
+    const int T = g_threads;
+    if (T <= 1 || N < 4096) {
+        moments_range(0, N, R, P, out);
+        return;
+    }
+    double *part = (double *)malloc(sizeof(double) * 27 * T);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) moments_range(N * t / T, N * (t + 1) / T, R, P, part + 27 * t);
+    for (int q = 0; q < 27; ++q) {
+        out[q] = 0.0;
+        for (int t = 0; t < T; ++t) out[q] += part[27 * t + q];
+    }
+    free(part);
 }
 
 static double mom(const double m[27], int i, int j) {
@@ -1159,6 +1228,7 @@ static void mat_vec(const double M[9], const double v[3], double out[3]) {
 }
 
 void orc_transform_to(const orc_trafo *t, int64_t N, double *R) {
+    ORC_PAR
     for (int64_t i = 0; i < N; ++i) {
         double v[3] = {R[3 * i] - t->origin[0], R[3 * i + 1] - t->origin[1], R[3 * i + 2] - t->origin[2]};
         mat_vec(t->M, v, R + 3 * i);
@@ -1166,6 +1236,7 @@ void orc_transform_to(const orc_trafo *t, int64_t N, double *R) {
 }
 
 void orc_rotate_to(const orc_trafo *t, int64_t N, double *V) {
+    ORC_PAR
     for (int64_t i = 0; i < N; ++i) {
         double v[3] = {V[3 * i], V[3 * i + 1], V[3 * i + 2]};
         mat_vec(t->M, v, V + 3 * i);
