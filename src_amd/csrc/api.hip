@@ -75,6 +75,7 @@ struct opalsc_ctx {
     DevBuf binsorted;         // int32 bin of each resident particle (bin order)
     DevBuf binpart;           // calcGammas partial sums
     DevBuf scb;               // Scalars per bin
+    DevBuf scbc;              // Scalars of the non-empty bins, in bin order (the batched solve's)
     double qmax = 0.0;        // max |Q| of the resident bunch
     bool q_uniform = false;   // every particle of the resident bunch carries the charge q_value:
     double q_value = 0.0;     // the tracker then keeps Q as that one number (no per-particle array)
@@ -266,7 +267,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     c->grid.destroy();
     free_bunch(c->bunch);
     DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb,      &c->statsb,  &c->tmpd,
-                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->fm_desc, &c->fm_data,
+                    &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->scbc, &c->fm_desc, &c->fm_data,
                     &c->os_dense, &c->osbox};
     for (DevBuf *b : bs) b->release();
     void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
@@ -711,7 +712,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
     if (int e = check_n(c, N)) return e;
     if (N == 0) return OPALSC_OK;
     if (c->grid.ensure_sums(c->err)) return OPALSC_ENOMEM;
-    if (image && c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
+    if (!c->scbc.ensure(sizeof(Scalars) * kMaxBins)) FAIL(c, OPALSC_ENOMEM, "bin scalars");
     Bunch &b = c->bunch;
     const bool exact = c->grid.fp64;
     if (int e = upload(c, c->aosR, R, (size_t)N * 24)) return e;
@@ -759,22 +760,28 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
         for (int k = 0; k < nbins; ++k)
             launch_bin_gamma(c->stream, (int64_t)cnt[k], b.vals2 + start[k], c->aosP.as<double>(), c->partials, k,
                              c->gamb.as<double>());
-    c->grid.clear_sums();
-    for (int k = 0; k < nbins; ++k) {
-        if (cnt[k] == 0) continue;  // an empty bin has no field
-        const int64_t s0 = start[k], nk = (int64_t)cnt[k];
-        launch_bin_scalars(c->stream, c->d_sc, c->gamb.as<double>(), k, nk, &c->d_counters[0], exact);
-        c->grid.zero_rho(c->stream);
-        launch_deposit(c->stream, 0, exact, nk, b.px + s0, b.py + s0, b.pz + s0, nullptr, b.qd + s0,
-                       (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], nullptr);
-        c->grid.green(c->d_sc, exact);
-        c->grid.solve(c->d_sc, 0, 1);
-        c->grid.grad_accum(c->d_sc, -1.0);
-        if (image) {
-            c->grid.green_image(c->d_sc, exact);
-            c->grid.solve(c->d_sc, 0, 1, true);
-            c->grid.grad_accum(c->d_sc, +1.0);
+    // the non-empty bins (an empty bin has no field) solved as one batch, each
+    // with the mesh scalars and its own Lorentz frame, charge scale and shift
+    int nz = 0;
+    for (int k = 0; k < nbins; ++k) nz += cnt[k] > 0 ? 1 : 0;
+    if (nz == 0) {
+        c->grid.clear_sums();
+    } else {
+        if (c->grid.ensure_bins(nz, image != 0, c->err)) return OPALSC_ENOMEM;
+        Scalars *scz = c->scbc.as<Scalars>();
+        c->grid.zero_bin_rho(nz, c->stream);
+        for (int k = 0, z = 0; k < nbins; ++k) {
+            if (cnt[k] == 0) continue;
+            const int64_t s0 = start[k], nk = (int64_t)cnt[k];
+            HIPCHK(c, hipMemcpyAsync(scz + z, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToDevice, c->stream));
+            launch_bin_scalars(c->stream, scz + z, c->gamb.as<double>(), k, nk, &c->d_counters[0], exact);
+            launch_deposit(c->stream, 0, exact, nk, b.px + s0, b.py + s0, b.pz + s0, nullptr, b.qd + s0,
+                           c->grid.bin_rho(z), scz + z, &c->d_counters[1], nullptr);
+            ++z;
         }
+        c->grid.green_bins(scz, nz, exact, image != 0, c->stream);
+        c->grid.solve_bins(scz, nz, image != 0, nullptr);
+        c->grid.field_sums(scz, nz, image != 0);
         CHECK_LAUNCH(c);
     }
     launch_gather_binned(c->stream, c->grid.fp64, N, b.x, b.y, b.z, c->grid.Esum, c->grid.Bsum, c->d_sc,
@@ -1124,10 +1131,17 @@ extern "C" int opalsc_bunch_get_bins(opalsc_ctx *c, int *nbins, int64_t *counts,
 }
 
 // Buffers the space-charge step of c needs beyond those of create.
+static int nonempty_bins(const BinRanges &r) {
+    int z = 0;
+    for (int k = 0; k < r.nbins; ++k) z += r.start[k + 1] > r.start[k] ? 1 : 0;
+    return z;
+}
+
 static int prepare_track(opalsc_ctx *c) {
     if (c->bins.nbins > 0 && c->track_sc) {
         if (c->grid.ensure_sums(c->err)) return OPALSC_ENOMEM;
-        if (c->bin_image && c->grid.ensure_image(c->err)) return OPALSC_ENOMEM;
+        if (c->grid.ensure_bins(std::max(nonempty_bins(c->bins), 1), c->bin_image, c->err)) return OPALSC_ENOMEM;
+        if (!c->scbc.ensure(sizeof(Scalars) * kMaxBins)) FAIL(c, OPALSC_ENOMEM, "bin scalars");
     }
     (void)gamma_blocks(c->grid.fp64, c->bunch.N);  // occupancy query outside any stream capture
     return OPALSC_OK;
@@ -1167,44 +1181,39 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             c->prof.mark(PROF_SORT);
         }
         if (binned) {
-            // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its image
+            // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its
+            // image; the non-empty bins are solved as one batch (one launch per pass)
             launch_track_bin_scalars(c->stream, b, c->bins, c->bin_qi, exact, c->d_sc, c->binpart.as<double>(),
                                      c->gamb.as<double>(), c->scb.as<Scalars>());
-            c->grid.clear_sums();
+            Scalars *scz = c->scbc.as<Scalars>();
+            launch_compact_bin_scalars(c->stream, c->scb.as<Scalars>(), c->bins, scz);
+            const int nz = nonempty_bins(c->bins);
             c->prof.mark(PROF_SCALAR);
-            // A bin's Green's functions need only its scalars: they run on the
-            // second stream beside the bin's deposit and x/y passes, after the
-            // previous bin's last z pass has read the shared tables.
-            const bool ov = overlap && !c->prof.enabled && c->stream2 && c->grid.ncell >= (size_t)96 * 96 * 96;
-            if (ov) (void)hipEventRecord(c->ev_mesh, c->stream);
-            for (int k = 0; k < c->bins.nbins; ++k) {
+            // the bins' Green's functions need only their scalars: they run on the
+            // second stream beside the deposits and the forward x/y passes
+            const bool ov = nz > 0 && overlap && !c->prof.enabled && c->stream2 && c->grid.ncell >= (size_t)96 * 96 * 96;
+            if (ov) {
+                (void)hipEventRecord(c->ev_mesh, c->stream);
+                (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
+                c->grid.green_bins(scz, nz, cr, c->bin_image, c->stream2);
+                (void)hipEventRecord(c->ev_green, c->stream2);
+            }
+            if (nz > 0) c->grid.zero_bin_rho(nz, c->stream);
+            c->prof.mark(PROF_ZERO);
+            for (int k = 0, z = 0; k < c->bins.nbins; ++k) {
                 const int64_t lo = c->bins.start[k], hi = c->bins.start[k + 1];
                 if (hi == lo) continue;  // an empty bin has no charge
-                const Scalars *sk = c->scb.as<Scalars>() + k;
-                if (ov) {
-                    (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
-                    c->grid.green(sk, cr, c->stream2);
-                    (void)hipEventRecord(c->ev_green, c->stream2);
-                    if (c->bin_image) {
-                        c->grid.green_image(sk, cr, c->stream2);
-                        (void)hipEventRecord(c->ev_gimg, c->stream2);
-                    }
-                }
-                c->grid.zero_rho(c->stream);
-                c->prof.mark(PROF_ZERO);
                 launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
-                                     (long long *)c->grid.rho_i64, sk, &c->d_counters[1],
-                                     sort_now ? nullptr : b.chunkbox);
-                c->prof.mark(PROF_DEPOSIT);
-                if (!ov) c->grid.green(sk, cr);
-                c->grid.solve(sk, 0, 1, false, ov ? c->ev_green : nullptr);
-                c->grid.grad_accum(sk, -1.0);
-                if (c->bin_image) {
-                    if (!ov) c->grid.green_image(sk, cr);
-                    c->grid.solve(sk, 0, 1, true, ov ? c->ev_gimg : nullptr);
-                    c->grid.grad_accum(sk, +1.0);
-                }
-                if (ov) (void)hipEventRecord(c->ev_mesh, c->stream);  // this bin's z passes are issued
+                                     c->grid.bin_rho(z), scz + z, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
+                ++z;
+            }
+            c->prof.mark(PROF_DEPOSIT);
+            if (nz > 0) {
+                if (!ov) c->grid.green_bins(scz, nz, cr, c->bin_image, c->stream);
+                c->grid.solve_bins(scz, nz, c->bin_image, ov ? c->ev_green : nullptr);
+                c->grid.field_sums(scz, nz, c->bin_image);
+            } else {
+                c->grid.clear_sums();
             }
         } else {
             // The Green's function needs only the mesh: without profiling it runs

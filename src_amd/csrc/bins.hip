@@ -199,6 +199,20 @@ void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges
     k_track_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, r, qi, exact ? 1 : 0, scb);
 }
 
+// The scalars of the non-empty bins, in bin order, for the batched solve
+// (an empty bin has no charge and no field).
+__global__ void k_compact_bin_scalars(const Scalars *scb, BinRanges r, Scalars *out) {
+    int z = 0;
+    for (int k = 0; k < r.nbins; ++k)
+        if (r.start[k + 1] > r.start[k]) {
+            if ((int)threadIdx.x == z) out[z] = scb[k];
+            ++z;
+        }
+}
+void launch_compact_bin_scalars(hipStream_t s, const Scalars *scb, const BinRanges &r, Scalars *out) {
+    if (r.nbins > 0) k_compact_bin_scalars<<<1, kMaxBins, 0, s>>>(scb, r, out);
+}
+
 // bin keys of the resident bunch for its first bin-major order: the bin of
 // particle id[i] (bins outside [0, nbins) become nbins and sort last)
 __global__ void k_bin_keys_by_id(int64_t N, const int32_t *bin, const uint32_t *id, int nbins, uint32_t *keys,

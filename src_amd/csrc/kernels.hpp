@@ -122,6 +122,24 @@ struct PoissonGrid {
     void rho_to_double(const Scalars *d_sc, double *out);
     size_t rho_words() const { return fp64 ? 2 * ncell : ncell; }
     void zero_rho(hipStream_t s);  // clears every word of rho_i64
+
+    // Energy bins solved as one batch: nb grids of this shape, each with its own
+    // fixed-point charge, spectra, potentials (the bin's and its image's) and
+    // Green's function tables; every pass is one launch for all of them.
+    int bin_cap = 0;
+    bool bin_img_cap = false;
+    void *bb_rho = nullptr, *bb_A = nullptr, *bb_B = nullptr, *bb_Bi = nullptr, *bb_phi = nullptr,
+         *bb_phii = nullptr, *bb_T = nullptr, *bb_G = nullptr, *bb_Gh = nullptr, *bb_HA = nullptr,
+         *bb_HB = nullptr, *bb_Gi = nullptr;
+    int ensure_bins(int nb, bool image, std::string &err);
+    long long *bin_rho(int b) const { return (long long *)bb_rho + (size_t)b * rho_words(); }
+    void zero_bin_rho(int nb, hipStream_t s);
+    // scb: nb Scalars, one per grid of the batch
+    void green_bins(const Scalars *scb, int nb, bool cr, bool image, hipStream_t s);
+    void solve_bins(const Scalars *scb, int nb, bool image, hipEvent_t green_ready);
+    // Esum, Bsum := the sums over the nb grids (and their images) of E and of
+    // the magnetic field it implies (written, not accumulated)
+    void field_sums(const Scalars *scb, int nb, bool image);
     void widen(int which, double *out);
     void profile_mark(int id) {
         if (prof) prof->mark(id);
@@ -268,6 +286,8 @@ void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges
                               const Scalars *sc, double *partials, double *gammas, Scalars *scb);
 void launch_bin_keys_by_id(hipStream_t s, int64_t N, const int32_t *bin, const uint32_t *id, int nbins,
                            uint32_t *keys, uint32_t *vals);
+// out[z] = scb[k] for the z-th non-empty bin k (the batched solve's scalars)
+void launch_compact_bin_scalars(hipStream_t s, const Scalars *scb, const BinRanges &r, Scalars *out);
 
 // Sorting (sort.hip)
 size_t sort_temp_bytes(int64_t N);

@@ -78,7 +78,9 @@ __device__ inline double igf_corner(int i, int j, int k, double h0, double h1, d
 // x over the (n0+2)*(n1+2) corners of a plane, y over k.
 template <bool CR>
 __global__ __launch_bounds__(256) void k_igf_corners(double *T, int n0, int n1, int n2,
-                                                     const Scalars *sc) {
+                                                     const Scalars *sc, size_t tB) {
+    sc += blockIdx.z;  // batch: one table per grid, tB doubles apart
+    T += blockIdx.z * tB;
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2];
     const double vol = h0 * h1 * h2;
     const int m0 = n0 + 2, plane = m0 * (n1 + 2);
@@ -93,7 +95,9 @@ __global__ __launch_bounds__(256) void k_igf_corners(double *T, int n0, int n1, 
 // the second table also - n2*h2, evaluated left to right as there.
 template <bool CR>
 __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, int n1, int n2,
-                                                             const Scalars *sc, int second) {
+                                                             const Scalars *sc, int second, size_t tB) {
+    sc += blockIdx.z;
+    T += blockIdx.z * tB;
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2], zshift = sc->zshift;
     const double vol = h0 * h1 * h2;
     const int m0 = n0 + 2, plane = m0 * (n1 + 2);
@@ -122,7 +126,10 @@ constexpr int kIgfDiffK = 8;
 // thread, so no other block writes G[0] afterwards): the first run must hold
 // both planes, and n2 >= 4 (PoissonGrid::init) guarantees plane 1 exists.
 static_assert(kIgfDiffK >= 2, "the run of planes 0..kIgfDiffK-1 must include plane 1 for the G[0,0,0] quirk");
-__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1, int n2, int quirk) {
+__global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, int n0, int n1, int n2, int quirk,
+                                                  size_t tB, size_t gB) {
+    T += blockIdx.z * tB;
+    G += blockIdx.z * gB;
     const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= gplane) return;
@@ -155,7 +162,9 @@ __global__ __launch_bounds__(256) void k_igf_diff(const double *T, double *G, in
 // STANDARD Green's function, SpecializedGreensFunction<3>::calculate
 // (FFTPoissonSolver.cpp:32-43): G = 1/sqrt(i^2 h0^2 + j^2 h1^2 + k^2 h2^2), G[0] := G[0,0,1].
 __global__ __launch_bounds__(256) void k_green_standard(double *G, int n0, int n1, int n2,
-                                                        const Scalars *sc) {
+                                                        const Scalars *sc, size_t gB) {
+    sc += blockIdx.z;
+    G += blockIdx.z * gB;
     const size_t total = (size_t)(n0 + 1) * (n1 + 1) * (n2 + 1);
     const double hsq0 = sc->hs[0] * sc->hs[0], hsq1 = sc->hs[1] * sc->hs[1],
                  hsq2 = sc->hs[2] * sc->hs[2];
@@ -184,9 +193,11 @@ constexpr int half_q() { return E / 2 + 1; }
 // Octant line `line` = (k, j) of G (pitch n0+1, FP64) -> row of Gh (pitch gp).
 template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(
-    const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp) {
+    const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp, size_t gB, size_t ghB) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
+    G += blockIdx.z * gB;
+    Gh += blockIdx.z * ghB;
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, P = pl.P, L = pl.L, m = n0 + 1, total = W * m;
@@ -222,9 +233,10 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 // multiple of W and the padding columns hold zeros), i in [0, npts).
 template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_strided(T *Gh, FftPlan pl, int npts, size_t S,
-                                                             size_t ostride) {
+                                                             size_t ostride, size_t ghB) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
+    Gh += blockIdx.z * ghB;
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * npts;
@@ -263,8 +275,12 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 // Gi[kz][ky][kx] (pitch gp).
 template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_green_zfft(
-    const T *Ha, const T *Hb, typename cplx_of<T>::type *Gi, FftPlan pl, int n1, int n2, int gp) {
+    const T *Ha, const T *Hb, typename cplx_of<T>::type *Gi, FftPlan pl, int n1, int n2, int gp, size_t hB,
+    size_t giB) {
     using C = typename cplx_of<T>::type;
+    Ha += blockIdx.z * hB;
+    Hb += blockIdx.z * hB;
+    Gi += blockIdx.z * giB;
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * L;
@@ -301,11 +317,13 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 template <typename T, int E, int SRC>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_xfwd(const void *src, typename cplx_of<T>::type *A,
                                                       FftPlan pl, int n0, int nlines, int kp,
-                                                      const Scalars *sc) {
+                                                      const Scalars *sc, size_t sB, size_t aB) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
     extern __shared__ char smem[];
     C *buf = (C *)smem;
+    sc += blockIdx.z;  // batch: grid blockIdx.z, its source sB cells and its spectrum aB elements apart
+    A += blockIdx.z * aB;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, K0 = n0 + 1, total = W * n0;
     const float rn = 1.0f / (float)n0, rk = 1.0f / (float)K0;
     const int line0 = blockIdx.x * W;
@@ -323,7 +341,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
                                         double>::type;
     double q2e_lo = 0.0;
     if constexpr (kTwoWords) q2e_lo = sc->q2e_lo;
-    const S *s = (const S *)src;
+    const S *s = (const S *)src + blockIdx.z * sB;
     S r[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -375,8 +393,12 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
                                                          typename cplx_of<T>::type *out, FftPlan pl,
                                                          int nin, int nout, size_t inS, size_t inO,
                                                          size_t outS, size_t outO, const T *Gh,
-                                                         int gp, int n1, int n2, int L1) {
+                                                         int gp, int n1, int n2, int L1, size_t inB, size_t outB,
+                                                         size_t ghB) {
     using C = typename cplx_of<T>::type;
+    in += blockIdx.z * inB;
+    out += blockIdx.z * outB;
+    if (Gh) Gh += blockIdx.z * ghB;
     constexpr int Q = (MODE == 1 || MODE == 4) ? E : E / 2;  // whole-line passes read E per thread
     extern __shared__ char smem[];
     C *buf = (C *)smem;
@@ -467,9 +489,12 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 template <typename T, int E, int OUTSCALE>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_xinv(const typename cplx_of<T>::type *A, T *phi,
                                                       FftPlan pl, int n0, int nlines, int kp,
-                                                      double invM, const Scalars *sc) {
+                                                      double invM, const Scalars *sc, size_t aB, size_t phiB) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
+    sc += blockIdx.z;
+    A += blockIdx.z * aB;
+    phi += blockIdx.z * phiB;
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1, total = W * K0;
@@ -527,6 +552,22 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 // bin's own field and +1 for its image charge (PartBunch.cpp:246-247,423-424);
 // CIC gathering is linear, so gathering these sums once equals the
 // reference's per-bin gathers added at each particle.
+// Grad at cell (i, j, k) of phi (p points at it): central differences, the
+// one-sided second-order stencils on the border faces of each component.
+template <typename T>
+__device__ inline void grad_stencil(const T *p, int i, int j, int k, int n0, int n1, int n2, size_t sy, size_t sz,
+                                    T id0, T id1, T id2, T &r0, T &r1, T &r2) {
+    if (i == 0) r0 = id0 * ((T(-1.0) * p[2] + T(4.0) * p[1]) - T(3.0) * p[0]);
+    else if (i == n0 - 1) r0 = id0 * ((T(1.0) * p[-2] - T(4.0) * p[-1]) + T(3.0) * p[0]);
+    else r0 = id0 * (p[1] - p[-1]);
+    if (j == 0) r1 = id1 * ((T(-1.0) * p[2 * sy] + T(4.0) * p[sy]) - T(3.0) * p[0]);
+    else if (j == n1 - 1) r1 = id1 * ((T(1.0) * p[-2 * (ptrdiff_t)sy] - T(4.0) * p[-(ptrdiff_t)sy]) + T(3.0) * p[0]);
+    else r1 = id1 * (p[sy] - p[-(ptrdiff_t)sy]);
+    if (k == 0) r2 = id2 * ((T(-1.0) * p[2 * sz] + T(4.0) * p[sz]) - T(3.0) * p[0]);
+    else if (k == n2 - 1) r2 = id2 * ((T(1.0) * p[-2 * (ptrdiff_t)sz] - T(4.0) * p[-(ptrdiff_t)sz]) + T(3.0) * p[0]);
+    else r2 = id2 * (p[sz] - p[-(ptrdiff_t)sz]);
+}
+
 template <typename T, bool ACC = false>
 __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>::type *Eg, int n0,
                                               int n1, int n2, const Scalars *sc,
@@ -541,17 +582,8 @@ __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>:
         const int i = (int)(c % n0);
         const int j = (int)((c / n0) % n1);
         const int k = (int)(c / sz);
-        const T *p = phi + c;
         T r0, r1, r2;
-        if (i == 0) r0 = id0 * ((T(-1.0) * p[2] + T(4.0) * p[1]) - T(3.0) * p[0]);
-        else if (i == n0 - 1) r0 = id0 * ((T(1.0) * p[-2] - T(4.0) * p[-1]) + T(3.0) * p[0]);
-        else r0 = id0 * (p[1] - p[-1]);
-        if (j == 0) r1 = id1 * ((T(-1.0) * p[2 * sy] + T(4.0) * p[sy]) - T(3.0) * p[0]);
-        else if (j == n1 - 1) r1 = id1 * ((T(1.0) * p[-2 * (ptrdiff_t)sy] - T(4.0) * p[-(ptrdiff_t)sy]) + T(3.0) * p[0]);
-        else r1 = id1 * (p[sy] - p[-(ptrdiff_t)sy]);
-        if (k == 0) r2 = id2 * ((T(-1.0) * p[2 * sz] + T(4.0) * p[sz]) - T(3.0) * p[0]);
-        else if (k == n2 - 1) r2 = id2 * ((T(1.0) * p[-2 * (ptrdiff_t)sz] - T(4.0) * p[-(ptrdiff_t)sz]) + T(3.0) * p[0]);
-        else r2 = id2 * (p[sz] - p[-(ptrdiff_t)sz]);
+        grad_stencil(phi + c, i, j, k, n0, n1, n2, sy, sz, id0, id1, id2, r0, r1, r2);
         typename vec4_of<T>::type e;
         e.x = -r0 * g0;
         e.y = -r1 * g1;
@@ -569,6 +601,47 @@ __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>:
         } else {
             Eg[c] = e;
         }
+    }
+}
+
+// The energy bins' fields summed on the grid in one pass: for each of the nb
+// bins (scalars scb[b], potential phi + b*phiB, and its image's phii + b*phiB
+// when phii is given), E_b = -Grad(phi_b) * (g_b, g_b, 1/g_b) and the magnetic
+// field betaC_b * (-E_y, E_x) of the bin (betaC_b * (E_y, -E_x) of its image),
+// added in bin order (bin, then its image) as the per-bin accumulation adds
+// them; Eg and Bg are written once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_field_sums(const T *phi, const T *phii, size_t phiB, int nb,
+                                                    typename vec4_of<T>::type *Eg, typename vec4_of<T>::type *Bg,
+                                                    int n0, int n1, int n2, const Scalars *scb) {
+    using V = typename vec4_of<T>::type;
+    const size_t total = (size_t)n0 * n1 * n2;
+    const size_t sy = n0, sz = (size_t)n0 * n1;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total;
+         c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % n0);
+        const int j = (int)((c / n0) % n1);
+        const int k = (int)(c / sz);
+        V E = {T(0), T(0), T(0), T(0)}, B = {T(0), T(0), T(0), T(0)};
+        for (int b = 0; b < nb; ++b) {
+            const Scalars *sc = scb + b;
+            const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
+            const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
+            for (int img = 0; img < (phii ? 2 : 1); ++img) {
+                const T *p = (img ? phii : phi) + b * phiB + c;
+                const T bs = (T)((img ? 1.0 : -1.0) * sc->betaC);
+                T r0, r1, r2;
+                grad_stencil(p, i, j, k, n0, n1, n2, sy, sz, id0, id1, id2, r0, r1, r2);
+                const T ex = -r0 * g0, ey = -r1 * g1, ez = -r2 * g2;
+                E.x += ex;
+                E.y += ey;
+                E.z += ez;
+                B.x += bs * ey;
+                B.y -= bs * ex;
+            }
+        }
+        Eg[c] = E;
+        Bg[c] = B;
     }
 }
 
@@ -746,65 +819,104 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
 
 void PoissonGrid::destroy() {
     void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2],
-                  HimgA, HimgB, Gimg, Esum, Bsum, Gper};
+                  HimgA, HimgB, Gimg, Esum, Bsum, Gper, bb_rho, bb_A, bb_B, bb_Bi, bb_phi, bb_phii, bb_T, bb_G,
+                  bb_Gh, bb_HA, bb_HB, bb_Gi};
     for (void *p : ps)
         if (p) (void)hipFree(p);
+    bb_rho = bb_A = bb_B = bb_Bi = bb_phi = bb_phii = bb_T = bb_G = bb_Gh = bb_HA = bb_HB = bb_Gi = nullptr;
+    bin_cap = 0;
+    bin_img_cap = false;
     rho_i64 = rho_f64 = A = B = Goct = Tcorner = Ghat = phi = Eg = nullptr;
     HimgA = HimgB = Gimg = Esum = Bsum = Gper = nullptr;
     twd[0] = twd[1] = twd[2] = nullptr;
 }
 
+// The buffers a Green's function or a solve works on: nb grids of this
+// shape, grid b's array starting b*stride elements after grid 0's (nb = 1,
+// strides 0: the context's own grid).  Every kernel takes grid b as
+// blockIdx.z and its scalars as d_sc + b.
+struct GridSet {
+    int nb = 1;
+    const void *rho = nullptr;  // x-pass source (fixed-point words or doubles), rhoB cells apart
+    size_t rhoB = 0;
+    void *A = nullptr, *B = nullptr, *Bi = nullptr;  // spectra (complex), AB / BB elements apart
+    size_t AB = 0, BB = 0;
+    void *phi = nullptr, *phii = nullptr;  // potential of the charge and of its image, phiB apart
+    size_t phiB = 0;
+    double *T = nullptr, *Goct = nullptr;  // corner table, octant (FP64)
+    size_t TB = 0, GB = 0;
+    void *Ghat = nullptr, *HA = nullptr, *HB = nullptr;  // real transforms (grid precision)
+    size_t GhB = 0;
+    void *Gimg = nullptr;  // image: complex transform over the doubled z
+    size_t GiB = 0;
+};
+
+static GridSet own_set(PoissonGrid &g, int src_kind) {
+    GridSet b;
+    b.rho = src_kind == 0 ? g.rho_i64 : g.rho_f64;
+    b.A = g.A;
+    b.B = g.B;
+    b.phi = g.phi;
+    b.T = (double *)g.Tcorner;
+    b.Goct = (double *)g.Goct;
+    b.Ghat = g.Ghat;
+    b.HA = g.HimgA;
+    b.HB = g.HimgB;
+    b.Gimg = g.Gimg;
+    return b;
+}
+
 // DCT-I along x (octant G, FP64, pitch n0+1 -> H, pitch gp) and along y, in place in H.
 template <typename T>
-static void dct_xy(PoissonGrid &g, const double *G, T *H, hipStream_t st) {
+static void dct_xy(PoissonGrid &g, const GridSet &gs, const double *G, T *H, hipStream_t st) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C), gp = (size_t)g.gp;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     {
         const FftPlan &pl = g.plan[0];
         int nlines = (n1 + 1) * (n2 + 1);
-        dim3 grid((nlines + pl.W - 1) / pl.W);
+        dim3 grid((nlines + pl.W - 1) / pl.W, 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    G, H, pl, n0, nlines, g.gp));
+                                    G, H, pl, n0, nlines, g.gp, gs.GB, gs.GhB));
     }
     {
         const FftPlan &pl = g.plan[1];
-        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1);
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    H, pl, n1 + 1, gp, gp * (n1 + 1)));
+                                    H, pl, n1 + 1, gp, gp * (n1 + 1), gs.GhB));
     }
 }
 
 template <typename T>
-static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t st) {
+static void green_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, bool cr, hipStream_t st) {
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const bool igf = g.greens != 1;
     if (!igf) {
-        k_green_standard<<<1024, 256, 0, st>>>((double *)g.Goct, n0, n1, n2, d_sc);
+        k_green_standard<<<dim3(1024, 1, gs.nb), 256, 0, st>>>(gs.Goct, n0, n1, n2, d_sc, gs.GB);
     } else {
-        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
-        if (cr) k_igf_corners<true><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
-        else k_igf_corners<false><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc);
-        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK);
-        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, n2, 1);
+        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2, gs.nb);
+        if (cr) k_igf_corners<true><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, gs.TB);
+        else k_igf_corners<false><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, gs.TB);
+        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK, gs.nb);
+        k_igf_diff<<<grid2, 256, 0, st>>>(gs.T, gs.Goct, n0, n1, n2, 1, gs.TB, gs.GB);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const size_t gp = (size_t)g.gp;
-    dct_xy<T>(g, (const double *)g.Goct, (T *)g.Ghat, st);
+    dct_xy<T>(g, gs, gs.Goct, (T *)gs.Ghat, st);
     if (g.pz) {
         // periodic z: the z line is G[k] for k in [0, n2), one-sided (mirrorRhoField,
-        // FFTPoissonSolver.cpp:661-662), so its transform is complex
+        // FFTPoissonSolver.cpp:661-662), so its transform is complex (one grid only)
         const FftPlan &pl = g.plan[2];
         dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
         OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    (const T *)g.Ghat, (const T *)g.Ghat, (C *)g.Gper, pl, n1, n2, g.gp));
+                                    (const T *)g.Ghat, (const T *)g.Ghat, (C *)g.Gper, pl, n1, n2, g.gp, 0, 0));
     } else {
         const FftPlan &pl = g.plan[2];
-        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    (T *)g.Ghat, pl, n2 + 1, gp * (n1 + 1), gp));
+                                    (T *)gs.Ghat, pl, n2 + 1, gp * (n1 + 1), gp, gs.GhB));
     }
     g.profile_mark(PROF_DCT);
 }
@@ -813,33 +925,34 @@ static void green_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t
 // FFTPoissonSolver.cpp:546-639) for the shift in the scalars: both corner
 // tables, their differences, DCT-I in x and y, complex FFT along the doubled z.
 template <typename T>
-static void green_image_impl(PoissonGrid &g, const Scalars *d_sc, bool cr, hipStream_t st) {
+static void green_image_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, bool cr, hipStream_t st) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    T *H[2] = {(T *)g.HimgA, (T *)g.HimgB};
+    T *H[2] = {(T *)gs.HA, (T *)gs.HB};
     for (int second = 0; second < 2; ++second) {
-        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2);
-        if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
-        else k_igf_corners_shifted<false><<<grid, 256, 0, st>>>((double *)g.Tcorner, n0, n1, n2, d_sc, second);
-        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK);
-        k_igf_diff<<<grid2, 256, 0, st>>>((const double *)g.Tcorner, (double *)g.Goct, n0, n1, n2, 0);
-        dct_xy<T>(g, (const double *)g.Goct, H[second], st);
+        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2, gs.nb);
+        if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, second, gs.TB);
+        else k_igf_corners_shifted<false><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, second, gs.TB);
+        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK, gs.nb);
+        k_igf_diff<<<grid2, 256, 0, st>>>(gs.T, gs.Goct, n0, n1, n2, 0, gs.TB, gs.GB);
+        dct_xy<T>(g, gs, gs.Goct, H[second], st);
     }
     g.profile_mark(PROF_GREEN);
     {
         const FftPlan &pl = g.plan[2];
-        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1);
+        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_green_zfft<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    H[0], H[1], (C *)g.Gimg, pl, n1, n2, g.gp));
+                                    H[0], H[1], (C *)gs.Gimg, pl, n1, n2, g.gp, gs.GhB, gs.GiB));
     }
     g.profile_mark(PROF_DCT);
 }
 
 void PoissonGrid::green(const Scalars *d_sc, bool cr, hipStream_t s) {
     if (!s) s = stream;
-    if (fp64) green_impl<double>(*this, d_sc, cr, s);
-    else green_impl<float>(*this, d_sc, cr, s);
+    const GridSet gs = own_set(*this, 0);
+    if (fp64) green_impl<double>(*this, gs, d_sc, cr, s);
+    else green_impl<float>(*this, gs, d_sc, cr, s);
 }
 
 int PoissonGrid::ensure_image(std::string &err) {
@@ -863,81 +976,222 @@ int PoissonGrid::ensure_image(std::string &err) {
 
 void PoissonGrid::green_image(const Scalars *d_sc, bool cr, hipStream_t s) {
     if (!s) s = stream;
-    if (fp64) green_image_impl<double>(*this, d_sc, cr, s);
-    else green_image_impl<float>(*this, d_sc, cr, s);
+    const GridSet gs = own_set(*this, 0);
+    if (fp64) green_image_impl<double>(*this, gs, d_sc, cr, s);
+    else green_image_impl<float>(*this, gs, d_sc, cr, s);
+}
+
+// The convolution passes (see the top of this file) on a GridSet.
+template <typename T>
+static void pass_x_fwd(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, int src_kind) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C);
+    const int n0 = g.n[0], nlines = g.n[1] * g.n[2];
+    const FftPlan &pl = g.plan[0];
+    dim3 grid((nlines + pl.W - 1) / pl.W, 1, gs.nb);
+    if (src_kind == 0)
+        OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    gs.rho, (C *)gs.A, pl, n0, nlines, g.kp, d_sc, gs.rhoB, gs.AB))
+    else
+        OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    gs.rho, (C *)gs.A, pl, n0, nlines, g.kp, d_sc, gs.rhoB, gs.AB))
+    g.profile_mark(PROF_FFT_X);
+}
+
+template <typename T>
+static void pass_y(PoissonGrid &g, const GridSet &gs, bool inverse, const void *in, void *out) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C), kp = (size_t)g.kp;
+    const int n1 = g.n[1], n2 = g.n[2], L1 = g.L[1];
+    const FftPlan &pl = g.plan[1];
+    dim3 grid((g.K0 + pl.W - 1) / pl.W, n2, gs.nb);
+    if (!inverse)
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (const C *)in, (C *)out, pl, n1, L1, kp, kp * n1, kp, kp * L1, nullptr, 0, n1, n2,
+                                    L1, gs.AB, gs.BB, 0))
+    else
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (const C *)in, (C *)out, pl, L1, n1, kp, kp * L1, kp, kp * n1, nullptr, 0, n1, n2,
+                                    L1, gs.BB, gs.AB, 0))
+    g.profile_mark(PROF_FFT_Y);
+}
+
+// MODE 2: times Ghat (in place allowed); 3: image (times -Gimg, z flipped); 4: periodic z
+template <typename T>
+static void pass_z(PoissonGrid &g, const GridSet &gs, int mode, const void *in, void *out) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C), kp = (size_t)g.kp;
+    const int n1 = g.n[1], n2 = g.n[2], L1 = g.L[1];
+    const FftPlan &pl = g.plan[2];
+    dim3 grid((g.K0 + pl.W - 1) / pl.W, L1, gs.nb);
+    const C *ci = (const C *)in;
+    C *co = (C *)out;
+    if (mode == 4)
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 4><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    ci, co, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Gper, g.gp, n1, n2, L1,
+                                    gs.BB, gs.BB, 0))
+    else if (mode == 3)
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 3><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    ci, co, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)gs.Gimg, g.gp, n1, n2, L1,
+                                    gs.BB, gs.BB, 2 * gs.GiB))
+    else
+        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    ci, co, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)gs.Ghat, g.gp, n1, n2, L1,
+                                    gs.BB, gs.BB, gs.GhB))
+    g.profile_mark(PROF_FFT_Z);
+}
+
+template <typename T>
+static void pass_x_inv(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, int outscale, void *phi) {
+    using C = typename cplx_of<T>::type;
+    const size_t cs = sizeof(C);
+    const int n0 = g.n[0], nlines = g.n[1] * g.n[2];
+    const FftPlan &pl = g.plan[0];
+    dim3 grid((nlines + pl.W - 1) / pl.W, 1, gs.nb);
+    const double invM = 1.0 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]);
+    if (outscale)
+        OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (const C *)gs.A, (T *)phi, pl, n0, nlines, g.kp, invM, d_sc, gs.AB, gs.phiB))
+    else
+        OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
+                                    (const C *)gs.A, (T *)phi, pl, n0, nlines, g.kp, invM, d_sc, gs.AB, gs.phiB))
+    g.profile_mark(PROF_FFT_X);
 }
 
 template <typename T>
 static void solve_impl(PoissonGrid &g, const Scalars *d_sc, int src_kind, int outscale, bool image,
                        hipEvent_t green_ready) {
-    using C = typename cplx_of<T>::type;
-    const size_t cs = sizeof(C);
-    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], K0 = g.K0;
-    const int L1 = g.L[1];
-    const size_t kp = (size_t)g.kp;
-    C *A = (C *)g.A, *B = (C *)g.B;
-    const int nlines = n1 * n2;
-    {
-        const FftPlan &pl = g.plan[0];
-        dim3 grid((nlines + pl.W - 1) / pl.W);
-        if (src_kind == 0)
-            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        g.rho_i64, A, pl, n0, nlines, g.kp, d_sc))
-        else
-            OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        g.rho_f64, A, pl, n0, nlines, g.kp, d_sc))
-    }
-    g.profile_mark(PROF_FFT_X);
-    {
-        const FftPlan &pl = g.plan[1];
-        dim3 grid((K0 + pl.W - 1) / pl.W, n2);
-        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                    A, B, pl, n1, L1, kp, kp * n1, kp, kp * L1, nullptr, 0, n1, n2, L1));
-    }
-    g.profile_mark(PROF_FFT_Y);
+    const GridSet gs = own_set(g, src_kind);
+    pass_x_fwd<T>(g, gs, d_sc, src_kind);
+    pass_y<T>(g, gs, false, gs.A, gs.B);
     // the Green's function, computed on another stream, is read from here on
     if (green_ready) (void)hipStreamWaitEvent(g.stream, green_ready, 0);
-    {
-        const FftPlan &pl = g.plan[2];
-        dim3 grid((K0 + pl.W - 1) / pl.W, L1);
-        if (g.pz)
-            OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 4><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Gper, g.gp,
-                                        n1, n2, L1))
-        else if (image)
-            OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 3><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Gimg, g.gp,
-                                        n1, n2, L1))
-        else
-            OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 2><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        B, B, pl, n2, n2, kp * L1, kp, kp * L1, kp, (const T *)g.Ghat, g.gp,
-                                        n1, n2, L1))
-    }
-    g.profile_mark(PROF_FFT_Z);
-    {
-        const FftPlan &pl = g.plan[1];
-        dim3 grid((K0 + pl.W - 1) / pl.W, n2);
-        OPALSC_DISPATCH_E(pl.E, k_strided<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                    B, A, pl, L1, n1, kp, kp * L1, kp, kp * n1, nullptr, 0, n1, n2, L1));
-    }
-    g.profile_mark(PROF_FFT_Y);
-    {
-        const FftPlan &pl = g.plan[0];
-        dim3 grid((nlines + pl.W - 1) / pl.W);
-        const double invM = 1.0 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]);
-        if (outscale)
-            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        A, (T *)g.phi, pl, n0, nlines, g.kp, invM, d_sc))
-        else
-            OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
-                                        A, (T *)g.phi, pl, n0, nlines, g.kp, invM, d_sc))
-    }
-    g.profile_mark(PROF_FFT_X);
+    pass_z<T>(g, gs, g.pz ? 4 : image ? 3 : 2, gs.B, gs.B);
+    pass_y<T>(g, gs, true, gs.B, gs.A);
+    pass_x_inv<T>(g, gs, d_sc, outscale, gs.phi);
 }
 
 void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale, bool image, hipEvent_t green_ready) {
     if (fp64) solve_impl<double>(*this, d_sc, src_kind, outscale, image, green_ready);
     else solve_impl<float>(*this, d_sc, src_kind, outscale, image, green_ready);
+}
+
+// Per-bin buffers of the batched energy-bin solve (allocated on first use,
+// grown when more bins or the image charge are asked for).
+int PoissonGrid::ensure_bins(int nb, bool image, std::string &err) {
+    if (nb <= bin_cap && (!image || bin_img_cap)) return 0;
+    void **ps[] = {&bb_rho, &bb_A, &bb_B, &bb_Bi, &bb_phi, &bb_phii, &bb_T, &bb_G, &bb_Gh, &bb_HA, &bb_HB, &bb_Gi};
+    for (void **q : ps) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    bin_cap = 0;
+    bin_img_cap = false;
+    const size_t rs = fp64 ? 8 : 4, cs = 2 * rs, m = (size_t)nb;
+    const size_t szA = (size_t)kp * n[1] * n[2] * cs, szB = (size_t)kp * L[1] * n[2] * cs;
+    const size_t szT = (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8, szGh = (size_t)gp * (n[1] + 1) * (n[2] + 1) * rs;
+    const size_t szGi = (size_t)L[2] * (n[1] + 1) * gp * cs;
+    struct {
+        void **p;
+        size_t bytes;
+        bool zero;
+    } want[] = {{&bb_rho, rho_words() * 8, false}, {&bb_A, szA, true},      {&bb_B, szB, true},
+                {&bb_phi, ncell * rs, false},      {&bb_T, szT, false},     {&bb_G, noct * 8, false},
+                {&bb_Gh, szGh, true},              {&bb_Bi, image ? szB : 0, true},
+                {&bb_phii, image ? ncell * rs : 0, false}, {&bb_HA, image ? szGh : 0, true},
+                {&bb_HB, image ? szGh : 0, true},  {&bb_Gi, image ? szGi : 0, true}};
+    for (auto &w : want) {
+        if (w.bytes == 0) continue;
+        if (hipMalloc(w.p, m * w.bytes) != hipSuccess) {
+            err = "hipMalloc of the energy-bin grids failed (" + std::to_string(m * w.bytes) + " bytes)";
+            return -1;
+        }
+        // padding columns are read as whole tiles and must hold (and keep) zeros
+        if (w.zero && hipMemset(*w.p, 0, m * w.bytes) != hipSuccess) {
+            err = "hipMemset failed";
+            return -1;
+        }
+    }
+    bin_cap = nb;
+    bin_img_cap = image;
+    return 0;
+}
+
+static GridSet bin_set(PoissonGrid &g, int nb) {
+    GridSet b;
+    b.nb = nb;
+    b.rho = g.bb_rho;
+    b.rhoB = g.ncell;
+    b.A = g.bb_A;
+    b.AB = (size_t)g.kp * g.n[1] * g.n[2];
+    b.B = g.bb_B;
+    b.Bi = g.bb_Bi;
+    b.BB = (size_t)g.kp * g.L[1] * g.n[2];
+    b.phi = g.bb_phi;
+    b.phii = g.bb_phii;
+    b.phiB = g.ncell;
+    b.T = (double *)g.bb_T;
+    b.TB = (size_t)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
+    b.Goct = (double *)g.bb_G;
+    b.GB = g.noct;
+    b.Ghat = g.bb_Gh;
+    b.HA = g.bb_HA;
+    b.HB = g.bb_HB;
+    b.GhB = (size_t)g.gp * (g.n[1] + 1) * (g.n[2] + 1);
+    b.Gimg = g.bb_Gi;
+    b.GiB = (size_t)g.L[2] * (g.n[1] + 1) * g.gp;
+    return b;
+}
+
+void PoissonGrid::zero_bin_rho(int nb, hipStream_t s) { launch_fill_i64(s, (size_t)nb * rho_words(), (long long *)bb_rho); }
+
+void PoissonGrid::green_bins(const Scalars *scb, int nb, bool cr, bool image, hipStream_t s) {
+    if (!s) s = stream;
+    const GridSet gs = bin_set(*this, nb);
+    if (fp64) {
+        green_impl<double>(*this, gs, scb, cr, s);
+        if (image) green_image_impl<double>(*this, gs, scb, cr, s);
+    } else {
+        green_impl<float>(*this, gs, scb, cr, s);
+        if (image) green_image_impl<float>(*this, gs, scb, cr, s);
+    }
+}
+
+// Every bin's solve (and its image's) as one batch: the forward x and y
+// passes once for both (they transform the same charge), the image z pass into
+// its own spectrum before the bin's in-place z pass, then the inverse passes.
+template <typename T>
+static void solve_bins_impl(PoissonGrid &g, const GridSet &gs, const Scalars *scb, bool image, hipEvent_t green_ready) {
+    pass_x_fwd<T>(g, gs, scb, 0);
+    pass_y<T>(g, gs, false, gs.A, gs.B);
+    if (green_ready) (void)hipStreamWaitEvent(g.stream, green_ready, 0);
+    if (image) pass_z<T>(g, gs, 3, gs.B, gs.Bi);
+    pass_z<T>(g, gs, 2, gs.B, gs.B);
+    pass_y<T>(g, gs, true, gs.B, gs.A);
+    pass_x_inv<T>(g, gs, scb, 1, gs.phi);
+    if (image) {
+        pass_y<T>(g, gs, true, gs.Bi, gs.A);
+        pass_x_inv<T>(g, gs, scb, 1, gs.phii);
+    }
+}
+
+void PoissonGrid::solve_bins(const Scalars *scb, int nb, bool image, hipEvent_t green_ready) {
+    const GridSet gs = bin_set(*this, nb);
+    if (fp64) solve_bins_impl<double>(*this, gs, scb, image, green_ready);
+    else solve_bins_impl<float>(*this, gs, scb, image, green_ready);
+}
+
+void PoissonGrid::field_sums(const Scalars *scb, int nb, bool image) {
+    const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
+    if (fp64)
+        k_field_sums<double><<<blocks, 256, 0, stream>>>((const double *)bb_phi, image ? (const double *)bb_phii : nullptr,
+                                                         ncell, nb, (double4 *)Esum, (double4 *)Bsum, n[0], n[1], n[2],
+                                                         scb);
+    else
+        k_field_sums<float><<<blocks, 256, 0, stream>>>((const float *)bb_phi, image ? (const float *)bb_phii : nullptr,
+                                                        ncell, nb, (float4 *)Esum, (float4 *)Bsum, n[0], n[1], n[2],
+                                                        scb);
+    profile_mark(PROF_GRAD);
 }
 
 void PoissonGrid::grad(const Scalars *d_sc) {
