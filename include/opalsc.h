@@ -364,6 +364,13 @@ int opalsc_host_atan_cr(int64_t n, const double *x, double *y);
 int opalsc_host_log_cr(int64_t n, const double *x, double *y);
 int opalsc_device_atan_cr(opalsc_ctx *ctx, int64_t n, const double *x, double *y);
 int opalsc_device_log_cr(opalsc_ctx *ctx, int64_t n, const double *x, double *y);
+/* The IGF corner term F(v0, v1, v2) of FFTPoissonSolver::integratedGreensFunction
+ * (src/Solvers/FFTPoissonSolver.cpp:479-489) at n points v[3i..3i+2], cell volume 1:
+ * fast = 0 with the correctly rounded functions (FP64 solve), fast = 1 with the
+ * reduced-instruction form of the mixed-precision solve.  Host and device
+ * evaluations of either form return the same bits. */
+int opalsc_host_igf_term(int64_t n, const double *v, int fast, double *y);
+int opalsc_device_igf_term(opalsc_ctx *ctx, int64_t n, const double *v, int fast, double *y);
 
 /* --- timing --------------------------------------------------------------
  * Per-kernel HIP-event timing of the tracker, recorded on the context's

@@ -5,6 +5,7 @@
 #include "common.hpp"
 #include "crmath.hpp"
 #include "frame.hpp"
+#include "igfmath.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -1596,6 +1597,25 @@ extern "C" int opalsc_device_atan_cr(opalsc_ctx *c, int64_t n, const double *x, 
 }
 extern "C" int opalsc_device_log_cr(opalsc_ctx *c, int64_t n, const double *x, double *y) {
     return device_cr(c, 1, n, x, y);
+}
+extern "C" int opalsc_host_igf_term(int64_t n, const double *v, int fast, double *y) {
+    if (n < 0 || (n > 0 && (!v || !y))) return OPALSC_EINVAL;
+    for (int64_t i = 0; i < n; ++i)
+        y[i] = fast ? igf::term_fast(v[3 * i], v[3 * i + 1], v[3 * i + 2], 1.0)
+                    : igf::term_cr(v[3 * i], v[3 * i + 1], v[3 * i + 2], 1.0);
+    return OPALSC_OK;
+}
+extern "C" int opalsc_device_igf_term(opalsc_ctx *c, int64_t n, const double *v, int fast, double *y) {
+    if (!c || n < 0 || (n > 0 && (!v || !y))) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if (n == 0) return OPALSC_OK;
+    if (int e = upload(c, c->aosR, v, (size_t)n * 24)) return e;
+    if (!c->aosP.ensure((size_t)n * 8)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
+    launch_igf_term_test(c->stream, fast, n, c->aosR.as<double>(), c->aosP.as<double>());
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(y, c->aosP.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -220,6 +220,7 @@ void launch_iota(hipStream_t s, int64_t N, uint32_t *v);
 void launch_fill_i64(hipStream_t s, size_t n, long long *p);
 void launch_copy_calibration(hipStream_t s, size_t bytes, const void *src, void *dst);
 void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y);
+void launch_igf_term_test(hipStream_t s, int fast, int64_t n, const double *v, double *y);
 
 // Tracker kernels
 // bkeys (may be null): the six bound keys of launch_bounds_from_boxes, reset for the step

@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "crmath.hpp"
 #include "frame.hpp"
+#include "igfmath.hpp"
 #include "kernels.hpp"
 #include "gather.hpp"
 #include "reduce.hpp"
@@ -1042,6 +1043,14 @@ __global__ void k_cr_test(int which, int64_t n, const double *x, double *y) {
 }
 void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double *y) {
     if (n > 0) k_cr_test<<<stream_blocks(n), kThreads, 0, s>>>(which, n, x, y);
+}
+__global__ void k_igf_term_test(int fast, int64_t n, const double *v, double *y) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = fast ? igf::term_fast(v[3 * i], v[3 * i + 1], v[3 * i + 2], 1.0)
+                    : igf::term_cr(v[3 * i], v[3 * i + 1], v[3 * i + 2], 1.0);
+}
+void launch_igf_term_test(hipStream_t s, int fast, int64_t n, const double *v, double *y) {
+    if (n > 0) k_igf_term_test<<<stream_blocks(n), kThreads, 0, s>>>(fast, n, v, y);
 }
 
 // ---- tracker ----------------------------------------------------------------------

@@ -19,8 +19,8 @@
 // its transform Ghat is real and even as well: it is the 3D DCT-I of the
 // (n+1)^3 octant, computed here with three DCT passes instead of a (2n)^3 r2c.
 #include "common.hpp"
-#include "crmath.hpp"
 #include "fft.hpp"
+#include "igfmath.hpp"
 #include "kernels.hpp"
 
 #include <cmath>
@@ -34,43 +34,23 @@ namespace opalsc {
 // ---------------------------------------------------------------------------
 
 // The IGF antiderivative at corner (v0, v1, v2): FFTPoissonSolver.cpp:479-489
-// (and, with shifted v2, FFTPoissonSolver.cpp:563-573, 589-599).
+// (and, with shifted v2, FFTPoissonSolver.cpp:563-573, 589-599); CR: the
+// reference's operations with correctly rounded atan and log (FP64 solve),
+// otherwise the reduced-instruction form of the mixed-precision solve.
 template <bool CR>
-__device__ inline double igf_term(double v0, double v1, double v2, double cellVolume) {
-    double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-    double a0, a1, a2, l0, l1, l2;
-    if constexpr (CR) {
-        a0 = crm::atan_cr(v0 * v1 / (v2 * r));
-        a1 = crm::atan_cr(v0 * v2 / (v1 * r));
-        a2 = crm::atan_cr(v1 * v2 / (v0 * r));
-        l0 = crm::log_cr(v0 + r);
-        l1 = crm::log_cr(v1 + r);
-        l2 = crm::log_cr(v2 + r);
-    } else {
-        a0 = atan(v0 * v1 / (v2 * r));
-        a1 = atan(v0 * v2 / (v1 * r));
-        a2 = atan(v1 * v2 / (v0 * r));
-        l0 = log(v0 + r);
-        l1 = log(v1 + r);
-        l2 = log(v2 + r);
-    }
-    double t = -v2 * v2 * a0 / 2;
-    t += -v1 * v1 * a1 / 2;
-    t += -v0 * v0 * a2 / 2;
-    t += v1 * v2 * l0;
-    t += v0 * v2 * l1;
-    t += v0 * v1 * l2;
-    return t / cellVolume;
+__device__ inline double igf_term(double v0, double v1, double v2, double cellVolume, double inv_volume) {
+    if constexpr (CR) return igf::term_cr(v0, v1, v2, cellVolume);
+    else return igf::term_fast(v0, v1, v2, inv_volume);
 }
 
 // FFTPoissonSolver::integratedGreensFunction, FFTPoissonSolver.cpp:472-491
 template <bool CR>
 __device__ inline double igf_corner(int i, int j, int k, double h0, double h1, double h2,
-                                    double cellVolume) {
+                                    double cellVolume, double inv_volume) {
     double v0 = i * h0 - h0 / 2;
     double v1 = j * h1 - h1 / 2;
     double v2 = k * h2 - h2 / 2;
-    return igf_term<CR>(v0, v1, v2, cellVolume);
+    return igf_term<CR>(v0, v1, v2, cellVolume, inv_volume);
 }
 
 // Corner antiderivatives T[k][j][i], i,j,k in [0, n+1], each evaluated once
@@ -82,12 +62,12 @@ __global__ __launch_bounds__(256) void k_igf_corners(double *T, int n0, int n1, 
     sc += blockIdx.z;  // batch: one table per grid, tB doubles apart
     T += blockIdx.z * tB;
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2];
-    const double vol = h0 * h1 * h2;
+    const double vol = h0 * h1 * h2, ivol = 1.0 / vol;
     const int m0 = n0 + 2, plane = m0 * (n1 + 2);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= plane) return;
     const int j = div_small(c, m0, 1.0f / (float)m0), i = c - j * m0, k = blockIdx.y;
-    T[(size_t)k * plane + c] = igf_corner<CR>(i, j, k, h0, h1, h2, vol);
+    T[(size_t)k * plane + c] = igf_corner<CR>(i, j, k, h0, h1, h2, vol, ivol);
 }
 
 // The two corner tables of FFTPoissonSolver::shiftedIntGreensFunction
@@ -99,7 +79,7 @@ __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, 
     sc += blockIdx.z;
     T += blockIdx.z * tB;
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2], zshift = sc->zshift;
-    const double vol = h0 * h1 * h2;
+    const double vol = h0 * h1 * h2, ivol = 1.0 / vol;
     const int m0 = n0 + 2, plane = m0 * (n1 + 2);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= plane) return;
@@ -108,7 +88,7 @@ __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, 
     double v1 = j * h1 - h1 / 2;
     double v2 = k * h2 - h2 / 2 + zshift;
     if (second) v2 = v2 - n2 * h2;
-    T[(size_t)k * plane + c] = igf_term<CR>(v0, v1, v2, vol);
+    T[(size_t)k * plane + c] = igf_term<CR>(v0, v1, v2, vol, ivol);
 }
 
 // G octant [0,n0]x[0,n1]x[0,n2] from T: the 8-corner difference in the

@@ -43,7 +43,7 @@ EXPORTS = [
     "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init",
     "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
-    "opalsc_device_log_cr", "opalsc_profile_enable", "opalsc_profile_read",
+    "opalsc_device_log_cr", "opalsc_host_igf_term", "opalsc_device_igf_term", "opalsc_profile_enable", "opalsc_profile_read",
     "opalsc_copy_bandwidth", "opalsc_fromfile_count", "opalsc_fromfile_read", "opalsc_stat_row_get",
     "opalsc_stat_write", "opalsc_fieldmap_add", "opalsc_fieldmap_clear", "opalsc_fieldmap_eval",
 ]
@@ -156,6 +156,8 @@ def lib():
         "opalsc_host_log_cr": ([i64, dp, dp], C.c_int),
         "opalsc_device_atan_cr": ([vp, i64, dp, dp], C.c_int),
         "opalsc_device_log_cr": ([vp, i64, dp, dp], C.c_int),
+        "opalsc_host_igf_term": ([i64, dp, C.c_int, dp], C.c_int),
+        "opalsc_device_igf_term": ([vp, i64, dp, C.c_int, dp], C.c_int),
         "opalsc_profile_enable": ([vp, C.c_int], C.c_int),
         "opalsc_profile_read": ([vp, C.c_int, C.POINTER(C.c_char_p), dp, dp], C.c_int),
         "opalsc_copy_bandwidth": ([vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
@@ -228,6 +230,16 @@ def host_log_cr(x):
     x = _d(x).ravel()
     y = np.empty_like(x)
     _check(None, lib().opalsc_host_log_cr(len(x), _ptr(x), _ptr(y)))
+    return y
+
+
+def host_igf_term(v, fast):
+    """The IGF corner term at points v (n x 3), cell volume 1, evaluated on the
+    host: fast=False the correctly rounded form, fast=True the mixed-precision
+    solve's form."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    y = np.empty(len(v))
+    _check(None, lib().opalsc_host_igf_term(len(v), _ptr(v), 1 if fast else 0, _ptr(y)))
     return y
 
 
@@ -396,6 +408,12 @@ class SpaceChargeContext:
         x = _d(x).ravel()
         y = np.empty_like(x)
         self._c(lib().opalsc_device_log_cr(self._h, len(x), _ptr(x), _ptr(y)))
+        return y
+
+    def device_igf_term(self, v, fast):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        y = np.empty(len(v))
+        self._c(lib().opalsc_device_igf_term(self._h, len(v), _ptr(v), 1 if fast else 0, _ptr(y)))
         return y
 
     # ---- device-resident tracker (ParallelTTracker::execute) ----
