@@ -1240,17 +1240,20 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     }
     const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
     int nparts = gamma_blocks(c->grid.fp64, b.N);
-    // the common mixed case in blocks that run once (2.30 -> 2.20 ms at 1e8 particles)
-    const bool oneshot = !binned && !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
+    // in blocks that run once unless a DC beam or field maps need the persistent
+    // loop (2.30 -> 2.20 ms at 1e8 particles, mixed)
+    const bool oneshot = !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
     // the next step of this call takes its bounds from the statistics this pass leaves
     const bool boxes_out = oneshot && next && c->track_sc && c->osbox.p;
-    if (binned && c->track_sc)
+    const bool bsum = binned && c->track_sc;  // the bins' summed fields
+    if (oneshot)
+        nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, b, bsum ? c->grid.Esum : c->grid.Eg, c->d_sc,
+                                       c->d_ts, c->os_dense.as<double>(), c->partials, ext,
+                                       boxes_out ? c->osbox.as<double>() : nullptr,
+                                       bsum ? c->grid.Bsum : nullptr);
+    else if (bsum)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
                            c->grid.Bsum, ext, false);
-    else if (oneshot)
-        nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts,
-                                       c->os_dense.as<double>(), c->partials, ext,
-                                       boxes_out ? c->osbox.as<double>() : nullptr);
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
                            ext, c->grid.pz);

@@ -233,14 +233,16 @@ void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Sc
 void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int n[3], double dh,
                        const TrackState *ts, bool exact, bool q_roundtrip, Scalars *sc);
 int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
-// the mixed gather pass in blocks that run once (no bins, DC beam or maps):
+// the gather pass in blocks that run once (no DC beam or field maps):
 // os_blocks(N) rows of kMom doubles in dense; returns the partial blocks for k_track_post
 int os_blocks(int64_t N);
 int os_blocks64(int64_t N);
 // osbox (used with next): per one-shot block the bounds (kOsBoxStats doubles) of the
-// positions written, which the next step's bounds come from
+// positions written, which the next step's bounds come from.  Bg: the energy
+// bins' summed magnetic field grid (then B is gathered, not formed from E).
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                          const TrackState *ts, double *dense, double *partials, int ext, double *osbox);
+                          const TrackState *ts, double *dense, double *partials, int ext, double *osbox,
+                          const void *Bg = nullptr);
 // blocks of statistics launch_track_gamma_os leaves (one per one-shot block)
 int64_t os_box_count(bool fp64, int64_t N);
 // The bounds pass from the previous gather pass's statistics: the beam-frame

@@ -1437,7 +1437,7 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
 // BINNED (energy bins): Eg and Bg hold the sums over the bins of E and of the
 // magnetic field each bin and its image imply (ParallelTTracker.cpp:567-579),
 // both gathered.  Four waves per SIMD (at most 128 VGPRs) keep enough loads in
-// flight.  The plain FP64 case (no bins, DC beam or field maps) runs in
+// flight.  The FP64 case without a DC beam or field maps runs in
 // k_track_gamma_os instead.
 template <bool NEXT, bool BINNED, bool MAPS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_track_gamma(
@@ -1661,8 +1661,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     store_moment_partials(m, partials, lds);
 }
-// The common case of the mixed-precision gather pass (no energy bins, no DC
-// beam, no field maps) in blocks that run once: block b owns kOsU*kThreads
+// The common case of the mixed-precision gather pass (no DC beam, no field
+// maps; with BINNED the energy bins' summed E and B grids) in blocks that run once: block b owns kOsU*kThreads
 // consecutive particles (its XCD's share of the bunch in order, as in the
 // persistent loop), loads all of them before the first field gather, and
 // leaves the block's plain-double moment sums in dense[b*kMom + k] (wave
@@ -1675,11 +1675,11 @@ int os_blocks(int64_t N) {
     return (int)(per * kXcds);
 }
 
-template <bool NEXT, int EXT>
+template <bool NEXT, int EXT, bool BINNED>
 __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double *x, double *y, double *z, double *px,
                                                               double *py, double *pz, const float4 *Eg,
-                                                              const Scalars *sc, const TrackState *ts,
-                                                              double *dense, double *osbox) {
+                                                              const float4 *Bg, const Scalars *sc,
+                                                              const TrackState *ts, double *dense, double *osbox) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
     const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
@@ -1737,9 +1737,14 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
                 bool ok = cic_dim_fast(r[0], 0.0, ih[0], l[0], d[0]);
                 ok &= cic_dim_fast(r[1], 0.0, ih[1], l[1], d[1]);
                 ok &= cic_dim_fast(r[2], 0.0, ih[2], l[2], d[2]);
-                if (ok) gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
-                Bf[0] = -betaC * Ef[1];
-                Bf[1] = betaC * Ef[0];
+                if (ok) {
+                    gather_E_fast(Eg, l, d, n0, n1, n2, Ef);
+                    if constexpr (BINNED) gather_E_fast(Bg, l, d, n0, n1, n2, Bf);
+                }
+                if constexpr (!BINNED) {
+                    Bf[0] = -betaC * Ef[1];
+                    Bf[1] = betaC * Ef[0];
+                }
             }
             if constexpr (EXT != 0)
                 for (int k = 0; k < 3; ++k) {
@@ -1790,8 +1795,8 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
     }
 }
 
-// The FP64 gather pass (no energy bins, DC beam or field maps) in blocks that
-// run once, kOsU64 particles per thread, the per-particle sequence of
+// The FP64 gather pass (no DC beam or field maps; with BINNED the energy
+// bins' summed E and B grids) in blocks that run once, kOsU64 particles per thread, the per-particle sequence of
 // k_track_gamma unchanged; moment sums as in k_track_gamma_mixed_os.
 int os_blocks64(int64_t N) {
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
@@ -1799,10 +1804,11 @@ int os_blocks64(int64_t N) {
     return (int)(per * kXcds);
 }
 
-template <bool NEXT>
+template <bool NEXT, bool BINNED>
 __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, double *y, double *z, double *px,
-                                                        double *py, double *pz, const double4 *Eg, const Scalars *sc,
-                                                        const TrackState *ts, double *dense, double *osbox) {
+                                                        double *py, double *pz, const double4 *Eg, const double4 *Bg,
+                                                        const Scalars *sc, const TrackState *ts, double *dense,
+                                                        double *osbox) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
@@ -1846,9 +1852,14 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
                 bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
                 ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
                 ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
-                if (ok) gather_E<double>(Eg, l, d, n0, n1, n2, Ef);
-                Bf[0] = Bf[0] - betaC * Ef[1];
-                Bf[1] = Bf[1] + betaC * Ef[0];
+                if (ok) {
+                    gather_E<double>(Eg, l, d, n0, n1, n2, Ef);
+                    if constexpr (BINNED) gather_E<double>(Bg, l, d, n0, n1, n2, Bf);
+                }
+                if constexpr (!BINNED) {
+                    Bf[0] = Bf[0] - betaC * Ef[1];
+                    Bf[1] = Bf[1] + betaC * Ef[0];
+                }
                 double t3[3];
                 transform_to(sc->M_br, sc->o_br, r);
                 mat_vec(sc->M_br, Ef, t3);
@@ -1931,22 +1942,58 @@ __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64
         }
 }
 
+namespace {
+struct OsArgs {
+    hipStream_t s;
+    Bunch *b;
+    const void *Eg, *Bg;
+    const Scalars *sc;
+    const TrackState *ts;
+    double *dense, *osbox;
+};
+template <bool NEXT, bool BINNED>
+void launch_os64(const OsArgs &a, int blocks) {
+    k_track_gamma_os<NEXT, BINNED><<<blocks, kThreads, 0, a.s>>>(a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py,
+                                                                  a.b->pz, (const double4 *)a.Eg,
+                                                                  (const double4 *)a.Bg, a.sc, a.ts, a.dense,
+                                                                  NEXT ? a.osbox : nullptr);
+}
+template <bool NEXT, int EXT, bool BINNED>
+void launch_os_mixed(const OsArgs &a, int blocks) {
+    k_track_gamma_mixed_os<NEXT, EXT, BINNED><<<blocks, kThreads, 0, a.s>>>(
+        a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const float4 *)a.Eg, (const float4 *)a.Bg, a.sc,
+        a.ts, a.dense, NEXT ? a.osbox : nullptr);
+}
+template <bool NEXT, int EXT>
+void launch_os_mixed_b(const OsArgs &a, int blocks) {
+    if (a.Bg) launch_os_mixed<NEXT, EXT, true>(a, blocks);
+    else launch_os_mixed<NEXT, EXT, false>(a, blocks);
+}
+}  // namespace
+
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
-                          const TrackState *ts, double *dense, double *partials, int ext, double *osbox) {
+                          const TrackState *ts, double *dense, double *partials, int ext, double *osbox,
+                          const void *Bg) {
+    const OsArgs a{s, &b, Eg, Bg, sc, ts, dense, osbox};
     if (fp64) {
         const int blocks64 = os_blocks64(b.N);
-        if (next) k_track_gamma_os<true><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense, osbox);
-        else k_track_gamma_os<false><<<blocks64, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const double4 *)Eg, sc, ts, dense, nullptr);
+        if (next) {
+            if (Bg) launch_os64<true, true>(a, blocks64);
+            else launch_os64<true, false>(a, blocks64);
+        } else {
+            if (Bg) launch_os64<false, true>(a, blocks64);
+            else launch_os64<false, false>(a, blocks64);
+        }
         k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks64, partials);
         return kOsReduceBlocks;
     }
     const int blocks = os_blocks(b.N);
     if (next) {
-        if (ext) k_track_gamma_mixed_os<true, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, osbox);
-        else k_track_gamma_mixed_os<true, 0><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, osbox);
+        if (ext) launch_os_mixed_b<true, 1>(a, blocks);
+        else launch_os_mixed_b<true, 0>(a, blocks);
     } else {
-        if (ext) k_track_gamma_mixed_os<false, 1><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, nullptr);
-        else k_track_gamma_mixed_os<false, 0><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, b.px, b.py, b.pz, (const float4 *)Eg, sc, ts, dense, nullptr);
+        if (ext) launch_os_mixed_b<false, 1>(a, blocks);
+        else launch_os_mixed_b<false, 0>(a, blocks);
     }
     k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks, partials);
     return kOsReduceBlocks;
