@@ -2,6 +2,7 @@
 // the context's device memory and stream and sequences the kernels.  No C++
 // exception crosses the boundary; every entry point returns a status code.
 #include "../../include/opalsc.h"
+#include "bingamma.hpp"
 #include "common.hpp"
 #include "crmath.hpp"
 #include "frame.hpp"
@@ -75,6 +76,7 @@ struct opalsc_ctx {
     bool bin_image = true;
     DevBuf binsorted;         // int32 bin of each resident particle (bin order)
     DevBuf binpart;           // calcGammas partial sums
+    DevBuf binchunks;         // calcGammas chunk sums (bingamma.hpp), 2 per one-shot gather chunk
     DevBuf scb;               // Scalars per bin
     DevBuf scbc;              // Scalars of the non-empty bins, in bin order (the batched solve's)
     double qmax = 0.0;        // max |Q| of the resident bunch
@@ -269,7 +271,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     free_bunch(c->bunch);
     DevBuf *bs[] = {&c->aosR, &c->aosP, &c->aosEf, &c->aosBf, &c->dtb,      &c->statsb,  &c->tmpd,
                     &c->binb, &c->gamb, &c->cntb,  &c->binsorted, &c->binpart, &c->scb, &c->scbc, &c->fm_desc, &c->fm_data,
-                    &c->os_dense, &c->osbox};
+                    &c->os_dense, &c->osbox, &c->binchunks};
     for (DevBuf *b : bs) b->release();
     void *ps[] = {c->d_sc, c->d_ts, c->partials, c->d_counters};
     for (void *p : ps)
@@ -1024,6 +1026,9 @@ extern "C" int opalsc_track_init(opalsc_ctx *c, const opalsc_track_params *p) {
         c->os_dense.release();  // the persistent gather pass then serves
     if (!c->osbox.ensure((size_t)std::max(os_box_count(true, cap1), os_box_count(false, cap1)) * kOsBoxStats * 8))
         c->osbox.release();  // every step then runs the full bounds pass
+    const int64_t nch = std::max((cap1 + kOsChunk64 - 1) / kOsChunk64, (cap1 + kOsChunk - 1) / kOsChunk);
+    if (!c->binchunks.ensure((size_t)nch * 2 * 8))
+        c->binchunks.release();  // energy bins then sum over the bins' momenta every step
     c->have_track = true;
     ++c->gen;
     c->track_sc = p->space_charge != 0;
@@ -1162,7 +1167,13 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     const bool binned = c->bins.nbins > 0;
     unsigned long long *bkeys = &c->d_counters[8];
+    // energy bins whose calcGammas comes from chunk sums (bingamma.hpp): the
+    // previous gather pass of this call left them, or they are summed here,
+    // before the bounds pass and the sort move particles
+    const int64_t gchunk = c->grid.fp64 ? kOsChunk64 : kOsChunk;
+    const bool chunked = binned && c->track_sc && c->binchunks.p && bins_chunked(c->bins, gchunk);
     launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
+    if (chunked && !boxes) launch_bin_gamma_chunks(c->stream, c->grid.fp64, b, c->bins, c->binchunks.as<double>());
     c->prof.mark(PROF_SCALAR);
     if (c->track_sc && !push && boxes) {
         // the previous gather pass left the statistics of these positions
@@ -1184,10 +1195,10 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
         if (binned) {
             // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its
             // image; the non-empty bins are solved as one batch (one launch per pass)
-            launch_track_bin_scalars(c->stream, b, c->bins, c->bin_qi, exact, c->d_sc, c->binpart.as<double>(),
-                                     c->gamb.as<double>(), c->scb.as<Scalars>());
             Scalars *scz = c->scbc.as<Scalars>();
-            launch_compact_bin_scalars(c->stream, c->scb.as<Scalars>(), c->bins, scz);
+            launch_track_bin_scalars(c->stream, b, c->bins, c->bin_qi, exact, c->d_sc, c->binpart.as<double>(),
+                                     chunked ? c->binchunks.as<double>() : nullptr, gchunk, c->gamb.as<double>(),
+                                     c->scb.as<Scalars>(), scz);
             const int nz = nonempty_bins(c->bins);
             c->prof.mark(PROF_SCALAR);
             // the bins' Green's functions need only their scalars: they run on the
@@ -1250,7 +1261,8 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
         nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, b, bsum ? c->grid.Esum : c->grid.Eg, c->d_sc,
                                        c->d_ts, c->os_dense.as<double>(), c->partials, ext,
                                        boxes_out ? c->osbox.as<double>() : nullptr,
-                                       bsum ? c->grid.Bsum : nullptr);
+                                       bsum ? c->grid.Bsum : nullptr, &c->bins,
+                                       boxes_out && chunked ? c->binchunks.as<double>() : nullptr);
     else if (bsum)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
                            c->grid.Bsum, ext, false);

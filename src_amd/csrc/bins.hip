@@ -8,6 +8,7 @@
 // image charge.  The particles are ordered by bin, so each bin's deposit is
 // one contiguous range.  The fields of all bins are summed on the grid and
 // gathered once (CIC gathering is linear).
+#include "bingamma.hpp"
 #include "common.hpp"
 #include "gather.hpp"
 #include "kernels.hpp"
@@ -176,41 +177,96 @@ __global__ __launch_bounds__(kReduceThreads) void k_track_bin_gamma_final(const 
 
 // Per-bin scalars of the step: copies of the step's scalars (mesh, frames)
 // with bin b's Lorentz-frame values and charge qi.
+// Also writes the scalars of the non-empty bins, in bin order, to scbc (the
+// batched solve's bins: an empty bin has no charge and no field).
 __global__ void k_track_bin_scalars(const Scalars *sc, const double *gammas, BinRanges r, double qi,
-                                    int exact, Scalars *scb) {
+                                    int exact, Scalars *scb, Scalars *scbc) {
     const int b = threadIdx.x;
     if (b >= r.nbins) return;
     Scalars s = *sc;
     s.qi = qi;
+    const int64_t count = r.start[b + 1] - r.start[b];
     // every deposit is |qi*dT| <= |qi|*dT*(1+2^-52)
-    bin_scalars(&s, gammas[b], r.start[b + 1] - r.start[b], fabs(qi) * s.dT * (1.0 + 0x1p-50), exact != 0);
+    bin_scalars(&s, gammas[b], count, fabs(qi) * s.dT * (1.0 + 0x1p-50), exact != 0);
     scb[b] = s;
+    if (count > 0) {
+        int z = 0;
+        for (int k = 0; k < b; ++k) z += r.start[k + 1] > r.start[k] ? 1 : 0;
+        scbc[z] = s;
+    }
+}
+
+// calcGammas from the chunk sums (the layout bingamma.hpp describes): block b
+// adds, in chunk order, g0 of the chunks that start inside bin b and g1 of the
+// chunk that starts before it (double-double), then divides by the count.
+__global__ __launch_bounds__(kReduceThreads) void k_bin_gamma_from_chunks(const double *binp, int64_t C,
+                                                                         BinRanges r, double *gammas) {
+    __shared__ double lds[16 * 2];
+    const int b = blockIdx.x;
+    const int64_t s0 = r.start[b], s1 = r.start[b + 1];
+    double h[1] = {0.0}, l[1] = {0.0};
+    if (s1 > s0) {
+        const int64_t c0 = s0 / C, c1 = (s1 - 1) / C;
+        for (int64_t c = c0 + threadIdx.x; c <= c1; c += blockDim.x) {
+            const double g = c * C >= s0 ? binp[2 * c] : binp[2 * c + 1];
+            const dd v = dd_add(dd{h[0], l[0]}, dd{g, 0.0});
+            h[0] = v.hi;
+            l[0] = v.lo;
+        }
+    }
+    block_sum_dd<1>(h, l, lds);
+    if (threadIdx.x == 0) gammas[b] = s1 > s0 ? (h[0] + l[0]) / (double)(s1 - s0) : 1.0;
+}
+
+// The chunk sums of the resident momenta, for a step whose previous gather
+// pass did not leave them: U particles per thread, as in the one-shot gather
+// pass of the same chunk size C = U*kThreads.
+template <int U>
+__global__ __launch_bounds__(256) void k_bin_gamma_chunks(int64_t N, const double *__restrict__ px,
+                                                          const double *__restrict__ py,
+                                                          const double *__restrict__ pz, BinRanges r,
+                                                          double *binp) {
+    __shared__ double lds[2 * (kThreads / 64)];
+    constexpr int64_t C = (int64_t)U * kThreads;
+    const int64_t chunk = blockIdx.x, first = chunk * C, base = first + threadIdx.x;
+    const int64_t split = bin_chunk_split(r, first, N);
+    double g0 = 0.0, g1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t i = base + u * kThreads;
+        if (i >= N) break;
+        const double g = lorentz_gamma(px[i], py[i], pz[i]);
+        if (i < split) g0 += g;
+        else g1 += g;
+    }
+    bin_chunk_store(g0, g1, lds, binp, chunk, true);
+}
+
+void launch_bin_gamma_chunks(hipStream_t s, bool fp64, const Bunch &bunch, const BinRanges &r, double *binp) {
+    if (bunch.N <= 0 || r.nbins <= 0) return;
+    const int64_t C = fp64 ? kOsChunk64 : kOsChunk;
+    const unsigned nch = (unsigned)((bunch.N + C - 1) / C);
+    if (fp64) k_bin_gamma_chunks<kOsU64><<<nch, kThreads, 0, s>>>(bunch.N, bunch.px, bunch.py, bunch.pz, r, binp);
+    else k_bin_gamma_chunks<kOsU><<<nch, kThreads, 0, s>>>(bunch.N, bunch.px, bunch.py, bunch.pz, r, binp);
 }
 
 void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges &r, double qi, bool exact,
-                              const Scalars *sc, double *partials, double *gammas, Scalars *scb) {
+                              const Scalars *sc, double *partials, const double *binp, int64_t chunk,
+                              double *gammas, Scalars *scb, Scalars *scbc) {
     if (r.nbins <= 0) return;
-    // blocks per bin: about 8 particles per thread in the most populated bin
-    int64_t most = 0;
-    for (int k = 0; k < r.nbins; ++k) most = std::max(most, r.start[k + 1] - r.start[k]);
-    const int nx = (int)std::min<int64_t>(kBinGammaBlocks, std::max<int64_t>(1, (most + 2047) / 2048));
-    k_track_bin_gamma_partials<<<dim3(nx, r.nbins), kThreads, 0, s>>>(bunch.px, bunch.py, bunch.pz, r, partials);
-    k_track_bin_gamma_final<<<r.nbins, kReduceThreads, 0, s>>>(partials, nx, r, gammas);
-    k_track_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, r, qi, exact ? 1 : 0, scb);
-}
-
-// The scalars of the non-empty bins, in bin order, for the batched solve
-// (an empty bin has no charge and no field).
-__global__ void k_compact_bin_scalars(const Scalars *scb, BinRanges r, Scalars *out) {
-    int z = 0;
-    for (int k = 0; k < r.nbins; ++k)
-        if (r.start[k + 1] > r.start[k]) {
-            if ((int)threadIdx.x == z) out[z] = scb[k];
-            ++z;
-        }
-}
-void launch_compact_bin_scalars(hipStream_t s, const Scalars *scb, const BinRanges &r, Scalars *out) {
-    if (r.nbins > 0) k_compact_bin_scalars<<<1, kMaxBins, 0, s>>>(scb, r, out);
+    static_assert(kMaxBins <= 64, "one thread per bin");
+    if (binp) {
+        k_bin_gamma_from_chunks<<<r.nbins, kReduceThreads, 0, s>>>(binp, chunk, r, gammas);
+    } else {
+        // blocks per bin: about 8 particles per thread in the most populated bin
+        int64_t most = 0;
+        for (int k = 0; k < r.nbins; ++k) most = std::max(most, r.start[k + 1] - r.start[k]);
+        const int nx = (int)std::min<int64_t>(kBinGammaBlocks, std::max<int64_t>(1, (most + 2047) / 2048));
+        k_track_bin_gamma_partials<<<dim3(nx, r.nbins), kThreads, 0, s>>>(bunch.px, bunch.py, bunch.pz, r,
+                                                                           partials);
+        k_track_bin_gamma_final<<<r.nbins, kReduceThreads, 0, s>>>(partials, nx, r, gammas);
+    }
+    k_track_bin_scalars<<<1, 64, 0, s>>>(sc, gammas, r, qi, exact ? 1 : 0, scb, scbc);
 }
 
 // bin keys of the resident bunch for its first bin-major order: the bin of

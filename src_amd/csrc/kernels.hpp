@@ -149,6 +149,11 @@ struct PoissonGrid {
 constexpr int kSlots = 64;           // doubles per block in a partial-sums buffer
 constexpr int kBoxStats = 12;        // per chunk: min[3], max[3], sum[3], sum of squares[3]
 constexpr int kOsBoxStats = 6;       // per one-shot gather block: min[3], max[3] of the positions written
+// particles per thread and per block of the one-shot gather passes (mixed, FP64)
+constexpr int kOsU = 4;
+constexpr int kOsChunk = kThreads * kOsU;
+constexpr int kOsU64 = 2;
+constexpr int kOsChunk64 = kThreads * kOsU64;
 
 // Particle buffers in structure-of-arrays form.
 struct Bunch {
@@ -239,10 +244,13 @@ int os_blocks(int64_t N);
 int os_blocks64(int64_t N);
 // osbox (used with next): per one-shot block the bounds (kOsBoxStats doubles) of the
 // positions written, which the next step's bounds come from.  Bg: the energy
-// bins' summed magnetic field grid (then B is gathered, not formed from E).
+// bins' summed magnetic field grid (then B is gathered, not formed from E);
+// with it and next, binp (not null) receives the chunk sums of bingamma.hpp
+// for the bin ranges *bins.
+struct BinRanges;
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                           const TrackState *ts, double *dense, double *partials, int ext, double *osbox,
-                          const void *Bg = nullptr);
+                          const void *Bg = nullptr, const BinRanges *bins = nullptr, double *binp = nullptr);
 // blocks of statistics launch_track_gamma_os leaves (one per one-shot block)
 int64_t os_box_count(bool fp64, int64_t N);
 // The bounds pass from the previous gather pass's statistics: the beam-frame
@@ -284,13 +292,18 @@ struct BinRanges {
     int64_t start[kMaxBins + 1] = {0};
 };
 constexpr int kBinGammaBlocks = 4096;  // at most this many partial-sum blocks per bin for calcGammas
-// per-bin calcGammas and the per-bin scalars scb[b] (partials: kMaxBins*kBinGammaBlocks*2 doubles)
+// per-bin calcGammas and the per-bin scalars scb[b], and those of the non-empty
+// bins in bin order in scbc.  binp: the chunk sums (chunk particles per chunk)
+// of bingamma.hpp; null: a pass over the bins' momenta (partials:
+// kMaxBins*kBinGammaBlocks*2 doubles)
 void launch_track_bin_scalars(hipStream_t s, const Bunch &bunch, const BinRanges &r, double qi, bool exact,
-                              const Scalars *sc, double *partials, double *gammas, Scalars *scb);
+                              const Scalars *sc, double *partials, const double *binp, int64_t chunk,
+                              double *gammas, Scalars *scb, Scalars *scbc);
+// The chunk sums of bingamma.hpp from the resident momenta (2 doubles per
+// one-shot chunk of the precision's gather pass).
+void launch_bin_gamma_chunks(hipStream_t s, bool fp64, const Bunch &bunch, const BinRanges &r, double *binp);
 void launch_bin_keys_by_id(hipStream_t s, int64_t N, const int32_t *bin, const uint32_t *id, int nbins,
                            uint32_t *keys, uint32_t *vals);
-// out[z] = scb[k] for the z-th non-empty bin k (the batched solve's scalars)
-void launch_compact_bin_scalars(hipStream_t s, const Scalars *scb, const BinRanges &r, Scalars *out);
 
 // Sorting (sort.hip)
 size_t sort_temp_bytes(int64_t N);

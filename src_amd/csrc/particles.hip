@@ -7,6 +7,7 @@
 // block order, and sums are carried as double-double so that the moment sums
 // are the correctly rounded sums of the per-particle terms.
 #include "common.hpp"
+#include "bingamma.hpp"
 #include "crmath.hpp"
 #include "frame.hpp"
 #include "igfmath.hpp"
@@ -17,11 +18,6 @@
 namespace opalsc {
 
 constexpr int kXcds = 8;             // XCDs of an MI355X; workgroups are dealt to them round-robin
-// particles per thread and per block of the one-shot gather passes (mixed, FP64)
-constexpr int kOsU = 4;
-constexpr int kOsChunk = kThreads * kOsU;
-constexpr int kOsU64 = 2;
-constexpr int kOsChunk64 = kThreads * kOsU64;
 
 int red_blocks(int64_t N) {
     int64_t b = (N + kThreads - 1) / kThreads;
@@ -745,6 +741,23 @@ __device__ inline void accumulate_moments(double (&m)[kMom], double x, double y,
         m[8 + 3 * d] += p * r;
     }
     m[15] += sqrt(1.0 + (px * px + py * py + pz * pz));
+}
+// The same, returning the particle's sqrt(1 + P.P) (the value added to m[15]).
+__device__ inline double accumulate_moments_g(double (&m)[kMom], double x, double y, double z, double px,
+                                              double py, double pz) {
+    const double part[6] = {x, px, y, py, z, pz};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[i] += part[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double r = part[2 * d], p = part[2 * d + 1];
+        m[6 + 3 * d] += r * r;
+        m[7 + 3 * d] += p * p;
+        m[8 + 3 * d] += p * r;
+    }
+    const double g = lorentz_gamma(px, py, pz);
+    m[15] += g;
+    return g;
 }
 
 __device__ inline void store_moment_partials(double (&m)[kMom], double *partials, double *lds) {
@@ -1679,9 +1692,11 @@ template <bool NEXT, int EXT, bool BINNED>
 __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double *x, double *y, double *z, double *px,
                                                               double *py, double *pz, const float4 *Eg,
                                                               const float4 *Bg, const Scalars *sc,
-                                                              const TrackState *ts, double *dense, double *osbox) {
+                                                              const TrackState *ts, double *dense, double *osbox,
+                                                              BinRanges br, double *binp) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
+    __shared__ double lgam[2 * (kThreads / 64)];
     const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
     const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
@@ -1689,6 +1704,7 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    double g0 = 0.0, g1 = 0.0;  // BINNED with binp: the chunk's sums of bingamma.hpp
     if (chunk < nchunks) {
         const bool sc_on = ts->space_charge != 0;
         const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
@@ -1721,6 +1737,9 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
             rr[u][4] = py[j];
             rr[u][5] = pz[j];
         }
+        int64_t split = 0;
+        if constexpr (BINNED && NEXT)
+            if (binp) split = bin_chunk_split(br, chunk * kOsChunk, N);
 #pragma unroll
         for (int u = 0; u < kOsU; ++u) {
             const int64_t i = base + u * kThreads;
@@ -1756,7 +1775,13 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
             transform_to_fast(Mc, oc, r);
             double pn[3];
             mat_vec_fast(Mc, pb, pn);
-            accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+            if constexpr (BINNED && NEXT) {
+                const double g = accumulate_moments_g(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+                if (i < split) g0 += g;
+                else g1 += g;
+            } else {
+                accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+            }
             if constexpr (NEXT) {
                 push_half_fast(r[0], r[1], r[2], pn[0], pn[1], pn[2], half_cdt);
                 for (int d = 0; d < 3; ++d) {
@@ -1793,6 +1818,8 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
                 }
         }
     }
+    if constexpr (BINNED && NEXT)
+        if (binp) bin_chunk_store(g0, g1, lgam, binp, chunk, chunk < nchunks);
 }
 
 // The FP64 gather pass (no DC beam or field maps; with BINNED the energy
@@ -1808,9 +1835,10 @@ template <bool NEXT, bool BINNED>
 __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, double *y, double *z, double *px,
                                                         double *py, double *pz, const double4 *Eg, const double4 *Bg,
                                                         const Scalars *sc, const TrackState *ts, double *dense,
-                                                        double *osbox) {
+                                                        double *osbox, BinRanges br, double *binp) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
+    __shared__ double lgam[2 * (kThreads / 64)];
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
     const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
@@ -1818,6 +1846,7 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    double g0 = 0.0, g1 = 0.0;  // BINNED with binp: the chunk's sums of bingamma.hpp
     if (chunk < nchunks) {
         const bool sc_on = ts->space_charge != 0;
         const double dT = ts->dT, charge = ts->charge, mass = ts->mass_eV;
@@ -1838,6 +1867,9 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
             rr[u][4] = py[j];
             rr[u][5] = pz[j];
         }
+        int64_t split = 0;
+        if constexpr (BINNED && NEXT)
+            if (binp) split = bin_chunk_split(br, chunk * kOsChunk64, N);
 #pragma unroll
         for (int u = 0; u < kOsU64; ++u) {
             const int64_t i = base + u * kThreads;
@@ -1880,7 +1912,13 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
             push_half(r[0], r[1], r[2], p[0], p[1], p[2], dT);
             transform_to(sc->M_u, sc->o_u, r);
             mat_vec(sc->M_u, p, pn);
-            accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+            if constexpr (BINNED && NEXT) {
+                const double g = accumulate_moments_g(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+                if (i < split) g0 += g;
+                else g1 += g;
+            } else {
+                accumulate_moments(m, r[0], r[1], r[2], pn[0], pn[1], pn[2]);
+            }
             if constexpr (NEXT) {
                 push_half(r[0], r[1], r[2], pn[0], pn[1], pn[2], dT);
                 for (int d = 0; d < 3; ++d) {
@@ -1917,6 +1955,8 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
                 }
         }
     }
+    if constexpr (BINNED && NEXT)
+        if (binp) bin_chunk_store(g0, g1, lgam, binp, chunk, chunk < nchunks);
 }
 
 // Dense per-block moment sums -> at most kOsReduceBlocks double-double partials
@@ -1950,19 +1990,22 @@ struct OsArgs {
     const Scalars *sc;
     const TrackState *ts;
     double *dense, *osbox;
+    const BinRanges *br;
+    double *binp;
 };
 template <bool NEXT, bool BINNED>
 void launch_os64(const OsArgs &a, int blocks) {
     k_track_gamma_os<NEXT, BINNED><<<blocks, kThreads, 0, a.s>>>(a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py,
                                                                   a.b->pz, (const double4 *)a.Eg,
                                                                   (const double4 *)a.Bg, a.sc, a.ts, a.dense,
-                                                                  NEXT ? a.osbox : nullptr);
+                                                                  NEXT ? a.osbox : nullptr, *a.br,
+                                                                  BINNED && NEXT ? a.binp : nullptr);
 }
 template <bool NEXT, int EXT, bool BINNED>
 void launch_os_mixed(const OsArgs &a, int blocks) {
     k_track_gamma_mixed_os<NEXT, EXT, BINNED><<<blocks, kThreads, 0, a.s>>>(
         a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const float4 *)a.Eg, (const float4 *)a.Bg, a.sc,
-        a.ts, a.dense, NEXT ? a.osbox : nullptr);
+        a.ts, a.dense, NEXT ? a.osbox : nullptr, *a.br, BINNED && NEXT ? a.binp : nullptr);
 }
 template <bool NEXT, int EXT>
 void launch_os_mixed_b(const OsArgs &a, int blocks) {
@@ -1973,8 +2016,9 @@ void launch_os_mixed_b(const OsArgs &a, int blocks) {
 
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                           const TrackState *ts, double *dense, double *partials, int ext, double *osbox,
-                          const void *Bg) {
-    const OsArgs a{s, &b, Eg, Bg, sc, ts, dense, osbox};
+                          const void *Bg, const BinRanges *bins, double *binp) {
+    static const BinRanges none;
+    const OsArgs a{s, &b, Eg, Bg, sc, ts, dense, osbox, bins ? bins : &none, bins ? binp : nullptr};
     if (fp64) {
         const int blocks64 = os_blocks64(b.N);
         if (next) {

@@ -468,6 +468,43 @@ def test_tracker_binned_matches_oracle(sort_interval):
             assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])
 
 
+@pytest.mark.parametrize("precision", [sc.FP64, sc.MIXED])
+@pytest.mark.parametrize("tiny_bin", [False, True])
+def test_tracker_binned_one_call_equals_single_steps(precision, tiny_bin):
+    """Energy bins with an empty bin and unbinned particles, re-sorted inside
+    the call.  Inside one call each bin's mean gamma comes from the chunk sums
+    the previous gather pass left; a call of one step sums the same chunks
+    itself.  With every non-empty bin larger than a gather chunk both agree bit
+    for bit; a bin smaller than a chunk makes every step sum over the bins'
+    momenta, which agrees as well."""
+    n, N, steps = (16, 16, 32), 120000, 5
+    R, P, Q = gaussian_bunch(N, seed=41, q_total=1e-9, sigma_p=(1e-3, 1e-3, 5e-3))
+    R[:, 2] += 0.005
+    rank = np.empty(N, np.int64)
+    rank[np.argsort(P[:, 2], kind="stable")] = np.arange(N)
+    # bins 0, 1, 3, 4 by pz rank (bin 2 empty), the highest-pz group unbinned
+    edges = np.array([50000, 80000, 100000, 100300 if tiny_bin else 117000, N])
+    bins = np.array([0, 1, 3, 4, -1], np.int32)[np.searchsorted(edges, rank, side="right")]
+    bins[np.random.default_rng(5).random(N) < 0.02] = -1
+    runs = []
+    for split in (False, True):
+        with ctx(n, precision=precision, N=N, sort_interval=2) as c:
+            c.set_bunch(R, P, Q)
+            c.set_bins(bins, 5, 1e-9 / N, image=True)
+            c.track_init(1e-11, Bz=1.0, Ez0=1e6, rf_freq=50e6, ref_p=(0.0, 0.0, 0.046181))
+            st = [c.track(1)[0] for _ in range(steps)] if split else c.track(steps)
+            runs.append((c.get_bunch(), st, c.get_bins()[1]))
+    (Ra, Pa, _), sa, ga = runs[0]
+    (Rb, Pb, _), sb, gb = runs[1]
+    counts = np.bincount(bins[bins >= 0], minlength=5)
+    assert counts[2] == 0 and (counts[[0, 1, 3, 4]].min() < 1024) == tiny_bin
+    assert np.array_equal(ga, gb)
+    assert np.array_equal(Ra, Rb) and np.array_equal(Pa, Pb)
+    for a, b in zip(sa, sb):
+        for k in ("rmean", "pmean", "rrms", "prms", "eps_norm"):
+            assert np.array_equal(a[k], b[k]), k
+
+
 def test_tracker_binned_without_image_and_mixed():
     R, P, ref, gpu, _ = _binned_track_case((32, 32, 32), 80000, 10, 3, precision=sc.MIXED, image=False)
     Ro, Po, _, _, so = ref
