@@ -371,6 +371,17 @@ int opalsc_device_log_cr(opalsc_ctx *ctx, int64_t n, const double *x, double *y)
  * evaluations of either form return the same bits. */
 int opalsc_host_igf_term(int64_t n, const double *v, int fast, double *y);
 int opalsc_device_igf_term(opalsc_ctx *ctx, int64_t n, const double *v, int fast, double *y);
+/* The real octant G[k][j][i], (n0+1)(n1+1)(n2+1) doubles x-fastest, of
+ * FFTPoissonSolver::integratedGreensFunction (table 0, with G[0,0,0] := G[0,0,1],
+ * src/Solvers/FFTPoissonSolver.cpp:459-522,657) or of the image charge's
+ * shiftedIntGreensFunction tables (1: z corners + zshift, 2: also - n2*h2;
+ * src/Solvers/FFTPoissonSolver.cpp:546-639) for spacing h.  Host: fast = 0 the
+ * FP64 solve's corner table with correctly rounded atan/log, fast = 1 the
+ * mixed solve's one-pass form (the 8-corner difference within 16 cells of the
+ * origin, the order-4 cell average of 1/r beyond).  Device: the context's
+ * own kernels (its precision picks the form) on its grid size. */
+int opalsc_host_green_octant(const int n[3], const double h[3], double zshift, int table, int fast, double *out);
+int opalsc_device_green_octant(opalsc_ctx *ctx, const double h[3], double zshift, int table, double *out);
 
 /* --- timing --------------------------------------------------------------
  * Per-kernel HIP-event timing of the tracker, recorded on the context's

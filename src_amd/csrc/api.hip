@@ -350,7 +350,7 @@ extern "C" int opalsc_compute_potential_dev(opalsc_ctx *c, double *rho_dev, cons
     if (!is_device_range(rho_dev, nc * 8)) FAIL(c, OPALSC_EINVAL, "rho_dev is not device memory");
     potential_scalars(c, hr);
     HIPCHK(c, hipMemcpyAsync(c->grid.rho_f64, rho_dev, nc * 8, hipMemcpyDeviceToDevice, c->stream));
-    c->grid.green(c->d_sc, c->grid.fp64);
+    c->grid.green(c->d_sc);
     c->grid.solve(c->d_sc, 1, 0);
     CHECK_LAUNCH(c);
     c->grid.widen(1, rho_dev);
@@ -375,7 +375,7 @@ extern "C" int opalsc_compute_potential(opalsc_ctx *c, double *rho, const double
     upload_scalars(c, s);
     const size_t nc = c->grid.ncell;
     HIPCHK(c, hipMemcpyAsync(c->grid.rho_f64, rho, nc * 8, hipMemcpyHostToDevice, c->stream));
-    c->grid.green(c->d_sc, c->grid.fp64);
+    c->grid.green(c->d_sc);
     c->grid.solve(c->d_sc, 1, 0);
     CHECK_LAUNCH(c);
     c->grid.widen(1, (double *)c->grid.rho_f64);
@@ -403,7 +403,7 @@ extern "C" int opalsc_compute_potential_shifted(opalsc_ctx *c, double *rho, cons
     upload_scalars(c, s);
     const size_t nc = c->grid.ncell;
     HIPCHK(c, hipMemcpyAsync(c->grid.rho_f64, rho, nc * 8, hipMemcpyHostToDevice, c->stream));
-    c->grid.green_image(c->d_sc, c->grid.fp64);
+    c->grid.green_image(c->d_sc);
     c->grid.solve(c->d_sc, 1, 0, true);
     CHECK_LAUNCH(c);
     c->grid.widen(1, (double *)c->grid.rho_f64);
@@ -654,7 +654,7 @@ static int self_fields_core(opalsc_ctx *c, int64_t N, const double *R, const dou
     c->grid.zero_rho(c->stream);
     launch_deposit(c->stream, 0, c->grid.fp64, N, b.x, b.y, b.z, nullptr, b.qd, (long long *)c->grid.rho_i64,
                    c->d_sc, &c->d_counters[1], nullptr);
-    c->grid.green(c->d_sc, c->grid.fp64);
+    c->grid.green(c->d_sc);
     c->grid.solve(c->d_sc, 0, 1);
     c->grid.grad(c->d_sc);
     launch_gather_selffield(c->stream, c->grid.fp64, N, b.x, b.y, b.z, c->grid.Eg, c->d_sc, Ef, Bf);
@@ -782,7 +782,7 @@ extern "C" int opalsc_compute_self_fields_binned(opalsc_ctx *c, int64_t N, const
                            c->grid.bin_rho(z), scz + z, &c->d_counters[1], nullptr);
             ++z;
         }
-        c->grid.green_bins(scz, nz, exact, image != 0, c->stream);
+        c->grid.green_bins(scz, nz, image != 0, c->stream);
         c->grid.solve_bins(scz, nz, image != 0, nullptr);
         c->grid.field_sums(scz, nz, image != 0);
         CHECK_LAUNCH(c);
@@ -1163,7 +1163,6 @@ static int prepare_track(opalsc_ctx *c) {
 // bunches side by side).
 static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, bool overlap, bool boxes) {
     Bunch &b = c->bunch;
-    const bool cr = c->grid.fp64;     // correctly rounded IGF on the FP64 path
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     const bool binned = c->bins.nbins > 0;
     unsigned long long *bkeys = &c->d_counters[8];
@@ -1207,7 +1206,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             if (ov) {
                 (void)hipEventRecord(c->ev_mesh, c->stream);
                 (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
-                c->grid.green_bins(scz, nz, cr, c->bin_image, c->stream2);
+                c->grid.green_bins(scz, nz, c->bin_image, c->stream2);
                 (void)hipEventRecord(c->ev_green, c->stream2);
             }
             if (nz > 0) c->grid.zero_bin_rho(nz, c->stream);
@@ -1221,7 +1220,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             }
             c->prof.mark(PROF_DEPOSIT);
             if (nz > 0) {
-                if (!ov) c->grid.green_bins(scz, nz, cr, c->bin_image, c->stream);
+                if (!ov) c->grid.green_bins(scz, nz, c->bin_image, c->stream);
                 c->grid.solve_bins(scz, nz, c->bin_image, ov ? c->ev_green : nullptr);
                 c->grid.field_sums(scz, nz, c->bin_image);
             } else {
@@ -1236,7 +1235,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             if (overlap) {
                 (void)hipEventRecord(c->ev_mesh, c->stream);
                 (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
-                c->grid.green(c->d_sc, cr, c->stream2);
+                c->grid.green(c->d_sc, c->stream2);
                 (void)hipEventRecord(c->ev_green, c->stream2);
             }
             c->grid.zero_rho(c->stream);
@@ -1244,7 +1243,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             launch_deposit(c->stream, c->q_uniform ? 3 : 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr,
                            (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
             c->prof.mark(PROF_DEPOSIT);
-            if (!overlap) c->grid.green(c->d_sc, cr);
+            if (!overlap) c->grid.green(c->d_sc);
             c->grid.solve(c->d_sc, 0, 1, false, overlap ? c->ev_green : nullptr);
             c->grid.grad(c->d_sc);
         }
@@ -1620,6 +1619,72 @@ extern "C" int opalsc_host_igf_term(int64_t n, const double *v, int fast, double
                     : igf::term_cr(v[3 * i], v[3 * i + 1], v[3 * i + 2], 1.0);
     return OPALSC_OK;
 }
+extern "C" int opalsc_host_green_octant(const int n[3], const double h[3], double zshift, int table, int fast,
+                                        double *out) {
+    if (!n || !h || !out || table < 0 || table > 2 || n[0] < 1 || n[1] < 1 || n[2] < 1) return OPALSC_EINVAL;
+    const int g0 = n[0] + 1, g1 = n[1] + 1, g2 = n[2] + 1;
+    const double vol = h[0] * h[1] * h[2], ivol = 1.0 / vol;
+    const bool shifted = table != 0, second = table == 2;
+    if (fast) {
+        const igf::GreenMoments m = igf::green_moments(h[0], h[1], h[2]);
+        for (int k = 0; k < g2; ++k)
+            for (int j = 0; j < g1; ++j)
+                for (int i = 0; i < g0; ++i) {
+                    const int kq = (!shifted && i == 0 && j == 0 && k == 0) ? 1 : k;
+                    out[((size_t)k * g1 + j) * g0 + i] =
+                        igf::green_point_fast(m, i, j, kq, h[0], h[1], h[2], zshift, shifted, second, n[2], ivol);
+                }
+        return OPALSC_OK;
+    }
+    // the corner table and its 8-corner difference, as the FP64 kernels evaluate them
+    const int m0 = n[0] + 2, m1 = n[1] + 2, m2 = n[2] + 2;
+    std::vector<double> T((size_t)m0 * m1 * m2);
+    for (int k = 0; k < m2; ++k)
+        for (int j = 0; j < m1; ++j)
+            for (int i = 0; i < m0; ++i) {
+                const double v0 = i * h[0] - h[0] / 2, v1 = j * h[1] - h[1] / 2;
+                double v2 = k * h[2] - h[2] / 2;
+                if (shifted) {
+                    v2 = v2 + zshift;
+                    if (second) v2 = v2 - n[2] * h[2];
+                }
+                T[((size_t)k * m1 + j) * m0 + i] = igf::term_cr(v0, v1, v2, vol);
+            }
+    auto t = [&](int i, int j, int k) { return T[((size_t)k * m1 + j) * m0 + i]; };
+    for (int k = 0; k < g2; ++k)
+        for (int j = 0; j < g1; ++j)
+            for (int i = 0; i < g0; ++i) {
+                double g = t(i + 1, j + 1, k + 1);
+                g += t(i + 1, j, k);
+                g += t(i, j + 1, k);
+                g += t(i, j, k + 1);
+                g -= t(i + 1, j + 1, k);
+                g -= t(i + 1, j, k + 1);
+                g -= t(i, j + 1, k + 1);
+                g -= t(i, j, k);
+                out[((size_t)k * g1 + j) * g0 + i] = g;
+            }
+    if (!shifted) out[0] = out[(size_t)g1 * g0];  // G[0,0,0] := G[0,0,1]
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_device_green_octant(opalsc_ctx *c, const double h[3], double zshift, int table, double *out) {
+    if (!c || !h || !out || table < 0 || table > 2) return OPALSC_EINVAL;
+    if (int e_ = use_device(c)) return e_;
+    if (c->grid.greens == 1) FAIL(c, OPALSC_EINVAL, "the context's Green's function is not the integrated one");
+    if (table != 0 && c->grid.pz) FAIL(c, OPALSC_EINVAL, "a periodic-z context has no image-charge tables");
+    Scalars s;
+    std::memset(&s, 0, sizeof(s));
+    for (int d = 0; d < 3; ++d) s.hs[d] = h[d];
+    s.zshift = zshift;
+    if (int e = upload(c, c->aosR, &s, sizeof(s))) return e;
+    c->grid.octant(c->aosR.as<Scalars>(), table, c->stream);
+    CHECK_LAUNCH(c);
+    HIPCHK(c, hipMemcpyAsync(out, c->grid.Goct, c->grid.noct * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OPALSC_OK;
+}
+
 extern "C" int opalsc_device_igf_term(opalsc_ctx *c, int64_t n, const double *v, int fast, double *y) {
     if (!c || n < 0 || (n > 0 && (!v || !y))) return OPALSC_EINVAL;
     if (int e_ = use_device(c)) return e_;

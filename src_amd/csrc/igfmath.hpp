@@ -181,5 +181,97 @@ __host__ __device__ inline double term_fast(double x, double y, double z, double
     return t * inv_volume;
 }
 
+// ---------------------------------------------------------------------------
+// The Green's function value itself, for the mixed-precision solve
+// ---------------------------------------------------------------------------
+// G at a grid point is the 8-corner difference of F / cellVolume, i.e. the
+// average of 1/|r| over the cell of size h0 x h1 x h2 centred at c.  Far from
+// the origin the average has the Taylor expansion (odd moments vanish,
+// <s_d^2> = h_d^2/12, <s_d^4> = h_d^4/80)
+//   <1/r> = 1/r + sum_d h_d^2/24 d_dd(1/r) + sum_d h_d^4/1920 d_dddd(1/r)
+//           + sum_{d<e} h_d^2 h_e^2/576 d_ddee(1/r) + O((h/r)^6 / r),
+//   d_dd(1/r)   = (3 x_d^2 - r^2) / r^5,
+//   d_dddd(1/r) = 3 (35 x_d^4 - 30 x_d^2 r^2 + 3 r^4) / r^9,
+//   d_ddee(1/r) = (105 x_d^2 x_e^2 - 15 r^2 (x_d^2 + x_e^2) + 3 r^4) / r^9.
+// At |c| >= kGreenFarCells * max(h) its relative error is below 2e-10 (and
+// falls as (h/r)^6), smaller than the rounding the 8-corner difference of
+// FP64 corner terms keeps there (~2e-15 (r/h)^3: 8e-12 at 16 cells, 3e-8 at
+// 256); nearer cells take the exact difference.
+constexpr double kGreenFarCells = 16.0;
+
+struct GreenMoments {
+    double a[3];  // h_d^2/24
+    double b[3];  // 3 h_d^4/1920
+    double c[3];  // h_0^2 h_1^2/576, h_0^2 h_2^2/576, h_1^2 h_2^2/576
+    double far2;  // (kGreenFarCells * max h)^2
+};
+__host__ __device__ inline GreenMoments green_moments(double h0, double h1, double h2) {
+    GreenMoments m;
+    const double hs[3] = {h0 * h0, h1 * h1, h2 * h2};
+    for (int d = 0; d < 3; ++d) {
+        m.a[d] = hs[d] / 24;
+        m.b[d] = 3 * hs[d] * hs[d] / 1920;
+    }
+    m.c[0] = hs[0] * hs[1] / 576;
+    m.c[1] = hs[0] * hs[2] / 576;
+    m.c[2] = hs[1] * hs[2] / 576;
+    const double hm = fmax(h0, fmax(h1, h2)) * kGreenFarCells;
+    m.far2 = hm * hm;
+    return m;
+}
+
+// The order-4 cell average of 1/r for a cell centred at (x, y, z).
+__host__ __device__ inline double green_far(const GreenMoments &m, double x, double y, double z) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z, r2 = x2 + y2 + z2;
+    const double ir = 1.0 / sqrt(r2), ir2 = ir * ir;
+    const double ir5 = ir * ir2 * ir2, ir9 = ir5 * ir2 * ir2;
+    const double r4 = r2 * r2;
+    const double s2 = m.a[0] * (3 * x2 - r2) + m.a[1] * (3 * y2 - r2) + m.a[2] * (3 * z2 - r2);
+    double s4 = m.b[0] * (35 * x2 * x2 - 30 * x2 * r2 + 3 * r4);
+    s4 += m.b[1] * (35 * y2 * y2 - 30 * y2 * r2 + 3 * r4);
+    s4 += m.b[2] * (35 * z2 * z2 - 30 * z2 * r2 + 3 * r4);
+    s4 += m.c[0] * (105 * x2 * y2 - 15 * r2 * (x2 + y2) + 3 * r4);
+    s4 += m.c[1] * (105 * x2 * z2 - 15 * r2 * (x2 + z2) + 3 * r4);
+    s4 += m.c[2] * (105 * y2 * z2 - 15 * r2 * (y2 + z2) + 3 * r4);
+    return ir + (s2 * ir5 + s4 * ir9);
+}
+
+// G at octant point (i, j, k) of the mixed-precision solve: zshift = 0 and
+// second = false for FFTPoissonSolver::integratedGreensFunction (the caller
+// applies the G[0,0,0] := G[0,0,1] quirk), otherwise the table of
+// shiftedIntGreensFunction (FFTPoissonSolver.cpp:552-603) with z corners
+// (k h2 - h2/2) + zshift [- n2 h2].  Near cells: the 8-corner difference of
+// the corner terms in the order of FFTPoissonSolver.cpp:510-517.
+__host__ __device__ inline double green_point_fast(const GreenMoments &m, int i, int j, int k, double h0, double h1,
+                                                   double h2, double zshift, bool shifted, bool second, int n2,
+                                                   double inv_volume) {
+    const double cx = i * h0, cy = j * h1;
+    double cz = k * h2;
+    if (shifted) {
+        cz = cz + zshift;
+        if (second) cz = cz - n2 * h2;
+    }
+    if (cx * cx + cy * cy + cz * cz >= m.far2) return green_far(m, cx, cy, cz);
+    // the corners in the reference's sum order +(1,1,1) +(1,0,0) +(0,1,0) +(0,0,1)
+    // -(1,1,0) -(1,0,1) -(0,1,1) -(0,0,0), one nibble (dx dy dz) each; a rolled
+    // loop, so this rare path does not size the registers of the common one
+    constexpr unsigned kOrder = 0x03561247u;
+    double g = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        const unsigned c = (kOrder >> (4 * q)) & 7u;
+        const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
+        const double v0 = (i + dx) * h0 - h0 / 2, v1 = (j + dy) * h1 - h1 / 2;
+        double v2 = (k + dz) * h2 - h2 / 2;
+        if (shifted) {
+            v2 = v2 + zshift;
+            if (second) v2 = v2 - n2 * h2;
+        }
+        const double t = term_fast(v0, v1, v2, inv_volume);
+        g = q < 4 ? g + t : g - t;
+    }
+    return g;
+}
+
 }  // namespace igf
 }  // namespace opalsc

@@ -106,9 +106,13 @@ struct PoissonGrid {
 
     int init(const int nn[3], bool fp64, int greens, bool periodic_z, hipStream_t s, std::string &err);
     void destroy();
-    void green(const Scalars *d_sc, bool correctly_rounded, hipStream_t s = nullptr);  // s: default stream
+    // the FP64 grid evaluates the reference's corner table with correctly rounded
+    // atan/log, the mixed grid the one-pass octant (igfmath.hpp)
+    void green(const Scalars *d_sc, hipStream_t s = nullptr);  // s: default stream
+    // only the real octant G of table 0 (plain) or 1/2 (the image charge's) into Goct
+    void octant(const Scalars *d_sc, int table, hipStream_t s = nullptr);
     int ensure_image(std::string &err);
-    void green_image(const Scalars *d_sc, bool correctly_rounded, hipStream_t s = nullptr);  // shift: Scalars::zshift
+    void green_image(const Scalars *d_sc, hipStream_t s = nullptr);  // shift: Scalars::zshift
     // src_kind 0: int64 fixed-point rho_i64 (self-field scaling); 1: rho_f64 as is
     // outscale 1: phi *= vol*coupling (self fields); 0: potential only
     // image: convolve with the image-charge Green's function (negated, z flipped)
@@ -135,7 +139,7 @@ struct PoissonGrid {
     long long *bin_rho(int b) const { return (long long *)bb_rho + (size_t)b * rho_words(); }
     void zero_bin_rho(int nb, hipStream_t s);
     // scb: nb Scalars, one per grid of the batch
-    void green_bins(const Scalars *scb, int nb, bool cr, bool image, hipStream_t s);
+    void green_bins(const Scalars *scb, int nb, bool image, hipStream_t s);
     void solve_bins(const Scalars *scb, int nb, bool image, hipEvent_t green_ready);
     // Esum, Bsum := the sums over the nb grids (and their images) of E and of
     // the magnetic field it implies (written, not accumulated)

@@ -33,53 +33,41 @@ namespace opalsc {
 // Green's function octant (FP64 in both precisions)
 // ---------------------------------------------------------------------------
 
-// The IGF antiderivative at corner (v0, v1, v2): FFTPoissonSolver.cpp:479-489
-// (and, with shifted v2, FFTPoissonSolver.cpp:563-573, 589-599); CR: the
-// reference's operations with correctly rounded atan and log (FP64 solve),
-// otherwise the reduced-instruction form of the mixed-precision solve.
-template <bool CR>
-__device__ inline double igf_term(double v0, double v1, double v2, double cellVolume, double inv_volume) {
-    if constexpr (CR) return igf::term_cr(v0, v1, v2, cellVolume);
-    else return igf::term_fast(v0, v1, v2, inv_volume);
-}
-
-// FFTPoissonSolver::integratedGreensFunction, FFTPoissonSolver.cpp:472-491
-template <bool CR>
-__device__ inline double igf_corner(int i, int j, int k, double h0, double h1, double h2,
-                                    double cellVolume, double inv_volume) {
+// FFTPoissonSolver::integratedGreensFunction, FFTPoissonSolver.cpp:472-491: the
+// corner term (igf::term_cr, the reference's operations with correctly rounded
+// atan and log) at corner (i, j, k)
+__device__ inline double igf_corner(int i, int j, int k, double h0, double h1, double h2, double cellVolume) {
     double v0 = i * h0 - h0 / 2;
     double v1 = j * h1 - h1 / 2;
     double v2 = k * h2 - h2 / 2;
-    return igf_term<CR>(v0, v1, v2, cellVolume, inv_volume);
+    return igf::term_cr(v0, v1, v2, cellVolume);
 }
 
 // Corner antiderivatives T[k][j][i], i,j,k in [0, n+1], each evaluated once
-// (FFTPoissonSolver.cpp:472-491; the reference's tmpgreen field).  Grid:
-// x over the (n0+2)*(n1+2) corners of a plane, y over k.
-template <bool CR>
+// (FFTPoissonSolver.cpp:472-491; the reference's tmpgreen field) for the FP64
+// solve.  Grid: x over the (n0+2)*(n1+2) corners of a plane, y over k.
 __global__ __launch_bounds__(256) void k_igf_corners(double *T, int n0, int n1, int n2,
                                                      const Scalars *sc, size_t tB) {
     sc += blockIdx.z;  // batch: one table per grid, tB doubles apart
     T += blockIdx.z * tB;
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2];
-    const double vol = h0 * h1 * h2, ivol = 1.0 / vol;
+    const double vol = h0 * h1 * h2;
     const int m0 = n0 + 2, plane = m0 * (n1 + 2);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= plane) return;
     const int j = div_small(c, m0, 1.0f / (float)m0), i = c - j * m0, k = blockIdx.y;
-    T[(size_t)k * plane + c] = igf_corner<CR>(i, j, k, h0, h1, h2, vol, ivol);
+    T[(size_t)k * plane + c] = igf_corner(i, j, k, h0, h1, h2, vol);
 }
 
 // The two corner tables of FFTPoissonSolver::shiftedIntGreensFunction
 // (FFTPoissonSolver.cpp:552-603): z coordinate (k*h2 - h2/2) + zshift, and for
-// the second table also - n2*h2, evaluated left to right as there.
-template <bool CR>
+// the second table also - n2*h2, evaluated left to right as there (FP64 solve).
 __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, int n1, int n2,
                                                              const Scalars *sc, int second, size_t tB) {
     sc += blockIdx.z;
     T += blockIdx.z * tB;
     const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2], zshift = sc->zshift;
-    const double vol = h0 * h1 * h2, ivol = 1.0 / vol;
+    const double vol = h0 * h1 * h2;
     const int m0 = n0 + 2, plane = m0 * (n1 + 2);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= plane) return;
@@ -88,7 +76,7 @@ __global__ __launch_bounds__(256) void k_igf_corners_shifted(double *T, int n0, 
     double v1 = j * h1 - h1 / 2;
     double v2 = k * h2 - h2 / 2 + zshift;
     if (second) v2 = v2 - n2 * h2;
-    T[(size_t)k * plane + c] = igf_term<CR>(v0, v1, v2, vol, ivol);
+    T[(size_t)k * plane + c] = igf::term_cr(v0, v1, v2, vol);
 }
 
 // G octant [0,n0]x[0,n1]x[0,n2] from T: the 8-corner difference in the
@@ -157,6 +145,33 @@ __global__ __launch_bounds__(256) void k_green_standard(double *G, int n0, int n
         double g = (double)(i * i) * hsq0 + (double)(j * j) * hsq1 + (double)(k * k) * hsq2;
         G[q] = 1.0 / sqrt(g);
     }
+}
+
+// The mixed-precision solve's octant in one pass, without a corner table:
+// igf::green_point_fast at every point (the exact 8-corner difference near
+// the origin, the cell-average expansion elsewhere).  shifted: the image
+// charge's tables (second: the one shifted by a further -n2*h2), which have no
+// G[0,0,0] quirk.  Grid: x over the (n0+1)*(n1+1) points of a plane, y over k,
+// z over the batch's grids (gB doubles apart).
+__global__ __launch_bounds__(256) void k_green_fast(double *G, int n0, int n1, int n2, const Scalars *sc,
+                                                   int shifted, int second, size_t gB) {
+    sc += blockIdx.z;
+    G += blockIdx.z * gB;
+    const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2], zshift = sc->zshift;
+    const double ivol = 1.0 / (h0 * h1 * h2);
+    const igf::GreenMoments m = igf::green_moments(h0, h1, h2);
+    const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= gplane) return;
+    const int j = div_small(c, g0, 1.0f / (float)g0), i = c - j * g0, k = blockIdx.y;
+    const int kq = (!shifted && c == 0 && k == 0) ? 1 : k;  // G[0,0,0] := G[0,0,1] (FFTPoissonSolver.cpp:657)
+    G[(size_t)k * gplane + c] =
+        igf::green_point_fast(m, i, j, kq, h0, h1, h2, zshift, shifted != 0, second != 0, n2, ivol);
+}
+void launch_green_fast(hipStream_t s, double *G, const int n[3], const Scalars *sc, int shifted, int second,
+                       int nb, size_t gB) {
+    dim3 grid(((n[0] + 1) * (n[1] + 1) + 255) / 256, n[2] + 1, nb);
+    k_green_fast<<<grid, 256, 0, s>>>(G, n[0], n[1], n[2], sc, shifted, second, gB);
 }
 
 // ---------------------------------------------------------------------------
@@ -766,7 +781,7 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
     };
     if (!alloc(&rho_i64, rho_words() * 8) || !alloc(&rho_f64, ncell * 8) || !alloc(&A, szA) ||
         !alloc(&B, szB) || !alloc(&Goct, noct * 8) ||
-        !alloc(&Tcorner, (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8) || !alloc(&Ghat, szG) ||
+        (fp64 && !alloc(&Tcorner, (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8)) || !alloc(&Ghat, szG) ||
         !alloc(&phi, ncell * rs) || !alloc(&Eg, ncell * 4 * rs))
         return -1;
     // padding columns are read as whole tiles and must hold (and keep) zeros
@@ -867,18 +882,32 @@ static void dct_xy(PoissonGrid &g, const GridSet &gs, const double *G, T *H, hip
     }
 }
 
+// The IGF octant of table 0 (integratedGreensFunction, with the G[0,0,0] quirk)
+// or of the image charge's table 1 / 2 (shiftedIntGreensFunction's first and
+// second corner tables) into gs.Goct: on the FP64 grid the corner table with
+// correctly rounded atan/log and its 8-corner difference, on the mixed grid
+// the one-pass octant.
+static void igf_octant(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, int table, hipStream_t st) {
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    if (!g.fp64) {
+        launch_green_fast(st, gs.Goct, g.n, d_sc, table != 0, table == 2, gs.nb, gs.GB);
+        return;
+    }
+    dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2, gs.nb);
+    if (table == 0) k_igf_corners<<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, gs.TB);
+    else k_igf_corners_shifted<<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, table == 2, gs.TB);
+    dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK, gs.nb);
+    k_igf_diff<<<grid2, 256, 0, st>>>(gs.T, gs.Goct, n0, n1, n2, table == 0, gs.TB, gs.GB);
+}
+
 template <typename T>
-static void green_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, bool cr, hipStream_t st) {
+static void green_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, hipStream_t st) {
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const bool igf = g.greens != 1;
     if (!igf) {
         k_green_standard<<<dim3(1024, 1, gs.nb), 256, 0, st>>>(gs.Goct, n0, n1, n2, d_sc, gs.GB);
     } else {
-        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2, gs.nb);
-        if (cr) k_igf_corners<true><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, gs.TB);
-        else k_igf_corners<false><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, gs.TB);
-        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK, gs.nb);
-        k_igf_diff<<<grid2, 256, 0, st>>>(gs.T, gs.Goct, n0, n1, n2, 1, gs.TB, gs.GB);
+        igf_octant(g, gs, d_sc, 0, st);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
@@ -905,17 +934,13 @@ static void green_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, b
 // FFTPoissonSolver.cpp:546-639) for the shift in the scalars: both corner
 // tables, their differences, DCT-I in x and y, complex FFT along the doubled z.
 template <typename T>
-static void green_image_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, bool cr, hipStream_t st) {
+static void green_image_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, hipStream_t st) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     T *H[2] = {(T *)gs.HA, (T *)gs.HB};
     for (int second = 0; second < 2; ++second) {
-        dim3 grid(((n0 + 2) * (n1 + 2) + 255) / 256, n2 + 2, gs.nb);
-        if (cr) k_igf_corners_shifted<true><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, second, gs.TB);
-        else k_igf_corners_shifted<false><<<grid, 256, 0, st>>>(gs.T, n0, n1, n2, d_sc, second, gs.TB);
-        dim3 grid2(((n0 + 1) * (n1 + 1) + 255) / 256, (n2 + kIgfDiffK) / kIgfDiffK, gs.nb);
-        k_igf_diff<<<grid2, 256, 0, st>>>(gs.T, gs.Goct, n0, n1, n2, 0, gs.TB, gs.GB);
+        igf_octant(g, gs, d_sc, 1 + second, st);
         dct_xy<T>(g, gs, gs.Goct, H[second], st);
     }
     g.profile_mark(PROF_GREEN);
@@ -928,11 +953,16 @@ static void green_image_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d
     g.profile_mark(PROF_DCT);
 }
 
-void PoissonGrid::green(const Scalars *d_sc, bool cr, hipStream_t s) {
+void PoissonGrid::octant(const Scalars *d_sc, int table, hipStream_t s) {
+    if (!s) s = stream;
+    igf_octant(*this, own_set(*this, 0), d_sc, table, s);
+}
+
+void PoissonGrid::green(const Scalars *d_sc, hipStream_t s) {
     if (!s) s = stream;
     const GridSet gs = own_set(*this, 0);
-    if (fp64) green_impl<double>(*this, gs, d_sc, cr, s);
-    else green_impl<float>(*this, gs, d_sc, cr, s);
+    if (fp64) green_impl<double>(*this, gs, d_sc, s);
+    else green_impl<float>(*this, gs, d_sc, s);
 }
 
 int PoissonGrid::ensure_image(std::string &err) {
@@ -954,11 +984,11 @@ int PoissonGrid::ensure_image(std::string &err) {
     return 0;
 }
 
-void PoissonGrid::green_image(const Scalars *d_sc, bool cr, hipStream_t s) {
+void PoissonGrid::green_image(const Scalars *d_sc, hipStream_t s) {
     if (!s) s = stream;
     const GridSet gs = own_set(*this, 0);
-    if (fp64) green_image_impl<double>(*this, gs, d_sc, cr, s);
-    else green_image_impl<float>(*this, gs, d_sc, cr, s);
+    if (fp64) green_image_impl<double>(*this, gs, d_sc, s);
+    else green_image_impl<float>(*this, gs, d_sc, s);
 }
 
 // The convolution passes (see the top of this file) on a GridSet.
@@ -1069,7 +1099,8 @@ int PoissonGrid::ensure_bins(int nb, bool image, std::string &err) {
     bin_img_cap = false;
     const size_t rs = fp64 ? 8 : 4, cs = 2 * rs, m = (size_t)nb;
     const size_t szA = (size_t)kp * n[1] * n[2] * cs, szB = (size_t)kp * L[1] * n[2] * cs;
-    const size_t szT = (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8, szGh = (size_t)gp * (n[1] + 1) * (n[2] + 1) * rs;
+    // the corner table only for the FP64 solve (the mixed one evaluates G in one pass)
+    const size_t szT = fp64 ? (size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 8 : 0, szGh = (size_t)gp * (n[1] + 1) * (n[2] + 1) * rs;
     const size_t szGi = (size_t)L[2] * (n[1] + 1) * gp * cs;
     struct {
         void **p;
@@ -1125,15 +1156,15 @@ static GridSet bin_set(PoissonGrid &g, int nb) {
 
 void PoissonGrid::zero_bin_rho(int nb, hipStream_t s) { launch_fill_i64(s, (size_t)nb * rho_words(), (long long *)bb_rho); }
 
-void PoissonGrid::green_bins(const Scalars *scb, int nb, bool cr, bool image, hipStream_t s) {
+void PoissonGrid::green_bins(const Scalars *scb, int nb, bool image, hipStream_t s) {
     if (!s) s = stream;
     const GridSet gs = bin_set(*this, nb);
     if (fp64) {
-        green_impl<double>(*this, gs, scb, cr, s);
-        if (image) green_image_impl<double>(*this, gs, scb, cr, s);
+        green_impl<double>(*this, gs, scb, s);
+        if (image) green_image_impl<double>(*this, gs, scb, s);
     } else {
-        green_impl<float>(*this, gs, scb, cr, s);
-        if (image) green_image_impl<float>(*this, gs, scb, cr, s);
+        green_impl<float>(*this, gs, scb, s);
+        if (image) green_image_impl<float>(*this, gs, scb, s);
     }
 }
 

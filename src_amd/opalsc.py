@@ -43,7 +43,8 @@ EXPORTS = [
     "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init",
     "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
-    "opalsc_device_log_cr", "opalsc_host_igf_term", "opalsc_device_igf_term", "opalsc_profile_enable", "opalsc_profile_read",
+    "opalsc_device_log_cr", "opalsc_host_igf_term", "opalsc_device_igf_term",
+    "opalsc_host_green_octant", "opalsc_device_green_octant", "opalsc_profile_enable", "opalsc_profile_read",
     "opalsc_copy_bandwidth", "opalsc_fromfile_count", "opalsc_fromfile_read", "opalsc_stat_row_get",
     "opalsc_stat_write", "opalsc_fieldmap_add", "opalsc_fieldmap_clear", "opalsc_fieldmap_eval",
 ]
@@ -158,6 +159,8 @@ def lib():
         "opalsc_device_log_cr": ([vp, i64, dp, dp], C.c_int),
         "opalsc_host_igf_term": ([i64, dp, C.c_int, dp], C.c_int),
         "opalsc_device_igf_term": ([vp, i64, dp, C.c_int, dp], C.c_int),
+        "opalsc_host_green_octant": ([C.POINTER(C.c_int), dp, C.c_double, C.c_int, C.c_int, dp], C.c_int),
+        "opalsc_device_green_octant": ([vp, dp, C.c_double, C.c_int, dp], C.c_int),
         "opalsc_profile_enable": ([vp, C.c_int], C.c_int),
         "opalsc_profile_read": ([vp, C.c_int, C.POINTER(C.c_char_p), dp, dp], C.c_int),
         "opalsc_copy_bandwidth": ([vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
@@ -241,6 +244,16 @@ def host_igf_term(v, fast):
     y = np.empty(len(v))
     _check(None, lib().opalsc_host_igf_term(len(v), _ptr(v), 1 if fast else 0, _ptr(y)))
     return y
+
+
+def host_green_octant(n, h, zshift=0.0, table=0, fast=False):
+    """The real Green's function octant G[k, j, i] of an n-cell grid with
+    spacing h on the host (table 0 plain, 1 and 2 the image charge's)."""
+    nn = (C.c_int * 3)(*[int(v) for v in n])
+    hh = np.ascontiguousarray(h, dtype=np.float64)
+    out = np.empty((int(n[2]) + 1, int(n[1]) + 1, int(n[0]) + 1))
+    _check(None, lib().opalsc_host_green_octant(nn, _ptr(hh), float(zshift), int(table), 1 if fast else 0, _ptr(out)))
+    return out
 
 
 def _check(ctx, rc):
@@ -409,6 +422,12 @@ class SpaceChargeContext:
         y = np.empty_like(x)
         self._c(lib().opalsc_device_log_cr(self._h, len(x), _ptr(x), _ptr(y)))
         return y
+
+    def device_green_octant(self, h, zshift=0.0, table=0):
+        hh = np.ascontiguousarray(h, dtype=np.float64)
+        out = np.empty((self.n[2] + 1, self.n[1] + 1, self.n[0] + 1))
+        self._c(lib().opalsc_device_green_octant(self._h, _ptr(hh), float(zshift), int(table), _ptr(out)))
+        return out
 
     def device_igf_term(self, v, fast):
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)

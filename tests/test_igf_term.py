@@ -99,3 +99,68 @@ def test_device_corner_term_equals_host_bits(fast):
     v = np.concatenate(vs + [np.array([[1e-5, 2e-5, 0.0], [1e-5, -2e-5, -0.0]])])
     with sc.SpaceChargeContext((8, 8, 8), max_particles=16) as c:
         np.testing.assert_array_equal(c.device_igf_term(v, fast), sc.host_igf_term(v, fast))
+
+
+def _G_longdouble(n, h, zshift, table):
+    """The octant from the 8-corner difference of the corner term in 80-bit
+    long double (the cell average of 1/r to ~1e-19 (r/h)^3)."""
+    hx, hy, hz = (LD(v) for v in h)
+    k, j, i = np.meshgrid(np.arange(n[2] + 1), np.arange(n[1] + 1), np.arange(n[0] + 1), indexing="ij")
+    g = np.zeros(i.shape, LD)
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                z = (k + dz).astype(LD) * hz - hz / 2
+                if table:
+                    z = z + LD(zshift)
+                    if table == 2:
+                        z = z - LD(n[2]) * hz
+                v = np.stack([((i + dx).astype(LD) * hx - hx / 2).ravel(), ((j + dy).astype(LD) * hy - hy / 2).ravel(),
+                              z.ravel()], 1)
+                # + for the corners with an odd count of upper indices (+++, +00, 0+0, 00+)
+                g += (-1) ** (3 - dx - dy - dz) * _F_longdouble(v).reshape(i.shape)
+    g /= hx * hy * hz
+    if table == 0:
+        g[0, 0, 0] = g[1, 0, 0]
+    return g
+
+
+OCTANTS = [
+    ((40, 36, 44), (1e-5, 1.2e-5, 1.3e-5), 0.0, 0),
+    ((40, 40, 40), (1e-5, 1e-5, 3e-5), 0.0, 0),        # z cells 3x longer
+    ((32, 32, 48), (1e-5, 1e-5, 2e-5), -20.3 * 2e-5, 1),  # image table with the singularity inside
+    ((32, 32, 48), (1e-5, 1e-5, 2e-5), -20.3 * 2e-5, 2),
+]
+
+
+@pytest.mark.parametrize("n,h,zshift,table", OCTANTS)
+@pytest.mark.parametrize("fast", [False, True])
+def test_green_octant_matches_long_double(n, h, zshift, table, fast):
+    """Both forms of the octant against the long-double difference at every
+    point.  The FP64 form (the reference's FP64 corner table) keeps the
+    rounding its 8-corner difference cancels up, ~2e-15 (r/h)^3 (measured up to
+    3.8e-9 on these grids); the mixed form (that difference near the origin,
+    the order-4 cell average of 1/r from 16 cells on) stays below 3.1e-10,
+    largest just past the 16-cell boundary of the 1:1:3 grid."""
+    ref = _G_longdouble(n, h, zshift, table)
+    g = sc.host_green_octant(n, h, zshift, table, fast)
+    rel = np.abs(g.astype(LD) - ref) / np.abs(ref)
+    assert float(rel.max()) < (5e-10 if fast else 1e-8)
+
+
+def test_green_octant_forms_agree_on_a_large_grid():
+    n, h = (64, 64, 64), (1e-5, 1e-5, 1.27e-5)
+    a = sc.host_green_octant(n, h, fast=False)
+    b = sc.host_green_octant(n, h, fast=True)
+    assert np.abs(b - a).max() / np.abs(a).max() < 1e-10
+    assert b[0, 0, 0] == b[1, 0, 0]  # G[0,0,0] := G[0,0,1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [sc.FP64, sc.MIXED])
+@pytest.mark.parametrize("table", [0, 1, 2])
+def test_device_green_octant_equals_host_bits(precision, table):
+    n, h, zshift = (24, 16, 32), (1e-5, 1.1e-5, 2e-5), -12.7 * 2e-5
+    with sc.SpaceChargeContext(n, precision=precision, max_particles=16) as c:
+        d = c.device_green_octant(h, zshift, table)
+    np.testing.assert_array_equal(d, sc.host_green_octant(n, h, zshift, table, precision == sc.MIXED))
