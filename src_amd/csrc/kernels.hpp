@@ -121,8 +121,6 @@ struct PoissonGrid {
     void grad(const Scalars *d_sc);
     int ensure_sums(std::string &err);
     void clear_sums();
-    // Esum += E, Bsum += (s*betaC*Ey, -s*betaC*Ex): s = -1 a bin's field, +1 its image
-    void grad_accum(const Scalars *d_sc, double sign);
     void rho_to_double(const Scalars *d_sc, double *out);
     size_t rho_words() const { return fp64 ? 2 * ncell : ncell; }
     void zero_rho(hipStream_t s);  // clears every word of rho_i64

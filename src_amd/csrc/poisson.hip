@@ -542,11 +542,6 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 // E = -Grad(phi) * (gz, gz, 1/gz)   (PartBunch.cpp:573-575; Grad Cell->Cell,
 // ippl/src/Meshes/UniformCartesian.hpp:2127-2200; ZeroFace guards)
 // ---------------------------------------------------------------------------
-// ACC (energy bins): instead of storing E, add it to Eg and add the magnetic
-// field it implies to Bg: Bx += s*betaC*Ey, By -= s*betaC*Ex, with s = -1 for a
-// bin's own field and +1 for its image charge (PartBunch.cpp:246-247,423-424);
-// CIC gathering is linear, so gathering these sums once equals the
-// reference's per-bin gathers added at each particle.
 // Grad at cell (i, j, k) of phi (p points at it): central differences, the
 // one-sided second-order stencils on the border faces of each component.
 template <typename T>
@@ -563,14 +558,12 @@ __device__ inline void grad_stencil(const T *p, int i, int j, int k, int n0, int
     else r2 = id2 * (p[sz] - p[-(ptrdiff_t)sz]);
 }
 
-template <typename T, bool ACC = false>
+template <typename T>
 __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>::type *Eg, int n0,
-                                              int n1, int n2, const Scalars *sc,
-                                              typename vec4_of<T>::type *Bg = nullptr, double sign = 0.0) {
+                                              int n1, int n2, const Scalars *sc) {
     const size_t total = (size_t)n0 * n1 * n2;
     const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
     const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
-    const T bs = (T)(sign * sc->betaC);
     const size_t sy = n0, sz = (size_t)n0 * n1;
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total;
          c += (size_t)gridDim.x * blockDim.x) {
@@ -584,18 +577,7 @@ __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>:
         e.y = -r1 * g1;
         e.z = -r2 * g2;
         e.w = T(0);
-        if constexpr (ACC) {
-            typename vec4_of<T>::type a = Eg[c], b = Bg[c];
-            a.x += e.x;
-            a.y += e.y;
-            a.z += e.z;
-            b.x += bs * e.y;
-            b.y -= bs * e.x;
-            Eg[c] = a;
-            Bg[c] = b;
-        } else {
-            Eg[c] = e;
-        }
+        Eg[c] = e;
     }
 }
 
@@ -1226,17 +1208,6 @@ void PoissonGrid::clear_sums() {
     const size_t words = ncell * (fp64 ? 4 : 2);  // one 4-vector per cell, in 8-byte words
     launch_fill_i64(stream, words, (long long *)Esum);
     launch_fill_i64(stream, words, (long long *)Bsum);
-}
-
-void PoissonGrid::grad_accum(const Scalars *d_sc, double sign) {
-    const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
-    if (fp64)
-        k_grad<double, true><<<blocks, 256, 0, stream>>>((const double *)phi, (double4 *)Esum, n[0], n[1], n[2],
-                                                         d_sc, (double4 *)Bsum, sign);
-    else
-        k_grad<float, true><<<blocks, 256, 0, stream>>>((const float *)phi, (float4 *)Esum, n[0], n[1], n[2],
-                                                        d_sc, (float4 *)Bsum, sign);
-    profile_mark(PROF_GRAD);
 }
 
 void PoissonGrid::zero_rho(hipStream_t s) { launch_fill_i64(s, rho_words(), (long long *)rho_i64); }
