@@ -141,10 +141,11 @@ def c4_specs():
     return specs
 
 
-def c4_context(sc, k, spec, precision):
-    """Bunch k of the 64-bunch set (seed 1000 + k, 1 nC) ready to track."""
+def c4_context(sc, k, spec, precision, sort_interval=20):
+    """Bunch k of the 64-bunch set (seed 1000 + k, 1 nC) ready to track,
+    re-sorted by cell every sort_interval steps."""
     N, n, sig, bg = spec
-    c = sc.SpaceChargeContext((n, n, n), precision=precision, max_particles=N)
+    c = sc.SpaceChargeContext((n, n, n), precision=precision, max_particles=N, sort_interval=sort_interval)
     c.generate_bunch(N, 1000 + k, (sig, sig, sig), (1e-3, 1e-3, 1e-3), bg, 1e-9)
     c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, ref_p=(0.0, 0.0, bg))
     return c
@@ -172,9 +173,13 @@ def run_c4(sc, steps, warmup, log):
     t_seq = time.perf_counter() - t0
     for c in cs:
         c.close()
-    for mode, graph in (("batched_streams", False), ("batched_graph", True)):
+    # fused: one launch per pass for all bunches of a kernel class / grid shape
+    # (re-sorts every 20 steps launched directly); per_bunch: one graph branch
+    # per bunch (no re-sort)
+    modes = (("per_bunch_graph", False, True), ("batched_streams", True, False), ("batched_graph", True, True))
+    for mode, fused, graph in modes:
         cs = make()
-        with sc.Batch(cs) as b:
+        with sc.Batch(cs, fused=fused) as b:
             b.step(warmup, graph=graph, want_stats=False)
             t0 = time.perf_counter()
             b.step(steps, graph=graph, want_stats=False)
@@ -184,12 +189,13 @@ def run_c4(sc, steps, warmup, log):
         out[mode + "_ms_per_step"] = 1e3 * t / steps
         out[mode + "_particle_steps_per_s"] = total * steps / t
     c4_bytes = sum(step_bytes(Nb, nb, 4) for Nb, nb, _, _ in specs)
-    for mode in ("batched_streams", "batched_graph"):
+    for mode, _, _ in modes:
         out[mode + "_roofline_step"] = step_roofline(c4_bytes, out[mode + "_ms_per_step"])
     out["sequential_ms_per_step"] = 1e3 * t_seq / steps
     out["sequential_particle_steps_per_s"] = total * steps / t_seq
-    log("[bench] c4: sequential %.2f ms/step, batched streams %.2f, batched graph %.2f" %
-        (out["sequential_ms_per_step"], out["batched_streams_ms_per_step"], out["batched_graph_ms_per_step"]))
+    log("[bench] c4: sequential %.2f ms/step, per-bunch graph %.2f, fused streams %.2f, fused graph %.2f" %
+        (out["sequential_ms_per_step"], out["per_bunch_graph_ms_per_step"], out["batched_streams_ms_per_step"],
+         out["batched_graph_ms_per_step"]))
     return out
 
 

@@ -346,14 +346,26 @@ typedef struct opalsc_batch opalsc_batch;
 int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, int nctx);
 int opalsc_batch_destroy(opalsc_batch *batch);
 const char *opalsc_batch_last_error(const opalsc_batch *batch);
-/* nsteps steps of every bunch.  use_graph != 0: one step of every bunch is
- * captured once as a HIP graph (the bunches as parallel branches) and
- * replayed per step; it is captured again when a context's bunch, bins or
- * tracker set-up changed.  use_graph = 0: the same kernels launched on the
- * bunches' streams.  Each step is the one opalsc_track(ctx, 1, ...) runs (its
- * own first half push, no re-sort), so results equal single-step calls bit
- * for bit.  per_step (may be NULL) receives [nsteps][nctx] statistics. */
+/* nsteps steps of every bunch (the ParallelTTracker step loop,
+ * ParallelTTracker.cpp:338-388, of many independent bunches at once).  Each
+ * step is the one opalsc_track(ctx, 1, ...) runs (its own first half push, a
+ * re-sort by cell every sort_interval steps of the context), so results equal
+ * single-step calls bit for bit.
+ * Fused form (default, when no bunch has energy bins, field maps or a DC beam):
+ * every particle pass is one launch over all bunches of a kernel class
+ * (precision, equal charges, external field), every Poisson pass one launch
+ * over all grids of one shape; use_graph != 0 captures one step as a HIP graph
+ * (grid shapes on parallel branches) and replays it per step, steps with a
+ * re-sort are launched directly.  Otherwise (or after
+ * opalsc_batch_set_fused(batch, 0)) each bunch's kernels form a branch of the
+ * graph (use_graph != 0) or run on the bunch's stream; the per-bunch form does
+ * not re-sort.  The batch re-plans or re-captures when a context's bunch, bins
+ * or tracker set-up changed.  per_step (may be NULL) receives [nsteps][nctx]
+ * statistics. */
 int opalsc_step_batched(opalsc_batch *batch, int nsteps, opalsc_stats *per_step, int use_graph);
+/* enable != 0 (default): the fused form where every bunch allows it; 0: the
+ * per-bunch form.  No counterpart in the reference (a batching choice). */
+int opalsc_batch_set_fused(opalsc_batch *batch, int enable);
 int opalsc_track_get_params(opalsc_ctx *ctx, opalsc_track_params *out);
 /* Mesh (hr, origin) of the most recent track step or boundp call. */
 int opalsc_get_mesh(opalsc_ctx *ctx, double hr[3], double origin[3]);

@@ -1043,8 +1043,10 @@ static int bin_bits(int nbins) {  // bits of the values 0 .. nbins
     return bits;
 }
 
-// Reorders every particle array of the resident bunch by perm (b.vals2).
-static void permute_bunch(opalsc_ctx *c) {
+// Reorders every particle array of the resident bunch by perm (b.vals2), on
+// stream st (default: the context's).
+static void permute_bunch(opalsc_ctx *c, hipStream_t st = nullptr) {
+    if (!st) st = c->stream;
     ++c->gen;  // the arrays trade places with the permutation buffer
     Bunch &b = c->bunch;
     const int64_t N = b.N;
@@ -1052,30 +1054,33 @@ static void permute_bunch(opalsc_ctx *c) {
     double **arrs[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz, &b.q};
     for (double **a : arrs) {
         if (*a == b.q && c->q_uniform) continue;  // the charge is one number
-        launch_permute(c->stream, N, b.vals2, *a, b.tmp);
+        launch_permute(st, N, b.vals2, *a, b.tmp);
         std::swap(*a, b.tmp);
     }
-    launch_permute_u32(c->stream, N, b.vals2, b.id, b.keys);  // the sort's input keys are no longer needed
+    launch_permute_u32(st, N, b.vals2, b.id, b.keys);  // the sort's input keys are no longer needed
     std::swap(b.id, b.keys);
 }
 
 // Cell order (3D Morton) of the resident bunch; with energy bins, cell order
-// within each bin, so the bins keep their ranges.
-static void sort_bunch(opalsc_ctx *c) {
+// within each bin, so the bins keep their ranges.  sc: the step's mesh (default:
+// the context's scalars); st: the stream (default: the context's).
+static void sort_bunch(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st = nullptr) {
+    if (!sc) sc = c->d_sc;
+    if (!st) st = c->stream;
     Bunch &b = c->bunch;
     const int64_t N = b.N;
     int bits = key_bits(c->cfg.n);
     if (c->bins.nbins > 0) {
         const int bb = bin_bits(c->bins.nbins);
         const int drop = bits + bb > 32 ? bits + bb - 32 : 0;
-        launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, true, b.keys, b.vals, c->binsorted.as<int32_t>(),
-                         bits - drop, drop);
+        launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals, c->binsorted.as<int32_t>(), bits - drop,
+                         drop);
         bits = bits - drop + bb;
     } else {
-        launch_cell_keys(c->stream, N, b.x, b.y, b.z, c->d_sc, true, b.keys, b.vals);
+        launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals);
     }
-    sort_pairs(c->stream, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, bits);
-    permute_bunch(c);
+    sort_pairs(st, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, bits);
+    permute_bunch(c, st);
 }
 
 extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbins, double qi, int image) {
@@ -1319,16 +1324,63 @@ extern "C" int opalsc_step(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
 // ---------------------------------------------------------------------------
 // many independent bunches: one graph of one step of every bunch
 // ---------------------------------------------------------------------------
+// Fused form (the default): the bunches fall into kernel classes (precision,
+// one charge for every particle, external field, space charge, deposit box
+// size) and grid groups (grid shape, precision, Green's function).  Every
+// particle pass is one launch per class over a device table of BatchItems;
+// every Poisson pass is one launch per group over that group's grids (the
+// strided per-grid buffers of PoissonGrid's batched solve, owned by the
+// batch), each group's Green's function and solve on its own stream.  A step
+// of every bunch is then a fixed sequence of a few dozen launches, captured
+// once as a HIP graph and replayed per step.  Bunches with energy bins, field
+// maps or a DC beam, or a profiled context, take the per-bunch form: one step
+// of every bunch as a parallel branch of a graph.
+namespace {
+struct BatchClass {
+    bool exact = false, quni = false, ext = false, sc_on = false, large = false;
+    int i0 = 0, n = 0;            // rows [i0, i0 + n) of the item table
+    int off[BP_COUNT] = {0};      // each pass's block offsets: first int in the offset table
+    int blocks[BP_COUNT] = {0};   // each pass's blocks
+};
+struct BatchGroup {
+    int n[3] = {0, 0, 0};
+    bool fp64 = false;
+    int greens = 0;
+    int nb = 0, sc0 = 0;  // grids of the group; its first Scalars in the batch's array
+    PoissonGrid pg;
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+};
+constexpr int kStageSlots = 8;  // item-table uploads in flight before the host waits
+}  // namespace
+
 struct opalsc_batch {
     std::vector<opalsc_ctx *> ctx;
-    std::vector<uint64_t> gen;         // generations the graph was captured at
-    std::vector<void *> stats_ptr;     // statistics buffers it writes
     hipStream_t master = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
+    std::string err;
+    // per-bunch form
+    std::vector<uint64_t> gen;      // generations the graph was captured at
+    std::vector<void *> stats_ptr;  // statistics buffers it writes
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::string err;
+    // fused form
+    bool fused_on = true;
+    bool plan_ok = false;
+    std::vector<long long> sig;       // the bunches' shapes and classes the plan was made for
+    std::vector<int> row_ctx;         // context of each item-table row
+    std::vector<int> ctx_group, ctx_slot, ctx_scal;  // per context: grid group (-1: none), slot, Scalars index
+    std::vector<BatchClass> classes;
+    std::vector<BatchGroup *> groups;
+    BatchItem *items_d = nullptr;
+    int *offs_d = nullptr;
+    Scalars *scal_d = nullptr;
+    BatchItem *stage = nullptr;  // pinned staging of the item table, kStageSlots copies
+    int stage_slot = 0;
+    hipEvent_t ev_mesh = nullptr, ev_dep = nullptr;
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fexec = nullptr;
 };
 
 #define BFAIL(b, code, msg) \
@@ -1349,6 +1401,40 @@ static void batch_drop_graph(opalsc_batch *b) {
     b->graph = nullptr;
 }
 
+static void fused_drop_graph(opalsc_batch *b) {
+    if (b->fexec) (void)hipGraphExecDestroy(b->fexec);
+    if (b->fgraph) (void)hipGraphDestroy(b->fgraph);
+    b->fexec = nullptr;
+    b->fgraph = nullptr;
+}
+
+static void fused_release(opalsc_batch *b) {
+    fused_drop_graph(b);
+    for (BatchGroup *g : b->groups) {
+        g->pg.destroy();
+        if (g->done) (void)hipEventDestroy(g->done);
+        if (g->s) (void)hipStreamDestroy(g->s);
+        delete g;
+    }
+    b->groups.clear();
+    b->classes.clear();
+    b->row_ctx.clear();
+    b->ctx_group.clear();
+    b->ctx_slot.clear();
+    b->ctx_scal.clear();
+    if (b->items_d) (void)hipFree(b->items_d);
+    if (b->offs_d) (void)hipFree(b->offs_d);
+    if (b->scal_d) (void)hipFree(b->scal_d);
+    if (b->stage) (void)hipHostFree(b->stage);
+    b->items_d = nullptr;
+    b->offs_d = nullptr;
+    b->scal_d = nullptr;
+    b->stage = nullptr;
+    b->stage_slot = 0;
+    b->plan_ok = false;
+    b->sig.clear();
+}
+
 extern "C" int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, int n) {
     if (!out || !ctxs || n < 1) return OPALSC_EINVAL;
     *out = nullptr;
@@ -1366,11 +1452,15 @@ extern "C" int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, 
     b->stats_ptr.assign(n, nullptr);
     b->join.assign(n, nullptr);
     bool ok = hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_mesh, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_dep, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; ok && i < n; ++i) ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         if (b->master) (void)hipStreamDestroy(b->master);
-        if (b->fork) (void)hipEventDestroy(b->fork);
+        hipEvent_t evs[] = {b->fork, b->ev_mesh, b->ev_dep};
+        for (hipEvent_t e : evs)
+            if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : b->join)
             if (e) (void)hipEventDestroy(e);
         delete b;
@@ -1382,17 +1472,27 @@ extern "C" int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, 
 
 extern "C" int opalsc_batch_destroy(opalsc_batch *b) {
     if (!b) return OPALSC_OK;
+    (void)hipSetDevice(b->ctx[0]->cfg.device);
     if (b->master) (void)hipStreamSynchronize(b->master);
     batch_drop_graph(b);
+    fused_release(b);
     for (hipEvent_t e : b->join)
         if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
+    hipEvent_t evs[] = {b->fork, b->ev_mesh, b->ev_dep};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
     if (b->master) (void)hipStreamDestroy(b->master);
     delete b;
     return OPALSC_OK;
 }
 
 extern "C" const char *opalsc_batch_last_error(const opalsc_batch *b) { return b ? b->err.c_str() : ""; }
+
+extern "C" int opalsc_batch_set_fused(opalsc_batch *b, int enable) {
+    if (!b) return OPALSC_EINVAL;
+    b->fused_on = enable != 0;
+    return OPALSC_OK;
+}
 
 // Captures one step of every bunch (push, no fused next push, no sort) as the
 // parallel branches of one graph: each bunch's kernels on its own stream,
@@ -1426,6 +1526,275 @@ static int batch_capture(opalsc_batch *b) {
     return OPALSC_OK;
 }
 
+// ---- fused form ----
+
+// A context the fused step covers: tracking set up, a non-empty bunch, no
+// energy bins, field maps or DC beam, the one-shot gather pass available, no
+// profiling.
+static bool fused_ok(const opalsc_ctx *c) {
+    return c->have_track && c->bunch.N > 0 && c->bins.nbins == 0 && c->fmaps.empty() && !c->grid.pz &&
+           c->os_dense.p && !c->prof.enabled;
+}
+
+// What the plan depends on, per context (a change makes a new plan).
+static std::vector<long long> fused_signature(const opalsc_batch *b) {
+    std::vector<long long> s;
+    for (opalsc_ctx *c : b->ctx) {
+        if (!fused_ok(c)) return {};
+        s.push_back(c->bunch.N);
+        for (int d = 0; d < 3; ++d) s.push_back(c->cfg.n[d]);
+        s.push_back((c->grid.fp64 ? 1 : 0) | (c->q_uniform ? 2 : 0) | (c->track_ext ? 4 : 0) |
+                    (c->track_sc ? 8 : 0));
+        s.push_back(c->grid.greens);
+    }
+    return s;
+}
+
+static BatchView fused_view(const opalsc_batch *b, const BatchClass &k, int pass) {
+    BatchView v;
+    v.items = b->items_d + k.i0;
+    v.n = k.n;
+    v.off = pass == BP_ITEM ? nullptr : b->offs_d + k.off[pass];
+    v.blocks = pass == BP_ITEM ? k.n : k.blocks[pass];
+    return v;
+}
+
+static BatchView fused_view_all(const opalsc_batch *b) {
+    BatchView v;
+    v.items = b->items_d;
+    v.n = (int)b->row_ctx.size();
+    v.off = nullptr;
+    v.blocks = v.n;
+    return v;
+}
+
+static int fused_plan(opalsc_batch *b, std::vector<long long> &&sig) {
+    fused_release(b);
+    const int n = (int)b->ctx.size();
+    b->ctx_group.assign(n, -1);
+    b->ctx_slot.assign(n, 0);
+    b->ctx_scal.assign(n, 0);
+    // grid groups: the space-charge bunches of one grid shape, precision and Green's function
+    for (int i = 0; i < n; ++i) {
+        opalsc_ctx *c = b->ctx[i];
+        if (!c->track_sc) continue;
+        int g = 0;
+        for (; g < (int)b->groups.size(); ++g) {
+            const BatchGroup *q = b->groups[g];
+            if (q->n[0] == c->cfg.n[0] && q->n[1] == c->cfg.n[1] && q->n[2] == c->cfg.n[2] &&
+                q->fp64 == c->grid.fp64 && q->greens == c->grid.greens)
+                break;
+        }
+        if (g == (int)b->groups.size()) {
+            BatchGroup *q = new (std::nothrow) BatchGroup();
+            if (!q) BFAIL(b, OPALSC_ENOMEM, "batch group");
+            for (int d = 0; d < 3; ++d) q->n[d] = c->cfg.n[d];
+            q->fp64 = c->grid.fp64;
+            q->greens = c->grid.greens;
+            b->groups.push_back(q);
+        }
+        b->ctx_group[i] = g;
+        b->ctx_slot[i] = b->groups[g]->nb++;
+    }
+    // the batch's Scalars: each group's grids in slot order, then the bunches without space charge
+    int nscal = 0;
+    for (BatchGroup *q : b->groups) {
+        q->sc0 = nscal;
+        nscal += q->nb;
+    }
+    for (int i = 0; i < n; ++i)
+        b->ctx_scal[i] = b->ctx_group[i] >= 0 ? b->groups[b->ctx_group[i]]->sc0 + b->ctx_slot[i] : nscal++;
+    // kernel classes: item-table rows ordered by class, then by context
+    auto key = [&](int i) {
+        const opalsc_ctx *c = b->ctx[i];
+        return (c->grid.fp64 ? 1 : 0) | (c->q_uniform ? 2 : 0) | (c->track_ext ? 4 : 0) | (c->track_sc ? 8 : 0) |
+               (deposit_large_box(c->bunch.N) ? 16 : 0);
+    };
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+    b->row_ctx = order;
+    std::vector<int> offs;
+    for (int r = 0; r < n;) {
+        const int k0 = key(order[r]);
+        BatchClass k;
+        k.exact = k0 & 1;
+        k.quni = (k0 & 2) != 0;
+        k.ext = (k0 & 4) != 0;
+        k.sc_on = (k0 & 8) != 0;
+        k.large = (k0 & 16) != 0;
+        k.i0 = r;
+        while (r < n && key(order[r]) == k0) ++r;
+        k.n = r - k.i0;
+        for (int pass = 0; pass < BP_ITEM; ++pass) {
+            k.off[pass] = (int)offs.size();
+            int acc = 0;
+            for (int j = k.i0; j < k.i0 + k.n; ++j) {
+                offs.push_back(acc);
+                const int64_t N = b->ctx[order[j]]->bunch.N;
+                if (pass == BP_ALPHA) acc += alpha_blocks(N);
+                else if (pass == BP_DEPOSIT) acc += k.sc_on ? (int)((N + kDepositChunk - 1) / kDepositChunk) : 0;
+                else if (pass == BP_GATHER) acc += k.exact ? os_blocks64(N) : os_blocks(N);
+                else acc += batch_reduce_blocks();
+            }
+            offs.push_back(acc);
+            k.blocks[pass] = acc;
+        }
+        b->classes.push_back(k);
+    }
+    BHIP(b, hipMalloc((void **)&b->items_d, sizeof(BatchItem) * n));
+    BHIP(b, hipMalloc((void **)&b->offs_d, sizeof(int) * offs.size()));
+    BHIP(b, hipMalloc((void **)&b->scal_d, sizeof(Scalars) * std::max(nscal, 1)));
+    BHIP(b, hipHostMalloc((void **)&b->stage, sizeof(BatchItem) * n * kStageSlots, hipHostMallocDefault));
+    BHIP(b, hipMemcpy(b->offs_d, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice));
+    for (BatchGroup *q : b->groups) {
+        BHIP(b, hipStreamCreateWithFlags(&q->s, hipStreamNonBlocking));
+        BHIP(b, hipEventCreateWithFlags(&q->done, hipEventDisableTiming));
+        std::string e;
+        if (q->pg.init(q->n, q->fp64, q->greens, false, q->s, e) || q->pg.ensure_bins(q->nb, false, e) ||
+            q->pg.ensure_bin_fields(e))
+            BFAIL(b, OPALSC_ENOMEM, "batch grid group: " + e);
+    }
+    b->sig = std::move(sig);
+    b->plan_ok = true;
+    return OPALSC_OK;
+}
+
+// The item table of the current bunch arrays, uploaded in stream order on the
+// batch's stream.  nobox: contexts whose deposit forms its own chunk boxes
+// (re-sorted in this step).
+static int fused_upload(opalsc_batch *b, const std::vector<char> *nobox = nullptr) {
+    const int n = (int)b->row_ctx.size();
+    if (b->stage_slot == kStageSlots) {
+        BHIP(b, hipStreamSynchronize(b->master));  // every earlier upload has been read
+        b->stage_slot = 0;
+    }
+    BatchItem *st = b->stage + (size_t)b->stage_slot * n;
+    ++b->stage_slot;
+    for (int r = 0; r < n; ++r) {
+        const int i = b->row_ctx[r];
+        opalsc_ctx *c = b->ctx[i];
+        const Bunch &u = c->bunch;
+        BatchItem it;
+        std::memset(&it, 0, sizeof(it));
+        it.N = u.N;
+        it.x = u.x;
+        it.y = u.y;
+        it.z = u.z;
+        it.px = u.px;
+        it.py = u.py;
+        it.pz = u.pz;
+        it.q = u.q;
+        it.chunkbox = (nobox && (*nobox)[i]) ? nullptr : u.chunkbox;
+        it.sc = b->scal_d + b->ctx_scal[i];
+        it.sc_ctx = c->d_sc;
+        it.ts = c->d_ts;
+        it.partials = c->partials;
+        it.dense = c->os_dense.as<double>();
+        it.dropped = &c->d_counters[1];
+        it.stats = c->statsb.p;
+        for (int d = 0; d < 3; ++d) it.n[d] = c->cfg.n[d];
+        it.dh = c->dh;
+        const int g = b->ctx_group[i];
+        if (g >= 0) {
+            it.rho = b->groups[g]->pg.bin_rho(b->ctx_slot[i]);
+            it.E = b->groups[g]->pg.bin_E(b->ctx_slot[i]);
+        }
+        st[r] = it;
+    }
+    BHIP(b, hipMemcpyAsync(b->items_d, st, sizeof(BatchItem) * n, hipMemcpyHostToDevice, b->master));
+    return OPALSC_OK;
+}
+
+// One step of every bunch (the step opalsc_track(ctx, 1) takes) on the batch's
+// stream.  sorted: contexts re-sorted by cell after their mesh (then this step
+// is launched directly, not captured: the sort moves the bunch arrays).
+static int fused_step(opalsc_batch *b, const std::vector<int> &sorted) {
+    hipStream_t m = b->master;
+    for (const BatchClass &k : b->classes) launch_batch_pre(m, fused_view(b, k, BP_ITEM));
+    for (const BatchClass &k : b->classes) launch_batch_alpha(m, fused_view(b, k, BP_ALPHA), k.exact);
+    for (const BatchClass &k : b->classes)
+        if (k.sc_on) launch_batch_mesh(m, fused_view(b, k, BP_ITEM), k.exact);
+    if (!sorted.empty()) {
+        BHIP(b, hipEventRecord(b->fork, m));
+        std::vector<char> nobox(b->ctx.size(), 0);
+        for (int i : sorted) {
+            opalsc_ctx *c = b->ctx[i];
+            BHIP(b, hipStreamWaitEvent(c->stream, b->fork, 0));
+            sort_bunch(c, b->scal_d + b->ctx_scal[i], c->stream);
+            BHIP(b, hipEventRecord(b->join[i], c->stream));
+            BHIP(b, hipStreamWaitEvent(m, b->join[i], 0));
+            nobox[i] = 1;
+        }
+        if (int e = fused_upload(b, &nobox)) return e;  // the sorted bunches' arrays changed places
+    }
+    // the Green's functions need only the mesh: each group's on its own stream
+    BHIP(b, hipEventRecord(b->ev_mesh, m));
+    for (BatchGroup *g : b->groups) {
+        BHIP(b, hipStreamWaitEvent(g->s, b->ev_mesh, 0));
+        g->pg.green_bins(b->scal_d + g->sc0, g->nb, false, g->s);
+    }
+    for (BatchGroup *g : b->groups) g->pg.zero_bin_rho(g->nb, m);
+    for (const BatchClass &k : b->classes)
+        if (k.sc_on) launch_batch_deposit(m, fused_view(b, k, BP_DEPOSIT), k.exact, k.quni, k.large);
+    BHIP(b, hipEventRecord(b->ev_dep, m));
+    for (BatchGroup *g : b->groups) {
+        BHIP(b, hipStreamWaitEvent(g->s, b->ev_dep, 0));
+        g->pg.solve_bins(b->scal_d + g->sc0, g->nb, false, nullptr);
+        g->pg.grad_bins(b->scal_d + g->sc0, g->nb);
+        BHIP(b, hipEventRecord(g->done, g->s));
+        BHIP(b, hipStreamWaitEvent(m, g->done, 0));
+    }
+    for (const BatchClass &k : b->classes) launch_batch_gather(m, fused_view(b, k, BP_GATHER), k.exact, k.ext);
+    for (const BatchClass &k : b->classes) launch_batch_reduce(m, fused_view(b, k, BP_REDUCE), k.exact);
+    for (const BatchClass &k : b->classes) launch_batch_post(m, fused_view(b, k, BP_ITEM));
+    BHIP(b, hipGetLastError());
+    if (!sorted.empty())
+        if (int e = fused_upload(b)) return e;  // later steps take their boxes from the bounds pass again
+    return OPALSC_OK;
+}
+
+static int fused_capture(opalsc_batch *b) {
+    fused_drop_graph(b);
+    BHIP(b, hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
+    const int e = fused_step(b, {});
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->master, &g);
+    if (e != OPALSC_OK || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (e != OPALSC_OK) return e;
+        BFAIL(b, OPALSC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e2));
+    }
+    b->fgraph = g;
+    BHIP(b, hipGraphInstantiate(&b->fexec, b->fgraph, nullptr, nullptr, 0));
+    return OPALSC_OK;
+}
+
+static int fused_run(opalsc_batch *b, int nsteps, bool use_graph) {
+    hipStream_t m = b->master;
+    const int n = (int)b->ctx.size();
+    if (int e = fused_upload(b)) return e;
+    launch_batch_begin(m, fused_view_all(b));
+    for (int s = 0; s < nsteps; ++s) {
+        std::vector<int> sorted;
+        for (int i = 0; i < n; ++i) {
+            const opalsc_ctx *c = b->ctx[i];
+            const int interval = c->cfg.sort_interval;
+            if (c->track_sc && interval > 0 && (c->steps_done + s) % interval == 0) sorted.push_back(i);
+        }
+        if (sorted.empty() && use_graph) {
+            if (!b->fexec)
+                if (int e = fused_capture(b)) return e;
+            BHIP(b, hipGraphLaunch(b->fexec, m));
+        } else if (int e = fused_step(b, sorted)) {
+            return e;
+        }
+    }
+    launch_batch_end(m, fused_view_all(b));
+    BHIP(b, hipGetLastError());
+    return OPALSC_OK;
+}
+
 extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *per_step, int use_graph) {
     if (!b || nsteps < 0) return OPALSC_EINVAL;
     if (nsteps == 0) return OPALSC_OK;
@@ -1441,15 +1810,25 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
         if (prepare_track(c) != OPALSC_OK) BFAIL(b, OPALSC_ENOMEM, "bunch " + std::to_string(i) + ": " + c->err);
         recapture |= b->gen[i] != c->gen || b->stats_ptr[i] != c->statsb.p;
     }
-    // every bunch starts its call on its own stream; the batch's stream waits for them
+    std::vector<long long> sig = b->fused_on ? fused_signature(b) : std::vector<long long>();
+    const bool fused = !sig.empty();
+    // every bunch's earlier work on its own stream comes first
     for (int i = 0; i < n; ++i) {
         opalsc_ctx *c = b->ctx[i];
-        launch_track_begin_call(c->stream, c->d_ts);
-        BHIP(b, hipGetLastError());
+        if (!fused) {
+            launch_track_begin_call(c->stream, c->d_ts);
+            BHIP(b, hipGetLastError());
+        }
         BHIP(b, hipEventRecord(b->join[i], c->stream));
         BHIP(b, hipStreamWaitEvent(b->master, b->join[i], 0));
     }
-    if (use_graph) {
+    if (fused) {
+        if (!b->plan_ok || sig != b->sig)
+            if (int e = fused_plan(b, std::move(sig))) return e;
+        if (int e = fused_run(b, nsteps, use_graph != 0)) return e;
+        BHIP(b, hipEventRecord(b->fork, b->master));
+        for (int i = 0; i < n; ++i) BHIP(b, hipStreamWaitEvent(b->ctx[i]->stream, b->fork, 0));
+    } else if (use_graph) {
         if (recapture)
             if (int e = batch_capture(b)) return e;
         for (int s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->exec, b->master));
@@ -1463,6 +1842,7 @@ extern "C" int opalsc_step_batched(opalsc_batch *b, int nsteps, opalsc_stats *pe
                 BHIP(b, hipGetLastError());
             }
     }
+    BHIP(b, hipStreamSynchronize(b->master));
     for (int i = 0; i < n; ++i) {
         opalsc_ctx *c = b->ctx[i];
         c->steps_done += nsteps;

@@ -142,6 +142,12 @@ struct PoissonGrid {
     // Esum, Bsum := the sums over the nb grids (and their images) of E and of
     // the magnetic field it implies (written, not accumulated)
     void field_sums(const Scalars *scb, int nb, bool image);
+    // Independent bunches of this grid shape solved as one batch (the same
+    // per-grid buffers): E of every grid, vec4<T>[ncell] each, one launch
+    void *bb_E = nullptr;
+    int ensure_bin_fields(std::string &err);  // bb_E for bin_cap grids
+    void *bin_E(int b) const { return (char *)bb_E + (size_t)b * ncell * 4 * (fp64 ? 8 : 4); }
+    void grad_bins(const Scalars *scb, int nb);
     void widen(int which, double *out);
     void profile_mark(int id) {
         if (prof) prof->mark(id);
@@ -171,6 +177,66 @@ struct Bunch {
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
 };
+
+// ---- Batched steps of many independent bunches --------------------------------
+// One bunch of a batched step: its particle arrays, per-bunch state and the
+// grid slot of its shape group.  api.hip keeps a device table of these; each
+// particle pass is one launch over the bunches of a kernel class, a flat block
+// index finding its bunch through the pass's block offsets.
+struct BatchItem {
+    int64_t N;
+    double *x, *y, *z, *px, *py, *pz, *q;
+    double *chunkbox;
+    Scalars *sc;       // the bunch's scalars during the call (the batch's per-group array)
+    Scalars *sc_ctx;   // the context's own scalars (copied in and out around a call)
+    TrackState *ts;
+    double *partials;  // per-block partial sums
+    double *dense;     // per-block moment sums of the one-shot gather pass
+    long long *rho;    // the bunch's fixed-point charge grid
+    const void *E;     // the bunch's field grid (vec4 of the grid precision)
+    unsigned long long *dropped;
+    void *stats;       // the call's statistics array
+    int n[3];
+    double dh;
+};
+enum BatchPass { BP_ALPHA = 0, BP_DEPOSIT, BP_GATHER, BP_REDUCE, BP_ITEM, BP_COUNT };
+// The bunches of one kernel class in a pass (passed by value): items[0..n),
+// off[j] = first flat block of item j (device), off[n] = blocks = the blocks of
+// the launch (host copy).
+struct BatchView {
+    const BatchItem *items;
+    const int *off;
+    int n;
+    int blocks;
+};
+// flat block -> (item, block within the item, the item's block count)
+__device__ inline int batch_find(const BatchView &bv, int &bid, int &nblk) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = bv.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bv.off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    bid = b - bv.off[lo];
+    nblk = bv.off[lo + 1] - bv.off[lo];
+    return lo;
+}
+
+// Launchers of the batched particle passes (particles.hip).  Class flags:
+// exact (FP64 arithmetic), quni (one charge for every particle), ext (an
+// external field), large (the deposit's larger LDS box).
+void launch_batch_begin(hipStream_t s, const BatchView &v);          // call_step0, scalars in
+void launch_batch_end(hipStream_t s, const BatchView &v);            // scalars out
+void launch_batch_pre(hipStream_t s, const BatchView &v);
+void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact);
+void launch_batch_mesh(hipStream_t s, const BatchView &v, bool exact);
+void launch_batch_deposit(hipStream_t s, const BatchView &v, bool exact, bool quni, bool large);
+void launch_batch_gather(hipStream_t s, const BatchView &v, bool exact, bool ext);
+void launch_batch_reduce(hipStream_t s, const BatchView &v, bool exact);
+void launch_batch_post(hipStream_t s, const BatchView &v);
+int batch_reduce_blocks();  // partial rows the batched gather pass leaves for the post pass
+bool deposit_large_box(int64_t n);  // the deposit of n particles uses the larger LDS box
 
 // Launchers (particles.hip)
 int red_blocks(int64_t N);
@@ -241,9 +307,20 @@ void launch_track_mesh(hipStream_t s, const double *partials, int nb, const int 
                        const TrackState *ts, bool exact, bool q_roundtrip, Scalars *sc);
 int gamma_blocks(bool fp64, int64_t N);  // partial-sum blocks launch_track_gamma writes
 // the gather pass in blocks that run once (no DC beam or field maps):
-// os_blocks(N) rows of kMom doubles in dense; returns the partial blocks for k_track_post
-int os_blocks(int64_t N);
-int os_blocks64(int64_t N);
+// os_blocks(N) rows of kMom doubles in dense; returns the partial blocks for k_track_post.
+// Block b takes chunk (b mod 8) * per + b / 8: the blocks of one XCD stream one
+// contiguous eighth of the bunch.
+constexpr int kXcds = 8;  // XCDs of an MI355X; workgroups are dealt to them round-robin
+__host__ __device__ inline int os_blocks(int64_t N) {
+    const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
+    const int64_t per = (nchunks + kXcds - 1) / kXcds;
+    return (int)(per * kXcds);
+}
+__host__ __device__ inline int os_blocks64(int64_t N) {
+    const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
+    const int64_t per = (nchunks + kXcds - 1) / kXcds;
+    return (int)(per * kXcds);
+}
 // osbox (used with next): per one-shot block the bounds (kOsBoxStats doubles) of the
 // positions written, which the next step's bounds come from.  Bg: the energy
 // bins' summed magnetic field grid (then B is gathered, not formed from E);

@@ -17,7 +17,6 @@
 
 namespace opalsc {
 
-constexpr int kXcds = 8;             // XCDs of an MI355X; workgroups are dealt to them round-robin
 
 int red_blocks(int64_t N) {
     int64_t b = (N + kThreads - 1) / kThreads;
@@ -205,24 +204,40 @@ __device__ inline void rho_add(long long *rho, size_t c, long long v, int s) {
 // integers, summed in 64 bits).  BOX: chunk statistics supplied by the bounds
 // pass.  Only particles lo <= p < N deposit; block b owns chunk chunk0 + b of
 // the Nall-particle bunch.
-template <int MODE, bool EXACT, bool BOX, int kBoxCap>
+template <int MODE, bool EXACT, bool BOX, int kBoxCap, bool BATCH = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_deposit_lds(int64_t lo, int64_t N, int64_t Nall, int64_t chunk0,
                                                      const double *__restrict__ x,
                                                      const double *__restrict__ y,
                                                      const double *__restrict__ z, double *q,
                                                      const double *qdv, long long *rho,
                                                      const Scalars *sc, unsigned long long *dropped,
-                                                     const double *chunkbox) {
+                                                     const double *chunkbox, BatchView bv = BatchView{}) {
     __shared__ long long box[kBoxCap];
     __shared__ double red[4 * kBoxStats];
     __shared__ int boxinfo[7];
+    int bid = (int)blockIdx.x;
+    if constexpr (BATCH) {
+        int nblk = 0;
+        const BatchItem &it = bv.items[batch_find(bv, bid, nblk)];
+        lo = 0;
+        N = Nall = it.N;
+        chunk0 = 0;
+        x = it.x;
+        y = it.y;
+        z = it.z;
+        q = it.q;
+        rho = it.rho;
+        sc = it.sc;
+        dropped = it.dropped;
+        chunkbox = it.chunkbox;
+    }
     const double o0 = sc->origin[0], o1 = sc->origin[1], o2 = sc->origin[2];
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const double ih0 = 1.0 / h0, ih1 = 1.0 / h1, ih2 = 1.0 / h2;
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const double e2q = sc->e2q, dT = sc->dT;
     const int lsh = sc->lo_shift;
-    const int64_t chunk = chunk0 + blockIdx.x;
+    const int64_t chunk = chunk0 + bid;
     const int64_t base = chunk * kChunk;
 
     // 1. the chunk's particles, all loads issued before any is used; thread t
@@ -307,7 +322,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // own: the block forms its chunk's statistics itself (no statistics supplied,
     // or only a bounding box, NaN sums, that overflows the LDS box)
     bool own = !BOX;
-    if constexpr (BOX) {
+    if constexpr (BOX) own = chunkbox == nullptr;  // a batched bunch re-sorted in this step
+    if (!own) {
         for (int k = 0; k < kBoxStats; ++k) st[k] = chunkbox[chunk * kBoxStats + k];
         if (!(st[6] == st[6])) {
             int l0 = 0, h0_ = 0;
@@ -516,6 +532,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     if (nd) atomicAdd(dropped, nd);
 }
+
+bool deposit_large_box(int64_t n) { return n <= kLargeBoxMaxParticles; }
 
 void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64_t hi, int64_t Nall,
                           const double *x, const double *y, const double *z, double *q, const double *qd,
@@ -1084,9 +1102,20 @@ __device__ inline double ord_val(unsigned long long k) {
     return __longlong_as_double((long long)u);
 }
 
+template <bool BATCH = false>
 __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
-                            unsigned long long *bkeys) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+                            unsigned long long *bkeys, BatchView bv = BatchView{}) {
+    if constexpr (BATCH) {
+        if (threadIdx.x != 0) return;
+        const BatchItem &it = bv.items[blockIdx.x];
+        ts = it.ts;
+        sc = it.sc;
+        N = it.N;
+        dropped = it.dropped;
+        bkeys = nullptr;
+    } else if (threadIdx.x != 0 || blockIdx.x != 0) {
+        return;
+    }
     *dropped = 0;  // the step's count of deposits outside the grid
     if (bkeys)
         for (int d = 0; d < 3; ++d) {
@@ -1162,15 +1191,30 @@ void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, uns
 // every chunk's beam-frame position statistics (bounds, sums, sums of squares)
 // to chunkbox (so the deposit reads positions once), and leaves its running
 // bounds in partials.
-template <bool EXACT, bool PUSH>
+template <bool EXACT, bool PUSH, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restrict__ x,
                                                      double *__restrict__ y, double *__restrict__ z,
                                                      const double *__restrict__ px,
                                                      const double *__restrict__ py,
                                                      const double *__restrict__ pz, const Scalars *sc,
                                                      const TrackState *ts, double *partials,
-                                                     double *chunkbox) {
+                                                     double *chunkbox, BatchView bv = BatchView{}) {
     __shared__ double lds[4 * kBoxStats];
+    int bid = (int)blockIdx.x, nblk = (int)gridDim.x;
+    if constexpr (BATCH) {
+        const BatchItem &it = bv.items[batch_find(bv, bid, nblk)];
+        N = it.N;
+        x = it.x;
+        y = it.y;
+        z = it.z;
+        px = it.px;
+        py = it.py;
+        pz = it.pz;
+        sc = it.sc;
+        ts = it.ts;
+        partials = it.partials;
+        chunkbox = it.chunkbox;
+    }
     const double dT = ts->dT;
     const double half_cdt = 0.5 * (kC * dT);
     const bool sc_on = ts->space_charge != 0;
@@ -1180,7 +1224,7 @@ __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restri
     double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double bmx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     const int64_t nchunks = (N + kChunk - 1) / kChunk;
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    for (int64_t c = bid; c < nchunks; c += nblk) {
         double st[kBoxStats];
         for (int d = 0; d < 3; ++d) {
             st[d] = __builtin_inf();
@@ -1222,14 +1266,15 @@ __global__ __launch_bounds__(256) void k_track_alpha(int64_t N, double *__restri
     }
     if (threadIdx.x == 0)
         for (int d = 0; d < 3; ++d) {
-            partials[blockIdx.x * kSlots + d] = bmn[d];
-            partials[blockIdx.x * kSlots + 3 + d] = bmx[d];
+            partials[bid * kSlots + d] = bmn[d];
+            partials[bid * kSlots + 3 + d] = bmx[d];
         }
 }
-int alpha_blocks(int64_t N) {
-    int64_t nchunks = (N + kChunk - 1) / kChunk;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, kRedBlocksMax));
+__host__ __device__ inline int alpha_blocks_of(int64_t N) {
+    const int64_t nchunks = (N + kChunk - 1) / kChunk;
+    return (int)(nchunks < 1 ? 1 : nchunks > kRedBlocksMax ? kRedBlocksMax : nchunks);
 }
+int alpha_blocks(int64_t N) { return alpha_blocks_of(N); }
 void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Scalars *sc,
                         const TrackState *ts, double *partials) {
     if (b.N <= 0) return;
@@ -1398,10 +1443,22 @@ void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars
     else k_bounds_scan<false><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
 }
 
+template <bool BATCH = false>
 __global__ __launch_bounds__(512) void k_track_mesh(const double *partials, int nb, int n0, int n1,
                                                     int n2, double dh, const TrackState *ts, int exact,
-                                                    int q_roundtrip, Scalars *sc) {
+                                                    int q_roundtrip, Scalars *sc, BatchView bv = BatchView{}) {
     __shared__ double lds[16 * 6];
+    if constexpr (BATCH) {
+        const BatchItem &it = bv.items[blockIdx.x];
+        partials = it.partials;
+        nb = alpha_blocks_of(it.N);
+        n0 = it.n[0];
+        n1 = it.n[1];
+        n2 = it.n[2];
+        dh = it.dh;
+        ts = it.ts;
+        sc = it.sc;
+    }
     double mn[3], mx[3];
     reduce_partials_minmax(partials, nb, mn, mx, lds);
     if (threadIdx.x != 0) return;
@@ -1682,24 +1739,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // butterflies, then the four waves in order).  A stream of short-lived blocks
 // keeps more particle loads in flight than a persistent grid-stride loop
 // (2.30 -> 2.20 ms at 1e8 particles; 2 or 8 particles per thread: 2.27, 2.31 ms).
-int os_blocks(int64_t N) {
-    const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
-    const int64_t per = (nchunks + kXcds - 1) / kXcds;
-    return (int)(per * kXcds);
-}
 
-template <bool NEXT, int EXT, bool BINNED>
+template <bool NEXT, int EXT, bool BINNED, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double *x, double *y, double *z, double *px,
                                                               double *py, double *pz, const float4 *Eg,
                                                               const float4 *Bg, const Scalars *sc,
                                                               const TrackState *ts, double *dense, double *osbox,
-                                                              BinRanges br, double *binp) {
+                                                              BinRanges br, double *binp,
+                                                              BatchView bv = BatchView{}) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
     __shared__ double lgam[2 * (kThreads / 64)];
+    int bid = (int)blockIdx.x;
+    if constexpr (BATCH) {
+        int nblk = 0;
+        const BatchItem &it = bv.items[batch_find(bv, bid, nblk)];
+        N = it.N;
+        x = it.x;
+        y = it.y;
+        z = it.z;
+        px = it.px;
+        py = it.py;
+        pz = it.pz;
+        Eg = (const float4 *)it.E;
+        sc = it.sc;
+        ts = it.ts;
+        dense = it.dense;
+    }
     const int64_t nchunks = (N + kOsChunk - 1) / kOsChunk;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
-    const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const int64_t chunk = (int64_t)(bid % kXcds) * per + bid / kXcds;
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
@@ -1806,7 +1875,7 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
     if (threadIdx.x < kMom) {
         double t = wsum[0][threadIdx.x];
         for (int w = 1; w < kThreads / 64; ++w) t += wsum[w][threadIdx.x];
-        dense[(int64_t)blockIdx.x * kMom + threadIdx.x] = t;
+        dense[(int64_t)bid * kMom + threadIdx.x] = t;
     }
     if constexpr (NEXT) {
         if (osbox) {
@@ -1825,23 +1894,35 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
 // The FP64 gather pass (no DC beam or field maps; with BINNED the energy
 // bins' summed E and B grids) in blocks that run once, kOsU64 particles per thread, the per-particle sequence of
 // k_track_gamma unchanged; moment sums as in k_track_gamma_mixed_os.
-int os_blocks64(int64_t N) {
-    const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
-    const int64_t per = (nchunks + kXcds - 1) / kXcds;
-    return (int)(per * kXcds);
-}
 
-template <bool NEXT, bool BINNED>
+template <bool NEXT, bool BINNED, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, double *y, double *z, double *px,
                                                         double *py, double *pz, const double4 *Eg, const double4 *Bg,
                                                         const Scalars *sc, const TrackState *ts, double *dense,
-                                                        double *osbox, BinRanges br, double *binp) {
+                                                        double *osbox, BinRanges br, double *binp,
+                                                        BatchView bv = BatchView{}) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
     __shared__ double lgam[2 * (kThreads / 64)];
+    int bid = (int)blockIdx.x;
+    if constexpr (BATCH) {
+        int nblk = 0;
+        const BatchItem &it = bv.items[batch_find(bv, bid, nblk)];
+        N = it.N;
+        x = it.x;
+        y = it.y;
+        z = it.z;
+        px = it.px;
+        py = it.py;
+        pz = it.pz;
+        Eg = (const double4 *)it.E;
+        sc = it.sc;
+        ts = it.ts;
+        dense = it.dense;
+    }
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
-    const int64_t chunk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const int64_t chunk = (int64_t)(bid % kXcds) * per + bid / kXcds;
     double m[kMom];
     for (int k = 0; k < kMom; ++k) m[k] = 0.0;
     double bmn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
@@ -1943,7 +2024,7 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
     if (threadIdx.x < kMom) {
         double t = wsum[0][threadIdx.x];
         for (int w = 1; w < kThreads / 64; ++w) t += wsum[w][threadIdx.x];
-        dense[(int64_t)blockIdx.x * kMom + threadIdx.x] = t;
+        dense[(int64_t)bid * kMom + threadIdx.x] = t;
     }
     if constexpr (NEXT) {
         if (osbox) {
@@ -1962,12 +2043,21 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
 // Dense per-block moment sums -> at most kOsReduceBlocks double-double partials
 // (rows in a fixed order per block), the layout k_track_post reads.
 constexpr int kOsReduceBlocks = 256;
-__global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64_t rows, double *partials) {
+template <bool BATCH = false, bool FP64 = false>
+__global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64_t rows, double *partials,
+                                                      BatchView bv = BatchView{}) {
     __shared__ double lds[4 * kMom * 2];
     double h[kMom], l[kMom];
     for (int k = 0; k < kMom; ++k) h[k] = l[k] = 0.0;
-    const int64_t per = (rows + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = blockIdx.x * per, r1 = r0 + per < rows ? r0 + per : rows;
+    int bid = (int)blockIdx.x, nblk = (int)gridDim.x;
+    if constexpr (BATCH) {
+        const BatchItem &it = bv.items[batch_find(bv, bid, nblk)];
+        dense = it.dense;
+        partials = it.partials;
+        rows = FP64 ? os_blocks64(it.N) : os_blocks(it.N);
+    }
+    const int64_t per = (rows + nblk - 1) / nblk;
+    const int64_t r0 = bid * per, r1 = r0 + per < rows ? r0 + per : rows;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
         for (int k = 0; k < kMom; ++k) {
             dd s = dd_add(dd{h[k], l[k]}, dd{dense[r * kMom + k], 0.0});
@@ -1977,8 +2067,8 @@ __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64
     block_sum_dd<kMom>(h, l, lds);
     if (threadIdx.x == 0)
         for (int k = 0; k < kMom; ++k) {
-            partials[blockIdx.x * kSlots + 2 * k] = h[k];
-            partials[blockIdx.x * kSlots + 2 * k + 1] = l[k];
+            partials[bid * kSlots + 2 * k] = h[k];
+            partials[bid * kSlots + 2 * k + 1] = l[k];
         }
 }
 
@@ -2124,10 +2214,19 @@ void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const voi
 
 // Step statistics (DataSink / calcBeamParameters) and the momentum sums the
 // next step's rotation and Lorentz factor need.
+template <bool BATCH = false>
 __global__ __launch_bounds__(512) void k_track_post(const double *partials, int nb, TrackState *ts,
                                                     Scalars *sc, StatsOut *out,
-                                                    const unsigned long long *dropped) {
+                                                    const unsigned long long *dropped, BatchView bv = BatchView{}) {
     __shared__ double lds[16 * kMom * 2];
+    if constexpr (BATCH) {
+        const BatchItem &it = bv.items[blockIdx.x];
+        partials = it.partials;
+        ts = it.ts;
+        sc = it.sc;
+        out = (StatsOut *)it.stats;
+        dropped = it.dropped;
+    }
     double m[kMom];
     reduce_partials_dd<kMom>(partials, nb, m, lds);
     if (threadIdx.x != 0) return;
@@ -2154,6 +2253,103 @@ void launch_track_post(hipStream_t s, const double *partials, int nb, TrackState
 void launch_track_init_moments(hipStream_t s, const double *partials, int nb, TrackState *ts,
                                Scalars *sc) {
     k_track_post<<<1, kReduceThreads, 0, s>>>(partials, nb, ts, sc, nullptr, nullptr);
+}
+
+
+// ---- batched steps of many bunches (one launch per pass and kernel class) -----
+// Each kernel is the single-bunch kernel with its arguments taken from the
+// bunch's BatchItem and its block index relative to the bunch's first block,
+// so a batched step computes what single steps of each bunch compute, bit for
+// bit.
+
+// DIR 0: the call starts (call_step0 := step, the context's scalars into the
+// batch's array); 1: it ends (the batch's scalars back to the context)
+template <int DIR>
+__global__ void k_batch_scalars(BatchView bv) {
+    const BatchItem &it = bv.items[blockIdx.x];
+    constexpr int W = (int)(sizeof(Scalars) / sizeof(double));
+    static_assert(sizeof(Scalars) % sizeof(double) == 0, "Scalars copies as doubles");
+    const double *src = (const double *)(DIR == 0 ? it.sc_ctx : it.sc);
+    double *dst = (double *)(DIR == 0 ? it.sc : it.sc_ctx);
+    for (int k = threadIdx.x; k < W; k += blockDim.x) dst[k] = src[k];
+    if (DIR == 0 && threadIdx.x == 0) it.ts->call_step0 = it.ts->step;
+}
+void launch_batch_begin(hipStream_t s, const BatchView &v) {
+    if (v.n > 0) k_batch_scalars<0><<<v.n, 64, 0, s>>>(v);
+}
+void launch_batch_end(hipStream_t s, const BatchView &v) {
+    if (v.n > 0) k_batch_scalars<1><<<v.n, 64, 0, s>>>(v);
+}
+void launch_batch_pre(hipStream_t s, const BatchView &v) {
+    if (v.n > 0) k_track_pre<true><<<v.n, 64, 0, s>>>(nullptr, nullptr, 0, nullptr, nullptr, v);
+}
+static unsigned batch_blocks(const BatchView &v) { return v.n > 0 ? (unsigned)v.blocks : 0u; }
+void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact) {
+    const unsigned nb = batch_blocks(v);
+    if (nb == 0) return;
+    if (exact)
+        k_track_alpha<true, true, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                nullptr, nullptr, nullptr, nullptr, v);
+    else
+        k_track_alpha<false, true, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                 nullptr, nullptr, nullptr, nullptr, nullptr, v);
+}
+void launch_batch_mesh(hipStream_t s, const BatchView &v, bool exact) {
+    if (v.n > 0)
+        k_track_mesh<true><<<v.n, kReduceThreads, 0, s>>>(nullptr, 0, 0, 0, 0, 0.0, nullptr, exact ? 1 : 0, 1,
+                                                          nullptr, v);
+}
+void launch_batch_deposit(hipStream_t s, const BatchView &v, bool exact, bool quni, bool large) {
+    const unsigned nb = batch_blocks(v);
+    if (nb == 0) return;
+#define OPALSC_BDEP(M_, E_, B_)                                                                            \
+    do {                                                                                                   \
+        if (large)                                                                                         \
+            k_deposit_lds<M_, E_, B_, kBoxCapLarge, true><<<nb, kThreads, 0, s>>>(                          \
+                0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, v); \
+        else                                                                                               \
+            k_deposit_lds<M_, E_, B_, kBoxCapSmall, true><<<nb, kThreads, 0, s>>>(                          \
+                0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, v); \
+    } while (0)
+    // BOX instances: a bunch whose chunk boxes are stale (re-sorted in this
+    // step) has a null chunkbox in its item and forms its own
+    if (quni) {
+        if (exact) OPALSC_BDEP(3, true, true);
+        else OPALSC_BDEP(3, false, true);
+    } else {
+        if (exact) OPALSC_BDEP(1, true, true);
+        else OPALSC_BDEP(1, false, true);
+    }
+#undef OPALSC_BDEP
+}
+void launch_batch_gather(hipStream_t s, const BatchView &v, bool exact, bool ext) {
+    const unsigned nb = batch_blocks(v);
+    if (nb == 0) return;
+    static const BinRanges none;
+    if (exact)
+        k_track_gamma_os<false, false, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                      nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                      nullptr, nullptr, none, nullptr, v);
+    else if (ext)
+        k_track_gamma_mixed_os<false, 1, false, true><<<nb, kThreads, 0, s>>>(
+            0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, none, nullptr, v);
+    else
+        k_track_gamma_mixed_os<false, 0, false, true><<<nb, kThreads, 0, s>>>(
+            0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, none, nullptr, v);
+}
+int batch_reduce_blocks() { return kOsReduceBlocks; }
+void launch_batch_reduce(hipStream_t s, const BatchView &v, bool exact) {
+    const unsigned nb = batch_blocks(v);
+    if (nb == 0) return;
+    if (exact) k_reduce_dense<true, true><<<nb, kThreads, 0, s>>>(nullptr, 0, nullptr, v);
+    else k_reduce_dense<true, false><<<nb, kThreads, 0, s>>>(nullptr, 0, nullptr, v);
+}
+void launch_batch_post(hipStream_t s, const BatchView &v) {
+    if (v.n > 0)
+        k_track_post<true><<<v.n, kReduceThreads, 0, s>>>(nullptr, kOsReduceBlocks, nullptr, nullptr, nullptr, nullptr,
+                                                           v);
 }
 
 }  // namespace opalsc

@@ -558,10 +558,14 @@ __device__ inline void grad_stencil(const T *p, int i, int j, int k, int n0, int
     else r2 = id2 * (p[sz] - p[-(ptrdiff_t)sz]);
 }
 
+// grid b = blockIdx.y of a batch: phi + b*total, Eg + b*total, scalars sc + b
 template <typename T>
 __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>::type *Eg, int n0,
                                               int n1, int n2, const Scalars *sc) {
     const size_t total = (size_t)n0 * n1 * n2;
+    phi += blockIdx.y * total;
+    Eg += blockIdx.y * total;
+    sc += blockIdx.y;
     const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
     const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
     const size_t sy = n0, sz = (size_t)n0 * n1;
@@ -797,10 +801,10 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
 void PoissonGrid::destroy() {
     void *ps[] = {rho_i64, rho_f64, A, B, Goct, Tcorner, Ghat, phi, Eg, twd[0], twd[1], twd[2],
                   HimgA, HimgB, Gimg, Esum, Bsum, Gper, bb_rho, bb_A, bb_B, bb_Bi, bb_phi, bb_phii, bb_T, bb_G,
-                  bb_Gh, bb_HA, bb_HB, bb_Gi};
+                  bb_Gh, bb_HA, bb_HB, bb_Gi, bb_E};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    bb_rho = bb_A = bb_B = bb_Bi = bb_phi = bb_phii = bb_T = bb_G = bb_Gh = bb_HA = bb_HB = bb_Gi = nullptr;
+    bb_rho = bb_A = bb_B = bb_Bi = bb_phi = bb_phii = bb_T = bb_G = bb_Gh = bb_HA = bb_HB = bb_Gi = bb_E = nullptr;
     bin_cap = 0;
     bin_img_cap = false;
     rho_i64 = rho_f64 = A = B = Goct = Tcorner = Ghat = phi = Eg = nullptr;
@@ -1072,7 +1076,7 @@ void PoissonGrid::solve(const Scalars *d_sc, int src_kind, int outscale, bool im
 // grown when more bins or the image charge are asked for).
 int PoissonGrid::ensure_bins(int nb, bool image, std::string &err) {
     if (nb <= bin_cap && (!image || bin_img_cap)) return 0;
-    void **ps[] = {&bb_rho, &bb_A, &bb_B, &bb_Bi, &bb_phi, &bb_phii, &bb_T, &bb_G, &bb_Gh, &bb_HA, &bb_HB, &bb_Gi};
+    void **ps[] = {&bb_rho, &bb_A, &bb_B, &bb_Bi, &bb_phi, &bb_phii, &bb_T, &bb_G, &bb_Gh, &bb_HA, &bb_HB, &bb_Gi, &bb_E};
     for (void **q : ps) {
         if (*q) (void)hipFree(*q);
         *q = nullptr;
@@ -1184,6 +1188,24 @@ void PoissonGrid::field_sums(const Scalars *scb, int nb, bool image) {
         k_field_sums<float><<<blocks, 256, 0, stream>>>((const float *)bb_phi, image ? (const float *)bb_phii : nullptr,
                                                         ncell, nb, (float4 *)Esum, (float4 *)Bsum, n[0], n[1], n[2],
                                                         scb);
+    profile_mark(PROF_GRAD);
+}
+
+int PoissonGrid::ensure_bin_fields(std::string &err) {
+    if (bb_E) return 0;
+    const size_t bytes = (size_t)bin_cap * ncell * 4 * (fp64 ? 8 : 4);
+    if (bytes == 0 || hipMalloc(&bb_E, bytes) != hipSuccess) {
+        bb_E = nullptr;
+        err = "hipMalloc of the batched field grids failed";
+        return -1;
+    }
+    return 0;
+}
+
+void PoissonGrid::grad_bins(const Scalars *scb, int nb) {
+    const dim3 grid((unsigned)std::min<size_t>((ncell + 255) / 256, 65536), (unsigned)nb);
+    if (fp64) k_grad<double><<<grid, 256, 0, stream>>>((const double *)bb_phi, (double4 *)bb_E, n[0], n[1], n[2], scb);
+    else k_grad<float><<<grid, 256, 0, stream>>>((const float *)bb_phi, (float4 *)bb_E, n[0], n[1], n[2], scb);
     profile_mark(PROF_GRAD);
 }
 

@@ -41,7 +41,7 @@ EXPORTS = [
     "opalsc_kick", "opalsc_beam_parameters", "opalsc_beam_parameters_noalloc", "opalsc_bunch_set",
     "opalsc_bunch_generate",
     "opalsc_bunch_get", "opalsc_bunch_set_bins", "opalsc_bunch_get_bins", "opalsc_track_init",
-    "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_track", "opalsc_track_get_params",
+    "opalsc_batch_create", "opalsc_batch_destroy", "opalsc_batch_last_error", "opalsc_step_batched", "opalsc_batch_set_fused", "opalsc_track", "opalsc_track_get_params",
     "opalsc_get_mesh", "opalsc_host_atan_cr", "opalsc_host_log_cr", "opalsc_device_atan_cr",
     "opalsc_device_log_cr", "opalsc_host_igf_term", "opalsc_device_igf_term",
     "opalsc_host_green_octant", "opalsc_device_green_octant", "opalsc_profile_enable", "opalsc_profile_read",
@@ -144,6 +144,7 @@ def lib():
         "opalsc_batch_destroy": ([vp], C.c_int),
         "opalsc_batch_last_error": ([vp], C.c_char_p),
         "opalsc_step_batched": ([vp, C.c_int, dp, C.c_int], C.c_int),
+        "opalsc_batch_set_fused": ([vp, C.c_int], C.c_int),
         "opalsc_track_get_params": ([vp, C.POINTER(TrackParams)], C.c_int),
         "opalsc_get_mesh": ([vp, dp, dp], C.c_int),
         "opalsc_host_atan_cr": ([i64, dp, dp], C.c_int),
@@ -550,7 +551,7 @@ class Batch:
     """Several independent bunches stepped together (opalsc_step_batched).
     The contexts must be set up for tracking and outlive the batch."""
 
-    def __init__(self, contexts):
+    def __init__(self, contexts, fused=True):
         self.contexts = list(contexts)
         arr = (C.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
         h = C.c_void_p()
@@ -558,6 +559,7 @@ class Batch:
         if rc != 0:
             raise OpalscError("opalsc_batch_create failed: %d" % rc)
         self._h = h
+        lib().opalsc_batch_set_fused(self._h, 1 if fused else 0)
 
     def step(self, nsteps, graph=True, want_stats=True):
         n = len(self.contexts)

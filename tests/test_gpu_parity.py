@@ -702,6 +702,90 @@ def test_batch_equals_single_steps(graph):
         c.close()
 
 
+def _fused_setups():
+    """Bunches of every kernel class the fused batch forms: FP64 and mixed,
+    equal and unequal charges, with and without an external field, without
+    space charge; two share a grid shape (one grid group, two slots), two
+    re-sort within the call (every 2 and 3 steps)."""
+    specs = [  # N, n, precision, Bz, unequal charges, space charge, sort interval
+        (3000, (16, 16, 16), sc.FP64, 0.0, False, True, 0),
+        (20000, (32, 32, 32), sc.MIXED, 0.5, False, True, 2),
+        (5000, (12, 24, 16), sc.FP64, 1.0, True, True, 3),
+        (60000, (16, 32, 24), sc.MIXED, 0.0, True, True, 0),
+        (7000, (32, 32, 32), sc.MIXED, 0.0, False, True, 0),
+        (4000, (16, 16, 16), sc.MIXED, 0.3, False, False, 0),
+    ]
+    out = []
+    for k, (N, n, prec, Bz, uneq, scon, si) in enumerate(specs):
+        R, P, Q = gaussian_bunch(N, seed=70 + k, q_total=1e-9 * (k + 1))
+        if uneq:
+            Q = Q * np.random.default_rng(80 + k).uniform(0.7, 1.3, N)
+        out.append(dict(N=N, n=n, prec=prec, Bz=Bz, R=R, P=P, Q=Q, sc=scon, si=si))
+    return out
+
+
+def _make_fused_ctx(u):
+    c = sc.SpaceChargeContext(u["n"], precision=u["prec"], max_particles=u["N"], sort_interval=u["si"])
+    c.set_bunch(u["R"], u["P"], u["Q"])
+    c.track_init(1e-11, Bz=u["Bz"], ref_p=(0.0, 0.0, 0.046181), space_charge=u["sc"])
+    return c
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_fused_batch_equals_single_steps(graph):
+    """The fused batch (one launch per pass and kernel class, one per pass and
+    grid shape) equals opalsc_track(ctx, 1) per bunch bit for bit, with re-sorts
+    inside a call (launched directly) between graph replays."""
+    setups = _fused_setups()
+    steps = 7
+    ref = []
+    for u in setups:
+        c = _make_fused_ctx(u)
+        st = [c.track(1)[0] for _ in range(steps)]
+        ref.append((c.get_bunch(), st))
+        c.close()
+    ctxs = [_make_fused_ctx(u) for u in setups]
+    try:
+        with sc.Batch(ctxs) as b:
+            sts = b.step(4, graph=graph) + b.step(3, graph=graph)
+        for i, (c, (bunch_ref, st_ref)) in enumerate(zip(ctxs, ref)):
+            R, P, Q = c.get_bunch()
+            assert np.array_equal(R, bunch_ref[0]) and np.array_equal(P, bunch_ref[1]), i
+            assert np.array_equal(Q, bunch_ref[2]), i
+            for s in range(steps):
+                for k in ("rrms", "prms", "eps_norm", "rmean", "pmean", "t", "dropped"):
+                    assert np.array_equal(sts[s][i][k], st_ref[s][k]), (i, s, k)
+        # a single step of a context after the batch continues from the batch's state
+        u = setups[1]
+        c = _make_fused_ctx(u)
+        for _ in range(steps + 1):
+            c.track(1)
+        ctxs[1].track(1)
+        assert np.array_equal(ctxs[1].get_bunch()[0], c.get_bunch()[0])
+        c.close()
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_fused_and_per_bunch_batches_agree():
+    """The fused and the per-bunch forms of a batch give the same steps."""
+    setups = [u for u in _fused_setups() if u["si"] == 0]
+    out = []
+    for fused in (True, False):
+        ctxs = [_make_fused_ctx(u) for u in setups]
+        with sc.Batch(ctxs, fused=fused) as b:
+            st = b.step(3)
+        out.append(([c.get_bunch() for c in ctxs], st))
+        for c in ctxs:
+            c.close()
+    (b1, s1), (b2, s2) = out
+    for i in range(len(setups)):
+        assert np.array_equal(b1[i][0], b2[i][0]) and np.array_equal(b1[i][1], b2[i][1]), i
+        for s in range(3):
+            assert np.array_equal(s1[s][i]["eps_norm"], s2[s][i]["eps_norm"]), (i, s)
+
+
 def test_batch_recaptures_after_a_new_bunch():
     setups = _batch_setups()[:2]
     ctxs = [_make_batch_ctx(u) for u in setups]
