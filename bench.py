@@ -151,7 +151,7 @@ def c4_context(sc, k, spec, precision, sort_interval=20):
     return c
 
 
-def run_c4(sc, steps, warmup, log):
+def run_c4(sc, steps, warmup, log, only_fused=False):
     """64 independent bunches (c4_specs, per-bunch seeds, mixed precision),
     stepped one bunch after another with opalsc_track, then together with
     opalsc_step_batched without and with the HIP graph."""
@@ -164,19 +164,23 @@ def run_c4(sc, steps, warmup, log):
     from src_amd.opalsc import lib
     out = {"workload": "64 bunches, %d particles in total (1e4-1e6 each), grids 16/32/48/64^3, mixed precision"
                        % total, "steps": steps}
-    cs = make()
-    for c in cs:
-        c.track(warmup, want_stats=False)
-    t0 = time.perf_counter()
-    for c in cs:
-        c.track(steps, want_stats=False)
-    t_seq = time.perf_counter() - t0
-    for c in cs:
-        c.close()
+    t_seq = float("nan")
+    if not only_fused:
+        cs = make()
+        for c in cs:
+            c.track(warmup, want_stats=False)
+        t0 = time.perf_counter()
+        for c in cs:
+            c.track(steps, want_stats=False)
+        t_seq = time.perf_counter() - t0
+        for c in cs:
+            c.close()
     # fused: one launch per pass for all bunches of a kernel class / grid shape
     # (re-sorts every 20 steps launched directly); per_bunch: one graph branch
     # per bunch (no re-sort)
     modes = (("per_bunch_graph", False, True), ("batched_streams", True, False), ("batched_graph", True, True))
+    if only_fused:
+        modes = modes[2:]
     for mode, fused, graph in modes:
         cs = make()
         with sc.Batch(cs, fused=fused) as b:
@@ -193,9 +197,8 @@ def run_c4(sc, steps, warmup, log):
         out[mode + "_roofline_step"] = step_roofline(c4_bytes, out[mode + "_ms_per_step"])
     out["sequential_ms_per_step"] = 1e3 * t_seq / steps
     out["sequential_particle_steps_per_s"] = total * steps / t_seq
-    log("[bench] c4: sequential %.2f ms/step, per-bunch graph %.2f, fused streams %.2f, fused graph %.2f" %
-        (out["sequential_ms_per_step"], out["per_bunch_graph_ms_per_step"], out["batched_streams_ms_per_step"],
-         out["batched_graph_ms_per_step"]))
+    log("[bench] c4: " + ", ".join("%s %.2f ms/step" % (k[:-12], v) for k, v in out.items()
+                                   if k.endswith("_ms_per_step")))
     return out
 
 
@@ -387,6 +390,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the FP64 1e7-particle measurement")
     ap.add_argument("--c4-only", action="store_true",
                     help="run only the 64-bunch batched measurement and print its record (for profiling)")
+    ap.add_argument("--c4-fused-only", action="store_true",
+                    help="with --c4-only: only the fused batch with the HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the space-charge step of one "
@@ -398,7 +403,7 @@ def main():
         if args.c5_only:
             print(json.dumps(run_c5(sc, args.steps, args.warmup, args.sort_interval, log)))
         else:
-            print(json.dumps(run_c4(sc, args.steps, args.warmup, log)))
+            print(json.dumps(run_c4(sc, args.steps, args.warmup, log, args.c4_fused_only)))
         return
     cfg = CONFIGS[args.config]
     N, n = cfg["N"], cfg["n"]

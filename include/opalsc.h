@@ -347,10 +347,11 @@ int opalsc_batch_create(opalsc_batch **out, opalsc_ctx *const *ctxs, int nctx);
 int opalsc_batch_destroy(opalsc_batch *batch);
 const char *opalsc_batch_last_error(const opalsc_batch *batch);
 /* nsteps steps of every bunch (the ParallelTTracker step loop,
- * ParallelTTracker.cpp:338-388, of many independent bunches at once).  Each
- * step is the one opalsc_track(ctx, 1, ...) runs (its own first half push, a
- * re-sort by cell every sort_interval steps of the context), so results equal
- * single-step calls bit for bit.
+ * ParallelTTracker.cpp:338-388, of many independent bunches at once).  The
+ * fused form computes what opalsc_track(ctx, nsteps, ...) computes for every
+ * bunch (the next step's first half push fused into the gather pass, a re-sort
+ * by cell every sort_interval steps of the context); the per-bunch form takes
+ * nsteps single steps.  Either equals single-step calls bit for bit.
  * Fused form (default, when no bunch has energy bins, field maps or a DC beam):
  * every particle pass is one launch over all bunches of a kernel class
  * (precision, equal charges, external field), every Poisson pass one launch

@@ -1379,8 +1379,9 @@ struct opalsc_batch {
     BatchItem *stage = nullptr;  // pinned staging of the item table, kStageSlots copies
     int stage_slot = 0;
     hipEvent_t ev_mesh = nullptr, ev_dep = nullptr;
-    hipGraph_t fgraph = nullptr;
-    hipGraphExec_t fexec = nullptr;
+    // captured steps by (first half push, next step's half push fused): index push + 2 * next
+    hipGraph_t fgraph[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t fexec[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 #define BFAIL(b, code, msg) \
@@ -1402,10 +1403,12 @@ static void batch_drop_graph(opalsc_batch *b) {
 }
 
 static void fused_drop_graph(opalsc_batch *b) {
-    if (b->fexec) (void)hipGraphExecDestroy(b->fexec);
-    if (b->fgraph) (void)hipGraphDestroy(b->fgraph);
-    b->fexec = nullptr;
-    b->fgraph = nullptr;
+    for (int k = 0; k < 4; ++k) {
+        if (b->fexec[k]) (void)hipGraphExecDestroy(b->fexec[k]);
+        if (b->fgraph[k]) (void)hipGraphDestroy(b->fgraph[k]);
+        b->fexec[k] = nullptr;
+        b->fgraph[k] = nullptr;
+    }
 }
 
 static void fused_release(opalsc_batch *b) {
@@ -1635,7 +1638,7 @@ static int fused_plan(opalsc_batch *b, std::vector<long long> &&sig) {
                 if (pass == BP_ALPHA) acc += alpha_blocks(N);
                 else if (pass == BP_DEPOSIT) acc += k.sc_on ? (int)((N + kDepositChunk - 1) / kDepositChunk) : 0;
                 else if (pass == BP_GATHER) acc += k.exact ? os_blocks64(N) : os_blocks(N);
-                else acc += batch_reduce_blocks();
+                else acc += batch_reduce_blocks(k.exact, N);
             }
             offs.push_back(acc);
             k.blocks[pass] = acc;
@@ -1706,13 +1709,17 @@ static int fused_upload(opalsc_batch *b, const std::vector<char> *nobox = nullpt
     return OPALSC_OK;
 }
 
-// One step of every bunch (the step opalsc_track(ctx, 1) takes) on the batch's
-// stream.  sorted: contexts re-sorted by cell after their mesh (then this step
-// is launched directly, not captured: the sort moves the bunch arrays).
-static int fused_step(opalsc_batch *b, const std::vector<int> &sorted) {
+// One step of every bunch on the batch's stream, the step opalsc_track takes
+// inside a call: push, the step's first half push (the call's first step);
+// next, the gather pass also applies the next step's (all but the call's last
+// step), so a later step's bounds pass reads positions only.  sorted: contexts
+// re-sorted by cell after their mesh (then this step is launched directly, not
+// captured: the sort moves the bunch arrays).
+static int fused_step(opalsc_batch *b, const std::vector<int> &sorted, bool push, bool next) {
     hipStream_t m = b->master;
     for (const BatchClass &k : b->classes) launch_batch_pre(m, fused_view(b, k, BP_ITEM));
-    for (const BatchClass &k : b->classes) launch_batch_alpha(m, fused_view(b, k, BP_ALPHA), k.exact);
+    for (const BatchClass &k : b->classes)
+        if (push || k.sc_on) launch_batch_alpha(m, fused_view(b, k, BP_ALPHA), k.exact, push);
     for (const BatchClass &k : b->classes)
         if (k.sc_on) launch_batch_mesh(m, fused_view(b, k, BP_ITEM), k.exact);
     if (!sorted.empty()) {
@@ -1745,19 +1752,20 @@ static int fused_step(opalsc_batch *b, const std::vector<int> &sorted) {
         BHIP(b, hipEventRecord(g->done, g->s));
         BHIP(b, hipStreamWaitEvent(m, g->done, 0));
     }
-    for (const BatchClass &k : b->classes) launch_batch_gather(m, fused_view(b, k, BP_GATHER), k.exact, k.ext);
+    for (const BatchClass &k : b->classes)
+        launch_batch_gather(m, fused_view(b, k, BP_GATHER), k.exact, k.ext, next);
     for (const BatchClass &k : b->classes) launch_batch_reduce(m, fused_view(b, k, BP_REDUCE), k.exact);
-    for (const BatchClass &k : b->classes) launch_batch_post(m, fused_view(b, k, BP_ITEM));
+    for (const BatchClass &k : b->classes) launch_batch_post(m, fused_view(b, k, BP_ITEM), k.exact);
     BHIP(b, hipGetLastError());
     if (!sorted.empty())
         if (int e = fused_upload(b)) return e;  // later steps take their boxes from the bounds pass again
     return OPALSC_OK;
 }
 
-static int fused_capture(opalsc_batch *b) {
-    fused_drop_graph(b);
+static int fused_capture(opalsc_batch *b, bool push, bool next) {
+    const int v = (push ? 1 : 0) + (next ? 2 : 0);
     BHIP(b, hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
-    const int e = fused_step(b, {});
+    const int e = fused_step(b, {}, push, next);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(b->master, &g);
     if (e != OPALSC_OK || e2 != hipSuccess) {
@@ -1765,8 +1773,8 @@ static int fused_capture(opalsc_batch *b) {
         if (e != OPALSC_OK) return e;
         BFAIL(b, OPALSC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e2));
     }
-    b->fgraph = g;
-    BHIP(b, hipGraphInstantiate(&b->fexec, b->fgraph, nullptr, nullptr, 0));
+    b->fgraph[v] = g;
+    BHIP(b, hipGraphInstantiate(&b->fexec[v], b->fgraph[v], nullptr, nullptr, 0));
     return OPALSC_OK;
 }
 
@@ -1782,11 +1790,13 @@ static int fused_run(opalsc_batch *b, int nsteps, bool use_graph) {
             const int interval = c->cfg.sort_interval;
             if (c->track_sc && interval > 0 && (c->steps_done + s) % interval == 0) sorted.push_back(i);
         }
+        const bool push = s == 0, next = s + 1 < nsteps;
+        const int v = (push ? 1 : 0) + (next ? 2 : 0);
         if (sorted.empty() && use_graph) {
-            if (!b->fexec)
-                if (int e = fused_capture(b)) return e;
-            BHIP(b, hipGraphLaunch(b->fexec, m));
-        } else if (int e = fused_step(b, sorted)) {
+            if (!b->fexec[v])
+                if (int e = fused_capture(b, push, next)) return e;
+            BHIP(b, hipGraphLaunch(b->fexec[v], m));
+        } else if (int e = fused_step(b, sorted, push, next)) {
             return e;
         }
     }

@@ -229,13 +229,16 @@ __device__ inline int batch_find(const BatchView &bv, int &bid, int &nblk) {
 void launch_batch_begin(hipStream_t s, const BatchView &v);          // call_step0, scalars in
 void launch_batch_end(hipStream_t s, const BatchView &v);            // scalars out
 void launch_batch_pre(hipStream_t s, const BatchView &v);
-void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact);
+// push: the step's first half push (else the previous gather pass applied it)
+void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact, bool push);
 void launch_batch_mesh(hipStream_t s, const BatchView &v, bool exact);
 void launch_batch_deposit(hipStream_t s, const BatchView &v, bool exact, bool quni, bool large);
-void launch_batch_gather(hipStream_t s, const BatchView &v, bool exact, bool ext);
+// next: also the next step's first half push
+void launch_batch_gather(hipStream_t s, const BatchView &v, bool exact, bool ext, bool next);
 void launch_batch_reduce(hipStream_t s, const BatchView &v, bool exact);
-void launch_batch_post(hipStream_t s, const BatchView &v);
-int batch_reduce_blocks();  // partial rows the batched gather pass leaves for the post pass
+void launch_batch_post(hipStream_t s, const BatchView &v, bool exact);
+// partial rows the batched gather pass of an N-particle bunch leaves for the post pass
+int batch_reduce_blocks(bool exact, int64_t N);
 bool deposit_large_box(int64_t n);  // the deposit of n particles uses the larger LDS box
 
 // Launchers (particles.hip)

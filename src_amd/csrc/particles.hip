@@ -2040,9 +2040,16 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
         if (binp) bin_chunk_store(g0, g1, lgam, binp, chunk, chunk < nchunks);
 }
 
-// Dense per-block moment sums -> at most kOsReduceBlocks double-double partials
-// (rows in a fixed order per block), the layout k_track_post reads.
+// Dense per-block moment sums -> min(rows, kOsReduceBlocks) double-double
+// partials (rows in a fixed order per block), the layout k_track_post reads.
+// With at most kOsReduceBlocks rows each partial is one row, {v + 0, 0}: what
+// the block sum of one value and zeros gives (adding an exact zero to a
+// double-double is the identity, up to the sign of a zero), without the block
+// reduction.
 constexpr int kOsReduceBlocks = 256;
+__host__ __device__ inline int os_reduce_blocks(int64_t rows) {
+    return (int)(rows < kOsReduceBlocks ? (rows < 1 ? 1 : rows) : kOsReduceBlocks);
+}
 template <bool BATCH = false, bool FP64 = false>
 __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64_t rows, double *partials,
                                                       BatchView bv = BatchView{}) {
@@ -2058,6 +2065,14 @@ __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64
     }
     const int64_t per = (rows + nblk - 1) / nblk;
     const int64_t r0 = bid * per, r1 = r0 + per < rows ? r0 + per : rows;
+    if (per == 1) {
+        if (threadIdx.x < kMom) {
+            const double v = r0 < rows ? dense[r0 * kMom + threadIdx.x] : 0.0;
+            partials[bid * kSlots + 2 * threadIdx.x] = v + 0.0;
+            partials[bid * kSlots + 2 * threadIdx.x + 1] = 0.0;
+        }
+        return;
+    }
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
         for (int k = 0; k < kMom; ++k) {
             dd s = dd_add(dd{h[k], l[k]}, dd{dense[r * kMom + k], 0.0});
@@ -2118,8 +2133,9 @@ int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const v
             if (Bg) launch_os64<false, true>(a, blocks64);
             else launch_os64<false, false>(a, blocks64);
         }
-        k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks64, partials);
-        return kOsReduceBlocks;
+        const int nr = os_reduce_blocks(blocks64);
+        k_reduce_dense<<<nr, kThreads, 0, s>>>(dense, blocks64, partials);
+        return nr;
     }
     const int blocks = os_blocks(b.N);
     if (next) {
@@ -2129,8 +2145,9 @@ int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const v
         if (ext) launch_os_mixed_b<false, 1>(a, blocks);
         else launch_os_mixed_b<false, 0>(a, blocks);
     }
-    k_reduce_dense<<<kOsReduceBlocks, kThreads, 0, s>>>(dense, blocks, partials);
-    return kOsReduceBlocks;
+    const int nr = os_reduce_blocks(blocks);
+    k_reduce_dense<<<nr, kThreads, 0, s>>>(dense, blocks, partials);
+    return nr;
 }
 
 // Blocks of `kernel` (kThreads each) that the device holds at once, per device.
@@ -2220,8 +2237,10 @@ __global__ __launch_bounds__(512) void k_track_post(const double *partials, int 
                                                     const unsigned long long *dropped, BatchView bv = BatchView{}) {
     __shared__ double lds[16 * kMom * 2];
     if constexpr (BATCH) {
+        // nb: 1 for the FP64 gather pass's rows, 0 for the mixed one's
         const BatchItem &it = bv.items[blockIdx.x];
         partials = it.partials;
+        nb = os_reduce_blocks(nb ? os_blocks64(it.N) : os_blocks(it.N));
         ts = it.ts;
         sc = it.sc;
         out = (StatsOut *)it.stats;
@@ -2284,15 +2303,20 @@ void launch_batch_pre(hipStream_t s, const BatchView &v) {
     if (v.n > 0) k_track_pre<true><<<v.n, 64, 0, s>>>(nullptr, nullptr, 0, nullptr, nullptr, v);
 }
 static unsigned batch_blocks(const BatchView &v) { return v.n > 0 ? (unsigned)v.blocks : 0u; }
-void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact) {
+void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact, bool push) {
     const unsigned nb = batch_blocks(v);
     if (nb == 0) return;
-    if (exact)
-        k_track_alpha<true, true, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                nullptr, nullptr, nullptr, nullptr, v);
-    else
-        k_track_alpha<false, true, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                 nullptr, nullptr, nullptr, nullptr, nullptr, v);
+#define OPALSC_BALPHA(E_, P_)                                                                                     \
+    k_track_alpha<E_, P_, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, \
+                                                        nullptr, nullptr, nullptr, nullptr, v)
+    if (exact) {
+        if (push) OPALSC_BALPHA(true, true);
+        else OPALSC_BALPHA(true, false);
+    } else {
+        if (push) OPALSC_BALPHA(false, true);
+        else OPALSC_BALPHA(false, false);
+    }
+#undef OPALSC_BALPHA
 }
 void launch_batch_mesh(hipStream_t s, const BatchView &v, bool exact) {
     if (v.n > 0)
@@ -2322,33 +2346,38 @@ void launch_batch_deposit(hipStream_t s, const BatchView &v, bool exact, bool qu
     }
 #undef OPALSC_BDEP
 }
-void launch_batch_gather(hipStream_t s, const BatchView &v, bool exact, bool ext) {
-    const unsigned nb = batch_blocks(v);
-    if (nb == 0) return;
+template <bool NEXT>
+static void batch_gather(hipStream_t s, const BatchView &v, unsigned nb, bool exact, bool ext) {
     static const BinRanges none;
     if (exact)
-        k_track_gamma_os<false, false, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                      nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                      nullptr, nullptr, none, nullptr, v);
+        k_track_gamma_os<NEXT, false, true><<<nb, kThreads, 0, s>>>(0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                     nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                     nullptr, nullptr, none, nullptr, v);
     else if (ext)
-        k_track_gamma_mixed_os<false, 1, false, true><<<nb, kThreads, 0, s>>>(
+        k_track_gamma_mixed_os<NEXT, 1, false, true><<<nb, kThreads, 0, s>>>(
             0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
             nullptr, none, nullptr, v);
     else
-        k_track_gamma_mixed_os<false, 0, false, true><<<nb, kThreads, 0, s>>>(
+        k_track_gamma_mixed_os<NEXT, 0, false, true><<<nb, kThreads, 0, s>>>(
             0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
             nullptr, none, nullptr, v);
 }
-int batch_reduce_blocks() { return kOsReduceBlocks; }
+void launch_batch_gather(hipStream_t s, const BatchView &v, bool exact, bool ext, bool next) {
+    const unsigned nb = batch_blocks(v);
+    if (nb == 0) return;
+    if (next) batch_gather<true>(s, v, nb, exact, ext);
+    else batch_gather<false>(s, v, nb, exact, ext);
+}
+int batch_reduce_blocks(bool exact, int64_t N) { return os_reduce_blocks(exact ? os_blocks64(N) : os_blocks(N)); }
 void launch_batch_reduce(hipStream_t s, const BatchView &v, bool exact) {
     const unsigned nb = batch_blocks(v);
     if (nb == 0) return;
     if (exact) k_reduce_dense<true, true><<<nb, kThreads, 0, s>>>(nullptr, 0, nullptr, v);
     else k_reduce_dense<true, false><<<nb, kThreads, 0, s>>>(nullptr, 0, nullptr, v);
 }
-void launch_batch_post(hipStream_t s, const BatchView &v) {
+void launch_batch_post(hipStream_t s, const BatchView &v, bool exact) {
     if (v.n > 0)
-        k_track_post<true><<<v.n, kReduceThreads, 0, s>>>(nullptr, kOsReduceBlocks, nullptr, nullptr, nullptr, nullptr,
+        k_track_post<true><<<v.n, kReduceThreads, 0, s>>>(nullptr, exact ? 1 : 0, nullptr, nullptr, nullptr, nullptr,
                                                            v);
 }
 
