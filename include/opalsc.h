@@ -367,6 +367,23 @@ int opalsc_step_batched(opalsc_batch *batch, int nsteps, opalsc_stats *per_step,
 /* enable != 0 (default): the fused form where every bunch allows it; 0: the
  * per-bunch form.  No counterpart in the reference (a batching choice). */
 int opalsc_batch_set_fused(opalsc_batch *batch, int enable);
+/* Autotuning (no counterpart in the reference: tile shapes of this library's
+ * kernels; process-wide, set before creating the contexts or batches it should
+ * apply to, not thread-safe).  src_amd/autotune.py times the candidates on the
+ * device and writes the table the library is built with.
+ * FFT tiles: E elements per thread for the transforms of length L of one
+ * precision (a workgroup holds W = 512 E / L lines).  _candidates writes the E
+ * values the kernels take (at most cap) and returns their count;
+ * _set overrides the table (E = 0 removes the override; OPALSC_EINVAL for an E
+ * the kernels cannot take); _get returns the E a grid created now would use. */
+int opalsc_tune_fft_candidates(int L, int precision, int *E, int cap);
+int opalsc_tune_fft_set(int L, int precision, int E);
+int opalsc_tune_fft_get(int L, int precision);
+/* Deposit: a bunch of at most max_particles particles deposits through the
+ * larger LDS box (5632 cells, three blocks per CU), a larger one through the
+ * smaller box (4864 cells, four blocks per CU). */
+int opalsc_tune_deposit_set(int64_t max_particles);
+int64_t opalsc_tune_deposit_get(void);
 int opalsc_track_get_params(opalsc_ctx *ctx, opalsc_track_params *out);
 /* Mesh (hr, origin) of the most recent track step or boundp call. */
 int opalsc_get_mesh(opalsc_ctx *ctx, double hr[3], double origin[3]);

@@ -6,6 +6,7 @@ scalar C++) and without packed-FP32 instructions.  Objects go to
 src_amd/_build/, the library next to this file.
 """
 import concurrent.futures as cf
+import json
 import os
 import shutil
 import subprocess
@@ -17,14 +18,38 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libopalsc.so")
 
+TUNING = os.path.join(HERE, "tuning.json")  # written by src_amd/autotune.py on an MI355X
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OPALSC_ARCH", "gfx950")
 FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
-    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-I", BUILD,
 ]
+
+DEFAULT_LARGE_BOX_MAX = 8 << 20  # particles; the deposit's larger LDS box up to this size
+
+
+def write_tuning_header():
+    """_build/tuning.hpp from the autotuner's table: FFT tile entries
+    {L, fp64, E} and the deposit's large-box threshold (defaults without a table)."""
+    tiles, large = [], DEFAULT_LARGE_BOX_MAX
+    if os.path.exists(TUNING):
+        with open(TUNING) as f:
+            t = json.load(f)
+        for e in t.get("fft_tiles", []):
+            tiles.append("{%d, %d, %d}, " % (int(e["L"]), 1 if e["precision"] == "fp64" else 0, int(e["E"])))
+        large = int(t.get("deposit", {}).get("large_box_max_particles", large))
+    text = ("// generated by src_amd/build.py from src_amd/tuning.json; do not edit\n"
+            "#pragma once\n"
+            "#define OPALSC_TUNED_FFT_TILES %s\n"
+            "#define OPALSC_TUNED_LARGE_BOX_MAX ((int64_t)%dLL)\n" % ("".join(tiles), large))
+    path = os.path.join(BUILD, "tuning.hpp")
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
 SOURCES = ["api.hip", "bins.hip", "particles.hip", "poisson.hip", "sort.hip", "noalloc.hip", "io.cpp"]
 
 
@@ -48,6 +73,7 @@ def build(verbose=False, jobs=4):
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
     os.makedirs(BUILD, exist_ok=True)
+    write_tuning_header()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB + ".tmp"

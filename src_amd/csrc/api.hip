@@ -1491,6 +1491,14 @@ extern "C" int opalsc_batch_destroy(opalsc_batch *b) {
 
 extern "C" const char *opalsc_batch_last_error(const opalsc_batch *b) { return b ? b->err.c_str() : ""; }
 
+extern "C" int opalsc_tune_deposit_set(int64_t max_particles) {
+    if (max_particles < 0) return OPALSC_EINVAL;
+    set_deposit_large_box_max(max_particles);
+    return OPALSC_OK;
+}
+
+extern "C" int64_t opalsc_tune_deposit_get(void) { return deposit_large_box_max(); }
+
 extern "C" int opalsc_batch_set_fused(opalsc_batch *b, int enable) {
     if (!b) return OPALSC_EINVAL;
     b->fused_on = enable != 0;

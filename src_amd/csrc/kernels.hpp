@@ -240,6 +240,8 @@ void launch_batch_post(hipStream_t s, const BatchView &v, bool exact);
 // partial rows the batched gather pass of an N-particle bunch leaves for the post pass
 int batch_reduce_blocks(bool exact, int64_t N);
 bool deposit_large_box(int64_t n);  // the deposit of n particles uses the larger LDS box
+void set_deposit_large_box_max(int64_t n);
+int64_t deposit_large_box_max();
 
 // Launchers (particles.hip)
 int red_blocks(int64_t N);

@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "gather.hpp"
 #include "reduce.hpp"
+#include "tuning.hpp"  // generated from src_amd/tuning.json by src_amd/build.py
 
 namespace opalsc {
 
@@ -176,7 +177,9 @@ __device__ inline int small_div(int a, int b, float rb) {
 // the device with fewer blocks than that, and there the larger box (fewer
 // chunks falling back to global atomics) wins (64 small bunches: 6.2 -> 5.9 ms).
 constexpr int kBoxCapSmall = 4864, kBoxCapLarge = 5632;
-constexpr int64_t kLargeBoxMaxParticles = int64_t(8) << 20;
+// Bunches of at most this many particles deposit through the larger LDS box
+// (one block fewer per CU); set by the autotuner's table or opalsc_tune_deposit_set.
+static int64_t g_large_box_max = OPALSC_TUNED_LARGE_BOX_MAX;
 
 // Adds a cell sum v (deposit scale) to the charge grid: one word per cell on
 // the mixed path; on the FP64 path the exact split of fixed_point_scale,
@@ -533,7 +536,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (nd) atomicAdd(dropped, nd);
 }
 
-bool deposit_large_box(int64_t n) { return n <= kLargeBoxMaxParticles; }
+bool deposit_large_box(int64_t n) { return n <= g_large_box_max; }
+void set_deposit_large_box_max(int64_t n) { g_large_box_max = n; }
+int64_t deposit_large_box_max() { return g_large_box_max; }
 
 void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64_t hi, int64_t Nall,
                           const double *x, const double *y, const double *z, double *q, const double *qd,
@@ -543,7 +548,7 @@ void launch_deposit_range(hipStream_t s, int mode, bool exact, int64_t lo, int64
     const int64_t chunk0 = lo / kChunk;
     const unsigned blocks = (unsigned)((hi - 1) / kChunk - chunk0 + 1);
     const bool box = chunkbox != nullptr;
-    const bool large = hi - lo <= kLargeBoxMaxParticles;
+    const bool large = deposit_large_box(hi - lo);
 #define OPALSC_DEP(M_, E_, B_)                                                                              \
     do {                                                                                                    \
         if (large)                                                                                          \
@@ -2071,6 +2076,31 @@ __global__ __launch_bounds__(256) void k_reduce_dense(const double *dense, int64
             partials[bid * kSlots + 2 * threadIdx.x] = v + 0.0;
             partials[bid * kSlots + 2 * threadIdx.x + 1] = 0.0;
         }
+        return;
+    }
+    if (per <= 64) {
+        // every row in the first wave: the other waves' sums are exact zeros, so
+        // the block sum is the first wave's butterfly followed by three
+        // additions of a zero (kept for the sign of a zero sum)
+        static_assert(kThreads == 256, "four waves per block");
+        if (threadIdx.x >= 64) return;
+        if (r0 + threadIdx.x < r1)
+            for (int k = 0; k < kMom; ++k) h[k] = dense[(r0 + threadIdx.x) * kMom + k] + 0.0;
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int k = 0; k < kMom; ++k) {
+                const double oh = __shfl_xor(h[k], off, 64), ol = __shfl_xor(l[k], off, 64);
+                const dd r = dd_add(dd{h[k], l[k]}, dd{oh, ol});
+                h[k] = r.hi;
+                l[k] = r.lo;
+            }
+        if (threadIdx.x == 0)
+            for (int k = 0; k < kMom; ++k) {
+                dd a = {h[k], l[k]};
+                for (int w = 1; w < kThreads / 64; ++w) a = dd_add(a, dd{0.0, 0.0});
+                partials[bid * kSlots + 2 * k] = a.hi;
+                partials[bid * kSlots + 2 * k + 1] = a.lo;
+            }
         return;
     }
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)

@@ -18,8 +18,10 @@
 // The Green's function is real and even in every axis on the doubled grid, so
 // its transform Ghat is real and even as well: it is the 3D DCT-I of the
 // (n+1)^3 octant, computed here with three DCT passes instead of a (2n)^3 r2c.
+#include "../../include/opalsc.h"
 #include "common.hpp"
 #include "fft.hpp"
+#include "tuning.hpp"  // generated from src_amd/tuning.json by src_amd/build.py
 #include "igfmath.hpp"
 #include "kernels.hpp"
 
@@ -649,8 +651,55 @@ __global__ void k_widen(const T *in, double *out, size_t n, int comps, int strid
 // Host side
 // ---------------------------------------------------------------------------
 
-// Tile shape for length L: the smallest E (elements per thread) whose tile has
-// W >= 128 bytes / complex size lines, within the register budget.
+// ---- tile shapes: the autotuned table, overrides, the default rule ----------
+// Tiles are chosen per transform length L and precision by E, the elements per
+// thread (W = 512 E / L lines per workgroup).  The table the autotuner wrote
+// (src_amd/tuning.json, compiled in as tuning.hpp) comes first, then
+// overrides set through opalsc_tune_fft_set, then the default rule: the
+// smallest E whose tile rows are whole 128-byte lines.
+struct TileEntry {
+    int L, fp64, E;
+};
+static const TileEntry kTunedTiles[] = {OPALSC_TUNED_FFT_TILES{0, 0, 0}};
+static std::vector<TileEntry> g_tile_override;
+
+// E values the kernels can take for length L (radices 8/4/2 then 3 dividing
+// E, W a power of two, the tile within the register budget and within the
+// 160 KiB of LDS a gfx950 workgroup can hold: the tile of pitch W + 1, and the
+// FP32 twiddles)
+static int tile_candidates(int L, bool fp64, int *out, int cap) {
+    int a = 0, b = 0, m = L;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    while (m % 3 == 0) { m /= 3; ++b; }
+    if (m != 1 || b > 1 || a < 2) return -1;
+    static const int c2[] = {4, 8, 16, 32}, c3[] = {6, 12, 24, 48};
+    const int *cand = b ? c3 : c2;
+    const int emax = fp64 ? (b ? 24 : 16) : (b ? 48 : 32);
+    int k = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int e = cand[c];
+        if (e > emax || (e * kFftThreads) % L != 0) continue;
+        const int W = e * kFftThreads / L;
+        if (W < 1 || (W & (W - 1)) != 0) continue;
+        const size_t cs = fp64 ? 16 : 8;
+        if ((size_t)L * (W + 1) * cs + (fp64 ? 0 : (size_t)L * cs) > 160 * 1024) continue;
+        if (k < cap) out[k] = e;
+        ++k;
+    }
+    return k;
+}
+
+static int tile_lookup(int L, bool fp64) {
+    for (const TileEntry &t : g_tile_override)
+        if (t.L == L && t.fp64 == (fp64 ? 1 : 0)) return t.E;
+    for (const TileEntry &t : kTunedTiles)
+        if (t.L == L && t.fp64 == (fp64 ? 1 : 0)) return t.E;
+    return 0;
+}
+
+// Tile shape for length L: the tuned or overridden E if the kernels take it,
+// else the smallest E whose tile has W >= 128 bytes / complex size lines,
+// within the register budget.
 static bool make_plan(int L, bool fp64, FftPlan &pl) {
     int a = 0, b = 0, m = L;
     while (m % 2 == 0) { m /= 2; ++a; }
@@ -658,15 +707,14 @@ static bool make_plan(int L, bool fp64, FftPlan &pl) {
     if (m != 1 || b > 1 || a < 2) return false;
     pl.L = L;
     const int wmin = fp64 ? 8 : 16;
-    static const int c2[] = {4, 8, 16, 32}, c3[] = {6, 12, 24, 48};
-    const int *cand = b ? c3 : c2;
-    const int emax = fp64 ? (b ? 24 : 16) : (b ? 48 : 32);
+    int cand[8];
+    const int nc = tile_candidates(L, fp64, cand, 8);
     pl.E = 0;
-    for (int c = 0; c < 4; ++c) {
-        const int e = cand[c];
-        if (e > emax || (e * kFftThreads) % L != 0) continue;
-        pl.E = e;
-        if (e * kFftThreads / L >= wmin) break;
+    const int tuned = tile_lookup(L, fp64);
+    for (int c = 0; c < nc && tuned; ++c)
+        if (cand[c] == tuned) pl.E = tuned;
+    for (int c = 0; c < nc && pl.E == 0; ++c) {
+        if (cand[c] * kFftThreads / L >= wmin || c == nc - 1) pl.E = cand[c];
     }
     if (pl.E == 0) return false;
     pl.W = pl.E * kFftThreads / L;
@@ -1249,3 +1297,34 @@ void PoissonGrid::widen(int which, double *out) {
 }
 
 }  // namespace opalsc
+
+// ---- autotuning entry points (include/opalsc.h) ----
+extern "C" int opalsc_tune_fft_candidates(int L, int precision, int *E, int cap) {
+    if (cap < 0 || (cap > 0 && !E)) return OPALSC_EINVAL;
+    const int k = opalsc::tile_candidates(L, precision == OPALSC_FP64, E, cap);
+    return k < 0 ? OPALSC_EINVAL : k;
+}
+
+extern "C" int opalsc_tune_fft_set(int L, int precision, int E) {
+    using opalsc::g_tile_override;
+    const int fp64 = precision == OPALSC_FP64 ? 1 : 0;
+    bool ok = E == 0;
+    int cand[8];
+    const int nc = opalsc::tile_candidates(L, fp64 != 0, cand, 8);
+    for (int c = 0; c < nc; ++c) ok |= cand[c] == E;
+    if (!ok) return OPALSC_EINVAL;  // the previous setting stays
+    for (size_t i = 0; i < g_tile_override.size(); ++i)
+        if (g_tile_override[i].L == L && g_tile_override[i].fp64 == fp64) {
+            g_tile_override.erase(g_tile_override.begin() + (long)i);
+            break;
+        }
+    if (E != 0) g_tile_override.push_back({L, fp64, E});
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_tune_fft_get(int L, int precision) {
+    opalsc::FftPlan pl;
+    if (!opalsc::make_plan(L, precision == OPALSC_FP64, pl)) return OPALSC_EINVAL;
+    return pl.E;
+}
+

@@ -47,6 +47,8 @@ EXPORTS = [
     "opalsc_host_green_octant", "opalsc_device_green_octant", "opalsc_profile_enable", "opalsc_profile_read",
     "opalsc_copy_bandwidth", "opalsc_fromfile_count", "opalsc_fromfile_read", "opalsc_stat_row_get",
     "opalsc_stat_write", "opalsc_fieldmap_add", "opalsc_fieldmap_clear", "opalsc_fieldmap_eval",
+    "opalsc_tune_fft_candidates", "opalsc_tune_fft_set", "opalsc_tune_fft_get", "opalsc_tune_deposit_set",
+    "opalsc_tune_deposit_get",
 ]
 
 
@@ -165,6 +167,11 @@ def lib():
         "opalsc_profile_enable": ([vp, C.c_int], C.c_int),
         "opalsc_profile_read": ([vp, C.c_int, C.POINTER(C.c_char_p), dp, dp], C.c_int),
         "opalsc_copy_bandwidth": ([vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "opalsc_tune_fft_candidates": ([C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int], C.c_int),
+        "opalsc_tune_fft_set": ([C.c_int, C.c_int, C.c_int], C.c_int),
+        "opalsc_tune_fft_get": ([C.c_int, C.c_int], C.c_int),
+        "opalsc_tune_deposit_set": ([C.c_int64], C.c_int),
+        "opalsc_tune_deposit_get": ([], C.c_int64),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -195,6 +202,33 @@ def _per_particle(a, N, name, vec=False):
     if a.shape != want:
         raise ValueError("%s must have shape %s, got %s" % (name, want, a.shape))
     return a
+
+
+def tune_fft_candidates(L, precision):
+    """Tile parameters E (elements per thread) the FFT kernels take for length L."""
+    buf = (C.c_int * 8)()
+    k = lib().opalsc_tune_fft_candidates(int(L), int(precision), buf, 8)
+    if k < 0:
+        raise ValueError("no FFT tiles for length %d" % L)
+    return list(buf[:k])
+
+
+def tune_fft_set(L, precision, E):
+    """Override the tile of length-L transforms for grids created afterwards (E = 0: the table's)."""
+    _check(None, lib().opalsc_tune_fft_set(int(L), int(precision), int(E)))
+
+
+def tune_fft_get(L, precision):
+    return lib().opalsc_tune_fft_get(int(L), int(precision))
+
+
+def tune_deposit_set(max_particles):
+    """Bunches of at most max_particles deposit through the larger LDS box."""
+    _check(None, lib().opalsc_tune_deposit_set(int(max_particles)))
+
+
+def tune_deposit_get():
+    return int(lib().opalsc_tune_deposit_get())
 
 
 def stats_to_dict(s):

@@ -113,3 +113,46 @@ def test_mirror_rejects_short_per_particle_arrays_before_the_c_call():
     with pytest.raises(ValueError, match="dt"):
         c.computeSelfFieldsBinned(R, good["P"], np.zeros(4, np.int32), 1, 1e-12, np.zeros(1), 1e-11,
                                   np.zeros(3), np.ones(3))
+
+
+def test_fft_tile_candidates_and_overrides():
+    """The autotuner's knobs: the tiles the kernels take per transform length
+    (E elements per thread, W = 512 E / L lines a power of two, radix 3 only
+    for L = 3 * 2^a), overrides that apply and reset, invalid tiles refused."""
+    assert sc.tune_fft_candidates(32, sc.MIXED) == [4, 8, 16, 32]
+    assert sc.tune_fft_candidates(96, sc.MIXED) == [6, 12, 24]
+    assert sc.tune_fft_candidates(96, sc.FP64) == [6, 12]
+    assert sc.tune_fft_candidates(512, sc.FP64) == [4, 8, 16]
+    for L in (32, 64, 96, 128, 256, 512):
+        for p in (sc.MIXED, sc.FP64):
+            cs = 16 if p == sc.FP64 else 8
+            for E in sc.tune_fft_candidates(L, p):
+                W = E * 512 // L
+                assert E * 512 % L == 0 and W & (W - 1) == 0
+                assert L * (W + 1) * cs + (0 if p == sc.FP64 else L * cs) <= 160 * 1024
+    with pytest.raises(ValueError):
+        sc.tune_fft_candidates(40, sc.MIXED)  # 5 * 2^3: no radix-5 stage
+    base = sc.tune_fft_get(128, sc.MIXED)
+    assert base in sc.tune_fft_candidates(128, sc.MIXED)
+    try:
+        sc.tune_fft_set(128, sc.MIXED, 32)
+        assert sc.tune_fft_get(128, sc.MIXED) == 32
+        assert sc.tune_fft_get(128, sc.FP64) in sc.tune_fft_candidates(128, sc.FP64)
+        with pytest.raises(sc.OpalscError):
+            sc.tune_fft_set(128, sc.MIXED, 6)  # not a tile of a power-of-two length
+        assert sc.tune_fft_get(128, sc.MIXED) == 32  # a refused tile leaves the override
+    finally:
+        sc.tune_fft_set(128, sc.MIXED, 0)
+    assert sc.tune_fft_get(128, sc.MIXED) == base
+
+
+def test_deposit_tuning_threshold_round_trip():
+    old = sc.tune_deposit_get()
+    try:
+        sc.tune_deposit_set(12345)
+        assert sc.tune_deposit_get() == 12345
+        with pytest.raises(sc.OpalscError):
+            sc.tune_deposit_set(-1)
+    finally:
+        sc.tune_deposit_set(old)
+    assert sc.tune_deposit_get() == old
