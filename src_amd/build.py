@@ -69,6 +69,36 @@ def _compile(src, verbose):
     return obj
 
 
+H5LIB = os.path.join(HERE, "libopalsc_h5.so")
+HDF5_PREFIXES = [os.environ.get("OPALSC_HDF5", ""), "/opt/conda", "/usr", "/usr/local"]
+
+
+def hdf5_prefix():
+    for p in HDF5_PREFIXES:
+        if p and os.path.exists(os.path.join(p, "include", "hdf5.h")):
+            return p
+    return None
+
+
+def build_h5(verbose=False):
+    """libopalsc_h5.so (H5Part files, host code) against the HDF5 C library;
+    skipped with a message where no HDF5 is installed."""
+    prefix = hdf5_prefix()
+    if prefix is None:
+        print("build: no HDF5 found (OPALSC_HDF5); libopalsc_h5.so not built", file=sys.stderr)
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(prefix, "include"),
+           os.path.join(CSRC, "h5part.cpp"), "-o", H5LIB + ".tmp", "-L", os.path.join(prefix, "lib"),
+           "-Wl,-rpath," + os.path.join(prefix, "lib"), "-lhdf5"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for h5part.cpp:\n" + r.stdout)
+    os.replace(H5LIB + ".tmp", H5LIB)
+    return H5LIB
+
+
 def build(verbose=False, jobs=4):
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
@@ -82,6 +112,7 @@ def build(verbose=False, jobs=4):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)
     os.replace(tmp, LIB)
+    build_h5(verbose)
     return LIB
 
 

@@ -514,12 +514,13 @@ class SpaceChargeContext:
         return counts[:nb.value], gammas[:nb.value]
 
     def track_init(self, dT, charge=1.0, mass_eV=M_P_EV, coupling=COUPLING, Bz=0.0, Ez0=0.0,
-                   rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), space_charge=True):
+                   rf_freq=0.0, rf_phase=0.0, ref_p=(0.0, 0.0, 1.0), space_charge=True, t=0.0,
+                   path_length=0.0, ref_r=(0.0, 0.0, 0.0)):
         p = TrackParams()
         p.dT, p.charge, p.mass_eV, p.coupling = dT, charge, mass_eV, coupling
         p.Bz, p.Ez0, p.rf_freq, p.rf_phase = Bz, Ez0, rf_freq, rf_phase
-        p.t, p.pathLength = 0.0, 0.0
-        p.RefPartR[:] = [0.0, 0.0, 0.0]
+        p.t, p.pathLength = float(t), float(path_length)
+        p.RefPartR[:] = list(ref_r)
         p.RefPartP[:] = list(ref_p)
         p.space_charge = int(space_charge)
         self._c(lib().opalsc_track_init(self._h, C.byref(p)))

@@ -4,6 +4,8 @@ Bars (from the FP64 drop-in requirement): cell indices, CIC offsets, the
 cell sort, push and kick bit-exact; rho/phi/E/Ef/Bf within 1e-12 relative L2;
 statistics within 1e-10 relative where the statistic is well conditioned.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -1027,3 +1029,42 @@ def test_device_array_entry_points_equal_host_ones():
                 c._c(L.opalsc_compute_potential_dev(c._h, host.ctypes.data, ptr(hr)))
         finally:
             H.free()
+
+
+def test_h5part_dump_and_restart(tmp_path):
+    """A tracker dump (writeStep of the resident bunch) and a restart from it
+    (doRestartOpalT over readStep): the restarted tracker continues as the
+    original one does, and the step attributes are the tracker's statistics."""
+    h5 = pytest.importorskip("src_amd.h5part")
+    if not os.path.exists(h5.LIB_PATH):
+        pytest.skip("libopalsc_h5.so not built (no HDF5)")
+    n, N = (16, 16, 16), 5000
+    R, P, Q = gaussian_bunch(N, seed=91, q_total=1e-9)
+    path = str(tmp_path / "dump.h5")
+    with ctx(n, N=N) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.5)
+        st = c.track(3)
+        h5.create(path, dump_frequency=1)
+        k = h5.write_tracker_step(path, c, local_track_step=3, global_track_step=3)
+        tp = c.track_params()
+        Rd, Pd, Qd = c.get_bunch()
+        ref = c.track(3)
+        Rr, Pr, _ = c.get_bunch()
+    assert k == 0
+    d = h5.read_step(path)
+    a = d["attrs"]
+    assert np.array_equal(d["R"], Rd) and np.array_equal(d["P"], Pd) and np.array_equal(d["Q"], Qd)
+    assert a.time == tp.t and a.spos == tp.pathLength and list(a.ref_p) == list(tp.RefPartP)
+    assert np.allclose(list(a.rms_x), st[-1]["rrms"], rtol=1e-12, atol=0)
+    assert np.allclose(list(a.eps_norm), st[-1]["eps_norm"], rtol=1e-12, atol=0)
+    assert list(a.max_x) == list(Rd.max(axis=0)) and list(a.min_p) == list(Pd.min(axis=0))
+    assert a.charge == pytest.approx(1e-9, rel=1e-12) and a.local_track_step == 3
+    with ctx(n, N=N) as c2:
+        h5.restart(c2, path, 1e-11, Bz=0.5)
+        got = c2.track(3)
+        Rg, Pg, _ = c2.get_bunch()
+    assert rel_l2(Rg, Rr) < 1e-13 and rel_l2(Pg - Pd, Pr - Pd) < 1e-10
+    for x, y in zip(got, ref):
+        assert x["t"] == y["t"] and x["pathLength"] == pytest.approx(y["pathLength"], rel=1e-14)
+        assert np.allclose(x["eps_norm"], y["eps_norm"], rtol=1e-10, atol=0)
