@@ -64,7 +64,7 @@ typedef struct {
                               OPALSC_BC_PERIODIC (DC beams: the solver is periodic in z,
                               FFTPoissonSolver.cpp:60,130-155, the field's z faces
                               periodic and particles wrapped in z, PartBunch.cpp:1079-1098;
-                              no energy bins; n[2] = 2^a or 3*2^a with a >= 2) */
+                              no energy bins; n[2] = 2^a, 3*2^a or 5*2^a with a >= 2) */
 } opalsc_config;
 
 /* DataSink statistics (src/Structure/DataSink.cpp:360-391) as computed by

@@ -12,9 +12,9 @@
 // are read into registers, twiddled, transformed, and written back after a
 // barrier.  E = W*L/kFftThreads elements per thread is a compile-time constant
 // divisible by every radix used, so each thread owns exactly E/R butterflies
-// per stage.  Radices 8, 4 and 2 come first and radix 3 (L = 3*2^a) last, so
-// the span Ns of every stage is a power of two and all index arithmetic is
-// shifts and masks.  Twiddles come from LDS (FP32) or through the cache (FP64,
+// per stage.  Radices 8, 4 and 2 come first and radix 3 (L = 3*2^a) or 5
+// (L = 5*2^a) last, so the span Ns of every stage is a power of two and all
+// index arithmetic is shifts and masks.  Twiddles come from LDS (FP32) or through the cache (FP64,
 // whose tiles leave no room for them).
 #pragma once
 #include "common.hpp"
@@ -73,6 +73,26 @@ __device__ inline void dft3(C *v, bool inv) {
     v[2] = csub(t2, r);
 }
 
+// X_k = sum_n x_n w^(nk), w = exp(-+2 pi i / 5): with t1 = x1 + x4, t2 = x2 + x3,
+// t3 = x1 - x4, t4 = x2 - x3, X1,4 = (x0 + c1 t1 + c2 t2) -+ i (s1 t3 + s2 t4) and
+// X2,3 = (x0 + c2 t1 + c1 t2) -+ i (s2 t3 - s1 t4), c_m = cos(2 pi m / 5), s_m = sin(2 pi m / 5)
+template <typename C>
+__device__ inline void dft5(C *v, bool inv) {
+    using T = decltype(v[0].x);
+    const T c1 = T(0.309016994374947424102293417182819059), c2 = T(-0.809016994374947424102293417182819059);
+    const T s1 = T(0.951056516295153572116439333379382143), s2 = T(0.587785252292473129168705954639072769);
+    const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const C a1 = C{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+    const C a2 = C{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    const C b1 = cmul_mi(C{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, inv);
+    const C b2 = cmul_mi(C{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, inv);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+}
+
 template <typename C>
 __device__ inline void dft4(C *v, bool inv) {
     C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
@@ -112,6 +132,7 @@ __device__ inline void dftR(C *v, bool inv) {
     if constexpr (R == 2) dft2(v, inv);
     else if constexpr (R == 3) dft3(v, inv);
     else if constexpr (R == 4) dft4(v, inv);
+    else if constexpr (R == 5) dft5(v, inv);
     else dft8(v, inv);
 }
 
@@ -173,6 +194,9 @@ __device__ inline void fft_lds(C *buf, const C *tw, const FftPlan &pl, bool inv)
         }
         if constexpr (E % 3 == 0) {
             if (R == 3) { fft_stage<C, E, 3>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); continue; }
+        }
+        if constexpr (E % 5 == 0) {
+            if (R == 5) { fft_stage<C, E, 5>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); continue; }
         }
     }
 }

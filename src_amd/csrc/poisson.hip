@@ -667,18 +667,26 @@ static std::vector<TileEntry> g_tile_override;
 // E, W a power of two, the tile within the register budget and within the
 // 160 KiB of LDS a gfx950 workgroup can hold: the tile of pitch W + 1, and the
 // FP32 twiddles)
+// the factorisation L = 2^a * odd, odd in {1, 3, 5}; false otherwise
+static bool split_length(int L, int &a, int &odd) {
+    a = 0;
+    int m = L;
+    while (m > 0 && m % 2 == 0) { m /= 2; ++a; }
+    odd = m;
+    return a >= 2 && (m == 1 || m == 3 || m == 5);
+}
+
 static int tile_candidates(int L, bool fp64, int *out, int cap) {
-    int a = 0, b = 0, m = L;
-    while (m % 2 == 0) { m /= 2; ++a; }
-    while (m % 3 == 0) { m /= 3; ++b; }
-    if (m != 1 || b > 1 || a < 2) return -1;
-    static const int c2[] = {4, 8, 16, 32}, c3[] = {6, 12, 24, 48};
-    const int *cand = b ? c3 : c2;
-    const int emax = fp64 ? (b ? 24 : 16) : (b ? 48 : 32);
+    int a = 0, odd = 1;
+    if (!split_length(L, a, odd)) return -1;
+    // E a multiple of every radix: the odd one and 2 (E even)
+    static const int c2[] = {4, 8, 16, 32}, c3[] = {6, 12, 24, 48}, c5[] = {10, 20, 40, 0};
+    const int *cand = odd == 3 ? c3 : odd == 5 ? c5 : c2;
+    const int emax = fp64 ? (odd == 1 ? 16 : odd == 3 ? 24 : 20) : (odd == 1 ? 32 : odd == 3 ? 48 : 40);
     int k = 0;
     for (int c = 0; c < 4; ++c) {
         const int e = cand[c];
-        if (e > emax || (e * kFftThreads) % L != 0) continue;
+        if (e == 0 || e > emax || (e * kFftThreads) % L != 0) continue;
         const int W = e * kFftThreads / L;
         if (W < 1 || (W & (W - 1)) != 0) continue;
         const size_t cs = fp64 ? 16 : 8;
@@ -701,10 +709,8 @@ static int tile_lookup(int L, bool fp64) {
 // else the smallest E whose tile has W >= 128 bytes / complex size lines,
 // within the register budget.
 static bool make_plan(int L, bool fp64, FftPlan &pl) {
-    int a = 0, b = 0, m = L;
-    while (m % 2 == 0) { m /= 2; ++a; }
-    while (m % 3 == 0) { m /= 3; ++b; }
-    if (m != 1 || b > 1 || a < 2) return false;
+    int a = 0, odd = 1;
+    if (!split_length(L, a, odd)) return false;
     pl.L = L;
     const int wmin = fp64 ? 8 : 16;
     int cand[8];
@@ -730,7 +736,7 @@ static bool make_plan(int L, bool fp64, FftPlan &pl) {
     while (rem >= 3 && use8) { pl.radix[pl.nst++] = 8; rem -= 3; }
     while (rem >= 2 && use4) { pl.radix[pl.nst++] = 4; rem -= 2; }
     while (rem >= 1) { pl.radix[pl.nst++] = 2; rem -= 1; }
-    if (b) pl.radix[pl.nst++] = 3;  // last: every earlier span is a power of two
+    if (odd > 1) pl.radix[pl.nst++] = odd;  // last: every earlier span is a power of two
     if (pl.nst > 16) return false;
     for (int s = 0; s < pl.nst; ++s)
         if (pl.E % pl.radix[s] != 0) return false;
@@ -759,10 +765,13 @@ static size_t lds_bytes(const FftPlan &pl, size_t csize) {
     case 4: { constexpr int EV = 4; __VA_ARGS__; } break;      \
     case 6: { constexpr int EV = 6; __VA_ARGS__; } break;      \
     case 8: { constexpr int EV = 8; __VA_ARGS__; } break;      \
+    case 10: { constexpr int EV = 10; __VA_ARGS__; } break;    \
     case 12: { constexpr int EV = 12; __VA_ARGS__; } break;    \
     case 16: { constexpr int EV = 16; __VA_ARGS__; } break;    \
+    case 20: { constexpr int EV = 20; __VA_ARGS__; } break;    \
     case 24: { constexpr int EV = 24; __VA_ARGS__; } break;    \
     case 32: if constexpr (sizeof(T) == 4) { constexpr int EV = 32; __VA_ARGS__; } break; \
+    case 40: if constexpr (sizeof(T) == 4) { constexpr int EV = 40; __VA_ARGS__; } break; \
     case 48: if constexpr (sizeof(T) == 4) { constexpr int EV = 48; __VA_ARGS__; } break; \
     default: break;                                            \
     }
@@ -780,8 +789,8 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
         if (nn[d] < 4) { err = "n must be >= 4 in every dimension"; return -1; }
         if (!make_plan(L[d], fp64, plan[d])) {
             err = "unsupported grid size n=" + std::to_string(nn[d]) +
-                  ((pz && d == 2) ? " for periodic z (need n = 2^a or 3*2^a, a>=2)"
-                                  : " (need n = 2^a or 3*2^a, a>=1)");
+                  ((pz && d == 2) ? " for periodic z (need n = 2^a, 3*2^a or 5*2^a, a>=2)"
+                                  : " (need n = 2^a, 3*2^a or 5*2^a, a>=1)");
             return -1;
         }
     }

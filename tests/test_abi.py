@@ -130,8 +130,11 @@ def test_fft_tile_candidates_and_overrides():
                 W = E * 512 // L
                 assert E * 512 % L == 0 and W & (W - 1) == 0
                 assert L * (W + 1) * cs + (0 if p == sc.FP64 else L * cs) <= 160 * 1024
-    with pytest.raises(ValueError):
-        sc.tune_fft_candidates(40, sc.MIXED)  # 5 * 2^3: no radix-5 stage
+    assert sc.tune_fft_candidates(80, sc.MIXED) == [10, 20]
+    assert sc.tune_fft_candidates(80, sc.FP64) == [10]
+    for L in (56, 200, 36, 6):  # 7 * 2^3, 25 * 2^3, 9 * 2^2, 3 * 2^1: no tiles
+        with pytest.raises(ValueError):
+            sc.tune_fft_candidates(L, sc.MIXED)
     base = sc.tune_fft_get(128, sc.MIXED)
     assert base in sc.tune_fft_candidates(128, sc.MIXED)
     try:

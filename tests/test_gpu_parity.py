@@ -99,7 +99,8 @@ def test_sort_by_cell_large_keys_ragged_and_ties(n, N):
 
 
 @pytest.mark.parametrize("greens", [0, 1])
-@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12), (8, 48, 12), (64, 32, 96)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12), (8, 48, 12), (64, 32, 96), (20, 40, 10),
+                               (80, 24, 40)])
 def test_compute_potential_fp64(n, greens):
     rng = np.random.default_rng(4)
     rho = rng.normal(size=n[0] * n[1] * n[2])
@@ -110,7 +111,7 @@ def test_compute_potential_fp64(n, greens):
     assert rel_l2(phi, ref) < 1e-12
 
 
-@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (8, 48, 12), (128, 64, 32)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (8, 48, 12), (128, 64, 32), (40, 20, 80)])
 def test_compute_potential_mixed(n):
     """FP32 convolution (FP64 Green's function): tolerance 1e-5 relative L2."""
     rng = np.random.default_rng(9)
@@ -533,7 +534,7 @@ def test_tracker_binned_without_image_and_mixed():
 
 @pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-12), (sc.MIXED, 1e-5)])
 @pytest.mark.parametrize("greens", [0, 1])
-@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 24, 8), (64, 32, 96)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 24, 8), (64, 32, 96), (10, 16, 20)])
 def test_compute_potential_periodic_z(n, greens, precision, tol):
     """DC-beam solver (BCFFTZ=PERIODIC): x and y doubled, z periodic with the
     one-sided Green's function."""
