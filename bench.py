@@ -380,6 +380,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--sort-interval", type=int, default=20)
+    ap.add_argument("--sort-drop", type=int, default=None,
+                    help="re-sort by cell blocks: Morton key bits dropped (default: the tuning table's)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -398,6 +400,8 @@ def main():
               "bunch on one MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import src_amd as sc
+    if args.sort_drop is not None:
+        sc.tune_sort_set(args.sort_drop)
     if args.c5_only or args.c4_only:
         log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
         if args.c5_only:

@@ -13,6 +13,9 @@ src_amd/build.py compiles into the library (rebuild after running it):
 * Deposit: the particle count up to which a bunch deposits through the larger
   LDS box (three blocks per CU) rather than the smaller one (four per CU),
   from the deposit's time at 1e6 ... 6.4e7 particles on a 256^3 grid.
+* Re-sort granularity: the Morton key bits the tracker's re-sort drops (0, 3,
+  6: order by cell, by 2^3 or 4^3 blocks of cells), from the step time of a
+  2e7-particle bunch on a 256^3 grid re-sorted every 20 steps (40 steps).
 
 Usage: python -m src_amd.autotune [--out src_amd/tuning.json] [--quick]
 """
@@ -126,6 +129,28 @@ def tune_deposit(log):
             "ms": {str(N): {"large": round(a, 4), "small": round(b, 4)} for N, (a, b) in res.items()}}
 
 
+def tune_sort(log, reps):
+    res = {}
+    for drop in (0, 3, 6):
+        sc.tune_sort_set(drop)
+        c = _ctx(256, sc.MIXED, 20_000_000, 9)
+        try:
+            c.track(3, want_stats=False)
+            best = math.inf
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                c.track(40, want_stats=False)
+                best = min(best, (time.perf_counter() - t0) / 40)
+        finally:
+            c.close()
+        res[drop] = 1e3 * best
+        log("[autotune] re-sort dropping %d key bits: %.4f ms/step" % (drop, res[drop]))
+    best = min(res, key=res.get)
+    sc.tune_sort_set(best)
+    return {"drop_bits": best, "ms_per_step": {str(k): round(v, 4) for k, v in res.items()},
+            "workload": "256^3 grid, 2e7 particles, re-sort every 20 steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
@@ -137,7 +162,7 @@ def main():
     for n, prec, _, _ in SHAPES:  # start from the default rule, not an earlier table
         sc.tune_fft_set(2 * n, prec, 0)
     table = {"generated_by": "python -m src_amd.autotune", "fft_tiles": tune_fft(log, args.steps, args.reps),
-             "deposit": tune_deposit(log)}
+             "deposit": tune_deposit(log), "sort": tune_sort(log, args.reps)}
     table["seconds"] = round(time.time() - t0, 1)
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)

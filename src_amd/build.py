@@ -34,17 +34,19 @@ DEFAULT_LARGE_BOX_MAX = 8 << 20  # particles; the deposit's larger LDS box up to
 def write_tuning_header():
     """_build/tuning.hpp from the autotuner's table: FFT tile entries
     {L, fp64, E} and the deposit's large-box threshold (defaults without a table)."""
-    tiles, large = [], DEFAULT_LARGE_BOX_MAX
+    tiles, large, drop = [], DEFAULT_LARGE_BOX_MAX, 0
     if os.path.exists(TUNING):
         with open(TUNING) as f:
             t = json.load(f)
         for e in t.get("fft_tiles", []):
             tiles.append("{%d, %d, %d}, " % (int(e["L"]), 1 if e["precision"] == "fp64" else 0, int(e["E"])))
         large = int(t.get("deposit", {}).get("large_box_max_particles", large))
+        drop = int(t.get("sort", {}).get("drop_bits", drop))
     text = ("// generated by src_amd/build.py from src_amd/tuning.json; do not edit\n"
             "#pragma once\n"
             "#define OPALSC_TUNED_FFT_TILES %s\n"
-            "#define OPALSC_TUNED_LARGE_BOX_MAX ((int64_t)%dLL)\n" % ("".join(tiles), large))
+            "#define OPALSC_TUNED_LARGE_BOX_MAX ((int64_t)%dLL)\n"
+            "#define OPALSC_TUNED_SORT_DROP_BITS %d\n" % ("".join(tiles), large, drop))
     path = os.path.join(BUILD, "tuning.hpp")
     old = open(path).read() if os.path.exists(path) else None
     if old != text:

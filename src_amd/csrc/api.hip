@@ -8,6 +8,7 @@
 #include "frame.hpp"
 #include "igfmath.hpp"
 #include "kernels.hpp"
+#include "tuning.hpp"  // generated from src_amd/tuning.json by src_amd/build.py
 
 #include <algorithm>
 #include <cmath>
@@ -185,10 +186,17 @@ static int alloc_bunch(opalsc_ctx *c, int64_t cap) {
     const size_t nchunks = (size_t)(std::max<int64_t>(cap, 1) + kDepositChunk - 1) / kDepositChunk;
     if (hipMalloc((void **)&b.chunkbox, nchunks * kBoxStats * sizeof(double)) != hipSuccess)
         FAIL(c, OPALSC_ENOMEM, "hipMalloc of chunk boxes failed");
+    // a tracker that re-sorts: the gather pass writes the re-sorted arrays here
+    if (c->cfg.sort_interval > 0)
+        for (double *&a : b.alt)
+            if (hipMalloc((void **)&a, (size_t)std::max<int64_t>(cap, 1) * 8) != hipSuccess)
+                FAIL(c, OPALSC_ENOMEM, "hipMalloc of re-sort arrays failed");
     return 0;
 }
 
 static void free_bunch(Bunch &b) {
+    for (double *a : b.alt)
+        if (a) (void)hipFree(a);
     void *ps[] = {b.x, b.y, b.z, b.px, b.py, b.pz, b.q, b.tmp, b.qd, b.id, b.keys, b.vals, b.keys2, b.vals2, b.sort_tmp,
                   b.chunkbox};
     for (void *p : ps)
@@ -1061,25 +1069,40 @@ static void permute_bunch(opalsc_ctx *c, hipStream_t st = nullptr) {
     std::swap(b.id, b.keys);
 }
 
-// Cell order (3D Morton) of the resident bunch; with energy bins, cell order
-// within each bin, so the bins keep their ranges.  sc: the step's mesh (default:
-// the context's scalars); st: the stream (default: the context's).
-static void sort_bunch(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st = nullptr) {
+// The permutation (b.vals2) that orders the resident bunch by cell (3D
+// Morton); with energy bins, by cell within each bin, so the bins keep their
+// ranges.  sc: the step's mesh (default: the context's scalars); st: the
+// stream (default: the context's).
+// The tracker's re-sort orders by the Morton key without its g_sort_drop
+// lowest bits (blocks of 2^(drop/3) cells per axis; particles keep their
+// relative order within a block), set by the tuning table or opalsc_tune_sort_set.
+static int g_sort_drop = OPALSC_TUNED_SORT_DROP_BITS;
+
+static void sort_keys(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st = nullptr) {
     if (!sc) sc = c->d_sc;
     if (!st) st = c->stream;
     Bunch &b = c->bunch;
     const int64_t N = b.N;
     int bits = key_bits(c->cfg.n);
+    const int coarse = std::min(g_sort_drop, bits - 1);
     if (c->bins.nbins > 0) {
         const int bb = bin_bits(c->bins.nbins);
-        const int drop = bits + bb > 32 ? bits + bb - 32 : 0;
+        const int drop = std::max(coarse, bits + bb > 32 ? bits + bb - 32 : 0);
         launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals, c->binsorted.as<int32_t>(), bits - drop,
                          drop);
         bits = bits - drop + bb;
+    } else if (coarse > 0) {
+        launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals, nullptr, 0, coarse);
+        bits -= coarse;
     } else {
         launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals);
     }
     sort_pairs(st, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, bits);
+}
+
+// Cell order of the resident bunch, every array permuted.
+static void sort_bunch(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st = nullptr) {
+    sort_keys(c, sc, st);
     permute_bunch(c, st);
 }
 
@@ -1176,6 +1199,15 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     // before the bounds pass and the sort move particles
     const int64_t gchunk = c->grid.fp64 ? kOsChunk64 : kOsChunk;
     const bool chunked = binned && c->track_sc && c->binchunks.p && bins_chunked(c->bins, gchunk);
+    const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
+    // the gather pass in blocks that run once unless a DC beam or field maps need
+    // the persistent loop (2.30 -> 2.20 ms at 1e8 particles, mixed)
+    const bool oneshot = !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
+    // a re-sort is applied by the one-shot gather pass (it reads the old
+    // arrays through the permutation and writes the new ones), so the deposit
+    // reads the old order with its chunk boxes and no array is permuted on
+    // its own
+    const bool deferred = sort_now && oneshot && b.alt[0];
     launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
     if (chunked && !boxes) launch_bin_gamma_chunks(c->stream, c->grid.fp64, b, c->bins, c->binchunks.as<double>());
     c->prof.mark(PROF_SCALAR);
@@ -1193,9 +1225,11 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                           c->d_sc);
         c->prof.mark(PROF_SCALAR);
         if (sort_now) {
-            sort_bunch(c);  // the bounds pass's chunk boxes no longer match the particle order
+            if (deferred) sort_keys(c);
+            else sort_bunch(c);  // the bounds pass's chunk boxes no longer match the particle order
             c->prof.mark(PROF_SORT);
         }
+        const double *boxes_in = sort_now && !deferred ? nullptr : b.chunkbox;
         if (binned) {
             // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its
             // image; the non-empty bins are solved as one batch (one launch per pass)
@@ -1220,7 +1254,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                 const int64_t lo = c->bins.start[k], hi = c->bins.start[k + 1];
                 if (hi == lo) continue;  // an empty bin has no charge
                 launch_deposit_range(c->stream, 2, exact, lo, hi, b.N, b.x, b.y, b.z, nullptr, nullptr,
-                                     c->grid.bin_rho(z), scz + z, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
+                                     c->grid.bin_rho(z), scz + z, &c->d_counters[1], boxes_in);
                 ++z;
             }
             c->prof.mark(PROF_DEPOSIT);
@@ -1246,28 +1280,40 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             c->grid.zero_rho(c->stream);
             c->prof.mark(PROF_ZERO);
             launch_deposit(c->stream, c->q_uniform ? 3 : 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr,
-                           (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], sort_now ? nullptr : b.chunkbox);
+                           (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], boxes_in);
             c->prof.mark(PROF_DEPOSIT);
             if (!overlap) c->grid.green(c->d_sc);
             c->grid.solve(c->d_sc, 0, 1, false, overlap ? c->ev_green : nullptr);
             c->grid.grad(c->d_sc);
         }
     }
-    const int ext = c->fmaps.empty() ? (c->track_ext ? 1 : 0) : 2;
     int nparts = gamma_blocks(c->grid.fp64, b.N);
-    // in blocks that run once unless a DC beam or field maps need the persistent
-    // loop (2.30 -> 2.20 ms at 1e8 particles, mixed)
-    const bool oneshot = !c->grid.pz && ext != 2 && b.N > 0 && c->os_dense.p;
     // the next step of this call takes its bounds from the statistics this pass leaves
     const bool boxes_out = oneshot && next && c->track_sc && c->osbox.p;
     const bool bsum = binned && c->track_sc;  // the bins' summed fields
-    if (oneshot)
-        nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, b, bsum ? c->grid.Esum : c->grid.Eg, c->d_sc,
+    if (oneshot) {
+        PermSrc ps{b.vals2, b.x, b.y, b.z, b.px, b.py, b.pz, b.id, b.keys, c->q_uniform ? nullptr : b.q, b.tmp};
+        Bunch out = b;  // the arrays the pass writes
+        if (deferred)
+            for (int k = 0; k < 6; ++k) {
+                double **dst[] = {&out.x, &out.y, &out.z, &out.px, &out.py, &out.pz};
+                *dst[k] = b.alt[k];
+            }
+        nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, out, bsum ? c->grid.Esum : c->grid.Eg, c->d_sc,
                                        c->d_ts, c->os_dense.as<double>(), c->partials, ext,
                                        boxes_out ? c->osbox.as<double>() : nullptr,
                                        bsum ? c->grid.Bsum : nullptr, &c->bins,
-                                       boxes_out && chunked ? c->binchunks.as<double>() : nullptr);
-    else if (bsum)
+                                       boxes_out && chunked ? c->binchunks.as<double>() : nullptr,
+                                       deferred ? &ps : nullptr);
+        if (deferred) {
+            // the re-sorted arrays are the bunch's from here on
+            ++c->gen;
+            double **cur[] = {&b.x, &b.y, &b.z, &b.px, &b.py, &b.pz};
+            for (int k = 0; k < 6; ++k) std::swap(*cur[k], b.alt[k]);
+            std::swap(b.id, b.keys);
+            if (!c->q_uniform) std::swap(b.q, b.tmp);
+        }
+    } else if (bsum)
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Esum, c->d_sc, c->d_ts, c->partials,
                            c->grid.Bsum, ext, false);
     else
@@ -1498,6 +1544,14 @@ extern "C" int opalsc_tune_deposit_set(int64_t max_particles) {
 }
 
 extern "C" int64_t opalsc_tune_deposit_get(void) { return deposit_large_box_max(); }
+
+extern "C" int opalsc_tune_sort_set(int drop_bits) {
+    if (drop_bits < 0 || drop_bits > 24) return OPALSC_EINVAL;
+    g_sort_drop = drop_bits;
+    return OPALSC_OK;
+}
+
+extern "C" int opalsc_tune_sort_get(void) { return g_sort_drop; }
 
 extern "C" int opalsc_batch_set_fused(opalsc_batch *b, int enable) {
     if (!b) return OPALSC_EINVAL;

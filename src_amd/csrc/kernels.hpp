@@ -174,6 +174,8 @@ struct Bunch {
     double *qd = nullptr;        // Q*dt (self-field entry point)
     double *chunkbox = nullptr;  // per-chunk beam-frame position statistics (kBoxStats doubles)
     uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
+    // tracker re-sorts applied by the gather pass: the arrays it writes x, y, z, px, py, pz into
+    double *alt[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
 };
@@ -242,6 +244,18 @@ int batch_reduce_blocks(bool exact, int64_t N);
 bool deposit_large_box(int64_t n);  // the deposit of n particles uses the larger LDS box
 void set_deposit_large_box_max(int64_t n);
 int64_t deposit_large_box_max();
+
+// A re-sort applied by the one-shot gather pass itself: particle i of the
+// sorted order is read from the old arrays at perm[i] and written at i of the
+// pass's (new) arrays; id (and Q when it is not one number) move along.
+struct PermSrc {
+    const uint32_t *perm;
+    const double *x, *y, *z, *px, *py, *pz;
+    const uint32_t *id;
+    uint32_t *id_out;
+    const double *q;  // null: the bunch keeps Q as one number
+    double *q_out;
+};
 
 // Launchers (particles.hip)
 int red_blocks(int64_t N);
@@ -332,9 +346,11 @@ __host__ __device__ inline int os_blocks64(int64_t N) {
 // with it and next, binp (not null) receives the chunk sums of bingamma.hpp
 // for the bin ranges *bins.
 struct BinRanges;
+// perm (may be null): the pass also applies a re-sort (PermSrc); b holds the new arrays.
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                           const TrackState *ts, double *dense, double *partials, int ext, double *osbox,
-                          const void *Bg = nullptr, const BinRanges *bins = nullptr, double *binp = nullptr);
+                          const void *Bg = nullptr, const BinRanges *bins = nullptr, double *binp = nullptr,
+                          const PermSrc *perm = nullptr);
 // blocks of statistics launch_track_gamma_os leaves (one per one-shot block)
 int64_t os_box_count(bool fp64, int64_t N);
 // The bounds pass from the previous gather pass's statistics: the beam-frame

@@ -938,6 +938,7 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
         if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
             key = cell_key(l[0], l[1], l[2], n0, n1);
         if (bin) key = ((uint32_t)bin[p] << bshift) | (key >> drop);
+        else key >>= drop;
         keys[p] = key;
         vals[p] = (uint32_t)p;
     }
@@ -1745,13 +1746,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // keeps more particle loads in flight than a persistent grid-stride loop
 // (2.30 -> 2.20 ms at 1e8 particles; 2 or 8 particles per thread: 2.27, 2.31 ms).
 
-template <bool NEXT, int EXT, bool BINNED, bool BATCH = false>
+template <bool NEXT, int EXT, bool BINNED, bool BATCH = false, bool PERM = false>
 __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double *x, double *y, double *z, double *px,
                                                               double *py, double *pz, const float4 *Eg,
                                                               const float4 *Bg, const Scalars *sc,
                                                               const TrackState *ts, double *dense, double *osbox,
                                                               BinRanges br, double *binp,
-                                                              BatchView bv = BatchView{}) {
+                                                              BatchView bv = BatchView{}, PermSrc ps = PermSrc{}) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
     __shared__ double lgam[2 * (kThreads / 64)];
@@ -1800,17 +1801,37 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
         const double half_cdt = 0.5 * (kC * dT);
         const int64_t base = chunk * kOsChunk + threadIdx.x;
         double rr[kOsU][6];
+        uint32_t pj[kOsU];
 #pragma unroll
         for (int u = 0; u < kOsU; ++u) {
             const int64_t i = base + u * kThreads;
             const int64_t j = i < N ? i : N - 1;
-            rr[u][0] = x[j];
-            rr[u][1] = y[j];
-            rr[u][2] = z[j];
-            rr[u][3] = px[j];
-            rr[u][4] = py[j];
-            rr[u][5] = pz[j];
+            if constexpr (PERM) {
+                pj[u] = ps.perm[j];
+                rr[u][0] = ps.x[pj[u]];
+                rr[u][1] = ps.y[pj[u]];
+                rr[u][2] = ps.z[pj[u]];
+                rr[u][3] = ps.px[pj[u]];
+                rr[u][4] = ps.py[pj[u]];
+                rr[u][5] = ps.pz[pj[u]];
+            } else {
+                rr[u][0] = x[j];
+                rr[u][1] = y[j];
+                rr[u][2] = z[j];
+                rr[u][3] = px[j];
+                rr[u][4] = py[j];
+                rr[u][5] = pz[j];
+            }
         }
+        if constexpr (PERM)
+#pragma unroll
+            for (int u = 0; u < kOsU; ++u) {
+                const int64_t i = base + u * kThreads;
+                if (i < N) {
+                    ps.id_out[i] = ps.id[pj[u]];
+                    if (ps.q) ps.q_out[i] = ps.q[pj[u]];
+                }
+            }
         int64_t split = 0;
         if constexpr (BINNED && NEXT)
             if (binp) split = bin_chunk_split(br, chunk * kOsChunk, N);
@@ -1900,12 +1921,12 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
 // bins' summed E and B grids) in blocks that run once, kOsU64 particles per thread, the per-particle sequence of
 // k_track_gamma unchanged; moment sums as in k_track_gamma_mixed_os.
 
-template <bool NEXT, bool BINNED, bool BATCH = false>
+template <bool NEXT, bool BINNED, bool BATCH = false, bool PERM = false>
 __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, double *y, double *z, double *px,
                                                         double *py, double *pz, const double4 *Eg, const double4 *Bg,
                                                         const Scalars *sc, const TrackState *ts, double *dense,
                                                         double *osbox, BinRanges br, double *binp,
-                                                        BatchView bv = BatchView{}) {
+                                                        BatchView bv = BatchView{}, PermSrc ps = PermSrc{}) {
     __shared__ double wsum[kThreads / 64][kMom];
     __shared__ double lbox[4 * 6];
     __shared__ double lgam[2 * (kThreads / 64)];
@@ -1942,17 +1963,37 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
         const double betaC = sc->betaC;
         const int64_t base = chunk * kOsChunk64 + threadIdx.x;
         double rr[kOsU64][6];
+        uint32_t pj[kOsU64];
 #pragma unroll
         for (int u = 0; u < kOsU64; ++u) {
             const int64_t i = base + u * kThreads;
             const int64_t j = i < N ? i : N - 1;
-            rr[u][0] = x[j];
-            rr[u][1] = y[j];
-            rr[u][2] = z[j];
-            rr[u][3] = px[j];
-            rr[u][4] = py[j];
-            rr[u][5] = pz[j];
+            if constexpr (PERM) {
+                pj[u] = ps.perm[j];
+                rr[u][0] = ps.x[pj[u]];
+                rr[u][1] = ps.y[pj[u]];
+                rr[u][2] = ps.z[pj[u]];
+                rr[u][3] = ps.px[pj[u]];
+                rr[u][4] = ps.py[pj[u]];
+                rr[u][5] = ps.pz[pj[u]];
+            } else {
+                rr[u][0] = x[j];
+                rr[u][1] = y[j];
+                rr[u][2] = z[j];
+                rr[u][3] = px[j];
+                rr[u][4] = py[j];
+                rr[u][5] = pz[j];
+            }
         }
+        if constexpr (PERM)
+#pragma unroll
+            for (int u = 0; u < kOsU64; ++u) {
+                const int64_t i = base + u * kThreads;
+                if (i < N) {
+                    ps.id_out[i] = ps.id[pj[u]];
+                    if (ps.q) ps.q_out[i] = ps.q[pj[u]];
+                }
+            }
         int64_t split = 0;
         if constexpr (BINNED && NEXT)
             if (binp) split = bin_chunk_split(br, chunk * kOsChunk64, N);
@@ -2127,20 +2168,31 @@ struct OsArgs {
     double *dense, *osbox;
     const BinRanges *br;
     double *binp;
+    const PermSrc *perm;
 };
+template <bool NEXT, bool BINNED, bool PERM>
+void launch_os64_p(const OsArgs &a, int blocks) {
+    k_track_gamma_os<NEXT, BINNED, false, PERM><<<blocks, kThreads, 0, a.s>>>(
+        a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const double4 *)a.Eg, (const double4 *)a.Bg, a.sc,
+        a.ts, a.dense, NEXT ? a.osbox : nullptr, *a.br, BINNED && NEXT ? a.binp : nullptr, BatchView{},
+        PERM ? *a.perm : PermSrc{});
+}
 template <bool NEXT, bool BINNED>
 void launch_os64(const OsArgs &a, int blocks) {
-    k_track_gamma_os<NEXT, BINNED><<<blocks, kThreads, 0, a.s>>>(a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py,
-                                                                  a.b->pz, (const double4 *)a.Eg,
-                                                                  (const double4 *)a.Bg, a.sc, a.ts, a.dense,
-                                                                  NEXT ? a.osbox : nullptr, *a.br,
-                                                                  BINNED && NEXT ? a.binp : nullptr);
+    if (a.perm) launch_os64_p<NEXT, BINNED, true>(a, blocks);
+    else launch_os64_p<NEXT, BINNED, false>(a, blocks);
+}
+template <bool NEXT, int EXT, bool BINNED, bool PERM>
+void launch_os_mixed_p(const OsArgs &a, int blocks) {
+    k_track_gamma_mixed_os<NEXT, EXT, BINNED, false, PERM><<<blocks, kThreads, 0, a.s>>>(
+        a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const float4 *)a.Eg, (const float4 *)a.Bg, a.sc,
+        a.ts, a.dense, NEXT ? a.osbox : nullptr, *a.br, BINNED && NEXT ? a.binp : nullptr, BatchView{},
+        PERM ? *a.perm : PermSrc{});
 }
 template <bool NEXT, int EXT, bool BINNED>
 void launch_os_mixed(const OsArgs &a, int blocks) {
-    k_track_gamma_mixed_os<NEXT, EXT, BINNED><<<blocks, kThreads, 0, a.s>>>(
-        a.b->N, a.b->x, a.b->y, a.b->z, a.b->px, a.b->py, a.b->pz, (const float4 *)a.Eg, (const float4 *)a.Bg, a.sc,
-        a.ts, a.dense, NEXT ? a.osbox : nullptr, *a.br, BINNED && NEXT ? a.binp : nullptr);
+    if (a.perm) launch_os_mixed_p<NEXT, EXT, BINNED, true>(a, blocks);
+    else launch_os_mixed_p<NEXT, EXT, BINNED, false>(a, blocks);
 }
 template <bool NEXT, int EXT>
 void launch_os_mixed_b(const OsArgs &a, int blocks) {
@@ -2151,9 +2203,9 @@ void launch_os_mixed_b(const OsArgs &a, int blocks) {
 
 int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,
                           const TrackState *ts, double *dense, double *partials, int ext, double *osbox,
-                          const void *Bg, const BinRanges *bins, double *binp) {
+                          const void *Bg, const BinRanges *bins, double *binp, const PermSrc *perm) {
     static const BinRanges none;
-    const OsArgs a{s, &b, Eg, Bg, sc, ts, dense, osbox, bins ? bins : &none, bins ? binp : nullptr};
+    const OsArgs a{s, &b, Eg, Bg, sc, ts, dense, osbox, bins ? bins : &none, bins ? binp : nullptr, perm};
     if (fp64) {
         const int blocks64 = os_blocks64(b.N);
         if (next) {

@@ -337,7 +337,13 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     using S = typename std::conditional<SRC == 0, typename std::conditional<kTwoWords, longlong2, long long>::type,
                                         double>::type;
     double q2e_lo = 0.0;
-    if constexpr (kTwoWords) q2e_lo = sc->q2e_lo;
+    int lsh = 0;
+    long long lmask = 0;
+    if constexpr (kTwoWords) {
+        q2e_lo = sc->q2e_lo;
+        lsh = sc->lo_shift;
+        lmask = ((long long)1 << lsh) - 1;
+    }
     const S *s = (const S *)src + blockIdx.z * sB;
     S r[Q];
 #pragma unroll
@@ -353,7 +359,10 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
             const int ln = div_small(e, n0, rn), x = e - ln * n0;
             double d;
             if constexpr (kTwoWords) {
-                d = (double)r[q].x * q2e + (double)r[q].y * q2e_lo;
+                // the canonical words (lo in [0, 2^lo_shift)) of the cell's exact
+                // sum, so the value does not depend on how deposits were grouped
+                const long long hi = r[q].x + (r[q].y >> lsh), lo = r[q].y & lmask;
+                d = (double)hi * q2e + (double)lo * q2e_lo;
                 d = d / dT;
                 d = d * tmp2;
             } else if constexpr (SRC == 0) {
@@ -631,9 +640,17 @@ __global__ __launch_bounds__(256) void k_field_sums(const T *phi, const T *phii,
 // rho entering the solver, for diagnostics
 __global__ void k_rho_to_double(const long long *rho, double *out, size_t n, const Scalars *sc, int two_words) {
     const double q2e = sc->q2e, q2e_lo = sc->q2e_lo, dT = sc->dT, tmp2 = sc->tmp2;
+    const int lsh = sc->lo_shift;
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
          c += (size_t)gridDim.x * blockDim.x) {
-        double d = two_words ? (double)rho[2 * c] * q2e + (double)rho[2 * c + 1] * q2e_lo : (double)rho[c] * q2e;
+        double d;
+        if (two_words) {  // canonical words, as the x pass reads them
+            const long long hi = rho[2 * c] + (rho[2 * c + 1] >> lsh);
+            const long long lo = rho[2 * c + 1] & (((long long)1 << lsh) - 1);
+            d = (double)hi * q2e + (double)lo * q2e_lo;
+        } else {
+            d = (double)rho[c] * q2e;
+        }
         d = d / dT;
         d = d * tmp2;
         out[c] = d;

@@ -48,7 +48,7 @@ EXPORTS = [
     "opalsc_copy_bandwidth", "opalsc_fromfile_count", "opalsc_fromfile_read", "opalsc_stat_row_get",
     "opalsc_stat_write", "opalsc_fieldmap_add", "opalsc_fieldmap_clear", "opalsc_fieldmap_eval",
     "opalsc_tune_fft_candidates", "opalsc_tune_fft_set", "opalsc_tune_fft_get", "opalsc_tune_deposit_set",
-    "opalsc_tune_deposit_get",
+    "opalsc_tune_deposit_get", "opalsc_tune_sort_set", "opalsc_tune_sort_get",
 ]
 
 
@@ -172,6 +172,8 @@ def lib():
         "opalsc_tune_fft_get": ([C.c_int, C.c_int], C.c_int),
         "opalsc_tune_deposit_set": ([C.c_int64], C.c_int),
         "opalsc_tune_deposit_get": ([], C.c_int64),
+        "opalsc_tune_sort_set": ([C.c_int], C.c_int),
+        "opalsc_tune_sort_get": ([], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -229,6 +231,15 @@ def tune_deposit_set(max_particles):
 
 def tune_deposit_get():
     return int(lib().opalsc_tune_deposit_get())
+
+
+def tune_sort_set(drop_bits):
+    """The tracker's re-sort orders by the cell's Morton key without its drop_bits lowest bits."""
+    _check(None, lib().opalsc_tune_sort_set(int(drop_bits)))
+
+
+def tune_sort_get():
+    return int(lib().opalsc_tune_sort_get())
 
 
 def stats_to_dict(s):

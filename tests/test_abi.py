@@ -159,3 +159,15 @@ def test_deposit_tuning_threshold_round_trip():
     finally:
         sc.tune_deposit_set(old)
     assert sc.tune_deposit_get() == old
+
+
+def test_sort_granularity_round_trip():
+    old = sc.tune_sort_get()
+    try:
+        for d in (0, 3, 6):
+            sc.tune_sort_set(d)
+            assert sc.tune_sort_get() == d
+        with pytest.raises(sc.OpalscError):
+            sc.tune_sort_set(-3)
+    finally:
+        sc.tune_sort_set(old)
