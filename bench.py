@@ -47,9 +47,10 @@ CONFIGS = {
 
 # energy-binned configuration: 5e7 particles in 10 bins of populations in the
 # ratio 2:1 from bin to bin, 128^3 grid per bin with its image charge, solenoid
-# plus RF cavity field, mixed precision
+# plus RF cavity field, mixed precision; the bunch centred 6 sigma in front of
+# the cathode plane z = 0 that the image charges mirror it in
 C5 = dict(N=50_000_000, n=128, ekin=1.0, q=10e-9, nbins=10, sigma_p=5e-3, Bz=1.0, Ez0=1e6, rf=50e6,
-          seed=5)
+          seed=5, z0=6e-3)
 
 
 def make_ctx(sc, cfg, precision, sort_interval):
@@ -86,24 +87,34 @@ def geometric_bins_by_pz(pz, p0, sigma, nbins):
     return np.searchsorted(edges, pz, side="right").astype(np.int32)
 
 
-def run_c5(sc, steps, warmup, sort_interval, log):
-    """The energy-binned step (one Poisson solve per bin and one for its image)."""
-    from src_amd.opalsc import lib, _ptr
+def c5_context(sc, precision, sort_interval=20):
+    """The energy-binned configuration's bunch in its 10 bins, ready to track."""
     cfg = C5
     N, n = cfg["N"], cfg["n"]
-    c = sc.SpaceChargeContext((n, n, n), precision=sc.MIXED, max_particles=N, sort_interval=sort_interval)
+    c = sc.SpaceChargeContext((n, n, n), precision=precision, max_particles=N, sort_interval=sort_interval)
     g, bg = gamma_betagamma(cfg["ekin"])
     sp = cfg["sigma_p"]
     c.generate_bunch(N, cfg["seed"], (1e-3, 1e-3, 1e-3), (sp, sp, sp), bg, cfg["q"])
-    P = np.zeros((N, 3))
-    c._c(lib().opalsc_bunch_get(c._h, None, _ptr(P), None))
+    R, P, Q = c.get_bunch()
+    R[:, 2] += cfg["z0"]
+    c.set_bunch(R, P, Q)
+    del R, Q
     bins = geometric_bins_by_pz(P[:, 2], bg, sp, cfg["nbins"])
     del P
     c.set_bins(bins, cfg["nbins"], cfg["q"] / N, image=True)
-    counts, _ = c.get_bins()
     del bins
     c.track_init(1e-11, charge=1.0, mass_eV=MP_EV, Bz=cfg["Bz"], Ez0=cfg["Ez0"], rf_freq=cfg["rf"],
                  ref_p=(0.0, 0.0, bg))
+    return c
+
+
+def run_c5(sc, steps, warmup, sort_interval, log):
+    """The energy-binned step (one Poisson solve per bin and one for its image)."""
+    cfg = C5
+    N, n = cfg["N"], cfg["n"]
+    sp = cfg["sigma_p"]
+    c = c5_context(sc, sc.MIXED, sort_interval)
+    counts, _ = c.get_bins()
     t, st = timed_run(c, steps, warmup)
     ms = 1e3 * t / steps
     c.profile_enable(True)
@@ -115,9 +126,10 @@ def run_c5(sc, steps, warmup, sort_interval, log):
     log("[bench] c5 binned: %.3f ms/step" % ms)
     return {"workload": "5e7 macroparticles in %d energy bins (populations %s), %d^3 grid per bin "
                         "(%d^3 doubled FFT) plus its image charge, %g MeV protons, sigma_p %g, Bz %g T, "
-                        "Ez %g V/m at %g MHz, mixed precision, sort every %d steps" %
+                        "Ez %g V/m at %g MHz, bunch centred %g mm in front of the image plane, mixed precision, "
+                        "sort every %d steps" %
                         (cfg["nbins"], "/".join(str(int(x)) for x in counts), n, 2 * n, cfg["ekin"], sp,
-                         cfg["Bz"], cfg["Ez0"], cfg["rf"] / 1e6, sort_interval),
+                         cfg["Bz"], cfg["Ez0"], cfg["rf"] / 1e6, 1e3 * cfg["z0"], sort_interval),
             "ms_per_step": ms, "particle_steps_per_s": N * steps / t, "steps": steps,
             "poisson_solves_per_step": 2 * int((counts > 0).sum()),
             "roofline_step": step_roofline(step_bytes(N, n, 4, solves=2 * int((counts > 0).sum())), ms),

@@ -213,7 +213,9 @@ int opalsc_compute_self_fields_binned(opalsc_ctx *ctx, int64_t N, const double *
 /* Grids of the most recent compute_self_fields / track step: OPALSC_GRID_RHO
  * (charge density entering the solver, n0*n1*n2), OPALSC_GRID_PHI (potential
  * in V, n0*n1*n2), OPALSC_GRID_EFIELD (AoS [n0*n1*n2][3] after the Lorentz
- * scaling).  Mixed precision grids are widened to double. */
+ * scaling; after a tracker step with energy bins, the sum over the bins and
+ * their images that the gather reads).  Mixed precision grids are widened to
+ * double. */
 int opalsc_get_grid(opalsc_ctx *ctx, int which, double *out);
 
 /* --- pusher --------------------------------------------------------------

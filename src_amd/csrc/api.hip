@@ -819,7 +819,9 @@ extern "C" int opalsc_get_grid(opalsc_ctx *c, int which, double *out) {
         HIPCHK(c, hipMemcpyAsync(out, c->grid.rho_f64, nc * 8, hipMemcpyDeviceToHost, c->stream));
     } else if (which == OPALSC_GRID_EFIELD) {
         if (!c->tmpd.ensure(nc * 24)) FAIL(c, OPALSC_ENOMEM, "staging allocation failed");
-        c->grid.widen(2, c->tmpd.as<double>());
+        // a tracker step with energy bins leaves the bins' summed field
+        const bool sums = c->bins.nbins > 0 && c->track_sc && c->grid.Esum && c->steps_done > 0;
+        c->grid.widen(2, c->tmpd.as<double>(), sums ? c->grid.Esum : nullptr);
         HIPCHK(c, hipMemcpyAsync(out, c->tmpd.p, nc * 24, hipMemcpyDeviceToHost, c->stream));
     } else {
         FAIL(c, OPALSC_EINVAL, "unknown grid");

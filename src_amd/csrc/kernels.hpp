@@ -148,7 +148,7 @@ struct PoissonGrid {
     int ensure_bin_fields(std::string &err);  // bb_E for bin_cap grids
     void *bin_E(int b) const { return (char *)bb_E + (size_t)b * ncell * 4 * (fp64 ? 8 : 4); }
     void grad_bins(const Scalars *scb, int nb);
-    void widen(int which, double *out);
+    void widen(int which, double *out, const void *efield = nullptr);  // efield: another E grid (default Eg)
     void profile_mark(int id) {
         if (prof) prof->mark(id);
     }

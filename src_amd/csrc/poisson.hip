@@ -1312,13 +1312,14 @@ void PoissonGrid::rho_to_double(const Scalars *d_sc, double *out) {
     k_rho_to_double<<<1024, 256, 0, stream>>>((const long long *)rho_i64, out, ncell, d_sc, fp64 ? 1 : 0);
 }
 
-void PoissonGrid::widen(int which, double *out) {
+void PoissonGrid::widen(int which, double *out, const void *efield) {
     if (which == 1) {
         if (fp64) (void)hipMemcpyAsync(out, phi, ncell * 8, hipMemcpyDeviceToDevice, stream);
         else k_widen<float><<<1024, 256, 0, stream>>>((const float *)phi, out, ncell, 1, 1);
     } else {
-        if (fp64) k_widen<double><<<1024, 256, 0, stream>>>((const double *)Eg, out, ncell, 3, 4);
-        else k_widen<float><<<1024, 256, 0, stream>>>((const float *)Eg, out, ncell, 3, 4);
+        const void *E = efield ? efield : Eg;
+        if (fp64) k_widen<double><<<1024, 256, 0, stream>>>((const double *)E, out, ncell, 3, 4);
+        else k_widen<float><<<1024, 256, 0, stream>>>((const float *)E, out, ncell, 3, 4);
     }
 }
 

@@ -1,11 +1,15 @@
 """The benchmark configurations at their own sizes, on the GPU.
 
 C1: 1e5 particles, 32^3, 100 steps, FP64 against the oracle step by step.
+C2: 1e7 particles, 128^3, Bz 0.5 T, FP64: tracker steps against the oracle's
+    (on all host cores) at the full size.
 C3: 1e8 particles, 256^3 (10 % halo, 250 MeV): the FP64 charge grid against
     the oracle's scatter of the same positions, and the mixed-precision step
     against the FP64 step (E-field and emittance gates).
 C4: the 64-bunch batch against per-bunch steps, and three of its bunches in
     FP64 against the oracle.
+C5: 5e7 particles in 10 energy bins with images, 128^3, solenoid and RF: the
+    mixed-precision step against the FP64 step (summed field, emittance).
 """
 import sys
 import time
@@ -55,6 +59,61 @@ def test_c1_full_run_matches_oracle_every_step():
             assert abs(a["eps_norm"][d] / b["eps_norm"][d] - 1) < 1e-10, ("eps_norm", d)
         assert abs(a["eps_norm"][2] / b["eps_norm"][2] - 1) < 1e-8
     log("C1 worst rrms relative difference over 100 steps: %.2e" % worst)
+
+
+def test_c2_fp64_tracker_full_size_matches_oracle():
+    """C2 (1e7 particles, 1 MeV, 10 nC, 128^3, Bz 0.5 T, FP64) at its full
+    size: two tracker steps against the oracle's (OpenMP on the host cores):
+    positions within 1e-12 relative L2, momentum changes within 1e-9,
+    statistics within 1e-10."""
+    cfg = bench.CONFIGS["c2"]
+    c = bench.make_ctx(sc, cfg, sc.FP64, 20)
+    try:
+        R, P, Q = c.get_bunch()
+        sg = c.track(2)
+        Rg, Pg, _ = c.get_bunch()
+    finally:
+        c.close()
+    _, bg = bench.gamma_betagamma(cfg["ekin"])
+    O.lib().orc_set_threads(bench.host_threads())
+    try:
+        tr = O.make_tracker((128, 128, 128), 1e-11, ref_p=(0.0, 0.0, bg), Bz=cfg["Bz"], mass=bench.MP_EV)
+        t0 = time.time()
+        Ro, Po, _, _, so = O.track(tr, R, P, Q, np.full(len(R), 1e-11), 2)
+        log("oracle C2 two steps: %.1f s" % (time.time() - t0))
+    finally:
+        O.lib().orc_set_threads(1)
+    e_r, e_p = rel_l2(Rg, Ro), rel_l2(Pg - P, Po - P)
+    log("C2 FP64 vs oracle: R %.2e, dP %.2e" % (e_r, e_p))
+    assert e_r < 1e-12 and e_p < 1e-9
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm"):
+            assert np.allclose(a[k], b[k], rtol=1e-10, atol=0), (k, a[k], b[k])
+
+
+def test_c5_mixed_step_against_fp64_step():
+    """C5 at its full size (5e7 particles in 10 energy bins of 2:1 populations,
+    each with its image charge, 128^3, Bz 1 T, Ez 1 MV/m at 50 MHz): the mixed
+    step's summed field (bins and images, what the gather reads) within 1e-5
+    relative L2 of the FP64 step's from the same bunch, and the normalised
+    emittance within 1e-6 over 21 steps."""
+    cs = [bench.c5_context(sc, prec) for prec in (sc.FP64, sc.MIXED)]
+    try:
+        s = [c.track(1) for c in cs]
+        E64, E32 = (c.get_grid(sc.GRID_EFIELD) for c in cs)
+        e_rel = rel_l2(E32, E64)
+        del E64, E32
+        for k, c in enumerate(cs):
+            s[k] += c.track(20)
+    finally:
+        for c in cs:
+            c.close()
+    s64, s32 = s
+    emit = max(abs(a["eps_norm"][d] / b["eps_norm"][d] - 1) for a, b in zip(s32, s64) for d in range(3))
+    log("C5 mixed vs FP64: summed E rel L2 %.3e, emittance %.3e over %d steps" % (e_rel, emit, len(s64)))
+    assert len(s64) == 21
+    assert e_rel <= 1e-5
+    assert emit <= 1e-6
 
 
 @pytest.fixture(scope="module")
