@@ -58,7 +58,7 @@ struct opalsc_ctx {
     std::string err;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // the tracker's Green's function, concurrent with the deposit
-    hipEvent_t ev_mesh = nullptr, ev_green = nullptr, ev_gimg = nullptr;
+    hipEvent_t ev_mesh = nullptr, ev_green = nullptr, ev_gimg = nullptr, ev_sort = nullptr;
     PoissonGrid grid;
     Bunch bunch;
     Scalars *d_sc = nullptr;
@@ -288,6 +288,7 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->ev_mesh) (void)hipEventDestroy(c->ev_mesh);
     if (c->ev_green) (void)hipEventDestroy(c->ev_green);
+    if (c->ev_sort) (void)hipEventDestroy(c->ev_sort);
     if (c->ev_gimg) (void)hipEventDestroy(c->ev_gimg);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1210,6 +1211,13 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     // reads the old order with its chunk boxes and no array is permuted on
     // its own
     const bool deferred = sort_now && oneshot && b.alt[0];
+    // the second stream takes the Green's function beside the deposit and the
+    // x/y passes on grids of 96^3 cells and more (on smaller ones the extra
+    // synchronisation costs more than it hides), and then a deferred re-sort
+    // (keys and radix passes), which only the gather pass waits for
+    const bool side = overlap && !c->prof.enabled && c->stream2 && c->ev_sort && c->grid.ncell >= (size_t)96 * 96 * 96;
+    const bool sort_side = deferred && side;
+    bool sort_launched = false;
     launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
     if (chunked && !boxes) launch_bin_gamma_chunks(c->stream, c->grid.fp64, b, c->bins, c->binchunks.as<double>());
     c->prof.mark(PROF_SCALAR);
@@ -1226,7 +1234,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
         launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, !binned,
                           c->d_sc);
         c->prof.mark(PROF_SCALAR);
-        if (sort_now) {
+        if (sort_now && !sort_side) {
             if (deferred) sort_keys(c);
             else sort_bunch(c);  // the bounds pass's chunk boxes no longer match the particle order
             c->prof.mark(PROF_SORT);
@@ -1243,12 +1251,19 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             c->prof.mark(PROF_SCALAR);
             // the bins' Green's functions need only their scalars: they run on the
             // second stream beside the deposits and the forward x/y passes
-            const bool ov = nz > 0 && overlap && !c->prof.enabled && c->stream2 && c->grid.ncell >= (size_t)96 * 96 * 96;
-            if (ov) {
+            const bool ov = nz > 0 && side;
+            if (ov || sort_side) {
                 (void)hipEventRecord(c->ev_mesh, c->stream);
                 (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
+            }
+            if (ov) {
                 c->grid.green_bins(scz, nz, c->bin_image, c->stream2);
                 (void)hipEventRecord(c->ev_green, c->stream2);
+            }
+            if (sort_side) {
+                sort_keys(c, nullptr, c->stream2);
+                (void)hipEventRecord(c->ev_sort, c->stream2);
+                sort_launched = true;
             }
             if (nz > 0) c->grid.zero_bin_rho(nz, c->stream);
             c->prof.mark(PROF_ZERO);
@@ -1272,12 +1287,17 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             // on the second stream, beside the deposit and the x and y passes,
             // and the z pass waits for it.
             // (on small grids the extra synchronisation costs more than it hides)
-            overlap = overlap && !c->prof.enabled && c->stream2 && c->grid.ncell >= (size_t)96 * 96 * 96;
+            overlap = side;
             if (overlap) {
                 (void)hipEventRecord(c->ev_mesh, c->stream);
                 (void)hipStreamWaitEvent(c->stream2, c->ev_mesh, 0);
                 c->grid.green(c->d_sc, c->stream2);
                 (void)hipEventRecord(c->ev_green, c->stream2);
+                if (sort_side) {
+                    sort_keys(c, nullptr, c->stream2);
+                    (void)hipEventRecord(c->ev_sort, c->stream2);
+                    sort_launched = true;
+                }
             }
             c->grid.zero_rho(c->stream);
             c->prof.mark(PROF_ZERO);
@@ -1301,6 +1321,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                 double **dst[] = {&out.x, &out.y, &out.z, &out.px, &out.py, &out.pz};
                 *dst[k] = b.alt[k];
             }
+        if (sort_launched) (void)hipStreamWaitEvent(c->stream, c->ev_sort, 0);  // the permutation
         nparts = launch_track_gamma_os(c->stream, c->grid.fp64, next, out, bsum ? c->grid.Esum : c->grid.Eg, c->d_sc,
                                        c->d_ts, c->os_dense.as<double>(), c->partials, ext,
                                        boxes_out ? c->osbox.as<double>() : nullptr,
@@ -1341,6 +1362,7 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
         HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_mesh, hipEventDisableTiming));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_green, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_sort, hipEventDisableTiming));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_gimg, hipEventDisableTiming));
     }
     c->prof.start();
