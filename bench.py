@@ -285,11 +285,11 @@ def algorithmic_bytes(cfg, wbytes):
 
 # profiler kernel class -> kernel symbol of the mixed-precision step
 KERNEL_SYMBOL = {
-    "gather_kick_push": "k_track_gamma_mixed_os<true,0,false>",
-    "deposit": "k_deposit_lds<3,false,true,4864>",
+    "gather_kick_push": "k_track_gamma_mixed_os<true,0,false,false,false>",
+    "deposit": "k_deposit_lds<3,false,true,4864,false>",
     "rotate_bounds": "k_bounds_scan<false>",
-    "push_rotate_bounds": "k_track_alpha<false,true>",
-    "fft_z_conv": "k_strided<float,16,2>",
+    "push_rotate_bounds": "k_track_alpha<false,true,false>",
+    "fft_z_conv": "k_strided<float,8,2>",
     "grad": "k_grad<float>",
 }
 

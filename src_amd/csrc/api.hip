@@ -84,6 +84,9 @@ struct opalsc_ctx {
     bool q_uniform = false;   // every particle of the resident bunch carries the charge q_value:
     double q_value = 0.0;     // the tracker then keeps Q as that one number (no per-particle array)
     bool have_bunch = false, have_track = false;
+    // the resident bunch was re-sorted by cell since it was set: a re-sort can
+    // then be applied by the gather pass (an arbitrary order is permuted first)
+    bool cell_ordered = false;
     bool track_sc = false;    // the tracker computes space-charge fields
     bool track_ext = false;   // the tracker has a nonzero external field
     Profiler prof;
@@ -921,6 +924,7 @@ extern "C" int opalsc_bunch_set(opalsc_ctx *c, int64_t N, const double *R, const
     c->q_value = N > 0 ? Q[0] : 0.0;
     c->have_bunch = true;
     c->have_track = false;
+    c->cell_ordered = false;
     ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
@@ -942,6 +946,7 @@ extern "C" int opalsc_bunch_generate(opalsc_ctx *c, const opalsc_gauss_params *g
     c->q_value = qeach;
     c->have_bunch = true;
     c->have_track = false;
+    c->cell_ordered = false;
     ++c->gen;
     c->bins.nbins = 0;
     return OPALSC_OK;
@@ -1121,6 +1126,7 @@ extern "C" int opalsc_bunch_set_bins(opalsc_ctx *c, const int32_t *bin, int nbin
         return OPALSC_OK;
     }
     if (nbins < 0 || nbins > kMaxBins) FAIL(c, OPALSC_EINVAL, "nbins must be in [0, 64]");
+    c->cell_ordered = false;  // ordered by bin from here on
     Bunch &b = c->bunch;
     const int64_t N = b.N;
     if (N > 0 && !bin) FAIL(c, OPALSC_EINVAL, "null bin array");
@@ -1210,7 +1216,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     // arrays through the permutation and writes the new ones), so the deposit
     // reads the old order with its chunk boxes and no array is permuted on
     // its own
-    const bool deferred = sort_now && oneshot && b.alt[0];
+    const bool deferred = sort_now && oneshot && b.alt[0] && c->cell_ordered;
     // the second stream takes the Green's function beside the deposit and the
     // x/y passes on grids of 96^3 cells and more (on smaller ones the extra
     // synchronisation costs more than it hides), and then a deferred re-sort
@@ -1239,6 +1245,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
             else sort_bunch(c);  // the bounds pass's chunk boxes no longer match the particle order
             c->prof.mark(PROF_SORT);
         }
+        if (sort_now) c->cell_ordered = true;
         const double *boxes_in = sort_now && !deferred ? nullptr : b.chunkbox;
         if (binned) {
             // ParallelTTracker.cpp:567-579: every bin in its own Lorentz frame, and its
@@ -1815,6 +1822,7 @@ static int fused_step(opalsc_batch *b, const std::vector<int> &sorted, bool push
             opalsc_ctx *c = b->ctx[i];
             BHIP(b, hipStreamWaitEvent(c->stream, b->fork, 0));
             sort_bunch(c, b->scal_d + b->ctx_scal[i], c->stream);
+            c->cell_ordered = true;
             BHIP(b, hipEventRecord(b->join[i], c->stream));
             BHIP(b, hipStreamWaitEvent(m, b->join[i], 0));
             nobox[i] = 1;
