@@ -1829,16 +1829,17 @@ static int fused_step(opalsc_batch *b, const std::vector<int> &sorted, bool push
         }
         if (int e = fused_upload(b, &nobox)) return e;  // the sorted bunches' arrays changed places
     }
-    // the Green's functions need only the mesh: each group's on its own stream
+    // the Green's functions need only the mesh: each group's on its own stream,
+    // beside the deposit (enqueued first, so that it starts first)
     BHIP(b, hipEventRecord(b->ev_mesh, m));
-    for (BatchGroup *g : b->groups) {
-        BHIP(b, hipStreamWaitEvent(g->s, b->ev_mesh, 0));
-        g->pg.green_bins(b->scal_d + g->sc0, g->nb, false, g->s);
-    }
     for (BatchGroup *g : b->groups) g->pg.zero_bin_rho(g->nb, m);
     for (const BatchClass &k : b->classes)
         if (k.sc_on) launch_batch_deposit(m, fused_view(b, k, BP_DEPOSIT), k.exact, k.quni, k.large);
     BHIP(b, hipEventRecord(b->ev_dep, m));
+    for (BatchGroup *g : b->groups) {
+        BHIP(b, hipStreamWaitEvent(g->s, b->ev_mesh, 0));
+        g->pg.green_bins(b->scal_d + g->sc0, g->nb, false, g->s);
+    }
     for (BatchGroup *g : b->groups) {
         BHIP(b, hipStreamWaitEvent(g->s, b->ev_dep, 0));
         g->pg.solve_bins(b->scal_d + g->sc0, g->nb, false, nullptr);
