@@ -111,6 +111,31 @@ def test_compute_potential_fp64(n, greens):
     assert rel_l2(phi, ref) < 1e-12
 
 
+@pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-12), (sc.MIXED, 1e-5)])
+@pytest.mark.parametrize("n", [(32, 48, 20), (64, 24, 40)])
+def test_every_fft_tile_the_autotuner_can_pick(n, precision, tol):
+    """Every tile (elements per thread E) the kernels offer for each transform
+    length, set as the autotuner sets it: the solve stays within the bar of
+    the oracle (radix sequences 8/4/2 with 3 or 5 last, tiles of 4 to 64 lines)."""
+    rng = np.random.default_rng(21)
+    rho = rng.normal(size=n[0] * n[1] * n[2])
+    hr = np.array([1.1e-4, 0.9e-4, 2.3e-4])
+    ref = O.compute_potential(n, rho, hr, 0)
+    lengths = sorted({2 * m for m in n})
+    cands = {L: sc.tune_fft_candidates(L, precision) for L in lengths}
+    try:
+        for k in range(max(len(v) for v in cands.values())):
+            for L in lengths:
+                sc.tune_fft_set(L, precision, cands[L][min(k, len(cands[L]) - 1)])
+            with ctx(n, precision=precision) as c:
+                phi = c.computePotential(rho, hr)
+            err = rel_l2(phi, ref)
+            assert err < tol, ({L: sc.tune_fft_get(L, precision) for L in lengths}, err)
+    finally:
+        for L in lengths:
+            sc.tune_fft_set(L, precision, 0)
+
+
 @pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (8, 48, 12), (128, 64, 32), (40, 20, 80)])
 def test_compute_potential_mixed(n):
     """FP32 convolution (FP64 Green's function): tolerance 1e-5 relative L2."""
@@ -934,6 +959,26 @@ def test_tracker_few_particles():
     assert rel_l2(Pg - P, Po - P) < 1e-9
     for a, b in zip(sg, so):
         assert np.allclose(a["rrms"], b["rrms"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("precision,tol_r,tol_s", [(sc.FP64, 1e-12, 1e-10), (sc.MIXED, 1e-9, 1e-6)])
+def test_tracker_on_a_radix5_grid(precision, tol_r, tol_s):
+    """A 40 x 20 x 40 grid (transform lengths 80 and 40: a radix-5 stage) in the
+    tracker, five steps against the oracle."""
+    n, N, steps = (40, 20, 40), 30000, 5
+    R, P, Q = gaussian_bunch(N, seed=17, q_total=1e-9)
+    dt = np.full(N, 1e-11)
+    tr = O.make_tracker(n, 1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.4)
+    Ro, Po, _, _, so = O.track(tr, R, P, Q, dt, steps)
+    with ctx(n, precision=precision, N=N, sort_interval=2) as c:
+        c.set_bunch(R, P, Q)
+        c.track_init(1e-11, ref_p=(0.0, 0.0, 0.046181), Bz=0.4)
+        sg = c.track(steps)
+        Rg, Pg, _ = c.get_bunch()
+    assert rel_l2(Rg, Ro) < tol_r
+    for a, b in zip(sg, so):
+        for k in ("rrms", "eps_norm"):
+            assert np.allclose(a[k], b[k], rtol=tol_s, atol=0), (k, a[k], b[k])
 
 
 @pytest.mark.parametrize("precision,N", [(sc.FP64, 511), (sc.FP64, 513), (sc.FP64, 4097),
