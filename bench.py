@@ -272,6 +272,8 @@ def algorithmic_bytes(cfg, wbytes):
         "deposit": N * 24 + n3 * 16,
         # read+write x,y,z,px,py,pz; read the E grid (4 words per cell) once
         "gather_kick_push": N * 96 + n3 * 4 * wbytes,
+        # the same applying a re-sort: also the permutation and the ids (4 B each, read and written)
+        "gather_kick_push_resort": N * 108 + n3 * 4 * wbytes,
         # x pass: int64 rho -> A, and A -> phi (two launches; per-launch average)
         "fft_x": (n3 * 8 + K0 * n * n * cw + K0 * n * n * cw + n3 * wbytes) / 2,
         # y passes: A -> B, B -> A (two launches)

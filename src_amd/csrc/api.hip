@@ -123,7 +123,8 @@ namespace opalsc {
 const char *prof_name(int id) {
     static const char *names[PROF_COUNT] = {"push_rotate_bounds", "scalar", "sort", "zero_rho",
                                             "deposit", "green", "green_dct", "fft_x", "fft_y",
-                                            "fft_z_conv", "grad", "gather_kick_push", "rotate_bounds"};
+                                            "fft_z_conv", "grad", "gather_kick_push", "rotate_bounds",
+                                            "gather_kick_push_resort"};
     return (id >= 0 && id < PROF_COUNT) ? names[id] : "?";
 }
 void Profiler::start() {
@@ -1349,7 +1350,7 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     else
         launch_track_gamma(c->stream, c->grid.fp64, next, b, c->grid.Eg, c->d_sc, c->d_ts, c->partials, nullptr,
                            ext, c->grid.pz);
-    c->prof.mark(PROF_GATHER_PUSH);
+    c->prof.mark(deferred ? PROF_GATHER_RESORT : PROF_GATHER_PUSH);
     launch_track_post(c->stream, c->partials, nparts, c->d_ts, c->d_sc, c->statsb.p, &c->d_counters[1]);
     c->prof.mark(PROF_SCALAR);
     return boxes_out;

@@ -54,6 +54,7 @@ enum ProfId {
     PROF_GRAD,             // E = -Grad(phi)
     PROF_GATHER_PUSH,      // gather + kick + push + frame update + moments
     PROF_ROTATE_BOUNDS,    // beam-frame rotation + bounds of already pushed positions
+    PROF_GATHER_RESORT,    // the gather pass that also applies a re-sort (reads through the permutation)
     PROF_COUNT
 };
 
