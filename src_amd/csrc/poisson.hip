@@ -187,7 +187,14 @@ void launch_green_fast(hipStream_t s, double *G, const int n[3], const Scalars *
 template <int E>
 constexpr int half_q() { return E / 2 + 1; }
 
-// Octant line `line` = (k, j) of G (pitch n0+1, FP64) -> row of Gh (pitch gp).
+// The DCT-I passes transform two real lines in one complex line: the even
+// extension of a real line has a real DFT, so FFT(a + i b) = A + i B with A and
+// B real, and the real and imaginary parts of the result are the two lines'
+// transforms.  The pair is two neighbouring lines (of similar magnitude, so
+// the rounding each adds to the other is that of the pair's larger value).
+
+// Octant lines (k, j) of G (pitch n0+1, FP64) -> rows of Gh (pitch gp); complex
+// line p of the tile holds real lines 2p and 2p+1 of the block's 2W.
 template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(
     const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp, size_t gB, size_t ghB) {
@@ -199,68 +206,78 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     C *buf = (C *)smem;
     const int W = pl.W, P = pl.P, L = pl.L, m = n0 + 1, total = W * m;
     const float rm = 1.0f / (float)m;
-    const int line0 = blockIdx.x * W;
+    const int line0 = blockIdx.x * 2 * W;
     const C *tw = fft_twiddles(buf, pl);
-    T r[Q];
+    T r[Q], u[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = min(threadIdx.x + q * kFftThreads, total - 1);  // clamped: loads stay unconditional
         const int ln = div_small(e, m, rm), i = e - ln * m;
-        r[q] = (T)G[(size_t)min(line0 + ln, nlines - 1) * m + i];
+        r[q] = (T)G[(size_t)min(line0 + 2 * ln, nlines - 1) * m + i];
+        u[q] = (T)G[(size_t)min(line0 + 2 * ln + 1, nlines - 1) * m + i];
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = threadIdx.x + q * kFftThreads;
         if (e < total) {
             const int ln = div_small(e, m, rm), i = e - ln * m;
-            buf[i * P + ln] = C{r[q], T(0)};
-            if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{r[q], T(0)};
+            buf[i * P + ln] = C{r[q], u[q]};
+            if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{r[q], u[q]};
         }
     }
     __syncthreads();
     fft_lds<C, E>(buf, tw, pl, false);
     for (int e = threadIdx.x; e < total; e += kFftThreads) {
         const int ln = div_small(e, m, rm), k = e - ln * m;
-        if (line0 + ln < nlines) Gh[(size_t)(line0 + ln) * gp + k] = buf[k * P + ln].x;
+        const C v = buf[k * P + ln];
+        if (line0 + 2 * ln < nlines) Gh[(size_t)(line0 + 2 * ln) * gp + k] = v.x;
+        if (line0 + 2 * ln + 1 < nlines) Gh[(size_t)(line0 + 2 * ln + 1) * gp + k] = v.y;
     }
 }
 
 // DCT-I along a strided axis of Gh in place: element i of line (x, o) at
-// o*ostride + x + i*S, x in [x0, x0+W) (whole tiles: the row pitch is a
-// multiple of W and the padding columns hold zeros), i in [0, npts).
+// o*ostride + x + i*S, x in [x0, x0+2W) (whole tiles: the row pitch is a
+// multiple of 2W), i in [0, npts); complex column p of the tile holds the real
+// columns x0+2p and x0+2p+1 (one two-element load and store each).  Columns
+// x >= ncols are padding: they hold zeros and are stored as zeros, since the
+// transform of a padding column paired with a real one is that column's
+// rounding, which later calls would transform again and feed back.
 template <typename T, int E>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_strided(T *Gh, FftPlan pl, int npts, size_t S,
-                                                             size_t ostride, size_t ghB) {
+                                                             size_t ostride, size_t ghB, int ncols) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
     Gh += blockIdx.z * ghB;
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * npts;
-    const int x0 = blockIdx.x * W;
+    const int x0 = blockIdx.x * 2 * W;
     const size_t base = (size_t)blockIdx.y * ostride + x0;
     const int nn = npts - 1;
     const C *tw = fft_twiddles(buf, pl);
-    T r[Q];
+    C r[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = min(threadIdx.x + q * kFftThreads, total - 1);
-        r[q] = Gh[base + (e & (W - 1)) + (size_t)(e >> lw) * S];
+        r[q] = *(const C *)(Gh + base + 2 * (e & (W - 1)) + (size_t)(e >> lw) * S);
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = threadIdx.x + q * kFftThreads;
         if (e < total) {
             const int ln = e & (W - 1), i = e >> lw;
-            buf[i * P + ln] = C{r[q], T(0)};
-            if (i >= 1 && i <= nn - 1) buf[(L - i) * P + ln] = C{r[q], T(0)};
+            buf[i * P + ln] = r[q];
+            if (i >= 1 && i <= nn - 1) buf[(L - i) * P + ln] = r[q];
         }
     }
     __syncthreads();
     fft_lds<C, E>(buf, tw, pl, false);
     for (int e = threadIdx.x; e < total; e += kFftThreads) {
-        const int ln = e & (W - 1), k = e >> lw;
-        Gh[base + ln + (size_t)k * S] = buf[k * P + ln].x;
+        const int ln = e & (W - 1), k = e >> lw, x = x0 + 2 * ln;
+        C v = buf[k * P + ln];
+        if (x >= ncols) v.x = T(0);
+        if (x + 1 >= ncols) v.y = T(0);
+        *(C *)(Gh + base + 2 * ln + (size_t)k * S) = v;
     }
 }
 
@@ -828,7 +845,8 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
     // tile-aligned pitches: whole 128-byte rows in every strided pass
     const int tilew = std::max(std::max(plan[1].W, plan[2].W), (int)(128 / cs));
     kp = (K0 + tilew - 1) / tilew * tilew;
-    const int gtile = (int)(128 / rs);
+    // Ghat rows: whole 128-byte lines and whole tiles of the DCT passes (2W real columns)
+    const int gtile = std::max((int)(128 / rs), 2 * std::max(plan[1].W, plan[2].W));
     gp = (kp + gtile - 1) / gtile * gtile;
     ncell = (size_t)n[0] * n[1] * n[2];
     noct = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
@@ -930,15 +948,15 @@ static void dct_xy(PoissonGrid &g, const GridSet &gs, const double *G, T *H, hip
     {
         const FftPlan &pl = g.plan[0];
         int nlines = (n1 + 1) * (n2 + 1);
-        dim3 grid((nlines + pl.W - 1) / pl.W, 1, gs.nb);
+        dim3 grid((nlines + 2 * pl.W - 1) / (2 * pl.W), 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
                                     G, H, pl, n0, nlines, g.gp, gs.GB, gs.GhB));
     }
     {
         const FftPlan &pl = g.plan[1];
-        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n2 + 1, gs.nb);
+        dim3 grid((n0 + 1 + 2 * pl.W - 1) / (2 * pl.W), n2 + 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    H, pl, n1 + 1, gp, gp * (n1 + 1), gs.GhB));
+                                    H, pl, n1 + 1, gp, gp * (n1 + 1), gs.GhB, n0 + 1));
     }
 }
 
@@ -983,9 +1001,9 @@ static void green_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, h
                                     (const T *)g.Ghat, (const T *)g.Ghat, (C *)g.Gper, pl, n1, n2, g.gp, 0, 0));
     } else {
         const FftPlan &pl = g.plan[2];
-        dim3 grid((n0 + 1 + pl.W - 1) / pl.W, n1 + 1, gs.nb);
+        dim3 grid((n0 + 1 + 2 * pl.W - 1) / (2 * pl.W), n1 + 1, gs.nb);
         OPALSC_DISPATCH_E(pl.E, k_dct_strided<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    (T *)gs.Ghat, pl, n2 + 1, gp * (n1 + 1), gp, gs.GhB));
+                                    (T *)gs.Ghat, pl, n2 + 1, gp * (n1 + 1), gp, gs.GhB, n0 + 1));
     }
     g.profile_mark(PROF_DCT);
 }
