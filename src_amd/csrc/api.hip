@@ -59,6 +59,7 @@ struct opalsc_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // the tracker's Green's function, concurrent with the deposit
     hipEvent_t ev_mesh = nullptr, ev_green = nullptr, ev_gimg = nullptr, ev_sort = nullptr;
+    hipEvent_t ev_start = nullptr, ev_zero = nullptr;  // a step's start; its charge grid cleared
     PoissonGrid grid;
     Bunch bunch;
     Scalars *d_sc = nullptr;
@@ -294,6 +295,8 @@ extern "C" int opalsc_destroy(opalsc_ctx *c) {
     if (c->ev_green) (void)hipEventDestroy(c->ev_green);
     if (c->ev_sort) (void)hipEventDestroy(c->ev_sort);
     if (c->ev_gimg) (void)hipEventDestroy(c->ev_gimg);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_zero) (void)hipEventDestroy(c->ev_zero);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1225,6 +1228,17 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     const bool side = overlap && !c->prof.enabled && c->stream2 && c->ev_sort && c->grid.ncell >= (size_t)96 * 96 * 96;
     const bool sort_side = deferred && side;
     bool sort_launched = false;
+    // the single grid's charge words are cleared on the second stream at the
+    // step's start, beside the statistics, bounds and mesh kernels (one block
+    // each), instead of between the mesh and the deposit; the previous step's
+    // x pass, which read them, is done by then
+    const bool zero_side = side && c->track_sc && !binned && c->ev_zero;
+    if (zero_side) {
+        (void)hipEventRecord(c->ev_start, c->stream);
+        (void)hipStreamWaitEvent(c->stream2, c->ev_start, 0);
+        c->grid.zero_rho(c->stream2);
+        (void)hipEventRecord(c->ev_zero, c->stream2);
+    }
     launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
     if (chunked && !boxes) launch_bin_gamma_chunks(c->stream, c->grid.fp64, b, c->bins, c->binchunks.as<double>());
     c->prof.mark(PROF_SCALAR);
@@ -1307,7 +1321,8 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
                     sort_launched = true;
                 }
             }
-            c->grid.zero_rho(c->stream);
+            if (zero_side) (void)hipStreamWaitEvent(c->stream, c->ev_zero, 0);
+            else c->grid.zero_rho(c->stream);
             c->prof.mark(PROF_ZERO);
             launch_deposit(c->stream, c->q_uniform ? 3 : 1, exact, b.N, b.x, b.y, b.z, b.q, nullptr,
                            (long long *)c->grid.rho_i64, c->d_sc, &c->d_counters[1], boxes_in);
@@ -1372,6 +1387,8 @@ extern "C" int opalsc_track(opalsc_ctx *c, int nsteps, opalsc_stats *per_step) {
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_green, hipEventDisableTiming));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_sort, hipEventDisableTiming));
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_gimg, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_zero, hipEventDisableTiming));
     }
     c->prof.start();
     launch_track_begin_call(c->stream, c->d_ts);

@@ -336,6 +336,21 @@ __host__ __device__ inline int os_blocks(int64_t N) {
     const int64_t per = (nchunks + kXcds - 1) / kXcds;
     return (int)(per * kXcds);
 }
+// The block's (x, y, z) index with the grid's blocks reordered so that each
+// XCD takes one contiguous range of the x-fastest linear order: the workgroup
+// of linear index g (in dispatch order) runs on XCD g mod 8, and neighbouring
+// indices of the returned order (tiles that share 128-byte lines) run on the
+// same XCD, one after the other, so the second fetch of a shared line hits
+// that XCD's L2.  A bijection of the grid onto itself for any size.
+__device__ inline void xcd_block(int &bx, int &by, int &bz) {
+    const int gx = (int)gridDim.x, gxy = gx * (int)gridDim.y, nt = gxy * (int)gridDim.z;
+    const int g = (int)blockIdx.x + gx * ((int)blockIdx.y + (int)gridDim.y * (int)blockIdx.z);
+    const int q = nt / kXcds, r = nt % kXcds, xcd = g % kXcds, slot = g / kXcds;
+    const int l = xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+    bz = l / gxy;
+    by = (l - bz * gxy) / gx;
+    bx = l - bz * gxy - by * gx;
+}
 __host__ __device__ inline int os_blocks64(int64_t N) {
     const int64_t nchunks = (N + kOsChunk64 - 1) / kOsChunk64;
     const int64_t per = (nchunks + kXcds - 1) / kXcds;

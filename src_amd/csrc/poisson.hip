@@ -247,12 +247,14 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
                                                              size_t ostride, size_t ghB, int ncols) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
-    Gh += blockIdx.z * ghB;
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    Gh += bz * ghB;
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * npts;
-    const int x0 = blockIdx.x * 2 * W;
-    const size_t base = (size_t)blockIdx.y * ostride + x0;
+    const int x0 = bx * 2 * W;
+    const size_t base = (size_t)by * ostride + x0;
     const int nn = npts - 1;
     const C *tw = fft_twiddles(buf, pl);
     C r[Q];
@@ -419,15 +421,16 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
                                                          int gp, int n1, int n2, int L1, size_t inB, size_t outB,
                                                          size_t ghB) {
     using C = typename cplx_of<T>::type;
-    in += blockIdx.z * inB;
-    out += blockIdx.z * outB;
-    if (Gh) Gh += blockIdx.z * ghB;
+    int bx, o, bz;
+    xcd_block(bx, o, bz);
+    in += bz * inB;
+    out += bz * outB;
+    if (Gh) Gh += bz * ghB;
     constexpr int Q = (MODE == 1 || MODE == 4) ? E : E / 2;  // whole-line passes read E per thread
     extern __shared__ char smem[];
     C *buf = (C *)smem;
     const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, total = W * nin;
-    const int kx0 = blockIdx.x * W;
-    const int o = blockIdx.y;
+    const int kx0 = bx * W;
     const size_t ib = (size_t)o * inO + kx0, ob = (size_t)o * outO + kx0;
     const C *tw = fft_twiddles(buf, pl);
     C r[Q];
@@ -436,6 +439,23 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         const int e = min(threadIdx.x + q * kFftThreads, total - 1);
         r[q] = in[ib + (size_t)(e >> lw) * inS + (e & (W - 1))];
     }
+    // MODE 2 (o is ky; Ghat folded into the octant, Ghat[2n-k] = Ghat[k]):
+    // the thread's Ghat factors are loaded with the input, so that their
+    // latency overlaps the input's instead of following the forward transform,
+    // when E of them fit in 16 registers; larger tiles load them after the
+    // transform, in halves
+    constexpr bool GPRE = MODE == 2 && E * sizeof(T) <= 64;
+    T g[GPRE ? E : 1];
+    const int fy = o <= n1 ? o : L1 - o;
+    auto gload = [&](int q) -> T {
+        const int e = threadIdx.x + q * kFftThreads;
+        const int kz = e >> lw;
+        const int fz = kz <= n2 ? kz : L - kz;
+        return Gh[((size_t)fz * (n1 + 1) + fy) * gp + kx0 + (e & (W - 1))];
+    };
+    if constexpr (GPRE)
+#pragma unroll
+        for (int q = 0; q < E; ++q) g[q] = gload(q);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = threadIdx.x + q * kFftThreads;
@@ -447,29 +467,29 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     }
     __syncthreads();
     if constexpr (MODE == 2) {
-        // o is ky; Ghat folded into the octant (Ghat[2n-k] = Ghat[k]), all loads
-        // of a thread issued together
         fft_lds<C, E>(buf, tw, pl, false);
-        const int fy = o <= n1 ? o : L1 - o;
-        // in halves of E/2 loads in flight: the whole E would hold E more
-        // registers across the multiply and spill at E = 16
-        constexpr int H = E >= 8 ? E / 2 : E;
+        if constexpr (GPRE) {
 #pragma unroll
-        for (int h = 0; h < E; h += H) {
-            T g[H];
-#pragma unroll
-            for (int q = 0; q < H; ++q) {
-                const int e = threadIdx.x + (h + q) * kFftThreads;
-                const int kz = e >> lw;
-                const int fz = kz <= n2 ? kz : L - kz;
-                g[q] = Gh[((size_t)fz * (n1 + 1) + fy) * gp + kx0 + (e & (W - 1))];
-            }
-#pragma unroll
-            for (int q = 0; q < H; ++q) {
-                const int e = threadIdx.x + (h + q) * kFftThreads;
+            for (int q = 0; q < E; ++q) {
+                const int e = threadIdx.x + q * kFftThreads;
                 const int idx = (e >> lw) * P + (e & (W - 1));
                 const C v = buf[idx];
                 buf[idx] = C{v.x * g[q], v.y * g[q]};
+            }
+        } else {
+            constexpr int H = E / 2;
+#pragma unroll
+            for (int h = 0; h < E; h += H) {
+                T gh[H];
+#pragma unroll
+                for (int q = 0; q < H; ++q) gh[q] = gload(h + q);
+#pragma unroll
+                for (int q = 0; q < H; ++q) {
+                    const int e = threadIdx.x + (h + q) * kFftThreads;
+                    const int idx = (e >> lw) * P + (e & (W - 1));
+                    const C v = buf[idx];
+                    buf[idx] = C{v.x * gh[q], v.y * gh[q]};
+                }
             }
         }
         __syncthreads();
@@ -741,8 +761,13 @@ static int tile_lookup(int L, bool fp64) {
 
 // Tile shape for length L: the tuned or overridden E if the kernels take it,
 // else the smallest E whose tile has W >= 128 bytes / complex size lines,
-// within the register budget.
-static bool make_plan(int L, bool fp64, FftPlan &pl) {
+// within the register budget.  LDS pitch: the x passes fill and drain their
+// tiles along the transform (lanes of a wave P elements apart), which an odd
+// pitch W + 1 spreads over the banks; the strided (y, z) passes fill, drain
+// and run every Stockham stage across the tile's W lines (lanes 1 element
+// apart, rows of W), where the pitch W keeps two rows of a W = 8 FP32 tile
+// on disjoint banks and W + 1 would overlap them.
+static bool make_plan(int L, bool fp64, bool strided, FftPlan &pl) {
     int a = 0, odd = 1;
     if (!split_length(L, a, odd)) return false;
     pl.L = L;
@@ -762,7 +787,7 @@ static bool make_plan(int L, bool fp64, FftPlan &pl) {
     pl.lw = 0;
     while ((1 << pl.lw) < pl.W) ++pl.lw;
     if ((1 << pl.lw) != pl.W) return false;
-    pl.P = pl.W + 1;
+    pl.P = strided ? pl.W : pl.W + 1;
     pl.nst = 0;
     // every radix must divide E (each thread owns E/R whole butterflies)
     const bool use8 = (pl.E % 8) == 0, use4 = (pl.E % 4) == 0;
@@ -821,7 +846,7 @@ int PoissonGrid::init(const int nn[3], bool fp64_, int greens_, bool periodic_z,
         // periodic z: the z transform has the grid's own length (FFTPoissonSolver.cpp:130-155)
         L[d] = (pz && d == 2) ? nn[d] : 2 * nn[d];
         if (nn[d] < 4) { err = "n must be >= 4 in every dimension"; return -1; }
-        if (!make_plan(L[d], fp64, plan[d])) {
+        if (!make_plan(L[d], fp64, d != 0, plan[d])) {
             err = "unsupported grid size n=" + std::to_string(nn[d]) +
                   ((pz && d == 2) ? " for periodic z (need n = 2^a, 3*2^a or 5*2^a, a>=2)"
                                   : " (need n = 2^a, 3*2^a or 5*2^a, a>=1)");
@@ -1369,7 +1394,7 @@ extern "C" int opalsc_tune_fft_set(int L, int precision, int E) {
 
 extern "C" int opalsc_tune_fft_get(int L, int precision) {
     opalsc::FftPlan pl;
-    if (!opalsc::make_plan(L, precision == OPALSC_FP64, pl)) return OPALSC_EINVAL;
+    if (!opalsc::make_plan(L, precision == OPALSC_FP64, false, pl)) return OPALSC_EINVAL;
     return pl.E;
 }
 
