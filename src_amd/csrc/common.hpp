@@ -419,6 +419,46 @@ __device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+// Wave sums of 16 values per lane by recursive halving: at distance d = 32,
+// 16, 8, 4 a lane keeps the half of its remaining values selected by its lane
+// bit d, sends the other half to lane ^ d and adds what that lane sends; then
+// two butterfly steps (d = 2, 1).  Every partial sum pairs the same lanes in
+// the same order as wave_sum (v + partner at d = 32, 16, ..., 1), so the sums
+// equal wave_sum's bit for bit, with 17 exchanged values instead of 96.
+// Returns the sum of value (lane >> 2) over the wave.
+__device__ inline double wave_sum16_split(const double (&v)[16], int lane) {
+    double a[8], b[4], c[2];
+    {
+        const bool up = lane & 32;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double keep = up ? v[8 + i] : v[i], send = up ? v[i] : v[8 + i];
+            a[i] = keep + __shfl_xor(send, 32, 64);
+        }
+    }
+    {
+        const bool up = lane & 16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double keep = up ? a[4 + i] : a[i], send = up ? a[i] : a[4 + i];
+            b[i] = keep + __shfl_xor(send, 16, 64);
+        }
+    }
+    {
+        const bool up = lane & 8;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double keep = up ? b[2 + i] : b[i], send = up ? b[i] : b[2 + i];
+            c[i] = keep + __shfl_xor(send, 8, 64);
+        }
+    }
+    const bool up = lane & 4;
+    const double keep = up ? c[1] : c[0], send = up ? c[0] : c[1];
+    double t = keep + __shfl_xor(send, 4, 64);
+    t += __shfl_xor(t, 2, 64);
+    t += __shfl_xor(t, 1, 64);
+    return t;
+}
 __device__ inline double wave_min(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
     return v;

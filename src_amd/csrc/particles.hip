@@ -1893,10 +1893,11 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
         }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < kMom; ++k) m[k] = wave_sum(m[k]);
-    if (lane == 0)
-        for (int k = 0; k < kMom; ++k) wsum[wave][k] = m[k];
+    static_assert(kMom == 16, "wave_sum16_split reduces 16 values");
+    {
+        const double t = wave_sum16_split(m, lane);  // the wave's sum of moment lane >> 2
+        if ((lane & 3) == 0) wsum[wave][lane >> 2] = t;
+    }
     __syncthreads();
     if (threadIdx.x < kMom) {
         double t = wsum[0][threadIdx.x];
@@ -1905,7 +1906,7 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
     }
     if constexpr (NEXT) {
         if (osbox) {
-            block_minmax(bmn, bmx, lbox);
+            block_minmax_split(bmn, bmx, lbox);
             if (threadIdx.x == 0 && chunk < nchunks)
                 for (int d = 0; d < 3; ++d) {
                     osbox[chunk * kOsBoxStats + d] = bmn[d];
@@ -2062,10 +2063,11 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
         }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < kMom; ++k) m[k] = wave_sum(m[k]);
-    if (lane == 0)
-        for (int k = 0; k < kMom; ++k) wsum[wave][k] = m[k];
+    static_assert(kMom == 16, "wave_sum16_split reduces 16 values");
+    {
+        const double t = wave_sum16_split(m, lane);  // the wave's sum of moment lane >> 2
+        if ((lane & 3) == 0) wsum[wave][lane >> 2] = t;
+    }
     __syncthreads();
     if (threadIdx.x < kMom) {
         double t = wsum[0][threadIdx.x];
@@ -2074,7 +2076,7 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
     }
     if constexpr (NEXT) {
         if (osbox) {
-            block_minmax(bmn, bmx, lbox);
+            block_minmax_split(bmn, bmx, lbox);
             if (threadIdx.x == 0 && chunk < nchunks)
                 for (int d = 0; d < 3; ++d) {
                     osbox[chunk * kOsBoxStats + d] = bmn[d];

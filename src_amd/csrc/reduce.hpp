@@ -71,6 +71,50 @@ __device__ inline void block_minmax(double (&mn)[3], double (&mx)[3], double *ld
     __syncthreads();
 }
 
+// block_minmax for the gather passes' block boxes: the maxima negated into
+// minima (exact), the six minima reduced in a wave by recursive halving (10
+// exchanged values instead of 36; a minimum does not depend on the order),
+// thread 0 gets the block's result.
+__device__ inline void block_minmax_split(double (&mn)[3], double (&mx)[3], double *lds) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const double v[8] = {mn[0], mn[1], mn[2], -mx[0], -mx[1], -mx[2], __builtin_inf(), __builtin_inf()};
+    double a[4], b[2];
+    {
+        const bool up = lane & 32;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double keep = up ? v[4 + i] : v[i], send = up ? v[i] : v[4 + i];
+            a[i] = fmin(keep, __shfl_xor(send, 32, 64));
+        }
+    }
+    {
+        const bool up = lane & 16;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double keep = up ? a[2 + i] : a[i], send = up ? a[i] : a[2 + i];
+            b[i] = fmin(keep, __shfl_xor(send, 16, 64));
+        }
+    }
+    const bool up = lane & 8;
+    double t = fmin(up ? b[1] : b[0], __shfl_xor(up ? b[0] : b[1], 8, 64));
+    t = fmin(t, __shfl_xor(t, 4, 64));
+    t = fmin(t, __shfl_xor(t, 2, 64));
+    t = fmin(t, __shfl_xor(t, 1, 64));
+    const int idx = lane >> 3;  // the value this lane's t reduces
+    if ((lane & 7) == 0 && idx < 6) lds[wave * 6 + idx] = idx < 3 ? t : -t;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = lds[d];
+            mx[d] = lds[3 + d];
+            for (int w = 1; w < nw; ++w) {
+                mn[d] = fmin(mn[d], lds[w * 6 + d]);
+                mx[d] = fmax(mx[d], lds[w * 6 + 3 + d]);
+            }
+        }
+    __syncthreads();
+}
+
 // Per-thread chunk statistics of positions: min[3], max[3], sum[3], sum of squares[3].
 __device__ inline void box_stats_init(double (&st)[kBoxStats]) {
     for (int d = 0; d < 3; ++d) {
