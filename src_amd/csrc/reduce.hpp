@@ -21,8 +21,62 @@ inline int stream_blocks(int64_t N) {
 }
 
 // ---- deterministic block reductions ---------------------------------------
+// One step of a recursive-halving wave reduction of 2H double-doubles: the
+// lane keeps the half selected by its lane bit d and adds the other half of
+// lane ^ d (the same operands as a butterfly step; dd_add is commutative bit
+// for bit, since TwoSum's error term is exact).
+template <int H>
+__device__ inline void dd_halve(const double (&h)[2 * H], const double (&l)[2 * H], double (&ho)[H],
+                                double (&lo)[H], int lane, int d) {
+    const bool up = lane & d;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double kh = up ? h[H + i] : h[i], kl = up ? l[H + i] : l[i];
+        const double sh = up ? h[i] : h[H + i], sl = up ? l[i] : l[H + i];
+        const dd r = dd_add(dd{kh, kl}, dd{__shfl_xor(sh, d, 64), __shfl_xor(sl, d, 64)});
+        ho[i] = r.hi;
+        lo[i] = r.lo;
+    }
+}
+
+// Block sum of K double-doubles per thread; thread 0 receives the result.
+// Across a wave: butterflies (lane ^ 32, ^ 16, ..., ^ 1); across waves: wave
+// 0's sum, then waves 1, 2, ... added in order.  K = 16 computes the same sums
+// with recursive halving within the wave (17 exchanged values instead of 96)
+// and one thread per value across the waves.
 template <int K>
 __device__ inline void block_sum_dd(double (&h)[K], double (&l)[K], double *lds) {
+    if constexpr (K == 16) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+        double h8[8], l8[8], h4[4], l4[4], h2[2], l2[2], h1[1], l1[1];
+        dd_halve<8>(h, l, h8, l8, lane, 32);
+        dd_halve<4>(h8, l8, h4, l4, lane, 16);
+        dd_halve<2>(h4, l4, h2, l2, lane, 8);
+        dd_halve<1>(h2, l2, h1, l1, lane, 4);
+        dd t = {h1[0], l1[0]};
+        for (int off = 2; off > 0; off >>= 1) t = dd_add(t, dd{__shfl_xor(t.hi, off, 64), __shfl_xor(t.lo, off, 64)});
+        const int k = lane >> 2;  // the value t sums
+        if ((lane & 3) == 0) {
+            lds[(wave * K + k) * 2] = t.hi;
+            lds[(wave * K + k) * 2 + 1] = t.lo;
+        }
+        __syncthreads();
+        if (threadIdx.x < K) {
+            const int q = threadIdx.x;
+            dd a = {lds[2 * q], lds[2 * q + 1]};
+            for (int w = 1; w < nw; ++w) a = dd_add(a, dd{lds[(w * K + q) * 2], lds[(w * K + q) * 2 + 1]});
+            lds[2 * q] = a.hi;  // only thread q reads or writes value q's slots here
+            lds[2 * q + 1] = a.lo;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int q = 0; q < K; ++q) {
+                h[q] = lds[2 * q];
+                l[q] = lds[2 * q + 1];
+            }
+        __syncthreads();
+        return;
+    }
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -182,7 +236,7 @@ __device__ inline void reduce_partials_minmax(const double *partials, int nb, do
             mn[d] = fmin(mn[d], partials[b * kSlots + d]);
             mx[d] = fmax(mx[d], partials[b * kSlots + 3 + d]);
         }
-    block_minmax(mn, mx, lds);
+    block_minmax_split(mn, mx, lds);
 }
 
 }  // namespace opalsc
