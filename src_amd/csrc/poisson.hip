@@ -153,8 +153,9 @@ __global__ __launch_bounds__(256) void k_green_standard(double *G, int n0, int n
 // igf::green_point_fast at every point (the exact 8-corner difference near
 // the origin, the cell-average expansion elsewhere).  shifted: the image
 // charge's tables (second: the one shifted by a further -n2*h2), which have no
-// G[0,0,0] quirk.  Grid: x over the (n0+1)*(n1+1) points of a plane, y over k,
-// z over the batch's grids (gB doubles apart).
+// G[0,0,0] quirk.  Grid: x over the (n0+1)*(n1+1) points of a plane, y over
+// runs of kGreenPlanes planes, z over the batch's grids (gB doubles apart).
+constexpr int kGreenPlanes = 4;
 __global__ __launch_bounds__(256) void k_green_fast(double *G, int n0, int n1, int n2, const Scalars *sc,
                                                    int shifted, int second, size_t gB) {
     sc += blockIdx.z;
@@ -165,14 +166,21 @@ __global__ __launch_bounds__(256) void k_green_fast(double *G, int n0, int n1, i
     const int g0 = n0 + 1, gplane = g0 * (n1 + 1);
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= gplane) return;
-    const int j = div_small(c, g0, 1.0f / (float)g0), i = c - j * g0, k = blockIdx.y;
-    const int kq = (!shifted && c == 0 && k == 0) ? 1 : k;  // G[0,0,0] := G[0,0,1] (FFTPoissonSolver.cpp:657)
-    G[(size_t)k * gplane + c] =
-        igf::green_point_fast(m, i, j, kq, h0, h1, h2, zshift, shifted != 0, second != 0, n2, ivol);
+    const int j = div_small(c, g0, 1.0f / (float)g0), i = c - j * g0;
+    // kGreenPlanes planes per thread: the cell moments and 1/volume (FP64
+    // divisions, the same for every point) once per thread, not per point
+#pragma unroll
+    for (int t = 0; t < kGreenPlanes; ++t) {
+        const int k = blockIdx.y * kGreenPlanes + t;
+        if (k > n2) break;
+        const int kq = (!shifted && c == 0 && k == 0) ? 1 : k;  // G[0,0,0] := G[0,0,1] (FFTPoissonSolver.cpp:657)
+        G[(size_t)k * gplane + c] =
+            igf::green_point_fast(m, i, j, kq, h0, h1, h2, zshift, shifted != 0, second != 0, n2, ivol);
+    }
 }
 void launch_green_fast(hipStream_t s, double *G, const int n[3], const Scalars *sc, int shifted, int second,
                        int nb, size_t gB) {
-    dim3 grid(((n[0] + 1) * (n[1] + 1) + 255) / 256, n[2] + 1, nb);
+    dim3 grid(((n[0] + 1) * (n[1] + 1) + 255) / 256, (n[2] + kGreenPlanes) / kGreenPlanes, nb);
     k_green_fast<<<grid, 256, 0, s>>>(G, n[0], n[1], n[2], sc, shifted, second, gB);
 }
 
