@@ -203,9 +203,14 @@ constexpr int half_q() { return E / 2 + 1; }
 
 // Octant lines (k, j) of G (pitch n0+1, FP64) -> rows of Gh (pitch gp); complex
 // line p of the tile holds real lines 2p and 2p+1 of the block's 2W.
-template <typename T, int E>
+// GREEN (the mixed-precision grid's IGF tables): the octant values are
+// evaluated here, by the function k_green_fast writes them with (table
+// 0: the G[0,0,0] quirk; 1, 2: the image charge's shifted tables), instead of
+// read from G, so the FP64 octant is never stored; scalars sc + blockIdx.z.
+template <typename T, int E, bool GREEN = false>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(
-    const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp, size_t gB, size_t ghB) {
+    const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp, size_t gB, size_t ghB,
+    const Scalars *sc = nullptr, int table = 0, int n1 = 0, int n2 = 0) {
     using C = typename cplx_of<T>::type;
     constexpr int Q = half_q<E>();
     G += blockIdx.z * gB;
@@ -216,21 +221,42 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     const float rm = 1.0f / (float)m;
     const int line0 = blockIdx.x * 2 * W;
     const C *tw = fft_twiddles(buf, pl);
-    T r[Q], u[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int e = min(threadIdx.x + q * kFftThreads, total - 1);  // clamped: loads stay unconditional
-        const int ln = div_small(e, m, rm), i = e - ln * m;
-        r[q] = (T)G[(size_t)min(line0 + 2 * ln, nlines - 1) * m + i];
-        u[q] = (T)G[(size_t)min(line0 + 2 * ln + 1, nlines - 1) * m + i];
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int e = threadIdx.x + q * kFftThreads;
-        if (e < total) {
+    if constexpr (GREEN) {
+        // no loads to keep in flight: each pair of values goes to LDS as it is formed
+        sc += blockIdx.z;
+        const double h0 = sc->hs[0], h1 = sc->hs[1], h2 = sc->hs[2], zshift = sc->zshift;
+        const double ivol = 1.0 / (h0 * h1 * h2);
+        const igf::GreenMoments gm = igf::green_moments(h0, h1, h2);
+        const int m1 = n1 + 1;
+        const float rm1 = 1.0f / (float)m1;
+        auto point = [&](int line, int i) -> T {
+            const int k = div_small(line, m1, rm1), j = line - k * m1;
+            const int kq = (table == 0 && i == 0 && j == 0 && k == 0) ? 1 : k;  // G[0,0,0] := G[0,0,1]
+            return (T)igf::green_point_fast(gm, i, j, kq, h0, h1, h2, zshift, table != 0, table == 2, n2, ivol);
+        };
+        for (int e = threadIdx.x; e < total; e += kFftThreads) {
             const int ln = div_small(e, m, rm), i = e - ln * m;
-            buf[i * P + ln] = C{r[q], u[q]};
-            if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{r[q], u[q]};
+            const C v = C{point(min(line0 + 2 * ln, nlines - 1), i), point(min(line0 + 2 * ln + 1, nlines - 1), i)};
+            buf[i * P + ln] = v;
+            if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = v;
+        }
+    } else {
+        T r[Q], u[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int e = min(threadIdx.x + q * kFftThreads, total - 1);  // clamped: loads stay unconditional
+            const int ln = div_small(e, m, rm), i = e - ln * m;
+            r[q] = (T)G[(size_t)min(line0 + 2 * ln, nlines - 1) * m + i];
+            u[q] = (T)G[(size_t)min(line0 + 2 * ln + 1, nlines - 1) * m + i];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int e = threadIdx.x + q * kFftThreads;
+            if (e < total) {
+                const int ln = div_small(e, m, rm), i = e - ln * m;
+                buf[i * P + ln] = C{r[q], u[q]};
+                if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{r[q], u[q]};
+            }
         }
     }
     __syncthreads();
@@ -973,8 +999,10 @@ static GridSet own_set(PoissonGrid &g, int src_kind) {
 }
 
 // DCT-I along x (octant G, FP64, pitch n0+1 -> H, pitch gp) and along y, in place in H.
+// table >= 0: the octant of IGF table `table` evaluated in the x pass (G unused).
 template <typename T>
-static void dct_xy(PoissonGrid &g, const GridSet &gs, const double *G, T *H, hipStream_t st) {
+static void dct_xy(PoissonGrid &g, const GridSet &gs, const double *G, T *H, hipStream_t st, int table = -1,
+                   const Scalars *d_sc = nullptr) {
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C), gp = (size_t)g.gp;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -982,8 +1010,13 @@ static void dct_xy(PoissonGrid &g, const GridSet &gs, const double *G, T *H, hip
         const FftPlan &pl = g.plan[0];
         int nlines = (n1 + 1) * (n2 + 1);
         dim3 grid((nlines + 2 * pl.W - 1) / (2 * pl.W), 1, gs.nb);
-        OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
-                                    G, H, pl, n0, nlines, g.gp, gs.GB, gs.GhB));
+        if (table >= 0) {
+            OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV, true><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
+                                        nullptr, H, pl, n0, nlines, g.gp, 0, gs.GhB, d_sc, table, n1, n2));
+        } else {
+            OPALSC_DISPATCH_E(pl.E, k_dct_x<T, EV><<<grid, kFftThreads, lds_bytes(pl, cs), st>>>(
+                                        G, H, pl, n0, nlines, g.gp, gs.GB, gs.GhB));
+        }
     }
     {
         const FftPlan &pl = g.plan[1];
@@ -1015,16 +1048,18 @@ template <typename T>
 static void green_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, hipStream_t st) {
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const bool igf = g.greens != 1;
+    // the mixed grid's IGF octant is evaluated inside the x pass of the DCT
+    const bool fused = igf && !g.fp64;
     if (!igf) {
         k_green_standard<<<dim3(1024, 1, gs.nb), 256, 0, st>>>(gs.Goct, n0, n1, n2, d_sc, gs.GB);
-    } else {
+    } else if (!fused) {
         igf_octant(g, gs, d_sc, 0, st);
     }
     g.profile_mark(PROF_GREEN);
     using C = typename cplx_of<T>::type;
     const size_t cs = sizeof(C);
     const size_t gp = (size_t)g.gp;
-    dct_xy<T>(g, gs, gs.Goct, (T *)gs.Ghat, st);
+    dct_xy<T>(g, gs, gs.Goct, (T *)gs.Ghat, st, fused ? 0 : -1, d_sc);
     if (g.pz) {
         // periodic z: the z line is G[k] for k in [0, n2), one-sided (mirrorRhoField,
         // FFTPoissonSolver.cpp:661-662), so its transform is complex (one grid only)
@@ -1051,8 +1086,12 @@ static void green_image_impl(PoissonGrid &g, const GridSet &gs, const Scalars *d
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     T *H[2] = {(T *)gs.HA, (T *)gs.HB};
     for (int second = 0; second < 2; ++second) {
-        igf_octant(g, gs, d_sc, 1 + second, st);
-        dct_xy<T>(g, gs, gs.Goct, H[second], st);
+        if (g.fp64) {
+            igf_octant(g, gs, d_sc, 1 + second, st);
+            dct_xy<T>(g, gs, gs.Goct, H[second], st);
+        } else {
+            dct_xy<T>(g, gs, nullptr, H[second], st, 1 + second, d_sc);  // octant evaluated in the x pass
+        }
     }
     g.profile_mark(PROF_GREEN);
     {
