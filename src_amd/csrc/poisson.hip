@@ -640,31 +640,47 @@ __device__ inline void grad_stencil(const T *p, int i, int j, int k, int n0, int
     else r2 = id2 * (p[sz] - p[-(ptrdiff_t)sz]);
 }
 
-// grid b = blockIdx.y of a batch: phi + b*total, Eg + b*total, scalars sc + b
+// Cell tiles of the Grad kernels: a block of 256 threads covers tw consecutive
+// i (a power of two, lane-fastest) by 256/tw rows j of plane k = blockIdx.y;
+// blockIdx.x = bj * nbi + bi.  No per-cell integer division.
+struct CellTile {
+    int lw, nbi;  // log2 of the tile width; tiles along i
+};
+static CellTile cell_tile(int n0) {
+    int lw = 6;  // 64 cells, narrower for narrow grids
+    while (lw > 3 && (1 << (lw - 1)) >= n0) --lw;
+    return CellTile{lw, (n0 + (1 << lw) - 1) >> lw};
+}
+__device__ inline bool tile_cell(const CellTile &t, int n0, int n1, int &i, int &j) {
+    const int bj = (int)blockIdx.x / t.nbi, bi = (int)blockIdx.x - bj * t.nbi;
+    i = (bi << t.lw) + ((int)threadIdx.x & ((1 << t.lw) - 1));
+    j = bj * (256 >> t.lw) + ((int)threadIdx.x >> t.lw);
+    return i < n0 && j < n1;
+}
+
+// grid b = blockIdx.z of a batch: phi + b*total, Eg + b*total, scalars sc + b
 template <typename T>
 __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>::type *Eg, int n0,
-                                              int n1, int n2, const Scalars *sc) {
+                                              int n1, int n2, const Scalars *sc, CellTile t) {
     const size_t total = (size_t)n0 * n1 * n2;
-    phi += blockIdx.y * total;
-    Eg += blockIdx.y * total;
-    sc += blockIdx.y;
+    phi += blockIdx.z * total;
+    Eg += blockIdx.z * total;
+    sc += blockIdx.z;
     const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
     const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
     const size_t sy = n0, sz = (size_t)n0 * n1;
-    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total;
-         c += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % n0);
-        const int j = (int)((c / n0) % n1);
-        const int k = (int)(c / sz);
-        T r0, r1, r2;
-        grad_stencil(phi + c, i, j, k, n0, n1, n2, sy, sz, id0, id1, id2, r0, r1, r2);
-        typename vec4_of<T>::type e;
-        e.x = -r0 * g0;
-        e.y = -r1 * g1;
-        e.z = -r2 * g2;
-        e.w = T(0);
-        Eg[c] = e;
-    }
+    int i, j;
+    if (!tile_cell(t, n0, n1, i, j)) return;
+    const int k = blockIdx.y;
+    const size_t c = (size_t)k * sz + (size_t)j * sy + i;
+    T r0, r1, r2;
+    grad_stencil(phi + c, i, j, k, n0, n1, n2, sy, sz, id0, id1, id2, r0, r1, r2);
+    typename vec4_of<T>::type e;
+    e.x = -r0 * g0;
+    e.y = -r1 * g1;
+    e.z = -r2 * g2;
+    e.w = T(0);
+    Eg[c] = e;
 }
 
 // The energy bins' fields summed on the grid in one pass: for each of the nb
@@ -672,40 +688,49 @@ __global__ __launch_bounds__(256) void k_grad(const T *phi, typename vec4_of<T>:
 // when phii is given), E_b = -Grad(phi_b) * (g_b, g_b, 1/g_b) and the magnetic
 // field betaC_b * (-E_y, E_x) of the bin (betaC_b * (E_y, -E_x) of its image),
 // added in bin order (bin, then its image) as the per-bin accumulation adds
-// them; Eg and Bg are written once.
+// them; Eg and Bg are written once.  The bins' factors are formed once per
+// block, in LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_field_sums(const T *phi, const T *phii, size_t phiB, int nb,
                                                     typename vec4_of<T>::type *Eg, typename vec4_of<T>::type *Bg,
-                                                    int n0, int n1, int n2, const Scalars *scb) {
+                                                    int n0, int n1, int n2, const Scalars *scb, CellTile t) {
     using V = typename vec4_of<T>::type;
-    const size_t total = (size_t)n0 * n1 * n2;
-    const size_t sy = n0, sz = (size_t)n0 * n1;
-    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total;
-         c += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % n0);
-        const int j = (int)((c / n0) % n1);
-        const int k = (int)(c / sz);
-        V E = {T(0), T(0), T(0), T(0)}, B = {T(0), T(0), T(0), T(0)};
-        for (int b = 0; b < nb; ++b) {
-            const Scalars *sc = scb + b;
-            const T id0 = (T)(0.5 / sc->hr[0]), id1 = (T)(0.5 / sc->hr[1]), id2 = (T)(0.5 / sc->hr[2]);
-            const T g0 = (T)sc->gscale[0], g1 = (T)sc->gscale[1], g2 = (T)sc->gscale[2];
-            for (int img = 0; img < (phii ? 2 : 1); ++img) {
-                const T *p = (img ? phii : phi) + b * phiB + c;
-                const T bs = (T)((img ? 1.0 : -1.0) * sc->betaC);
-                T r0, r1, r2;
-                grad_stencil(p, i, j, k, n0, n1, n2, sy, sz, id0, id1, id2, r0, r1, r2);
-                const T ex = -r0 * g0, ey = -r1 * g1, ez = -r2 * g2;
-                E.x += ex;
-                E.y += ey;
-                E.z += ez;
-                B.x += bs * ey;
-                B.y -= bs * ex;
-            }
-        }
-        Eg[c] = E;
-        Bg[c] = B;
+    __shared__ T fac[kMaxBins][7];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const Scalars *sc = scb + b;
+        fac[b][0] = (T)(0.5 / sc->hr[0]);
+        fac[b][1] = (T)(0.5 / sc->hr[1]);
+        fac[b][2] = (T)(0.5 / sc->hr[2]);
+        fac[b][3] = (T)sc->gscale[0];
+        fac[b][4] = (T)sc->gscale[1];
+        fac[b][5] = (T)sc->gscale[2];
+        fac[b][6] = (T)sc->betaC;
     }
+    __syncthreads();
+    const size_t sy = n0, sz = (size_t)n0 * n1;
+    int i, j;
+    if (!tile_cell(t, n0, n1, i, j)) return;
+    const int k = blockIdx.y;
+    const size_t c = (size_t)k * sz + (size_t)j * sy + i;
+    V E = {T(0), T(0), T(0), T(0)}, B = {T(0), T(0), T(0), T(0)};
+    for (int b = 0; b < nb; ++b) {
+        const T id0 = fac[b][0], id1 = fac[b][1], id2 = fac[b][2];
+        const T g0 = fac[b][3], g1 = fac[b][4], g2 = fac[b][5];
+        for (int img = 0; img < (phii ? 2 : 1); ++img) {
+            const T *p = (img ? phii : phi) + b * phiB + c;
+            const T bs = img ? fac[b][6] : -fac[b][6];
+            T r0, r1, r2;
+            grad_stencil(p, i, j, k, n0, n1, n2, sy, sz, id0, id1, id2, r0, r1, r2);
+            const T ex = -r0 * g0, ey = -r1 * g1, ez = -r2 * g2;
+            E.x += ex;
+            E.y += ey;
+            E.z += ez;
+            B.x += bs * ey;
+            B.y -= bs * ex;
+        }
+    }
+    Eg[c] = E;
+    Bg[c] = B;
 }
 
 // rho entering the solver, for diagnostics
@@ -1342,16 +1367,22 @@ void PoissonGrid::solve_bins(const Scalars *scb, int nb, bool image, hipEvent_t 
     else solve_bins_impl<float>(*this, gs, scb, image, green_ready);
 }
 
+// grid of the Grad kernels: cell tiles over (i, j), planes k, then the batch
+static dim3 grad_grid(const int n[3], const CellTile &t, int nb) {
+    return dim3((unsigned)(t.nbi * ((n[1] + (256 >> t.lw) - 1) / (256 >> t.lw))), (unsigned)n[2], (unsigned)nb);
+}
+
 void PoissonGrid::field_sums(const Scalars *scb, int nb, bool image) {
-    const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
+    const CellTile t = cell_tile(n[0]);
+    const dim3 grid = grad_grid(n, t, 1);
     if (fp64)
-        k_field_sums<double><<<blocks, 256, 0, stream>>>((const double *)bb_phi, image ? (const double *)bb_phii : nullptr,
-                                                         ncell, nb, (double4 *)Esum, (double4 *)Bsum, n[0], n[1], n[2],
-                                                         scb);
+        k_field_sums<double><<<grid, 256, 0, stream>>>((const double *)bb_phi, image ? (const double *)bb_phii : nullptr,
+                                                       ncell, nb, (double4 *)Esum, (double4 *)Bsum, n[0], n[1], n[2],
+                                                       scb, t);
     else
-        k_field_sums<float><<<blocks, 256, 0, stream>>>((const float *)bb_phi, image ? (const float *)bb_phii : nullptr,
-                                                        ncell, nb, (float4 *)Esum, (float4 *)Bsum, n[0], n[1], n[2],
-                                                        scb);
+        k_field_sums<float><<<grid, 256, 0, stream>>>((const float *)bb_phi, image ? (const float *)bb_phii : nullptr,
+                                                      ncell, nb, (float4 *)Esum, (float4 *)Bsum, n[0], n[1], n[2],
+                                                      scb, t);
     profile_mark(PROF_GRAD);
 }
 
@@ -1367,16 +1398,18 @@ int PoissonGrid::ensure_bin_fields(std::string &err) {
 }
 
 void PoissonGrid::grad_bins(const Scalars *scb, int nb) {
-    const dim3 grid((unsigned)std::min<size_t>((ncell + 255) / 256, 65536), (unsigned)nb);
-    if (fp64) k_grad<double><<<grid, 256, 0, stream>>>((const double *)bb_phi, (double4 *)bb_E, n[0], n[1], n[2], scb);
-    else k_grad<float><<<grid, 256, 0, stream>>>((const float *)bb_phi, (float4 *)bb_E, n[0], n[1], n[2], scb);
+    const CellTile t = cell_tile(n[0]);
+    const dim3 grid = grad_grid(n, t, nb);
+    if (fp64) k_grad<double><<<grid, 256, 0, stream>>>((const double *)bb_phi, (double4 *)bb_E, n[0], n[1], n[2], scb, t);
+    else k_grad<float><<<grid, 256, 0, stream>>>((const float *)bb_phi, (float4 *)bb_E, n[0], n[1], n[2], scb, t);
     profile_mark(PROF_GRAD);
 }
 
 void PoissonGrid::grad(const Scalars *d_sc) {
-    const int blocks = (int)std::min<size_t>((ncell + 255) / 256, 65536);
-    if (fp64) k_grad<double><<<blocks, 256, 0, stream>>>((const double *)phi, (double4 *)Eg, n[0], n[1], n[2], d_sc);
-    else k_grad<float><<<blocks, 256, 0, stream>>>((const float *)phi, (float4 *)Eg, n[0], n[1], n[2], d_sc);
+    const CellTile t = cell_tile(n[0]);
+    const dim3 grid = grad_grid(n, t, 1);
+    if (fp64) k_grad<double><<<grid, 256, 0, stream>>>((const double *)phi, (double4 *)Eg, n[0], n[1], n[2], d_sc, t);
+    else k_grad<float><<<grid, 256, 0, stream>>>((const float *)phi, (float4 *)Eg, n[0], n[1], n[2], d_sc, t);
     profile_mark(PROF_GRAD);
 }
 
