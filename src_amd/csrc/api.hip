@@ -614,6 +614,13 @@ static int key_bits(const int n[3]) {
     while (b < 32 && (1ull << b) < m) ++b;
     return b;
 }
+// the bits of the cell keys alone (cell_key_count is a power of two)
+static int key_bits_in_grid(const int n[3]) {
+    const uint64_t m = (uint64_t)cell_key_count(n[0], n[1], n[2]);
+    int b = 0;
+    while (b < 32 && (1ull << b) < m) ++b;
+    return b;
+}
 
 extern "C" int opalsc_sort_by_cell(opalsc_ctx *c, int64_t N, const double *R, const double origin[3],
                                    const double hr[3], uint32_t *perm) {
@@ -1105,7 +1112,7 @@ static void sort_keys(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st
         bits = bits - drop + bb;
     } else if (coarse > 0) {
         launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals, nullptr, 0, coarse);
-        bits -= coarse;
+        bits = std::max(1, key_bits_in_grid(c->cfg.n) - coarse);  // no key of its own for particles outside the grid
     } else {
         launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals);
     }
