@@ -269,6 +269,39 @@ __host__ __device__ inline uint32_t morton_spread(uint32_t v) {
 __host__ __device__ inline uint32_t cell_key(int i, int j, int k, int, int) {
     return morton_spread((uint32_t)i) | (morton_spread((uint32_t)j) << 1) | (morton_spread((uint32_t)k) << 2);
 }
+// Hilbert-curve index of cell (i, j, k) on a 2^b cube (b >= 1), by Skilling's
+// transpose method ("Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 2004): undo the excess rotations and reflections level by level, Gray-code,
+// then interleave the bits (i most significant in each triple).  Consecutive
+// indices are face neighbours, and each aligned run of 8 is a 2x2x2 block.
+__host__ __device__ inline uint32_t cell_key_hilbert(int i, int j, int k, int b) {
+    uint32_t X[3] = {(uint32_t)i, (uint32_t)j, (uint32_t)k};
+    const uint32_t M = 1u << (b - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int a = 0; a < 3; ++a) {
+            if (X[a] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[a]) & P;
+                X[0] ^= t;
+                X[a] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    uint32_t key = 0;
+    for (int q = b - 1; q >= 0; --q)
+        key = (key << 3) | (((X[0] >> q) & 1u) << 2) | (((X[1] >> q) & 1u) << 1) | ((X[2] >> q) & 1u);
+    return key;
+}
 __host__ __device__ inline uint32_t cell_key_count(int n0, int n1, int n2) {
     int m = n0 > n1 ? n0 : n1;
     m = m > n2 ? m : n2;

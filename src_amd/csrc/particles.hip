@@ -923,6 +923,7 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
     const double h0 = sc->hr[0], h1 = sc->hr[1], h2 = sc->hr[2];
     const int n0 = sc->n[0], n1 = sc->n[1], n2 = sc->n[2];
     const uint32_t out = cell_key_count(n0, n1, n2);
+    const int hb = (31 - __builtin_clz(out)) / 3;  // bits per axis of the key cube
     double M[9], o[3];
     for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
     for (int d = 0; d < 3; ++d) o[d] = sc->o_rb[d];
@@ -939,7 +940,7 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
         // need no extra bit for it
         uint32_t key = drop > 0 ? out - 1 : out;
         if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
-            key = cell_key(l[0], l[1], l[2], n0, n1);
+            key = drop > 0 ? cell_key_hilbert(l[0], l[1], l[2], hb) : cell_key(l[0], l[1], l[2], n0, n1);
         if (bin) key = ((uint32_t)bin[p] << bshift) | (key >> drop);
         else key >>= drop;
         keys[p] = key;
