@@ -386,10 +386,10 @@ int opalsc_tune_fft_get(int L, int precision);
  * smaller box (4864 cells, four blocks per CU). */
 int opalsc_tune_deposit_set(int64_t max_particles);
 int64_t opalsc_tune_deposit_get(void);
-/* Tracker re-sort: order by the Morton key of the cell without its drop_bits
- * lowest bits (0: by cell; 3: by blocks of 2x2x2 cells, 6: 4x4x4, ...), particles
- * keeping their relative order within a block.  Does not apply to
- * opalsc_sort_by_cell. */
+/* Tracker re-sort: order by the Hilbert-curve key of the cell without its
+ * drop_bits lowest bits (0: by cell; 3: by blocks of 2x2x2 cells, 6: 4x4x4, ...),
+ * particles keeping their relative order within a block.  Does not apply to
+ * opalsc_sort_by_cell (Morton order). */
 int opalsc_tune_sort_set(int drop_bits);
 int opalsc_tune_sort_get(void);
 int opalsc_track_get_params(opalsc_ctx *ctx, opalsc_track_params *out);

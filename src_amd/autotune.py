@@ -13,7 +13,7 @@ src_amd/build.py compiles into the library (rebuild after running it):
 * Deposit: the particle count up to which a bunch deposits through the larger
   LDS box (three blocks per CU) rather than the smaller one (four per CU),
   from the deposit's time at 1e6 ... 6.4e7 particles on a 256^3 grid.
-* Re-sort granularity: the Morton key bits the tracker's re-sort drops (0, 3,
+* Re-sort granularity: the Hilbert key bits the tracker's re-sort drops (0, 3,
   6: order by cell, by 2^3 or 4^3 blocks of cells), from the step time of a
   2e7-particle bunch on a 256^3 grid re-sorted every 20 steps (40 steps).
 

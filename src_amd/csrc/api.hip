@@ -1088,13 +1088,12 @@ static void permute_bunch(opalsc_ctx *c, hipStream_t st = nullptr) {
     std::swap(b.id, b.keys);
 }
 
-// The permutation (b.vals2) that orders the resident bunch by cell (3D
-// Morton); with energy bins, by cell within each bin, so the bins keep their
-// ranges.  sc: the step's mesh (default: the context's scalars); st: the
-// stream (default: the context's).
-// The tracker's re-sort orders by the Morton key without its g_sort_drop
-// lowest bits (blocks of 2^(drop/3) cells per axis; particles keep their
-// relative order within a block), set by the tuning table or opalsc_tune_sort_set.
+// The permutation (b.vals2) that orders the resident bunch by cell along a
+// Hilbert curve; with energy bins, by cell within each bin, so the bins keep
+// their ranges.  sc: the step's mesh (default: the context's scalars); st: the
+// stream (default: the context's).  The key drops its g_sort_drop lowest bits
+// (blocks of 2^(drop/3) cells per axis; particles keep their relative order
+// within a block), set by the tuning table or opalsc_tune_sort_set.
 static int g_sort_drop = OPALSC_TUNED_SORT_DROP_BITS;
 
 static void sort_keys(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st = nullptr) {
@@ -1112,9 +1111,10 @@ static void sort_keys(opalsc_ctx *c, const Scalars *sc = nullptr, hipStream_t st
         bits = bits - drop + bb;
     } else if (coarse > 0) {
         launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals, nullptr, 0, coarse);
-        bits = std::max(1, key_bits_in_grid(c->cfg.n) - coarse);  // no key of its own for particles outside the grid
+        bits = std::max(1, key_bits_in_grid(c->cfg.n) - coarse);
     } else {
         launch_cell_keys(st, N, b.x, b.y, b.z, sc, true, b.keys, b.vals);
+        bits = key_bits_in_grid(c->cfg.n);
     }
     sort_pairs(st, b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys2, b.vals, b.vals2, N, bits);
 }

@@ -935,12 +935,13 @@ __global__ void k_cell_keys(int64_t N, const double *x, const double *y, const d
         bool ok = cic_dim(r[0], o0, h0, l[0], d[0]);
         ok &= cic_dim(r[1], o1, h1, l[1], d[1]);
         ok &= cic_dim(r[2], o2, h2, l[2], d[2]);
-        // a coarse key (drop > 0: the tracker's re-sort, for locality only)
-        // files a particle outside the grid with the last block, so the keys
-        // need no extra bit for it
-        uint32_t key = drop > 0 ? out - 1 : out;
+        // the tracker's re-sort (rotate: positions in the reference frame) orders
+        // for locality only: Hilbert order, and a particle outside the grid files
+        // with the last cell, so the keys need no extra bit for it; the exact
+        // sort by cell orders by the Morton key, outside particles last
+        uint32_t key = rotate ? out - 1 : out;
         if (ok && l[0] >= 0 && l[1] >= 0 && l[2] >= 0 && l[0] < n0 && l[1] < n1 && l[2] < n2)
-            key = drop > 0 ? cell_key_hilbert(l[0], l[1], l[2], hb) : cell_key(l[0], l[1], l[2], n0, n1);
+            key = rotate ? cell_key_hilbert(l[0], l[1], l[2], hb) : cell_key(l[0], l[1], l[2], n0, n1);
         if (bin) key = ((uint32_t)bin[p] << bshift) | (key >> drop);
         else key >>= drop;
         keys[p] = key;

@@ -234,7 +234,7 @@ def tune_deposit_get():
 
 
 def tune_sort_set(drop_bits):
-    """The tracker's re-sort orders by the cell's Morton key without its drop_bits lowest bits."""
+    """The tracker's re-sort orders by the cell's Hilbert-curve key without its drop_bits lowest bits."""
     _check(None, lib().opalsc_tune_sort_set(int(drop_bits)))
 
 
