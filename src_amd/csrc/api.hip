@@ -1249,18 +1249,19 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
     if (chunked && !boxes) launch_bin_gamma_chunks(c->stream, c->grid.fp64, b, c->bins, c->binchunks.as<double>());
     c->prof.mark(PROF_SCALAR);
+    int bound_rows = alpha_blocks(b.N);  // partial rows of the bounds for launch_track_mesh
     if (c->track_sc && !push && boxes) {
         // the previous gather pass left the statistics of these positions
         launch_bounds_from_boxes(c->stream, c->grid.fp64, b, c->osbox.as<double>(), c->d_sc, bkeys);
         launch_bounds_scan(c->stream, exact, b, c->d_sc, bkeys, c->partials);
+        bound_rows = bounds_scan_blocks(b.N);
         c->prof.mark(PROF_ROTATE_BOUNDS);
     } else if (c->track_sc || push) {
         launch_track_alpha(c->stream, exact, push, b, c->d_sc, c->d_ts, c->partials);
         c->prof.mark(push ? PROF_PUSH_BOUNDS : PROF_ROTATE_BOUNDS);
     }
     if (c->track_sc) {
-        launch_track_mesh(c->stream, c->partials, alpha_blocks(b.N), c->cfg.n, c->dh, c->d_ts, exact, !binned,
-                          c->d_sc);
+        launch_track_mesh(c->stream, c->partials, bound_rows, c->cfg.n, c->dh, c->d_ts, exact, !binned, c->d_sc);
         c->prof.mark(PROF_SCALAR);
         if (sort_now && !sort_side) {
             if (deferred) sort_keys(c);

@@ -372,7 +372,8 @@ int64_t os_box_count(bool fp64, int64_t N);
 // The bounds pass from the previous gather pass's statistics: the beam-frame
 // chunk statistics into b.chunkbox and the six bound keys (reset by
 // launch_track_pre), then the exact bounds of the chunks that can hold an
-// extreme into partials (alpha_blocks(N) rows, as launch_track_alpha leaves them).
+// extreme into partials (bounds_scan_blocks(N) rows, in launch_track_alpha's layout).
+int bounds_scan_blocks(int64_t N);
 void launch_bounds_from_boxes(hipStream_t s, bool fp64, const Bunch &b, const double *osbox, const Scalars *sc,
                               unsigned long long *bkeys);
 void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc,

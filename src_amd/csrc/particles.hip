@@ -1446,10 +1446,15 @@ void launch_bounds_from_boxes(hipStream_t s, bool fp64, const Bunch &b, const do
     if (fp64) k_bounds_from_boxes<4><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, bkeys);
     else k_bounds_from_boxes<2><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, bkeys);
 }
+// A few thousand chunk boxes are checked per block (one broadcast read each)
+// and only the candidates scanned, so 1024 blocks suffice; fewer rows for
+// launch_track_mesh to fold.
+constexpr int kScanBlocksMax = 1024;
+int bounds_scan_blocks(int64_t N) { return std::min(alpha_blocks_of(N), kScanBlocksMax); }
 void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc,
                         const unsigned long long *bkeys, double *partials) {
     if (b.N <= 0) return;
-    const int blocks = alpha_blocks(b.N);  // the partial rows launch_track_mesh reduces
+    const int blocks = bounds_scan_blocks(b.N);  // the partial rows launch_track_mesh reduces
     if (exact) k_bounds_scan<true><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
     else k_bounds_scan<false><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
 }
