@@ -147,6 +147,10 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
     constexpr int NB = E / R;
     const int LR = L / R, Ns = 1 << ls, W = 1 << lw;
     const int twstep = L / (Ns * R);
+    // LDS indices and twiddle indices stay below 2^24: 24-bit multiplies (full
+    // rate; 32-bit ones are quarter rate), once per butterfly, and the R
+    // elements of a butterfly at uniform strides
+    const int LRP = LR * P, NsP = Ns * P;
     C v[NB][R];
     // the lane's element index, opaque to loop-invariant code motion: hoisted
     // out of the stage loop, the E load and store addresses stay live across
@@ -159,16 +163,21 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
         const int g = t + b * kFftThreads;
         const int line = g & (W - 1), j = g >> lw;
         const int k = j & (Ns - 1);
+        const int rb = __umul24(j, P) + line;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * LR) * P + line];
+        for (int r = 0; r < R; ++r) v[b][r] = buf[rb + r * LRP];
         // the first stage's twiddles are all 1 (k = 0)
-        if (ls != 0)
+        if (ls != 0) {
+            const int kt = __umul24(k, twstep);
+            int ti = 0;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                C w = tw[r * k * twstep];
+                ti += kt;  // r * k * twstep
+                C w = tw[ti];
                 if (inv) w.y = -w.y;
                 v[b][r] = cmul(v[b][r], w);
             }
+        }
         dftR<C, R>(v[b], inv);
     }
     __syncthreads();
@@ -178,8 +187,9 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
         const int line = g & (W - 1), j = g >> lw;
         const int k = j & (Ns - 1);
         const int idxD = (((j >> ls) * R) << ls) + k;
+        const int wb = __umul24(idxD, P) + line;
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[(idxD + r * Ns) * P + line] = v[b][r];
+        for (int r = 0; r < R; ++r) buf[wb + r * NsP] = v[b][r];
     }
     __syncthreads();
 }

@@ -237,8 +237,8 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         for (int e = threadIdx.x; e < total; e += kFftThreads) {
             const int ln = div_small(e, m, rm), i = e - ln * m;
             const C v = C{point(min(line0 + 2 * ln, nlines - 1), i), point(min(line0 + 2 * ln + 1, nlines - 1), i)};
-            buf[i * P + ln] = v;
-            if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = v;
+            buf[__umul24(i, P) + ln] = v;
+            if (i >= 1 && i <= n0 - 1) buf[__umul24(L - i, P) + ln] = v;
         }
     } else {
         T r[Q], u[Q];
@@ -254,8 +254,8 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
             const int e = threadIdx.x + q * kFftThreads;
             if (e < total) {
                 const int ln = div_small(e, m, rm), i = e - ln * m;
-                buf[i * P + ln] = C{r[q], u[q]};
-                if (i >= 1 && i <= n0 - 1) buf[(L - i) * P + ln] = C{r[q], u[q]};
+                buf[__umul24(i, P) + ln] = C{r[q], u[q]};
+                if (i >= 1 && i <= n0 - 1) buf[__umul24(L - i, P) + ln] = C{r[q], u[q]};
             }
         }
     }
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     fft_lds<C, E>(buf, tw, pl, false);
     for (int e = threadIdx.x; e < total; e += kFftThreads) {
         const int ln = div_small(e, m, rm), k = e - ln * m;
-        const C v = buf[k * P + ln];
+        const C v = buf[__umul24(k, P) + ln];
         if (line0 + 2 * ln < nlines) Gh[(size_t)(line0 + 2 * ln) * gp + k] = v.x;
         if (line0 + 2 * ln + 1 < nlines) Gh[(size_t)(line0 + 2 * ln + 1) * gp + k] = v.y;
     }
@@ -302,15 +302,15 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         const int e = threadIdx.x + q * kFftThreads;
         if (e < total) {
             const int ln = e & (W - 1), i = e >> lw;
-            buf[i * P + ln] = r[q];
-            if (i >= 1 && i <= nn - 1) buf[(L - i) * P + ln] = r[q];
+            buf[__umul24(i, P) + ln] = r[q];
+            if (i >= 1 && i <= nn - 1) buf[__umul24(L - i, P) + ln] = r[q];
         }
     }
     __syncthreads();
     fft_lds<C, E>(buf, tw, pl, false);
     for (int e = threadIdx.x; e < total; e += kFftThreads) {
         const int ln = e & (W - 1), k = e >> lw, x = x0 + 2 * ln;
-        C v = buf[k * P + ln];
+        C v = buf[__umul24(k, P) + ln];
         if (x >= ncols) v.x = T(0);
         if (x + 1 >= ncols) v.y = T(0);
         *(C *)(Gh + base + 2 * ln + (size_t)k * S) = v;
@@ -347,14 +347,14 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int e = threadIdx.x + q * kFftThreads;
-        buf[(e >> lw) * P + (e & (W - 1))] = C{r[q], T(0)};
+        buf[__umul24(e >> lw, P) + (e & (W - 1))] = C{r[q], T(0)};
     }
     __syncthreads();
     fft_lds<C, E>(buf, tw, pl, false);
     const size_t Sg = (size_t)gp * (n1 + 1);
     for (int e = threadIdx.x; e < total; e += kFftThreads) {
         const int ln = e & (W - 1), kz = e >> lw;
-        Gi[(size_t)kz * Sg + (size_t)ky * gp + x0 + ln] = buf[kz * P + ln];
+        Gi[(size_t)kz * Sg + (size_t)ky * gp + x0 + ln] = buf[__umul24(kz, P) + ln];
     }
 }
 
@@ -425,18 +425,18 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
             } else {
                 d = r[q];
             }
-            buf[x * P + ln] = C{(T)d, T(0)};
+            buf[__umul24(x, P) + ln] = C{(T)d, T(0)};
         }
     }
     for (int e = threadIdx.x; e < W * (L - n0); e += kFftThreads) {
         const int ln = e & (W - 1), x = n0 + (e >> lw);
-        buf[x * P + ln] = C{T(0), T(0)};
+        buf[__umul24(x, P) + ln] = C{T(0), T(0)};
     }
     __syncthreads();
     fft_lds<C, E>(buf, tw, pl, false);
     for (int e = threadIdx.x; e < W * K0; e += kFftThreads) {
         const int ln = div_small(e, K0, rk), k = e - ln * K0;
-        if (line0 + ln < nlines) A[(size_t)(line0 + ln) * kp + k] = buf[k * P + ln];
+        if (line0 + ln < nlines) A[(size_t)(line0 + ln) * kp + k] = buf[__umul24(k, P) + ln];
     }
 }
 
@@ -493,11 +493,11 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = threadIdx.x + q * kFftThreads;
-        if (e < total) buf[(e >> lw) * P + (e & (W - 1))] = r[q];
+        if (e < total) buf[__umul24(e >> lw, P) + (e & (W - 1))] = r[q];
     }
     for (int e = threadIdx.x; e < W * (L - nin); e += kFftThreads) {
         const int ln = e & (W - 1), i = nin + (e >> lw);
-        buf[i * P + ln] = C{T(0), T(0)};
+        buf[__umul24(i, P) + ln] = C{T(0), T(0)};
     }
     __syncthreads();
     if constexpr (MODE == 2) {
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
             for (int q = 0; q < E; ++q) {
                 const int e = threadIdx.x + q * kFftThreads;
-                const int idx = (e >> lw) * P + (e & (W - 1));
+                const int idx = __umul24(e >> lw, P) + (e & (W - 1));
                 const C v = buf[idx];
                 buf[idx] = C{v.x * g[q], v.y * g[q]};
             }
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
                 for (int q = 0; q < H; ++q) {
                     const int e = threadIdx.x + (h + q) * kFftThreads;
-                    const int idx = (e >> lw) * P + (e & (W - 1));
+                    const int idx = __umul24(e >> lw, P) + (e & (W - 1));
                     const C v = buf[idx];
                     buf[idx] = C{v.x * gh[q], v.y * gh[q]};
                 }
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const int e = threadIdx.x + q * kFftThreads;
-            const int idx = (e >> lw) * P + (e & (W - 1));
+            const int idx = __umul24(e >> lw, P) + (e & (W - 1));
             const C v = buf[idx];
             const C p = C{v.x * g[q].x - v.y * g[q].y, v.x * g[q].y + v.y * g[q].x};
             buf[idx] = MODE == 3 ? C{-p.x, -p.y} : p;
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         const int ln = e & (W - 1), i = e >> lw;
         // the image potential is read back flipped in z (FFTPoissonSolver.cpp:326-329)
         const int io = MODE == 3 ? nout - 1 - i : i;
-        out[ob + (size_t)io * outS + ln] = buf[i * P + ln];
+        out[ob + (size_t)io * outS + ln] = buf[__umul24(i, P) + ln];
     }
 }
 
@@ -590,8 +590,8 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         const int e = threadIdx.x + q * kFftThreads;
         if (e < total) {
             const int ln = div_small(e, K0, rk), k = e - ln * K0;
-            buf[k * P + ln] = r[q];
-            if (k >= 1 && k <= n0 - 1) buf[(L - k) * P + ln] = C{r[q].x, -r[q].y};
+            buf[__umul24(k, P) + ln] = r[q];
+            if (k >= 1 && k <= n0 - 1) buf[__umul24(L - k, P) + ln] = C{r[q].x, -r[q].y};
         }
     }
     __syncthreads();
@@ -606,12 +606,12 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         const int line = line0 + ln;
         if (line < nlines) {
             if constexpr (sizeof(T) == 8) {
-                double d = buf[x * P + ln].x * invM;
+                double d = buf[__umul24(x, P) + ln].x * invM;
                 d = d * s1;
                 d = d * s2;
                 phi[(size_t)line * n0 + x] = d;
             } else {
-                float d = buf[x * P + ln].x * (float)invM;
+                float d = buf[__umul24(x, P) + ln].x * (float)invM;
                 d = d * (float)s1;
                 d = d * (float)s2;
                 phi[(size_t)line * n0 + x] = d;
