@@ -1825,12 +1825,13 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
                 rr[u][4] = ps.py[pj[u]];
                 rr[u][5] = ps.pz[pj[u]];
             } else {
-                rr[u][0] = x[j];
-                rr[u][1] = y[j];
-                rr[u][2] = z[j];
-                rr[u][3] = px[j];
-                rr[u][4] = py[j];
-                rr[u][5] = pz[j];
+                // read once per step: non-temporal, so the field grid keeps the L2
+                rr[u][0] = __builtin_nontemporal_load(x + j);
+                rr[u][1] = __builtin_nontemporal_load(y + j);
+                rr[u][2] = __builtin_nontemporal_load(z + j);
+                rr[u][3] = __builtin_nontemporal_load(px + j);
+                rr[u][4] = __builtin_nontemporal_load(py + j);
+                rr[u][5] = __builtin_nontemporal_load(pz + j);
             }
         }
         if constexpr (PERM)
