@@ -1895,12 +1895,12 @@ __global__ __launch_bounds__(256) void k_track_gamma_mixed_os(int64_t N, double 
                     bmx[d] = fmax(bmx[d], r[d]);
                 }
             }
-            x[i] = r[0];
-            y[i] = r[1];
-            z[i] = r[2];
-            px[i] = pn[0];
-            py[i] = pn[1];
-            pz[i] = pn[2];
+            __builtin_nontemporal_store(r[0], x + i);
+            __builtin_nontemporal_store(r[1], y + i);
+            __builtin_nontemporal_store(r[2], z + i);
+            __builtin_nontemporal_store(pn[0], px + i);
+            __builtin_nontemporal_store(pn[1], py + i);
+            __builtin_nontemporal_store(pn[2], pz + i);
         }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
