@@ -1989,12 +1989,12 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
                 rr[u][4] = ps.py[pj[u]];
                 rr[u][5] = ps.pz[pj[u]];
             } else {
-                rr[u][0] = x[j];
-                rr[u][1] = y[j];
-                rr[u][2] = z[j];
-                rr[u][3] = px[j];
-                rr[u][4] = py[j];
-                rr[u][5] = pz[j];
+                rr[u][0] = __builtin_nontemporal_load(x + j);
+                rr[u][1] = __builtin_nontemporal_load(y + j);
+                rr[u][2] = __builtin_nontemporal_load(z + j);
+                rr[u][3] = __builtin_nontemporal_load(px + j);
+                rr[u][4] = __builtin_nontemporal_load(py + j);
+                rr[u][5] = __builtin_nontemporal_load(pz + j);
             }
         }
         if constexpr (PERM)
@@ -2065,12 +2065,12 @@ __global__ __launch_bounds__(256) void k_track_gamma_os(int64_t N, double *x, do
                     bmx[d] = fmax(bmx[d], r[d]);
                 }
             }
-            x[i] = r[0];
-            y[i] = r[1];
-            z[i] = r[2];
-            px[i] = pn[0];
-            py[i] = pn[1];
-            pz[i] = pn[2];
+            __builtin_nontemporal_store(r[0], x + i);
+            __builtin_nontemporal_store(r[1], y + i);
+            __builtin_nontemporal_store(r[2], z + i);
+            __builtin_nontemporal_store(pn[0], px + i);
+            __builtin_nontemporal_store(pn[1], py + i);
+            __builtin_nontemporal_store(pn[2], pz + i);
         }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
