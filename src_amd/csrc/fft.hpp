@@ -141,17 +141,21 @@ __device__ inline void dftR(C *v, bool inv) {
     else dft8(v, inv);
 }
 
-// One Stockham stage on LDS data; Ns = 2^ls = product of the radices already applied.
+// One Stockham stage on LDS data; Ns = 2^ls = product of the radices already
+// applied.  Every stage reads, for the thread's butterfly (line, j), the rows
+// j + r L/R of its line; it writes rows idxD + r Ns.  stage_read loads,
+// twiddles and transforms the thread's butterflies into registers;
+// stage_write stores them.
+// LDS indices and twiddle indices stay below 2^24: 24-bit multiplies (full
+// rate; 32-bit ones are quarter rate), once per butterfly, and the R elements
+// of a butterfly at uniform strides.
 template <typename C, int E, int R>
-__device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int ls, bool inv) {
+__device__ inline void stage_read(const C *buf, const C *tw, int L, int lw, int P, int ls, bool inv,
+                                  C (&v)[E / R][R]) {
     constexpr int NB = E / R;
     const int LR = L / R, Ns = 1 << ls, W = 1 << lw;
     const int twstep = L / (Ns * R);
-    // LDS indices and twiddle indices stay below 2^24: 24-bit multiplies (full
-    // rate; 32-bit ones are quarter rate), once per butterfly, and the R
-    // elements of a butterfly at uniform strides
-    const int LRP = LR * P, NsP = Ns * P;
-    C v[NB][R];
+    const int LRP = LR * P;
     // the lane's element index, opaque to loop-invariant code motion: hoisted
     // out of the stage loop, the E load and store addresses stay live across
     // every stage (and both transforms of the z pass) and spill at E = 16;
@@ -180,7 +184,14 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
         }
         dftR<C, R>(v[b], inv);
     }
-    __syncthreads();
+}
+template <typename C, int E, int R>
+__device__ inline void stage_write(C *buf, int L, int lw, int P, int ls, const C (&v)[E / R][R]) {
+    constexpr int NB = E / R;
+    const int Ns = 1 << ls, W = 1 << lw;
+    const int NsP = Ns * P;
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int g = t + b * kFftThreads;
@@ -191,14 +202,22 @@ __device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int 
 #pragma unroll
         for (int r = 0; r < R; ++r) buf[wb + r * NsP] = v[b][r];
     }
+}
+template <typename C, int E, int R>
+__device__ inline void fft_stage(C *buf, const C *tw, int L, int lw, int P, int ls, bool inv) {
+    C v[E / R][R];
+    stage_read<C, E, R>(buf, tw, L, lw, P, ls, inv, v);
+    __syncthreads();
+    stage_write<C, E, R>(buf, L, lw, P, ls, v);
     __syncthreads();
 }
 
-// Transform the W lines in buf (caller has synchronised after filling it and tw).
+// Stages [s0, s1) of the plan on LDS data, Ns = 2^ls before stage s0 (every
+// earlier radix a power of two); returns ls after them (the odd radix, last,
+// leaves it unchanged).
 template <typename C, int E>
-__device__ inline void fft_lds(C *buf, const C *tw, const FftPlan &pl, bool inv) {
-    int ls = 0;
-    for (int s = 0; s < pl.nst; ++s) {
+__device__ inline int fft_lds_range(C *buf, const C *tw, const FftPlan &pl, int s0, int s1, int ls, bool inv) {
+    for (int s = s0; s < s1; ++s) {
         const int R = pl.radix[s];
         if constexpr (E % 8 == 0) {
             if (R == 8) { fft_stage<C, E, 8>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); ls += 3; continue; }
@@ -216,6 +235,13 @@ __device__ inline void fft_lds(C *buf, const C *tw, const FftPlan &pl, bool inv)
             if (R == 5) { fft_stage<C, E, 5>(buf, tw, pl.L, pl.lw, pl.P, ls, inv); continue; }
         }
     }
+    return ls;
+}
+
+// Transform the W lines in buf (caller has synchronised after filling it and tw).
+template <typename C, int E>
+__device__ inline void fft_lds(C *buf, const C *tw, const FftPlan &pl, bool inv) {
+    (void)fft_lds_range<C, E>(buf, tw, pl, 0, pl.nst, 0, inv);
 }
 
 // Twiddle source of a kernel: FP32 tiles copy the table into LDS after the

@@ -440,6 +440,64 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     }
 }
 
+// The z convolution of one tile with the plan's first and last radix equal (R):
+// every Stockham stage reads, for the thread's butterflies, rows j + r L/R of
+// its line, and the last stage of a transform writes exactly those rows.  So
+// the first forward stage reads its inputs from HBM (rows at and beyond nin are
+// zero), the last forward stage, the product with Ghat and the first inverse
+// stage stay in registers, and the last inverse stage writes HBM: four LDS
+// round trips and their barriers fewer than transforming a tile loaded into LDS.
+template <typename T, int E, int R>
+__device__ inline void zconv_fused(const typename cplx_of<T>::type *in, typename cplx_of<T>::type *out,
+                                   typename cplx_of<T>::type *buf, const typename cplx_of<T>::type *tw,
+                                   const FftPlan &pl, size_t ib, size_t ob, size_t inS, size_t outS, int nin,
+                                   int nout, const T *Gh, int gp, int n1, int n2, int fy, int kx0) {
+    using C = typename cplx_of<T>::type;
+    constexpr int NB = E / R;
+    constexpr int lr = R == 8 ? 3 : (R == 4 ? 2 : 1);
+    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, LR = L / R, nst = pl.nst;
+    C v[NB][R];
+    T g[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int e = threadIdx.x + b * kFftThreads;
+        const int line = e & (W - 1), j = e >> lw;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = j + r * LR;
+            v[b][r] = row < nin ? in[ib + (size_t)row * inS + line] : C{T(0), T(0)};
+            const int fz = row <= n2 ? row : L - row;
+            g[b][r] = Gh[((size_t)fz * (n1 + 1) + fy) * gp + kx0 + line];
+        }
+        dftR<C, R>(v[b], false);  // first stage: its twiddles are all 1
+    }
+    stage_write<C, E, R>(buf, L, lw, P, 0, v);
+    __syncthreads();
+    int ls = fft_lds_range<C, E>(buf, tw, pl, 1, nst - 1, lr, false);
+    stage_read<C, E, R>(buf, tw, L, lw, P, ls, false, v);  // last forward stage: rows j + r L/R
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = C{v[b][r].x * g[b][r], v[b][r].y * g[b][r]};
+        dftR<C, R>(v[b], true);  // first inverse stage, same rows, twiddles 1
+    }
+    __syncthreads();  // every thread has read the forward transform's last stage
+    stage_write<C, E, R>(buf, L, lw, P, 0, v);
+    __syncthreads();
+    ls = fft_lds_range<C, E>(buf, tw, pl, 1, nst - 1, lr, true);
+    stage_read<C, E, R>(buf, tw, L, lw, P, ls, true, v);  // last inverse stage: rows j + r L/R
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int e = threadIdx.x + b * kFftThreads;
+        const int line = e & (W - 1), j = e >> lw;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = j + r * LR;
+            if (row < nout) out[ob + (size_t)row * outS + line] = v[b][r];
+        }
+    }
+}
+
 // Strided pass over one tile of W consecutive kx at fixed outer index o:
 // element i of line kx at in[o*inO + i*inS + kx], i < nin (zero beyond), the
 // transformed line written to out[o*outO + i*outS + kx] for i < nout.  Tiles
@@ -467,6 +525,27 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     const int kx0 = bx * W;
     const size_t ib = (size_t)o * inO + kx0, ob = (size_t)o * outO + kx0;
     const C *tw = fft_twiddles(buf, pl);
+    if constexpr (MODE == 2) {
+        if (pl.nst >= 2 && pl.radix[0] == pl.radix[pl.nst - 1]) {
+            const int fy = o <= n1 ? o : L1 - o;
+            const int R0 = pl.radix[0];
+            if constexpr (E % 8 == 0)
+                if (R0 == 8) {
+                    zconv_fused<T, E, 8>(in, out, buf, tw, pl, ib, ob, inS, outS, nin, nout, Gh, gp, n1, n2, fy, kx0);
+                    return;
+                }
+            if constexpr (E % 4 == 0)
+                if (R0 == 4) {
+                    zconv_fused<T, E, 4>(in, out, buf, tw, pl, ib, ob, inS, outS, nin, nout, Gh, gp, n1, n2, fy, kx0);
+                    return;
+                }
+            if constexpr (E % 2 == 0)
+                if (R0 == 2) {
+                    zconv_fused<T, E, 2>(in, out, buf, tw, pl, ib, ob, inS, outS, nin, nout, Gh, gp, n1, n2, fy, kx0);
+                    return;
+                }
+        }
+    }
     C r[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -854,6 +933,14 @@ static bool make_plan(int L, bool fp64, bool strided, FftPlan &pl) {
     while (rem >= 3 && use8) { pl.radix[pl.nst++] = 8; rem -= 3; }
     while (rem >= 2 && use4) { pl.radix[pl.nst++] = 4; rem -= 2; }
     while (rem >= 1) { pl.radix[pl.nst++] = 2; rem -= 1; }
+    // the powers of two in any order: the first radix also last where it occurs
+    // twice, so that a convolution pass can keep the last forward and the first
+    // inverse stage in registers (zconv_fused)
+    if (pl.nst >= 3 && pl.radix[1] == pl.radix[0]) {
+        const int r0 = pl.radix[0];
+        for (int k = 1; k < pl.nst - 1; ++k) pl.radix[k] = pl.radix[k + 1];
+        pl.radix[pl.nst - 1] = r0;
+    }
     if (odd > 1) pl.radix[pl.nst++] = odd;  // last: every earlier span is a power of two
     if (pl.nst > 16) return false;
     for (int s = 0; s < pl.nst; ++s)
