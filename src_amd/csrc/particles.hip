@@ -1133,10 +1133,13 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
             bkeys[d] = ord_key(__builtin_inf());       // bound from above of the minimum of beam-frame coordinate d
             bkeys[3 + d] = ord_key(-__builtin_inf());  // bound from below of its maximum
         }
+    // the maps are formed in registers and stored once (stores through sc
+    // could alias what is read back, which would reload every operand)
     TrackState s = *ts;
     sc->dT = s.dT;
     sc->coupling = s.coupling;
     sc->N = N;
+    double Mb[9], Mr[9], ob[3], orb[3];
     if (s.space_charge && N > 0) {
         const double pmean[3] = {sc->psum[0] / (double)N, sc->psum[1] / (double)N, sc->psum[2] / (double)N};
         double qa[4];
@@ -1146,19 +1149,20 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
         Trafo b2r = trafo_make(o, qb);
         Trafo r2b = trafo_inverted(b2r);
         for (int k = 0; k < 9; ++k) {
-            sc->M_br[k] = b2r.M[k];
-            sc->M_rb[k] = r2b.M[k];
+            Mb[k] = b2r.M[k];
+            Mr[k] = r2b.M[k];
         }
         for (int d = 0; d < 3; ++d) {
-            sc->o_br[d] = b2r.origin[d];
-            sc->o_rb[d] = r2b.origin[d];
+            ob[d] = b2r.origin[d];
+            orb[d] = r2b.origin[d];
         }
     } else {
         // no beam frame: the identity (only the mixed-precision gather pass reads it then)
-        for (int k = 0; k < 9; ++k) sc->M_br[k] = sc->M_rb[k] = (k % 4 == 0) ? 1.0 : 0.0;
-        for (int d = 0; d < 3; ++d) sc->o_br[d] = sc->o_rb[d] = 0.0;
+        for (int k = 0; k < 9; ++k) Mb[k] = Mr[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        for (int d = 0; d < 3; ++d) ob[d] = orb[d] = 0.0;
     }
-    ext_fields(s, s.t + 0.5 * s.dT, sc->Eext, sc->Bext);
+    double Eext[3], Bext[3];
+    ext_fields(s, s.t + 0.5 * s.dT, Eext, Bext);
     sc->fm_s0 = s.pathLength;  // the lab frame of the field maps during this step
     sc->fm_t = s.t + 0.5 * s.dT;
     s.t += s.dT;
@@ -1169,23 +1173,37 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
         const double o[3] = {0.0, 0.0, 0.0}, q[4] = {1.0, 0.0, 0.0, 0.0};
         upd = trafo_make(o, q);
     }
-    for (int k = 0; k < 9; ++k) sc->M_u[k] = upd.M[k];
-    for (int d = 0; d < 3; ++d) sc->o_u[d] = upd.origin[d];
     // composed maps of the mixed-precision gather pass: r_ref = M_br (r_b - o_br)
     // and r_new = M_u (r_ref - o_u) give r_new = M_c (r_b - o_c) with M_c = M_u M_br,
     // o_c = o_br + M_br^T o_u (M_br orthogonal)
-    const double *Mb = sc->M_br, *Mu = sc->M_u, *Mr = sc->M_rb;
+    const double *Mu = upd.M;
+    double Mc[9], oc[3], Eb[3], Bb[3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
-            sc->M_c[3 * i + j] = Mu[3 * i] * Mb[j] + Mu[3 * i + 1] * Mb[3 + j] + Mu[3 * i + 2] * Mb[6 + j];
-    for (int d = 0; d < 3; ++d) {
-        sc->o_c[d] = sc->o_br[d] + (Mb[d] * upd.origin[0] + Mb[3 + d] * upd.origin[1] + Mb[6 + d] * upd.origin[2]);
-        sc->o_c2[d] = sc->o_c[d];
-        sc->o_rb2[d] = sc->o_rb[d];
-        sc->ih[d] = 0.0;
+            Mc[3 * i + j] = Mu[3 * i] * Mb[j] + Mu[3 * i + 1] * Mb[3 + j] + Mu[3 * i + 2] * Mb[6 + j];
+    for (int d = 0; d < 3; ++d)
+        oc[d] = ob[d] + (Mb[d] * upd.origin[0] + Mb[3 + d] * upd.origin[1] + Mb[6 + d] * upd.origin[2]);
+    mat_vec(Mr, Eext, Eb);
+    mat_vec(Mr, Bext, Bb);
+    for (int k = 0; k < 9; ++k) {
+        sc->M_br[k] = Mb[k];
+        sc->M_rb[k] = Mr[k];
+        sc->M_u[k] = upd.M[k];
+        sc->M_c[k] = Mc[k];
     }
-    mat_vec(Mr, sc->Eext, sc->Eext_b);
-    mat_vec(Mr, sc->Bext, sc->Bext_b);
+    for (int d = 0; d < 3; ++d) {
+        sc->o_br[d] = ob[d];
+        sc->o_rb[d] = orb[d];
+        sc->o_u[d] = upd.origin[d];
+        sc->o_c[d] = oc[d];
+        sc->o_c2[d] = oc[d];
+        sc->o_rb2[d] = orb[d];
+        sc->ih[d] = 0.0;
+        sc->Eext[d] = Eext[d];
+        sc->Bext[d] = Bext[d];
+        sc->Eext_b[d] = Eb[d];
+        sc->Bext_b[d] = Bb[d];
+    }
     s.step += 1;
     *ts = s;
 }
@@ -1406,24 +1424,42 @@ __global__ __launch_bounds__(256) void k_bounds_scan(int64_t N, const double *__
     double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     const int64_t nchunks = (N + kChunk - 1) / kChunk;
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const double *cb = chunkbox + c * kBoxStats;
+    // the block's chunks are blockIdx.x + k gridDim.x (the candidates of a
+    // sorted bunch lie close together in memory; striding spreads them over the
+    // blocks).  Each thread tests one chunk's box, kThreads chunks at a time, so
+    // the tests' loads overlap; the block then scans the candidates in order.
+    __shared__ unsigned long long cmask[kThreads / 64];
+    for (int64_t k0 = 0; blockIdx.x + k0 * gridDim.x < nchunks; k0 += kThreads) {
+        const int64_t c = blockIdx.x + (k0 + threadIdx.x) * gridDim.x;
         bool cand = false;
-        for (int d = 0; d < 3; ++d) cand |= (cb[d] <= umin[d]) || (cb[3 + d] >= lmax[d]);
-        if (!cand) continue;
-        const int64_t base = c * kChunk + threadIdx.x;
+        if (c < nchunks) {
+            const double *cb = chunkbox + c * kBoxStats;
+            for (int d = 0; d < 3; ++d) cand |= (cb[d] <= umin[d]) || (cb[3 + d] >= lmax[d]);
+        }
+        const unsigned long long m = __ballot(cand);
+        if ((threadIdx.x & 63) == 0) cmask[threadIdx.x >> 6] = m;
+        __syncthreads();
+        for (int w = 0; w < kThreads / 64; ++w) {
+            unsigned long long mw = cmask[w];
+            while (mw) {
+                const int b = __builtin_ctzll(mw);
+                mw &= mw - 1;
+                const int64_t base = (blockIdx.x + (k0 + w * 64 + b) * gridDim.x) * kChunk + threadIdx.x;
 #pragma unroll 2
-        for (int k = 0; k < kPPT; ++k) {
-            const int64_t i = base + k * kThreads;
-            if (i >= N) break;
-            double r[3] = {x[i], y[i], z[i]};
-            if constexpr (EXACT) transform_to(M, o, r);
-            else transform_to_fast(M, o, r);
-            for (int d = 0; d < 3; ++d) {
-                mn[d] = fmin(mn[d], r[d]);
-                mx[d] = fmax(mx[d], r[d]);
+                for (int k = 0; k < kPPT; ++k) {
+                    const int64_t i = base + k * kThreads;
+                    if (i >= N) break;
+                    double r[3] = {x[i], y[i], z[i]};
+                    if constexpr (EXACT) transform_to(M, o, r);
+                    else transform_to_fast(M, o, r);
+                    for (int d = 0; d < 3; ++d) {
+                        mn[d] = fmin(mn[d], r[d]);
+                        mx[d] = fmax(mx[d], r[d]);
+                    }
+                }
             }
         }
+        __syncthreads();  // every wave has read cmask before the next round writes it
     }
     block_minmax(mn, mx, lds);
     if (threadIdx.x == 0)
