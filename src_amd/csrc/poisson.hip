@@ -957,7 +957,7 @@ static bool make_plan(int L, bool fp64, bool strided, FftPlan &pl) {
     while (rem >= 1) { pl.radix[pl.nst++] = 2; rem -= 1; }
     // the powers of two in any order: the first radix also last where it occurs
     // twice, so that a convolution pass can keep the last forward and the first
-    // inverse stage in registers (zconv_fused)
+    // inverse stage in registers (strided_fused)
     if (pl.nst >= 3 && pl.radix[1] == pl.radix[0]) {
         const int r0 = pl.radix[0];
         for (int k = 1; k < pl.nst - 1; ++k) pl.radix[k] = pl.radix[k + 1];
