@@ -364,19 +364,27 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
 
 // SRC 0: int64 fixed-point charge (value = i*q2e, then /dT, *tmp2:
 //        PartBunch.cpp:494-510); SRC 1: double rho as given.
+// The x passes transform two real lines in one complex line, as the DCT passes
+// do: line 2p of a block's 2W lines is the real part of complex line p and line
+// 2p + 1 its imaginary part.  With Z = FFT(a + i b), A[k] = (Z[k] + conj Z[L-k])/2
+// and B[k] = -i (Z[k] - conj Z[L-k])/2 (k < L/2 + 1, Z[L] = Z[0]); the inverse
+// pass transforms Z = Y_a + i Y_b with Y's Hermitian extension and takes the real
+// and imaginary parts.  The imaginary parts of Y[0] and Y[L/2], which a c2r
+// transform ignores, are dropped before the merge.
 template <typename T, int E, int SRC>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_xfwd(const void *src, typename cplx_of<T>::type *A,
                                                       FftPlan pl, int n0, int nlines, int kp,
                                                       const Scalars *sc, size_t sB, size_t aB) {
     using C = typename cplx_of<T>::type;
-    constexpr int Q = half_q<E>();
+    constexpr int Q = E;  // 2W lines of n0 = L/2: E elements per thread
     extern __shared__ char smem[];
     C *buf = (C *)smem;
+    T *bufr = (T *)smem;
     sc += blockIdx.z;  // batch: grid blockIdx.z, its source sB cells and its spectrum aB elements apart
     A += blockIdx.z * aB;
-    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, K0 = n0 + 1, total = W * n0;
+    const int W = pl.W, lw = pl.lw, P = pl.P, L = pl.L, K0 = n0 + 1, total = 2 * W * n0;
     const float rn = 1.0f / (float)n0, rk = 1.0f / (float)K0;
-    const int line0 = blockIdx.x * W;
+    const int line0 = blockIdx.x * 2 * W;
     double q2e = 1.0, dT = 1.0, tmp2 = 1.0;
     if constexpr (SRC == 0) {
         q2e = sc->q2e;
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
             } else {
                 d = r[q];
             }
-            buf[__umul24(x, P) + ln] = C{(T)d, T(0)};
+            bufr[2 * (__umul24(x, P) + (ln >> 1)) + (ln & 1)] = (T)d;
         }
     }
     for (int e = threadIdx.x; e < W * (L - n0); e += kFftThreads) {
@@ -434,9 +442,14 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     }
     __syncthreads();
     fft_lds<C, E>(buf, tw, pl, false);
-    for (int e = threadIdx.x; e < W * K0; e += kFftThreads) {
+    for (int e = threadIdx.x; e < 2 * W * K0; e += kFftThreads) {
         const int ln = div_small(e, K0, rk), k = e - ln * K0;
-        if (line0 + ln < nlines) A[(size_t)(line0 + ln) * kp + k] = buf[__umul24(k, P) + ln];
+        if (line0 + ln < nlines) {
+            const C a = buf[__umul24(k, P) + (ln >> 1)], b = buf[__umul24(k == 0 ? 0 : L - k, P) + (ln >> 1)];
+            const C v = (ln & 1) ? C{(a.y + b.y) * T(0.5), (b.x - a.x) * T(0.5)}
+                                 : C{(a.x + b.x) * T(0.5), (a.y - b.y) * T(0.5)};
+            A[(size_t)(line0 + ln) * kp + k] = v;
+        }
     }
 }
 
@@ -669,7 +682,7 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
                                                       FftPlan pl, int n0, int nlines, int kp,
                                                       double invM, const Scalars *sc, size_t aB, size_t phiB) {
     using C = typename cplx_of<T>::type;
-    constexpr int Q = half_q<E>();
+    constexpr int Q = half_q<E>();  // W line pairs of K0 = L/2 + 1
     sc += blockIdx.z;
     A += blockIdx.z * aB;
     phi += blockIdx.z * phiB;
@@ -677,22 +690,25 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
     C *buf = (C *)smem;
     const int W = pl.W, P = pl.P, L = pl.L, K0 = n0 + 1, total = W * K0;
     const float rn = 1.0f / (float)n0, rk = 1.0f / (float)K0;
-    const int line0 = blockIdx.x * W;
+    const int line0 = blockIdx.x * 2 * W;
     const C *tw = fft_twiddles(buf, pl);
-    C r[Q];
+    C ra[Q], rb[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = min(threadIdx.x + q * kFftThreads, total - 1);
-        const int ln = div_small(e, K0, rk), k = e - ln * K0;
-        r[q] = A[(size_t)min(line0 + ln, nlines - 1) * kp + k];
+        const int p = div_small(e, K0, rk), k = e - p * K0;
+        ra[q] = A[(size_t)min(line0 + 2 * p, nlines - 1) * kp + k];
+        rb[q] = A[(size_t)min(line0 + 2 * p + 1, nlines - 1) * kp + k];
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = threadIdx.x + q * kFftThreads;
         if (e < total) {
-            const int ln = div_small(e, K0, rk), k = e - ln * K0;
-            buf[__umul24(k, P) + ln] = r[q];
-            if (k >= 1 && k <= n0 - 1) buf[__umul24(L - k, P) + ln] = C{r[q].x, -r[q].y};
+            const int p = div_small(e, K0, rk), k = e - p * K0;
+            C a = ra[q], b = rb[q];
+            if (k == 0 || k == n0) a.y = b.y = T(0);
+            buf[__umul24(k, P) + p] = C{a.x - b.y, a.y + b.x};
+            if (k >= 1 && k <= n0 - 1) buf[__umul24(L - k, P) + p] = C{a.x + b.y, b.x - a.y};
         }
     }
     __syncthreads();
@@ -702,17 +718,19 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         s1 = sc->vol;
         s2 = sc->coupling;
     }
-    for (int e = threadIdx.x; e < W * n0; e += kFftThreads) {
+    const T *bufr = (const T *)smem;
+    for (int e = threadIdx.x; e < 2 * W * n0; e += kFftThreads) {
         const int ln = div_small(e, n0, rn), x = e - ln * n0;
         const int line = line0 + ln;
         if (line < nlines) {
+            const T v = bufr[2 * (__umul24(x, P) + (ln >> 1)) + (ln & 1)];
             if constexpr (sizeof(T) == 8) {
-                double d = buf[__umul24(x, P) + ln].x * invM;
+                double d = v * invM;
                 d = d * s1;
                 d = d * s2;
                 phi[(size_t)line * n0 + x] = d;
             } else {
-                float d = buf[__umul24(x, P) + ln].x * (float)invM;
+                float d = v * (float)invM;
                 d = d * (float)s1;
                 d = d * (float)s2;
                 phi[(size_t)line * n0 + x] = d;
@@ -1282,7 +1300,7 @@ static void pass_x_fwd(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, i
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], nlines = g.n[1] * g.n[2];
     const FftPlan &pl = g.plan[0];
-    dim3 grid((nlines + pl.W - 1) / pl.W, 1, gs.nb);
+    dim3 grid((nlines + 2 * pl.W - 1) / (2 * pl.W), 1, gs.nb);  // 2W real lines per block
     if (src_kind == 0)
         OPALSC_DISPATCH_E(pl.E, k_xfwd<T, EV, 0><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
                                     gs.rho, (C *)gs.A, pl, n0, nlines, g.kp, d_sc, gs.rhoB, gs.AB))
@@ -1341,7 +1359,7 @@ static void pass_x_inv(PoissonGrid &g, const GridSet &gs, const Scalars *d_sc, i
     const size_t cs = sizeof(C);
     const int n0 = g.n[0], nlines = g.n[1] * g.n[2];
     const FftPlan &pl = g.plan[0];
-    dim3 grid((nlines + pl.W - 1) / pl.W, 1, gs.nb);
+    dim3 grid((nlines + 2 * pl.W - 1) / (2 * pl.W), 1, gs.nb);  // 2W real lines per block
     const double invM = 1.0 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]);
     if (outscale)
         OPALSC_DISPATCH_E(pl.E, k_xinv<T, EV, 1><<<grid, kFftThreads, lds_bytes(pl, cs), g.stream>>>(
