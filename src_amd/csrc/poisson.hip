@@ -204,9 +204,10 @@ constexpr int half_q() { return E / 2 + 1; }
 // Octant lines (k, j) of G (pitch n0+1, FP64) -> rows of Gh (pitch gp); complex
 // line p of the tile holds real lines 2p and 2p+1 of the block's 2W.
 // GREEN (the mixed-precision grid's IGF tables): the octant values are
-// evaluated here, by the function k_green_fast writes them with (table
-// 0: the G[0,0,0] quirk; 1, 2: the image charge's shifted tables), instead of
-// read from G, so the FP64 octant is never stored; scalars sc + blockIdx.z.
+// evaluated here instead of read from G, so the octant is never stored (table
+// 0: the G[0,0,0] quirk; 1, 2: the image charge's shifted tables; scalars
+// sc + blockIdx.z): on an FP32 grid k_green_fast's near-field values rounded
+// to FP32 and its far-field expansion evaluated in FP32 (green_point_mixed).
 template <typename T, int E, bool GREEN = false>
 __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_dct_x(
     const double *G, T *Gh, FftPlan pl, int n0, int nlines, int gp, size_t gB, size_t ghB,
@@ -229,10 +230,22 @@ __global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(4))
         const igf::GreenMoments gm = igf::green_moments(h0, h1, h2);
         const int m1 = n1 + 1;
         const float rm1 = 1.0f / (float)m1;
+        const bool shifted = table != 0, second = table == 2;
+        // octant point (i, line): k, j of the line, with G[0,0,0] := G[0,0,1]
+        auto coords = [&](int line, int i, int &j, int &kq) {
+            const int k = div_small(line, m1, rm1);
+            j = line - k * m1;
+            kq = (table == 0 && i == 0 && j == 0 && k == 0) ? 1 : k;
+        };
+        // on an FP32 grid the far field in FP32 (green_point_mixed)
+        const igf::GreenMomentsF gmf = igf::green_moments_f(h0, h1, h2);
         auto point = [&](int line, int i) -> T {
-            const int k = div_small(line, m1, rm1), j = line - k * m1;
-            const int kq = (table == 0 && i == 0 && j == 0 && k == 0) ? 1 : k;  // G[0,0,0] := G[0,0,1]
-            return (T)igf::green_point_fast(gm, i, j, kq, h0, h1, h2, zshift, table != 0, table == 2, n2, ivol);
+            int j, kq;
+            coords(line, i, j, kq);
+            if constexpr (sizeof(T) == 4)
+                return igf::green_point_mixed(gm, gmf, i, j, kq, h0, h1, h2, zshift, shifted, second, n2, ivol);
+            else
+                return (T)igf::green_point_fast(gm, i, j, kq, h0, h1, h2, zshift, shifted, second, n2, ivol);
         };
         for (int e = threadIdx.x; e < total; e += kFftThreads) {
             const int ln = div_small(e, m, rm), i = e - ln * m;
