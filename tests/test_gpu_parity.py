@@ -100,8 +100,11 @@ def test_sort_by_cell_large_keys_ragged_and_ties(n, N):
 
 @pytest.mark.parametrize("greens", [0, 1])
 @pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (12, 12, 12), (8, 48, 12), (64, 32, 96), (20, 40, 10),
-                               (80, 24, 40)])
+                               (80, 24, 40), (16, 6, 10), (12, 10, 6)])
 def test_compute_potential_fp64(n, greens):
+    """Against the oracle's solve; (16, 6, 10) and (12, 10, 6) have 60 x lines,
+    fewer than one block's 2W, so the x passes' blocks are partly empty
+    (clamped loads, no stores)."""
     rng = np.random.default_rng(4)
     rho = rng.normal(size=n[0] * n[1] * n[2])
     hr = np.array([1.1e-4, 0.9e-4, 2.3e-4])
@@ -136,7 +139,7 @@ def test_every_fft_tile_the_autotuner_can_pick(n, precision, tol):
             sc.tune_fft_set(L, precision, 0)
 
 
-@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (8, 48, 12), (128, 64, 32), (40, 20, 80)])
+@pytest.mark.parametrize("n", [(16, 16, 16), (32, 16, 24), (8, 48, 12), (128, 64, 32), (40, 20, 80), (16, 6, 10)])
 def test_compute_potential_mixed(n):
     """FP32 convolution (FP64 Green's function): tolerance 1e-5 relative L2."""
     rng = np.random.default_rng(9)
