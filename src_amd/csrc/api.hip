@@ -1213,7 +1213,6 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
     Bunch &b = c->bunch;
     const bool exact = c->grid.fp64;  // reference-order particle arithmetic on the FP64 path
     const bool binned = c->bins.nbins > 0;
-    unsigned long long *bkeys = &c->d_counters[8];
     // energy bins whose calcGammas comes from chunk sums (bingamma.hpp): the
     // previous gather pass of this call left them, or they are summed here,
     // before the bounds pass and the sort move particles
@@ -1246,14 +1245,14 @@ static bool enqueue_step(opalsc_ctx *c, bool push, bool next, bool sort_now, boo
         c->grid.zero_rho(c->stream2);
         (void)hipEventRecord(c->ev_zero, c->stream2);
     }
-    launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1], bkeys);
+    launch_track_pre(c->stream, c->d_ts, c->d_sc, b.N, &c->d_counters[1]);
     if (chunked && !boxes) launch_bin_gamma_chunks(c->stream, c->grid.fp64, b, c->bins, c->binchunks.as<double>());
     c->prof.mark(PROF_SCALAR);
     int bound_rows = alpha_blocks(b.N);  // partial rows of the bounds for launch_track_mesh
     if (c->track_sc && !push && boxes) {
         // the previous gather pass left the statistics of these positions
-        launch_bounds_from_boxes(c->stream, c->grid.fp64, b, c->osbox.as<double>(), c->d_sc, bkeys);
-        launch_bounds_scan(c->stream, exact, b, c->d_sc, bkeys, c->partials);
+        launch_bounds_from_boxes(c->stream, c->grid.fp64, b, c->osbox.as<double>(), c->d_sc, c->partials);
+        launch_bounds_scan(c->stream, exact, b, c->d_sc, c->partials);
         bound_rows = bounds_scan_blocks(b.N);
         c->prof.mark(PROF_ROTATE_BOUNDS);
     } else if (c->track_sc || push) {

@@ -316,9 +316,7 @@ void launch_cr_test(hipStream_t s, int which, int64_t n, const double *x, double
 void launch_igf_term_test(hipStream_t s, int fast, int64_t n, const double *v, double *y);
 
 // Tracker kernels
-// bkeys (may be null): the six bound keys of launch_bounds_from_boxes, reset for the step
-void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
-                      unsigned long long *bkeys);
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped);
 int alpha_blocks(int64_t N);
 void launch_track_alpha(hipStream_t s, bool exact, bool push, Bunch &b, const Scalars *sc,
                         const TrackState *ts, double *partials);
@@ -370,14 +368,14 @@ int launch_track_gamma_os(hipStream_t s, bool fp64, bool next, Bunch &b, const v
 // blocks of statistics launch_track_gamma_os leaves (one per one-shot block)
 int64_t os_box_count(bool fp64, int64_t N);
 // The bounds pass from the previous gather pass's statistics: the beam-frame
-// chunk statistics into b.chunkbox and the six bound keys (reset by
-// launch_track_pre), then the exact bounds of the chunks that can hold an
-// extreme into partials (bounds_scan_blocks(N) rows, in launch_track_alpha's layout).
+// chunk statistics into b.chunkbox and per-block bounds on the extremes into
+// spare slots of partials' rows, then the exact bounds of the chunks that can
+// hold an extreme into partials (bounds_scan_blocks(N) rows, slots 0-5, in
+// launch_track_alpha's layout).
 int bounds_scan_blocks(int64_t N);
 void launch_bounds_from_boxes(hipStream_t s, bool fp64, const Bunch &b, const double *osbox, const Scalars *sc,
-                              unsigned long long *bkeys);
-void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc,
-                        const unsigned long long *bkeys, double *partials);
+                              double *partials);
+void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc, double *partials);
 // Bg != nullptr: energy bins, Eg/Bg the summed bin fields (grid E and B both gathered);
 // ext: the tracker has a nonzero external field
 void launch_track_gamma(hipStream_t s, bool fp64, bool next, Bunch &b, const void *Eg, const Scalars *sc,

@@ -1101,21 +1101,9 @@ void launch_igf_term_test(hipStream_t s, int fast, int64_t n, const double *v, d
 // external fields at t + dT/2 (ParallelTTracker.cpp:630), then t += dT and the
 // reference-particle update whose trafo transformBunch applies at step end
 // (ParallelTTracker.cpp:362-367,1199-1213).
-// Order-preserving map of doubles onto unsigned 64-bit keys, so that the
-// minimum or maximum of doubles is an integer atomicMin/atomicMax (exact and
-// independent of the order of the atomics).
-__device__ inline unsigned long long ord_key(double v) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-}
-__device__ inline double ord_val(unsigned long long k) {
-    const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)u);
-}
-
 template <bool BATCH = false>
 __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
-                            unsigned long long *bkeys, BatchView bv = BatchView{}) {
+                            BatchView bv = BatchView{}) {
     if constexpr (BATCH) {
         if (threadIdx.x != 0) return;
         const BatchItem &it = bv.items[blockIdx.x];
@@ -1123,16 +1111,10 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
         sc = it.sc;
         N = it.N;
         dropped = it.dropped;
-        bkeys = nullptr;
     } else if (threadIdx.x != 0 || blockIdx.x != 0) {
         return;
     }
     *dropped = 0;  // the step's count of deposits outside the grid
-    if (bkeys)
-        for (int d = 0; d < 3; ++d) {
-            bkeys[d] = ord_key(__builtin_inf());       // bound from above of the minimum of beam-frame coordinate d
-            bkeys[3 + d] = ord_key(-__builtin_inf());  // bound from below of its maximum
-        }
     // the maps are formed in registers and stored once (stores through sc
     // could alias what is read back, which would reload every operand)
     TrackState s = *ts;
@@ -1207,9 +1189,8 @@ __global__ void k_track_pre(TrackState *ts, Scalars *sc, int64_t N, unsigned lon
     s.step += 1;
     *ts = s;
 }
-void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped,
-                      unsigned long long *bkeys) {
-    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N, dropped, bkeys);
+void launch_track_pre(hipStream_t s, TrackState *ts, Scalars *sc, int64_t N, unsigned long long *dropped) {
+    k_track_pre<<<1, 64, 0, s>>>(ts, sc, N, dropped);
 }
 
 // PUSH: the step's first half push (timeIntegration1), written back; otherwise
@@ -1339,10 +1320,15 @@ constexpr double kBoxMarginUlps = 16.0 * 0x1p-53;
 // writes each chunk's widened beam-frame interval as the deposit's bounding
 // box, with NaN sums: a deposit block whose box overflows its LDS box forms
 // the chunk's statistics itself.
+// The block's bounds go to slots kBoxRowSlot.. of its row of the partial-sums
+// buffer, beside the slots 0-5 that k_bounds_scan writes: per-block rows
+// instead of six device-scope atomics per block, which execute at the memory
+// side one after another on the same six words.
+constexpr int kBoxRowSlot = 8;
 template <int OSPC>
 __global__ __launch_bounds__(256) void k_bounds_from_boxes(int64_t N, const double *__restrict__ osbox,
                                                            const Scalars *sc, double *chunkbox,
-                                                           unsigned long long *bkeys) {
+                                                           double *rows) {
     __shared__ double lds[4 * 6];
     constexpr int kOs = kChunk / OSPC;
     const int64_t nchunks = (N + kChunk - 1) / kChunk;
@@ -1401,8 +1387,8 @@ __global__ __launch_bounds__(256) void k_bounds_from_boxes(int64_t N, const doub
     block_minmax(umin, lmax, lds);
     if (threadIdx.x == 0)
         for (int d = 0; d < 3; ++d) {
-            atomicMin(&bkeys[d], ord_key(umin[d]));
-            atomicMax(&bkeys[3 + d], ord_key(lmax[d]));
+            rows[blockIdx.x * kSlots + kBoxRowSlot + d] = umin[d];
+            rows[blockIdx.x * kSlots + kBoxRowSlot + 3 + d] = lmax[d];
         }
 }
 
@@ -1411,15 +1397,34 @@ __global__ __launch_bounds__(256) void k_bounds_from_boxes(int64_t N, const doub
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_bounds_scan(int64_t N, const double *__restrict__ x,
                                                      const double *__restrict__ y, const double *__restrict__ z,
-                                                     const Scalars *sc, const double *chunkbox,
-                                                     const unsigned long long *bkeys, double *partials) {
+                                                     const Scalars *sc, const double *chunkbox, int nbox,
+                                                     double *partials) {
     __shared__ double lds[4 * 6];
+    __shared__ double best[6];
     double M[9], o[3], umin[3], lmax[3];
     for (int k = 0; k < 9; ++k) M[k] = sc->M_rb[k];
     for (int d = 0; d < 3; ++d) {
         o[d] = sc->o_rb[d];
-        umin[d] = ord_val(bkeys[d]);
-        lmax[d] = ord_val(bkeys[3 + d]);
+        umin[d] = __builtin_inf();
+        lmax[d] = -__builtin_inf();
+    }
+    // the best bounds over the nbox rows k_bounds_from_boxes left (slots the
+    // blocks' own results, slots 0-5, do not touch)
+    for (int r = threadIdx.x; r < nbox; r += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            umin[d] = fmin(umin[d], partials[r * kSlots + kBoxRowSlot + d]);
+            lmax[d] = fmax(lmax[d], partials[r * kSlots + kBoxRowSlot + 3 + d]);
+        }
+    block_minmax(umin, lmax, lds);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            best[d] = umin[d];
+            best[3 + d] = lmax[d];
+        }
+    __syncthreads();
+    for (int d = 0; d < 3; ++d) {
+        umin[d] = best[d];
+        lmax[d] = best[3 + d];
     }
     double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
@@ -1473,26 +1478,30 @@ int64_t os_box_count(bool fp64, int64_t N) {
     const int64_t k = fp64 ? kOsChunk64 : kOsChunk;
     return (N + k - 1) / k;
 }
+static int box_bound_blocks(int64_t N) {
+    const int64_t nchunks = (N + kChunk - 1) / kChunk;
+    return (int)std::min<int64_t>((nchunks + kThreads - 1) / kThreads, 1024);
+}
 void launch_bounds_from_boxes(hipStream_t s, bool fp64, const Bunch &b, const double *osbox, const Scalars *sc,
-                              unsigned long long *bkeys) {
+                              double *partials) {
     if (b.N <= 0) return;
-    const int64_t nchunks = (b.N + kChunk - 1) / kChunk;
-    const int blocks = (int)std::min<int64_t>((nchunks + kThreads - 1) / kThreads, 1024);
+    const int blocks = box_bound_blocks(b.N);
     static_assert(kChunk == 2 * kOsChunk && kChunk == 4 * kOsChunk64, "one-shot blocks tile the chunks");
-    if (fp64) k_bounds_from_boxes<4><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, bkeys);
-    else k_bounds_from_boxes<2><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, bkeys);
+    static_assert(kBoxRowSlot >= 6 && kBoxRowSlot + 6 <= kSlots, "box rows beside the scan's slots 0-5");
+    if (fp64) k_bounds_from_boxes<4><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, partials);
+    else k_bounds_from_boxes<2><<<blocks, kThreads, 0, s>>>(b.N, osbox, sc, b.chunkbox, partials);
 }
 // A few thousand chunk boxes are checked per block (one broadcast read each)
 // and only the candidates scanned, so 1024 blocks suffice; fewer rows for
 // launch_track_mesh to fold.
 constexpr int kScanBlocksMax = 1024;
 int bounds_scan_blocks(int64_t N) { return std::min(alpha_blocks_of(N), kScanBlocksMax); }
-void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc,
-                        const unsigned long long *bkeys, double *partials) {
+void launch_bounds_scan(hipStream_t s, bool exact, const Bunch &b, const Scalars *sc, double *partials) {
     if (b.N <= 0) return;
     const int blocks = bounds_scan_blocks(b.N);  // the partial rows launch_track_mesh reduces
-    if (exact) k_bounds_scan<true><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
-    else k_bounds_scan<false><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, bkeys, partials);
+    const int nbox = box_bound_blocks(b.N);
+    if (exact) k_bounds_scan<true><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, nbox, partials);
+    else k_bounds_scan<false><<<blocks, kThreads, 0, s>>>(b.N, b.x, b.y, b.z, sc, b.chunkbox, nbox, partials);
 }
 
 template <bool BATCH = false>
@@ -2431,7 +2440,7 @@ void launch_batch_end(hipStream_t s, const BatchView &v) {
     if (v.n > 0) k_batch_scalars<1><<<v.n, 64, 0, s>>>(v);
 }
 void launch_batch_pre(hipStream_t s, const BatchView &v) {
-    if (v.n > 0) k_track_pre<true><<<v.n, 64, 0, s>>>(nullptr, nullptr, 0, nullptr, nullptr, v);
+    if (v.n > 0) k_track_pre<true><<<v.n, 64, 0, s>>>(nullptr, nullptr, 0, nullptr, v);
 }
 static unsigned batch_blocks(const BatchView &v) { return v.n > 0 ? (unsigned)v.blocks : 0u; }
 void launch_batch_alpha(hipStream_t s, const BatchView &v, bool exact, bool push) {
